@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""bench.py — particle-substeps/s of the MLS-MPM substep on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): v60_working,
+a 128^3 grid at dx = 1 mm holding 1,048,576 water particles (70% pre-wetted
+slab in a porous value-noise bed, 30% poured at 2,000 per substep), with the
+red-black Gauss-Seidel pressure solve run to a 1e-4 relative residual every
+substep.  Inputs are synthetic (seeded generators); nothing is read from disk.
+
+A "step" is one substep of the hot path (P2G -> grid update -> classify ->
+RBGS to tolerance -> projection -> packing -> viscosity -> G2P).  The scene is
+first advanced through its pour (about 158 substeps, untimed) so every timed
+substep runs with all 1,048,576 particles in the domain; then `--warmup`
+untimed substeps, then exactly `--steps` timed substeps bracketed by stream
+synchronisation.  `value` = particles x substeps / wall seconds of the timed
+region, with all inputs resident in HBM.
+
+The JSON line also carries
+  roofline      for the kernel with the largest share of device time, from
+                HIP events recorded on the solver's stream over the timed region
+  cpu_baseline  the CPU oracle (single thread) timed on a bounded sample of the
+                same workload state on this machine's host
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-substeps/s at 1M particles,128³; RBGS iters to 1e-4 residual"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def roofline(kt) -> dict:
+    """Achieved algorithmic bandwidth of the dominant kernel.
+
+    Algorithmic bytes (per unit, counted once per launch):
+      P2G   80 B per particle read (pos, vel, C) + 24 B per touched cell written
+            (6 i32 lanes); counted here as 80 B/particle + 24 B x 27 nodes'
+            share, i.e. 80 B/particle + 24 B x cells / particles.
+      G2P   128 B per particle (48 read + 80 written) + 20 B per cell read.
+      RBGS  16 B per fluid cell per red+black iteration: read packed kind/fill
+            4 B, read rhs 4 B, read p 4 B (as a neighbour), write p 4 B.
+    """
+    shares = {"p2g": kt.p2g_us, "g2p": kt.g2p_us, "rbgs": kt.rbgs_us}
+    dom = max(shares, key=shares.get)
+    if dom == "rbgs":
+        launches = max(kt.rbgs_launches, 1)
+        # rbgs_cell_updates = iterations x fluid cells; 16 B per fluid cell per iteration
+        bytes_per_launch = 16.0 * kt.rbgs_cell_updates / launches
+        avg_us = kt.rbgs_us / launches
+        name = "rbgs_sweep"
+    elif dom == "p2g":
+        launches = max(kt.p2g_launches, 1)
+        bytes_per_launch = 80.0 * kt.p2g_particles / launches
+        avg_us = kt.p2g_us / launches
+        name = "p2g"
+    else:
+        launches = max(kt.g2p_launches, 1)
+        bytes_per_launch = 128.0 * kt.g2p_particles / launches
+        avg_us = kt.g2p_us / launches
+        name = "g2p"
+    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
+    return {
+        "kernel": name,
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": None,
+        "avg_launch_us": round(avg_us, 3),
+        "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
+        "device_time_share_us": {k: round(v, 1) for k, v in shares.items()},
+    }
+
+
+def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, budget_s: float) -> dict:
+    """The CPU oracle (single-threaded C restatement of the reference solver) on
+    the same post-pour state, timed for a bounded number of substeps."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import orc  # checker only: times the CPU restatement beside the GPU
+
+    from coffee_sim_amd.abi import Scene
+
+    s = w.settings.copy()
+    s.emit_per_substep = 0
+    s.emit_total = 0
+    n = particles.shape[0]
+    p = np.ascontiguousarray(particles, np.float32)
+    a = np.ascontiguousarray(affine, np.float32)
+    sc = Scene()
+    src = w.scene
+    sc.sdf, sc.sdf_class = src.sdf, src.sdf_class
+    sc.bed_lookup, sc.bed_records, sc.num_bed_records = src.bed_lookup, src.bed_records, src.num_bed_records
+    sc.particles = p.ctypes.data_as(C.POINTER(C.c_float))
+    sc.affine = a.ctypes.data_as(C.POINTER(C.c_float))
+    sc.num_particles = n
+    ref = orc.OracleSim(s, sc)
+    done, iters = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        ref.step_frame(w.frame_dt)
+        done += 1
+        iters += ref.solve_stats().iterations
+        el = time.perf_counter() - t0
+        log(f"cpu oracle substep {done}: {el:.1f}s, {iters} RBGS iterations so far")
+        if el >= budget_s or done >= 8:
+            break
+    el = time.perf_counter() - t0
+    ref.close()
+    return {
+        "value": round(n * done / el, 2),
+        "unit": "particle-substeps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} substeps of v60_working after the pour ({n} particles, RBGS to 1e-4: "
+                  f"{iters} iterations), single-threaded C oracle, {el:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=1, help="divide the grid side (tests only)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle sampling (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import coffee_sim_amd as cs
+
+    w = cs.SeededWorkload("v60_working", seed=args.seed, scale=args.scale)
+    ctx = cs.Context(0)
+    sim = cs.MpmSim3D(ctx, w.settings, w.scene)
+    dt = w.frame_dt
+    pour = (w.settings.emit_total + w.settings.emit_per_substep - 1) // max(w.settings.emit_per_substep, 1)
+    log(f"workload v60_working grid {list(w.settings.grid_dims)} initial {w.scene.num_particles} "
+        f"total {w.total_particles}; advancing {pour} pour substeps untimed")
+    t0 = time.perf_counter()
+    iters = 0
+    for k in range(pour):
+        sim.step_frame(dt)
+        iters += sim.solve_stats().iterations
+        if k % 20 == 0 or k == pour - 1:
+            log(f"pour substep {k + 1}/{pour}: particles {sim.particle_count()} "
+                f"RBGS iters {sim.solve_stats().iterations} elapsed {time.perf_counter() - t0:.1f}s")
+    for k in range(args.warmup):
+        sim.step_frame(dt)
+    sim.synchronize()
+    n_particles = sim.particle_count()
+    log(f"warmup done; timing {args.steps} substeps at {n_particles} particles")
+
+    sim.enable_timing(True)
+    sim.reset_kernel_timing()
+    iters_timed = []
+    sim.synchronize()
+    t0 = time.perf_counter()
+    psteps = 0
+    for k in range(args.steps):
+        sim.step_frame(dt)
+        iters_timed.append(sim.solve_stats().iterations)
+        psteps += sim.particle_count()
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = sim.kernel_timing()
+    sim.enable_timing(False)
+    log(f"timed {args.steps} substeps in {elapsed:.3f}s; RBGS iterations {iters_timed}")
+    log(f"kernel timing {kt.as_dict()}")
+
+    cpu = None
+    if args.cpu_budget > 0:
+        parts = sim.read_particles(0, n_particles)
+        aff = sim.read_affine(0, n_particles)
+        cpu = cpu_baseline(w, parts, aff, args.cpu_budget)
+
+    value = psteps / elapsed
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generators)",
+        "config": {
+            "workload": "v60_working: 128^3 grid, dx 1 mm, 1,048,576 particles, value-noise porous bed, "
+                        "RBGS to 1e-4 relative residual",
+            "grid": list(int(d) for d in w.settings.grid_dims),
+            "particles": int(n_particles),
+            "rbgs_tolerance": float(w.settings.rbgs_tolerance),
+            "rbgs_iterations_per_substep": round(float(np.mean(iters_timed)), 1),
+            "scale": args.scale,
+        },
+        "roofline": roofline(kt),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    sim.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,393 @@
+"""coffee_sim_amd — MI355X-native MLS-MPM substep of craigxchen/coffee-sim.
+
+Python host over the C ABI in include/coffee_sim.h.  `MpmSim3D` mirrors the
+reference's `MpmSim3D` (crates/sim-wasm/src/mpm_3d/mod.rs:179-567): the same
+constructor inputs (settings presets), `step_frame`, `reset`, kettle/spout
+setters and counters, plus readbacks of the particle and grid buffers that the
+reference's tests take through staging copies (physics_tests.rs:61-139).
+
+All numerics run in the native library (coffee_sim_amd/lib/libcoffee_sim.so,
+gfx950 kernels).  There is no Python or CPU fallback: if the library is not
+built, `load_library()` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+from .abi import (  # noqa: F401  (re-exported)
+    BedRecord,
+    Geometry,
+    Obstacle,
+    Scene,
+    Settings,
+    SolveStats,
+    Spout,
+    Workload,
+)
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "lib" / "libcoffee_sim.so"
+_lib = None
+
+
+class CoffeeSimError(RuntimeError):
+    def __init__(self, code: int, where: str, message: str):
+        super().__init__(f"{where} failed ({code}): {message}")
+        self.code = code
+
+
+def build(verbose: bool = False) -> Path:
+    from .build import build as _build
+
+    global _lib
+    _lib = None
+    return _build(verbose=verbose)
+
+
+def _bind(lib: C.CDLL) -> None:
+    P = C.POINTER
+    sigs = {
+        "cs_last_error": (C.c_char_p, []),
+        "cs_settings_preset": (C.c_int, [C.c_int, P(Settings)]),
+        "cs_workload_build": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, P(Workload)]),
+        "cs_workload_free": (C.c_int, [P(Workload)]),
+        "cs_generate_sdf": (C.c_int, [P(Settings), P(C.c_float)]),
+        "cs_generate_sdf_class": (C.c_int, [P(Settings), P(C.c_float), P(C.c_uint8)]),
+        "cs_device_count": (C.c_int, [P(C.c_int)]),
+        "cs_ctx_create": (C.c_int, [C.c_int, P(C.c_void_p)]),
+        "cs_ctx_destroy": (C.c_int, [C.c_void_p]),
+        "cs_sim_create": (C.c_int, [C.c_void_p, P(Settings), P(Scene), P(C.c_void_p)]),
+        "cs_sim_destroy": (C.c_int, [C.c_void_p]),
+        "cs_sim_reset": (C.c_int, [C.c_void_p]),
+        "cs_sim_step_frame": (C.c_int, [C.c_void_p, C.c_float]),
+        "cs_sim_synchronize": (C.c_int, [C.c_void_p]),
+        "cs_sim_set_kettle_angle": (C.c_int, [C.c_void_p, C.c_float]),
+        "cs_sim_set_spout_position": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+        "cs_sim_set_spout_target": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+        "cs_sim_get_counts": (C.c_int, [C.c_void_p, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
+        "cs_sim_get_time": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_uint32)]),
+        "cs_sim_get_inflow": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_float)]),
+        "cs_sim_write_particles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_float), P(C.c_float)]),
+        "cs_sim_set_particle_count": (C.c_int, [C.c_void_p, C.c_uint32]),
+        "cs_sim_read_particles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_float)]),
+        "cs_sim_read_affine": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_float)]),
+        "cs_sim_read_p2g_lanes": (C.c_int, [C.c_void_p, P(C.c_int32)]),
+        "cs_sim_read_grid_vel": (C.c_int, [C.c_void_p, P(C.c_float)]),
+        "cs_sim_read_pressure": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
+        "cs_sim_read_metrics": (C.c_int, [C.c_void_p, P(C.c_uint32)]),
+        "cs_sim_get_solve_stats": (C.c_int, [C.c_void_p, P(SolveStats)]),
+        "cs_sim_set_warm_start": (C.c_int, [C.c_void_p, P(C.c_float)]),
+        "cs_sim_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, P(C.c_size_t)]),
+        "cs_sim_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+        "cs_sim_get_pass_times": (C.c_int, [C.c_void_p, P(C.c_float)]),
+        "cs_sim_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
+        "cs_device_expf": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_uint32]),
+        "cs_sim_get_kernel_timing": (C.c_int, [C.c_void_p, P(abi.KernelTiming)]),
+        "cs_sim_reset_kernel_timing": (C.c_int, [C.c_void_p]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def load_library() -> C.CDLL:
+    """Loads the native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m coffee_sim_amd.build` "
+                "(there is no fallback implementation)"
+            )
+        lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_LOCAL | getattr(os, "RTLD_NOW", 2))
+        _bind(lib)
+        _lib = lib
+    return _lib
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != abi.CS_OK:
+        msg = load_library().cs_last_error()
+        raise CoffeeSimError(rc, where, msg.decode() if msg else "")
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def preset(name: str = "default_v60") -> Settings:
+    """`MpmSettings::{default_v60, benchmark_free_stream, benchmark_center_pour}` (mod.rs:106-161)."""
+    s = Settings()
+    _check(load_library().cs_settings_preset(abi.PRESETS[name], C.byref(s)), "cs_settings_preset")
+    return s
+
+
+class SeededWorkload:
+    """A seeded synthetic workload (settings + scene arrays) owned by the library."""
+
+    def __init__(self, name: str, seed: int = 1, scale: int = 1):
+        self._w = Workload()
+        self.name = name
+        _check(load_library().cs_workload_build(abi.WORKLOADS[name], seed, scale, C.byref(self._w)),
+               "cs_workload_build")
+
+    @property
+    def settings(self) -> Settings:
+        return self._w.settings
+
+    @property
+    def scene(self) -> Scene:
+        return self._w.scene
+
+    @property
+    def frame_dt(self) -> float:
+        return float(self._w.frame_dt)
+
+    @property
+    def substeps_total(self) -> int:
+        return int(self._w.substeps_total)
+
+    @property
+    def total_particles(self) -> int:
+        return int(self._w.total_particles)
+
+    def arrays(self) -> dict:
+        s, sc = self.settings, self.scene
+        nc = s.num_cells
+        ns = int(s.sdf_res) ** 3
+        n = int(sc.num_particles)
+        out = {
+            "sdf": np.ctypeslib.as_array(sc.sdf, shape=(ns,)).copy(),
+            "sdf_class": np.ctypeslib.as_array(sc.sdf_class, shape=(nc,)).copy(),
+            "particles": np.ctypeslib.as_array(sc.particles, shape=(n * 8,)).reshape(n, 8).copy(),
+        }
+        if sc.bed_lookup:
+            out["bed_lookup"] = np.ctypeslib.as_array(sc.bed_lookup, shape=(nc,)).copy()
+            nr = int(sc.num_bed_records)
+            rec = np.ctypeslib.as_array(C.cast(sc.bed_records, C.POINTER(C.c_float)), shape=(nr * 8,))
+            out["bed_records"] = rec.reshape(nr, 8).copy()
+        return out
+
+    def close(self) -> None:
+        if self._w._owner:
+            load_library().cs_workload_free(C.byref(self._w))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """Device + stream (the wgpu Device/Queue pair the reference borrows, mod.rs:196)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(load_library().cs_ctx_create(device, C.byref(self._h)), "cs_ctx_create")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_expf(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.empty_like(x)
+        _check(load_library().cs_device_expf(self._h, _fptr(x), _fptr(out), x.size), "cs_device_expf")
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            load_library().cs_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MpmSim3D:
+    """The reference's `MpmSim3D` over the C ABI (mod.rs:179-567)."""
+
+    def __init__(self, ctx: Context, settings: Settings, scene: Scene | None = None):
+        self.ctx = ctx
+        self.settings = settings.copy()
+        self._h = C.c_void_p()
+        sc = C.byref(scene) if scene is not None else None
+        _check(load_library().cs_sim_create(ctx.handle, C.byref(self.settings), sc, C.byref(self._h)),
+               "cs_sim_create")
+        self.nc = self.settings.num_cells
+        self.dims = tuple(int(d) for d in self.settings.grid_dims)
+
+    # ---- lifecycle -----------------------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            load_library().cs_sim_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self) -> None:
+        _check(load_library().cs_sim_reset(self._h), "cs_sim_reset")
+
+    def step_frame(self, dt: float) -> None:
+        _check(load_library().cs_sim_step_frame(self._h, C.c_float(dt)), "cs_sim_step_frame")
+
+    def synchronize(self) -> None:
+        _check(load_library().cs_sim_synchronize(self._h), "cs_sim_synchronize")
+
+    # ---- setters / getters (mod.rs:448-538) --------------------------------------------
+    def set_kettle_angle(self, angle_deg: float) -> None:
+        _check(load_library().cs_sim_set_kettle_angle(self._h, angle_deg), "set_kettle_angle")
+
+    def set_spout_position(self, x: float, y: float, z: float) -> None:
+        _check(load_library().cs_sim_set_spout_position(self._h, x, y, z), "set_spout_position")
+
+    def set_spout_target(self, x: float, y: float, z: float) -> None:
+        _check(load_library().cs_sim_set_spout_target(self._h, x, y, z), "set_spout_target")
+
+    def counts(self) -> tuple[int, int, int]:
+        w, b, m = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(load_library().cs_sim_get_counts(self._h, C.byref(w), C.byref(b), C.byref(m)), "get_counts")
+        return w.value, b.value, m.value
+
+    def particle_count(self) -> int:
+        w, b, _ = self.counts()
+        return w + b
+
+    def water_slots_used(self) -> int:
+        return self.counts()[0]
+
+    def times(self) -> tuple[float, float, int]:
+        t, m, d = C.c_float(), C.c_float(), C.c_uint32()
+        _check(load_library().cs_sim_get_time(self._h, C.byref(t), C.byref(m), C.byref(d)), "get_time")
+        return t.value, m.value, d.value
+
+    def total_time(self) -> float:
+        return self.times()[0]
+
+    def inflow(self) -> tuple[float, float, float]:
+        a, f, e = C.c_float(), C.c_float(), C.c_float()
+        _check(load_library().cs_sim_get_inflow(self._h, C.byref(a), C.byref(f), C.byref(e)), "get_inflow")
+        return a.value, f.value, e.value
+
+    # ---- data in / out ----------------------------------------------------------------
+    def write_particles(self, offset: int, particles: np.ndarray, affine: np.ndarray | None = None) -> None:
+        p = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 8)
+        a = None if affine is None else np.ascontiguousarray(affine, dtype=np.float32).reshape(-1, 12)
+        _check(load_library().cs_sim_write_particles(self._h, offset, p.shape[0], _fptr(p),
+                                                     None if a is None else _fptr(a)), "write_particles")
+
+    def set_particle_count(self, n: int) -> None:
+        _check(load_library().cs_sim_set_particle_count(self._h, n), "set_particle_count")
+
+    def read_particles(self, offset: int = 0, count: int | None = None) -> np.ndarray:
+        if count is None:
+            count = self.particle_count() - offset
+        out = np.empty((count, 8), np.float32)
+        _check(load_library().cs_sim_read_particles(self._h, offset, count, _fptr(out)), "read_particles")
+        return out
+
+    def read_affine(self, offset: int = 0, count: int | None = None) -> np.ndarray:
+        if count is None:
+            count = self.particle_count() - offset
+        out = np.empty((count, 12), np.float32)
+        _check(load_library().cs_sim_read_affine(self._h, offset, count, _fptr(out)), "read_affine")
+        return out
+
+    def read_p2g_lanes(self) -> np.ndarray:
+        out = np.empty((6, self.nc), np.int32)
+        _check(load_library().cs_sim_read_p2g_lanes(self._h, _iptr(out)), "read_p2g_lanes")
+        return out
+
+    def read_grid_vel(self) -> np.ndarray:
+        out = np.empty((self.nc, 4), np.float32)
+        _check(load_library().cs_sim_read_grid_vel(self._h, _fptr(out)), "read_grid_vel")
+        return out
+
+    def read_pressure(self) -> tuple[np.ndarray, np.ndarray]:
+        p = np.empty(self.nc, np.int32)
+        k = np.empty(self.nc, np.int32)
+        _check(load_library().cs_sim_read_pressure(self._h, _iptr(p), _iptr(k)), "read_pressure")
+        return p, k
+
+    def read_metrics(self) -> np.ndarray:
+        out = np.empty(abi.CS_METRICS_SLOTS, np.uint32)
+        _check(load_library().cs_sim_read_metrics(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32))),
+               "read_metrics")
+        return out
+
+    def solve_stats(self) -> SolveStats:
+        st = SolveStats()
+        _check(load_library().cs_sim_get_solve_stats(self._h, C.byref(st)), "get_solve_stats")
+        return st
+
+    def set_warm_start(self, pressure: np.ndarray) -> None:
+        p = np.ascontiguousarray(pressure, dtype=np.float32).reshape(-1)
+        if p.size != self.nc:
+            raise ValueError("warm start must hold one value per cell")
+        _check(load_library().cs_sim_set_warm_start(self._h, _fptr(p)), "set_warm_start")
+
+    def save(self) -> bytes:
+        n = C.c_size_t()
+        _check(load_library().cs_sim_save(self._h, None, 0, C.byref(n)), "save(size)")
+        buf = C.create_string_buffer(n.value)
+        _check(load_library().cs_sim_save(self._h, buf, n.value, C.byref(n)), "save")
+        return buf.raw[: n.value]
+
+    def load(self, blob: bytes) -> None:
+        buf = C.create_string_buffer(blob, len(blob))
+        _check(load_library().cs_sim_load(self._h, buf, len(blob)), "load")
+
+    def enable_timing(self, on: bool = True) -> None:
+        _check(load_library().cs_sim_enable_timing(self._h, int(on)), "enable_timing")
+
+    def pass_times_us(self) -> np.ndarray:
+        out = np.zeros(7, np.float32)
+        _check(load_library().cs_sim_get_pass_times(self._h, _fptr(out)), "get_pass_times")
+        return out
+
+    def kernel_timing(self) -> abi.KernelTiming:
+        kt = abi.KernelTiming()
+        _check(load_library().cs_sim_get_kernel_timing(self._h, C.byref(kt)), "get_kernel_timing")
+        return kt
+
+    def reset_kernel_timing(self) -> None:
+        _check(load_library().cs_sim_reset_kernel_timing(self._h), "reset_kernel_timing")
+
+
+def generate_sdf(settings: Settings) -> np.ndarray:
+    out = np.empty(int(settings.sdf_res) ** 3, np.float32)
+    _check(load_library().cs_generate_sdf(C.byref(settings), _fptr(out)), "cs_generate_sdf")
+    return out
+
+
+def generate_sdf_class(settings: Settings, sdf: np.ndarray) -> np.ndarray:
+    sdf = np.ascontiguousarray(sdf, dtype=np.float32)
+    out = np.empty(settings.num_cells, np.uint8)
+    _check(load_library().cs_generate_sdf_class(C.byref(settings), _fptr(sdf),
+                                                out.ctypes.data_as(C.POINTER(C.c_uint8))),
+           "cs_generate_sdf_class")
+    return out
+
+
+def device_count() -> int:
+    n = C.c_int()
+    _check(load_library().cs_device_count(C.byref(n)), "cs_device_count")
+    return n.value

@@ -1,0 +1,145 @@
+// cs_common.h — parameter block and binary32 helpers shared by the host code
+// and the gfx950 kernels of the MLS-MPM substep.
+//
+// Numeric contract (identical on host and device, so every pass is
+// reproducible bit for bit): binary32 arithmetic, no FMA contraction
+// (-ffp-contract=off), correctly rounded division and sqrt, NaN-avoiding
+// min/max whose ties return the first operand, float->i32 that truncates,
+// saturates and maps NaN to 0, and WGSL's mix(a,b,t) = a*(1-t) + b*t.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/coffee_sim.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CS_HD __host__ __device__ __forceinline__
+#else
+#define CS_HD static inline
+#endif
+
+// Fixed-point encoding of the grid lanes (state.rs:8-18).
+#define CS_FP_SCALE 262144.0f
+#define CS_FP_VALUE_LIMIT 8192.0f
+#define CS_METRICS_DIV_FP_SCALE 1024.0f
+#define CS_SDF_NO_CONSTRAINT 999.0f
+
+// Cell kinds (shader.rs:174-178).
+#define CS_CELL_AIR 0
+#define CS_CELL_SURFACE 1
+#define CS_CELL_INTERIOR 2
+#define CS_CELL_BED 3
+#define CS_CELL_SOLID 4
+
+// Packed per-cell kind/fill word used by the pressure passes: fluid cells hold
+// their liquid fill fraction (surface cells, in [0,1]), 1.0 (interior) or 2.0
+// (bed, fill 1); air is -1 and solid is -2.
+#define CS_KF_AIR -1.0f
+#define CS_KF_SOLID -2.0f
+#define CS_KF_BED 2.0f
+
+// Metric slots (shader.rs:67-71) plus this build's residual bookkeeping.
+#define CS_M_MAX_ABS_DIV 0
+#define CS_M_FLUID_CELLS 1
+#define CS_M_DIV_CLAMP 2
+#define CS_M_P_CLAMP 3
+#define CS_M_OVERFLOW 4
+
+CS_HD float cs_min(float a, float b) { return (a != a) ? b : ((b < a) ? b : a); }
+CS_HD float cs_max(float a, float b) { return (a != a) ? b : ((b > a) ? b : a); }
+CS_HD float cs_clamp(float x, float lo, float hi) { return cs_min(cs_max(x, lo), hi); }
+CS_HD float cs_mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+CS_HD float cs_smoothstep(float lo, float hi, float x) {
+  float t = cs_clamp((x - lo) / (hi - lo), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+CS_HD int32_t cs_f2i(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return 2147483647;
+  if (x <= -2147483648.0f) return (-2147483647 - 1);
+  return (int32_t)x;
+}
+CS_HD uint32_t cs_f2u(float x) {
+  if (x != x || x <= 0.0f) return 0u;
+  if (x >= 4294967296.0f) return 0xffffffffu;
+  return (uint32_t)x;
+}
+
+// exp(x): Cody-Waite reduction x = k ln2 + r with ln2 split so k*ln2_hi is
+// exact, then the degree-7 Taylor polynomial of e^r in Horner form
+// (|r| <= ln2/2, truncation below 1e-8 relative), scaled by 2^k through the
+// exponent field.
+CS_HD float cs_expf(float x) {
+  if (x != x) return x;
+  if (x > 88.0f) x = 88.0f;
+  if (x < -87.0f) x = -87.0f;
+  const float log2e = 1.44269502f;
+  const float ln2_hi = 0.693145751953125f;
+  const float ln2_lo = 1.42860677e-06f;
+  float kf = floorf(x * log2e + 0.5f);
+  float r = (x - kf * ln2_hi) - kf * ln2_lo;
+  float p = 1.0f / 5040.0f;
+  p = p * r + 1.0f / 720.0f;
+  p = p * r + 1.0f / 120.0f;
+  p = p * r + 1.0f / 24.0f;
+  p = p * r + 1.0f / 6.0f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  int32_t k = (int32_t)kf;
+  uint32_t bits = (uint32_t)(k + 127) << 23;
+  float scale;
+  __builtin_memcpy(&scale, &bits, 4);
+  return p * scale;
+}
+
+struct cs_f3 {
+  float x, y, z;
+};
+CS_HD cs_f3 f3(float x, float y, float z) { return cs_f3{x, y, z}; }
+CS_HD cs_f3 operator+(cs_f3 a, cs_f3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+CS_HD cs_f3 operator-(cs_f3 a, cs_f3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+CS_HD cs_f3 operator*(cs_f3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+CS_HD cs_f3 operator/(cs_f3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+CS_HD float dot3(cs_f3 a, cs_f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+CS_HD float len3(cs_f3 a) { return sqrtf(dot3(a, a)); }
+CS_HD cs_f3 mix3(cs_f3 a, cs_f3 b, float t) {
+  return f3(cs_mix(a.x, b.x, t), cs_mix(a.y, b.y, t), cs_mix(a.z, b.z, t));
+}
+
+// Per-substep parameter block: the reference's 256 B MpmUniforms
+// (state.rs:31-56, values from write_uniforms mod.rs:569-670) plus the
+// geometry and bed extensions.
+struct cs_params {
+  uint32_t gx, gy, gz, nc;
+  uint32_t num_particles, num_bed_cells, max_particles, use_sdf_cache;
+  float dt, gravity, dx, inv_dx;
+  float origin[3];
+  float bmax[3];
+  float bulk_K, viscosity, nominal_mass, p_vol;
+  float fp_scale, inv_fp_scale, vel_cap;
+  float sdf_res_f;
+  uint32_t sdf_res;
+  float friction, restitution, contact_offset;
+  float water_nu, min_k;
+  float div_clamp, pressure_clamp, metrics_div_fp_scale;
+  float j_alpha, j_expand_alpha, max_rest_fraction;
+  float wall_half;
+  cs_geometry geo;
+};
+
+// Static per-bed-cell record derived once from cs_bed_record (field mode of
+// shader.rs:361-666): everything of the bed formulas that does not depend on
+// the particles.
+struct cs_bed_cell {
+  float k_record;       // bed_compacted_permeability of the cell's record
+  float v_record[3];    // record velocity (G2P bed velocity sum)
+  float pore_capacity;  // shader.rs:654-658
+  float overfill_resp;  // shader.rs:660-662
+  float porosity;       // clamp(bed_compacted_porosity, 0.08, 1)
+  float solid_fraction; // 1 - porosity
+  float solid_div;      // bed_velocity_divergence(cell)
+  float bmv[3];         // bed_matrix_velocity_cell(cell)
+};

@@ -1,0 +1,41 @@
+// cs_kernels.h — host-side launchers for the gfx950 kernels in cs_kernels.hip.
+// Pointers are device pointers; every launcher enqueues on `stream` and
+// returns the launch status.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cs_common.h"
+
+hipError_t cs_launch_fill_i32(hipStream_t stream, int32_t* p, size_t n, int32_t v);
+hipError_t cs_launch_copy_i32(hipStream_t stream, const int32_t* src, int32_t* dst, size_t n);
+hipError_t cs_launch_p2g(hipStream_t stream, const cs_params& u, const float* part, const float* aff,
+                         int32_t* lanes, uint32_t* metrics);
+hipError_t cs_launch_grid(hipStream_t stream, const cs_params& u, const int32_t* lanes, float* gvel,
+                          const float* sdf);
+hipError_t cs_launch_classify(hipStream_t stream, const cs_params& u, const float* gvel, const uint8_t* cls,
+                              const int32_t* lanes, const int32_t* bed_id, const cs_bed_cell* bed, int8_t* kind,
+                              float* kf, int32_t* pres, float* rhs, int32_t* pack, uint32_t* metrics,
+                              const float* sdf);
+hipError_t cs_launch_warm_previous(hipStream_t stream, const cs_params& u, const float* kf, const int32_t* prev,
+                                   int32_t* pres);
+hipError_t cs_launch_warm_external(hipStream_t stream, const cs_params& u, const float* kf, const float* warm,
+                                   int32_t* pres, uint32_t* metrics);
+hipError_t cs_launch_rbgs_pair(hipStream_t stream, const cs_params& u, const float* kf, const float* rhs,
+                               int32_t* pres, uint32_t* metrics);
+hipError_t cs_launch_residual(hipStream_t stream, const cs_params& u, const float* kf, const float* rhs,
+                              const int32_t* pres, double* partials, uint32_t max_blocks, double* out);
+hipError_t cs_launch_project(hipStream_t stream, const cs_params& u, float* gvel, const float* kf,
+                             const int32_t* pres, const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf);
+hipError_t cs_launch_pack(hipStream_t stream, const cs_params& u, float* gvel, const float* kf, const int32_t* pack,
+                          const float* sdf);
+hipError_t cs_launch_visc(hipStream_t stream, const cs_params& u, const float* gin, float* gout, const uint8_t* cls,
+                          const float* sdf);
+hipError_t cs_launch_g2p(hipStream_t stream, const cs_params& u, float* part, float* aff, const float* gvel,
+                         const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf);
+hipError_t cs_launch_expf(hipStream_t stream, const float* in, float* out, uint32_t n);
+
+// Blocks used by the residual reduction (size of the partials buffer / 2).
+#define CS_RESIDUAL_BLOCKS 1024u

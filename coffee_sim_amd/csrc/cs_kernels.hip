@@ -1,0 +1,892 @@
+// cs_kernels.hip — gfx950 kernels of the MLS-MPM substep.
+//
+// Each kernel restates one or more WGSL passes of the reference solver
+// (crates/sim-wasm/src/mpm_3d/shader.rs) with the same binary32 operation
+// order, so a pass run here equals the same pass on the CPU bit for bit.
+// Passes that only touch their own cell are fused into the pass before them
+// where the reference's pass order allows it (mod.rs:339-383):
+//   k_grid      = grid_update + boundary_project
+//   k_project   = project_pressure + boundary_project
+//   k_pack      = packing_apply + boundary_project (packing_prepare is folded
+//                 into k_classify: it only reads the cell's kind and volumes)
+//   k_visc      = viscosity_prepare + viscosity_apply + boundary_project,
+//                 double-buffered instead of the reference's lane scratch.
+#include "cs_common.h"
+#include "cs_kernels.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t cell_index(const cs_params& u, uint32_t ix, uint32_t iy, uint32_t iz) {
+  return iz * u.gx * u.gy + iy * u.gx + ix;  // shader.rs:148-150
+}
+__device__ __forceinline__ bool on_grid(const cs_params& u, int x, int y, int z) {
+  return !(x < 0 || y < 0 || z < 0) && !((uint32_t)x >= u.gx || (uint32_t)y >= u.gy || (uint32_t)z >= u.gz);
+}
+__device__ __forceinline__ void decompose(const cs_params& u, uint32_t idx, uint32_t& ix, uint32_t& iy, uint32_t& iz) {
+  uint32_t plane = u.gx * u.gy;
+  iz = idx / plane;
+  uint32_t rem = idx % plane;
+  iy = rem / u.gx;
+  ix = rem % u.gx;
+}
+__device__ __forceinline__ cs_f3 cell_center(const cs_params& u, uint32_t ix, uint32_t iy, uint32_t iz) {
+  return f3(u.origin[0] + ((float)ix + 0.5f) * u.dx, u.origin[1] + ((float)iy + 0.5f) * u.dx,
+            u.origin[2] + ((float)iz + 0.5f) * u.dx);
+}
+
+// ---- SDF (shader.rs:673-715) ------------------------------------------------
+__device__ __forceinline__ float sdf_texel(const cs_params& u, const float* __restrict__ sdf, int x, int y, int z) {
+  int mx = (int)u.sdf_res - 1;
+  x = x < 0 ? 0 : (x > mx ? mx : x);
+  y = y < 0 ? 0 : (y > mx ? mx : y);
+  z = z < 0 ? 0 : (z > mx ? mx : z);
+  return sdf[((size_t)z * u.sdf_res + (size_t)y) * u.sdf_res + (size_t)x];
+}
+__device__ float sample_sdf(const cs_params& u, const float* __restrict__ sdf, cs_f3 p) {
+  float res = u.sdf_res_f;
+  float ux = (p.x + u.bmax[0]) / (u.bmax[0] * 2.0f) * res - 0.5f;
+  float uy = (p.y + u.bmax[1]) / (u.bmax[1] * 2.0f) * res - 0.5f;
+  float uz = (p.z + u.bmax[2]) / (u.bmax[2] * 2.0f) * res - 0.5f;
+  float flx = floorf(ux), fly = floorf(uy), flz = floorf(uz);
+  int bx = cs_f2i(flx), by = cs_f2i(fly), bz = cs_f2i(flz);
+  float fx = ux - flx, fy = uy - fly, fz = uz - flz;
+  float c000 = sdf_texel(u, sdf, bx, by, bz);
+  float c100 = sdf_texel(u, sdf, bx + 1, by, bz);
+  float c010 = sdf_texel(u, sdf, bx, by + 1, bz);
+  float c110 = sdf_texel(u, sdf, bx + 1, by + 1, bz);
+  float c001 = sdf_texel(u, sdf, bx, by, bz + 1);
+  float c101 = sdf_texel(u, sdf, bx + 1, by, bz + 1);
+  float c011 = sdf_texel(u, sdf, bx, by + 1, bz + 1);
+  float c111 = sdf_texel(u, sdf, bx + 1, by + 1, bz + 1);
+  float c00 = cs_mix(c000, c100, fx);
+  float c10 = cs_mix(c010, c110, fx);
+  float c01 = cs_mix(c001, c101, fx);
+  float c11 = cs_mix(c011, c111, fx);
+  float c0 = cs_mix(c00, c10, fy);
+  float c1 = cs_mix(c01, c11, fy);
+  return cs_mix(c0, c1, fz);
+}
+__device__ cs_f3 sdf_gradient(const cs_params& u, const float* __restrict__ sdf, cs_f3 p) {
+  float e = u.dx;
+  float gxv = sample_sdf(u, sdf, f3(p.x + e, p.y + 0.0f, p.z + 0.0f)) - sample_sdf(u, sdf, f3(p.x - e, p.y - 0.0f, p.z - 0.0f));
+  float gyv = sample_sdf(u, sdf, f3(p.x + 0.0f, p.y + e, p.z + 0.0f)) - sample_sdf(u, sdf, f3(p.x - 0.0f, p.y - e, p.z - 0.0f));
+  float gzv = sample_sdf(u, sdf, f3(p.x + 0.0f, p.y + 0.0f, p.z + e)) - sample_sdf(u, sdf, f3(p.x - 0.0f, p.y - 0.0f, p.z - e));
+  cs_f3 g = f3(gxv, gyv, gzv);
+  float len = len3(g);
+  if (len < 1e-8f) return f3(0.0f, 0.0f, 0.0f);
+  return g / len;
+}
+
+// boundary_project body (shader.rs:1638-1678) for a cell with mass >= 1e-6.
+__device__ cs_f3 boundary_apply(const cs_params& u, const float* __restrict__ sdf, cs_f3 v, cs_f3 cp) {
+  float sv = sample_sdf(u, sdf, cp);
+  if (sv < u.contact_offset) {
+    cs_f3 n = sdf_gradient(u, sdf, cp);
+    float vn = dot3(v, n);
+    if (vn < 0.0f) {
+      v = v - (n * vn) * (1.0f + u.restitution);
+      cs_f3 vt = v - n * dot3(v, n);
+      float vtl = len3(vt);
+      if (vtl > 1e-6f) {
+        float fi = cs_min(u.friction * fabsf(vn), vtl);
+        v = v - vt * (fi / vtl);
+      }
+    }
+  }
+  float margin = 2.0f * u.dx;
+  float bminx = u.origin[0] + margin, bminy = u.origin[1] + margin, bminz = u.origin[2] + margin;
+  float bmaxx = u.bmax[0] - margin, bmaxy = u.bmax[1] - margin, bmaxz = u.bmax[2] - margin;
+  if (cp.x < bminx && v.x < 0.0f) v.x = 0.0f;
+  if (cp.x > bmaxx && v.x > 0.0f) v.x = 0.0f;
+  if (cp.y < bminy && v.y < 0.0f) v.y = 0.0f;
+  if (cp.y > bmaxy && v.y > 0.0f) v.y = 0.0f;
+  if (cp.z < bminz && v.z < 0.0f) v.z = 0.0f;
+  if (cp.z > bmaxz && v.z > 0.0f) v.z = 0.0f;
+  return v;
+}
+
+__device__ __forceinline__ cs_f3 cap_speed(const cs_params& u, cs_f3 v) {
+  float speed = len3(v);
+  if (speed > u.vel_cap) v = v * (u.vel_cap / speed);
+  return v;
+}
+
+// ---- contact for particles (shader.rs:717-903, water only) -------------------
+struct contact {
+  cs_f3 pos, vel;
+};
+__device__ contact radial_barrier(const cs_params& u, cs_f3 pos, cs_f3 vel, float max_r) {
+  float rx = pos.x - 0.0f, rz = pos.z - 0.0f;
+  float r = sqrtf(rx * rx + rz * rz);
+  if (r > max_r && r > 1e-6f) {
+    float ox = rx / r, oz = rz / r;
+    pos.x = 0.0f + ox * max_r;
+    pos.z = 0.0f + oz * max_r;
+    float vn = vel.x * ox + vel.z * oz;
+    if (vn > 0.0f) {
+      float tx = vel.x - ox * vn, tz = vel.z - oz * vn;
+      vel.x = tx * (1.0f - u.friction * 0.35f);
+      vel.z = tz * (1.0f - u.friction * 0.35f);
+    }
+  }
+  return contact{pos, vel};
+}
+__device__ contact scene_obstacles(const cs_params& u, cs_f3 pos, cs_f3 vel) {
+  const cs_geometry& g = u.geo;
+  if (g.cone_enabled && pos.y <= g.cone_top_y && pos.y >= g.cone_bot_y) {
+    float t = cs_clamp((pos.y - g.cone_bot_y) / (g.cone_top_y - g.cone_bot_y), 0.0f, 1.0f);
+    float cone_r = cs_mix(g.cone_bot_radius, g.cone_top_radius, t) - u.contact_offset;
+    contact c = radial_barrier(u, pos, vel, cone_r);
+    pos = c.pos;
+    vel = c.vel;
+  }
+  if (g.cup_enabled && pos.y <= g.cup_top_y) {
+    float cup_r = g.cup_radius - u.wall_half - u.contact_offset;
+    contact c = radial_barrier(u, pos, vel, cup_r);
+    pos = c.pos;
+    vel = c.vel;
+    float floor_y = g.cup_bottom_y + u.wall_half + u.contact_offset;
+    if (pos.y < floor_y) {
+      pos.y = floor_y;
+      if (vel.y < 0.0f) {
+        vel.y = 0.0f;
+        vel.x *= 1.0f - u.friction * 0.55f;
+        vel.z *= 1.0f - u.friction * 0.55f;
+      }
+    }
+  }
+  return contact{pos, vel};
+}
+__device__ contact sdf_contact(const cs_params& u, const float* __restrict__ sdf, cs_f3 pos, cs_f3 vel) {
+  float sv = sample_sdf(u, sdf, pos);
+  if (sv < u.contact_offset) {
+    cs_f3 n = sdf_gradient(u, sdf, pos);
+    if (len3(n) > 1e-6f) {
+      pos = pos + n * (u.contact_offset - sv);
+      float vn = dot3(vel, n);
+      if (vn < 0.0f) {
+        vel = vel - (n * vn) * (1.0f + u.restitution);
+        cs_f3 vt = vel - n * dot3(vel, n);
+        float vtl = len3(vt);
+        if (vtl > 1e-6f) {
+          float fi = cs_min(u.friction * fabsf(vn), vtl);
+          vel = vel - vt * (fi / vtl);
+        }
+      }
+    }
+  }
+  return scene_obstacles(u, pos, vel);
+}
+
+// ---- kind/fill word ------------------------------------------------------------
+__device__ __forceinline__ bool kf_fluid(float kf) { return kf >= 0.0f; }
+__device__ __forceinline__ bool kf_solid(float kf) { return kf == CS_KF_SOLID; }
+__device__ __forceinline__ bool kf_bed(float kf) { return kf == CS_KF_BED; }
+__device__ __forceinline__ float kf_fill(float kf) { return kf == CS_KF_BED ? 1.0f : kf; }
+// pressure_face_weight (shader.rs:306-323) for a non-solid neighbour.
+__device__ __forceinline__ float face_w(float self_kf, float self_fill, float nkf) {
+  if (kf_bed(self_kf) || kf_bed(nkf)) return 1.0f;
+  if (!kf_fluid(nkf)) return self_fill;
+  return cs_min(self_fill, kf_fill(nkf));
+}
+
+__constant__ int c_off6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+
+// volume_projection_target_divergence (shader.rs:597-619)
+__device__ float volume_target_div(const cs_params& u, float rest, float cur) {
+  float cv = u.dx * u.dx * u.dx;
+  float eps = cs_max(cv * 1e-5f, u.p_vol * 1e-3f);
+  if (rest <= eps) return 0.0f;
+  float j = cur / rest;
+  float comp = cs_clamp(1.0f - j, 0.0f, 0.75f);
+  float expd = cs_clamp(j - 1.0f, 0.0f, 0.75f);
+  float target = comp * u.j_alpha - expd * u.j_expand_alpha;
+  float packed = cs_max(rest, cur) / cs_max(cv, 1e-8f);
+  float over = cs_max(packed - u.max_rest_fraction, 0.0f);
+  return cs_max(target, over * u.j_alpha);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+__global__ void k_fill_i32(int32_t* __restrict__ p, size_t n, int32_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// ---- P2G (shader.rs:955-1052), one thread per particle, global i32 atomics -------
+__global__ void __launch_bounds__(256) k_p2g_atomic(cs_params u, const float4* __restrict__ part,
+                                                    const float4* __restrict__ aff, int32_t* __restrict__ lanes,
+                                                    uint32_t* __restrict__ metrics) {
+  uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= u.num_particles) return;
+  float4 P = part[2 * pid], Vv = part[2 * pid + 1];
+  float4 A0 = aff[3 * pid], A1 = aff[3 * pid + 1], A2 = aff[3 * pid + 2];
+  float mass_p = Vv.w, J = P.w, phase = A0.w;
+  if (!(phase < 0.5f) || mass_p <= u.nominal_mass * 0.10f) return;
+  cs_f3 gp = (f3(P.x, P.y, P.z) - f3(u.origin[0], u.origin[1], u.origin[2])) * u.inv_dx;
+  int bx = cs_f2i(floorf(gp.x - 0.5f)), by = cs_f2i(floorf(gp.y - 0.5f)), bz = cs_f2i(floorf(gp.z - 0.5f));
+  cs_f3 fx = f3(gp.x - (float)bx, gp.y - (float)by, gp.z - (float)bz);
+  float wx[3], wy[3], wz[3];
+  wx[0] = 0.5f * (1.5f - fx.x) * (1.5f - fx.x);
+  wx[1] = 0.75f - (fx.x - 1.0f) * (fx.x - 1.0f);
+  wx[2] = 0.5f * (fx.x - 0.5f) * (fx.x - 0.5f);
+  wy[0] = 0.5f * (1.5f - fx.y) * (1.5f - fx.y);
+  wy[1] = 0.75f - (fx.y - 1.0f) * (fx.y - 1.0f);
+  wy[2] = 0.5f * (fx.y - 0.5f) * (fx.y - 0.5f);
+  wz[0] = 0.5f * (1.5f - fx.z) * (1.5f - fx.z);
+  wz[1] = 0.75f - (fx.z - 1.0f) * (fx.z - 1.0f);
+  wz[2] = 0.5f * (fx.z - 0.5f) * (fx.z - 0.5f);
+  cs_f3 ac0 = f3(mass_p * A0.x, mass_p * A0.y, mass_p * A0.z);
+  cs_f3 ac1 = f3(mass_p * A1.x, mass_p * A1.y, mass_p * A1.z);
+  cs_f3 ac2 = f3(mass_p * A2.x, mass_p * A2.y, mass_p * A2.z);
+  cs_f3 mv = f3(Vv.x, Vv.y, Vv.z) * mass_p;
+  float fp = u.fp_scale;
+  float rpv = u.p_vol * mass_p / cs_max(u.nominal_mass, 1e-6f);
+  float cpv = rpv * cs_clamp(J, 0.40f, 2.00f);
+  uint32_t nc = u.nc;
+  uint32_t overflow = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) {
+        int cx = bx + i, cy = by + j, cz = bz + k;
+        if (!on_grid(u, cx, cy, cz)) continue;
+        float w = wx[i] * wy[j] * wz[k];
+        cs_f3 dpos = (f3((float)i, (float)j, (float)k) - fx) * u.dx;
+        cs_f3 mom = (mv + f3(dot3(ac0, dpos), dot3(ac1, dpos), dot3(ac2, dpos))) * w;
+        float mfp = (w * mass_p) * fp, rfp = (w * rpv) * fp, cfp = (w * cpv) * fp;
+        float mxf = mom.x * fp, myf = mom.y * fp, mzf = mom.z * fp;
+        const float lim = 1.0e9f;
+        if (fabsf(mfp) > lim || fabsf(mxf) > lim || fabsf(myf) > lim || fabsf(mzf) > lim || fabsf(rfp) > lim ||
+            fabsf(cfp) > lim)
+          ++overflow;
+        uint32_t ci = cell_index(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+        atomicAdd(&lanes[ci], cs_f2i(mfp));
+        atomicAdd(&lanes[nc + ci], cs_f2i(mxf));
+        atomicAdd(&lanes[2u * nc + ci], cs_f2i(myf));
+        atomicAdd(&lanes[3u * nc + ci], cs_f2i(mzf));
+        atomicAdd(&lanes[4u * nc + ci], cs_f2i(rfp));
+        atomicAdd(&lanes[5u * nc + ci], cs_f2i(cfp));
+      }
+  if (overflow) atomicAdd(&metrics[CS_M_OVERFLOW], overflow);
+}
+
+// ---- grid_update + boundary_project (shader.rs:1056-1082, 1630-1679) -------------
+__global__ void __launch_bounds__(256) k_grid(cs_params u, const int32_t* __restrict__ lanes,
+                                              float4* __restrict__ gvel, const float* __restrict__ sdf) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc) return;
+  float inv_fp = u.inv_fp_scale;
+  float mass = (float)lanes[idx] * inv_fp;
+  if (mass < 1e-6f) {
+    gvel[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  cs_f3 v = f3((float)lanes[u.nc + idx] * inv_fp / mass, (float)lanes[2u * u.nc + idx] * inv_fp / mass,
+               (float)lanes[3u * u.nc + idx] * inv_fp / mass);
+  v.y += u.gravity * u.dt;
+  v = cap_speed(u, v);
+  uint32_t ix, iy, iz;
+  decompose(u, idx, ix, iy, iz);
+  v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
+  gvel[idx] = make_float4(v.x, v.y, v.z, mass);
+}
+
+// ---- classify_cells (shader.rs:1181-1367) + packing_prepare (shader.rs:1566-1583) ----
+// Writes kind, the kind/fill word, pressure 0, the right-hand side c = dx*dx*rhs
+// of the pressure update (shader.rs:1436-1437, from the quantised divergence)
+// and the packing pressure.
+__global__ void __launch_bounds__(256) k_classify(cs_params u, const float4* __restrict__ gvel,
+                                                  const uint8_t* __restrict__ cls, const int32_t* __restrict__ lanes,
+                                                  const int32_t* __restrict__ bed_id,
+                                                  const cs_bed_cell* __restrict__ bed, int8_t* __restrict__ kind_out,
+                                                  float* __restrict__ kf_out, int32_t* __restrict__ pres,
+                                                  float* __restrict__ rhs, int32_t* __restrict__ pack,
+                                                  uint32_t* __restrict__ metrics, const float* __restrict__ sdf) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc) return;
+  float occ = u.nominal_mass * 0.1f;
+  float mass = gvel[idx].w;
+  pres[idx] = 0;
+  uint32_t ix, iy, iz;
+  decompose(u, idx, ix, iy, iz);
+  cs_f3 cc = cell_center(u, ix, iy, iz);
+  // solid test from the cached class mask (use_sdf_cache, mod.rs:128) or from
+  // live SDF samples at the cell and neighbour centres (shader.rs:1207-1211, 1286-1333)
+  const bool cache = u.use_sdf_cache != 0u;
+  bool self_solid = cache ? cls[idx] != 0 : sample_sdf(u, sdf, cc) < 0.0f;
+  if (self_solid || mass <= occ) {
+    kind_out[idx] = self_solid ? CS_CELL_SOLID : CS_CELL_AIR;
+    kf_out[idx] = self_solid ? CS_KF_SOLID : CS_KF_AIR;
+    rhs[idx] = 0.0f;
+    pack[idx] = 0;
+    return;
+  }
+  int kind;
+  if (bed_id[idx] >= 0) {
+    kind = CS_CELL_BED;
+  } else {
+    bool air = false;
+    for (int n = 0; n < 6; ++n) {
+      int x = (int)ix + c_off6[n][0], y = (int)iy + c_off6[n][1], z = (int)iz + c_off6[n][2];
+      if (!on_grid(u, x, y, z)) { air = true; break; }
+      if (gvel[cell_index(u, x, y, z)].w <= occ) { air = true; break; }
+    }
+    kind = air ? CS_CELL_SURFACE : CS_CELL_INTERIOR;
+  }
+  float4 sv = gvel[idx];
+  float vxm = -sv.x, vxp = -sv.x, vym = -sv.y, vyp = -sv.y, vzm = -sv.z, vzp = -sv.z;
+  uint32_t plane = u.gx * u.gy;
+  float d = u.dx;
+#define CS_OPEN(nidx, px, py, pz) (cache ? !cls[nidx] : sample_sdf(u, sdf, f3(px, py, pz)) >= 0.0f)
+  if (ix > 0u && CS_OPEN(idx - 1u, cc.x + -d, cc.y + 0.0f, cc.z + 0.0f)) vxm = gvel[idx - 1u].x;
+  if (ix + 1u < u.gx && CS_OPEN(idx + 1u, cc.x + d, cc.y + 0.0f, cc.z + 0.0f)) vxp = gvel[idx + 1u].x;
+  if (iy > 0u && CS_OPEN(idx - u.gx, cc.x + 0.0f, cc.y + -d, cc.z + 0.0f)) vym = gvel[idx - u.gx].y;
+  if (iy + 1u < u.gy && CS_OPEN(idx + u.gx, cc.x + 0.0f, cc.y + d, cc.z + 0.0f)) vyp = gvel[idx + u.gx].y;
+  if (iz > 0u && CS_OPEN(idx - plane, cc.x + 0.0f, cc.y + 0.0f, cc.z + -d)) vzm = gvel[idx - plane].z;
+  if (iz + 1u < u.gz && CS_OPEN(idx + plane, cc.x + 0.0f, cc.y + 0.0f, cc.z + d)) vzp = gvel[idx + plane].z;
+#undef CS_OPEN
+  float div = 0.5f * u.inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
+  float rest = (float)lanes[4u * u.nc + idx] * u.inv_fp_scale;
+  float cur = (float)lanes[5u * u.nc + idx] * u.inv_fp_scale;
+  float target = volume_target_div(u, rest, cur);
+  float cv = u.dx * u.dx * u.dx;
+  if (kind == CS_CELL_BED) {
+    const cs_bed_cell& b = bed[bed_id[idx]];
+    // bed_pore_projection_target_divergence (shader.rs:621-666) with the
+    // particle-independent factors taken from the bed table
+    float pore = target;
+    float overfill = cs_max(cur - b.pore_capacity, 0.0f) / cs_max(cv, 1e-8f);
+    pore = cs_max(pore, overfill * b.overfill_resp);
+    target = (pore - b.solid_fraction * b.solid_div) / b.porosity;
+  }
+  float dval = div - target;
+  float dcl = cs_clamp(dval, -u.div_clamp, u.div_clamp);
+  uint32_t fires_d = dcl != dval ? 1u : 0u;
+  int32_t dfp = cs_f2i(dcl * u.fp_scale);
+  float fill;
+  if (kind == CS_CELL_SURFACE) {
+    float dep = cs_max(rest, cur) / cs_max(cv, 1e-8f);
+    fill = cs_clamp(dep, 0.0f, 1.0f);
+  } else {
+    fill = 1.0f;
+  }
+  float r = (float)dfp * u.inv_fp_scale * fill / cs_max(u.dt, 1e-6f);
+  rhs[idx] = u.dx * u.dx * r;
+  kind_out[idx] = (int8_t)kind;
+  kf_out[idx] = kind == CS_CELL_BED ? CS_KF_BED : (kind == CS_CELL_INTERIOR ? 1.0f : fill);
+  // packing_prepare
+  uint32_t fires_p = 0;
+  if (kind == CS_CELL_BED) {
+    pack[idx] = 0;
+  } else {
+    float packed = cs_max(rest, cur) / cs_max(cv, 1e-8f);
+    float over = cs_max(packed - cs_min(u.max_rest_fraction, 1.0f), 0.0f);
+    float pv = u.bulk_K * over;
+    float pcl = cs_clamp(pv, 0.0f, u.pressure_clamp);
+    fires_p = pcl != pv ? 1u : 0u;
+    pack[idx] = cs_f2i(pcl * u.fp_scale);
+  }
+  uint32_t fpd = cs_f2u(cs_clamp(fabsf(div) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f));
+  atomicMax(&metrics[CS_M_MAX_ABS_DIV], fpd);
+  atomicAdd(&metrics[CS_M_FLUID_CELLS], 1u);
+  if (fires_d) atomicAdd(&metrics[CS_M_DIV_CLAMP], 1u);
+  if (fires_p) atomicAdd(&metrics[CS_M_P_CLAMP], 1u);
+}
+
+// ---- warm starts (injected after classify zeroed the pressure) -------------------
+__global__ void k_warm_previous(cs_params u, const float* __restrict__ kf, const int32_t* __restrict__ prev,
+                                int32_t* __restrict__ pres) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc) return;
+  if (kf_fluid(kf[idx])) pres[idx] = prev[idx];
+}
+__global__ void k_warm_external(cs_params u, const float* __restrict__ kf, const float* __restrict__ warm,
+                                int32_t* __restrict__ pres, uint32_t* __restrict__ metrics) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc) return;
+  if (!kf_fluid(kf[idx])) return;
+  float v = warm[idx];
+  float cl = cs_clamp(v, -u.pressure_clamp, u.pressure_clamp);
+  if (cl != v) atomicAdd(&metrics[CS_M_P_CLAMP], 1u);
+  pres[idx] = cs_f2i(cl * u.fp_scale);
+}
+__global__ void k_copy_i32(const int32_t* __restrict__ src, int32_t* __restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// ---- red-black Gauss-Seidel half sweep (pressure_update, shader.rs:1371-1453) ------
+__global__ void __launch_bounds__(256) k_rbgs_half(cs_params u, uint32_t parity, const float* __restrict__ kf,
+                                                   const float* __restrict__ rhs, int32_t* __restrict__ pres,
+                                                   uint32_t* __restrict__ metrics) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc) return;
+  uint32_t ix, iy, iz;
+  decompose(u, idx, ix, iy, iz);
+  if (((ix + iy + iz) & 1u) != parity) return;
+  float skf = kf[idx];
+  if (!kf_fluid(skf)) return;  // non-fluid pressure is already 0
+  float sf = kf_fill(skf);
+  float count = 0.0f, sum = 0.0f;
+  for (int n = 0; n < 6; ++n) {
+    int x = (int)ix + c_off6[n][0], y = (int)iy + c_off6[n][1], z = (int)iz + c_off6[n][2];
+    if (!on_grid(u, x, y, z)) continue;
+    uint32_t ni = cell_index(u, x, y, z);
+    float nkf = kf[ni];
+    if (kf_solid(nkf)) continue;
+    float w = face_w(skf, sf, nkf);
+    if (w <= 0.0f) continue;
+    count += w;
+    if (kf_fluid(nkf)) sum += w * ((float)pres[ni] * u.inv_fp_scale);
+  }
+  float pn = 0.0f;
+  if (count > 0.0f) pn = (sum - rhs[idx]) / count;
+  float cl = cs_clamp(pn, -u.pressure_clamp, u.pressure_clamp);
+  if (cl != pn) atomicAdd(&metrics[CS_M_P_CLAMP], 1u);
+  pres[idx] = cs_f2i(cl * u.fp_scale);
+}
+
+// ---- residual ||b - A p|| / ||b|| over fluid cells (per-block partials) ------------
+__global__ void __launch_bounds__(256) k_residual(cs_params u, const float* __restrict__ kf,
+                                                  const float* __restrict__ rhs, const int32_t* __restrict__ pres,
+                                                  double* __restrict__ partials) {
+  __shared__ double s_rr[256], s_bb[256];
+  uint32_t tid = threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  for (uint32_t idx = blockIdx.x * blockDim.x + tid; idx < u.nc; idx += gridDim.x * blockDim.x) {
+    float skf = kf[idx];
+    if (!kf_fluid(skf)) continue;
+    uint32_t ix, iy, iz;
+    decompose(u, idx, ix, iy, iz);
+    float sf = kf_fill(skf);
+    float count = 0.0f;
+    double nb = 0.0;
+    for (int n = 0; n < 6; ++n) {
+      int x = (int)ix + c_off6[n][0], y = (int)iy + c_off6[n][1], z = (int)iz + c_off6[n][2];
+      if (!on_grid(u, x, y, z)) continue;
+      uint32_t ni = cell_index(u, x, y, z);
+      float nkf = kf[ni];
+      if (kf_solid(nkf)) continue;
+      float w = face_w(skf, sf, nkf);
+      if (w <= 0.0f) continue;
+      count += w;
+      if (kf_fluid(nkf)) nb += (double)w * (double)((float)pres[ni] * u.inv_fp_scale);
+    }
+    if (count <= 0.0f) continue;
+    double ap = (double)count * (double)((float)pres[idx] * u.inv_fp_scale) - nb;
+    double b = -(double)rhs[idx];
+    rr += (b - ap) * (b - ap);
+    bb += b * b;
+  }
+  s_rr[tid] = rr;
+  s_bb[tid] = bb;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      s_rr[tid] += s_rr[tid + s];
+      s_bb[tid] += s_bb[tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    partials[2 * blockIdx.x] = s_rr[0];
+    partials[2 * blockIdx.x + 1] = s_bb[0];
+  }
+}
+__global__ void k_residual_final(const double* __restrict__ partials, uint32_t nblocks, double* __restrict__ out) {
+  __shared__ double s_rr[256], s_bb[256];
+  uint32_t tid = threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  for (uint32_t i = tid; i < nblocks; i += blockDim.x) {
+    rr += partials[2 * i];
+    bb += partials[2 * i + 1];
+  }
+  s_rr[tid] = rr;
+  s_bb[tid] = bb;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      s_rr[tid] += s_rr[tid + s];
+      s_bb[tid] += s_bb[tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double r;
+    if (s_bb[0] > 0.0) r = sqrt(s_rr[0] / s_bb[0]);
+    else r = s_rr[0] > 0.0 ? __builtin_huge_val() : 0.0;
+    out[0] = r;
+  }
+}
+
+// ---- project_pressure + boundary_project (shader.rs:1457-1543, 1630-1679) ---------
+__global__ void __launch_bounds__(256) k_project(cs_params u, float4* __restrict__ gvel, const float* __restrict__ kf,
+                                                 const int32_t* __restrict__ pres, const int32_t* __restrict__ bed_id,
+                                                 const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc) return;
+  float4 gv = gvel[idx];
+  if (gv.w < 1e-6f) return;
+  uint32_t ix, iy, iz;
+  decompose(u, idx, ix, iy, iz);
+  cs_f3 v = f3(gv.x, gv.y, gv.z);
+  float skf = kf[idx];
+  if (kf_fluid(skf)) {
+    float ph = (float)pres[idx] * u.inv_fp_scale;
+    float sf = kf_fill(skf);
+    float pn[6];
+    for (int n = 0; n < 6; ++n) {
+      int x = (int)ix + c_off6[n][0], y = (int)iy + c_off6[n][1], z = (int)iz + c_off6[n][2];
+      float val = ph;
+      if (on_grid(u, x, y, z)) {
+        uint32_t ni = cell_index(u, x, y, z);
+        float nkf = kf[ni];
+        if (!kf_solid(nkf)) {
+          float np = kf_fluid(nkf) ? (float)pres[ni] * u.inv_fp_scale : 0.0f;
+          val = cs_mix(ph, np, face_w(skf, sf, nkf));
+        }
+      }
+      pn[n] = val;
+    }
+    float h = 0.5f * u.inv_dx;
+    cs_f3 grad = f3(h * (pn[1] - pn[0]), h * (pn[3] - pn[2]), h * (pn[5] - pn[4]));
+    v = v - grad * u.dt;
+    if (kf_bed(skf)) {
+      int32_t id = bed_id[idx];
+      if (id >= 0 && (uint32_t)id < u.num_bed_cells) {
+        const cs_bed_cell& b = bed[id];
+        float rate = u.water_nu / b.k_record;
+        float damping = 1.0f / (1.0f + cs_max(rate * u.dt, 0.0f));
+        cs_f3 bv = f3(b.bmv[0], b.bmv[1], b.bmv[2]);
+        v = bv + (v - bv) * damping;
+      }
+    }
+    v = cap_speed(u, v);
+  }
+  v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
+  gvel[idx] = make_float4(v.x, v.y, v.z, gv.w);
+}
+
+// ---- packing_apply + boundary_project (shader.rs:1585-1626, 1630-1679) ------------
+__global__ void __launch_bounds__(256) k_pack(cs_params u, float4* __restrict__ gvel, const float* __restrict__ kf,
+                                              const int32_t* __restrict__ pack, const float* __restrict__ sdf) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc) return;
+  float4 gv = gvel[idx];
+  if (gv.w < 1e-6f) return;
+  uint32_t ix, iy, iz;
+  decompose(u, idx, ix, iy, iz);
+  cs_f3 v = f3(gv.x, gv.y, gv.z);
+  float skf = kf[idx];
+  if (gv.w > u.nominal_mass * 0.1f && kf_fluid(skf)) {
+    float ph = (float)pack[idx] * u.inv_fp_scale;
+    float pn[6];
+    for (int n = 0; n < 6; ++n) {
+      int x = (int)ix + c_off6[n][0], y = (int)iy + c_off6[n][1], z = (int)iz + c_off6[n][2];
+      float val = ph;
+      if (on_grid(u, x, y, z)) {
+        uint32_t ni = cell_index(u, x, y, z);
+        float nkf = kf[ni];
+        if (!kf_solid(nkf)) val = kf_fluid(nkf) ? (float)pack[ni] * u.inv_fp_scale : 0.0f;
+      }
+      pn[n] = val;
+    }
+    float h = 0.5f * u.inv_dx;
+    cs_f3 grad = f3(h * (pn[1] - pn[0]), h * (pn[3] - pn[2]), h * (pn[5] - pn[4]));
+    v = v - grad * u.dt;
+    v = cap_speed(u, v);
+  }
+  v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
+  gvel[idx] = make_float4(v.x, v.y, v.z, gv.w);
+}
+
+// ---- viscosity_prepare + viscosity_apply + boundary_project (shader.rs:1086-1177) --
+__global__ void __launch_bounds__(256) k_visc(cs_params u, const float4* __restrict__ gin, float4* __restrict__ gout,
+                                              const uint8_t* __restrict__ cls, const float* __restrict__ sdf) {
+  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc) return;
+  float occ = u.nominal_mass * 0.1f;
+  float4 gv = gin[idx];
+  if (gv.w <= occ) {
+    gout[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  uint32_t ix, iy, iz;
+  decompose(u, idx, ix, iy, iz);
+  cs_f3 vh = f3(gv.x, gv.y, gv.z);
+  cs_f3 vs = vh;
+  if (u.viscosity > 0.0f && gv.w > u.nominal_mass * 2.0f) {
+    float alpha = cs_clamp(u.viscosity * u.dt / cs_max(u.dx * u.dx, 1e-6f), 0.0f, 0.14f);
+    cs_f3 nsum = f3(0.0f, 0.0f, 0.0f);
+    float wsum = 0.0f;
+    uint32_t nfl = 0;
+    for (int n = 0; n < 6; ++n) {
+      int x = (int)ix + c_off6[n][0], y = (int)iy + c_off6[n][1], z = (int)iz + c_off6[n][2];
+      if (!on_grid(u, x, y, z)) { wsum += 1.0f; continue; }
+      uint32_t ni = cell_index(u, x, y, z);
+      if (cls[ni]) { wsum += 1.0f; continue; }
+      float4 ng = gin[ni];
+      if (ng.w <= occ) continue;
+      float nw = cs_clamp(ng.w / cs_max(cs_max(gv.w, u.nominal_mass), 1e-6f), 0.0f, 1.0f);
+      nsum = nsum + f3(ng.x, ng.y, ng.z) * nw;
+      wsum += nw;
+      nfl += 1u;
+    }
+    if (!(nfl < 3u || wsum <= 1e-6f)) {
+      cs_f3 avg = nsum / wsum;
+      float blend = cs_clamp(alpha * wsum, 0.0f, 0.65f);
+      vs = mix3(vh, avg, blend);
+    }
+  }
+  // velocity_scratch_store / load round trip (shader.rs:266-283)
+  vs = cap_speed(u, vs);
+  cs_f3 v = f3((float)cs_f2i(vs.x * u.fp_scale) * u.inv_fp_scale, (float)cs_f2i(vs.y * u.fp_scale) * u.inv_fp_scale,
+               (float)cs_f2i(vs.z * u.fp_scale) * u.inv_fp_scale);
+  v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
+  gout[idx] = make_float4(v.x, v.y, v.z, gv.w);
+}
+
+// ---- G2P (shader.rs:1683-1881) ------------------------------------------------------
+__global__ void __launch_bounds__(256) k_g2p(cs_params u, float4* __restrict__ part, float4* __restrict__ aff,
+                                             const float4* __restrict__ gvel, const int32_t* __restrict__ bed_id,
+                                             const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf) {
+  uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= u.num_particles) return;
+  float4 P = part[2 * pid], Vv = part[2 * pid + 1];
+  float phase = aff[3 * pid].w;
+  if (!(phase < 0.5f)) return;
+  float mass_p = Vv.w;
+  if (mass_p <= u.nominal_mass * 0.10f) {
+    part[2 * pid + 1] = make_float4(Vv.x, Vv.y, Vv.z, 0.0f);
+    return;
+  }
+  cs_f3 xp = f3(P.x, P.y, P.z);
+  float J_old = P.w;
+  cs_f3 gp = (xp - f3(u.origin[0], u.origin[1], u.origin[2])) * u.inv_dx;
+  int bx = cs_f2i(floorf(gp.x - 0.5f)), by = cs_f2i(floorf(gp.y - 0.5f)), bz = cs_f2i(floorf(gp.z - 0.5f));
+  cs_f3 fx = f3(gp.x - (float)bx, gp.y - (float)by, gp.z - (float)bz);
+  float wx[3], wy[3], wz[3];
+  wx[0] = 0.5f * (1.5f - fx.x) * (1.5f - fx.x);
+  wx[1] = 0.75f - (fx.x - 1.0f) * (fx.x - 1.0f);
+  wx[2] = 0.5f * (fx.x - 0.5f) * (fx.x - 0.5f);
+  wy[0] = 0.5f * (1.5f - fx.y) * (1.5f - fx.y);
+  wy[1] = 0.75f - (fx.y - 1.0f) * (fx.y - 1.0f);
+  wy[2] = 0.5f * (fx.y - 0.5f) * (fx.y - 0.5f);
+  wz[0] = 0.5f * (1.5f - fx.z) * (1.5f - fx.z);
+  wz[1] = 0.75f - (fx.z - 1.0f) * (fx.z - 1.0f);
+  wz[2] = 0.5f * (fx.z - 0.5f) * (fx.z - 0.5f);
+  cs_f3 nv = f3(0, 0, 0), C0 = f3(0, 0, 0), C1 = f3(0, 0, 0), C2 = f3(0, 0, 0), bvs = f3(0, 0, 0);
+  float supported = 0.0f, local_mass = 0.0f, bed_overlap = 0.0f, bed_k = 0.0f;
+  float B = 4.0f * u.inv_dx * u.inv_dx;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) {
+        int cx = bx + i, cy = by + j, cz = bz + k;
+        if (!on_grid(u, cx, cy, cz)) continue;
+        float w = wx[i] * wy[j] * wz[k];
+        uint32_t ci = cell_index(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+        float4 g = gvel[ci];
+        cs_f3 gv = f3(g.x, g.y, g.z);
+        cs_f3 dpos = (f3((float)i, (float)j, (float)k) - fx) * u.dx;
+        if (g.w > 1e-6f) {
+          nv = nv + gv * w;
+          float wB = w * B;
+          C0 = C0 + (gv * wB) * dpos.x;
+          C1 = C1 + (gv * wB) * dpos.y;
+          C2 = C2 + (gv * wB) * dpos.z;
+          supported += w;
+          local_mass += w * g.w;
+        }
+        if (u.num_bed_cells > 0u) {
+          int32_t id = bed_id[ci];
+          if (id >= 0 && (uint32_t)id < u.num_bed_cells) {
+            const cs_bed_cell& b = bed[id];
+            bed_overlap += w;
+            bvs = bvs + f3(b.v_record[0], b.v_record[1], b.v_record[2]) * w;
+            bed_k += w * b.k_record;
+          }
+        }
+      }
+  float support_ratio = cs_clamp(supported, 0.0f, 1.0f);
+  float j_support = support_ratio;
+  if (supported > 1e-6f) {
+    float inv = 1.0f / supported;
+    nv = nv * inv;
+    C0 = C0 * inv;
+    C1 = C1 * inv;
+    C2 = C2 * inv;
+  }
+  bool in_cup = false;
+  if (u.geo.cup_enabled) {
+    float lim = u.geo.cup_radius + u.contact_offset;
+    in_cup = xp.y < u.geo.cup_top_y && (xp.x * xp.x + xp.z * xp.z) < lim * lim;
+  }
+  float dense_ratio = cs_clamp(local_mass / cs_max(u.nominal_mass * 4.0f, 1e-6f), 0.0f, 1.0f);
+  float porous = cs_clamp(bed_overlap, 0.0f, 1.0f);
+  cs_f3 pbv = f3(0, 0, 0);
+  float pk = u.min_k;
+  if (bed_overlap > 1e-6f) {
+    float ib = 1.0f / bed_overlap;
+    pbv = bvs * ib;
+    pk = cs_max(bed_k * ib, u.min_k);
+  }
+  if (support_ratio < 0.999f) {
+    cs_f3 ball = f3(Vv.x, Vv.y + u.gravity * u.dt, Vv.z);
+    float gain = cs_mix(in_cup ? 1.15f : 1.35f, 0.18f, porous);
+    float cap = cs_mix(in_cup ? 0.95f : 0.995f, 0.16f, porous);
+    float damp = cs_mix(1.0f, 0.12f, dense_ratio);
+    float pres = cs_clamp((1.0f - support_ratio) * gain * damp, 0.0f, cap);
+    nv = mix3(nv, ball, pres);
+    float ad = 1.0f - pres * 0.75f;
+    C0 = C0 * ad;
+    C1 = C1 * ad;
+    C2 = C2 * ad;
+  }
+  bool airborne = porous <= 0.05f && sample_sdf(u, sdf, xp) > u.contact_offset * 2.0f;
+  if (airborne) {
+    float dr = cs_clamp(local_mass / cs_max(u.nominal_mass * 4.0f, 1e-6f), 0.0f, 1.0f);
+    j_support = cs_min(j_support, dr);
+    cs_f3 ball = f3(Vv.x, Vv.y + u.gravity * u.dt, Vv.z);
+    float gain = in_cup ? 0.72f : 0.98f;
+    float cap = in_cup ? 0.88f : 0.98f;
+    float pres = cs_clamp((1.0f - dr) * gain, 0.0f, cap);
+    nv = mix3(nv, ball, pres);
+    float ad = 1.0f - pres * 0.65f;
+    C0 = C0 * ad;
+    C1 = C1 * ad;
+    C2 = C2 * ad;
+  }
+  if (porous > 1e-4f) {
+    cs_f3 rel = nv - pbv;
+    float rate = u.water_nu / pk;
+    float dd = 1.0f / (1.0f + cs_max(rate * u.dt * porous, 0.0f));
+    float rs = len3(rel);
+    float idd = 1.0f / (1.0f + cs_max(rs * u.dt * u.inv_dx * porous * 0.35f, 0.0f));
+    nv = pbv + (rel * dd) * idd;
+    float ad = cs_mix(1.0f, dd * idd, porous);
+    C0 = C0 * ad;
+    C1 = C1 * ad;
+    C2 = C2 * ad;
+    j_support = cs_max(j_support, porous);
+  }
+  float tr = C0.x + C1.y + C2.z;
+  float Jc = cs_clamp(J_old > 0.0f ? J_old : 1.0f, 0.40f, 2.00f);
+  float Ja = cs_clamp(Jc * cs_expf(cs_clamp(u.dt * tr, -0.35f, 0.35f)), 0.40f, 2.00f);
+  float Jb = cs_smoothstep(0.35f, 0.85f, cs_clamp(j_support, 0.0f, 1.0f));
+  float Jn = cs_clamp(cs_mix(Jc, Ja, Jb), 0.40f, 2.00f);
+  cs_f3 npos = xp + nv * u.dt;
+  cs_f3 mid = mix3(xp, npos, 0.5f);
+  contact c = sdf_contact(u, sdf, mid, nv);
+  nv = c.vel;
+  c = sdf_contact(u, sdf, npos, nv);
+  npos = c.pos;
+  nv = c.vel;
+  float margin = u.dx * 0.5f;
+  npos.x = cs_clamp(npos.x, u.origin[0] + margin, u.bmax[0] - margin);
+  npos.y = cs_clamp(npos.y, u.origin[1] + margin, u.bmax[1] - margin);
+  npos.z = cs_clamp(npos.z, u.origin[2] + margin, u.bmax[2] - margin);
+  part[2 * pid] = make_float4(npos.x, npos.y, npos.z, Jn);
+  part[2 * pid + 1] = make_float4(nv.x, nv.y, nv.z, mass_p);
+  aff[3 * pid] = make_float4(C0.x, C0.y, C0.z, phase);
+  aff[3 * pid + 1] = make_float4(C1.x, C1.y, C1.z, 0.0f);
+  aff[3 * pid + 2] = make_float4(C2.x, C2.y, C2.z, 0.0f);
+}
+
+// ---- exp check kernel (used by the verification entry point) -----------------------
+__global__ void k_expf(const float* __restrict__ in, float* __restrict__ out, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = cs_expf(in[i]);
+}
+
+// ---- launchers -------------------------------------------------------------------------
+namespace {
+constexpr uint32_t kBlock = 256;
+inline dim3 blocks_for(size_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+}  // namespace
+
+hipError_t cs_launch_fill_i32(hipStream_t s, int32_t* p, size_t n, int32_t v) {
+  if (n == 0) return hipSuccess;
+  size_t nb = (n + kBlock - 1) / kBlock;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)nb), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+hipError_t cs_launch_copy_i32(hipStream_t s, const int32_t* src, int32_t* dst, size_t n) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_i32, blocks_for(n), dim3(kBlock), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+hipError_t cs_launch_p2g(hipStream_t s, const cs_params& u, const float* part, const float* aff, int32_t* lanes,
+                         uint32_t* metrics) {
+  if (u.num_particles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_p2g_atomic, blocks_for(u.num_particles), dim3(kBlock), 0, s, u,
+                     reinterpret_cast<const float4*>(part), reinterpret_cast<const float4*>(aff), lanes, metrics);
+  return hipGetLastError();
+}
+hipError_t cs_launch_grid(hipStream_t s, const cs_params& u, const int32_t* lanes, float* gvel, const float* sdf) {
+  hipLaunchKernelGGL(k_grid, blocks_for(u.nc), dim3(kBlock), 0, s, u, lanes, reinterpret_cast<float4*>(gvel), sdf);
+  return hipGetLastError();
+}
+hipError_t cs_launch_classify(hipStream_t s, const cs_params& u, const float* gvel, const uint8_t* cls,
+                              const int32_t* lanes, const int32_t* bed_id, const cs_bed_cell* bed, int8_t* kind,
+                              float* kf, int32_t* pres, float* rhs, int32_t* pack, uint32_t* metrics,
+                              const float* sdf) {
+  hipLaunchKernelGGL(k_classify, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<const float4*>(gvel), cls,
+                     lanes, bed_id, bed, kind, kf, pres, rhs, pack, metrics, sdf);
+  return hipGetLastError();
+}
+hipError_t cs_launch_warm_previous(hipStream_t s, const cs_params& u, const float* kf, const int32_t* prev,
+                                   int32_t* pres) {
+  hipLaunchKernelGGL(k_warm_previous, blocks_for(u.nc), dim3(kBlock), 0, s, u, kf, prev, pres);
+  return hipGetLastError();
+}
+hipError_t cs_launch_warm_external(hipStream_t s, const cs_params& u, const float* kf, const float* warm,
+                                   int32_t* pres, uint32_t* metrics) {
+  hipLaunchKernelGGL(k_warm_external, blocks_for(u.nc), dim3(kBlock), 0, s, u, kf, warm, pres, metrics);
+  return hipGetLastError();
+}
+hipError_t cs_launch_rbgs_pair(hipStream_t s, const cs_params& u, const float* kf, const float* rhs, int32_t* pres,
+                               uint32_t* metrics) {
+  hipLaunchKernelGGL(k_rbgs_half, blocks_for(u.nc), dim3(kBlock), 0, s, u, 0u, kf, rhs, pres, metrics);
+  hipLaunchKernelGGL(k_rbgs_half, blocks_for(u.nc), dim3(kBlock), 0, s, u, 1u, kf, rhs, pres, metrics);
+  return hipGetLastError();
+}
+hipError_t cs_launch_residual(hipStream_t s, const cs_params& u, const float* kf, const float* rhs,
+                              const int32_t* pres, double* partials, uint32_t max_blocks, double* out) {
+  size_t nb = (u.nc + kBlock - 1) / kBlock;
+  if (nb > max_blocks) nb = max_blocks;
+  hipLaunchKernelGGL(k_residual, dim3((unsigned)nb), dim3(kBlock), 0, s, u, kf, rhs, pres, partials);
+  hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kBlock), 0, s, partials, (uint32_t)nb, out);
+  return hipGetLastError();
+}
+hipError_t cs_launch_project(hipStream_t s, const cs_params& u, float* gvel, const float* kf, const int32_t* pres,
+                             const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf) {
+  hipLaunchKernelGGL(k_project, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(gvel), kf, pres,
+                     bed_id, bed, sdf);
+  return hipGetLastError();
+}
+hipError_t cs_launch_pack(hipStream_t s, const cs_params& u, float* gvel, const float* kf, const int32_t* pack,
+                          const float* sdf) {
+  hipLaunchKernelGGL(k_pack, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(gvel), kf, pack, sdf);
+  return hipGetLastError();
+}
+hipError_t cs_launch_visc(hipStream_t s, const cs_params& u, const float* gin, float* gout, const uint8_t* cls,
+                          const float* sdf) {
+  hipLaunchKernelGGL(k_visc, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<const float4*>(gin),
+                     reinterpret_cast<float4*>(gout), cls, sdf);
+  return hipGetLastError();
+}
+hipError_t cs_launch_g2p(hipStream_t s, const cs_params& u, float* part, float* aff, const float* gvel,
+                         const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf) {
+  if (u.num_particles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_g2p, blocks_for(u.num_particles), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(part),
+                     reinterpret_cast<float4*>(aff), reinterpret_cast<const float4*>(gvel), bed_id, bed, sdf);
+  return hipGetLastError();
+}
+hipError_t cs_launch_expf(hipStream_t s, const float* in, float* out, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_expf, blocks_for(n), dim3(kBlock), 0, s, in, out, n);
+  return hipGetLastError();
+}

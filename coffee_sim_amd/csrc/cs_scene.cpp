@@ -1,0 +1,551 @@
+// cs_scene.cpp — CPU-side scene construction for the MLS-MPM substep:
+// units, settings presets, SDF and class-mask generation, the inflow emitter,
+// the parameter block and the static bed table.  Follows the reference's host
+// code (crates/sim-wasm/src/mpm_3d/{mod,state,inflow,units,brew_config}.rs) with
+// binary32 arithmetic in the same operation order, so the values handed to the
+// kernels are the reference's bit for bit.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "cs_host.h"
+
+namespace cs {
+
+// ---- units.rs:10-34 ---------------------------------------------------------------
+float sim_units_per_meter() {
+  volatile float top = 2.75f, bot = -3.02f, h = 0.10f;
+  float height_units = top - bot;
+  return height_units / h;
+}
+float max_velocity() { return 1.5f * sim_units_per_meter(); }
+float gravity_sim_units() { return -9.80665f * sim_units_per_meter(); }
+float ml_per_sim_unit_cubed() {
+  float m = 1.0f / sim_units_per_meter();
+  return m * m * m * 1000000.0f;
+}
+// brew_config.rs:160-164
+float kozeny_carman_permeability(float grind_diameter_um, float porosity) {
+  float d_m = grind_diameter_um * 1.0e-6f;
+  float solid = 1.0f - porosity;
+  return d_m * d_m * porosity * porosity * porosity / (180.0f * solid * solid);
+}
+
+// ---- sim-core Vec3 semantics (crates/sim-core/src/math.rs:37-92) -------------------
+static float rs_len(const float* v) { return sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+static void rs_normalize(float* v) {
+  float len = rs_len(v);
+  if (len <= 1.0e-6f || fabsf(len) <= 1.0e-6f) {
+    v[0] = v[1] = v[2] = 0.0f;
+    return;
+  }
+  v[0] /= len;
+  v[1] /= len;
+  v[2] /= len;
+}
+static void rs_cross(const float* a, const float* b, float* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// SpoutSettings::aim_at / translate_origin_to (inflow.rs:50-66)
+void spout_aim_at(cs_spout* sp, float x, float y, float z) {
+  sp->target[0] = x;
+  sp->target[1] = y;
+  sp->target[2] = z;
+  float d[3] = {x - sp->origin[0], y - sp->origin[1], z - sp->origin[2]};
+  if (rs_len(d) > 1e-5f) {
+    rs_normalize(d);
+    memcpy(sp->direction, d, sizeof(d));
+  }
+}
+void spout_translate_origin_to(cs_spout* sp, float x, float y, float z) {
+  float d[3] = {x - sp->origin[0], y - sp->origin[1], z - sp->origin[2]};
+  sp->origin[0] = x;
+  sp->origin[1] = y;
+  sp->origin[2] = z;
+  for (int c = 0; c < 3; ++c) sp->target[c] = sp->target[c] + d[c];
+}
+
+// ---- presets: MpmSettings::{default_v60, benchmark_free_stream,
+//      benchmark_center_pour} (mod.rs:106-177), SpoutSettings::default (inflow.rs:27-47)
+int settings_preset(int preset, cs_settings* s) {
+  if (!s || preset < CS_PRESET_DEFAULT_V60 || preset > CS_PRESET_CENTER_POUR) return CS_ERR_INVALID;
+  memset(s, 0, sizeof(*s));
+  s->bounds_size[0] = 14.0f;
+  s->bounds_size[1] = 20.0f;
+  s->bounds_size[2] = 14.0f;
+  const uint32_t gx = 80u;
+  const float dx = s->bounds_size[0] / (float)gx;
+  s->grid_dims[0] = gx;
+  s->grid_dims[1] = (uint32_t)ceilf(s->bounds_size[1] / dx);
+  s->grid_dims[2] = 80u;
+  s->max_particles = 220000u;
+  s->substeps = 10u;
+  s->gravity = gravity_sim_units();
+  s->bulk_modulus = 900.0f;
+  s->viscosity = 1.2f;  // DEFAULT_BREW.water_viscosity (brew_config.rs:124)
+  s->render_radius = dx * 0.7f;
+  s->pressure_rbgs_pairs = 40u;
+  s->use_sdf_cache = 1u;
+  // v60_support_cone against FilterConfig::default (filter.rs:211-223)
+  const float filter_height = cs_max(2.75f - -3.02f, 1e-6f);
+  const float slope = (4.10f - 0.0f) / filter_height;
+  const float outlet = 0.42f;  // DEFAULT_BREW.dripper_outlet_radius
+  s->num_obstacles = 2u;
+  s->obstacles[0].kind = CS_OBSTACLE_TRUNCATED_CONE;
+  s->obstacles[0].top_radius = outlet + slope * (3.0f - -3.0f);
+  s->obstacles[0].bot_radius = outlet;
+  s->obstacles[0].top_y = 3.0f;
+  s->obstacles[0].bot_y = -3.0f;
+  s->obstacles[1].kind = CS_OBSTACLE_CYLINDER;
+  s->obstacles[1].radius = 3.0f;
+  s->obstacles[1].top_y = -3.5f;
+  s->obstacles[1].bot_y = -8.0f;
+  cs_spout& sp = s->spout;
+  sp.origin[0] = -3.4f;
+  sp.origin[1] = 7.3f;
+  sp.origin[2] = 0.9f;
+  float dir[3] = {0.36f, -0.92f, -0.12f};
+  rs_normalize(dir);
+  memcpy(sp.direction, dir, sizeof(dir));
+  sp.target[0] = -0.7f;
+  sp.target[1] = 0.4f;
+  sp.target[2] = 0.0f;
+  sp.nozzle_radius = 0.18f;
+  sp.stem_radius = 0.24f;
+  sp.activation_angle_deg = 8.0f;
+  sp.full_flow_angle_deg = 56.0f;
+  sp.head_at_activation = 0.04f;
+  sp.head_at_full_angle = 24.0f;
+  sp.discharge_coeff = 0.92f;
+  sp.volume_to_ml = ml_per_sim_unit_cubed();
+  sp.max_flow_rate_ml_s = 12.0f;
+  sp.gentle_exit_speed = 0.12f * sim_units_per_meter();
+  sp.max_exit_speed = 0.45f * sim_units_per_meter();
+  sp.stem_length = 1.9f;
+  s->initial_kettle_angle_deg = 9.0f;
+  // extensions at their reference values
+  s->sdf_res = 128u;
+  s->particle_vol_factor = 0.25f;
+  s->wall_thickness = 0.4f;
+  s->water_viscosity_m2_s = 1.0e-6f;
+  s->min_bed_permeability = 1.0e-12f;
+  s->geometry.cone_enabled = 1u;
+  s->geometry.cone_bot_y = -3.0f;
+  s->geometry.cone_top_y = 3.0f;
+  s->geometry.cone_bot_radius = outlet;
+  s->geometry.cone_top_radius = outlet + 4.2634315f;
+  s->geometry.cup_enabled = 1u;
+  s->geometry.cup_top_y = -3.5f;
+  s->geometry.cup_radius = 3.0f;
+  s->geometry.cup_bottom_y = -8.0f;
+  s->emitter_mode = CS_EMIT_FLOW_MODEL;
+  s->rbgs_mode = CS_RBGS_FIXED_PAIRS;
+  s->rbgs_tolerance = 1.0e-4f;
+  s->rbgs_check_every = 8u;
+  s->rbgs_max_iters = 20000u;
+  s->warm_start = CS_WARM_ZERO;
+  s->has_bed_field = 0u;
+  if (preset == CS_PRESET_FREE_STREAM) {
+    sp.origin[0] = 0.0f;
+    sp.origin[1] = 6.8f;
+    sp.origin[2] = 0.0f;
+    spout_aim_at(&sp, 0.0f, -6.8f, 0.0f);
+  } else if (preset == CS_PRESET_CENTER_POUR) {
+    sp.origin[0] = 0.0f;
+    sp.origin[1] = 7.1f;
+    sp.origin[2] = 0.0f;
+    spout_aim_at(&sp, 0.0f, 0.4f, 0.0f);
+  }
+  return CS_OK;
+}
+
+int validate_settings(const cs_settings& s, std::string* why) {
+  auto fail = [&](const char* m) {
+    if (why) *why = m;
+    return CS_ERR_INVALID;
+  };
+  for (int c = 0; c < 3; ++c) {
+    if (s.grid_dims[c] == 0u || s.grid_dims[c] > 1024u) return fail("grid dims must be in [1, 1024]");
+    if (!(s.bounds_size[c] > 0.0f)) return fail("bounds_size must be positive");
+  }
+  if ((uint64_t)s.grid_dims[0] * s.grid_dims[1] * s.grid_dims[2] > (1ull << 30)) return fail("grid too large");
+  if (s.sdf_res < 2u || s.sdf_res > 1024u) return fail("sdf_res must be in [2, 1024]");
+  if (s.max_particles == 0u || s.max_particles > (1u << 28)) return fail("max_particles out of range");
+  if (s.num_obstacles > CS_MAX_OBSTACLES) return fail("too many obstacles");
+  if (s.rbgs_mode > CS_RBGS_TOLERANCE) return fail("unknown rbgs_mode");
+  if (s.warm_start > CS_WARM_EXTERNAL) return fail("unknown warm_start");
+  if (s.emitter_mode > CS_EMIT_FIXED_COUNT) return fail("unknown emitter_mode");
+  return CS_OK;
+}
+
+// ---- SDF generation (state.rs:206-276) -----------------------------------------------
+static float sdf_interior(const cs_obstacle& o, float px, float py, float pz) {
+  if (o.kind == CS_OBSTACLE_TRUNCATED_CONE) {
+    float local_y = py - o.center[1];
+    if (local_y > o.top_y || local_y < o.bot_y) return CS_SDF_NO_CONSTRAINT;
+    float height = o.top_y - o.bot_y;
+    if (height < 1e-10f) return CS_SDF_NO_CONSTRAINT;
+    float t = (local_y - o.bot_y) / height;
+    float cone_r = o.bot_radius + (o.top_radius - o.bot_radius) * t;
+    float dx = px - o.center[0], dz = pz - o.center[2];
+    return cone_r - sqrtf(dx * dx + dz * dz);
+  }
+  if (o.kind == CS_OBSTACLE_CYLINDER) {
+    float local_y = py - o.center[1];
+    if (local_y > o.top_y) return CS_SDF_NO_CONSTRAINT;
+    float dx = px - o.center[0], dz = pz - o.center[2];
+    float radial = o.radius - sqrtf(dx * dx + dz * dz);
+    float floor_d = local_y - o.bot_y;
+    return fminf(radial, floor_d);
+  }
+  return CS_SDF_NO_CONSTRAINT;
+}
+
+void generate_sdf(const cs_settings& s, float* data) {
+  const uint32_t n = s.sdf_res;
+  const size_t total = (size_t)n * n * n;
+  std::fill(data, data + total, CS_SDF_NO_CONSTRAINT);
+  const float bx = s.bounds_size[0], by = s.bounds_size[1], bz = s.bounds_size[2];
+  const float half = s.wall_thickness * 0.5f;
+  for (uint32_t oi = 0; oi < s.num_obstacles; ++oi) {
+    const cs_obstacle& o = s.obstacles[oi];
+    for (uint32_t iz = 0; iz < n; ++iz)
+      for (uint32_t iy = 0; iy < n; ++iy)
+        for (uint32_t ix = 0; ix < n; ++ix) {
+          float px = -bx * 0.5f + ((float)ix + 0.5f) * bx / (float)n;
+          float py = -by * 0.5f + ((float)iy + 0.5f) * by / (float)n;
+          float pz = -bz * 0.5f + ((float)iz + 0.5f) * bz / (float)n;
+          float sd = sdf_interior(o, px, py, pz) - half;
+          size_t idx = ((size_t)iz * n + iy) * n + ix;
+          if (data[idx] >= CS_SDF_NO_CONSTRAINT - 1.0f) data[idx] = sd;
+          else data[idx] = fmaxf(data[idx], sd);
+        }
+  }
+}
+
+static float texel(const float* d, uint32_t res, int x, int y, int z) {
+  int mx = (int)res - 1;
+  x = std::clamp(x, 0, mx);
+  y = std::clamp(y, 0, mx);
+  z = std::clamp(z, 0, mx);
+  return d[((size_t)z * res + y) * res + x];
+}
+
+// sample_sdf_from_data (state.rs:287-320): Rust's fract (x - trunc x) and the
+// lerp form c + (d - c) * f.
+static float sample_sdf_cpu(const cs_settings& s, const float* d, float px, float py, float pz) {
+  const float res = (float)s.sdf_res;
+  const float bx = s.bounds_size[0], by = s.bounds_size[1], bz = s.bounds_size[2];
+  float ux = (px + bx * 0.5f) / bx * res - 0.5f;
+  float uy = (py + by * 0.5f) / by * res - 0.5f;
+  float uz = (pz + bz * 0.5f) / bz * res - 0.5f;
+  int b0 = cs_f2i(floorf(ux)), b1 = cs_f2i(floorf(uy)), b2 = cs_f2i(floorf(uz));
+  float fx = ux - truncf(ux), fy = uy - truncf(uy), fz = uz - truncf(uz);
+  const uint32_t r = s.sdf_res;
+  float c000 = texel(d, r, b0, b1, b2), c100 = texel(d, r, b0 + 1, b1, b2);
+  float c010 = texel(d, r, b0, b1 + 1, b2), c110 = texel(d, r, b0 + 1, b1 + 1, b2);
+  float c001 = texel(d, r, b0, b1, b2 + 1), c101 = texel(d, r, b0 + 1, b1, b2 + 1);
+  float c011 = texel(d, r, b0, b1 + 1, b2 + 1), c111 = texel(d, r, b0 + 1, b1 + 1, b2 + 1);
+  float c00 = c000 + (c100 - c000) * fx;
+  float c10 = c010 + (c110 - c010) * fx;
+  float c01 = c001 + (c101 - c001) * fx;
+  float c11 = c011 + (c111 - c011) * fx;
+  float c0 = c00 + (c10 - c00) * fy;
+  float c1 = c01 + (c11 - c01) * fy;
+  return c0 + (c1 - c0) * fz;
+}
+
+// generate_sdf_class_data (state.rs:322-344)
+void generate_sdf_class(const cs_settings& s, const float* sdf, uint8_t* out) {
+  const uint32_t gx = s.grid_dims[0], gy = s.grid_dims[1], gz = s.grid_dims[2];
+  const float dx = s.bounds_size[0] / (float)gx;
+  const float ox = -s.bounds_size[0] * 0.5f, oy = -s.bounds_size[1] * 0.5f, oz = -s.bounds_size[2] * 0.5f;
+  for (uint32_t iz = 0; iz < gz; ++iz)
+    for (uint32_t iy = 0; iy < gy; ++iy)
+      for (uint32_t ix = 0; ix < gx; ++ix) {
+        float cx = ox + ((float)ix + 0.5f) * dx;
+        float cy = oy + ((float)iy + 0.5f) * dx;
+        float cz = oz + ((float)iz + 0.5f) * dx;
+        out[((size_t)iz * gy + iy) * gx + ix] = sample_sdf_cpu(s, sdf, cx, cy, cz) < 0.0f ? 1u : 0u;
+      }
+}
+
+// ---- inflow model (inflow.rs:244-293) -----------------------------------------------
+float pour_fraction_from_angle(float a, const cs_spout& sp) {
+  if (a <= sp.activation_angle_deg) return 0.0f;
+  if (a >= sp.full_flow_angle_deg) return 1.0f;
+  float t = (a - sp.activation_angle_deg) / (sp.full_flow_angle_deg - sp.activation_angle_deg);
+  return t * t * (3.0f - 2.0f * t);
+}
+float effective_head_from_angle(float a, const cs_spout& sp) {
+  if (a <= sp.activation_angle_deg) return 0.0f;
+  if (a >= sp.full_flow_angle_deg) return sp.head_at_full_angle;
+  float s = pour_fraction_from_angle(a, sp);
+  return sp.head_at_activation + s * (sp.head_at_full_angle - sp.head_at_activation);
+}
+float exit_speed_cap_from_angle(float a, const cs_spout& sp) {
+  float s = pour_fraction_from_angle(a, sp);
+  return sp.gentle_exit_speed + s * (sp.max_exit_speed - sp.gentle_exit_speed);
+}
+float exit_speed_from_head(float head, float max_exit_speed) {
+  if (head < 1e-6f) return 0.0f;
+  float speed = sqrtf(2.0f * 9.80665f * sim_units_per_meter() * head);
+  return fminf(speed, max_exit_speed);
+}
+float flow_rate_from_speed(float v, float nozzle_radius, float cd, float vol_to_ml, float max_flow) {
+  if (v < 1e-6f) return 0.0f;
+  float area = 3.14159265358979323846f * nozzle_radius * nozzle_radius;
+  float q = cd * area * v;
+  return fminf(q * vol_to_ml, max_flow);
+}
+
+// Golden-angle disk (inflow.rs:165-219).
+static void emit_disk(const cs_spout& sp, Inflow& in, uint32_t count, float dt, float speed, float jet_radius,
+                      float mass, std::vector<float>& out) {
+  const float* dir = sp.direction;
+  float up[3] = {0.0f, 1.0f, 0.0f};
+  if (!(fabsf(dir[1]) < 0.9f)) {
+    up[0] = 1.0f;
+    up[1] = 0.0f;
+  }
+  float t1[3], t2[3];
+  rs_cross(dir, up, t1);
+  rs_normalize(t1);
+  rs_cross(dir, t1, t2);
+  rs_normalize(t2);
+  const float vel[3] = {dir[0] * speed, dir[1] * speed, dir[2] * speed};
+  const float eo[3] = {sp.origin[0] + dir[0] * 0.18f, sp.origin[1] + dir[1] * 0.18f, sp.origin[2] + dir[2] * 0.18f};
+  const float golden = 2.399963f, radial_irr = 0.7548777f;
+  size_t base = out.size();
+  out.resize(base + (size_t)count * 8);
+  for (uint32_t i = 0; i < count; ++i) {
+    uint32_t sample = in.emission_sequence + i;
+    float sf = (float)sample;
+    float ru = (sf + 0.5f) * radial_irr;
+    ru = ru - truncf(ru);
+    float r = jet_radius * sqrtf(ru);
+    float theta = sf * golden;
+    float age = (((float)i + 0.5f) / (float)count) * dt;
+    float ct = cosf(theta), st = sinf(theta);
+    float* p = &out[base + (size_t)i * 8];
+    for (int c = 0; c < 3; ++c) {
+      float off = t1[c] * r * ct + t2[c] * r * st;
+      p[c] = eo[c] + off - vel[c] * age;
+    }
+    p[3] = 1.0f;
+    p[4] = vel[0];
+    p[5] = vel[1];
+    p[6] = vel[2];
+    p[7] = mass;
+  }
+  in.emission_sequence += count;
+}
+
+uint32_t emit(const cs_settings& s, Inflow& in, float dt, uint32_t current, std::vector<float>& out,
+              uint32_t* dropped) {
+  const float mass = 80.0f / 320.0f;  // MASS_UNITS_PER_ML / PARTICLES_PER_ML (mod.rs:278)
+  uint32_t avail = s.max_particles > current ? s.max_particles - current : 0u;
+  if (s.emitter_mode == CS_EMIT_FIXED_COUNT) {
+    uint32_t want = s.emit_per_substep;
+    if (s.emit_total) {
+      uint32_t left = s.emit_total > in.emitted_total ? s.emit_total - in.emitted_total : 0u;
+      want = std::min(want, left);
+    }
+    uint32_t count = std::min(want, avail);
+    if (dropped) *dropped += want - count;
+    if (count == 0) return 0;
+    emit_disk(s.spout, in, count, dt, s.emit_speed, s.emit_jet_radius, mass, out);
+    in.emitted_total += count;
+    return count;
+  }
+  // InflowState::update (inflow.rs:109-120)
+  float head = effective_head_from_angle(in.kettle_angle_deg, s.spout);
+  float cap = exit_speed_cap_from_angle(in.kettle_angle_deg, s.spout);
+  in.exit_speed = exit_speed_from_head(head, cap);
+  in.flow_rate = flow_rate_from_speed(in.exit_speed, s.spout.nozzle_radius, s.spout.discharge_coeff,
+                                      s.spout.volume_to_ml, s.spout.max_flow_rate_ml_s);
+  if (in.flow_rate < 1e-6f) {
+    in.accumulator = 0.0f;
+    return 0;
+  }
+  in.accumulator += in.flow_rate * 320.0f * dt;
+  uint32_t requested = cs_f2u(in.accumulator);
+  if (requested == 0) return 0;
+  in.accumulator -= (float)requested;
+  uint32_t count = std::min(requested, avail);
+  if (dropped) *dropped += requested - count;
+  if (count == 0) return 0;
+  emit_disk(s.spout, in, count, dt, in.exit_speed, s.spout.nozzle_radius * 0.58f, mass, out);
+  return count;
+}
+
+// ---- write_uniforms (mod.rs:569-670) -----------------------------------------------------
+void make_params(const cs_settings& s, uint32_t num_particles, uint32_t num_bed_cells, float dt, cs_params* u) {
+  memset(u, 0, sizeof(*u));
+  u->gx = s.grid_dims[0];
+  u->gy = s.grid_dims[1];
+  u->gz = s.grid_dims[2];
+  u->nc = u->gx * u->gy * u->gz;
+  u->num_particles = num_particles;
+  u->num_bed_cells = num_bed_cells;
+  u->max_particles = s.max_particles;
+  u->use_sdf_cache = s.use_sdf_cache;
+  const float dx = s.bounds_size[0] / (float)u->gx;
+  const float inv_dx = 1.0f / dx;
+  u->dt = dt;
+  u->gravity = s.gravity;
+  u->dx = dx;
+  u->inv_dx = inv_dx;
+  for (int c = 0; c < 3; ++c) {
+    u->origin[c] = -s.bounds_size[c] * 0.5f;
+    u->bmax[c] = s.bounds_size[c] * 0.5f;
+  }
+  u->bulk_K = s.bulk_modulus;
+  u->viscosity = s.viscosity;
+  u->nominal_mass = 80.0f / 320.0f;
+  u->p_vol = dx * dx * dx * s.particle_vol_factor;
+  u->fp_scale = CS_FP_SCALE;
+  u->inv_fp_scale = 1.0f / CS_FP_SCALE;
+  u->vel_cap = max_velocity();
+  u->sdf_res = s.sdf_res;
+  u->sdf_res_f = (float)s.sdf_res;
+  u->friction = 0.3f;
+  u->restitution = 0.0f;
+  u->contact_offset = 0.05f;
+  u->water_nu = s.water_viscosity_m2_s;
+  u->min_k = s.min_bed_permeability;
+  float bound = 4.0f * max_velocity() * inv_dx;
+  u->div_clamp = fminf(bound, CS_FP_VALUE_LIMIT - 1.0f);
+  u->pressure_clamp = CS_FP_VALUE_LIMIT - 1.0f;
+  u->metrics_div_fp_scale = CS_METRICS_DIV_FP_SCALE;
+  u->j_alpha = 32.0f;
+  u->j_expand_alpha = 2.0f;
+  u->max_rest_fraction = 1.20f;
+  u->wall_half = s.wall_thickness * 0.5f;
+  u->geo = s.geometry;
+}
+
+// ---- static bed table (field mode of shader.rs:380-449, 621-666) -------------------------
+static float kozeny_factor(float phi_in) {
+  float phi = cs_clamp(phi_in, 0.08f, 0.82f);
+  float solid = cs_max(1.0f - phi, 1e-4f);
+  return (phi * phi * phi) / (solid * solid);
+}
+static float compacted_porosity(const cs_bed_record& b) {
+  float base = cs_clamp(b.porosity, 0.18f, 0.70f);
+  float comp = cs_clamp(b.compaction, 0.0f, 2.25f);
+  return cs_clamp(base - comp * 0.12f, 0.16f, base);
+}
+
+bool build_bed_table(const cs_settings& s, const int32_t* lookup, const cs_bed_record* rec, uint32_t nrec,
+                     std::vector<int32_t>& cell_index, std::vector<cs_bed_cell>& table) {
+  const uint32_t gx = s.grid_dims[0], gy = s.grid_dims[1], gz = s.grid_dims[2];
+  const size_t nc = (size_t)gx * gy * gz;
+  cell_index.assign(nc, -1);
+  table.clear();
+  if (!lookup || !rec || nrec == 0) return true;
+  for (size_t i = 0; i < nc; ++i)
+    if (lookup[i] >= (int32_t)nrec || lookup[i] < -1) return false;
+  cs_params u;
+  make_params(s, 0, 0, 0.0f, &u);
+  const float dx = u.dx, inv_dx = u.inv_dx;
+  auto at = [&](int x, int y, int z) -> int32_t {
+    if (x < 0 || y < 0 || z < 0 || x >= (int)gx || y >= (int)gy || z >= (int)gz) return -2;
+    return lookup[((size_t)z * gy + y) * gx + x];
+  };
+  // bed_matrix_velocity_cell (shader.rs:380-407)
+  auto bmv = [&](int x, int y, int z, float* out) {
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, w = 0.0f;
+    for (int di = -1; di <= 1; ++di)
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int dk = -1; dk <= 1; ++dk) {
+          int32_t id = at(x + di, y + dj, z + dk);
+          if (id < 0) continue;
+          sx = sx + rec[id].velocity[0];
+          sy = sy + rec[id].velocity[1];
+          sz = sz + rec[id].velocity[2];
+          w += 1.0f;
+        }
+    if (w <= 0.0f) {
+      out[0] = out[1] = out[2] = 0.0f;
+    } else {
+      out[0] = sx / w;
+      out[1] = sy / w;
+      out[2] = sz / w;
+    }
+  };
+  // bed_velocity_load_cell (shader.rs:409-417)
+  auto bvl = [&](int x, int y, int z, float* out) {
+    if (x < 0 || y < 0 || z < 0 || x >= (int)gx || y >= (int)gy || z >= (int)gz) {
+      out[0] = out[1] = out[2] = 0.0f;
+      return;
+    }
+    bmv(x, y, z, out);
+  };
+  const float cv = dx * dx * dx;
+  for (uint32_t iz = 0; iz < gz; ++iz)
+    for (uint32_t iy = 0; iy < gy; ++iy)
+      for (uint32_t ix = 0; ix < gx; ++ix) {
+        size_t c = ((size_t)iz * gy + iy) * gx + ix;
+        int32_t id = lookup[c];
+        if (id < 0) continue;
+        const cs_bed_record& b = rec[id];
+        cs_bed_cell t;
+        memset(&t, 0, sizeof(t));
+        // bed_compacted_permeability (shader.rs:442-449)
+        float base_p = cs_clamp(b.porosity, 0.18f, 0.70f);
+        float compacted = compacted_porosity(b);
+        float ratio = kozeny_factor(compacted) / cs_max(kozeny_factor(base_p), 1e-8f);
+        t.k_record = cs_max(b.permeability * ratio, u.min_k);
+        t.v_record[0] = b.velocity[0];
+        t.v_record[1] = b.velocity[1];
+        t.v_record[2] = b.velocity[2];
+        // bed_pore_projection_target_divergence factors (shader.rs:628-662)
+        float ccy = u.origin[1] + ((float)iy + 0.5f) * dx;
+        float base = compacted;
+        float sat = cs_clamp(b.saturation, 0.0f, 1.0f);
+        float comp = cs_clamp(b.compaction, 0.0f, 2.25f);
+        float surface_t = cs_smoothstep(-0.25f, 0.75f, (ccy - b.sample_y) * inv_dx);
+        float open = cs_clamp((1.0f - 0.65f * sat) * (1.0f - 0.12f * comp), 0.20f, 1.0f);
+        float opening = surface_t * cs_clamp(1.0f, 0.0f, 1.0f) * open;  // bed_surface_void_scale = 1
+        float surf_por = cs_min(base + 0.35f, 0.82f);
+        float eff = cs_mix(base, surf_por, opening);
+        float occ = cs_mix(1.0f, 0.35f, opening);
+        float satc = cs_mix(1.0f, 0.32f, sat);
+        float compc = cs_mix(1.0f, 0.58f, cs_clamp(comp / 1.5f, 0.0f, 1.0f));
+        t.pore_capacity = eff * cv * cs_max(1.0f, 0.0f) * satc * compc;  // bed_pore_capacity_scale = 1
+        t.overfill_resp = 18.0f * occ * cs_mix(1.0f, 1.45f, sat);      // bed_pore_overfill_alpha = 18
+        t.porosity = cs_clamp(compacted, 0.08f, 1.0f);
+        t.solid_fraction = 1.0f - t.porosity;
+        // bed_velocity_divergence (shader.rs:419-427)
+        float a[3], bb[3];
+        bvl((int)ix - 1, (int)iy, (int)iz, a);
+        float vxm = a[0];
+        bvl((int)ix + 1, (int)iy, (int)iz, a);
+        float vxp = a[0];
+        bvl((int)ix, (int)iy - 1, (int)iz, a);
+        float vym = a[1];
+        bvl((int)ix, (int)iy + 1, (int)iz, a);
+        float vyp = a[1];
+        bvl((int)ix, (int)iy, (int)iz - 1, a);
+        float vzm = a[2];
+        bvl((int)ix, (int)iy, (int)iz + 1, a);
+        float vzp = a[2];
+        t.solid_div = 0.5f * inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
+        bmv((int)ix, (int)iy, (int)iz, bb);
+        t.bmv[0] = bb[0];
+        t.bmv[1] = bb[1];
+        t.bmv[2] = bb[2];
+        cell_index[c] = (int32_t)table.size();
+        table.push_back(t);
+      }
+  return true;
+}
+
+}  // namespace cs

@@ -1,0 +1,737 @@
+// cs_sim.cpp — C ABI of the MI355X MLS-MPM substep (include/coffee_sim.h).
+//
+// Mirrors MpmSim3D (crates/sim-wasm/src/mpm_3d/mod.rs:179-567): a handle owns
+// its device buffers and stream, step_frame splits the frame into substeps and
+// runs emit -> params -> clear -> P2G -> grid -> classify -> pressure solve ->
+// project -> packing -> viscosity -> G2P on one HIP stream.
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "cs_host.h"
+#include "cs_kernels.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  g_error = std::string(what) + ": " + hipGetErrorString(e);
+  return CS_ERR_DEVICE;
+}
+#define CS_HIP(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+enum Pass { P_P2G = 0, P_GRID, P_CLASSIFY, P_SOLVE, P_POST, P_G2P, P_END, P_COUNT };
+
+}  // namespace
+
+struct cs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct cs_sim {
+  cs_ctx* ctx = nullptr;
+  cs_settings s;
+  uint32_t gx = 0, gy = 0, gz = 0;
+  size_t nc = 0;
+  // device state
+  float* part = nullptr;     // 8 floats per particle
+  float* aff = nullptr;      // 12 floats per particle
+  int32_t* lanes = nullptr;  // [6][nc]
+  float* gv = nullptr;       // float4 per cell
+  float* gv2 = nullptr;      // float4 per cell (after viscosity)
+  float* sdf = nullptr;
+  uint8_t* cls = nullptr;
+  int32_t* bed_id = nullptr;
+  cs_bed_cell* bed = nullptr;
+  uint32_t num_bed_cells = 0;
+  int8_t* kind = nullptr;
+  float* kf = nullptr;
+  int32_t* pres = nullptr;
+  float* rhs = nullptr;
+  int32_t* pack = nullptr;
+  int32_t* prev_p = nullptr;
+  float* warm = nullptr;
+  bool has_warm = false;
+  uint32_t* metrics = nullptr;
+  double* res_partials = nullptr;
+  double* res_out = nullptr;
+  // host state
+  double* h_res = nullptr;    // pinned
+  float* h_stage = nullptr;   // pinned emission staging
+  size_t h_stage_cap = 0;
+  hipEvent_t stage_done = nullptr;
+  std::vector<float> init_particles, init_affine;
+  uint32_t init_count = 0;
+  cs::Inflow inflow;
+  uint32_t num_water = 0;
+  float total_time = 0.0f;
+  float total_emitted_mass = 0.0f;
+  uint32_t total_dropped = 0;
+  cs_solve_stats stats;
+  bool final_residual_pending = false;
+  bool timing = false;
+  hipEvent_t ev[P_COUNT] = {};
+  float pass_us[7] = {};
+  // per-kernel timing (cs_kernel_timing): [p2g start, end, g2p start, end, rbgs start, end]
+  hipEvent_t kev[6] = {};
+  cs_kernel_timing kt = {};
+  std::vector<float> emit_buf;
+};
+
+namespace {
+
+void free_sim(cs_sim* m) {
+  if (!m) return;
+  void* dev[] = {m->part, m->aff, m->lanes, m->gv,   m->gv2,  m->sdf,     m->cls,          m->bed_id,
+                 m->bed,  m->kind, m->kf,  m->pres, m->rhs,  m->pack,    m->prev_p,       m->warm,
+                 m->metrics, m->res_partials, m->res_out};
+  for (void* p : dev)
+    if (p) (void)hipFree(p);
+  if (m->h_res) (void)hipHostFree(m->h_res);
+  if (m->h_stage) (void)hipHostFree(m->h_stage);
+  if (m->stage_done) (void)hipEventDestroy(m->stage_done);
+  for (auto& e : m->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : m->kev)
+    if (e) (void)hipEventDestroy(e);
+  delete m;
+}
+
+int upload_initial(cs_sim* m) {
+  hipStream_t st = m->ctx->stream;
+  CS_HIP(hipMemsetAsync(m->part, 0, (size_t)m->s.max_particles * 32, st));
+  CS_HIP(hipMemsetAsync(m->aff, 0, (size_t)m->s.max_particles * 48, st));
+  if (m->init_count) {
+    CS_HIP(hipMemcpyAsync(m->part, m->init_particles.data(), (size_t)m->init_count * 32, hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(m->aff, m->init_affine.data(), (size_t)m->init_count * 48, hipMemcpyHostToDevice, st));
+  }
+  CS_HIP(hipMemsetAsync(m->prev_p, 0, m->nc * sizeof(int32_t), st));
+  CS_HIP(hipStreamSynchronize(st));
+  return CS_OK;
+}
+
+void mark(cs_sim* m, int pass) {
+  if (m->timing) (void)hipEventRecord(m->ev[pass], m->ctx->stream);
+}
+void kmark(cs_sim* m, int which) {
+  if (m->timing) (void)hipEventRecord(m->kev[which], m->ctx->stream);
+}
+double kelapsed_us(cs_sim* m, int a, int b) {
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, m->kev[a], m->kev[b]) != hipSuccess) return 0.0;
+  return (double)ms * 1000.0;
+}
+
+// Residual of the current pressure; blocks on the stream.
+int residual_now(cs_sim* m, const cs_params& u, double* out) {
+  hipStream_t st = m->ctx->stream;
+  CS_HIP(cs_launch_residual(st, u, m->kf, m->rhs, m->pres, m->res_partials, CS_RESIDUAL_BLOCKS, m->res_out));
+  CS_HIP(hipMemcpyAsync(m->h_res, m->res_out, sizeof(double), hipMemcpyDeviceToHost, st));
+  CS_HIP(hipStreamSynchronize(st));
+  *out = m->h_res[0];
+  return CS_OK;
+}
+
+int solve(cs_sim* m, const cs_params& u) {
+  hipStream_t st = m->ctx->stream;
+  cs_solve_stats& S = m->stats;
+  memset(&S, 0, sizeof(S));
+  S.final_residual = NAN;
+  m->final_residual_pending = false;
+  bool external = false;
+  if (m->s.warm_start == CS_WARM_PREVIOUS) {
+    CS_HIP(cs_launch_warm_previous(st, u, m->kf, m->prev_p, m->pres));
+  } else if (m->s.warm_start == CS_WARM_EXTERNAL && m->has_warm) {
+    CS_HIP(cs_launch_warm_external(st, u, m->kf, m->warm, m->pres, m->metrics));
+    external = true;
+  }
+  if (m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS) {
+    kmark(m, 4);
+    for (uint32_t k = 0; k < m->s.pressure_rbgs_pairs; ++k)
+      CS_HIP(cs_launch_rbgs_pair(st, u, m->kf, m->rhs, m->pres, m->metrics));
+    kmark(m, 5);
+    if (m->timing) {
+      CS_HIP(hipEventSynchronize(m->kev[5]));
+      m->kt.rbgs_us += kelapsed_us(m, 4, 5);
+      m->kt.rbgs_launches += 2ull * m->s.pressure_rbgs_pairs;
+      m->kt.rbgs_cells_swept += 2ull * m->s.pressure_rbgs_pairs * m->nc;
+    }
+    S.iterations = m->s.pressure_rbgs_pairs;
+    // residual of the final pressure, read lazily by cs_sim_get_solve_stats
+    CS_HIP(cs_launch_residual(st, u, m->kf, m->rhs, m->pres, m->res_partials, CS_RESIDUAL_BLOCKS, m->res_out));
+    CS_HIP(hipMemcpyAsync(m->h_res, m->res_out, sizeof(double), hipMemcpyDeviceToHost, st));
+    m->final_residual_pending = true;
+  } else {
+    double r = 0.0;
+    int rc = residual_now(m, u, &r);
+    if (rc) return rc;
+    if (external && r > 1.0) {
+      // the supplied start is worse than the zero start: fall back
+      CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
+      S.fell_back = 1;
+      if ((rc = residual_now(m, u, &r))) return rc;
+    }
+    S.initial_residual = r;
+    const uint32_t every = m->s.rbgs_check_every ? m->s.rbgs_check_every : 8u;
+    const uint32_t cap = m->s.rbgs_max_iters;
+    const double tol = (double)m->s.rbgs_tolerance;
+    uint32_t it = 0;
+    while (!(r <= tol) && it < cap) {
+      uint32_t chunk = every - (it % every);
+      chunk = std::min(chunk, cap - it);
+      kmark(m, 4);
+      for (uint32_t k = 0; k < chunk; ++k)
+        CS_HIP(cs_launch_rbgs_pair(st, u, m->kf, m->rhs, m->pres, m->metrics));
+      kmark(m, 5);
+      it += chunk;
+      if ((rc = residual_now(m, u, &r))) return rc;
+      if (m->timing) {
+        m->kt.rbgs_us += kelapsed_us(m, 4, 5);
+        m->kt.rbgs_launches += 2ull * chunk;
+        m->kt.rbgs_cells_swept += 2ull * chunk * m->nc;
+      }
+    }
+    S.iterations = it;
+    S.final_residual = r;
+    S.converged = r <= tol ? 1u : 0u;
+  }
+  if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_copy_i32(st, m->pres, m->prev_p, m->nc));
+  return CS_OK;
+}
+
+int substep(cs_sim* m, float sub_dt) {
+  hipStream_t st = m->ctx->stream;
+  // 1. emission on the host (inflow.rs:122-241), appended after the live particles
+  m->emit_buf.clear();
+  uint32_t emitted = cs::emit(m->s, m->inflow, sub_dt, m->num_water, m->emit_buf, &m->total_dropped);
+  if (emitted) {
+    size_t bytes = (size_t)emitted * 32;
+    CS_HIP(hipEventSynchronize(m->stage_done));
+    if (bytes > m->h_stage_cap) {
+      if (m->h_stage) CS_HIP(hipHostFree(m->h_stage));
+      m->h_stage = nullptr;
+      CS_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->h_stage), bytes));
+      m->h_stage_cap = bytes;
+    }
+    memcpy(m->h_stage, m->emit_buf.data(), bytes);
+    CS_HIP(hipMemcpyAsync(m->part + (size_t)m->num_water * 8, m->h_stage, bytes, hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemsetAsync(m->aff + (size_t)m->num_water * 12, 0, (size_t)emitted * 48, st));
+    CS_HIP(hipEventRecord(m->stage_done, st));
+    m->num_water += emitted;
+    m->total_emitted_mass += (float)emitted * (80.0f / 320.0f);
+  }
+  // 2. parameter block (write_uniforms, mod.rs:569-670)
+  cs_params u;
+  cs::make_params(m->s, m->num_water, m->num_bed_cells, sub_dt, &u);
+  // 3. clear (mod.rs:308-309) and metrics_clear (shader.rs:2282-2287)
+  mark(m, P_P2G);
+  CS_HIP(cs_launch_fill_i32(st, m->lanes, m->nc * 6, 0));
+  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->metrics), CS_METRICS_SLOTS, 0));
+  kmark(m, 0);
+  CS_HIP(cs_launch_p2g(st, u, m->part, m->aff, m->lanes, m->metrics));
+  kmark(m, 1);
+  mark(m, P_GRID);
+  CS_HIP(cs_launch_grid(st, u, m->lanes, m->gv, m->sdf));
+  mark(m, P_CLASSIFY);
+  CS_HIP(cs_launch_classify(st, u, m->gv, m->cls, m->lanes, m->bed_id, m->bed, m->kind, m->kf, m->pres, m->rhs,
+                            m->pack, m->metrics, m->sdf));
+  mark(m, P_SOLVE);
+  int rc = solve(m, u);
+  if (rc) return rc;
+  mark(m, P_POST);
+  CS_HIP(cs_launch_project(st, u, m->gv, m->kf, m->pres, m->bed_id, m->bed, m->sdf));
+  CS_HIP(cs_launch_pack(st, u, m->gv, m->kf, m->pack, m->sdf));
+  CS_HIP(cs_launch_visc(st, u, m->gv, m->gv2, m->cls, m->sdf));
+  mark(m, P_G2P);
+  kmark(m, 2);
+  CS_HIP(cs_launch_g2p(st, u, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf));
+  kmark(m, 3);
+  mark(m, P_END);
+  if (m->timing) {
+    CS_HIP(hipEventSynchronize(m->ev[P_END]));
+    for (int p = 0; p < 6; ++p) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, m->ev[p], m->ev[p + 1]);
+      m->pass_us[p] = ms * 1000.0f;
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, m->ev[P_P2G], m->ev[P_END]);
+    m->pass_us[6] = ms * 1000.0f;
+    m->kt.p2g_us += kelapsed_us(m, 0, 1);
+    m->kt.p2g_launches += 1;
+    m->kt.p2g_particles += u.num_particles;
+    m->kt.g2p_us += kelapsed_us(m, 2, 3);
+    m->kt.g2p_launches += 1;
+    m->kt.g2p_particles += u.num_particles;
+    uint32_t slots[CS_METRICS_SLOTS];
+    CS_HIP(hipMemcpy(slots, m->metrics, sizeof(slots), hipMemcpyDeviceToHost));
+    m->kt.rbgs_iterations += m->stats.iterations;
+    m->kt.rbgs_cell_updates += (uint64_t)m->stats.iterations * slots[CS_M_FLUID_CELLS];
+  }
+  m->total_time += sub_dt;
+  return CS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+const char* cs_last_error(void) { return g_error.c_str(); }
+
+int cs_settings_preset(int preset, cs_settings* out) {
+  if (!out) return fail(CS_ERR_INVALID, "null settings");
+  int rc = cs::settings_preset(preset, out);
+  if (rc) return fail(rc, "unknown preset");
+  return CS_OK;
+}
+
+int cs_generate_sdf(const cs_settings* s, float* out) {
+  if (!s || !out) return fail(CS_ERR_INVALID, "null argument");
+  std::string why;
+  if (cs::validate_settings(*s, &why)) return fail(CS_ERR_INVALID, why);
+  cs::generate_sdf(*s, out);
+  return CS_OK;
+}
+
+int cs_generate_sdf_class(const cs_settings* s, const float* sdf, uint8_t* out) {
+  if (!s || !sdf || !out) return fail(CS_ERR_INVALID, "null argument");
+  std::string why;
+  if (cs::validate_settings(*s, &why)) return fail(CS_ERR_INVALID, why);
+  cs::generate_sdf_class(*s, sdf, out);
+  return CS_OK;
+}
+
+int cs_device_count(int* out) {
+  if (!out) return fail(CS_ERR_INVALID, "null argument");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *out = n;
+  return CS_OK;
+}
+
+int cs_ctx_create(int device, cs_ctx** out) {
+  if (!out) return fail(CS_ERR_INVALID, "null output");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CS_ERR_DEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(CS_ERR_INVALID, "device index out of range");
+  CS_HIP(hipSetDevice(device));
+  cs_ctx* c = new cs_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreateWithFlags");
+  }
+  *out = c;
+  return CS_OK;
+}
+
+int cs_ctx_destroy(cs_ctx* ctx) {
+  if (!ctx) return CS_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CS_OK;
+}
+
+int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_sim** out) {
+  if (!ctx || !s || !out) return fail(CS_ERR_INVALID, "null argument");
+  *out = nullptr;
+  std::string why;
+  if (cs::validate_settings(*s, &why)) return fail(CS_ERR_INVALID, why);
+  CS_HIP(hipSetDevice(ctx->device));
+  cs_sim* m = new cs_sim();
+  m->ctx = ctx;
+  m->s = *s;
+  m->gx = s->grid_dims[0];
+  m->gy = s->grid_dims[1];
+  m->gz = s->grid_dims[2];
+  m->nc = (size_t)m->gx * m->gy * m->gz;
+  const size_t nsdf = (size_t)s->sdf_res * s->sdf_res * s->sdf_res;
+  // scene arrays on the host first (state.rs:178-182)
+  std::vector<float> sdf_h;
+  const float* sdf_src = scene ? scene->sdf : nullptr;
+  if (!sdf_src) {
+    sdf_h.resize(nsdf);
+    cs::generate_sdf(*s, sdf_h.data());
+    sdf_src = sdf_h.data();
+  }
+  std::vector<uint8_t> cls_h;
+  const uint8_t* cls_src = scene ? scene->sdf_class : nullptr;
+  if (!cls_src) {
+    cls_h.resize(m->nc);
+    cs::generate_sdf_class(*s, sdf_src, cls_h.data());
+    cls_src = cls_h.data();
+  }
+  std::vector<int32_t> bed_index;
+  std::vector<cs_bed_cell> bed_table;
+  if (s->has_bed_field && scene && scene->bed_lookup && scene->bed_records) {
+    if (!cs::build_bed_table(*s, scene->bed_lookup, scene->bed_records, scene->num_bed_records, bed_index, bed_table)) {
+      free_sim(m);
+      return fail(CS_ERR_INVALID, "bed lookup index out of range");
+    }
+  } else {
+    bed_index.assign(m->nc, -1);
+  }
+  m->num_bed_cells = (uint32_t)bed_table.size();
+  if (scene && scene->particles && scene->num_particles) {
+    if (scene->num_particles > s->max_particles) {
+      free_sim(m);
+      return fail(CS_ERR_RANGE, "initial particles exceed max_particles");
+    }
+    m->init_count = scene->num_particles;
+    m->init_particles.assign(scene->particles, scene->particles + (size_t)m->init_count * 8);
+    if (scene->affine) m->init_affine.assign(scene->affine, scene->affine + (size_t)m->init_count * 12);
+    else m->init_affine.assign((size_t)m->init_count * 12, 0.0f);
+  }
+  hipError_t e = hipSuccess;
+  const size_t mp = s->max_particles;
+#define ALLOC(ptr, n)                                         \
+  if (e == hipSuccess) e = dalloc(&(ptr), (n));
+  ALLOC(m->part, mp * 8);
+  ALLOC(m->aff, mp * 12);
+  ALLOC(m->lanes, m->nc * 6);
+  ALLOC(m->gv, m->nc * 4);
+  ALLOC(m->gv2, m->nc * 4);
+  ALLOC(m->sdf, nsdf);
+  ALLOC(m->cls, m->nc);
+  ALLOC(m->bed_id, m->nc);
+  ALLOC(m->bed, bed_table.size());
+  ALLOC(m->kind, m->nc);
+  ALLOC(m->kf, m->nc);
+  ALLOC(m->pres, m->nc);
+  ALLOC(m->rhs, m->nc);
+  ALLOC(m->pack, m->nc);
+  ALLOC(m->prev_p, m->nc);
+  ALLOC(m->warm, m->nc);
+  ALLOC(m->metrics, CS_METRICS_SLOTS);
+  ALLOC(m->res_partials, 2 * CS_RESIDUAL_BLOCKS);
+  ALLOC(m->res_out, 1);
+#undef ALLOC
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_res), sizeof(double));
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&m->stage_done, hipEventDisableTiming);
+  for (auto& ev : m->ev)
+    if (e == hipSuccess) e = hipEventCreate(&ev);
+  for (auto& ev : m->kev)
+    if (e == hipSuccess) e = hipEventCreate(&ev);
+  if (e != hipSuccess) {
+    free_sim(m);
+    return fail(e == hipErrorOutOfMemory ? CS_ERR_NOMEM : CS_ERR_DEVICE, std::string("allocation: ") + hipGetErrorString(e));
+  }
+  hipStream_t st = ctx->stream;
+  e = hipMemcpyAsync(m->sdf, sdf_src, nsdf * sizeof(float), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->cls, cls_src, m->nc, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->bed_id, bed_index.data(), m->nc * sizeof(int32_t), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !bed_table.empty())
+    e = hipMemcpyAsync(m->bed, bed_table.data(), bed_table.size() * sizeof(cs_bed_cell), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(m->gv, 0, m->nc * 16, st);
+  if (e == hipSuccess) e = hipMemsetAsync(m->gv2, 0, m->nc * 16, st);
+  if (e == hipSuccess) e = hipMemsetAsync(m->pres, 0, m->nc * 4, st);
+  if (e == hipSuccess) e = hipMemsetAsync(m->kind, 0, m->nc, st);
+  if (e == hipSuccess) e = hipMemsetAsync(m->lanes, 0, m->nc * 24, st);
+  if (e == hipSuccess) e = hipMemsetAsync(m->metrics, 0, CS_METRICS_SLOTS * 4, st);
+  if (e == hipSuccess) e = hipEventRecord(m->stage_done, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    free_sim(m);
+    return hip_fail(e, "scene upload");
+  }
+  int rc = cs_sim_reset(m);
+  if (rc) {
+    free_sim(m);
+    return rc;
+  }
+  *out = m;
+  return CS_OK;
+}
+
+int cs_sim_destroy(cs_sim* m) {
+  if (!m) return CS_OK;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  free_sim(m);
+  return CS_OK;
+}
+
+int cs_sim_reset(cs_sim* m) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  m->num_water = m->init_count;
+  m->total_time = 0.0f;
+  m->total_emitted_mass = 0.0f;
+  m->total_dropped = 0;
+  m->inflow = cs::Inflow();
+  m->inflow.kettle_angle_deg = m->s.initial_kettle_angle_deg;
+  m->has_warm = false;
+  memset(&m->stats, 0, sizeof(m->stats));
+  m->final_residual_pending = false;
+  return upload_initial(m);
+}
+
+int cs_sim_step_frame(cs_sim* m, float dt) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  if (!(dt >= 0.0f)) return fail(CS_ERR_INVALID, "dt must be a non-negative number");
+  (void)hipSetDevice(m->ctx->device);
+  dt = cs_min(dt, 1.0f / 30.0f);  // mod.rs:265
+  const uint32_t n = std::max(m->s.substeps, 1u);
+  const float sub_dt = dt / (float)n;
+  for (uint32_t i = 0; i < n; ++i) {
+    int rc = substep(m, sub_dt);
+    if (rc) return rc;
+  }
+  return CS_OK;
+}
+
+int cs_sim_synchronize(cs_sim* m) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  CS_HIP(hipStreamSynchronize(m->ctx->stream));
+  return CS_OK;
+}
+
+int cs_sim_set_kettle_angle(cs_sim* m, float a) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  m->inflow.kettle_angle_deg = a;
+  return CS_OK;
+}
+int cs_sim_set_spout_position(cs_sim* m, float x, float y, float z) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  cs::spout_translate_origin_to(&m->s.spout, x, y, z);
+  return CS_OK;
+}
+int cs_sim_set_spout_target(cs_sim* m, float x, float y, float z) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  cs::spout_aim_at(&m->s.spout, x, y, z);
+  return CS_OK;
+}
+int cs_sim_get_counts(cs_sim* m, uint32_t* w, uint32_t* b, uint32_t* mp) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  if (w) *w = m->num_water;
+  if (b) *b = 0;
+  if (mp) *mp = m->s.max_particles;
+  return CS_OK;
+}
+int cs_sim_get_time(cs_sim* m, float* t, float* mass, uint32_t* dropped) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  if (t) *t = m->total_time;
+  if (mass) *mass = m->total_emitted_mass;
+  if (dropped) *dropped = m->total_dropped;
+  return CS_OK;
+}
+int cs_sim_get_inflow(cs_sim* m, float* a, float* f, float* e) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  if (a) *a = m->inflow.kettle_angle_deg;
+  if (f) *f = m->inflow.flow_rate;
+  if (e) *e = m->inflow.exit_speed;
+  return CS_OK;
+}
+
+int cs_sim_write_particles(cs_sim* m, uint32_t off, uint32_t count, const float* p8, const float* a12) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  if ((uint64_t)off + count > m->s.max_particles) return fail(CS_ERR_RANGE, "write past max_particles");
+  if (count == 0) return CS_OK;
+  hipStream_t st = m->ctx->stream;
+  if (p8) CS_HIP(hipMemcpyAsync(m->part + (size_t)off * 8, p8, (size_t)count * 32, hipMemcpyHostToDevice, st));
+  if (a12) CS_HIP(hipMemcpyAsync(m->aff + (size_t)off * 12, a12, (size_t)count * 48, hipMemcpyHostToDevice, st));
+  CS_HIP(hipStreamSynchronize(st));
+  return CS_OK;
+}
+int cs_sim_set_particle_count(cs_sim* m, uint32_t n) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  if (n > m->s.max_particles) return fail(CS_ERR_RANGE, "count past max_particles");
+  m->num_water = n;
+  return CS_OK;
+}
+
+static int read_dev(cs_sim* m, void* dst, const void* src, size_t bytes) {
+  if (!dst) return fail(CS_ERR_INVALID, "null destination");
+  if (bytes == 0) return CS_OK;
+  hipStream_t st = m->ctx->stream;
+  CS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+  CS_HIP(hipStreamSynchronize(st));
+  return CS_OK;
+}
+
+int cs_sim_read_particles(cs_sim* m, uint32_t off, uint32_t count, float* p8) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  if ((uint64_t)off + count > m->s.max_particles) return fail(CS_ERR_RANGE, "read past max_particles");
+  return read_dev(m, p8, m->part + (size_t)off * 8, (size_t)count * 32);
+}
+int cs_sim_read_affine(cs_sim* m, uint32_t off, uint32_t count, float* a12) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  if ((uint64_t)off + count > m->s.max_particles) return fail(CS_ERR_RANGE, "read past max_particles");
+  return read_dev(m, a12, m->aff + (size_t)off * 12, (size_t)count * 48);
+}
+int cs_sim_read_p2g_lanes(cs_sim* m, int32_t* l6) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  return read_dev(m, l6, m->lanes, m->nc * 24);
+}
+int cs_sim_read_grid_vel(cs_sim* m, float* v4) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  return read_dev(m, v4, m->gv2, m->nc * 16);
+}
+int cs_sim_read_pressure(cs_sim* m, int32_t* p, int32_t* kind) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  int rc = read_dev(m, p, m->pres, m->nc * 4);
+  if (rc || !kind) return rc;
+  std::vector<int8_t> k(m->nc);
+  if ((rc = read_dev(m, k.data(), m->kind, m->nc))) return rc;
+  for (size_t i = 0; i < m->nc; ++i) kind[i] = k[i];
+  return CS_OK;
+}
+int cs_sim_read_metrics(cs_sim* m, uint32_t* slots) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  return read_dev(m, slots, m->metrics, CS_METRICS_SLOTS * 4);
+}
+int cs_sim_get_solve_stats(cs_sim* m, cs_solve_stats* out) {
+  if (!m || !out) return fail(CS_ERR_INVALID, "null argument");
+  if (m->final_residual_pending) {
+    CS_HIP(hipStreamSynchronize(m->ctx->stream));
+    m->stats.final_residual = m->h_res[0];
+    m->stats.converged = m->stats.final_residual <= (double)m->s.rbgs_tolerance ? 1u : 0u;
+    m->final_residual_pending = false;
+  }
+  uint32_t slots[CS_METRICS_SLOTS];
+  int rc = cs_sim_read_metrics(m, slots);
+  if (rc) return rc;
+  m->stats.fluid_cells = slots[CS_M_FLUID_CELLS];
+  *out = m->stats;
+  return CS_OK;
+}
+
+int cs_sim_set_warm_start(cs_sim* m, const float* p) {
+  if (!m || !p) return fail(CS_ERR_INVALID, "null argument");
+  hipStream_t st = m->ctx->stream;
+  CS_HIP(hipMemcpyAsync(m->warm, p, m->nc * sizeof(float), hipMemcpyHostToDevice, st));
+  CS_HIP(hipStreamSynchronize(st));
+  m->has_warm = true;
+  return CS_OK;
+}
+
+// Checkpoint layout: magic, version, counts, inflow, counters, particles, affine, previous pressure.
+struct SaveHeader {
+  uint32_t magic, version, num_water, max_particles;
+  uint64_t nc;
+  cs::Inflow inflow;
+  float total_time, total_emitted_mass;
+  uint32_t total_dropped, pad;
+};
+static const uint32_t kMagic = 0x43534d31u;  // "CSM1"
+
+int cs_sim_save(cs_sim* m, void* buf, size_t cap, size_t* written) {
+  if (!m || !written) return fail(CS_ERR_INVALID, "null argument");
+  size_t need = sizeof(SaveHeader) + (size_t)m->num_water * 80 + m->nc * 4;
+  *written = need;
+  if (!buf) return CS_OK;  // size query
+  if (cap < need) return fail(CS_ERR_RANGE, "checkpoint buffer too small");
+  SaveHeader h{};
+  h.magic = kMagic;
+  h.version = 1;
+  h.num_water = m->num_water;
+  h.max_particles = m->s.max_particles;
+  h.nc = m->nc;
+  h.inflow = m->inflow;
+  h.total_time = m->total_time;
+  h.total_emitted_mass = m->total_emitted_mass;
+  h.total_dropped = m->total_dropped;
+  char* p = static_cast<char*>(buf);
+  memcpy(p, &h, sizeof(h));
+  p += sizeof(h);
+  int rc = read_dev(m, p, m->part, (size_t)m->num_water * 32);
+  if (rc) return rc;
+  p += (size_t)m->num_water * 32;
+  if ((rc = read_dev(m, p, m->aff, (size_t)m->num_water * 48))) return rc;
+  p += (size_t)m->num_water * 48;
+  return read_dev(m, p, m->prev_p, m->nc * 4);
+}
+
+int cs_sim_load(cs_sim* m, const void* buf, size_t size) {
+  if (!m || !buf) return fail(CS_ERR_INVALID, "null argument");
+  if (size < sizeof(SaveHeader)) return fail(CS_ERR_INVALID, "checkpoint truncated");
+  SaveHeader h;
+  memcpy(&h, buf, sizeof(h));
+  if (h.magic != kMagic || h.version != 1) return fail(CS_ERR_INVALID, "not a checkpoint");
+  if (h.nc != m->nc || h.max_particles != m->s.max_particles || h.num_water > m->s.max_particles)
+    return fail(CS_ERR_INVALID, "checkpoint does not match this simulation's settings");
+  size_t need = sizeof(SaveHeader) + (size_t)h.num_water * 80 + m->nc * 4;
+  if (size < need) return fail(CS_ERR_INVALID, "checkpoint truncated");
+  const char* p = static_cast<const char*>(buf) + sizeof(h);
+  hipStream_t st = m->ctx->stream;
+  CS_HIP(hipMemsetAsync(m->part, 0, (size_t)m->s.max_particles * 32, st));
+  CS_HIP(hipMemsetAsync(m->aff, 0, (size_t)m->s.max_particles * 48, st));
+  CS_HIP(hipMemcpyAsync(m->part, p, (size_t)h.num_water * 32, hipMemcpyHostToDevice, st));
+  p += (size_t)h.num_water * 32;
+  CS_HIP(hipMemcpyAsync(m->aff, p, (size_t)h.num_water * 48, hipMemcpyHostToDevice, st));
+  p += (size_t)h.num_water * 48;
+  CS_HIP(hipMemcpyAsync(m->prev_p, p, m->nc * 4, hipMemcpyHostToDevice, st));
+  CS_HIP(hipStreamSynchronize(st));
+  m->num_water = h.num_water;
+  m->inflow = h.inflow;
+  m->total_time = h.total_time;
+  m->total_emitted_mass = h.total_emitted_mass;
+  m->total_dropped = h.total_dropped;
+  return CS_OK;
+}
+
+int cs_sim_enable_timing(cs_sim* m, int enable) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  m->timing = enable != 0;
+  return CS_OK;
+}
+int cs_sim_get_pass_times(cs_sim* m, float* us7) {
+  if (!m || !us7) return fail(CS_ERR_INVALID, "null argument");
+  memcpy(us7, m->pass_us, sizeof(m->pass_us));
+  return CS_OK;
+}
+
+int cs_sim_get_kernel_timing(cs_sim* m, cs_kernel_timing* out) {
+  if (!m || !out) return fail(CS_ERR_INVALID, "null argument");
+  *out = m->kt;
+  return CS_OK;
+}
+int cs_sim_reset_kernel_timing(cs_sim* m) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  m->kt = cs_kernel_timing{};
+  return CS_OK;
+}
+
+int cs_device_expf(cs_ctx* ctx, const float* in, float* out, uint32_t n) {
+  if (!ctx || (!in && n) || (!out && n)) return fail(CS_ERR_INVALID, "null argument");
+  if (n == 0) return CS_OK;
+  CS_HIP(hipSetDevice(ctx->device));
+  float *din = nullptr, *dout = nullptr;
+  CS_HIP(dalloc(&din, n));
+  hipError_t e = dalloc(&dout, n);
+  if (e == hipSuccess) e = hipMemcpyAsync(din, in, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = cs_launch_expf(ctx->stream, din, dout, n);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(din);
+  if (dout) (void)hipFree(dout);
+  if (e != hipSuccess) return hip_fail(e, "cs_device_expf");
+  return CS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,314 @@
+/*
+ * coffee_sim.h — C ABI of the MI355X-native MLS-MPM substep.
+ *
+ * Drop-in boundary for the GPU solver of craigxchen/coffee-sim.  The reference
+ * owns its solver through `MpmSim3D` (crates/sim-wasm/src/mpm_3d/mod.rs:179-193)
+ * and talks to the device through one wgpu bind group
+ * (crates/sim-wasm/src/mpm_3d/pipelines.rs:31-141).  Every entry point below
+ * replaces one method of `MpmSim3D` (or one buffer write/readback the reference
+ * performs through `queue.write_buffer` / `map_async`); the citation on each
+ * declaration names it.
+ *
+ * Conventions
+ *  - Every function returns int: CS_OK (0) or a negative CS_ERR_* code.  Nothing
+ *    aborts; `cs_last_error()` returns a message for the last failure on the
+ *    calling thread.
+ *  - Handles own their device memory.  Host pointers are borrowed for the call;
+ *    copies are synchronous.
+ *  - One `cs_sim` is externally single-threaded (the reference's `&mut self`).
+ *  - Byte layouts of particles (32 B), affine (48 B) and the grid lanes are the
+ *    reference's (state.rs:90-118; shader.rs:24-33) so a Rust host can
+ *    `bytemuck`-cast exactly as inflow.rs:175-217 does today.
+ *  - Axes are the reference's: y is up, cell = iz*gx*gy + iy*gx + ix
+ *    (shader.rs:148-150).
+ */
+#ifndef COFFEE_SIM_H
+#define COFFEE_SIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_ABI_VERSION 1
+
+/* ---- error codes ------------------------------------------------------- */
+#define CS_OK 0
+#define CS_ERR_INVALID -1  /* bad argument / inconsistent settings          */
+#define CS_ERR_DEVICE -2   /* HIP runtime error or no device                 */
+#define CS_ERR_NOMEM -3    /* device or host allocation failed               */
+#define CS_ERR_RANGE -4    /* offset/count outside the allocated capacity    */
+#define CS_ERR_STATE -5    /* call not valid in the current state            */
+
+#define CS_MAX_OBSTACLES 4
+#define CS_METRICS_SLOTS 8
+
+/* ---- scene description -------------------------------------------------- */
+
+/* Obstacle kinds, mpm/mod.rs:71-85 (`Obstacle`). */
+#define CS_OBSTACLE_NONE 0
+#define CS_OBSTACLE_TRUNCATED_CONE 1
+#define CS_OBSTACLE_CYLINDER 2
+
+typedef struct cs_obstacle {
+  int32_t kind;
+  float center[3];
+  float top_radius; /* cone */
+  float bot_radius; /* cone */
+  float radius;     /* cylinder */
+  float top_y;
+  float bot_y;
+} cs_obstacle;
+
+/* Spout / kettle model, mirrors `SpoutSettings` (mpm/inflow.rs:9-25). */
+typedef struct cs_spout {
+  float origin[3];
+  float direction[3];
+  float target[3];
+  float nozzle_radius;
+  float stem_radius;
+  float activation_angle_deg;
+  float full_flow_angle_deg;
+  float head_at_activation;
+  float head_at_full_angle;
+  float discharge_coeff;
+  float volume_to_ml;
+  float max_flow_rate_ml_s;
+  float gentle_exit_speed;
+  float max_exit_speed;
+  float stem_length;
+} cs_spout;
+
+/* Emitter modes. */
+#define CS_EMIT_FLOW_MODEL 0  /* reference: kettle angle -> head -> flow (inflow.rs:109-293) */
+#define CS_EMIT_FIXED_COUNT 1 /* fixed particles/substep at a given speed and jet radius    */
+
+/* Pressure solve modes. */
+#define CS_RBGS_FIXED_PAIRS 0 /* reference: `pressure_rbgs_pairs` red+black pairs (mod.rs:352-357) */
+#define CS_RBGS_TOLERANCE 1   /* iterate until the relative residual <= tolerance            */
+
+/* Warm starts for the pressure solve (the reference always starts at 0, shader.rs:1187). */
+#define CS_WARM_ZERO 0
+#define CS_WARM_PREVIOUS 1 /* previous substep's converged pressure                        */
+#define CS_WARM_EXTERNAL 2 /* pressure supplied by cs_sim_set_warm_start (learned predictor) */
+
+/* Analytic contact fallbacks that the reference hard-codes in WGSL
+ * (shader.rs:744-823, 1778-1779).  Reference mode plugs in its constants;
+ * config scenes describe their own cone and disable the cup. */
+typedef struct cs_geometry {
+  uint32_t cone_enabled;
+  float cone_bot_y, cone_top_y;       /* shader.rs:751-752: -3, 3             */
+  float cone_bot_radius;              /* dripper_outlet_radius (0.42)         */
+  float cone_top_radius;              /* outlet + 4.2634315 (shader.rs:101)   */
+  uint32_t cup_enabled;
+  float cup_top_y;                    /* -3.5 (shader.rs:805)                  */
+  float cup_radius;                   /* 3.0  (shader.rs:806)                  */
+  float cup_bottom_y;                 /* -8.0 (shader.rs:107)                  */
+} cs_geometry;
+
+/* Superset of `MpmSettings` (mpm/mod.rs:87-104).  Fields after `initial_kettle_angle_deg`
+ * are extensions; cs_settings_reference() fills them with the reference's values. */
+typedef struct cs_settings {
+  float bounds_size[3];
+  uint32_t grid_dims[3];
+  uint32_t max_particles;
+  uint32_t substeps;
+  float gravity;
+  float bulk_modulus;
+  float viscosity;
+  float render_radius;
+  uint32_t pressure_rbgs_pairs;
+  uint32_t use_sdf_cache;
+  uint32_t num_obstacles;
+  cs_obstacle obstacles[CS_MAX_OBSTACLES];
+  cs_spout spout;
+  float initial_kettle_angle_deg;
+  /* ---- extensions ---- */
+  uint32_t sdf_res;            /* 128 (state.rs:20)                               */
+  float particle_vol_factor;   /* p_vol = dx^3 * factor; 0.25 (mod.rs:576)         */
+  float wall_thickness;        /* OBSTACLE_WALL_THICKNESS 0.4 (mod.rs:33)          */
+  float water_viscosity_m2_s;  /* Darcy nu, 1e-6 (brew_config.rs:125)              */
+  float min_bed_permeability;  /* 1e-12 (brew_config.rs:126)                       */
+  cs_geometry geometry;
+  uint32_t emitter_mode;       /* CS_EMIT_*                                        */
+  uint32_t emit_per_substep;   /* fixed-count emitter                              */
+  uint32_t emit_total;         /* stop after this many particles (0 = unlimited)   */
+  float emit_speed;            /* scene units / s                                  */
+  float emit_jet_radius;       /* scene units                                      */
+  uint32_t rbgs_mode;          /* CS_RBGS_*                                        */
+  float rbgs_tolerance;        /* relative residual target (1e-4)                  */
+  uint32_t rbgs_check_every;   /* iterations between residual checks (8)           */
+  uint32_t rbgs_max_iters;     /* iteration cap (20000)                            */
+  uint32_t warm_start;         /* CS_WARM_*                                        */
+  uint32_t has_bed_field;      /* a per-cell porous bed is supplied with the scene */
+} cs_settings;
+
+/* Per-cell porous bed ("field mode").  The reference stamps bed-particle ids into
+ * `bed_lookup` and reads porosity / permeability / compaction / saturation from
+ * `bed_extract` and the stamping particle's height (shader.rs:419-666, 2291-2335);
+ * a field bed gives every bed cell its own record with the same meaning. */
+typedef struct cs_bed_record {
+  float porosity;     /* be.bed.y                                   */
+  float permeability; /* be.bed.z (m^2)                             */
+  float compaction;   /* be.bed.w                                   */
+  float saturation;   /* be.extract.w                               */
+  float sample_y;     /* height of the bed sample (particles[id].pos.y) */
+  float velocity[3];  /* bed matrix velocity (particles[id].vel)    */
+} cs_bed_record;
+
+/* Host-side scene arrays handed to cs_sim_create.  Any pointer may be NULL:
+ * the SDF and class mask are then generated from the obstacles exactly as
+ * MpmBuffers::new does (state.rs:178-182). */
+typedef struct cs_scene {
+  const float* sdf;          /* sdf_res^3, x fastest                         */
+  const uint8_t* sdf_class;  /* gx*gy*gz, 1 = solid                           */
+  const int32_t* bed_lookup; /* gx*gy*gz, -1 or index into bed_records        */
+  const cs_bed_record* bed_records;
+  uint32_t num_bed_records;
+  const float* particles;    /* 8 floats per particle: pos.xyz, J, vel.xyz, m  */
+  const float* affine;       /* 12 floats per particle: C columns, .w phase    */
+  uint32_t num_particles;    /* initial water particles                       */
+} cs_scene;
+
+/* Solver statistics of the most recent substep. */
+typedef struct cs_solve_stats {
+  uint32_t iterations;        /* red+black pairs executed                    */
+  uint32_t fluid_cells;       /* cells in the pressure system                */
+  double initial_residual;    /* before the first sweep                      */
+  double final_residual;      /* at exit (NaN when never measured)           */
+  uint32_t converged;         /* 1 when final_residual <= tolerance          */
+  uint32_t fell_back;         /* external warm start rejected (residual grew) */
+} cs_solve_stats;
+
+/* ---- settings presets (host only, no device needed) ---------------------- */
+
+/* MpmSettings::default_v60 / benchmark_free_stream / benchmark_center_pour
+ * (mpm/mod.rs:107-161) plus reference values for every extension field. */
+#define CS_PRESET_DEFAULT_V60 0
+#define CS_PRESET_FREE_STREAM 1
+#define CS_PRESET_CENTER_POUR 2
+int cs_settings_preset(int preset, cs_settings* out);
+
+/* Seeded synthetic workloads (no reference counterpart; the reference has no RNG).
+ *   CS_WORKLOAD_V60_SMALL: 64^3, dx 2 mm, 32,768-particle jittered cylinder over a flat bed.
+ *   CS_WORKLOAD_V60_WORKING: 128^3, dx 1 mm, 1,048,576 particles (70% pre-wetted slab,
+ *   30% poured at 2,000 particles/substep), value-noise porosity, Kozeny-Carman permeability.
+ * `scale` (>= 1) divides the grid side and particle counts for quick tests (1 = full size).
+ * The arrays are allocated by the library; release them with cs_workload_free. */
+#define CS_WORKLOAD_V60_SMALL 0
+#define CS_WORKLOAD_V60_WORKING 1
+typedef struct cs_workload {
+  cs_settings settings;
+  cs_scene scene;
+  float frame_dt;          /* seconds per cs_sim_step_frame call               */
+  uint32_t substeps_total; /* run length of the workload                       */
+  uint32_t total_particles;/* initial + emitted particles                      */
+  void* _owner;            /* library-owned storage                            */
+} cs_workload;
+int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs_workload* out);
+int cs_workload_free(cs_workload* w);
+
+/* Scene generators exposed for verification (state.rs:250-344). */
+int cs_generate_sdf(const cs_settings* s, float* out_sdf /* sdf_res^3 */);
+int cs_generate_sdf_class(const cs_settings* s, const float* sdf, uint8_t* out_class /* gx*gy*gz */);
+
+/* ---- context ------------------------------------------------------------- */
+typedef struct cs_ctx cs_ctx;
+typedef struct cs_sim cs_sim;
+
+/* Device + stream.  Replaces the wgpu Device/Queue pair that MpmSim3D borrows
+ * per call (mod.rs:196, mod.rs:264). */
+int cs_ctx_create(int device, cs_ctx** out);
+int cs_ctx_destroy(cs_ctx* ctx);
+const char* cs_last_error(void);
+int cs_device_count(int* out);
+
+/* ---- simulation ---------------------------------------------------------- */
+
+/* MpmSim3D::new (mod.rs:196-220): buffers (state.rs:85-201), SDF + class
+ * textures, bed lookup, initial particles. */
+int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_sim** out);
+int cs_sim_destroy(cs_sim* sim);
+
+/* MpmSim3D::reset (mod.rs:432-446): counters, inflow state, initial particles. */
+int cs_sim_reset(cs_sim* sim);
+
+/* MpmSim3D::step_frame (mod.rs:264-430): dt clamped to 1/30, split into
+ * `substeps`, each substep = emit -> uniforms -> clear -> P2G -> grid update ->
+ * boundary -> classify -> pressure -> project -> packing -> viscosity -> G2P. */
+int cs_sim_step_frame(cs_sim* sim, float dt);
+
+/* Blocks until every pass enqueued by this handle has finished (the reference's
+ * device.poll(wait_indefinitely), physics_tests.rs:103). */
+int cs_sim_synchronize(cs_sim* sim);
+
+/* Setters/getters, mod.rs:448-538. */
+int cs_sim_set_kettle_angle(cs_sim* sim, float angle_deg);
+int cs_sim_set_spout_position(cs_sim* sim, float x, float y, float z);
+int cs_sim_set_spout_target(cs_sim* sim, float x, float y, float z);
+int cs_sim_get_counts(cs_sim* sim, uint32_t* num_water, uint32_t* num_bed, uint32_t* max_particles);
+int cs_sim_get_time(cs_sim* sim, float* total_time, float* total_emitted_mass, uint32_t* total_dropped);
+int cs_sim_get_inflow(cs_sim* sim, float* kettle_angle, float* flow_rate_ml_s, float* exit_speed);
+
+/* Particle append contract (inflow.rs:221-235): writes `count` particles at
+ * slot `offset` (particle8 = 8 floats, affine12 = 12 floats per particle). */
+int cs_sim_write_particles(cs_sim* sim, uint32_t offset, uint32_t count,
+                           const float* particle8, const float* affine12);
+int cs_sim_set_particle_count(cs_sim* sim, uint32_t num_water);
+
+/* Readbacks (the reference's staging copy + map_async, physics_tests.rs:61-139).
+ * Sizes are in bytes and must match the buffer extent requested. */
+int cs_sim_read_particles(cs_sim* sim, uint32_t offset, uint32_t count, float* particle8);
+int cs_sim_read_affine(cs_sim* sim, uint32_t offset, uint32_t count, float* affine12);
+/* The 6 i32 lanes (mass, mom xyz, rest vol, current vol) as accumulated by the
+ * most recent P2G, lane-major [6][cells] (state.rs:106-111). */
+int cs_sim_read_p2g_lanes(cs_sim* sim, int32_t* lanes6);
+/* grid_vel (v.xyz, mass) after the last pass of the most recent substep. */
+int cs_sim_read_grid_vel(cs_sim* sim, float* vel4);
+/* Pressure (i32 fixed point, 2^-18) and cell kind (0 air .. 4 solid) of the most recent solve. */
+int cs_sim_read_pressure(cs_sim* sim, int32_t* pressure_fp, int32_t* kind);
+/* Metric slots (state.rs:22-28). */
+int cs_sim_read_metrics(cs_sim* sim, uint32_t* slots8);
+int cs_sim_get_solve_stats(cs_sim* sim, cs_solve_stats* out);
+
+/* Warm start for the next pressure solve (CS_WARM_EXTERNAL): float pressure per cell. */
+int cs_sim_set_warm_start(cs_sim* sim, const float* pressure);
+
+/* Checkpoint / restore of the persistent state (particles, affine, inflow, counters). */
+int cs_sim_save(cs_sim* sim, void* buffer, size_t capacity, size_t* written);
+int cs_sim_load(cs_sim* sim, const void* buffer, size_t size);
+
+/* Device-side timing of the passes of the most recent substep, in microseconds,
+ * measured with HIP events on the solver's stream: [p2g, grid, classify,
+ * pressure, project+packing+viscosity, g2p, total]. */
+int cs_sim_get_pass_times(cs_sim* sim, float* us7);
+int cs_sim_enable_timing(cs_sim* sim, int enable);
+
+/* Device time of the three hot kernels summed since timing was enabled,
+ * measured with HIP events on the solver's stream around their launches. */
+typedef struct cs_kernel_timing {
+  double p2g_us;               /* P2G launches                                  */
+  uint64_t p2g_launches;
+  uint64_t p2g_particles;      /* particles scattered                           */
+  double g2p_us;               /* G2P launches                                  */
+  uint64_t g2p_launches;
+  uint64_t g2p_particles;
+  double rbgs_us;              /* red-black sweep launches (residual checks excluded) */
+  uint64_t rbgs_launches;
+  uint64_t rbgs_iterations;    /* red+black pairs                               */
+  uint64_t rbgs_cell_updates;  /* fluid-cell updates (iterations x fluid cells)  */
+  uint64_t rbgs_cells_swept;   /* cells the sweep launches visited              */
+} cs_kernel_timing;
+int cs_sim_get_kernel_timing(cs_sim* sim, cs_kernel_timing* out);
+int cs_sim_reset_kernel_timing(cs_sim* sim);
+
+/* Verification entry point (no reference counterpart): evaluates the solver's
+ * device exp (used by the J update, shader.rs:1853) on `n` inputs. */
+int cs_device_expf(cs_ctx* ctx, const float* in, float* out, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COFFEE_SIM_H */

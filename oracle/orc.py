@@ -1,0 +1,236 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).  TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as
+the checker; the product (coffee_sim_amd) never imports it.  `OracleSim` has
+the same methods as coffee_sim_amd.MpmSim3D so a parity test can drive both.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from coffee_sim_amd.abi import CS_METRICS_SLOTS, PRESETS, Scene, Settings, SolveStats, Spout
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liborc.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE), "-B", "liborc.so"], check=True)
+    global _lib
+    _lib = None
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        P = C.POINTER
+        f32, i32p, f32p = C.c_float, P(C.c_int32), P(C.c_float)
+        sigs = {
+            "orc_settings_preset": (C.c_int, [C.c_int, P(Settings)]),
+            "orc_generate_sdf": (None, [P(Settings), f32p]),
+            "orc_generate_sdf_class": (None, [P(Settings), f32p, P(C.c_uint8)]),
+            "orc_effective_head_from_angle": (f32, [f32, P(Spout)]),
+            "orc_exit_speed_cap_from_angle": (f32, [f32, P(Spout)]),
+            "orc_pour_fraction_from_angle": (f32, [f32, P(Spout)]),
+            "orc_exit_speed_from_head": (f32, [f32, f32]),
+            "orc_flow_rate_from_speed": (f32, [f32, f32, f32, f32, f32]),
+            "orc_spout_aim_at": (None, [P(Spout), f32, f32, f32]),
+            "orc_spout_translate_origin_to": (None, [P(Spout), f32, f32, f32]),
+            "orc_kozeny_carman_permeability": (f32, [f32, f32]),
+            "orc_expf": (f32, [f32]),
+            "orc_smoothstep": (f32, [f32, f32, f32]),
+            "orc_f2i": (C.c_int32, [f32]),
+            "orc_create": (C.c_void_p, [P(Settings), P(Scene)]),
+            "orc_destroy": (None, [C.c_void_p]),
+            "orc_reset": (C.c_int, [C.c_void_p]),
+            "orc_step_frame": (C.c_int, [C.c_void_p, f32]),
+            "orc_set_kettle_angle": (C.c_int, [C.c_void_p, f32]),
+            "orc_set_spout_position": (C.c_int, [C.c_void_p, f32, f32, f32]),
+            "orc_set_spout_target": (C.c_int, [C.c_void_p, f32, f32, f32]),
+            "orc_set_warm_start": (C.c_int, [C.c_void_p, f32p]),
+            "orc_write_particles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, f32p, f32p]),
+            "orc_set_particle_count": (C.c_int, [C.c_void_p, C.c_uint32]),
+            "orc_get_counts": (None, [C.c_void_p, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
+            "orc_get_time": (None, [C.c_void_p, P(f32), P(f32), P(C.c_uint32)]),
+            "orc_get_inflow": (None, [C.c_void_p, P(f32), P(f32), P(f32)]),
+            "orc_read_particles": (None, [C.c_void_p, C.c_uint32, C.c_uint32, f32p]),
+            "orc_read_affine": (None, [C.c_void_p, C.c_uint32, C.c_uint32, f32p]),
+            "orc_read_p2g_lanes": (None, [C.c_void_p, i32p]),
+            "orc_read_grid_vel": (None, [C.c_void_p, f32p]),
+            "orc_read_pressure": (None, [C.c_void_p, i32p, i32p]),
+            "orc_read_metrics": (None, [C.c_void_p, P(C.c_uint32)]),
+            "orc_get_solve_stats": (None, [C.c_void_p, P(SolveStats)]),
+            "orc_residual": (C.c_double, [C.c_void_p]),
+            "orc_read_system": (None, [C.c_void_p, f32p, f32p, f32p]),
+            "orc_debug_step_until": (C.c_int, [C.c_void_p, f32, C.c_int]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+PASS = {"p2g": 1, "grid": 2, "classify": 3, "rbgs": 4, "project": 5, "packing": 6, "viscosity": 7, "g2p": 8}
+
+
+def preset(name: str) -> Settings:
+    s = Settings()
+    rc = lib().orc_settings_preset(PRESETS[name], C.byref(s))
+    assert rc == 0
+    return s
+
+
+def generate_sdf(s: Settings) -> np.ndarray:
+    out = np.empty(int(s.sdf_res) ** 3, np.float32)
+    lib().orc_generate_sdf(C.byref(s), _f(out))
+    return out
+
+
+def generate_sdf_class(s: Settings, sdf: np.ndarray) -> np.ndarray:
+    out = np.empty(s.num_cells, np.uint8)
+    sdf = np.ascontiguousarray(sdf, np.float32)
+    lib().orc_generate_sdf_class(C.byref(s), _f(sdf), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out
+
+
+class OracleSim:
+    """CPU restatement of MpmSim3D with the same surface as coffee_sim_amd.MpmSim3D."""
+
+    def __init__(self, settings: Settings, scene: Scene | None = None):
+        self.settings = settings.copy()
+        h = lib().orc_create(C.byref(self.settings), C.byref(scene) if scene is not None else None)
+        if not h:
+            raise RuntimeError("orc_create failed")
+        self._h = C.c_void_p(h)
+        self.nc = self.settings.num_cells
+        self.dims = tuple(int(d) for d in self.settings.grid_dims)
+
+    def close(self):
+        if self._h:
+            lib().orc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        lib().orc_reset(self._h)
+
+    def step_frame(self, dt: float):
+        lib().orc_step_frame(self._h, dt)
+
+    def debug_step_until(self, sub_dt: float, stop_after: str):
+        lib().orc_debug_step_until(self._h, sub_dt, PASS[stop_after])
+
+    def set_kettle_angle(self, a):
+        lib().orc_set_kettle_angle(self._h, a)
+
+    def set_spout_position(self, x, y, z):
+        lib().orc_set_spout_position(self._h, x, y, z)
+
+    def set_spout_target(self, x, y, z):
+        lib().orc_set_spout_target(self._h, x, y, z)
+
+    def set_warm_start(self, p):
+        p = np.ascontiguousarray(p, np.float32).reshape(-1)
+        lib().orc_set_warm_start(self._h, _f(p))
+
+    def write_particles(self, offset, particles, affine=None):
+        p = np.ascontiguousarray(particles, np.float32).reshape(-1, 8)
+        a = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(-1, 12)
+        rc = lib().orc_write_particles(self._h, offset, p.shape[0], _f(p), None if a is None else _f(a))
+        assert rc == 0
+
+    def set_particle_count(self, n):
+        assert lib().orc_set_particle_count(self._h, n) == 0
+
+    def counts(self):
+        w, b, m = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        lib().orc_get_counts(self._h, C.byref(w), C.byref(b), C.byref(m))
+        return w.value, b.value, m.value
+
+    def particle_count(self):
+        w, b, _ = self.counts()
+        return w + b
+
+    def times(self):
+        t, m, d = C.c_float(), C.c_float(), C.c_uint32()
+        lib().orc_get_time(self._h, C.byref(t), C.byref(m), C.byref(d))
+        return t.value, m.value, d.value
+
+    def inflow(self):
+        a, f, e = C.c_float(), C.c_float(), C.c_float()
+        lib().orc_get_inflow(self._h, C.byref(a), C.byref(f), C.byref(e))
+        return a.value, f.value, e.value
+
+    def read_particles(self, offset=0, count=None):
+        if count is None:
+            count = self.particle_count() - offset
+        out = np.empty((count, 8), np.float32)
+        lib().orc_read_particles(self._h, offset, count, _f(out))
+        return out
+
+    def read_affine(self, offset=0, count=None):
+        if count is None:
+            count = self.particle_count() - offset
+        out = np.empty((count, 12), np.float32)
+        lib().orc_read_affine(self._h, offset, count, _f(out))
+        return out
+
+    def read_p2g_lanes(self):
+        out = np.empty((6, self.nc), np.int32)
+        lib().orc_read_p2g_lanes(self._h, _i(out))
+        return out
+
+    def read_grid_vel(self):
+        out = np.empty((self.nc, 4), np.float32)
+        lib().orc_read_grid_vel(self._h, _f(out))
+        return out
+
+    def read_pressure(self):
+        p = np.empty(self.nc, np.int32)
+        k = np.empty(self.nc, np.int32)
+        lib().orc_read_pressure(self._h, _i(p), _i(k))
+        return p, k
+
+    def read_metrics(self):
+        out = np.empty(CS_METRICS_SLOTS, np.uint32)
+        lib().orc_read_metrics(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return out
+
+    def solve_stats(self) -> SolveStats:
+        st = SolveStats()
+        lib().orc_get_solve_stats(self._h, C.byref(st))
+        return st
+
+    def residual(self) -> float:
+        return float(lib().orc_residual(self._h))
+
+    def read_system(self):
+        w = np.empty((self.nc, 6), np.float32)
+        cnt = np.empty(self.nc, np.float32)
+        c = np.empty(self.nc, np.float32)
+        lib().orc_read_system(self._h, _f(w), _f(cnt), _f(c))
+        return w, cnt, c

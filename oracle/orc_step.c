@@ -1,0 +1,985 @@
+/* orc_step.c — CPU oracle, device half: the WGSL compute kernels of
+ * crates/sim-wasm/src/mpm_3d/shader.rs restated pass by pass, the substep order
+ * of MpmSim3D::step_frame (mod.rs:264-430) and the residual-driven pressure
+ * solve.  TEST INFRASTRUCTURE ONLY (see orc.h). */
+#include <stdio.h>
+
+#include "orc_internal.h"
+
+typedef struct { int32_t x, y, z; } i3;
+
+static inline size_t cidx(const orc_params* u, uint32_t ix, uint32_t iy, uint32_t iz) {
+  return (size_t)iz * u->gx * u->gy + (size_t)iy * u->gx + ix; /* shader.rs:148-150 */
+}
+static inline int on_grid(const orc_params* u, int32_t x, int32_t y, int32_t z) {
+  return !(x < 0 || y < 0 || z < 0) && !((uint32_t)x >= u->gx || (uint32_t)y >= u->gy || (uint32_t)z >= u->gz);
+}
+static inline void decompose(const orc_params* u, size_t idx, uint32_t* ix, uint32_t* iy, uint32_t* iz) {
+  size_t plane = (size_t)u->gx * u->gy;
+  *iz = (uint32_t)(idx / plane);
+  size_t rem = idx % plane;
+  *iy = (uint32_t)(rem / u->gx);
+  *ix = (uint32_t)(rem % u->gx);
+}
+
+/* lane helpers (shader.rs:152-164) */
+#define LANE(S, l, c) ((S)->grid[(size_t)(l) * (S)->nc + (c)])
+static inline int32_t kind_load(orc_sim* S, size_t c) { return LANE(S, 3, c); }
+static inline float pressure_load(orc_sim* S, const orc_params* u, size_t c) {
+  return (float)LANE(S, 0, c) * u->inv_fp_scale;
+}
+/* shader.rs:202-209 */
+static inline void pressure_store(orc_sim* S, const orc_params* u, size_t c, float v) {
+  float lim = u->pressure_clamp;
+  float cl = o_clamp(v, -lim, lim);
+  if (cl != v) S->metrics[O_M_P_CLAMP] += 1u;
+  LANE(S, 0, c) = o_f2i(cl * u->fp_scale);
+}
+static inline float packing_load(orc_sim* S, const orc_params* u, size_t c) {
+  return (float)LANE(S, 2, c) * u->inv_fp_scale;
+}
+/* shader.rs:215-222 */
+static inline void packing_store(orc_sim* S, const orc_params* u, size_t c, float v) {
+  float cl = o_clamp(v, 0.0f, u->pressure_clamp);
+  if (cl != v) S->metrics[O_M_P_CLAMP] += 1u;
+  LANE(S, 2, c) = o_f2i(cl * u->fp_scale);
+}
+static inline float div_load(orc_sim* S, const orc_params* u, size_t c) {
+  return (float)LANE(S, 1, c) * u->inv_fp_scale;
+}
+/* shader.rs:352-359 */
+static inline void div_store(orc_sim* S, const orc_params* u, size_t c, float v) {
+  float lim = u->div_clamp;
+  float cl = o_clamp(v, -lim, lim);
+  if (cl != v) S->metrics[O_M_DIV_CLAMP] += 1u;
+  LANE(S, 1, c) = o_f2i(cl * u->fp_scale);
+}
+static inline float rest_vol(orc_sim* S, const orc_params* u, size_t c) {
+  return (float)LANE(S, 4, c) * u->inv_fp_scale;
+}
+static inline float cur_vol(orc_sim* S, const orc_params* u, size_t c) {
+  return (float)LANE(S, 5, c) * u->inv_fp_scale;
+}
+/* shader.rs:274-283 */
+static inline void vscratch_store(orc_sim* S, const orc_params* u, size_t c, v3 v) {
+  float speed = vlen(v);
+  if (speed > u->vel_cap) v = vmul(v, u->vel_cap / speed);
+  LANE(S, 1, c) = o_f2i(v.x * u->fp_scale);
+  LANE(S, 2, c) = o_f2i(v.y * u->fp_scale);
+  LANE(S, 3, c) = o_f2i(v.z * u->fp_scale);
+}
+static inline v3 vscratch_load(orc_sim* S, const orc_params* u, size_t c) {
+  return V((float)LANE(S, 1, c) * u->inv_fp_scale, (float)LANE(S, 2, c) * u->inv_fp_scale,
+           (float)LANE(S, 3, c) * u->inv_fp_scale);
+}
+
+static inline int is_fluid(int32_t k) { return k == O_CELL_INTERIOR || k == O_CELL_SURFACE || k == O_CELL_BED; }
+static inline int is_solid(int32_t k) { return k == O_CELL_SOLID; }
+
+/* shader.rs:293-304 */
+static float fill_fraction(orc_sim* S, const orc_params* u, size_t c, int32_t kind) {
+  if (kind == O_CELL_INTERIOR || kind == O_CELL_BED) return 1.0f;
+  if (kind != O_CELL_SURFACE) return 0.0f;
+  float cv = o_max(u->dx * u->dx * u->dx, 1e-8f);
+  float dep = o_max(rest_vol(S, u, c), cur_vol(S, u, c)) / cv;
+  return o_clamp(dep, 0.0f, 1.0f);
+}
+/* shader.rs:306-323 */
+static float face_weight(orc_sim* S, const orc_params* u, int32_t self_kind, float self_fill, size_t nc, int32_t nk) {
+  if (is_solid(nk)) return 0.0f;
+  if (self_kind == O_CELL_BED || nk == O_CELL_BED) return 1.0f;
+  if (!is_fluid(nk)) return self_fill;
+  return o_min(self_fill, fill_fraction(S, u, nc, nk));
+}
+
+/* ---- SDF sampling (shader.rs:673-715) -------------------------------------- */
+static float load_sdf_texel(orc_sim* S, const orc_params* u, int32_t x, int32_t y, int32_t z) {
+  int32_t mx = o_f2i(u->sdf_res) - 1;
+  x = x < 0 ? 0 : (x > mx ? mx : x);
+  y = y < 0 ? 0 : (y > mx ? mx : y);
+  z = z < 0 ? 0 : (z > mx ? mx : z);
+  size_t r = S->sdf_res;
+  return S->sdf[(size_t)z * r * r + (size_t)y * r + (size_t)x];
+}
+static float sample_sdf(orc_sim* S, const orc_params* u, v3 p) {
+  float res = u->sdf_res;
+  float bsx = u->bmax[0] * 2.0f, bsy = u->bmax[1] * 2.0f, bsz = u->bmax[2] * 2.0f;
+  float ux = (p.x + u->bmax[0]) / bsx * res - 0.5f;
+  float uy = (p.y + u->bmax[1]) / bsy * res - 0.5f;
+  float uz = (p.z + u->bmax[2]) / bsz * res - 0.5f;
+  float flx = floorf(ux), fly = floorf(uy), flz = floorf(uz);
+  int32_t bx = o_f2i(flx), by = o_f2i(fly), bz = o_f2i(flz);
+  float fx = ux - flx, fy = uy - fly, fz = uz - flz; /* WGSL fract */
+  float c000 = load_sdf_texel(S, u, bx, by, bz);
+  float c100 = load_sdf_texel(S, u, bx + 1, by, bz);
+  float c010 = load_sdf_texel(S, u, bx, by + 1, bz);
+  float c110 = load_sdf_texel(S, u, bx + 1, by + 1, bz);
+  float c001 = load_sdf_texel(S, u, bx, by, bz + 1);
+  float c101 = load_sdf_texel(S, u, bx + 1, by, bz + 1);
+  float c011 = load_sdf_texel(S, u, bx, by + 1, bz + 1);
+  float c111 = load_sdf_texel(S, u, bx + 1, by + 1, bz + 1);
+  float c00 = o_mix(c000, c100, fx);
+  float c10 = o_mix(c010, c110, fx);
+  float c01 = o_mix(c001, c101, fx);
+  float c11 = o_mix(c011, c111, fx);
+  float c0 = o_mix(c00, c10, fy);
+  float c1 = o_mix(c01, c11, fy);
+  return o_mix(c0, c1, fz);
+}
+static v3 sdf_gradient(orc_sim* S, const orc_params* u, v3 p) {
+  float e = u->dx;
+  float gxv = sample_sdf(S, u, V(p.x + e, p.y + 0.0f, p.z + 0.0f)) - sample_sdf(S, u, V(p.x - e, p.y - 0.0f, p.z - 0.0f));
+  float gyv = sample_sdf(S, u, V(p.x + 0.0f, p.y + e, p.z + 0.0f)) - sample_sdf(S, u, V(p.x - 0.0f, p.y - e, p.z - 0.0f));
+  float gzv = sample_sdf(S, u, V(p.x + 0.0f, p.y + 0.0f, p.z + e)) - sample_sdf(S, u, V(p.x - 0.0f, p.y - 0.0f, p.z - e));
+  v3 g = V(gxv, gyv, gzv);
+  float len = vlen(g);
+  if (len < 1e-8f) return V(0.0f, 0.0f, 0.0f);
+  return vdiv(g, len);
+}
+static int class_solid(orc_sim* S, const orc_params* u, int32_t x, int32_t y, int32_t z) {
+  return S->cls[cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z)] != 0;
+}
+
+/* ---- contact (shader.rs:717-903) ------------------------------------------ */
+typedef struct { v3 pos, vel; } contact_t;
+static contact_t radial_barrier(const orc_params* u, v3 pos, v3 vel, float cx, float cz, float max_r) {
+  float rx = pos.x - cx, rz = pos.z - cz;
+  float r = sqrtf(rx * rx + rz * rz);
+  if (r > max_r && r > 1e-6f) {
+    float ox = rx / r, oz = rz / r;
+    pos.x = cx + ox * max_r;
+    pos.z = cz + oz * max_r;
+    float vn = vel.x * ox + vel.z * oz;
+    if (vn > 0.0f) {
+      float tx = vel.x - ox * vn, tz = vel.z - oz * vn;
+      vel.x = tx * (1.0f - u->friction * 0.35f);
+      vel.z = tz * (1.0f - u->friction * 0.35f);
+    }
+  }
+  contact_t c = {pos, vel};
+  return c;
+}
+/* resolve_scene_obstacles with is_bed = false (water) */
+static contact_t scene_obstacles(const orc_params* u, v3 pos, v3 vel) {
+  const cs_geometry* g = &u->geo;
+  if (g->cone_enabled && pos.y <= g->cone_top_y && pos.y >= g->cone_bot_y) {
+    float t = o_clamp((pos.y - g->cone_bot_y) / (g->cone_top_y - g->cone_bot_y), 0.0f, 1.0f);
+    float cone_r = o_mix(g->cone_bot_radius, g->cone_top_radius, t) - u->contact_offset;
+    contact_t c = radial_barrier(u, pos, vel, 0.0f, 0.0f, cone_r);
+    pos = c.pos; vel = c.vel;
+  }
+  if (g->cup_enabled && pos.y <= g->cup_top_y) {
+    float cup_r = g->cup_radius - u->wall_half - u->contact_offset;
+    contact_t c = radial_barrier(u, pos, vel, 0.0f, 0.0f, cup_r);
+    pos = c.pos; vel = c.vel;
+    float floor_y = g->cup_bottom_y + u->wall_half + u->contact_offset;
+    if (pos.y < floor_y) {
+      pos.y = floor_y;
+      if (vel.y < 0.0f) {
+        vel.y = 0.0f;
+        vel.x *= 1.0f - u->friction * 0.55f;
+        vel.z *= 1.0f - u->friction * 0.55f;
+      }
+    }
+  }
+  contact_t c = {pos, vel};
+  return c;
+}
+static contact_t sdf_contact(orc_sim* S, const orc_params* u, v3 pos, v3 vel) {
+  float sv = sample_sdf(S, u, pos);
+  if (sv < u->contact_offset) {
+    v3 n = sdf_gradient(S, u, pos);
+    if (vlen(n) > 1e-6f) {
+      pos = vadd(pos, vmul(n, u->contact_offset - sv));
+      float vn = vdot(vel, n);
+      if (vn < 0.0f) {
+        vel = vsub(vel, vmul(vmul(n, vn), 1.0f + u->restitution));
+        v3 vt = vsub(vel, vmul(n, vdot(vel, n)));
+        float vtl = vlen(vt);
+        if (vtl > 1e-6f) {
+          float fi = o_min(u->friction * fabsf(vn), vtl);
+          vel = vsub(vel, vmul(vt, fi / vtl));
+        }
+      }
+    }
+  }
+  return scene_obstacles(u, pos, vel);
+}
+
+/* ---- bed record access (field mode of shader.rs:361-449) ------------------- */
+static int bed_valid(orc_sim* S, int32_t id) { return id >= 0 && (uint32_t)id < S->num_bed; }
+static float kozeny_factor(float phi_in) { /* shader.rs:429-433 */
+  float phi = o_clamp(phi_in, 0.08f, 0.82f);
+  float solid = o_max(1.0f - phi, 1e-4f);
+  return (phi * phi * phi) / (solid * solid);
+}
+static float bed_compacted_porosity(orc_sim* S, int32_t id) { /* shader.rs:435-440 */
+  const cs_bed_record* b = &S->bed[id];
+  float base = o_clamp(b->porosity, 0.18f, 0.70f);
+  float comp = o_clamp(b->compaction, 0.0f, 2.25f);
+  return o_clamp(base - comp * 0.12f, 0.16f, base);
+}
+static float bed_compacted_permeability(orc_sim* S, const orc_params* u, int32_t id) { /* shader.rs:442-449 */
+  const cs_bed_record* b = &S->bed[id];
+  float base = o_clamp(b->porosity, 0.18f, 0.70f);
+  float compacted = bed_compacted_porosity(S, id);
+  float ratio = kozeny_factor(compacted) / o_max(kozeny_factor(base), 1e-8f);
+  return o_max(b->permeability * ratio, u->min_k);
+}
+static v3 bed_matrix_velocity_cell(orc_sim* S, const orc_params* u, i3 cell) { /* shader.rs:380-407 */
+  v3 sum = V(0.0f, 0.0f, 0.0f);
+  float wsum = 0.0f;
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        int32_t x = cell.x + di, y = cell.y + dj, z = cell.z + dk;
+        if (!on_grid(u, x, y, z)) continue;
+        int32_t id = S->bed_lookup[cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z)];
+        if (!bed_valid(S, id)) continue;
+        sum = vadd(sum, V(S->bed[id].velocity[0], S->bed[id].velocity[1], S->bed[id].velocity[2]));
+        wsum += 1.0f;
+      }
+  if (wsum <= 0.0f) return V(0.0f, 0.0f, 0.0f);
+  return vdiv(sum, wsum);
+}
+static v3 bed_velocity_load_cell(orc_sim* S, const orc_params* u, i3 c) { /* shader.rs:409-417 */
+  if (!on_grid(u, c.x, c.y, c.z)) return V(0.0f, 0.0f, 0.0f);
+  return bed_matrix_velocity_cell(S, u, c);
+}
+static float bed_velocity_divergence(orc_sim* S, const orc_params* u, i3 c) { /* shader.rs:419-427 */
+  i3 a;
+  a = c; a.x -= 1; float vxm = bed_velocity_load_cell(S, u, a).x;
+  a = c; a.x += 1; float vxp = bed_velocity_load_cell(S, u, a).x;
+  a = c; a.y -= 1; float vym = bed_velocity_load_cell(S, u, a).y;
+  a = c; a.y += 1; float vyp = bed_velocity_load_cell(S, u, a).y;
+  a = c; a.z -= 1; float vzm = bed_velocity_load_cell(S, u, a).z;
+  a = c; a.z += 1; float vzp = bed_velocity_load_cell(S, u, a).z;
+  return 0.5f * u->inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
+}
+
+/* shader.rs:597-619 */
+static float volume_target_div(const orc_params* u, float rest, float cur) {
+  float cv = u->dx * u->dx * u->dx;
+  float eps = o_max(cv * 1e-5f, u->p_vol * 1e-3f);
+  if (rest <= eps) return 0.0f;
+  float j = cur / rest;
+  float comp = o_clamp(1.0f - j, 0.0f, 0.75f);
+  float exp_ = o_clamp(j - 1.0f, 0.0f, 0.75f);
+  float target = comp * u->j_alpha - exp_ * u->j_expand_alpha;
+  float packed = o_max(rest, cur) / o_max(cv, 1e-8f);
+  float over = o_max(packed - u->max_rest_fraction, 0.0f);
+  target = o_max(target, over * u->j_alpha);
+  return target;
+}
+/* shader.rs:621-666 with the bed sample height taken from the bed record */
+static float bed_pore_target_div(orc_sim* S, const orc_params* u, int32_t id, v3 cc, float rest, float cur) {
+  float target = volume_target_div(u, rest, cur);
+  float cv = u->dx * u->dx * u->dx;
+  const cs_bed_record* b = &S->bed[id];
+  float base = bed_compacted_porosity(S, id);
+  float sat = o_clamp(b->saturation, 0.0f, 1.0f);
+  float comp = o_clamp(b->compaction, 0.0f, 2.25f);
+  float bed_y = b->sample_y;
+  float surface_t = o_smooth(-0.25f, 0.75f, (cc.y - bed_y) * u->inv_dx);
+  float open = o_clamp((1.0f - 0.65f * sat) * (1.0f - 0.12f * comp), 0.20f, 1.0f);
+  float opening = surface_t * o_clamp(u->surface_void_scale, 0.0f, 1.0f) * open;
+  float surf_por = o_min(base + 0.35f, 0.82f);
+  float eff = o_mix(base, surf_por, opening);
+  float occ = o_mix(1.0f, 0.35f, opening);
+  float satc = o_mix(1.0f, 0.32f, sat);
+  float compc = o_mix(1.0f, 0.58f, o_clamp(comp / 1.5f, 0.0f, 1.0f));
+  float cap = eff * cv * o_max(u->pore_capacity_scale, 0.0f) * satc * compc;
+  float overfill = o_max(cur - cap, 0.0f) / o_max(cv, 1e-8f);
+  float resp = u->pore_overfill_alpha * occ * o_mix(1.0f, 1.45f, sat);
+  target = o_max(target, overfill * resp);
+  return target;
+}
+
+/* ---- P2G (shader.rs:955-1052) --------------------------------------------- */
+static void k_p2g(orc_sim* S, const orc_params* u) {
+  uint32_t np = u->num_water + u->num_bed_particles;
+  float thr = u->nominal_mass * 0.10f;
+  for (uint32_t pid = 0; pid < np; ++pid) {
+    const float* p = &S->particles[(size_t)pid * 8];
+    const float* a = &S->affine[(size_t)pid * 12];
+    v3 xp = V(p[0], p[1], p[2]), vp = V(p[4], p[5], p[6]);
+    float J = p[3], mass_p = p[7], phase = a[3];
+    if (!(phase < 0.5f) || mass_p <= thr) continue;
+    v3 gp = vmul(vsub(xp, V(u->origin[0], u->origin[1], u->origin[2])), u->inv_dx);
+    int32_t bx = o_f2i(floorf(gp.x - 0.5f)), by = o_f2i(floorf(gp.y - 0.5f)), bz = o_f2i(floorf(gp.z - 0.5f));
+    v3 fx = V(gp.x - (float)bx, gp.y - (float)by, gp.z - (float)bz);
+    float wx[3], wy[3], wz[3];
+    wx[0] = 0.5f * (1.5f - fx.x) * (1.5f - fx.x);
+    wx[1] = 0.75f - (fx.x - 1.0f) * (fx.x - 1.0f);
+    wx[2] = 0.5f * (fx.x - 0.5f) * (fx.x - 0.5f);
+    wy[0] = 0.5f * (1.5f - fx.y) * (1.5f - fx.y);
+    wy[1] = 0.75f - (fx.y - 1.0f) * (fx.y - 1.0f);
+    wy[2] = 0.5f * (fx.y - 0.5f) * (fx.y - 0.5f);
+    wz[0] = 0.5f * (1.5f - fx.z) * (1.5f - fx.z);
+    wz[1] = 0.75f - (fx.z - 1.0f) * (fx.z - 1.0f);
+    wz[2] = 0.5f * (fx.z - 0.5f) * (fx.z - 0.5f);
+    v3 ac0 = V(mass_p * a[0], mass_p * a[1], mass_p * a[2]);
+    v3 ac1 = V(mass_p * a[4], mass_p * a[5], mass_p * a[6]);
+    v3 ac2 = V(mass_p * a[8], mass_p * a[9], mass_p * a[10]);
+    float fp = u->fp_scale, cdx = u->dx;
+    float rpv = u->p_vol * mass_p / o_max(u->nominal_mass, 1e-6f);
+    float cpv = rpv * o_clamp(J, 0.40f, 2.00f);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          int32_t cx = bx + i, cy = by + j, cz = bz + k;
+          if (!on_grid(u, cx, cy, cz)) continue;
+          float w = wx[i] * wy[j] * wz[k];
+          v3 dpos = vmul(vsub(V((float)i, (float)j, (float)k), fx), cdx);
+          float mc = w * mass_p, rvc = w * rpv, cvc = w * cpv;
+          v3 inner = vadd(vmul(vp, mass_p), V(vdot(ac0, dpos), vdot(ac1, dpos), vdot(ac2, dpos)));
+          v3 mom = vmul(inner, w);
+          size_t ci = cidx(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+          float lim = 1.0e9f;
+          float mfp = mc * fp, rfp = rvc * fp, cfp = cvc * fp;
+          float mxf = mom.x * fp, myf = mom.y * fp, mzf = mom.z * fp;
+          if (fabsf(mfp) > lim || fabsf(mxf) > lim || fabsf(myf) > lim || fabsf(mzf) > lim ||
+              fabsf(rfp) > lim || fabsf(cfp) > lim)
+            S->metrics[O_M_OVERFLOW] += 1u;
+          LANE(S, 0, ci) = o_wrap_add(LANE(S, 0, ci), o_f2i(mfp));
+          LANE(S, 1, ci) = o_wrap_add(LANE(S, 1, ci), o_f2i(mxf));
+          LANE(S, 2, ci) = o_wrap_add(LANE(S, 2, ci), o_f2i(myf));
+          LANE(S, 3, ci) = o_wrap_add(LANE(S, 3, ci), o_f2i(mzf));
+          LANE(S, 4, ci) = o_wrap_add(LANE(S, 4, ci), o_f2i(rfp));
+          LANE(S, 5, ci) = o_wrap_add(LANE(S, 5, ci), o_f2i(cfp));
+        }
+  }
+}
+
+/* ---- grid_update (shader.rs:1056-1082) ------------------------------------ */
+static void k_grid_update(orc_sim* S, const orc_params* u) {
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    float inv_fp = u->inv_fp_scale;
+    float mass = (float)LANE(S, 0, idx) * inv_fp;
+    if (mass < 1e-6f) continue;
+    v3 v = V((float)LANE(S, 1, idx) * inv_fp / mass, (float)LANE(S, 2, idx) * inv_fp / mass,
+             (float)LANE(S, 3, idx) * inv_fp / mass);
+    v.y += u->gravity * u->dt;
+    float speed = vlen(v);
+    if (speed > u->vel_cap) v = vmul(v, u->vel_cap / speed);
+    float* g = &S->grid_vel[idx * 4];
+    g[0] = v.x; g[1] = v.y; g[2] = v.z; g[3] = mass;
+  }
+}
+
+/* ---- boundary_project (shader.rs:1630-1679) -------------------------------- */
+static void k_boundary(orc_sim* S, const orc_params* u) {
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    float* g = &S->grid_vel[idx * 4];
+    if (g[3] < 1e-6f) continue;
+    v3 v = V(g[0], g[1], g[2]);
+    uint32_t ix, iy, iz;
+    decompose(u, idx, &ix, &iy, &iz);
+    v3 cp = V(u->origin[0] + ((float)ix + 0.5f) * u->dx, u->origin[1] + ((float)iy + 0.5f) * u->dx,
+              u->origin[2] + ((float)iz + 0.5f) * u->dx);
+    float sv = sample_sdf(S, u, cp);
+    if (sv < u->contact_offset) {
+      v3 n = sdf_gradient(S, u, cp);
+      float vn = vdot(v, n);
+      if (vn < 0.0f) {
+        v = vsub(v, vmul(vmul(n, vn), 1.0f + u->restitution));
+        v3 vt = vsub(v, vmul(n, vdot(v, n)));
+        float vtl = vlen(vt);
+        if (vtl > 1e-6f) {
+          float fi = o_min(u->friction * fabsf(vn), vtl);
+          v = vsub(v, vmul(vt, fi / vtl));
+        }
+      }
+    }
+    float margin = 2.0f * u->dx;
+    float bminx = u->origin[0] + margin, bminy = u->origin[1] + margin, bminz = u->origin[2] + margin;
+    float bmaxx = u->bmax[0] - margin, bmaxy = u->bmax[1] - margin, bmaxz = u->bmax[2] - margin;
+    if (cp.x < bminx && v.x < 0.0f) v.x = 0.0f;
+    if (cp.x > bmaxx && v.x > 0.0f) v.x = 0.0f;
+    if (cp.y < bminy && v.y < 0.0f) v.y = 0.0f;
+    if (cp.y > bmaxy && v.y > 0.0f) v.y = 0.0f;
+    if (cp.z < bminz && v.z < 0.0f) v.z = 0.0f;
+    if (cp.z > bmaxz && v.z > 0.0f) v.z = 0.0f;
+    g[0] = v.x; g[1] = v.y; g[2] = v.z;
+  }
+}
+
+/* ---- classify_cells (shader.rs:1181-1367) ---------------------------------- */
+static const int32_t OFF6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+
+static void k_classify(orc_sim* S, const orc_params* u) {
+  float occ = u->nominal_mass * 0.1f;
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    float mass = S->grid_vel[idx * 4 + 3];
+    pressure_store(S, u, idx, 0.0f);
+    uint32_t ix, iy, iz;
+    decompose(u, idx, &ix, &iy, &iz);
+    v3 cc = V(u->origin[0] + ((float)ix + 0.5f) * u->dx, u->origin[1] + ((float)iy + 0.5f) * u->dx,
+              u->origin[2] + ((float)iz + 0.5f) * u->dx);
+    int self_solid = u->use_sdf_cache ? class_solid(S, u, (int32_t)ix, (int32_t)iy, (int32_t)iz)
+                                      : (sample_sdf(S, u, cc) < 0.0f);
+    if (self_solid) {
+      LANE(S, 3, idx) = O_CELL_SOLID;
+      div_store(S, u, idx, 0.0f);
+      continue;
+    }
+    if (mass <= occ) {
+      LANE(S, 3, idx) = O_CELL_AIR;
+      div_store(S, u, idx, 0.0f);
+      continue;
+    }
+    if (S->bed_lookup[idx] >= 0) {
+      LANE(S, 3, idx) = O_CELL_BED;
+    } else {
+      int air = 0;
+      for (int n = 0; n < 6; ++n) {
+        int32_t x = (int32_t)ix + OFF6[n][0], y = (int32_t)iy + OFF6[n][1], z = (int32_t)iz + OFF6[n][2];
+        if (!on_grid(u, x, y, z)) { air = 1; break; }
+        if (S->grid_vel[cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z) * 4 + 3] <= occ) { air = 1; break; }
+      }
+      LANE(S, 3, idx) = air ? O_CELL_SURFACE : O_CELL_INTERIOR;
+    }
+    int32_t kind = kind_load(S, idx);
+    if (!is_fluid(kind)) { div_store(S, u, idx, 0.0f); continue; }
+    const float* sv = &S->grid_vel[idx * 4];
+    float dxv = u->dx;
+    float vxm = -sv[0], vxp = -sv[0], vym = -sv[1], vyp = -sv[1], vzm = -sv[2], vzp = -sv[2];
+#define OPEN(cond_sdf_pos, cx, cy, cz) \
+  (u->use_sdf_cache ? !class_solid(S, u, (cx), (cy), (cz)) : (sample_sdf(S, u, cond_sdf_pos) >= 0.0f))
+    if (ix > 0u && OPEN(V(cc.x + -dxv, cc.y + 0.0f, cc.z + 0.0f), (int32_t)ix - 1, (int32_t)iy, (int32_t)iz))
+      vxm = S->grid_vel[cidx(u, ix - 1u, iy, iz) * 4 + 0];
+    if (ix + 1u < u->gx && OPEN(V(cc.x + dxv, cc.y + 0.0f, cc.z + 0.0f), (int32_t)ix + 1, (int32_t)iy, (int32_t)iz))
+      vxp = S->grid_vel[cidx(u, ix + 1u, iy, iz) * 4 + 0];
+    if (iy > 0u && OPEN(V(cc.x + 0.0f, cc.y + -dxv, cc.z + 0.0f), (int32_t)ix, (int32_t)iy - 1, (int32_t)iz))
+      vym = S->grid_vel[cidx(u, ix, iy - 1u, iz) * 4 + 1];
+    if (iy + 1u < u->gy && OPEN(V(cc.x + 0.0f, cc.y + dxv, cc.z + 0.0f), (int32_t)ix, (int32_t)iy + 1, (int32_t)iz))
+      vyp = S->grid_vel[cidx(u, ix, iy + 1u, iz) * 4 + 1];
+    if (iz > 0u && OPEN(V(cc.x + 0.0f, cc.y + 0.0f, cc.z + -dxv), (int32_t)ix, (int32_t)iy, (int32_t)iz - 1))
+      vzm = S->grid_vel[cidx(u, ix, iy, iz - 1u) * 4 + 2];
+    if (iz + 1u < u->gz && OPEN(V(cc.x + 0.0f, cc.y + 0.0f, cc.z + dxv), (int32_t)ix, (int32_t)iy, (int32_t)iz + 1))
+      vzp = S->grid_vel[cidx(u, ix, iy, iz + 1u) * 4 + 2];
+#undef OPEN
+    float div = 0.5f * u->inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
+    float rest = rest_vol(S, u, idx), cur = cur_vol(S, u, idx);
+    float target = volume_target_div(u, rest, cur);
+    if (kind == O_CELL_BED) {
+      int32_t id = S->bed_lookup[idx];
+      if (id >= 0 && (uint32_t)id < S->num_bed) {
+        float pore = bed_pore_target_div(S, u, id, cc, rest, cur);
+        float por = o_clamp(bed_compacted_porosity(S, id), 0.08f, 1.0f);
+        float solid_fraction = 1.0f - por;
+        i3 c3 = {(int32_t)ix, (int32_t)iy, (int32_t)iz};
+        float solid_div = bed_velocity_divergence(S, u, c3);
+        target = (pore - solid_fraction * solid_div) / por;
+      }
+    }
+    div_store(S, u, idx, div - target);
+    float abs_div = fabsf(div);
+    uint32_t fpd = o_f2u(o_clamp(abs_div * u->metrics_div_fp_scale, 0.0f, 2147483648.0f));
+    if (fpd > S->metrics[O_M_MAX_ABS_DIV]) S->metrics[O_M_MAX_ABS_DIV] = fpd;
+    S->metrics[O_M_FLUID_CELLS] += 1u;
+  }
+}
+
+/* Face weights, their sum and the right-hand side of one fluid cell, exactly as
+ * pressure_update (shader.rs:1371-1439) forms them. */
+static void cell_system(orc_sim* S, const orc_params* u, size_t idx, int32_t kind, float* w6, size_t* nb6,
+                        int* nf6, float* count, float* c) {
+  uint32_t ix, iy, iz;
+  decompose(u, idx, &ix, &iy, &iz);
+  float self_fill = fill_fraction(S, u, idx, kind);
+  float cnt = 0.0f;
+  for (int n = 0; n < 6; ++n) {
+    w6[n] = 0.0f; nb6[n] = idx; nf6[n] = 0;
+    int32_t x = (int32_t)ix + OFF6[n][0], y = (int32_t)iy + OFF6[n][1], z = (int32_t)iz + OFF6[n][2];
+    if (!on_grid(u, x, y, z)) continue;
+    size_t ni = cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+    int32_t nk = kind_load(S, ni);
+    if (is_solid(nk)) continue;
+    float fw = face_weight(S, u, kind, self_fill, ni, nk);
+    if (fw <= 0.0f) continue;
+    cnt += fw;
+    w6[n] = fw; nb6[n] = ni; nf6[n] = is_fluid(nk);
+  }
+  *count = cnt;
+  float rhs = div_load(S, u, idx) * self_fill / o_max(u->dt, 1e-6f);
+  *c = u->dx * u->dx * rhs;
+}
+
+/* One row of the pressure system.  Kinds, fills and the divergence do not
+ * change during the solve, so the rows are formed once per solve; the
+ * arithmetic of each update is unchanged. */
+typedef struct {
+  size_t idx;
+  float w[6];
+  size_t nb[6];
+  int nf[6];
+  float count, c;
+} sys_row;
+
+static sys_row* g_rows[2] = {NULL, NULL};
+static size_t g_nrows[2] = {0, 0};
+static size_t g_cap[2] = {0, 0};
+
+/* Splits the fluid cells by colour (ix+iy+iz)&1 in ascending index order. */
+static void build_rows(orc_sim* S, const orc_params* u) {
+  g_nrows[0] = g_nrows[1] = 0;
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    int32_t kind = kind_load(S, idx);
+    if (!is_fluid(kind)) continue;
+    uint32_t ix, iy, iz;
+    decompose(u, idx, &ix, &iy, &iz);
+    int col = (int)((ix + iy + iz) & 1u);
+    if (g_nrows[col] == g_cap[col]) {
+      g_cap[col] = g_cap[col] ? g_cap[col] * 2 : 4096;
+      g_rows[col] = (sys_row*)realloc(g_rows[col], g_cap[col] * sizeof(sys_row));
+    }
+    sys_row* r = &g_rows[col][g_nrows[col]++];
+    r->idx = idx;
+    cell_system(S, u, idx, kind, r->w, r->nb, r->nf, &r->count, &r->c);
+  }
+}
+
+/* pressure_update (shader.rs:1371-1439) over one colour.  A cell of one
+ * colour reads only cells of the other colour, so the visiting order within a
+ * colour does not change any result; non-fluid cells keep the 0 written by
+ * classify. */
+static void k_rbgs_half(orc_sim* S, const orc_params* u, uint32_t parity) {
+  sys_row* rows = g_rows[parity];
+  size_t n = g_nrows[parity];
+  for (size_t i = 0; i < n; ++i) {
+    const sys_row* r = &rows[i];
+    float sum = 0.0f;
+    for (int k = 0; k < 6; ++k)
+      if (r->w[k] > 0.0f && r->nf[k]) sum += r->w[k] * pressure_load(S, u, r->nb[k]);
+    if (r->count <= 0.0f) { pressure_store(S, u, r->idx, 0.0f); continue; }
+    float p_new = (sum - r->c) / r->count;
+    pressure_store(S, u, r->idx, p_new);
+  }
+}
+
+/* Relative residual r = ||b - A p||_2 / ||b||_2 over fluid cells with a
+ * non-empty stencil, where the row of cell i reads count_i p_i - sum_n w_n p_n = -c_i
+ * (the fixed point of pressure_update); summed in ascending cell order. */
+static double residual_rows(orc_sim* S, const orc_params* u) {
+  double rr = 0.0, bb = 0.0;
+  size_t i0 = 0, i1 = 0;
+  while (i0 < g_nrows[0] || i1 < g_nrows[1]) {
+    const sys_row* r;
+    if (i1 >= g_nrows[1] || (i0 < g_nrows[0] && g_rows[0][i0].idx < g_rows[1][i1].idx)) r = &g_rows[0][i0++];
+    else r = &g_rows[1][i1++];
+    if (r->count <= 0.0f) continue;
+    double ap = (double)r->count * (double)pressure_load(S, u, r->idx);
+    for (int k = 0; k < 6; ++k)
+      if (r->w[k] > 0.0f && r->nf[k]) ap -= (double)r->w[k] * (double)pressure_load(S, u, r->nb[k]);
+    double b = -(double)r->c;
+    rr += (b - ap) * (b - ap);
+    bb += b * b;
+  }
+  if (bb > 0.0) return sqrt(rr / bb);
+  return rr > 0.0 ? INFINITY : 0.0;
+}
+
+double orc_residual(orc_sim* S) {
+  build_rows(S, &S->u);
+  return residual_rows(S, &S->u);
+}
+
+void orc_read_system(orc_sim* S, float* w6o, float* cnto, float* co) {
+  const orc_params* u = &S->u;
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    int32_t kind = kind_load(S, idx);
+    for (int n = 0; n < 6; ++n) w6o[idx * 6 + n] = 0.0f;
+    cnto[idx] = 0.0f; co[idx] = 0.0f;
+    if (!is_fluid(kind)) continue;
+    float w6[6], count, c;
+    size_t nb6[6];
+    int nf6[6];
+    cell_system(S, u, idx, kind, w6, nb6, nf6, &count, &c);
+    for (int n = 0; n < 6; ++n) w6o[idx * 6 + n] = w6[n];
+    cnto[idx] = count; co[idx] = c;
+  }
+}
+
+void orc_read_pressure(orc_sim* S, int32_t* p, int32_t* kind) {
+  /* lane 0 keeps the pressure to the end of the substep; the kind lane is
+   * overwritten by the viscosity scratch (shader.rs:274-283), so the copy
+   * taken at the end of the solve is returned */
+  memcpy(p, S->saved_p, S->nc * sizeof(int32_t));
+  if (kind) memcpy(kind, S->saved_kind, S->nc * sizeof(int32_t));
+}
+
+/* ---- pressure solve driver ------------------------------------------------- */
+static void solve_pressure(orc_sim* S, const orc_params* u) {
+  cs_solve_stats* st = &S->stats;
+  memset(st, 0, sizeof(*st));
+  st->final_residual = NAN;
+  uint32_t fluid = 0;
+  for (size_t i = 0; i < S->nc; ++i) fluid += is_fluid(kind_load(S, i)) ? 1u : 0u;
+  st->fluid_cells = fluid;
+  /* warm start injected after classify zeroed the pressure (shader.rs:1187) */
+  int warmed = 0;
+  if (S->s.warm_start == CS_WARM_PREVIOUS) {
+    for (size_t i = 0; i < S->nc; ++i)
+      if (is_fluid(kind_load(S, i))) LANE(S, 0, i) = S->prev_p[i];
+    warmed = 1;
+  } else if (S->s.warm_start == CS_WARM_EXTERNAL && S->has_warm) {
+    for (size_t i = 0; i < S->nc; ++i)
+      if (is_fluid(kind_load(S, i))) pressure_store(S, u, i, S->warm[i]);
+    warmed = 1;
+  }
+  build_rows(S, u);
+  if (S->s.rbgs_mode == CS_RBGS_FIXED_PAIRS) {
+    for (uint32_t k = 0; k < S->s.pressure_rbgs_pairs; ++k) {
+      k_rbgs_half(S, u, 0u);
+      k_rbgs_half(S, u, 1u);
+    }
+    st->iterations = S->s.pressure_rbgs_pairs;
+    st->final_residual = residual_rows(S, u);
+    st->converged = st->final_residual <= (double)S->s.rbgs_tolerance;
+  } else {
+    double r = residual_rows(S, u);
+    if (warmed && S->s.warm_start == CS_WARM_EXTERNAL && r > 1.0) {
+      /* the predicted start is worse than the zero start: fall back */
+      for (size_t i = 0; i < S->nc; ++i) LANE(S, 0, i) = 0;
+      st->fell_back = 1;
+      r = residual_rows(S, u);
+    }
+    st->initial_residual = r;
+    uint32_t it = 0, every = S->s.rbgs_check_every ? S->s.rbgs_check_every : 8u;
+    while (!(r <= (double)S->s.rbgs_tolerance) && it < S->s.rbgs_max_iters) {
+      k_rbgs_half(S, u, 0u);
+      k_rbgs_half(S, u, 1u);
+      ++it;
+      if (it % every == 0u || it >= S->s.rbgs_max_iters) r = residual_rows(S, u);
+    }
+    st->iterations = it;
+    st->final_residual = r;
+    st->converged = r <= (double)S->s.rbgs_tolerance;
+  }
+  for (size_t i = 0; i < S->nc; ++i) {
+    S->prev_p[i] = LANE(S, 0, i);
+    S->saved_p[i] = LANE(S, 0, i);
+    S->saved_kind[i] = LANE(S, 3, i);
+  }
+}
+
+/* ---- project_pressure (shader.rs:1457-1543) -------------------------------- */
+static float pressure_weighted_or_mirror(orc_sim* S, const orc_params* u, i3 c, float mirror, int32_t sk, float sf) {
+  if (!on_grid(u, c.x, c.y, c.z)) return mirror;
+  size_t ci = cidx(u, (uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z);
+  int32_t k = kind_load(S, ci);
+  if (is_solid(k)) return mirror;
+  float np = 0.0f;
+  if (is_fluid(k)) np = pressure_load(S, u, ci);
+  float fw = face_weight(S, u, sk, sf, ci, k);
+  return o_mix(mirror, np, fw);
+}
+static void k_project(orc_sim* S, const orc_params* u) {
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    float* g = &S->grid_vel[idx * 4];
+    if (g[3] < 1e-6f) continue;
+    int32_t kind = kind_load(S, idx);
+    if (!is_fluid(kind)) continue;
+    uint32_t ix, iy, iz;
+    decompose(u, idx, &ix, &iy, &iz);
+    float ph = pressure_load(S, u, idx);
+    float sf = fill_fraction(S, u, idx, kind);
+    i3 c = {(int32_t)ix, (int32_t)iy, (int32_t)iz}, a;
+    a = c; a.x -= 1; float pxm = pressure_weighted_or_mirror(S, u, a, ph, kind, sf);
+    a = c; a.x += 1; float pxp = pressure_weighted_or_mirror(S, u, a, ph, kind, sf);
+    a = c; a.y -= 1; float pym = pressure_weighted_or_mirror(S, u, a, ph, kind, sf);
+    a = c; a.y += 1; float pyp = pressure_weighted_or_mirror(S, u, a, ph, kind, sf);
+    a = c; a.z -= 1; float pzm = pressure_weighted_or_mirror(S, u, a, ph, kind, sf);
+    a = c; a.z += 1; float pzp = pressure_weighted_or_mirror(S, u, a, ph, kind, sf);
+    float h = 0.5f * u->inv_dx;
+    v3 grad = V(h * (pxp - pxm), h * (pyp - pym), h * (pzp - pzm));
+    v3 v = vsub(V(g[0], g[1], g[2]), vmul(grad, u->dt));
+    if (kind == O_CELL_BED) {
+      int32_t id = S->bed_lookup[idx];
+      if (id >= 0 && (uint32_t)id < S->num_bed) {
+        float k = bed_compacted_permeability(S, u, id);
+        float rate = u->water_nu / k;
+        float damping = 1.0f / (1.0f + o_max(rate * u->dt, 0.0f));
+        v3 bv = bed_matrix_velocity_cell(S, u, c);
+        v = vadd(bv, vmul(vsub(v, bv), damping));
+        /* the bed reaction impulse (shader.rs:1534-1535) feeds only the bed
+         * particle passes, which a field bed does not have */
+      }
+    }
+    float speed = vlen(v);
+    if (speed > u->vel_cap) v = vmul(v, u->vel_cap / speed);
+    g[0] = v.x; g[1] = v.y; g[2] = v.z;
+  }
+}
+
+/* ---- packing (shader.rs:1547-1626) ----------------------------------------- */
+static float packing_or_mirror(orc_sim* S, const orc_params* u, i3 c, float mirror) {
+  if (!on_grid(u, c.x, c.y, c.z)) return mirror;
+  size_t ci = cidx(u, (uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z);
+  int32_t k = kind_load(S, ci);
+  if (is_solid(k)) return mirror;
+  if (!is_fluid(k)) return 0.0f;
+  return packing_load(S, u, ci);
+}
+static void k_packing_prepare(orc_sim* S, const orc_params* u) {
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    int32_t kind = kind_load(S, idx);
+    if (!is_fluid(kind) || kind == O_CELL_BED) { packing_store(S, u, idx, 0.0f); continue; }
+    float cv = u->dx * u->dx * u->dx;
+    float packed = o_max(rest_vol(S, u, idx), cur_vol(S, u, idx)) / o_max(cv, 1e-8f);
+    float tgt = o_min(u->max_rest_fraction, 1.0f);
+    float over = o_max(packed - tgt, 0.0f);
+    packing_store(S, u, idx, u->bulk_K * over);
+  }
+}
+static void k_packing_apply(orc_sim* S, const orc_params* u) {
+  float occ = u->nominal_mass * 0.1f;
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    float* g = &S->grid_vel[idx * 4];
+    if (g[3] <= occ) continue;
+    int32_t kind = kind_load(S, idx);
+    if (!is_fluid(kind)) continue;
+    uint32_t ix, iy, iz;
+    decompose(u, idx, &ix, &iy, &iz);
+    i3 c = {(int32_t)ix, (int32_t)iy, (int32_t)iz}, a;
+    float ph = packing_load(S, u, idx);
+    a = c; a.x -= 1; float pxm = packing_or_mirror(S, u, a, ph);
+    a = c; a.x += 1; float pxp = packing_or_mirror(S, u, a, ph);
+    a = c; a.y -= 1; float pym = packing_or_mirror(S, u, a, ph);
+    a = c; a.y += 1; float pyp = packing_or_mirror(S, u, a, ph);
+    a = c; a.z -= 1; float pzm = packing_or_mirror(S, u, a, ph);
+    a = c; a.z += 1; float pzp = packing_or_mirror(S, u, a, ph);
+    float h = 0.5f * u->inv_dx;
+    v3 grad = V(h * (pxp - pxm), h * (pyp - pym), h * (pzp - pzm));
+    v3 v = vsub(V(g[0], g[1], g[2]), vmul(grad, u->dt));
+    float speed = vlen(v);
+    if (speed > u->vel_cap) v = vmul(v, u->vel_cap / speed);
+    g[0] = v.x; g[1] = v.y; g[2] = v.z;
+  }
+}
+
+/* ---- viscosity (shader.rs:1086-1177) ----------------------------------------- */
+static void k_viscosity_prepare(orc_sim* S, const orc_params* u) {
+  float occ = u->nominal_mass * 0.1f, support = u->nominal_mass * 2.0f;
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    const float* gv = &S->grid_vel[idx * 4];
+    v3 vh = V(gv[0], gv[1], gv[2]);
+    if (gv[3] <= occ || u->viscosity <= 0.0f) { vscratch_store(S, u, idx, vh); continue; }
+    uint32_t ix, iy, iz;
+    decompose(u, idx, &ix, &iy, &iz);
+    if (gv[3] <= support) { vscratch_store(S, u, idx, vh); continue; }
+    float alpha = o_clamp(u->viscosity * u->dt / o_max(u->dx * u->dx, 1e-6f), 0.0f, 0.14f);
+    v3 nsum = V(0.0f, 0.0f, 0.0f);
+    float wsum = 0.0f;
+    uint32_t nfl = 0;
+    for (int n = 0; n < 6; ++n) {
+      int32_t x = (int32_t)ix + OFF6[n][0], y = (int32_t)iy + OFF6[n][1], z = (int32_t)iz + OFF6[n][2];
+      if (!on_grid(u, x, y, z)) { wsum += 1.0f; continue; }
+      if (class_solid(S, u, x, y, z)) { wsum += 1.0f; continue; }
+      const float* ng = &S->grid_vel[cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z) * 4];
+      if (ng[3] <= occ) continue;
+      float nw = o_clamp(ng[3] / o_max(o_max(gv[3], u->nominal_mass), 1e-6f), 0.0f, 1.0f);
+      nsum = vadd(nsum, vmul(V(ng[0], ng[1], ng[2]), nw));
+      wsum += nw;
+      nfl += 1u;
+    }
+    if (nfl < 3u || wsum <= 1e-6f) { vscratch_store(S, u, idx, vh); continue; }
+    v3 avg = vdiv(nsum, wsum);
+    float blend = o_clamp(alpha * wsum, 0.0f, 0.65f);
+    vscratch_store(S, u, idx, V(o_mix(vh.x, avg.x, blend), o_mix(vh.y, avg.y, blend), o_mix(vh.z, avg.z, blend)));
+  }
+}
+static void k_viscosity_apply(orc_sim* S, const orc_params* u) {
+  float occ = u->nominal_mass * 0.1f;
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    float* g = &S->grid_vel[idx * 4];
+    float mass = g[3];
+    if (mass <= occ) { g[0] = g[1] = g[2] = g[3] = 0.0f; continue; }
+    v3 v = vscratch_load(S, u, idx);
+    g[0] = v.x; g[1] = v.y; g[2] = v.z; g[3] = mass;
+  }
+}
+
+/* ---- G2P (shader.rs:1683-1881) --------------------------------------------- */
+static void k_g2p(orc_sim* S, const orc_params* u) {
+  uint32_t np = u->num_water + u->num_bed_particles;
+  float thr = u->nominal_mass * 0.10f;
+  for (uint32_t pid = 0; pid < np; ++pid) {
+    float* p = &S->particles[(size_t)pid * 8];
+    float* a = &S->affine[(size_t)pid * 12];
+    v3 xp = V(p[0], p[1], p[2]);
+    float J_old = p[3], mass_p = p[7], phase = a[3];
+    if (!(phase < 0.5f)) continue;
+    if (mass_p <= thr) { p[7] = 0.0f; continue; }
+    v3 gp = vmul(vsub(xp, V(u->origin[0], u->origin[1], u->origin[2])), u->inv_dx);
+    int32_t bx = o_f2i(floorf(gp.x - 0.5f)), by = o_f2i(floorf(gp.y - 0.5f)), bz = o_f2i(floorf(gp.z - 0.5f));
+    v3 fx = V(gp.x - (float)bx, gp.y - (float)by, gp.z - (float)bz);
+    float wx[3], wy[3], wz[3];
+    wx[0] = 0.5f * (1.5f - fx.x) * (1.5f - fx.x);
+    wx[1] = 0.75f - (fx.x - 1.0f) * (fx.x - 1.0f);
+    wx[2] = 0.5f * (fx.x - 0.5f) * (fx.x - 0.5f);
+    wy[0] = 0.5f * (1.5f - fx.y) * (1.5f - fx.y);
+    wy[1] = 0.75f - (fx.y - 1.0f) * (fx.y - 1.0f);
+    wy[2] = 0.5f * (fx.y - 0.5f) * (fx.y - 0.5f);
+    wz[0] = 0.5f * (1.5f - fx.z) * (1.5f - fx.z);
+    wz[1] = 0.75f - (fx.z - 1.0f) * (fx.z - 1.0f);
+    wz[2] = 0.5f * (fx.z - 0.5f) * (fx.z - 0.5f);
+    v3 nv = V(0, 0, 0), C0 = V(0, 0, 0), C1 = V(0, 0, 0), C2 = V(0, 0, 0);
+    float supported = 0.0f, local_mass = 0.0f, bed_overlap = 0.0f, bed_perm_sum = 0.0f;
+    v3 bed_vsum = V(0, 0, 0);
+    float B = 4.0f * u->inv_dx * u->inv_dx;
+    float cdx = u->dx;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          int32_t cx = bx + i, cy = by + j, cz = bz + k;
+          if (!on_grid(u, cx, cy, cz)) continue;
+          float w = wx[i] * wy[j] * wz[k];
+          size_t ci = cidx(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+          const float* g = &S->grid_vel[ci * 4];
+          v3 gv = V(g[0], g[1], g[2]);
+          v3 dpos = vmul(vsub(V((float)i, (float)j, (float)k), fx), cdx);
+          float gm = g[3];
+          if (gm > 1e-6f) {
+            nv = vadd(nv, vmul(gv, w));
+            float wB = w * B;
+            C0 = vadd(C0, vmul(vmul(gv, wB), dpos.x));
+            C1 = vadd(C1, vmul(vmul(gv, wB), dpos.y));
+            C2 = vadd(C2, vmul(vmul(gv, wB), dpos.z));
+            supported += w;
+            local_mass += w * gm;
+          }
+          int32_t id = S->bed_lookup[ci];
+          if (S->num_bed > 0u && bed_valid(S, id)) {
+            bed_overlap += w;
+            bed_vsum = vadd(bed_vsum, vmul(V(S->bed[id].velocity[0], S->bed[id].velocity[1], S->bed[id].velocity[2]), w));
+            bed_perm_sum += w * bed_compacted_permeability(S, u, id);
+          }
+        }
+    float support_ratio = o_clamp(supported, 0.0f, 1.0f);
+    float j_support = support_ratio;
+    if (supported > 1e-6f) {
+      float inv = 1.0f / supported;
+      nv = vmul(nv, inv); C0 = vmul(C0, inv); C1 = vmul(C1, inv); C2 = vmul(C2, inv);
+    }
+    int in_cup = 0;
+    if (u->geo.cup_enabled) {
+      float lim = u->geo.cup_radius + u->contact_offset;
+      in_cup = xp.y < u->geo.cup_top_y && (xp.x * xp.x + xp.z * xp.z) < lim * lim;
+    }
+    float dense_ratio = o_clamp(local_mass / o_max(u->nominal_mass * 4.0f, 1e-6f), 0.0f, 1.0f);
+    float porous = o_clamp(bed_overlap, 0.0f, 1.0f);
+    v3 pbv = V(0, 0, 0);
+    float pk = u->min_k;
+    if (bed_overlap > 1e-6f) {
+      float ib = 1.0f / bed_overlap;
+      pbv = vmul(bed_vsum, ib);
+      pk = o_max(bed_perm_sum * ib, u->min_k);
+    }
+    if (support_ratio < 0.999f) {
+      v3 ball = V(p[4], p[5] + u->gravity * u->dt, p[6]);
+      float fgain = in_cup ? 1.15f : 1.35f;
+      float fcap = in_cup ? 0.95f : 0.995f;
+      float gain = o_mix(fgain, 0.18f, porous);
+      float cap = o_mix(fcap, 0.16f, porous);
+      float damp = o_mix(1.0f, 0.12f, dense_ratio);
+      float pres = o_clamp((1.0f - support_ratio) * gain * damp, 0.0f, cap);
+      nv = V(o_mix(nv.x, ball.x, pres), o_mix(nv.y, ball.y, pres), o_mix(nv.z, ball.z, pres));
+      float ad = 1.0f - pres * 0.75f;
+      C0 = vmul(C0, ad); C1 = vmul(C1, ad); C2 = vmul(C2, ad);
+    }
+    int airborne = porous <= 0.05f && sample_sdf(S, u, xp) > u->contact_offset * 2.0f;
+    if (airborne) {
+      float dense_mass = u->nominal_mass * 4.0f;
+      float dr = o_clamp(local_mass / o_max(dense_mass, 1e-6f), 0.0f, 1.0f);
+      j_support = o_min(j_support, dr);
+      v3 ball = V(p[4], p[5] + u->gravity * u->dt, p[6]);
+      float gain = in_cup ? 0.72f : 0.98f;
+      float cap = in_cup ? 0.88f : 0.98f;
+      float pres = o_clamp((1.0f - dr) * gain, 0.0f, cap);
+      nv = V(o_mix(nv.x, ball.x, pres), o_mix(nv.y, ball.y, pres), o_mix(nv.z, ball.z, pres));
+      float ad = 1.0f - pres * 0.65f;
+      C0 = vmul(C0, ad); C1 = vmul(C1, ad); C2 = vmul(C2, ad);
+    }
+    if (porous > 1e-4f) {
+      v3 rel = vsub(nv, pbv);
+      float rate = u->water_nu / pk;
+      float dd = 1.0f / (1.0f + o_max(rate * u->dt * porous, 0.0f));
+      float rs = vlen(rel);
+      float idd = 1.0f / (1.0f + o_max(rs * u->dt * u->inv_dx * porous * 0.35f, 0.0f));
+      nv = vadd(pbv, vmul(vmul(rel, dd), idd));
+      /* bed reaction impulse (shader.rs:1839-1843) feeds only bed particles */
+      float ad = o_mix(1.0f, dd * idd, porous);
+      C0 = vmul(C0, ad); C1 = vmul(C1, ad); C2 = vmul(C2, ad);
+      j_support = o_max(j_support, porous);
+    }
+    float tr = C0.x + C1.y + C2.z;
+    float Jc = o_clamp(J_old > 0.0f ? J_old : 1.0f, 0.40f, 2.00f);
+    float Ja = o_clamp(Jc * orc_expf(o_clamp(u->dt * tr, -0.35f, 0.35f)), 0.40f, 2.00f);
+    float Jb = o_smooth(0.35f, 0.85f, o_clamp(j_support, 0.0f, 1.0f));
+    float Jn = o_clamp(o_mix(Jc, Ja, Jb), 0.40f, 2.00f);
+    v3 np_ = vadd(xp, vmul(nv, u->dt));
+    v3 mid = V(o_mix(xp.x, np_.x, 0.5f), o_mix(xp.y, np_.y, 0.5f), o_mix(xp.z, np_.z, 0.5f));
+    contact_t c = sdf_contact(S, u, mid, nv);
+    nv = c.vel;
+    c = sdf_contact(S, u, np_, nv);
+    np_ = c.pos; nv = c.vel;
+    float margin = u->dx * 0.5f;
+    np_.x = o_clamp(np_.x, u->origin[0] + margin, u->bmax[0] - margin);
+    np_.y = o_clamp(np_.y, u->origin[1] + margin, u->bmax[1] - margin);
+    np_.z = o_clamp(np_.z, u->origin[2] + margin, u->bmax[2] - margin);
+    p[0] = np_.x; p[1] = np_.y; p[2] = np_.z; p[3] = Jn;
+    p[4] = nv.x; p[5] = nv.y; p[6] = nv.z; p[7] = mass_p;
+    a[0] = C0.x; a[1] = C0.y; a[2] = C0.z; a[3] = phase;
+    a[4] = C1.x; a[5] = C1.y; a[6] = C1.z; a[7] = 0.0f;
+    a[8] = C2.x; a[9] = C2.y; a[10] = C2.z; a[11] = 0.0f;
+  }
+}
+
+/* ---- one substep in step_frame order (mod.rs:271-418) ---------------------- */
+static int substep(orc_sim* S, float sub_dt, int stop_after, int emit) {
+  if (emit) orc_emit(S, sub_dt);
+  orc_write_uniforms(S, sub_dt, &S->u);
+  orc_params* u = &S->u;
+  memset(S->grid, 0, S->nc * 6 * sizeof(int32_t));
+  memset(S->grid_vel, 0, S->nc * 4 * sizeof(float));
+  memset(S->metrics, 0, sizeof(S->metrics));
+  k_p2g(S, u);
+  memcpy(S->p2g_lanes, S->grid, S->nc * 6 * sizeof(int32_t));
+  if (stop_after == ORC_PASS_P2G) return 0;
+  k_grid_update(S, u);
+  k_boundary(S, u);
+  if (stop_after == ORC_PASS_GRID) return 0;
+  k_classify(S, u);
+  if (stop_after == ORC_PASS_CLASSIFY) return 0;
+  solve_pressure(S, u);
+  if (stop_after == ORC_PASS_RBGS) return 0;
+  k_project(S, u);
+  k_boundary(S, u);
+  if (stop_after == ORC_PASS_PROJECT) return 0;
+  k_packing_prepare(S, u);
+  k_packing_apply(S, u);
+  k_boundary(S, u);
+  if (stop_after == ORC_PASS_PACKING) return 0;
+  k_viscosity_prepare(S, u);
+  k_viscosity_apply(S, u);
+  k_boundary(S, u);
+  if (stop_after == ORC_PASS_VISCOSITY) return 0;
+  k_g2p(S, u);
+  S->total_time += sub_dt;
+  return 0;
+}
+
+int orc_debug_step_until(orc_sim* S, float sub_dt, int stop_after) {
+  return substep(S, sub_dt, stop_after, 0);
+}
+
+/* MpmSim3D::step_frame (mod.rs:264-430) */
+int orc_step_frame(orc_sim* S, float dt) {
+  dt = o_min(dt, 1.0f / 30.0f);
+  uint32_t n = S->s.substeps > 1u ? S->s.substeps : 1u;
+  float sub_dt = dt / (float)n;
+  for (uint32_t i = 0; i < n; ++i) substep(S, sub_dt, 0, 1);
+  return CS_OK;
+}
