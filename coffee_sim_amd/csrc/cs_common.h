@@ -112,8 +112,12 @@ CS_HD cs_f3 mix3(cs_f3 a, cs_f3 b, float t) {
 // Per-substep parameter block: the reference's 256 B MpmUniforms
 // (state.rs:31-56, values from write_uniforms mod.rs:569-670) plus the
 // geometry and bed extensions.
+// Side of the cubic bricks the pressure solve is tiled into.
+#define CS_BRICK 8
+
 struct cs_params {
   uint32_t gx, gy, gz, nc;
+  uint32_t nbx, nby, nbz, nbricks;  // CS_BRICK^3 bricks covering the grid
   uint32_t num_particles, num_bed_cells, max_particles, use_sdf_cache;
   float dt, gravity, dx, inv_dx;
   float origin[3];

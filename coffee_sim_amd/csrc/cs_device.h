@@ -1,0 +1,200 @@
+// cs_device.h — device helpers shared by the kernel translation units:
+// grid indexing, SDF sampling and contact (shader.rs:148-150, 673-903), the
+// packed kind/fill word and the volume target of the pressure right-hand side.
+#pragma once
+
+#include "cs_common.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t cell_index(const cs_params& u, uint32_t ix, uint32_t iy, uint32_t iz) {
+  return iz * u.gx * u.gy + iy * u.gx + ix;  // shader.rs:148-150
+}
+__device__ __forceinline__ bool on_grid(const cs_params& u, int x, int y, int z) {
+  return !(x < 0 || y < 0 || z < 0) && !((uint32_t)x >= u.gx || (uint32_t)y >= u.gy || (uint32_t)z >= u.gz);
+}
+__device__ __forceinline__ void decompose(const cs_params& u, uint32_t idx, uint32_t& ix, uint32_t& iy, uint32_t& iz) {
+  uint32_t plane = u.gx * u.gy;
+  iz = idx / plane;
+  uint32_t rem = idx % plane;
+  iy = rem / u.gx;
+  ix = rem % u.gx;
+}
+__device__ __forceinline__ cs_f3 cell_center(const cs_params& u, uint32_t ix, uint32_t iy, uint32_t iz) {
+  return f3(u.origin[0] + ((float)ix + 0.5f) * u.dx, u.origin[1] + ((float)iy + 0.5f) * u.dx,
+            u.origin[2] + ((float)iz + 0.5f) * u.dx);
+}
+
+// ---- SDF (shader.rs:673-715) ------------------------------------------------
+__device__ __forceinline__ float sdf_texel(const cs_params& u, const float* __restrict__ sdf, int x, int y, int z) {
+  int mx = (int)u.sdf_res - 1;
+  x = x < 0 ? 0 : (x > mx ? mx : x);
+  y = y < 0 ? 0 : (y > mx ? mx : y);
+  z = z < 0 ? 0 : (z > mx ? mx : z);
+  return sdf[((size_t)z * u.sdf_res + (size_t)y) * u.sdf_res + (size_t)x];
+}
+__device__ float sample_sdf(const cs_params& u, const float* __restrict__ sdf, cs_f3 p) {
+  float res = u.sdf_res_f;
+  float ux = (p.x + u.bmax[0]) / (u.bmax[0] * 2.0f) * res - 0.5f;
+  float uy = (p.y + u.bmax[1]) / (u.bmax[1] * 2.0f) * res - 0.5f;
+  float uz = (p.z + u.bmax[2]) / (u.bmax[2] * 2.0f) * res - 0.5f;
+  float flx = floorf(ux), fly = floorf(uy), flz = floorf(uz);
+  int bx = cs_f2i(flx), by = cs_f2i(fly), bz = cs_f2i(flz);
+  float fx = ux - flx, fy = uy - fly, fz = uz - flz;
+  float c000 = sdf_texel(u, sdf, bx, by, bz);
+  float c100 = sdf_texel(u, sdf, bx + 1, by, bz);
+  float c010 = sdf_texel(u, sdf, bx, by + 1, bz);
+  float c110 = sdf_texel(u, sdf, bx + 1, by + 1, bz);
+  float c001 = sdf_texel(u, sdf, bx, by, bz + 1);
+  float c101 = sdf_texel(u, sdf, bx + 1, by, bz + 1);
+  float c011 = sdf_texel(u, sdf, bx, by + 1, bz + 1);
+  float c111 = sdf_texel(u, sdf, bx + 1, by + 1, bz + 1);
+  float c00 = cs_mix(c000, c100, fx);
+  float c10 = cs_mix(c010, c110, fx);
+  float c01 = cs_mix(c001, c101, fx);
+  float c11 = cs_mix(c011, c111, fx);
+  float c0 = cs_mix(c00, c10, fy);
+  float c1 = cs_mix(c01, c11, fy);
+  return cs_mix(c0, c1, fz);
+}
+__device__ cs_f3 sdf_gradient(const cs_params& u, const float* __restrict__ sdf, cs_f3 p) {
+  float e = u.dx;
+  float gxv = sample_sdf(u, sdf, f3(p.x + e, p.y + 0.0f, p.z + 0.0f)) - sample_sdf(u, sdf, f3(p.x - e, p.y - 0.0f, p.z - 0.0f));
+  float gyv = sample_sdf(u, sdf, f3(p.x + 0.0f, p.y + e, p.z + 0.0f)) - sample_sdf(u, sdf, f3(p.x - 0.0f, p.y - e, p.z - 0.0f));
+  float gzv = sample_sdf(u, sdf, f3(p.x + 0.0f, p.y + 0.0f, p.z + e)) - sample_sdf(u, sdf, f3(p.x - 0.0f, p.y - 0.0f, p.z - e));
+  cs_f3 g = f3(gxv, gyv, gzv);
+  float len = len3(g);
+  if (len < 1e-8f) return f3(0.0f, 0.0f, 0.0f);
+  return g / len;
+}
+
+// boundary_project body (shader.rs:1638-1678) for a cell with mass >= 1e-6.
+__device__ cs_f3 boundary_apply(const cs_params& u, const float* __restrict__ sdf, cs_f3 v, cs_f3 cp) {
+  float sv = sample_sdf(u, sdf, cp);
+  if (sv < u.contact_offset) {
+    cs_f3 n = sdf_gradient(u, sdf, cp);
+    float vn = dot3(v, n);
+    if (vn < 0.0f) {
+      v = v - (n * vn) * (1.0f + u.restitution);
+      cs_f3 vt = v - n * dot3(v, n);
+      float vtl = len3(vt);
+      if (vtl > 1e-6f) {
+        float fi = cs_min(u.friction * fabsf(vn), vtl);
+        v = v - vt * (fi / vtl);
+      }
+    }
+  }
+  float margin = 2.0f * u.dx;
+  float bminx = u.origin[0] + margin, bminy = u.origin[1] + margin, bminz = u.origin[2] + margin;
+  float bmaxx = u.bmax[0] - margin, bmaxy = u.bmax[1] - margin, bmaxz = u.bmax[2] - margin;
+  if (cp.x < bminx && v.x < 0.0f) v.x = 0.0f;
+  if (cp.x > bmaxx && v.x > 0.0f) v.x = 0.0f;
+  if (cp.y < bminy && v.y < 0.0f) v.y = 0.0f;
+  if (cp.y > bmaxy && v.y > 0.0f) v.y = 0.0f;
+  if (cp.z < bminz && v.z < 0.0f) v.z = 0.0f;
+  if (cp.z > bmaxz && v.z > 0.0f) v.z = 0.0f;
+  return v;
+}
+
+__device__ __forceinline__ cs_f3 cap_speed(const cs_params& u, cs_f3 v) {
+  float speed = len3(v);
+  if (speed > u.vel_cap) v = v * (u.vel_cap / speed);
+  return v;
+}
+
+// ---- contact for particles (shader.rs:717-903, water only) -------------------
+struct contact {
+  cs_f3 pos, vel;
+};
+__device__ contact radial_barrier(const cs_params& u, cs_f3 pos, cs_f3 vel, float max_r) {
+  float rx = pos.x - 0.0f, rz = pos.z - 0.0f;
+  float r = sqrtf(rx * rx + rz * rz);
+  if (r > max_r && r > 1e-6f) {
+    float ox = rx / r, oz = rz / r;
+    pos.x = 0.0f + ox * max_r;
+    pos.z = 0.0f + oz * max_r;
+    float vn = vel.x * ox + vel.z * oz;
+    if (vn > 0.0f) {
+      float tx = vel.x - ox * vn, tz = vel.z - oz * vn;
+      vel.x = tx * (1.0f - u.friction * 0.35f);
+      vel.z = tz * (1.0f - u.friction * 0.35f);
+    }
+  }
+  return contact{pos, vel};
+}
+__device__ contact scene_obstacles(const cs_params& u, cs_f3 pos, cs_f3 vel) {
+  const cs_geometry& g = u.geo;
+  if (g.cone_enabled && pos.y <= g.cone_top_y && pos.y >= g.cone_bot_y) {
+    float t = cs_clamp((pos.y - g.cone_bot_y) / (g.cone_top_y - g.cone_bot_y), 0.0f, 1.0f);
+    float cone_r = cs_mix(g.cone_bot_radius, g.cone_top_radius, t) - u.contact_offset;
+    contact c = radial_barrier(u, pos, vel, cone_r);
+    pos = c.pos;
+    vel = c.vel;
+  }
+  if (g.cup_enabled && pos.y <= g.cup_top_y) {
+    float cup_r = g.cup_radius - u.wall_half - u.contact_offset;
+    contact c = radial_barrier(u, pos, vel, cup_r);
+    pos = c.pos;
+    vel = c.vel;
+    float floor_y = g.cup_bottom_y + u.wall_half + u.contact_offset;
+    if (pos.y < floor_y) {
+      pos.y = floor_y;
+      if (vel.y < 0.0f) {
+        vel.y = 0.0f;
+        vel.x *= 1.0f - u.friction * 0.55f;
+        vel.z *= 1.0f - u.friction * 0.55f;
+      }
+    }
+  }
+  return contact{pos, vel};
+}
+__device__ contact sdf_contact(const cs_params& u, const float* __restrict__ sdf, cs_f3 pos, cs_f3 vel) {
+  float sv = sample_sdf(u, sdf, pos);
+  if (sv < u.contact_offset) {
+    cs_f3 n = sdf_gradient(u, sdf, pos);
+    if (len3(n) > 1e-6f) {
+      pos = pos + n * (u.contact_offset - sv);
+      float vn = dot3(vel, n);
+      if (vn < 0.0f) {
+        vel = vel - (n * vn) * (1.0f + u.restitution);
+        cs_f3 vt = vel - n * dot3(vel, n);
+        float vtl = len3(vt);
+        if (vtl > 1e-6f) {
+          float fi = cs_min(u.friction * fabsf(vn), vtl);
+          vel = vel - vt * (fi / vtl);
+        }
+      }
+    }
+  }
+  return scene_obstacles(u, pos, vel);
+}
+
+// ---- kind/fill word ------------------------------------------------------------
+__device__ __forceinline__ bool kf_fluid(float kf) { return kf >= 0.0f; }
+__device__ __forceinline__ bool kf_solid(float kf) { return kf == CS_KF_SOLID; }
+__device__ __forceinline__ bool kf_bed(float kf) { return kf == CS_KF_BED; }
+__device__ __forceinline__ float kf_fill(float kf) { return kf == CS_KF_BED ? 1.0f : kf; }
+// pressure_face_weight (shader.rs:306-323) for a non-solid neighbour.
+__device__ __forceinline__ float face_w(float self_kf, float self_fill, float nkf) {
+  if (kf_bed(self_kf) || kf_bed(nkf)) return 1.0f;
+  if (!kf_fluid(nkf)) return self_fill;
+  return cs_min(self_fill, kf_fill(nkf));
+}
+
+__constant__ static int c_off6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+
+// volume_projection_target_divergence (shader.rs:597-619)
+__device__ float volume_target_div(const cs_params& u, float rest, float cur) {
+  float cv = u.dx * u.dx * u.dx;
+  float eps = cs_max(cv * 1e-5f, u.p_vol * 1e-3f);
+  if (rest <= eps) return 0.0f;
+  float j = cur / rest;
+  float comp = cs_clamp(1.0f - j, 0.0f, 0.75f);
+  float expd = cs_clamp(j - 1.0f, 0.0f, 0.75f);
+  float target = comp * u.j_alpha - expd * u.j_expand_alpha;
+  float packed = cs_max(rest, cur) / cs_max(cv, 1e-8f);
+  float over = cs_max(packed - u.max_rest_fraction, 0.0f);
+  return cs_max(target, over * u.j_alpha);
+}
+
+}  // namespace

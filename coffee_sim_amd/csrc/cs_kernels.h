@@ -18,7 +18,7 @@ hipError_t cs_launch_grid(hipStream_t stream, const cs_params& u, const int32_t*
 hipError_t cs_launch_classify(hipStream_t stream, const cs_params& u, const float* gvel, const uint8_t* cls,
                               const int32_t* lanes, const int32_t* bed_id, const cs_bed_cell* bed, int8_t* kind,
                               float* kf, int32_t* pres, float* rhs, int32_t* pack, uint32_t* metrics,
-                              const float* sdf);
+                              const float* sdf, uint32_t* brick_flag);
 hipError_t cs_launch_warm_previous(hipStream_t stream, const cs_params& u, const float* kf, const int32_t* prev,
                                    int32_t* pres);
 hipError_t cs_launch_warm_external(hipStream_t stream, const cs_params& u, const float* kf, const float* warm,

@@ -392,6 +392,10 @@ void make_params(const cs_settings& s, uint32_t num_particles, uint32_t num_bed_
   u->gy = s.grid_dims[1];
   u->gz = s.grid_dims[2];
   u->nc = u->gx * u->gy * u->gz;
+  u->nbx = (u->gx + CS_BRICK - 1) / CS_BRICK;
+  u->nby = (u->gy + CS_BRICK - 1) / CS_BRICK;
+  u->nbz = (u->gz + CS_BRICK - 1) / CS_BRICK;
+  u->nbricks = u->nbx * u->nby * u->nbz;
   u->num_particles = num_particles;
   u->num_bed_cells = num_bed_cells;
   u->max_particles = s.max_particles;

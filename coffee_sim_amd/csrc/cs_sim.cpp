@@ -14,6 +14,7 @@
 
 #include "cs_host.h"
 #include "cs_kernels.h"
+#include "cs_solver.h"
 
 namespace {
 
@@ -76,6 +77,19 @@ struct cs_sim {
   uint32_t* metrics = nullptr;
   double* res_partials = nullptr;
   double* res_out = nullptr;
+  // brick-sparse solve
+  int32_t* pres2 = nullptr;          // second pressure buffer
+  uint32_t* brick_flag = nullptr;    // [nbricks]
+  uint32_t* brick_list = nullptr;    // [nbricks]
+  double* brick_partials = nullptr;  // [2 * nbricks]
+  cs_solve_state* dstate = nullptr;
+  cs_solve_state* h_state = nullptr; // pinned
+  uint32_t* h_done = nullptr;        // pinned, one slot per in-flight batch
+  hipEvent_t batch_ev[2] = {};
+  std::vector<hipEvent_t> chunk_ev;  // start/end pairs for sweep timing
+  bool dense_solver = false;         // one-thread-per-cell reference layout (CS_SOLVER=dense)
+  int halo = 4;                      // half sweeps per launch (CS_RBGS_HALO = 1, 2 or 4)
+  uint32_t batch_chunks = 4;         // residual checks enqueued ahead of the host
   // host state
   double* h_res = nullptr;    // pinned
   float* h_stage = nullptr;   // pinned emission staging
@@ -90,6 +104,7 @@ struct cs_sim {
   uint32_t total_dropped = 0;
   cs_solve_stats stats;
   bool final_residual_pending = false;
+  bool pending_from_state = false;  // lazily read residual lives in h_state (brick solve)
   bool timing = false;
   hipEvent_t ev[P_COUNT] = {};
   float pass_us[7] = {};
@@ -103,12 +118,19 @@ namespace {
 
 void free_sim(cs_sim* m) {
   if (!m) return;
-  void* dev[] = {m->part, m->aff, m->lanes, m->gv,   m->gv2,  m->sdf,     m->cls,          m->bed_id,
-                 m->bed,  m->kind, m->kf,  m->pres, m->rhs,  m->pack,    m->prev_p,       m->warm,
-                 m->metrics, m->res_partials, m->res_out};
+  void* dev[] = {m->part,       m->aff,        m->lanes,      m->gv,         m->gv2,    m->sdf,
+                 m->cls,        m->bed_id,     m->bed,        m->kind,       m->kf,     m->pres,
+                 m->rhs,        m->pack,       m->prev_p,     m->warm,       m->metrics, m->res_partials,
+                 m->res_out,    m->pres2,      m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
+  if (m->h_state) (void)hipHostFree(m->h_state);
+  if (m->h_done) (void)hipHostFree(m->h_done);
+  for (auto& e : m->batch_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : m->chunk_ev)
+    if (e) (void)hipEventDestroy(e);
   if (m->h_stage) (void)hipHostFree(m->h_stage);
   if (m->stage_done) (void)hipEventDestroy(m->stage_done);
   for (auto& e : m->ev)
@@ -153,7 +175,9 @@ int residual_now(cs_sim* m, const cs_params& u, double* out) {
   return CS_OK;
 }
 
-int solve(cs_sim* m, const cs_params& u) {
+// One-thread-per-cell solve over the whole grid (the reference's dispatch
+// shape), kept as a cross-check of the brick solve.
+int solve_dense(cs_sim* m, const cs_params& u) {
   hipStream_t st = m->ctx->stream;
   cs_solve_stats& S = m->stats;
   memset(&S, 0, sizeof(S));
@@ -220,6 +244,164 @@ int solve(cs_sim* m, const cs_params& u) {
   return CS_OK;
 }
 
+int32_t* buf_of(cs_sim* m, int cur) { return cur ? m->pres2 : m->pres; }
+
+// Enqueues `iters` red+black iterations as brick launches of `halo` half
+// sweeps (single-iteration launches for any odd remainder); returns the number
+// of launches.  Buffers alternate per launch starting from `*cur`.
+int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t iters, uint32_t* launches, int* cur) {
+  hipStream_t st = m->ctx->stream;
+  int32_t* buf[2] = {m->pres, m->pres2};
+  uint32_t n = 0;
+  uint32_t left = iters;
+  const uint32_t per = (uint32_t)m->halo / 2u;  // iterations per launch (halo 1 -> half iterations)
+  while (left > 0) {
+    if (m->halo >= 2 && left >= per) {
+      CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, m->halo, 0u, m->brick_list, m->kf, m->rhs, buf[*cur],
+                                   buf[*cur ^ 1], m->metrics, m->dstate));
+      *cur ^= 1;
+      ++n;
+      left -= per;
+    } else {
+      int h = m->halo >= 2 ? 2 : 1;
+      for (uint32_t c = 0; c < (uint32_t)(2 / h); ++c) {
+        CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, h, h == 1 ? c : 0u, m->brick_list, m->kf, m->rhs, buf[*cur],
+                                     buf[*cur ^ 1], m->metrics, m->dstate));
+        *cur ^= 1;
+        ++n;
+      }
+      left -= 1;
+    }
+  }
+  *launches = n;
+  return CS_OK;
+}
+
+// Brick-sparse solve (cs_solver.hip) with the convergence test on the device.
+int solve_bricks(cs_sim* m, const cs_params& u) {
+  hipStream_t st = m->ctx->stream;
+  cs_solve_stats& S = m->stats;
+  memset(&S, 0, sizeof(S));
+  S.final_residual = NAN;
+  m->final_residual_pending = false;
+  CS_HIP(cs_launch_brick_scan(st, u, m->brick_flag, m->brick_list, m->dstate));
+  bool external = false;
+  if (m->s.warm_start == CS_WARM_PREVIOUS) {
+    CS_HIP(cs_launch_warm_previous(st, u, m->kf, m->prev_p, m->pres));
+  } else if (m->s.warm_start == CS_WARM_EXTERNAL && m->has_warm) {
+    CS_HIP(cs_launch_warm_external(st, u, m->kf, m->warm, m->pres, m->metrics));
+    external = true;
+  }
+  const double tol = (double)m->s.rbgs_tolerance;
+  const bool fixed = m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS;
+  // the active brick count sizes the launches
+  CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
+  CS_HIP(hipStreamSynchronize(st));
+  const uint32_t nactive = m->h_state->active_bricks;
+  int cur = 0;
+  if (fixed) {
+    uint32_t launches = 0;
+    size_t ev0 = 0;
+    if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[ev0], st));
+    int rc = enqueue_sweeps(m, u, nactive, m->s.pressure_rbgs_pairs, &launches, &cur);
+    if (rc) return rc;
+    if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[ev0 + 1], st));
+    // the final residual for the statistics (mode 0 also records launches = 0)
+    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->kf, m->rhs, buf_of(m, cur), m->brick_partials,
+                                     m->dstate, 0u, 0u, 0u, tol));
+    S.iterations = m->s.pressure_rbgs_pairs;
+    if (cur != 0) {
+      CS_HIP(hipMemcpyAsync(m->pres, m->pres2, m->nc * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    }
+    CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
+    m->final_residual_pending = true;
+    m->pending_from_state = true;
+    if (m->timing) {
+      CS_HIP(hipEventSynchronize(m->chunk_ev[ev0 + 1]));
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, m->chunk_ev[ev0], m->chunk_ev[ev0 + 1]);
+      m->kt.rbgs_us += (double)ms * 1000.0;
+      m->kt.rbgs_launches += launches;
+      m->kt.rbgs_cells_swept += (uint64_t)launches * nactive * CS_BRICK * CS_BRICK * CS_BRICK;
+    }
+  } else {
+    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->kf, m->rhs, m->pres, m->brick_partials,
+                                     m->dstate, 0u, 0u, 0u, tol));
+    if (external) {
+      CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
+      CS_HIP(hipStreamSynchronize(st));
+      if (m->h_state->initial_residual > 1.0) {
+        // the supplied start is worse than the zero start: fall back
+        CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
+        S.fell_back = 1;
+        CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->kf, m->rhs, m->pres, m->brick_partials,
+                                         m->dstate, 0u, 0u, 0u, tol));
+      }
+    }
+    const uint32_t every = m->s.rbgs_check_every ? m->s.rbgs_check_every : 8u;
+    const uint32_t cap = m->s.rbgs_max_iters;
+    const size_t pool = m->chunk_ev.size() / 2;  // event pairs, >= 3 batches of chunks
+    uint32_t enq = 0, batch = 0, chunk_id = 0;
+    std::vector<uint32_t> batch_first;  // first chunk id of each batch
+    uint32_t harvested = 0;             // chunks whose sweep time has been added
+    auto harvest = [&](uint32_t upto) {
+      for (; harvested < upto; ++harvested) {
+        size_t e = (size_t)(harvested % pool) * 2;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, m->chunk_ev[e], m->chunk_ev[e + 1]);
+        m->kt.rbgs_us += (double)ms * 1000.0;
+      }
+    };
+    bool stop = false;
+    while (enq < cap && !stop) {
+      batch_first.push_back(chunk_id);
+      for (uint32_t c = 0; c < m->batch_chunks && enq < cap; ++c) {
+        uint32_t chunk = std::min(every, cap - enq);
+        uint32_t launches = 0;
+        size_t e = (size_t)(chunk_id % pool) * 2;
+        if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e], st));
+        int rc = enqueue_sweeps(m, u, nactive, chunk, &launches, &cur);
+        if (rc) return rc;
+        if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e + 1], st));
+        ++chunk_id;
+        CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->kf, m->rhs, buf_of(m, cur),
+                                         m->brick_partials, m->dstate, 1u, chunk, launches, tol));
+        enq += chunk;
+      }
+      CS_HIP(hipMemcpyAsync(&m->h_done[batch & 1u], &m->dstate->done, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      CS_HIP(hipEventRecord(m->batch_ev[batch & 1u], st));
+      if (batch > 0) {
+        CS_HIP(hipEventSynchronize(m->batch_ev[(batch - 1) & 1u]));
+        if (m->timing) harvest(batch_first[batch]);
+        if (m->h_done[(batch - 1) & 1u]) stop = true;
+      }
+      ++batch;
+    }
+    CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+    if (m->timing) {
+      // chunks after the converged one returned at once; count only executed ones
+      uint32_t executed = (m->h_state->iterations + every - 1) / every;
+      if (executed > harvested) harvest(std::min(executed, chunk_id));
+    }
+    const cs_solve_state& hs = *m->h_state;
+    S.iterations = hs.iterations;
+    S.initial_residual = hs.initial_residual;
+    S.final_residual = hs.residual;
+    S.converged = hs.done;
+    // the solved pressure is in pres2 when an odd number of launches ran
+    CS_HIP(cs_launch_select_final(st, u, nactive, m->brick_list, m->pres, m->pres2, m->dstate));
+    if (m->timing) {
+      m->kt.rbgs_launches += hs.launches;
+      m->kt.rbgs_cells_swept += (uint64_t)hs.launches * nactive * CS_BRICK * CS_BRICK * CS_BRICK;
+    }
+  }
+  if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_copy_i32(st, m->pres, m->prev_p, m->nc));
+  return CS_OK;
+}
+
+int solve(cs_sim* m, const cs_params& u) { return m->dense_solver ? solve_dense(m, u) : solve_bricks(m, u); }
+
 int substep(cs_sim* m, float sub_dt) {
   hipStream_t st = m->ctx->stream;
   // 1. emission on the host (inflow.rs:122-241), appended after the live particles
@@ -255,7 +437,7 @@ int substep(cs_sim* m, float sub_dt) {
   CS_HIP(cs_launch_grid(st, u, m->lanes, m->gv, m->sdf));
   mark(m, P_CLASSIFY);
   CS_HIP(cs_launch_classify(st, u, m->gv, m->cls, m->lanes, m->bed_id, m->bed, m->kind, m->kf, m->pres, m->rhs,
-                            m->pack, m->metrics, m->sdf));
+                            m->pack, m->metrics, m->sdf, m->brick_flag));
   mark(m, P_SOLVE);
   int rc = solve(m, u);
   if (rc) return rc;
@@ -431,12 +613,36 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   ALLOC(m->metrics, CS_METRICS_SLOTS);
   ALLOC(m->res_partials, 2 * CS_RESIDUAL_BLOCKS);
   ALLOC(m->res_out, 1);
+  cs_params geom;
+  cs::make_params(*s, 0, 0, 0.0f, &geom);
+  ALLOC(m->pres2, m->nc);
+  ALLOC(m->brick_flag, geom.nbricks);
+  ALLOC(m->brick_list, geom.nbricks);
+  ALLOC(m->brick_partials, 2 * (size_t)geom.nbricks);
+  ALLOC(m->dstate, 1);
 #undef ALLOC
+  // solver variant (benchmarking and cross-checks; both give identical results)
+  if (const char* v = getenv("CS_SOLVER")) m->dense_solver = strcmp(v, "dense") == 0;
+  if (const char* v = getenv("CS_RBGS_HALO")) {
+    int h = atoi(v);
+    if (h == 1 || h == 2 || h == 4) m->halo = h;
+  }
+  if (const char* v = getenv("CS_RBGS_BATCH")) {
+    int b = atoi(v);
+    if (b >= 1 && b <= 64) m->batch_chunks = (uint32_t)b;
+  }
+  m->chunk_ev.assign((size_t)m->batch_chunks * 3 * 2, nullptr);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_res), sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_state), sizeof(cs_solve_state));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_done), 2 * sizeof(uint32_t));
   if (e == hipSuccess) e = hipEventCreateWithFlags(&m->stage_done, hipEventDisableTiming);
   for (auto& ev : m->ev)
     if (e == hipSuccess) e = hipEventCreate(&ev);
   for (auto& ev : m->kev)
+    if (e == hipSuccess) e = hipEventCreate(&ev);
+  for (auto& ev : m->batch_ev)
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  for (auto& ev : m->chunk_ev)
     if (e == hipSuccess) e = hipEventCreate(&ev);
   if (e != hipSuccess) {
     free_sim(m);
@@ -609,9 +815,10 @@ int cs_sim_get_solve_stats(cs_sim* m, cs_solve_stats* out) {
   if (!m || !out) return fail(CS_ERR_INVALID, "null argument");
   if (m->final_residual_pending) {
     CS_HIP(hipStreamSynchronize(m->ctx->stream));
-    m->stats.final_residual = m->h_res[0];
+    m->stats.final_residual = m->pending_from_state ? m->h_state->residual : m->h_res[0];
     m->stats.converged = m->stats.final_residual <= (double)m->s.rbgs_tolerance ? 1u : 0u;
     m->final_residual_pending = false;
+    m->pending_from_state = false;
   }
   uint32_t slots[CS_METRICS_SLOTS];
   int rc = cs_sim_read_metrics(m, slots);

@@ -58,6 +58,14 @@ float value_noise(uint64_t seed, float x, float y, float z, float wavelength) {
   return v;
 }
 
+// Substep of the working-size workload.  The pressure lanes hold i32 fixed
+// point clamped at +-8191 (state.rs:15-18, shader.rs:202-209) and the
+// projection needs p ~ v dx / dt to stop a flow of speed v within a step; at
+// dx = 1 mm that stays representable up to about the speed cap only for
+// dt >= ~5e-4 s, which also keeps the CFL number at the cap below one cell
+// (86.55 u/s x 5e-4 s = 0.75 dx).
+constexpr float kWorkingDt = 5.0e-4f;
+
 struct Owner {
   std::vector<float> sdf;
   std::vector<uint8_t> cls;
@@ -67,10 +75,26 @@ struct Owner {
   std::vector<float> affine;
 };
 
-// Places 8 jittered particles (one per 2x2x2 sub-cell) in cell (ix,iy,iz).
+// Places jittered particles in cell (ix,iy,iz), one per 2x2x2 sub-cell.  With
+// fill < 1 (water in a porous bed's pores) the cell gets floor(8*fill + u)
+// particles in a random subset of its sub-cells, u uniform in [0,1).
 void seed_cell(std::vector<float>& out, uint64_t seed, const cs_params& u, uint32_t ix, uint32_t iy, uint32_t iz,
-               const float vel[3], uint32_t limit) {
-  for (int s = 0; s < 8 && out.size() / 8 < limit; ++s) {
+               const float vel[3], uint32_t limit, float fill = 1.0f) {
+  uint64_t cell_key = ((uint64_t)iz * 65536u + iy) * 65536u + ix;
+  int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+  int count = 8;
+  if (fill < 1.0f) {
+    count = (int)floorf(8.0f * fill + u01(seed, 11, cell_key));
+    for (int i = 7; i > 0; --i) {  // Fisher-Yates
+      int j = (int)(u01(seed, 12 + (uint64_t)i, cell_key) * (float)(i + 1));
+      if (j > i) j = i;
+      int t = order[i];
+      order[i] = order[j];
+      order[j] = t;
+    }
+  }
+  for (int q = 0; q < count && out.size() / 8 < limit; ++q) {
+    const int s = order[q];
     uint64_t pid = out.size() / 8;
     float sub[3] = {(float)(s & 1), (float)((s >> 1) & 1), (float)((s >> 2) & 1)};
     uint32_t cell[3] = {ix, iy, iz};
@@ -143,12 +167,12 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
     s.rbgs_check_every = 8u;
     s.rbgs_max_iters = 20000u;
     // 2,000 particles per substep (scaled with the particle count) from a
-    // spout tilted 45 degrees, at 90% of the speed cap; the jet radius keeps
+    // spout tilted 45 degrees, at half the speed cap; the jet radius keeps
     // the poured column at 8 particles per cell: N = 8 pi R^2 (v dt / dx).
     s.emit_per_substep = std::max(1u, 2000u / s3);
     s.emit_total = n_pour;
-    s.emit_speed = 0.9f * cs::max_velocity();
-    const float dt = 1.0e-4f;
+    s.emit_speed = 0.5f * cs::max_velocity();
+    const float dt = kWorkingDt;
     float cells = sqrtf((float)s.emit_per_substep / (8.0f * 3.14159265f * (s.emit_speed * dt / dx)));
     s.emit_jet_radius = cells * dx;
     s.spout.origin[0] = -0.30f * side;
@@ -203,13 +227,16 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
   const float vel_rest[3] = {0.0f, 0.0f, 0.0f};
   own->particles.reserve((size_t)n_initial * 8);
   if (working) {
-    // pre-wetted slab filling the open cone from the apex upward
+    // pre-wetted slab filling the open cone from the apex upward: bed cells
+    // hold water only in their pores (porosity x 8 particles, the pore
+    // capacity of shader.rs:654-658), open cells above the bed 8 per cell
     for (uint32_t iy = apex_cells; iy < n && own->particles.size() / 8 < n_initial; ++iy)
       for (uint32_t iz = 0; iz < n && own->particles.size() / 8 < n_initial; ++iz)
         for (uint32_t ix = 0; ix < n && own->particles.size() / 8 < n_initial; ++ix) {
           size_t c = ((size_t)iz * n + iy) * n + ix;
           if (own->cls[c]) continue;
-          seed_cell(own->particles, seed, u, ix, iy, iz, vel_rest, n_initial);
+          float fill = own->lookup[c] >= 0 ? own->records[own->lookup[c]].porosity : 1.0f;
+          seed_cell(own->particles, seed, u, ix, iy, iz, vel_rest, n_initial, fill);
         }
   } else {
     // jittered cylinder above the bed centre; radius 4 cells as specified,
@@ -250,7 +277,7 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
   sc.particles = own->particles.data();
   sc.affine = own->affine.data();
   sc.num_particles = placed;
-  out->frame_dt = 1.0e-4f;
+  out->frame_dt = working ? kWorkingDt : 1.0e-4f;
   out->substeps_total = working ? 1000u : 200u;
   out->total_particles = placed + n_pour;
   out->_owner = own.release();

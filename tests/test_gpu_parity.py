@@ -63,6 +63,32 @@ def test_v60_small_one_substep_grid_tolerance_and_bitwise(ctx, orc):
     compare_state(sim, ref, "v60_small substep 1")
 
 
+@pytest.mark.parametrize("variant", [{"CS_SOLVER": "dense"}, {"CS_RBGS_HALO": "1"}, {"CS_RBGS_HALO": "2"},
+                                     {"CS_RBGS_HALO": "4", "CS_RBGS_BATCH": "1"}],
+                         ids=["dense", "brick-h1", "brick-h2", "brick-h4-batch1"])
+def test_solver_variants_match_oracle(ctx, orc, monkeypatch, variant):
+    """Every pressure-solver layout (one thread per cell over the whole grid;
+    brick-sparse with 1, 2 or 4 half sweeps per launch) gives the oracle's
+    pressure, iteration count and particles bit for bit."""
+    for k, v in variant.items():
+        monkeypatch.setenv(k, v)
+    w = cs.SeededWorkload("v60_working", seed=3, scale=4)
+    sim, ref = _pair(ctx, orc, w.settings, w.scene)
+    for k in range(2):
+        sim.step_frame(w.frame_dt)
+        ref.step_frame(w.frame_dt)
+        assert sim.solve_stats().iterations == ref.solve_stats().iterations
+        compare_state(sim, ref, f"{variant} substep {k}")
+    # fixed-pairs mode through the same layout
+    s = w.settings.copy()
+    s.rbgs_mode = 0
+    s.pressure_rbgs_pairs = 13
+    sim2, ref2 = _pair(ctx, orc, s, w.scene)
+    sim2.step_frame(w.frame_dt)
+    ref2.step_frame(w.frame_dt)
+    compare_state(sim2, ref2, f"{variant} fixed 13 pairs")
+
+
 def test_v60_working_scaled_tolerance_solve(ctx, orc):
     """Residual-driven RBGS (tolerance 1e-4, checked every 8 iterations) with the
     value-noise bed and the fixed-count pour, at 1/4 linear size."""
