@@ -9,11 +9,15 @@ substep.  Inputs are synthetic (seeded generators); nothing is read from disk.
 
 A "step" is one substep of the hot path (P2G -> grid update -> classify ->
 RBGS to tolerance -> projection -> packing -> viscosity -> G2P).  The scene is
-first advanced through its pour (about 158 substeps, untimed) so every timed
-substep runs with all 1,048,576 particles in the domain; then `--warmup`
-untimed substeps, then exactly `--steps` timed substeps bracketed by stream
-synchronisation.  `value` = particles x substeps / wall seconds of the timed
-region, with all inputs resident in HBM.
+first advanced through its pour (about 158 substeps, untimed; `--state` keeps
+that state in a checkpoint file for later runs) so every timed substep runs
+with all 1,048,576 particles in the domain; then `--warmup` untimed substeps,
+then exactly `--steps` timed substeps bracketed by stream synchronisation.
+`value` = particles x substeps / wall seconds of the timed region, with all
+inputs resident in HBM.  Each solve starts from the previous substep's
+pressure (`--warm-start zero` starts from zero, which at this size needs more
+than the 20,000-iteration cap); one extra untimed substep from a zero start
+reports that iteration count beside the result.
 
 The JSON line also carries
   roofline      for the kernel with the largest share of device time, from
@@ -88,17 +92,24 @@ def roofline(kt) -> dict:
     }
 
 
-def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, budget_s: float) -> dict:
+def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, gpu_iters: float, budget_s: float) -> dict:
     """The CPU oracle (single-threaded C restatement of the reference solver) on
-    the same post-pour state, timed for a bounded number of substeps."""
+    the same post-pour state.
+
+    One substep solved to 1e-4 takes thousands of RBGS iterations, minutes of
+    CPU time at this size, so the sample is bounded: the oracle's own
+    tolerance-mode substep is timed from the same state with the iteration cap
+    at two values that it reaches before converging.  Substep time is linear in
+    the iteration count (a fixed cost for P2G, grid passes and G2P plus a cost
+    per iteration that includes the residual checks), and the two samples give
+    both terms; the baseline is that line evaluated at the GPU's measured mean
+    iterations per substep (identical inputs give identical iteration counts).
+    """
     sys.path.insert(0, str(ROOT / "oracle"))
     import orc  # checker only: times the CPU restatement beside the GPU
 
     from coffee_sim_amd.abi import Scene
 
-    s = w.settings.copy()
-    s.emit_per_substep = 0
-    s.emit_total = 0
     n = particles.shape[0]
     p = np.ascontiguousarray(particles, np.float32)
     a = np.ascontiguousarray(affine, np.float32)
@@ -109,26 +120,43 @@ def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, budget_s: float) 
     sc.particles = p.ctypes.data_as(C.POINTER(C.c_float))
     sc.affine = a.ctypes.data_as(C.POINTER(C.c_float))
     sc.num_particles = n
-    ref = orc.OracleSim(s, sc)
-    done, iters = 0, 0
-    t0 = time.perf_counter()
-    while True:
+
+    def timed(cap: int) -> tuple[float, int]:
+        s = w.settings.copy()
+        s.emit_per_substep = 0
+        s.emit_total = 0
+        s.rbgs_max_iters = cap
+        ref = orc.OracleSim(s, sc)
+        t0 = time.perf_counter()
         ref.step_frame(w.frame_dt)
-        done += 1
-        iters += ref.solve_stats().iterations
         el = time.perf_counter() - t0
-        log(f"cpu oracle substep {done}: {el:.1f}s, {iters} RBGS iterations so far")
-        if el >= budget_s or done >= 8:
-            break
-    el = time.perf_counter() - t0
-    ref.close()
+        it = ref.solve_stats().iterations
+        ref.close()
+        log(f"cpu oracle substep with iteration cap {cap}: {el:.2f}s, {it} iterations")
+        return el, it
+
+    lo_cap = 16
+    t_lo, it_lo = timed(lo_cap)
+    # spend about the budget on the second sample
+    per_iter_guess = max(t_lo / (lo_cap + 8), 1e-4)
+    hi_cap = int(min(max(lo_cap + 64, budget_s / per_iter_guess), 4096)) // 8 * 8
+    t_hi, it_hi = timed(hi_cap)
+    if it_hi <= it_lo:
+        # converged inside the first cap: the sample is a whole substep
+        per_sub, how = t_lo, f"one whole substep ({it_lo} iterations)"
+    else:
+        per_iter = (t_hi - t_lo) / (it_hi - it_lo)
+        fixed = t_lo - it_lo * per_iter
+        per_sub = fixed + gpu_iters * per_iter
+        how = (f"tolerance-mode substeps capped at {it_lo} and {it_hi} iterations ({t_lo:.2f} s, {t_hi:.2f} s): "
+               f"{fixed:.3f} s fixed + {1e3 * per_iter:.3f} ms per iteration, evaluated at "
+               f"{gpu_iters:.1f} iterations")
     return {
-        "value": round(n * done / el, 2),
+        "value": round(n / per_sub, 2),
         "unit": "particle-substeps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{done} substeps of v60_working after the pour ({n} particles, RBGS to 1e-4: "
-                  f"{iters} iterations), single-threaded C oracle, {el:.1f} s",
+        "sample": f"v60_working after the pour ({n} particles), single-threaded C oracle: {how}",
     }
 
 
@@ -140,6 +168,13 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--scale", type=int, default=1, help="divide the grid side (tests only)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle sampling (0 = skip)")
+    ap.add_argument("--warm-start", choices=("previous", "zero"), default="previous",
+                    help="initial pressure of each substep's solve: the previous substep's solution or zero")
+    ap.add_argument("--state", default=None,
+                    help="checkpoint of the post-pour state: loaded when the file exists (the pour is skipped), "
+                         "written after the pour otherwise")
+    ap.add_argument("--no-zero-probe", action="store_true",
+                    help="skip the one untimed substep from a zero start that reports its iteration count")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -148,21 +183,31 @@ def main() -> int:
 
     import coffee_sim_amd as cs
 
+    from coffee_sim_amd.abi import CS_WARM_PREVIOUS, CS_WARM_ZERO
+
     w = cs.SeededWorkload("v60_working", seed=args.seed, scale=args.scale)
+    settings = w.settings.copy()
+    settings.warm_start = CS_WARM_PREVIOUS if args.warm_start == "previous" else CS_WARM_ZERO
     ctx = cs.Context(0)
-    sim = cs.MpmSim3D(ctx, w.settings, w.scene)
+    sim = cs.MpmSim3D(ctx, settings, w.scene)
     dt = w.frame_dt
     pour = (w.settings.emit_total + w.settings.emit_per_substep - 1) // max(w.settings.emit_per_substep, 1)
-    log(f"workload v60_working grid {list(w.settings.grid_dims)} initial {w.scene.num_particles} "
-        f"total {w.total_particles}; advancing {pour} pour substeps untimed")
-    t0 = time.perf_counter()
-    iters = 0
-    for k in range(pour):
-        sim.step_frame(dt)
-        iters += sim.solve_stats().iterations
-        if k % 20 == 0 or k == pour - 1:
-            log(f"pour substep {k + 1}/{pour}: particles {sim.particle_count()} "
-                f"RBGS iters {sim.solve_stats().iterations} elapsed {time.perf_counter() - t0:.1f}s")
+    state = Path(args.state) if args.state else None
+    if state is not None and state.exists():
+        sim.load(state.read_bytes())
+        log(f"loaded the post-pour state from {state}: {sim.particle_count()} particles")
+    else:
+        log(f"workload v60_working grid {list(w.settings.grid_dims)} initial {w.scene.num_particles} "
+            f"total {w.total_particles}; advancing {pour} pour substeps untimed ({args.warm_start} start)")
+        t0 = time.perf_counter()
+        for k in range(pour):
+            sim.step_frame(dt)
+            if k % 20 == 0 or k == pour - 1:
+                log(f"pour substep {k + 1}/{pour}: particles {sim.particle_count()} "
+                    f"RBGS iters {sim.solve_stats().iterations} elapsed {time.perf_counter() - t0:.1f}s")
+        if state is not None:
+            state.write_bytes(sim.save())
+            log(f"saved the post-pour state to {state}")
     for k in range(args.warmup):
         sim.step_frame(dt)
     sim.synchronize()
@@ -186,11 +231,25 @@ def main() -> int:
     log(f"timed {args.steps} substeps in {elapsed:.3f}s; RBGS iterations {iters_timed}")
     log(f"kernel timing {kt.as_dict()}")
 
+    zero_probe = None
+    if not args.no_zero_probe:
+        # the same state solved from a zero start, one substep, untimed
+        zs = settings.copy()
+        zs.warm_start = CS_WARM_ZERO
+        sim0 = cs.MpmSim3D(ctx, zs, w.scene)
+        sim0.load(sim.save())
+        sim0.step_frame(dt)
+        st0 = sim0.solve_stats()
+        zero_probe = {"iterations": int(st0.iterations), "converged": bool(st0.converged),
+                      "final_residual": float(st0.final_residual)}
+        sim0.close()
+        log(f"zero-start probe: {zero_probe}")
+
     cpu = None
     if args.cpu_budget > 0:
         parts = sim.read_particles(0, n_particles)
         aff = sim.read_affine(0, n_particles)
-        cpu = cpu_baseline(w, parts, aff, args.cpu_budget)
+        cpu = cpu_baseline(w, parts, aff, float(np.mean(iters_timed)), args.cpu_budget)
 
     value = psteps / elapsed
     out = {
@@ -208,11 +267,16 @@ def main() -> int:
         "data": "synthetic (seeded generators)",
         "config": {
             "workload": "v60_working: 128^3 grid, dx 1 mm, 1,048,576 particles, value-noise porous bed, "
-                        "RBGS to 1e-4 relative residual",
+                        f"RBGS to 1e-4 relative residual from a {args.warm_start}-step start"
+                        if args.warm_start == "previous" else
+                        "v60_working: 128^3 grid, dx 1 mm, 1,048,576 particles, value-noise porous bed, "
+                        "RBGS to 1e-4 relative residual from a zero start",
             "grid": list(int(d) for d in w.settings.grid_dims),
             "particles": int(n_particles),
             "rbgs_tolerance": float(w.settings.rbgs_tolerance),
+            "rbgs_warm_start": args.warm_start,
             "rbgs_iterations_per_substep": round(float(np.mean(iters_timed)), 1),
+            "rbgs_zero_start_probe": zero_probe,
             "scale": args.scale,
         },
         "roofline": roofline(kt),

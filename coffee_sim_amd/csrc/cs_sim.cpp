@@ -79,6 +79,7 @@ struct cs_sim {
   double* res_out = nullptr;
   // brick-sparse solve
   int32_t* pres2 = nullptr;          // second pressure buffer
+  float* wc = nullptr;               // float4 per cell: face weights and diagonal
   uint32_t* brick_flag = nullptr;    // [nbricks]
   uint32_t* brick_list = nullptr;    // [nbricks]
   double* brick_partials = nullptr;  // [2 * nbricks]
@@ -88,7 +89,8 @@ struct cs_sim {
   hipEvent_t batch_ev[2] = {};
   std::vector<hipEvent_t> chunk_ev;  // start/end pairs for sweep timing
   bool dense_solver = false;         // one-thread-per-cell reference layout (CS_SOLVER=dense)
-  int halo = 4;                      // half sweeps per launch (CS_RBGS_HALO = 1, 2 or 4)
+  int halo = 2;                      // half sweeps per launch (CS_RBGS_HALO = 1, 2 or 4)
+  int threads = 512;                 // threads per brick workgroup (CS_RBGS_THREADS = 256 or 512)
   uint32_t batch_chunks = 4;         // residual checks enqueued ahead of the host
   // host state
   double* h_res = nullptr;    // pinned
@@ -121,7 +123,7 @@ void free_sim(cs_sim* m) {
   void* dev[] = {m->part,       m->aff,        m->lanes,      m->gv,         m->gv2,    m->sdf,
                  m->cls,        m->bed_id,     m->bed,        m->kind,       m->kf,     m->pres,
                  m->rhs,        m->pack,       m->prev_p,     m->warm,       m->metrics, m->res_partials,
-                 m->res_out,    m->pres2,      m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
+                 m->res_out,    m->pres2,      m->wc,      m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -257,7 +259,7 @@ int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ite
   const uint32_t per = (uint32_t)m->halo / 2u;  // iterations per launch (halo 1 -> half iterations)
   while (left > 0) {
     if (m->halo >= 2 && left >= per) {
-      CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, m->halo, 0u, m->brick_list, m->kf, m->rhs, buf[*cur],
+      CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, m->halo, m->threads, 0u, m->brick_list, m->wc, m->rhs, buf[*cur],
                                    buf[*cur ^ 1], m->metrics, m->dstate));
       *cur ^= 1;
       ++n;
@@ -265,8 +267,8 @@ int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ite
     } else {
       int h = m->halo >= 2 ? 2 : 1;
       for (uint32_t c = 0; c < (uint32_t)(2 / h); ++c) {
-        CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, h, h == 1 ? c : 0u, m->brick_list, m->kf, m->rhs, buf[*cur],
-                                     buf[*cur ^ 1], m->metrics, m->dstate));
+        CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, h, h == 1 ? 256 : m->threads, h == 1 ? c : 0u, m->brick_list,
+                                     m->wc, m->rhs, buf[*cur], buf[*cur ^ 1], m->metrics, m->dstate));
         *cur ^= 1;
         ++n;
       }
@@ -285,6 +287,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
   S.final_residual = NAN;
   m->final_residual_pending = false;
   CS_HIP(cs_launch_brick_scan(st, u, m->brick_flag, m->brick_list, m->dstate));
+  CS_HIP(cs_launch_face_weights(st, u, m->kf, m->wc));
   bool external = false;
   if (m->s.warm_start == CS_WARM_PREVIOUS) {
     CS_HIP(cs_launch_warm_previous(st, u, m->kf, m->prev_p, m->pres));
@@ -307,7 +310,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     if (rc) return rc;
     if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[ev0 + 1], st));
     // the final residual for the statistics (mode 0 also records launches = 0)
-    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->kf, m->rhs, buf_of(m, cur), m->brick_partials,
+    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur), m->brick_partials,
                                      m->dstate, 0u, 0u, 0u, tol));
     S.iterations = m->s.pressure_rbgs_pairs;
     if (cur != 0) {
@@ -325,7 +328,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
       m->kt.rbgs_cells_swept += (uint64_t)launches * nactive * CS_BRICK * CS_BRICK * CS_BRICK;
     }
   } else {
-    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->kf, m->rhs, m->pres, m->brick_partials,
+    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
                                      m->dstate, 0u, 0u, 0u, tol));
     if (external) {
       CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
@@ -334,7 +337,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
         // the supplied start is worse than the zero start: fall back
         CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
         S.fell_back = 1;
-        CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->kf, m->rhs, m->pres, m->brick_partials,
+        CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
                                          m->dstate, 0u, 0u, 0u, tol));
       }
     }
@@ -364,7 +367,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
         if (rc) return rc;
         if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e + 1], st));
         ++chunk_id;
-        CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->kf, m->rhs, buf_of(m, cur),
+        CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
                                          m->brick_partials, m->dstate, 1u, chunk, launches, tol));
         enq += chunk;
       }
@@ -616,6 +619,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   cs_params geom;
   cs::make_params(*s, 0, 0, 0.0f, &geom);
   ALLOC(m->pres2, m->nc);
+  ALLOC(m->wc, m->nc * 4);
   ALLOC(m->brick_flag, geom.nbricks);
   ALLOC(m->brick_list, geom.nbricks);
   ALLOC(m->brick_partials, 2 * (size_t)geom.nbricks);
@@ -627,6 +631,11 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
     int h = atoi(v);
     if (h == 1 || h == 2 || h == 4) m->halo = h;
   }
+  if (const char* v = getenv("CS_RBGS_THREADS")) {
+    int t = atoi(v);
+    if (t == 256 || t == 512) m->threads = t;
+  }
+  if (m->halo == 1) m->threads = 256;
   if (const char* v = getenv("CS_RBGS_BATCH")) {
     int b = atoi(v);
     if (b >= 1 && b <= 64) m->batch_chunks = (uint32_t)b;

@@ -12,21 +12,24 @@ struct cs_solve_state {
   uint32_t iterations;     // red+black pairs executed
   uint32_t launches;       // sweep launches executed (parity says which buffer is current)
   uint32_t active_bricks;  // bricks holding a fluid cell
-  uint32_t arrivals;       // workgroups of the running residual pass that finished
-  uint32_t pad;
+  uint32_t pad[2];
   double residual;         // latest relative residual
   double initial_residual; // before the first sweep
 };
 
 hipError_t cs_launch_brick_scan(hipStream_t s, const cs_params& u, const uint32_t* flags, uint32_t* list,
                                 cs_solve_state* st);
-// One launch = `halo` half sweeps (colour color0, color0^1, ...) over every active brick.
-hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, int halo, uint32_t color0,
-                                 const uint32_t* list, const float* kf, const float* rhs, const int32_t* pin,
-                                 int32_t* pout, uint32_t* metrics, const cs_solve_state* st);
+// Face weights and diagonal of every cell, float4 per cell in `wc` [4 * nc].
+hipError_t cs_launch_face_weights(hipStream_t s, const cs_params& u, const float* kf, float* wc);
+// One launch = `halo` half sweeps (colour color0, color0^1, ...) over every active brick;
+// (halo, threads) in {(1,256), (2,256), (2,512), (4,256), (4,512)}.
+hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, int halo, int threads,
+                                 uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
+                                 const int32_t* pin, int32_t* pout, uint32_t* metrics, const cs_solve_state* st);
 // mode 0: initial residual (always runs); mode 1: after `iters` iterations in `launches` launches.
+// Two launches: per-brick partials into `partials` [2 * nactive], then a one-workgroup reduce.
 hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
-                                     const float* kf, const float* rhs, const int32_t* p, double* partials,
+                                     const float* wc, const float* rhs, const int32_t* p, double* partials,
                                      cs_solve_state* st, uint32_t mode, uint32_t iters, uint32_t launches,
                                      double tol);
 hipError_t cs_launch_select_final(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,

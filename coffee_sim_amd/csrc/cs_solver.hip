@@ -2,9 +2,11 @@
 //
 // Restates pressure_update (shader.rs:1371-1439) over the fluid cells only.
 // The grid is tiled into 8x8x8 bricks; classify flags the bricks holding a
-// fluid cell and a single-block scan lists them in ascending order.  One
-// workgroup per active brick loads the brick plus an H-cell halo of pressure,
-// kind/fill word and right-hand side into LDS and runs H half sweeps there:
+// fluid cell and a single-block scan lists them in ascending order.  The face
+// weights and diagonal of every cell are formed once per substep
+// (k_face_weights).  One workgroup per active brick loads the brick plus an
+// H-cell halo of pressure, weights and right-hand side into LDS and runs H
+// half sweeps there:
 // after half sweep t every cell at least t+1 cells from the loaded region's
 // edge holds exactly the value the whole-grid sweep would give, so after H
 // sweeps the brick itself is exact and is written out.  Pressure is
@@ -35,42 +37,91 @@ __device__ __forceinline__ void brick_origin(const cs_params& u, uint32_t b, int
 
 constexpr float KF_OFFGRID = -3.0f;
 
-// One update of pressure_update (shader.rs:1386-1438) on LDS-resident data,
-// branch-free: all six neighbour words are read up front and the skips of
-// the reference become selects.  Adding +0 to a sum that starts at +0 and
-// only ever accumulates positive weights or nonzero products leaves it
-// unchanged, so the result equals the branching form bit for bit.
-__device__ __forceinline__ int32_t update_cell(const cs_params& u, float skf, float srhs, const int2* s_pk, int li,
-                                               int sx, int sy, bool& fired) {
-  const float sf = kf_fill(skf);
-  const bool sbed = kf_bed(skf);
-  const int off[6] = {-1, 1, -sx, sx, -sy, sy};
-  int2 nb[6];
-#pragma unroll
-  for (int n = 0; n < 6; ++n) nb[n] = s_pk[li + off[n]];
-  float count = 0.0f, sum = 0.0f;
-#pragma unroll
-  for (int n = 0; n < 6; ++n) {
-    const float nkf = __int_as_float(nb[n].y);
-    const bool nfluid = nkf >= 0.0f;
-    const bool skip = (nkf == KF_OFFGRID) | (nkf == CS_KF_SOLID);
-    const float nfill = nkf == CS_KF_BED ? 1.0f : nkf;
-    float w = cs_min(sf, nfill);
-    w = nfluid ? w : sf;
-    w = (sbed | (nkf == CS_KF_BED)) ? 1.0f : w;
-    const bool use = !skip && (w > 0.0f);
-    count += use ? w : 0.0f;
-    const float term = w * ((float)nb[n].x * u.inv_fp_scale);
-    sum += (use && nfluid) ? term : 0.0f;
-  }
+// Global index of local cell i of an R^3 region whose corner is (x0, y0, z0),
+// clamped onto the grid, and whether the unclamped cell lies on the grid.  The
+// staging loops load through the clamped index unconditionally and select
+// afterwards, so every load of a thread is issued before the first wait.
+template <int R>
+__device__ __forceinline__ uint32_t region_cell(const cs_params& u, int i, int x0, int y0, int z0, bool& on) {
+  const int lx = i % R, ly = (i / R) % R, lz = i / (R * R);
+  const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+  on = (uint32_t)gx < u.gx && (uint32_t)gy < u.gy && (uint32_t)gz < u.gz;
+  const uint32_t cx = (uint32_t)min(max(gx, 0), (int)u.gx - 1);
+  const uint32_t cy = (uint32_t)min(max(gy, 0), (int)u.gy - 1);
+  const uint32_t cz = (uint32_t)min(max(gz, 0), (int)u.gz - 1);
+  return (cz * u.gy + cy) * u.gx + cx;
+}
+
+// Neighbour offsets in the order pressure_update visits them (shader.rs:1393-1420):
+// -x, +x, -y, +y, -z, +z.
+constexpr int NB_DIR[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+
+// A fixed-point pressure as the update reads it (pressure_load, shader.rs:198).
+__device__ __forceinline__ float p_float(const cs_params& u, int32_t p) { return (float)p * u.inv_fp_scale; }
+
+// One update of pressure_update (shader.rs:1386-1438) from precomputed face
+// weights.  `self` holds the +x, +y, +z face weights and the diagonal count of
+// the cell; the -x, -y, -z weights are the +x, +y, +z weights of the neighbours
+// on those sides (the face weight is symmetric).  A face the reference skips
+// carries weight 0, whose product +-0 leaves the sum unchanged (the sum starts
+// at +0 and is never -0), so the result is the reference's bit for bit.
+__device__ __forceinline__ int32_t update_cell(const cs_params& u, float4 self, float srhs, float wxm, float wym,
+                                               float wzm, const int32_t* s_p, int li, int sx, int sy, bool& fired) {
+  float sum = 0.0f;
+  sum += wxm * p_float(u, s_p[li - 1]);
+  sum += self.x * p_float(u, s_p[li + 1]);
+  sum += wym * p_float(u, s_p[li - sx]);
+  sum += self.y * p_float(u, s_p[li + sx]);
+  sum += wzm * p_float(u, s_p[li - sy]);
+  sum += self.z * p_float(u, s_p[li + sy]);
   float pn = 0.0f;
-  if (count > 0.0f) pn = (sum - srhs) / count;
+  if (self.w > 0.0f) pn = (sum - srhs) / self.w;
   const float cl = cs_clamp(pn, -u.pressure_clamp, u.pressure_clamp);
   fired = cl != pn;
   return cs_f2i(cl * u.fp_scale);
 }
 
 }  // namespace
+
+// ---- face weights of the pressure system ------------------------------------------------
+// For every cell: .x/.y/.z = the weight pressure_update gives the +x/+y/+z
+// neighbour's pressure (pressure_face_weight, shader.rs:306-323, when that
+// neighbour is fluid and the face is used; 0 otherwise), .w = the diagonal
+// `count` accumulated over all six faces in the reference's order
+// (shader.rs:1393-1420); .w = -1 marks a cell that is not fluid.  The kinds
+// and fills do not change during a solve, so this runs once per substep.
+__global__ void __launch_bounds__(256) k_face_weights(cs_params u, const float* __restrict__ kf,
+                                                      float4* __restrict__ wc) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc) return;
+  const float skf = kf[idx];
+  if (!kf_fluid(skf)) {
+    wc[idx] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    return;
+  }
+  uint32_t ix, iy, iz;
+  decompose(u, idx, ix, iy, iz);
+  const float sf = kf_fill(skf);
+  const bool sbed = kf_bed(skf);
+  float count = 0.0f;
+  float wplus[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const int nx = (int)ix + NB_DIR[n][0], ny = (int)iy + NB_DIR[n][1], nz = (int)iz + NB_DIR[n][2];
+    float nkf = KF_OFFGRID;
+    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < (int)u.gx && ny < (int)u.gy && nz < (int)u.gz)
+      nkf = kf[cell_index(u, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz)];
+    const bool nfluid = nkf >= 0.0f;
+    const bool skip = (nkf == KF_OFFGRID) | (nkf == CS_KF_SOLID);
+    float w = cs_min(sf, kf_fill(nkf));
+    w = nfluid ? w : sf;
+    w = (sbed | kf_bed(nkf)) ? 1.0f : w;
+    const bool use = !skip && (w > 0.0f);
+    count += use ? w : 0.0f;
+    if (n & 1) wplus[n >> 1] = (use && nfluid) ? w : 0.0f;
+  }
+  wc[idx] = make_float4(wplus[0], wplus[1], wplus[2], count);
+}
 
 // ---- active brick list ----------------------------------------------------------------
 // Single block: ascending list of flagged bricks and their count.
@@ -98,51 +149,47 @@ __global__ void __launch_bounds__(1024) k_brick_scan(const uint32_t* __restrict_
   if (t == 1023u) st->active_bricks = s_cnt[1023];
 }
 
-template <int H>
-__global__ void __launch_bounds__(256) k_rbgs_brick(cs_params u, const uint32_t* __restrict__ list,
-                                                    uint32_t color0, const float* __restrict__ kf,
-                                                    const float* __restrict__ rhs, const int32_t* __restrict__ pin,
-                                                    int32_t* __restrict__ pout, uint32_t* __restrict__ metrics,
-                                                    const cs_solve_state* __restrict__ st) {
+template <int H, int NT>
+__global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* __restrict__ list, uint32_t color0,
+                                                   const float4* __restrict__ wc, const float* __restrict__ rhs,
+                                                   const int32_t* __restrict__ pin, int32_t* __restrict__ pout,
+                                                   uint32_t* __restrict__ metrics,
+                                                   const cs_solve_state* __restrict__ st) {
   constexpr int R = BR + 2 * H;  // even: BR is 8
   constexpr int R3 = R * R * R;
-  constexpr int NT = 256;
   constexpr int PER = (R3 + NT - 1) / NT;
-  __shared__ int2 s_pk[R3];  // (pressure fixed point, kind/fill word)
+  __shared__ float4 s_wc[R3];  // face weights and diagonal
   __shared__ float s_rhs[R3];
-  if (st->done) return;
+  __shared__ int32_t s_p[R3];  // fixed-point pressure
+  // both scalar loads in flight before the first wait
+  const uint32_t done = st->done;
+  const uint32_t brick = list[blockIdx.x];
+  if (done) return;
   int x0, y0, z0;
-  brick_origin(u, list[blockIdx.x], x0, y0, z0);
+  brick_origin(u, brick, x0, y0, z0);
   x0 -= H;
   y0 -= H;
   z0 -= H;
   const uint32_t base_par = (uint32_t)(x0 + y0 + z0) & 1u;
   {
-    float kfv[PER], rv[PER];
+    float4 wv[PER];
+    float rv[PER];
     int32_t pv[PER];
+    bool on[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int i = (int)threadIdx.x + k * NT;
-      kfv[k] = KF_OFFGRID;
-      rv[k] = 0.0f;
-      pv[k] = 0;
-      if (i < R3) {
-        const int lx = i % R, ly = (i / R) % R, lz = i / (R * R);
-        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)u.gx && gy < (int)u.gy && gz < (int)u.gz) {
-          const uint32_t idx = ((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx;
-          kfv[k] = kf[idx];
-          rv[k] = rhs[idx];
-          pv[k] = pin[idx];
-        }
-      }
+      const uint32_t idx = region_cell<R>(u, min((int)threadIdx.x + k * NT, R3 - 1), x0, y0, z0, on[k]);
+      wv[k] = wc[idx];
+      rv[k] = rhs[idx];
+      pv[k] = pin[idx];
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int i = (int)threadIdx.x + k * NT;
       if (i < R3) {
-        s_pk[i] = make_int2(pv[k], __float_as_int(kfv[k]));
-        s_rhs[i] = rv[k];
+        s_wc[i] = on[k] ? wv[k] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        s_rhs[i] = on[k] ? rv[k] : 0.0f;
+        s_p[i] = on[k] ? pv[k] : 0;
       }
     }
   }
@@ -162,21 +209,22 @@ __global__ void __launch_bounds__(256) k_rbgs_brick(cs_params u, const uint32_t*
       const int lz = lo + j / (half * span);
       const int lx = lo + 2 * m + (int)((color + base_par + (uint32_t)(ly + lz + lo)) & 1u);
       const int li = (lz * R + ly) * R + lx;
-      const float skf = __int_as_float(s_pk[li].y);
-      if (!(skf >= 0.0f)) continue;
+      const float4 self = s_wc[li];
+      if (!(self.w >= 0.0f)) continue;  // not fluid
       bool fired;
-      const int32_t np = update_cell(u, skf, s_rhs[li], s_pk, li, R, R * R, fired);
-      s_pk[li].x = np;
+      const int32_t np = update_cell(u, self, s_rhs[li], s_wc[li - 1].x, s_wc[li - R].y, s_wc[li - R * R].z, s_p,
+                                     li, R, R * R, fired);
+      s_p[li] = np;
       const bool interior = lx >= H && ly >= H && lz >= H && lx < H + BR && ly < H + BR && lz < H + BR;
       if (fired && interior) ++fires;  // each brick cell once, as the whole-grid sweep counts it
     }
     __syncthreads();
   }
-  for (int j = threadIdx.x; j < BR * BR * BR; j += blockDim.x) {
+  for (int j = threadIdx.x; j < BR * BR * BR; j += NT) {
     int lx = H + j % BR, ly = H + (j / BR) % BR, lz = H + j / (BR * BR);
     int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
     if (gx < (int)u.gx && gy < (int)u.gy && gz < (int)u.gz)
-      pout[((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx] = s_pk[(lz * R + ly) * R + lx].x;
+      pout[((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx] = s_p[(lz * R + ly) * R + lx];
   }
   if (fires) atomicAdd(&metrics[CS_M_P_CLAMP], fires);
 }
@@ -184,9 +232,10 @@ __global__ void __launch_bounds__(256) k_rbgs_brick(cs_params u, const uint32_t*
 // ---- residual over the active bricks ------------------------------------------------------
 // One workgroup per brick stages the brick and a one-cell halo of (pressure,
 // kind/fill) in LDS, forms b - A p for its fluid cells in double precision
-// and writes a per-brick partial; the last workgroup to finish sums the
-// partials in brick order (a fixed order) and advances the solve state as
-// k_residual_finalize does.
+// and writes a per-brick partial; a one-workgroup pass then sums the partials
+// in brick order (a fixed order) and advances the solve state.  (Electing the
+// last workgroup with an arrival counter instead serialises one device-scope
+// atomic per brick on a single address, which costs more than the launch.)
 __device__ __forceinline__ void finalize_state(cs_solve_state* st, double rr, double bb, uint32_t mode,
                                                uint32_t iters, uint32_t launches, double tol) {
   double r;
@@ -204,77 +253,66 @@ __device__ __forceinline__ void finalize_state(cs_solve_state* st, double rr, do
 }
 
 __global__ void __launch_bounds__(256) k_residual_bricks(cs_params u, const uint32_t* __restrict__ list,
-                                                         const float* __restrict__ kf, const float* __restrict__ rhs,
+                                                         const float4* __restrict__ wc, const float* __restrict__ rhs,
                                                          const int32_t* __restrict__ p, double* __restrict__ partials,
-                                                         cs_solve_state* __restrict__ st, uint32_t mode,
-                                                         uint32_t iters, uint32_t launches, double tol) {
+                                                         const cs_solve_state* __restrict__ st, uint32_t mode) {
   constexpr int R = BR + 2, R3 = R * R * R, NT = 256, PER = (R3 + NT - 1) / NT;
-  __shared__ int2 s_pk[R3];
+  constexpr int BPER = BR * BR * BR / NT;
+  __shared__ float4 s_wc[R3];
+  __shared__ int32_t s_p[R3];
   __shared__ double s_rr[NT], s_bb[NT];
-  __shared__ uint32_t s_last;
-  if (mode != 0u && st->done) return;
+  const uint32_t done = st->done;
+  const uint32_t brick = list[blockIdx.x];
+  if (mode != 0u && done) return;
   int x0, y0, z0;
-  brick_origin(u, list[blockIdx.x], x0, y0, z0);
+  brick_origin(u, brick, x0, y0, z0);
+  float rb[BPER];  // right-hand side of this thread's brick cells
+  bool rb_on[BPER];
+#pragma unroll
+  for (int k = 0; k < BPER; ++k) {
+    const uint32_t idx = region_cell<BR>(u, (int)threadIdx.x + k * NT, x0, y0, z0, rb_on[k]);
+    rb[k] = rhs[idx];
+  }
   x0 -= 1;
   y0 -= 1;
   z0 -= 1;
   {
-    float kfv[PER];
+    float4 wv[PER];
     int32_t pv[PER];
+    bool on[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int i = (int)threadIdx.x + k * NT;
-      kfv[k] = KF_OFFGRID;
-      pv[k] = 0;
-      if (i < R3) {
-        const int lx = i % R, ly = (i / R) % R, lz = i / (R * R);
-        const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)u.gx && gy < (int)u.gy && gz < (int)u.gz) {
-          const uint32_t idx = ((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx;
-          kfv[k] = kf[idx];
-          pv[k] = p[idx];
-        }
-      }
+      const uint32_t idx = region_cell<R>(u, min((int)threadIdx.x + k * NT, R3 - 1), x0, y0, z0, on[k]);
+      wv[k] = wc[idx];
+      pv[k] = p[idx];
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int i = (int)threadIdx.x + k * NT;
-      if (i < R3) s_pk[i] = make_int2(pv[k], __float_as_int(kfv[k]));
+      if (i < R3) {
+        s_wc[i] = on[k] ? wv[k] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        s_p[i] = on[k] ? pv[k] : 0;
+      }
     }
   }
   __syncthreads();
   double rr = 0.0, bb = 0.0;
-  for (int j = (int)threadIdx.x; j < BR * BR * BR; j += NT) {
+#pragma unroll
+  for (int k = 0; k < BPER; ++k) {
+    const int j = (int)threadIdx.x + k * NT;
     const int lx = 1 + j % BR, ly = 1 + (j / BR) % BR, lz = 1 + j / (BR * BR);
     const int li = (lz * R + ly) * R + lx;
-    const int2 self = s_pk[li];
-    const float skf = __int_as_float(self.y);
-    if (!(skf >= 0.0f)) continue;
-    const float sf = kf_fill(skf);
-    const bool sbed = kf_bed(skf);
+    const float4 self = s_wc[li];
+    const float count = self.w;
+    if (count <= 0.0f) continue;  // not fluid (-1), or a fluid cell with no used face
+    // residual_rows order: -x, +x, -y, +y, -z, +z; unused faces carry weight 0
+    const float wn[6] = {s_wc[li - 1].x, self.x, s_wc[li - R].y, self.y, s_wc[li - R * R].z, self.z};
     const int off[6] = {-1, 1, -R, R, -R * R, R * R};
-    int2 nb[6];
-#pragma unroll
-    for (int n = 0; n < 6; ++n) nb[n] = s_pk[li + off[n]];
-    float count = 0.0f;
     double ap_n = 0.0;
 #pragma unroll
-    for (int n = 0; n < 6; ++n) {
-      const float nkf = __int_as_float(nb[n].y);
-      const bool nfluid = nkf >= 0.0f;
-      const bool skip = (nkf == KF_OFFGRID) | (nkf == CS_KF_SOLID);
-      float w = cs_min(sf, nkf == CS_KF_BED ? 1.0f : nkf);
-      w = nfluid ? w : sf;
-      w = (sbed | (nkf == CS_KF_BED)) ? 1.0f : w;
-      const bool use = !skip && (w > 0.0f);
-      count += use ? w : 0.0f;
-      const double t = (double)w * (double)((float)nb[n].x * u.inv_fp_scale);
-      ap_n += (use && nfluid) ? t : 0.0;
-    }
-    if (count <= 0.0f) continue;
-    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    const double b = -(double)rhs[((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx];
-    const double ap = (double)count * (double)((float)self.x * u.inv_fp_scale) - ap_n;
+    for (int n = 0; n < 6; ++n) ap_n += (double)wn[n] * (double)p_float(u, s_p[li + off[n]]);
+    const double b = -(double)rb[k];
+    const double ap = (double)count * (double)p_float(u, s_p[li]) - ap_n;
     rr += (b - ap) * (b - ap);
     bb += b * b;
   }
@@ -291,18 +329,20 @@ __global__ void __launch_bounds__(256) k_residual_bricks(cs_params u, const uint
   if (threadIdx.x == 0) {
     partials[2 * blockIdx.x] = s_rr[0];
     partials[2 * blockIdx.x + 1] = s_bb[0];
-    __threadfence();
-    const uint32_t prev = __hip_atomic_fetch_add(&st->arrivals, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = prev + 1u == gridDim.x ? 1u : 0u;
   }
-  __syncthreads();
-  if (!s_last) return;
-  // last workgroup: fixed-order sum of the per-brick partials
-  rr = 0.0;
-  bb = 0.0;
-  for (uint32_t i = threadIdx.x; i < gridDim.x; i += NT) {
-    rr += __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    bb += __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Fixed-order sum of `n` per-brick partials, then the solve-state update.
+__global__ void __launch_bounds__(1024) k_residual_reduce(const double* __restrict__ partials, uint32_t n,
+                                                          cs_solve_state* __restrict__ st, uint32_t mode,
+                                                          uint32_t iters, uint32_t launches, double tol) {
+  constexpr uint32_t NT = 1024;
+  __shared__ double s_rr[NT], s_bb[NT];
+  if (mode != 0u && st->done) return;
+  double rr = 0.0, bb = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += NT) {
+    rr += partials[2 * i];
+    bb += partials[2 * i + 1];
   }
   s_rr[threadIdx.x] = rr;
   s_bb[threadIdx.x] = bb;
@@ -314,10 +354,7 @@ __global__ void __launch_bounds__(256) k_residual_bricks(cs_params u, const uint
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    finalize_state(st, s_rr[0], s_bb[0], mode, iters, launches, tol);
-    st->arrivals = 0u;
-  }
+  if (threadIdx.x == 0) finalize_state(st, s_rr[0], s_bb[0], mode, iters, launches, tol);
 }
 
 // The residual of a system with no fluid cell (r = 0: nothing to solve).
@@ -349,7 +386,6 @@ __global__ void k_state_init(cs_solve_state* st) {
     st->done = 0u;
     st->iterations = 0u;
     st->launches = 0u;
-    st->arrivals = 0u;
     st->residual = 0.0;
     st->initial_residual = 0.0;
   }
@@ -363,28 +399,40 @@ hipError_t cs_launch_brick_scan(hipStream_t s, const cs_params& u, const uint32_
   return hipGetLastError();
 }
 
-hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, int halo, uint32_t color0,
-                                 const uint32_t* list, const float* kf, const float* rhs, const int32_t* pin,
-                                 int32_t* pout, uint32_t* metrics, const cs_solve_state* st) {
+hipError_t cs_launch_face_weights(hipStream_t s, const cs_params& u, const float* kf, float* wc) {
+  hipLaunchKernelGGL(k_face_weights, dim3((u.nc + 255u) / 256u), dim3(256), 0, s, u, kf,
+                     reinterpret_cast<float4*>(wc));
+  return hipGetLastError();
+}
+
+hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, int halo, int threads,
+                                 uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
+                                 const int32_t* pin, int32_t* pout, uint32_t* metrics, const cs_solve_state* st) {
   if (nactive == 0) return hipSuccess;
-  dim3 g(nactive), b(256);
-  switch (halo) {
-    case 1: hipLaunchKernelGGL(k_rbgs_brick<1>, g, b, 0, s, u, list, color0, kf, rhs, pin, pout, metrics, st); break;
-    case 2: hipLaunchKernelGGL(k_rbgs_brick<2>, g, b, 0, s, u, list, color0, kf, rhs, pin, pout, metrics, st); break;
-    case 4: hipLaunchKernelGGL(k_rbgs_brick<4>, g, b, 0, s, u, list, color0, kf, rhs, pin, pout, metrics, st); break;
-    default: return hipErrorInvalidValue;
-  }
+  const float4* w4 = reinterpret_cast<const float4*>(wc);
+  const dim3 g(nactive);
+#define CS_BRICK_LAUNCH(H, NT) \
+  hipLaunchKernelGGL((k_rbgs_brick<H, NT>), g, dim3(NT), 0, s, u, list, color0, w4, rhs, pin, pout, metrics, st)
+  if (halo == 1 && threads == 256) CS_BRICK_LAUNCH(1, 256);
+  else if (halo == 2 && threads == 256) CS_BRICK_LAUNCH(2, 256);
+  else if (halo == 2 && threads == 512) CS_BRICK_LAUNCH(2, 512);
+  else if (halo == 4 && threads == 256) CS_BRICK_LAUNCH(4, 256);
+  else if (halo == 4 && threads == 512) CS_BRICK_LAUNCH(4, 512);
+  else return hipErrorInvalidValue;
+#undef CS_BRICK_LAUNCH
   return hipGetLastError();
 }
 
 hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
-                                     const float* kf, const float* rhs, const int32_t* p, double* partials,
+                                     const float* wc, const float* rhs, const int32_t* p, double* partials,
                                      cs_solve_state* st, uint32_t mode, uint32_t iters, uint32_t launches,
                                      double tol) {
-  if (nactive > 0)
-    hipLaunchKernelGGL(k_residual_bricks, dim3(nactive), dim3(256), 0, s, u, list, kf, rhs, p, partials, st, mode,
-                       iters, launches, tol);
-  else
+  if (nactive > 0) {
+    hipLaunchKernelGGL(k_residual_bricks, dim3(nactive), dim3(256), 0, s, u, list,
+                       reinterpret_cast<const float4*>(wc), rhs, p, partials, st, mode);
+    hipLaunchKernelGGL(k_residual_reduce, dim3(1), dim3(1024), 0, s, partials, nactive, st, mode, iters, launches,
+                       tol);
+  } else
     hipLaunchKernelGGL(k_residual_empty, dim3(1), dim3(64), 0, s, st, mode, iters, launches, tol);
   return hipGetLastError();
 }

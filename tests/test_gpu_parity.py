@@ -101,3 +101,73 @@ def test_v60_working_scaled_tolerance_solve(ctx, orc):
         assert sg.iterations == so.iterations, (k, sg.as_dict(), so.as_dict())
         assert sg.converged == so.converged
         compare_state(sim, ref, f"v60_working/4 substep {k}")
+
+
+def test_previous_step_warm_start_matches_oracle(ctx, orc):
+    """Each solve seeded with the previous substep's converged pressure (fluid
+    cells only) reaches the tolerance in the oracle's iteration count."""
+    w = cs.SeededWorkload("v60_working", seed=2, scale=4)
+    s = w.settings.copy()
+    s.warm_start = cs.abi.CS_WARM_PREVIOUS
+    sim, ref = _pair(ctx, orc, s, w.scene)
+    iters = []
+    for k in range(4):
+        sim.step_frame(w.frame_dt)
+        ref.step_frame(w.frame_dt)
+        sg, so = sim.solve_stats(), ref.solve_stats()
+        assert sg.iterations == so.iterations, (k, sg.as_dict(), so.as_dict())
+        iters.append(sg.iterations)
+        compare_state(sim, ref, f"previous warm start substep {k}")
+    # the carried pressure pays off after the first substep
+    assert min(iters[1:]) < iters[0]
+
+
+def test_external_warm_start_and_fallback_match_oracle(ctx, orc):
+    """A supplied pressure field seeds the solve; one worse than zero (residual
+    above 1) is rejected and the solve restarts from zero, on both sides."""
+    w = cs.SeededWorkload("v60_working", seed=4, scale=4)
+    s = w.settings.copy()
+    s.warm_start = cs.abi.CS_WARM_EXTERNAL
+    nc = int(np.prod(list(s.grid_dims)))
+    rng = np.random.default_rng(5)
+    for label, field, fell_back in (("good", None, 0), ("bad", rng.normal(0, 50.0, nc).astype(np.float32), 1)):
+        sim, ref = _pair(ctx, orc, s, w.scene)
+        if field is None:
+            # the zero-start solution of this very substep, as a perfect predictor would give it
+            z = w.settings.copy()
+            probe = cs.MpmSim3D(ctx, z, w.scene)
+            probe.step_frame(w.frame_dt)
+            p_fixed, _ = probe.read_pressure()
+            field = (p_fixed.astype(np.float64) / cs.abi.FP_SCALE).astype(np.float32)
+            probe.close()
+        sim.set_warm_start(field)
+        ref.set_warm_start(field)
+        sim.step_frame(w.frame_dt)
+        ref.step_frame(w.frame_dt)
+        sg, so = sim.solve_stats(), ref.solve_stats()
+        assert sg.iterations == so.iterations, (label, sg.as_dict(), so.as_dict())
+        assert sg.fell_back == so.fell_back == fell_back, label
+        compare_state(sim, ref, f"external warm start ({label})")
+
+
+def test_save_load_round_trip_continues_bitwise(ctx):
+    """A checkpoint taken mid-run and loaded into a fresh simulation continues
+    exactly as the original does."""
+    w = cs.SeededWorkload("v60_working", seed=1, scale=4)
+    s = w.settings.copy()
+    s.warm_start = cs.abi.CS_WARM_PREVIOUS
+    a = cs.MpmSim3D(ctx, s, w.scene)
+    for _ in range(3):
+        a.step_frame(w.frame_dt)
+    blob = a.save()
+    b = cs.MpmSim3D(ctx, s, w.scene)
+    b.load(blob)
+    assert b.particle_count() == a.particle_count()
+    for k in range(2):
+        a.step_frame(w.frame_dt)
+        b.step_frame(w.frame_dt)
+        assert a.solve_stats().iterations == b.solve_stats().iterations
+        n = a.particle_count()
+        assert_bitwise(f"particles after load, substep {k}", b.read_particles(0, n), a.read_particles(0, n))
+        assert_bitwise(f"affine after load, substep {k}", b.read_affine(0, n), a.read_affine(0, n))
+    assert a.times() == b.times()
