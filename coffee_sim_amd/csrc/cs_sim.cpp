@@ -89,8 +89,10 @@ struct cs_sim {
   hipEvent_t batch_ev[2] = {};
   std::vector<hipEvent_t> chunk_ev;  // start/end pairs for sweep timing
   bool dense_solver = false;         // one-thread-per-cell reference layout (CS_SOLVER=dense)
-  int halo = 2;                      // half sweeps per launch (CS_RBGS_HALO = 1, 2 or 4)
+  int halo = 4;                      // half sweeps per launch (CS_RBGS_HALO = 1, 2 or 4)
   int threads = 512;                 // threads per brick workgroup (CS_RBGS_THREADS = 256 or 512)
+  bool multi = true;                 // halo 4: one prefetching workgroup per compute unit (CS_RBGS_MULTI)
+  uint32_t multi_groups = 256;       // workgroups of the multi-brick launch: the compute-unit count
   uint32_t batch_chunks = 4;         // residual checks enqueued ahead of the host
   // host state
   double* h_res = nullptr;    // pinned
@@ -259,8 +261,12 @@ int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ite
   const uint32_t per = (uint32_t)m->halo / 2u;  // iterations per launch (halo 1 -> half iterations)
   while (left > 0) {
     if (m->halo >= 2 && left >= per) {
-      CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, m->halo, m->threads, 0u, m->brick_list, m->wc, m->rhs, buf[*cur],
-                                   buf[*cur ^ 1], m->metrics, m->dstate));
+      if (m->halo == 4 && m->multi)
+        CS_HIP(cs_launch_rbgs_multi(st, u, nactive, m->multi_groups, 0u, m->brick_list, m->wc, m->rhs, buf[*cur],
+                                    buf[*cur ^ 1], m->metrics, m->dstate));
+      else
+        CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, m->halo, m->threads, 0u, m->brick_list, m->wc, m->rhs,
+                                     buf[*cur], buf[*cur ^ 1], m->metrics, m->dstate));
       *cur ^= 1;
       ++n;
       left -= per;
@@ -630,6 +636,16 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (const char* v = getenv("CS_RBGS_HALO")) {
     int h = atoi(v);
     if (h == 1 || h == 2 || h == 4) m->halo = h;
+  }
+  if (const char* v = getenv("CS_RBGS_MULTI")) m->multi = atoi(v) != 0;
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->ctx->device) == hipSuccess && cus > 0)
+      m->multi_groups = (uint32_t)cus;
+  }
+  if (const char* v = getenv("CS_RBGS_GROUPS")) {
+    int g = atoi(v);
+    if (g >= 1 && g <= 4096) m->multi_groups = (uint32_t)g;
   }
   if (const char* v = getenv("CS_RBGS_THREADS")) {
     int t = atoi(v);

@@ -26,6 +26,11 @@ hipError_t cs_launch_face_weights(hipStream_t s, const cs_params& u, const float
 hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, int halo, int threads,
                                  uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
                                  const int32_t* pin, int32_t* pout, uint32_t* metrics, const cs_solve_state* st);
+// Four half sweeps per launch with `groups` workgroups of 1024 threads (one per compute unit),
+// each walking a run of the active-brick list and prefetching the next brick while it sweeps.
+hipError_t cs_launch_rbgs_multi(hipStream_t s, const cs_params& u, uint32_t nactive, uint32_t groups,
+                                uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
+                                const int32_t* pin, int32_t* pout, uint32_t* metrics, const cs_solve_state* st);
 // mode 0: initial residual (always runs); mode 1: after `iters` iterations in `launches` launches.
 // Two launches: per-brick partials into `partials` [2 * nactive], then a one-workgroup reduce.
 hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,

@@ -59,6 +59,11 @@ constexpr int NB_DIR[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 
 // A fixed-point pressure as the update reads it (pressure_load, shader.rs:198).
 __device__ __forceinline__ float p_float(const cs_params& u, int32_t p) { return (float)p * u.inv_fp_scale; }
 
+// pressure_update's fixed-point store of a clamped value: |v| <= pressure_clamp
+// = FP_VALUE_LIMIT - 1 (mod.rs:592), so v * fp_scale lies inside the i32
+// range and truncation equals the saturating conversion.
+__device__ __forceinline__ int32_t to_fixed_clamped(const cs_params& u, float v) { return (int32_t)(v * u.fp_scale); }
+
 // One update of pressure_update (shader.rs:1386-1438) from precomputed face
 // weights.  `self` holds the +x, +y, +z face weights and the diagonal count of
 // the cell; the -x, -y, -z weights are the +x, +y, +z weights of the neighbours
@@ -74,11 +79,13 @@ __device__ __forceinline__ int32_t update_cell(const cs_params& u, float4 self, 
   sum += self.y * p_float(u, s_p[li + sx]);
   sum += wzm * p_float(u, s_p[li - sy]);
   sum += self.z * p_float(u, s_p[li + sy]);
-  float pn = 0.0f;
-  if (self.w > 0.0f) pn = (sum - srhs) / self.w;
+  // count > 0 selects the quotient, else 0 (shader.rs:1422-1430); the division
+  // runs either way so the update has no branch
+  const float quot = (sum - srhs) / self.w;
+  const float pn = self.w > 0.0f ? quot : 0.0f;
   const float cl = cs_clamp(pn, -u.pressure_clamp, u.pressure_clamp);
   fired = cl != pn;
-  return cs_f2i(cl * u.fp_scale);
+  return to_fixed_clamped(u, cl);
 }
 
 }  // namespace
@@ -149,33 +156,22 @@ __global__ void __launch_bounds__(1024) k_brick_scan(const uint32_t* __restrict_
   if (t == 1023u) st->active_bricks = s_cnt[1023];
 }
 
+// ---- red-black sweeps over LDS-resident regions --------------------------------------------
+// A region is a brick plus an H-cell halo, R = 8 + 2H cells a side, staged as
+// face weights, right-hand side and fixed-point pressure (24 B per cell).
 template <int H, int NT>
-__global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* __restrict__ list, uint32_t color0,
-                                                   const float4* __restrict__ wc, const float* __restrict__ rhs,
-                                                   const int32_t* __restrict__ pin, int32_t* __restrict__ pout,
-                                                   uint32_t* __restrict__ metrics,
-                                                   const cs_solve_state* __restrict__ st) {
-  constexpr int R = BR + 2 * H;  // even: BR is 8
-  constexpr int R3 = R * R * R;
-  constexpr int PER = (R3 + NT - 1) / NT;
-  __shared__ float4 s_wc[R3];  // face weights and diagonal
-  __shared__ float s_rhs[R3];
-  __shared__ int32_t s_p[R3];  // fixed-point pressure
-  // both scalar loads in flight before the first wait
-  const uint32_t done = st->done;
-  const uint32_t brick = list[blockIdx.x];
-  if (done) return;
-  int x0, y0, z0;
-  brick_origin(u, brick, x0, y0, z0);
-  x0 -= H;
-  y0 -= H;
-  z0 -= H;
-  const uint32_t base_par = (uint32_t)(x0 + y0 + z0) & 1u;
-  {
-    float4 wv[PER];
-    float rv[PER];
-    int32_t pv[PER];
-    bool on[PER];
+struct Region {
+  static constexpr int R = BR + 2 * H;  // even: BR is 8
+  static constexpr int R3 = R * R * R;
+  static constexpr int PER = (R3 + NT - 1) / NT;
+  float4 wv[PER];
+  float rv[PER];
+  int32_t pv[PER];
+  bool on[PER];
+
+  // Issues every global load of the region at (x0, y0, z0) into registers.
+  __device__ __forceinline__ void fetch(const cs_params& u, int x0, int y0, int z0, const float4* __restrict__ wc,
+                                        const float* __restrict__ rhs, const int32_t* __restrict__ pin) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t idx = region_cell<R>(u, min((int)threadIdx.x + k * NT, R3 - 1), x0, y0, z0, on[k]);
@@ -183,6 +179,9 @@ __global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* 
       rv[k] = rhs[idx];
       pv[k] = pin[idx];
     }
+  }
+  // Writes the fetched region to LDS; off-grid cells become non-fluid.
+  __device__ __forceinline__ void stage(float4* s_wc, float* s_rhs, int32_t* s_p) const {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int i = (int)threadIdx.x + k * NT;
@@ -193,38 +192,195 @@ __global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* 
       }
     }
   }
-  __syncthreads();
+};
+
+// The cells a thread updates in each half sweep, fixed for the whole launch.
+// Half sweep s covers the cells at least s+1 cells inside the region; its
+// j-th cell of a colour sits at (lo + 2m + parity, ly, lz) with
+// (m, ly, lz) from j, so a thread keeps, per round of NT cells, the local
+// index of the even-x candidate, the parity of ly + lz + lo and whether the
+// cell is a brick cell for either x parity.  code = li0 | q << 16 |
+// inner_even << 17 | inner_odd << 18 | valid << 19.
+template <int H, int NT>
+struct SweepMap {
+  static constexpr int R = BR + 2 * H;
+  static constexpr int rounds(int s) {
+    const int span = R - 2 * (s + 1);
+    return ((span / 2) * span * span + NT - 1) / NT;
+  }
+  static constexpr int total() {
+    int t = 0;
+    for (int s = 0; s < H; ++s) t += rounds(s);
+    return t;
+  }
+  uint32_t code[total()];
+
+  __device__ __forceinline__ void build() {
+    int t = 0;
+#pragma unroll
+    for (int s = 0; s < H; ++s) {
+      const int lo = s + 1;
+      const int span = R - 2 * (s + 1);
+      const int half = span / 2;
+      const int n = half * span * span;
+#pragma unroll
+      for (int r = 0; r < rounds(s); ++r, ++t) {
+        const int j = (int)threadIdx.x + r * NT;
+        const int m = j % half;
+        const int ly = lo + (j / half) % span;
+        const int lz = lo + j / (half * span);
+        const int lx0 = lo + 2 * m;
+        const bool yz_in = ly >= H && lz >= H && ly < H + BR && lz < H + BR;
+        const uint32_t inner_even = yz_in && lx0 >= H && lx0 < H + BR;
+        const uint32_t inner_odd = yz_in && lx0 + 1 >= H && lx0 + 1 < H + BR;
+        const uint32_t q = (uint32_t)(ly + lz + lo) & 1u;
+        code[t] = j < n ? ((uint32_t)((lz * R + ly) * R + lx0) | q << 16 | inner_even << 17 | inner_odd << 18 | 1u << 19)
+                        : 0u;
+      }
+    }
+  }
+};
+
+// H half sweeps (colours color0, color0^1, ...) over a staged region whose
+// corner cell has parity base_par.  Each update is branch-free: every LDS read
+// is issued at once and a cell that is not fluid writes its own value back.
+// Returns the clamp fires of the brick cells (each brick cell counted once per
+// half sweep of its colour, as the whole-grid sweep counts it).
+template <int H, int NT>
+__device__ __forceinline__ uint32_t sweep_region(const cs_params& u, const SweepMap<H, NT>& map, uint32_t color0,
+                                                 uint32_t base_par, const float4* s_wc, const float* s_rhs,
+                                                 int32_t* s_p) {
+  constexpr int R = SweepMap<H, NT>::R;
   uint32_t fires = 0;
+  int t = 0;
 #pragma unroll
   for (int s = 0; s < H; ++s) {
-    // cells of this half sweep's colour at least s+1 cells inside the region
-    const uint32_t color = color0 ^ (uint32_t)(s & 1);
-    const int lo = s + 1;
-    const int span = R - 2 * (s + 1);  // even
-    const int half = span / 2;
-    const int n = half * span * span;
-    for (int j = (int)threadIdx.x; j < n; j += NT) {
-      const int m = j % half;
-      const int ly = lo + (j / half) % span;
-      const int lz = lo + j / (half * span);
-      const int lx = lo + 2 * m + (int)((color + base_par + (uint32_t)(ly + lz + lo)) & 1u);
-      const int li = (lz * R + ly) * R + lx;
-      const float4 self = s_wc[li];
-      if (!(self.w >= 0.0f)) continue;  // not fluid
-      bool fired;
-      const int32_t np = update_cell(u, self, s_rhs[li], s_wc[li - 1].x, s_wc[li - R].y, s_wc[li - R * R].z, s_p,
-                                     li, R, R * R, fired);
-      s_p[li] = np;
-      const bool interior = lx >= H && ly >= H && lz >= H && lx < H + BR && ly < H + BR && lz < H + BR;
-      if (fired && interior) ++fires;  // each brick cell once, as the whole-grid sweep counts it
+    const uint32_t cpar = color0 ^ (uint32_t)(s & 1) ^ base_par;
+#pragma unroll
+    for (int r = 0; r < SweepMap<H, NT>::rounds(s); ++r, ++t) {
+      const uint32_t code = map.code[t];
+      if (code >> 19) {
+        const uint32_t par = (cpar + (code >> 16)) & 1u;
+        const int li = (int)(code & 0xffffu) + (int)par;
+        const float4 self = s_wc[li];
+        const float srhs = s_rhs[li];
+        const int32_t pold = s_p[li];
+        bool fired;
+        const int32_t np = update_cell(u, self, srhs, s_wc[li - 1].x, s_wc[li - R].y, s_wc[li - R * R].z, s_p, li,
+                                       R, R * R, fired);
+        const bool fluid = self.w >= 0.0f;
+        s_p[li] = fluid ? np : pold;
+        const uint32_t inner = (code >> (17u + par)) & 1u;
+        fires += (fluid && fired) ? inner : 0u;
+      }
     }
     __syncthreads();
   }
+  return fires;
+}
+
+// Writes the brick cells of a swept region (corner x0, y0, z0) to `pout`.
+template <int H, int NT>
+__device__ __forceinline__ void store_brick(const cs_params& u, int x0, int y0, int z0, const int32_t* s_p,
+                                            int32_t* __restrict__ pout) {
+  constexpr int R = Region<H, NT>::R;
   for (int j = threadIdx.x; j < BR * BR * BR; j += NT) {
-    int lx = H + j % BR, ly = H + (j / BR) % BR, lz = H + j / (BR * BR);
-    int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    const int lx = H + j % BR, ly = H + (j / BR) % BR, lz = H + j / (BR * BR);
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
     if (gx < (int)u.gx && gy < (int)u.gy && gz < (int)u.gz)
       pout[((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx] = s_p[(lz * R + ly) * R + lx];
+  }
+}
+
+// One workgroup per active brick.
+template <int H, int NT>
+__global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* __restrict__ list, uint32_t color0,
+                                                   const float4* __restrict__ wc, const float* __restrict__ rhs,
+                                                   const int32_t* __restrict__ pin, int32_t* __restrict__ pout,
+                                                   uint32_t* __restrict__ metrics,
+                                                   const cs_solve_state* __restrict__ st) {
+  using Reg = Region<H, NT>;
+  __shared__ float4 s_wc[Reg::R3];
+  __shared__ float s_rhs[Reg::R3];
+  __shared__ int32_t s_p[Reg::R3];
+  // both scalar loads in flight before the first wait
+  const uint32_t done = st->done;
+  const uint32_t brick = list[blockIdx.x];
+  if (done) return;
+  int x0, y0, z0;
+  brick_origin(u, brick, x0, y0, z0);
+  x0 -= H;
+  y0 -= H;
+  z0 -= H;
+  Reg reg;
+  reg.fetch(u, x0, y0, z0, wc, rhs, pin);
+  SweepMap<H, NT> map;
+  map.build();
+  reg.stage(s_wc, s_rhs, s_p);
+  __syncthreads();
+  const uint32_t fires = sweep_region<H, NT>(u, map, color0, (uint32_t)(x0 + y0 + z0) & 1u, s_wc, s_rhs, s_p);
+  store_brick<H, NT>(u, x0, y0, z0, s_p, pout);
+  if (fires) atomicAdd(&metrics[CS_M_P_CLAMP], fires);
+}
+
+// A fixed grid of workgroups (one per compute unit when the region fills the
+// LDS), each walking a run of the active-brick list and fetching the next
+// brick's region into registers while it sweeps the current one, so the
+// memory latency of the staging loads is paid once per launch rather than
+// once per brick.  The list is split into eight contiguous runs, one per
+// group of workgroups with equal blockIdx.x % 8 (the XCD a workgroup is
+// dispatched to), so spatially neighbouring bricks share an L2.
+template <int H, int NT>
+__global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* __restrict__ list, uint32_t nactive,
+                                                   uint32_t color0, const float4* __restrict__ wc,
+                                                   const float* __restrict__ rhs, const int32_t* __restrict__ pin,
+                                                   int32_t* __restrict__ pout, uint32_t* __restrict__ metrics,
+                                                   const cs_solve_state* __restrict__ st) {
+  using Reg = Region<H, NT>;
+  __shared__ float4 s_wc[Reg::R3];
+  __shared__ float s_rhs[Reg::R3];
+  __shared__ int32_t s_p[Reg::R3];
+  const uint32_t done = st->done;
+  // this workgroup's bricks: run r = blockIdx.x % 8 of the list, every `per_run`-th from `slot`
+  const uint32_t runs = gridDim.x >= 8u ? 8u : 1u;
+  const uint32_t run = blockIdx.x % runs;
+  const uint32_t per_run = gridDim.x / runs;  // the host launches a multiple of 8 (or fewer than 8) groups
+  const uint32_t slot = blockIdx.x / runs;
+  const uint32_t run_lo = (uint32_t)(((uint64_t)nactive * run) / runs);
+  const uint32_t run_hi = (uint32_t)(((uint64_t)nactive * (run + 1u)) / runs);
+  uint32_t b = run_lo + slot;
+  if (done || b >= run_hi) return;
+  int x0, y0, z0;
+  brick_origin(u, list[b], x0, y0, z0);
+  x0 -= H;
+  y0 -= H;
+  z0 -= H;
+  Reg reg;
+  reg.fetch(u, x0, y0, z0, wc, rhs, pin);
+  SweepMap<H, NT> map;
+  map.build();
+  uint32_t fires = 0;
+  for (;;) {
+    reg.stage(s_wc, s_rhs, s_p);
+    __syncthreads();
+    // prefetch the next brick; its loads stay in flight through the sweeps
+    const uint32_t nb = b + per_run;
+    int nx0 = 0, ny0 = 0, nz0 = 0;
+    if (nb < run_hi) {
+      brick_origin(u, list[nb], nx0, ny0, nz0);
+      nx0 -= H;
+      ny0 -= H;
+      nz0 -= H;
+      reg.fetch(u, nx0, ny0, nz0, wc, rhs, pin);
+    }
+    fires += sweep_region<H, NT>(u, map, color0, (uint32_t)(x0 + y0 + z0) & 1u, s_wc, s_rhs, s_p);
+    store_brick<H, NT>(u, x0, y0, z0, s_p, pout);
+    if (nb >= run_hi) break;
+    __syncthreads();  // the store has read the LDS region before it is restaged
+    b = nb;
+    x0 = nx0;
+    y0 = ny0;
+    z0 = nz0;
   }
   if (fires) atomicAdd(&metrics[CS_M_P_CLAMP], fires);
 }
@@ -420,6 +576,17 @@ hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nac
   else if (halo == 4 && threads == 512) CS_BRICK_LAUNCH(4, 512);
   else return hipErrorInvalidValue;
 #undef CS_BRICK_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t cs_launch_rbgs_multi(hipStream_t s, const cs_params& u, uint32_t nactive, uint32_t groups,
+                                uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
+                                const int32_t* pin, int32_t* pout, uint32_t* metrics, const cs_solve_state* st) {
+  if (nactive == 0) return hipSuccess;
+  if (groups >= 8u) groups &= ~7u;  // whole runs
+  if (groups == 0u) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_rbgs_multi<4, 1024>), dim3(groups), dim3(1024), 0, s, u, list, nactive, color0,
+                     reinterpret_cast<const float4*>(wc), rhs, pin, pout, metrics, st);
   return hipGetLastError();
 }
 

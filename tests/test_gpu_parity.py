@@ -63,13 +63,22 @@ def test_v60_small_one_substep_grid_tolerance_and_bitwise(ctx, orc):
     compare_state(sim, ref, "v60_small substep 1")
 
 
-@pytest.mark.parametrize("variant", [{"CS_SOLVER": "dense"}, {"CS_RBGS_HALO": "1"}, {"CS_RBGS_HALO": "2"},
-                                     {"CS_RBGS_HALO": "4", "CS_RBGS_BATCH": "1"}],
-                         ids=["dense", "brick-h1", "brick-h2", "brick-h4-batch1"])
+@pytest.mark.parametrize("variant", [
+    {"CS_SOLVER": "dense"},
+    {"CS_RBGS_HALO": "1"},
+    {"CS_RBGS_HALO": "2", "CS_RBGS_THREADS": "256"},
+    {"CS_RBGS_HALO": "2", "CS_RBGS_THREADS": "512"},
+    {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "0", "CS_RBGS_BATCH": "1"},
+    {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1"},
+    {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_RBGS_GROUPS": "8"},
+    {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_RBGS_GROUPS": "3"},
+], ids=["dense", "brick-h1", "brick-h2-t256", "brick-h2-t512", "brick-h4-single-batch1", "brick-h4-multi",
+        "brick-h4-multi-8groups", "brick-h4-multi-3groups"])
 def test_solver_variants_match_oracle(ctx, orc, monkeypatch, variant):
     """Every pressure-solver layout (one thread per cell over the whole grid;
-    brick-sparse with 1, 2 or 4 half sweeps per launch) gives the oracle's
-    pressure, iteration count and particles bit for bit."""
+    brick-sparse with 1, 2 or 4 half sweeps per launch; the multi-brick launch
+    with few workgroups, so each walks several bricks and prefetches) gives
+    the oracle's pressure, iteration count and particles bit for bit."""
     for k, v in variant.items():
         monkeypatch.setenv(k, v)
     w = cs.SeededWorkload("v60_working", seed=3, scale=4)
@@ -171,3 +180,22 @@ def test_save_load_round_trip_continues_bitwise(ctx):
         assert_bitwise(f"particles after load, substep {k}", b.read_particles(0, n), a.read_particles(0, n))
         assert_bitwise(f"affine after load, substep {k}", b.read_affine(0, n), a.read_affine(0, n))
     assert a.times() == b.times()
+
+
+@pytest.mark.parametrize("variant", [{}, {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1"}],
+                         ids=["default", "brick-h4-multi"])
+def test_v60_working_full_size_fixed_pairs(ctx, orc, monkeypatch, variant):
+    """The full 128^3 working grid (more active bricks than compute units, so
+    every multi-brick workgroup walks several bricks): one substep with 13
+    fixed RBGS pairs (an odd count, so the remainder path runs) matches the
+    oracle bit for bit."""
+    for k, v in variant.items():
+        monkeypatch.setenv(k, v)
+    w = cs.SeededWorkload("v60_working", seed=1)
+    s = w.settings.copy()
+    s.rbgs_mode = 0
+    s.pressure_rbgs_pairs = 13
+    sim, ref = _pair(ctx, orc, s, w.scene)
+    sim.step_frame(w.frame_dt)
+    ref.step_frame(w.frame_dt)
+    compare_state(sim, ref, f"v60_working full size {variant}")
