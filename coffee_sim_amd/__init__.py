@@ -12,7 +12,9 @@ built, `load_library()` raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 import os
 from pathlib import Path
 
@@ -189,12 +191,27 @@ class SeededWorkload:
             pass
 
 
+# Handles still open at interpreter exit are closed before the HIP runtime
+# tears itself down (simulations first: each holds its context).
+_live_sims: "weakref.WeakSet" = weakref.WeakSet()
+_live_ctxs: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_handles() -> None:
+    for sim in list(_live_sims):
+        sim.close()
+    for ctx in list(_live_ctxs):
+        ctx.close()
+
+
 class Context:
     """Device + stream (the wgpu Device/Queue pair the reference borrows, mod.rs:196)."""
 
     def __init__(self, device: int = 0):
         self._h = C.c_void_p()
         _check(load_library().cs_ctx_create(device, C.byref(self._h)), "cs_ctx_create")
+        _live_ctxs.add(self)
 
     @property
     def handle(self):
@@ -230,6 +247,7 @@ class MpmSim3D:
                "cs_sim_create")
         self.nc = self.settings.num_cells
         self.dims = tuple(int(d) for d in self.settings.grid_dims)
+        _live_sims.add(self)
 
     # ---- lifecycle -----------------------------------------------------------------
     def close(self) -> None:

@@ -94,6 +94,8 @@ struct cs_sim {
   bool multi = true;                 // halo 4: one prefetching workgroup per compute unit (CS_RBGS_MULTI)
   uint32_t multi_groups = 256;       // workgroups of the multi-brick launch: the compute-unit count
   uint32_t batch_chunks = 4;         // residual checks enqueued ahead of the host
+  bool fuse_residual = true;         // residual test inside the multi-brick sweeps (CS_RBGS_FUSE)
+  double* group_partials = nullptr;  // [2 * 1024]: per-workgroup sums of the fused test
   // host state
   double* h_res = nullptr;    // pinned
   float* h_stage = nullptr;   // pinned emission staging
@@ -125,7 +127,7 @@ void free_sim(cs_sim* m) {
   void* dev[] = {m->part,       m->aff,        m->lanes,      m->gv,         m->gv2,    m->sdf,
                  m->cls,        m->bed_id,     m->bed,        m->kind,       m->kf,     m->pres,
                  m->rhs,        m->pack,       m->prev_p,     m->warm,       m->metrics, m->res_partials,
-                 m->res_out,    m->pres2,      m->wc,      m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
+                 m->res_out,    m->pres2,      m->wc,         m->group_partials,      m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -250,10 +252,25 @@ int solve_dense(cs_sim* m, const cs_params& u) {
 
 int32_t* buf_of(cs_sim* m, int cur) { return cur ? m->pres2 : m->pres; }
 
+// The residual test of the previous chunk, folded into the first two launches
+// of the next one (multi-brick launches only): the first sums the residual of
+// its input, the second decides.
+struct FusedCheck {
+  uint32_t iters;     // iterations of the chunk being judged
+  uint32_t launches;  // its sweep launches
+  double tol;
+};
+
+// Whether a chunk of `iters` iterations starts with two multi-brick launches.
+bool can_fuse(const cs_sim* m, uint32_t iters) { return m->halo == 4 && m->multi && iters >= 4; }
+
 // Enqueues `iters` red+black iterations as brick launches of `halo` half
 // sweeps (single-iteration launches for any odd remainder); returns the number
-// of launches.  Buffers alternate per launch starting from `*cur`.
-int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t iters, uint32_t* launches, int* cur) {
+// of launches.  Buffers alternate per launch starting from `*cur`.  With
+// `check` (only when can_fuse holds) the first two launches carry the
+// previous chunk's residual test.
+int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t iters, uint32_t* launches, int* cur,
+                   const FusedCheck* check = nullptr) {
   hipStream_t st = m->ctx->stream;
   int32_t* buf[2] = {m->pres, m->pres2};
   uint32_t n = 0;
@@ -261,12 +278,22 @@ int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ite
   const uint32_t per = (uint32_t)m->halo / 2u;  // iterations per launch (halo 1 -> half iterations)
   while (left > 0) {
     if (m->halo >= 2 && left >= per) {
-      if (m->halo == 4 && m->multi)
+      if (m->halo == 4 && m->multi) {
+        uint32_t mode = 0u, ci = 0u, cl = 0u;
+        double tol = 0.0;
+        if (check && n == 0) mode = CS_RES_COMPUTE;
+        if (check && n == 1) {
+          mode = CS_RES_DECIDE;
+          ci = check->iters;
+          cl = check->launches;
+          tol = check->tol;
+        }
         CS_HIP(cs_launch_rbgs_multi(st, u, nactive, m->multi_groups, 0u, m->brick_list, m->wc, m->rhs, buf[*cur],
-                                    buf[*cur ^ 1], m->metrics, m->dstate));
-      else
+                                    buf[*cur ^ 1], m->metrics, m->dstate, m->group_partials, mode, ci, cl, tol));
+      } else {
         CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, m->halo, m->threads, 0u, m->brick_list, m->wc, m->rhs,
                                      buf[*cur], buf[*cur ^ 1], m->metrics, m->dstate));
+      }
       *cur ^= 1;
       ++n;
       left -= per;
@@ -319,9 +346,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur), m->brick_partials,
                                      m->dstate, 0u, 0u, 0u, tol));
     S.iterations = m->s.pressure_rbgs_pairs;
-    if (cur != 0) {
-      CS_HIP(hipMemcpyAsync(m->pres, m->pres2, m->nc * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-    }
+    if (cur != 0) CS_HIP(cs_launch_select_final(st, u, nactive, m->brick_list, m->pres, m->pres2, m->dstate, false));
     CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
     m->final_residual_pending = true;
     m->pending_from_state = true;
@@ -362,19 +387,27 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
       }
     };
     bool stop = false;
+    bool pending = false;  // the last enqueued chunk still awaits its residual test
+    FusedCheck prev{0u, 0u, tol};
     while (enq < cap && !stop) {
       batch_first.push_back(chunk_id);
       for (uint32_t c = 0; c < m->batch_chunks && enq < cap; ++c) {
         uint32_t chunk = std::min(every, cap - enq);
+        const bool fuse = pending && m->fuse_residual && can_fuse(m, chunk);
+        if (pending && !fuse) {
+          CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
+                                           m->brick_partials, m->dstate, 1u, prev.iters, prev.launches, tol));
+        }
         uint32_t launches = 0;
         size_t e = (size_t)(chunk_id % pool) * 2;
         if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e], st));
-        int rc = enqueue_sweeps(m, u, nactive, chunk, &launches, &cur);
+        int rc = enqueue_sweeps(m, u, nactive, chunk, &launches, &cur, fuse ? &prev : nullptr);
         if (rc) return rc;
         if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e + 1], st));
         ++chunk_id;
-        CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
-                                         m->brick_partials, m->dstate, 1u, chunk, launches, tol));
+        pending = true;
+        prev.iters = chunk;
+        prev.launches = launches;
         enq += chunk;
       }
       CS_HIP(hipMemcpyAsync(&m->h_done[batch & 1u], &m->dstate->done, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -386,6 +419,10 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
       }
       ++batch;
     }
+    // the last chunk's test (a no-op once the state says done)
+    if (pending)
+      CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
+                                       m->brick_partials, m->dstate, 1u, prev.iters, prev.launches, tol));
     CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
     CS_HIP(hipStreamSynchronize(st));
     if (m->timing) {
@@ -399,7 +436,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     S.final_residual = hs.residual;
     S.converged = hs.done;
     // the solved pressure is in pres2 when an odd number of launches ran
-    CS_HIP(cs_launch_select_final(st, u, nactive, m->brick_list, m->pres, m->pres2, m->dstate));
+    CS_HIP(cs_launch_select_final(st, u, nactive, m->brick_list, m->pres, m->pres2, m->dstate, true));
     if (m->timing) {
       m->kt.rbgs_launches += hs.launches;
       m->kt.rbgs_cells_swept += (uint64_t)hs.launches * nactive * CS_BRICK * CS_BRICK * CS_BRICK;
@@ -626,6 +663,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   cs::make_params(*s, 0, 0, 0.0f, &geom);
   ALLOC(m->pres2, m->nc);
   ALLOC(m->wc, m->nc * 4);
+  ALLOC(m->group_partials, 2 * 1024);
   ALLOC(m->brick_flag, geom.nbricks);
   ALLOC(m->brick_list, geom.nbricks);
   ALLOC(m->brick_partials, 2 * (size_t)geom.nbricks);
@@ -638,6 +676,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
     if (h == 1 || h == 2 || h == 4) m->halo = h;
   }
   if (const char* v = getenv("CS_RBGS_MULTI")) m->multi = atoi(v) != 0;
+  if (const char* v = getenv("CS_RBGS_FUSE")) m->fuse_residual = atoi(v) != 0;
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->ctx->device) == hipSuccess && cus > 0)
@@ -645,7 +684,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   }
   if (const char* v = getenv("CS_RBGS_GROUPS")) {
     int g = atoi(v);
-    if (g >= 1 && g <= 4096) m->multi_groups = (uint32_t)g;
+    if (g >= 1 && g <= 1024) m->multi_groups = (uint32_t)g;
   }
   if (const char* v = getenv("CS_RBGS_THREADS")) {
     int t = atoi(v);

@@ -12,7 +12,8 @@ struct cs_solve_state {
   uint32_t iterations;     // red+black pairs executed
   uint32_t launches;       // sweep launches executed (parity says which buffer is current)
   uint32_t active_bricks;  // bricks holding a fluid cell
-  uint32_t pad[2];
+  uint32_t pending_fires;   // clamp fires of a residual-computing launch, committed by the deciding one
+  uint32_t pad;
   double residual;         // latest relative residual
   double initial_residual; // before the first sweep
 };
@@ -26,16 +27,23 @@ hipError_t cs_launch_face_weights(hipStream_t s, const cs_params& u, const float
 hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, int halo, int threads,
                                  uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
                                  const int32_t* pin, int32_t* pout, uint32_t* metrics, const cs_solve_state* st);
+// Convergence test folded into the multi-brick sweeps (never both in one launch).
+#define CS_RES_COMPUTE 1u  // sum the residual of the launch's input pressure into partials[groups]
+#define CS_RES_DECIDE 2u   // reduce the previous launch's partials, advance the state, stop if converged
+
 // Four half sweeps per launch with `groups` workgroups of 1024 threads (one per compute unit),
 // each walking a run of the active-brick list and prefetching the next brick while it sweeps.
 hipError_t cs_launch_rbgs_multi(hipStream_t s, const cs_params& u, uint32_t nactive, uint32_t groups,
                                 uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
-                                const int32_t* pin, int32_t* pout, uint32_t* metrics, const cs_solve_state* st);
+                                const int32_t* pin, int32_t* pout, uint32_t* metrics, cs_solve_state* st,
+                                double* partials, uint32_t res_mode, uint32_t chunk_iters, uint32_t chunk_launches,
+                                double tol);
 // mode 0: initial residual (always runs); mode 1: after `iters` iterations in `launches` launches.
 // Two launches: per-brick partials into `partials` [2 * nactive], then a one-workgroup reduce.
 hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
                                      const float* wc, const float* rhs, const int32_t* p, double* partials,
                                      cs_solve_state* st, uint32_t mode, uint32_t iters, uint32_t launches,
                                      double tol);
+// Brick cells of `b` into `a`: when the state's launch count is odd (`by_state`) or always.
 hipError_t cs_launch_select_final(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
-                                  int32_t* a, const int32_t* b, const cs_solve_state* st);
+                                  int32_t* a, const int32_t* b, const cs_solve_state* st, bool by_state);

@@ -323,6 +323,81 @@ __global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* 
   if (fires) atomicAdd(&metrics[CS_M_P_CLAMP], fires);
 }
 
+// ---- residual helpers ---------------------------------------------------------------------
+// Relative residual from the sums of squares (A14: r = |b - Ap| / |b| over fluid cells).
+__device__ __forceinline__ double relative_residual(double rr, double bb) {
+  if (bb > 0.0) return sqrt(rr / bb);
+  return rr > 0.0 ? __builtin_huge_val() : 0.0;
+}
+
+__device__ __forceinline__ void finalize_state(cs_solve_state* st, double rr, double bb, uint32_t mode,
+                                               uint32_t iters, uint32_t launches, double tol) {
+  const double r = relative_residual(rr, bb);
+  st->residual = r;
+  if (mode == 0u) {
+    st->initial_residual = r;
+    st->done = r <= tol ? 1u : 0u;
+  } else {
+    st->iterations += iters;
+    st->launches += launches;
+    if (r <= tol) st->done = 1u;
+  }
+}
+
+// Sum over a wavefront in a fixed butterfly order (every lane gets the total).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Squared residual and squared right-hand side of the brick cells of a staged
+// region (the one-cell halo around the brick is staged too), summed over the
+// workgroup in a fixed order; the totals are valid in thread 0.  `s_red`
+// holds two doubles per wavefront.
+template <int H, int NT>
+__device__ __forceinline__ void brick_residual(const cs_params& u, const float4* s_wc, const float* s_rhs,
+                                               const int32_t* s_p, double* s_red, double& rr_out, double& bb_out) {
+  constexpr int R = Region<H, NT>::R;
+  constexpr int NW = NT / 64;
+  double rr = 0.0, bb = 0.0;
+  for (int j = (int)threadIdx.x; j < BR * BR * BR; j += NT) {
+    const int li = ((H + j / (BR * BR)) * R + H + (j / BR) % BR) * R + H + j % BR;
+    const float4 self = s_wc[li];
+    const float count = self.w;
+    // residual_rows order: -x, +x, -y, +y, -z, +z; unused faces carry weight 0
+    const float wn[6] = {s_wc[li - 1].x, self.x, s_wc[li - R].y, self.y, s_wc[li - R * R].z, self.z};
+    const int off[6] = {-1, 1, -R, R, -R * R, R * R};
+    double ap_n = 0.0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) ap_n += (double)wn[n] * (double)p_float(u, s_p[li + off[n]]);
+    const double b = -(double)s_rhs[li];
+    const double ap = (double)count * (double)p_float(u, s_p[li]) - ap_n;
+    if (count > 0.0f) {
+      rr += (b - ap) * (b - ap);
+      bb += b * b;
+    }
+  }
+  rr = wave_sum(rr);
+  bb = wave_sum(bb);
+  const int w = (int)threadIdx.x / 64;
+  if ((threadIdx.x & 63u) == 0u) {
+    s_red[2 * w] = rr;
+    s_red[2 * w + 1] = bb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tr = 0.0, tb = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      tr += s_red[2 * k];
+      tb += s_red[2 * k + 1];
+    }
+    rr_out = tr;
+    bb_out = tb;
+  }
+}
+
 // A fixed grid of workgroups (one per compute unit when the region fills the
 // LDS), each walking a run of the active-brick list and fetching the next
 // brick's region into registers while it sweeps the current one, so the
@@ -330,17 +405,74 @@ __global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* 
 // once per brick.  The list is split into eight contiguous runs, one per
 // group of workgroups with equal blockIdx.x % 8 (the XCD a workgroup is
 // dispatched to), so spatially neighbouring bricks share an L2.
+//
+// The convergence test rides on the sweeps (`res_mode`):
+//  CS_RES_COMPUTE  the residual of the launch's input pressure is summed from
+//                  the staged regions before sweeping; each workgroup writes
+//                  one (rr, bb) pair to partials[blockIdx.x] (zeros when it
+//                  has no brick).
+//  CS_RES_DECIDE   before anything else every workgroup sums the previous
+//                  launch's gridDim.x partials in the same fixed order, so all
+//                  reach the same verdict; workgroup 0 advances the solve
+//                  state by (chunk_iters, chunk_launches).  When the residual
+//                  meets `tol` the launch writes nothing: the converged
+//                  pressure stays in its input's input buffer, which the
+//                  launch count now points at.
 template <int H, int NT>
 __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* __restrict__ list, uint32_t nactive,
                                                    uint32_t color0, const float4* __restrict__ wc,
                                                    const float* __restrict__ rhs, const int32_t* __restrict__ pin,
                                                    int32_t* __restrict__ pout, uint32_t* __restrict__ metrics,
-                                                   const cs_solve_state* __restrict__ st) {
+                                                   cs_solve_state* __restrict__ st, double* __restrict__ partials,
+                                                   uint32_t res_mode, uint32_t chunk_iters, uint32_t chunk_launches,
+                                                   double tol) {
   using Reg = Region<H, NT>;
+  constexpr int NW = NT / 64;
   __shared__ float4 s_wc[Reg::R3];
   __shared__ float s_rhs[Reg::R3];
   __shared__ int32_t s_p[Reg::R3];
-  const uint32_t done = st->done;
+  __shared__ double s_red[2 * NW];
+  __shared__ uint32_t s_stop;
+  if (st->done) return;
+  if (res_mode & CS_RES_DECIDE) {
+    double rr = 0.0, bb = 0.0;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += NT) {
+      rr += partials[2 * i];
+      bb += partials[2 * i + 1];
+    }
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    const int w = (int)threadIdx.x / 64;
+    if ((threadIdx.x & 63u) == 0u) {
+      s_red[2 * w] = rr;
+      s_red[2 * w + 1] = bb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tr = 0.0, tb = 0.0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        tr += s_red[2 * k];
+        tb += s_red[2 * k + 1];
+      }
+      const double r = relative_residual(tr, tb);
+      const uint32_t stop = r <= tol ? 1u : 0u;
+      if (blockIdx.x == 0) {
+        st->residual = r;
+        st->iterations += chunk_iters;
+        st->launches += chunk_launches;
+        // the computing launch's sweeps count only if the solve goes on past them
+        const uint32_t pf = st->pending_fires;
+        if (!stop && pf) atomicAdd(&metrics[CS_M_P_CLAMP], pf);
+        st->pending_fires = 0u;
+        if (stop) st->done = 1u;
+      }
+      s_stop = stop;
+    }
+    __syncthreads();
+    if (s_stop) return;
+  }
+  const bool compute = (res_mode & CS_RES_COMPUTE) != 0u;
   // this workgroup's bricks: run r = blockIdx.x % 8 of the list, every `per_run`-th from `slot`
   const uint32_t runs = gridDim.x >= 8u ? 8u : 1u;
   const uint32_t run = blockIdx.x % runs;
@@ -349,7 +481,13 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
   const uint32_t run_lo = (uint32_t)(((uint64_t)nactive * run) / runs);
   const uint32_t run_hi = (uint32_t)(((uint64_t)nactive * (run + 1u)) / runs);
   uint32_t b = run_lo + slot;
-  if (done || b >= run_hi) return;
+  if (b >= run_hi) {
+    if (compute && threadIdx.x == 0) {
+      partials[2 * blockIdx.x] = 0.0;
+      partials[2 * blockIdx.x + 1] = 0.0;
+    }
+    return;
+  }
   int x0, y0, z0;
   brick_origin(u, list[b], x0, y0, z0);
   x0 -= H;
@@ -360,6 +498,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
   SweepMap<H, NT> map;
   map.build();
   uint32_t fires = 0;
+  double acc_rr = 0.0, acc_bb = 0.0;  // thread 0: this workgroup's residual sums
   for (;;) {
     reg.stage(s_wc, s_rhs, s_p);
     __syncthreads();
@@ -373,6 +512,15 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
       nz0 -= H;
       reg.fetch(u, nx0, ny0, nz0, wc, rhs, pin);
     }
+    if (compute) {
+      double rr = 0.0, bb = 0.0;
+      brick_residual<H, NT>(u, s_wc, s_rhs, s_p, s_red, rr, bb);
+      if (threadIdx.x == 0) {
+        acc_rr += rr;
+        acc_bb += bb;
+      }
+      // s_red is reused by the next brick only after the sweeps' barriers
+    }
     fires += sweep_region<H, NT>(u, map, color0, (uint32_t)(x0 + y0 + z0) & 1u, s_wc, s_rhs, s_p);
     store_brick<H, NT>(u, x0, y0, z0, s_p, pout);
     if (nb >= run_hi) break;
@@ -382,7 +530,11 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
     y0 = ny0;
     z0 = nz0;
   }
-  if (fires) atomicAdd(&metrics[CS_M_P_CLAMP], fires);
+  if (compute && threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = acc_rr;
+    partials[2 * blockIdx.x + 1] = acc_bb;
+  }
+  if (fires) atomicAdd(compute ? &st->pending_fires : &metrics[CS_M_P_CLAMP], fires);
 }
 
 // ---- residual over the active bricks ------------------------------------------------------
@@ -392,22 +544,6 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
 // in brick order (a fixed order) and advances the solve state.  (Electing the
 // last workgroup with an arrival counter instead serialises one device-scope
 // atomic per brick on a single address, which costs more than the launch.)
-__device__ __forceinline__ void finalize_state(cs_solve_state* st, double rr, double bb, uint32_t mode,
-                                               uint32_t iters, uint32_t launches, double tol) {
-  double r;
-  if (bb > 0.0) r = sqrt(rr / bb);
-  else r = rr > 0.0 ? __builtin_huge_val() : 0.0;
-  st->residual = r;
-  if (mode == 0u) {
-    st->initial_residual = r;
-    st->done = r <= tol ? 1u : 0u;
-  } else {
-    st->iterations += iters;
-    st->launches += launches;
-    if (r <= tol) st->done = 1u;
-  }
-}
-
 __global__ void __launch_bounds__(256) k_residual_bricks(cs_params u, const uint32_t* __restrict__ list,
                                                          const float4* __restrict__ wc, const float* __restrict__ rhs,
                                                          const int32_t* __restrict__ p, double* __restrict__ partials,
@@ -521,12 +657,14 @@ __global__ void k_residual_empty(cs_solve_state* __restrict__ st, uint32_t mode,
   finalize_state(st, 0.0, 0.0, mode, iters, launches, tol);
 }
 
-// Copies the brick cells of the second buffer back to the first when an odd
-// number of sweep launches ran, so the solved pressure always ends in `a`.
+// Copies the brick cells of the second buffer back to the first: when an odd
+// number of sweep launches ran (`by_state`: the launch count in the solve
+// state decides), or unconditionally.  Cells outside the bricks are never
+// written in the second buffer, so only brick cells are copied.
 __global__ void __launch_bounds__(256) k_select_final(cs_params u, const uint32_t* __restrict__ list,
                                                       int32_t* __restrict__ a, const int32_t* __restrict__ b,
-                                                      const cs_solve_state* __restrict__ st) {
-  if ((st->launches & 1u) == 0u) return;
+                                                      const cs_solve_state* __restrict__ st, uint32_t by_state) {
+  if (by_state && (st->launches & 1u) == 0u) return;
   int x0, y0, z0;
   brick_origin(u, list[blockIdx.x], x0, y0, z0);
   for (int j = threadIdx.x; j < BR * BR * BR; j += blockDim.x) {
@@ -542,6 +680,7 @@ __global__ void k_state_init(cs_solve_state* st) {
     st->done = 0u;
     st->iterations = 0u;
     st->launches = 0u;
+    st->pending_fires = 0u;
     st->residual = 0.0;
     st->initial_residual = 0.0;
   }
@@ -581,12 +720,16 @@ hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nac
 
 hipError_t cs_launch_rbgs_multi(hipStream_t s, const cs_params& u, uint32_t nactive, uint32_t groups,
                                 uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
-                                const int32_t* pin, int32_t* pout, uint32_t* metrics, const cs_solve_state* st) {
+                                const int32_t* pin, int32_t* pout, uint32_t* metrics, cs_solve_state* st,
+                                double* partials, uint32_t res_mode, uint32_t chunk_iters, uint32_t chunk_launches,
+                                double tol) {
   if (nactive == 0) return hipSuccess;
   if (groups >= 8u) groups &= ~7u;  // whole runs
-  if (groups == 0u) return hipErrorInvalidValue;
+  if (groups == 0u || groups > 1024u) return hipErrorInvalidValue;
+  if ((res_mode & CS_RES_COMPUTE) && (res_mode & CS_RES_DECIDE)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_rbgs_multi<4, 1024>), dim3(groups), dim3(1024), 0, s, u, list, nactive, color0,
-                     reinterpret_cast<const float4*>(wc), rhs, pin, pout, metrics, st);
+                     reinterpret_cast<const float4*>(wc), rhs, pin, pout, metrics, st, partials, res_mode,
+                     chunk_iters, chunk_launches, tol);
   return hipGetLastError();
 }
 
@@ -605,8 +748,8 @@ hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t
 }
 
 hipError_t cs_launch_select_final(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
-                                  int32_t* a, const int32_t* b, const cs_solve_state* st) {
+                                  int32_t* a, const int32_t* b, const cs_solve_state* st, bool by_state) {
   if (nactive == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_select_final, dim3(nactive), dim3(256), 0, s, u, list, a, b, st);
+  hipLaunchKernelGGL(k_select_final, dim3(nactive), dim3(256), 0, s, u, list, a, b, st, by_state ? 1u : 0u);
   return hipGetLastError();
 }

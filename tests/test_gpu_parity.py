@@ -72,8 +72,9 @@ def test_v60_small_one_substep_grid_tolerance_and_bitwise(ctx, orc):
     {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1"},
     {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_RBGS_GROUPS": "8"},
     {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_RBGS_GROUPS": "3"},
+    {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_RBGS_FUSE": "0"},
 ], ids=["dense", "brick-h1", "brick-h2-t256", "brick-h2-t512", "brick-h4-single-batch1", "brick-h4-multi",
-        "brick-h4-multi-8groups", "brick-h4-multi-3groups"])
+        "brick-h4-multi-8groups", "brick-h4-multi-3groups", "brick-h4-multi-unfused"])
 def test_solver_variants_match_oracle(ctx, orc, monkeypatch, variant):
     """Every pressure-solver layout (one thread per cell over the whole grid;
     brick-sparse with 1, 2 or 4 half sweeps per launch; the multi-brick launch
@@ -199,3 +200,28 @@ def test_v60_working_full_size_fixed_pairs(ctx, orc, monkeypatch, variant):
     sim.step_frame(w.frame_dt)
     ref.step_frame(w.frame_dt)
     compare_state(sim, ref, f"v60_working full size {variant}")
+
+
+@pytest.mark.parametrize("variant", [{}, {"CS_RBGS_GROUPS": "8"}, {"CS_RBGS_FUSE": "0"}],
+                         ids=["fused", "fused-8groups", "unfused"])
+def test_residual_test_with_clamp_fires_matches_oracle(ctx, orc, monkeypatch, variant):
+    """A short substep (dt x 0.2) drives the pressure into its clamp while the
+    solve runs to a loose tolerance: the first substep converges after 208
+    iterations with clamp fires in every sweep, so the sweeps the device runs
+    past the converged state must not count; the second stops at the cap.
+    Pressure, iteration counts and the clamp-fire counter match the oracle."""
+    for k, v in variant.items():
+        monkeypatch.setenv(k, v)
+    w = cs.SeededWorkload("v60_working", seed=3, scale=4)
+    s = w.settings.copy()
+    s.rbgs_max_iters = 400
+    s.rbgs_tolerance = 0.2
+    sim, ref = _pair(ctx, orc, s, w.scene)
+    for k in range(2):
+        sim.step_frame(w.frame_dt * 0.2)
+        ref.step_frame(w.frame_dt * 0.2)
+        sg, so = sim.solve_stats(), ref.solve_stats()
+        assert sg.iterations == so.iterations, (k, sg.as_dict(), so.as_dict())
+        assert sg.converged == so.converged
+        assert ref.read_metrics()[cs.abi.METRIC_P_CLAMP] > 0
+        compare_state(sim, ref, f"clamp fires {variant} substep {k}")
