@@ -96,6 +96,7 @@ struct cs_sim {
   uint32_t batch_chunks = 4;         // residual checks enqueued ahead of the host
   bool fuse_residual = true;         // residual test inside the multi-brick sweeps (CS_RBGS_FUSE)
   double* group_partials = nullptr;  // [2 * 1024]: per-workgroup sums of the fused test
+  uint32_t* sweep_table = nullptr;   // per-thread cell lists of the multi-brick sweeps
   // host state
   double* h_res = nullptr;    // pinned
   float* h_stage = nullptr;   // pinned emission staging
@@ -127,7 +128,7 @@ void free_sim(cs_sim* m) {
   void* dev[] = {m->part,       m->aff,        m->lanes,      m->gv,         m->gv2,    m->sdf,
                  m->cls,        m->bed_id,     m->bed,        m->kind,       m->kf,     m->pres,
                  m->rhs,        m->pack,       m->prev_p,     m->warm,       m->metrics, m->res_partials,
-                 m->res_out,    m->pres2,      m->wc,         m->group_partials,      m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
+                 m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,      m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -289,7 +290,8 @@ int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ite
           tol = check->tol;
         }
         CS_HIP(cs_launch_rbgs_multi(st, u, nactive, m->multi_groups, 0u, m->brick_list, m->wc, m->rhs, buf[*cur],
-                                    buf[*cur ^ 1], m->metrics, m->dstate, m->group_partials, mode, ci, cl, tol));
+                                    buf[*cur ^ 1], m->metrics, m->dstate, m->group_partials, mode, ci, cl, tol,
+                                    m->sweep_table));
       } else {
         CS_HIP(cs_launch_rbgs_bricks(st, u, nactive, m->halo, m->threads, 0u, m->brick_list, m->wc, m->rhs,
                                      buf[*cur], buf[*cur ^ 1], m->metrics, m->dstate));
@@ -664,6 +666,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   ALLOC(m->pres2, m->nc);
   ALLOC(m->wc, m->nc * 4);
   ALLOC(m->group_partials, 2 * 1024);
+  ALLOC(m->sweep_table, CS_SWEEP_TABLE_WORDS);
   ALLOC(m->brick_flag, geom.nbricks);
   ALLOC(m->brick_list, geom.nbricks);
   ALLOC(m->brick_partials, 2 * (size_t)geom.nbricks);
@@ -713,7 +716,11 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
     return fail(e == hipErrorOutOfMemory ? CS_ERR_NOMEM : CS_ERR_DEVICE, std::string("allocation: ") + hipGetErrorString(e));
   }
   hipStream_t st = ctx->stream;
-  e = hipMemcpyAsync(m->sdf, sdf_src, nsdf * sizeof(float), hipMemcpyHostToDevice, st);
+  std::vector<uint32_t> sweep_table(CS_SWEEP_TABLE_WORDS);
+  cs_build_sweep_table(sweep_table.data());
+  e = hipMemcpyAsync(m->sweep_table, sweep_table.data(), sweep_table.size() * sizeof(uint32_t),
+                     hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->sdf, sdf_src, nsdf * sizeof(float), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(m->cls, cls_src, m->nc, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(m->bed_id, bed_index.data(), m->nc * sizeof(int32_t), hipMemcpyHostToDevice, st);
   if (e == hipSuccess && !bed_table.empty())
@@ -721,6 +728,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (e == hipSuccess) e = hipMemsetAsync(m->gv, 0, m->nc * 16, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->gv2, 0, m->nc * 16, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->pres, 0, m->nc * 4, st);
+  if (e == hipSuccess) e = hipMemsetAsync(m->pres2, 0, m->nc * 4, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->kind, 0, m->nc, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->lanes, 0, m->nc * 24, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->metrics, 0, CS_METRICS_SLOTS * 4, st);

@@ -33,11 +33,16 @@ hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nac
 
 // Four half sweeps per launch with `groups` workgroups of 1024 threads (one per compute unit),
 // each walking a run of the active-brick list and prefetching the next brick while it sweeps.
+// `sweep_table` holds CS_SWEEP_TABLE_WORDS codes from cs_build_sweep_table.
+#define CS_MULTI_HALO 4
+#define CS_MULTI_THREADS 1024
+#define CS_SWEEP_TABLE_WORDS (2 * CS_MULTI_HALO * CS_MULTI_THREADS)
+void cs_build_sweep_table(uint32_t* out /* CS_SWEEP_TABLE_WORDS */);
 hipError_t cs_launch_rbgs_multi(hipStream_t s, const cs_params& u, uint32_t nactive, uint32_t groups,
                                 uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
                                 const int32_t* pin, int32_t* pout, uint32_t* metrics, cs_solve_state* st,
                                 double* partials, uint32_t res_mode, uint32_t chunk_iters, uint32_t chunk_launches,
-                                double tol);
+                                double tol, const uint32_t* sweep_table);
 // mode 0: initial residual (always runs); mode 1: after `iters` iterations in `launches` launches.
 // Two launches: per-brick partials into `partials` [2 * nactive], then a one-workgroup reduce.
 hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,

@@ -18,6 +18,9 @@
 // the device: each sweep launch and residual pass returns at once when the
 // solve-state flag says done, so the host can enqueue ahead without reading
 // the residual back every chunk.
+#include <algorithm>
+#include <cstdlib>
+
 #include "cs_device.h"
 #include "cs_solver.h"
 
@@ -279,6 +282,38 @@ __device__ __forceinline__ uint32_t sweep_region(const cs_params& u, const Sweep
   return fires;
 }
 
+// Half sweeps over precomputed cell lists.  After half sweep s only the cells
+// within L1 distance H-1-s of the brick still matter (one neighbour step per
+// half sweep), so each sweep visits that diamond-shaped shell of the region
+// instead of the whole cube: for H = 4 the four colour lists hold 980, 688,
+// 448 and 256 cells, one per thread of a 1024-thread workgroup, where the
+// cube needs 1372 + 864 + 500 + 256.  codes[s] = li | inner << 17 | valid << 19
+// for this thread's cell of half sweep s (cs_build_sweep_table).
+template <int H, int NT>
+__device__ __forceinline__ uint32_t sweep_listed(const cs_params& u, const uint32_t (&codes)[H], const float4* s_wc,
+                                                 const float* s_rhs, int32_t* s_p) {
+  constexpr int R = BR + 2 * H;
+  uint32_t fires = 0;
+#pragma unroll
+  for (int s = 0; s < H; ++s) {
+    const uint32_t code = codes[s];
+    if (code >> 19) {
+      const int li = (int)(code & 0xffffu);
+      const float4 self = s_wc[li];
+      const float srhs = s_rhs[li];
+      const int32_t pold = s_p[li];
+      bool fired;
+      const int32_t np = update_cell(u, self, srhs, s_wc[li - 1].x, s_wc[li - R].y, s_wc[li - R * R].z, s_p, li, R,
+                                     R * R, fired);
+      const bool fluid = self.w >= 0.0f;
+      s_p[li] = fluid ? np : pold;
+      fires += (fluid && fired) ? ((code >> 17) & 1u) : 0u;
+    }
+    __syncthreads();
+  }
+  return fires;
+}
+
 // Writes the brick cells of a swept region (corner x0, y0, z0) to `pout`.
 template <int H, int NT>
 __device__ __forceinline__ void store_brick(const cs_params& u, int x0, int y0, int z0, const int32_t* s_p,
@@ -425,9 +460,10 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
                                                    int32_t* __restrict__ pout, uint32_t* __restrict__ metrics,
                                                    cs_solve_state* __restrict__ st, double* __restrict__ partials,
                                                    uint32_t res_mode, uint32_t chunk_iters, uint32_t chunk_launches,
-                                                   double tol) {
+                                                   double tol, const uint32_t* __restrict__ sweep_table) {
   using Reg = Region<H, NT>;
   constexpr int NW = NT / 64;
+  static_assert(H % 2 == 0, "the listed sweeps assume an even region corner");
   __shared__ float4 s_wc[Reg::R3];
   __shared__ float s_rhs[Reg::R3];
   __shared__ int32_t s_p[Reg::R3];
@@ -495,8 +531,12 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
   z0 -= H;
   Reg reg;
   reg.fetch(u, x0, y0, z0, wc, rhs, pin);
-  SweepMap<H, NT> map;
-  map.build();
+  // this thread's cell of each half sweep: colour color0 ^ (s & 1); the region
+  // corner (brick origin - H, H even) has even parity, so local parity is colour
+  uint32_t codes[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k)
+    codes[k] = sweep_table[((color0 ^ (uint32_t)(k & 1)) * H + (uint32_t)k) * NT + threadIdx.x];
   uint32_t fires = 0;
   double acc_rr = 0.0, acc_bb = 0.0;  // thread 0: this workgroup's residual sums
   for (;;) {
@@ -521,7 +561,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
       }
       // s_red is reused by the next brick only after the sweeps' barriers
     }
-    fires += sweep_region<H, NT>(u, map, color0, (uint32_t)(x0 + y0 + z0) & 1u, s_wc, s_rhs, s_p);
+    fires += sweep_listed<H, NT>(u, codes, s_wc, s_rhs, s_p);
     store_brick<H, NT>(u, x0, y0, z0, s_p, pout);
     if (nb >= run_hi) break;
     __syncthreads();  // the store has read the LDS region before it is restaged
@@ -722,15 +762,37 @@ hipError_t cs_launch_rbgs_multi(hipStream_t s, const cs_params& u, uint32_t nact
                                 uint32_t color0, const uint32_t* list, const float* wc, const float* rhs,
                                 const int32_t* pin, int32_t* pout, uint32_t* metrics, cs_solve_state* st,
                                 double* partials, uint32_t res_mode, uint32_t chunk_iters, uint32_t chunk_launches,
-                                double tol) {
+                                double tol, const uint32_t* sweep_table) {
   if (nactive == 0) return hipSuccess;
   if (groups >= 8u) groups &= ~7u;  // whole runs
   if (groups == 0u || groups > 1024u) return hipErrorInvalidValue;
   if ((res_mode & CS_RES_COMPUTE) && (res_mode & CS_RES_DECIDE)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_rbgs_multi<4, 1024>), dim3(groups), dim3(1024), 0, s, u, list, nactive, color0,
+  hipLaunchKernelGGL((k_rbgs_multi<CS_MULTI_HALO, CS_MULTI_THREADS>), dim3(groups), dim3(CS_MULTI_THREADS), 0, s, u, list, nactive, color0,
                      reinterpret_cast<const float4*>(wc), rhs, pin, pout, metrics, st, partials, res_mode,
-                     chunk_iters, chunk_launches, tol);
+                     chunk_iters, chunk_launches, tol, sweep_table);
   return hipGetLastError();
+}
+
+void cs_build_sweep_table(uint32_t* out) {
+  constexpr int H = CS_MULTI_HALO, NT = CS_MULTI_THREADS, R = BR + 2 * H;
+  auto dist = [](int l) { return std::max(0, std::max(H - l, l - (H + BR - 1))); };
+  for (int c = 0; c < 2; ++c) {
+    for (int sw = 0; sw < H; ++sw) {
+      uint32_t* row = out + (size_t)(c * H + sw) * NT;
+      std::fill(row, row + NT, 0u);
+      int n = 0;
+      for (int lz = 0; lz < R; ++lz)
+        for (int ly = 0; ly < R; ++ly)
+          for (int lx = 0; lx < R; ++lx) {
+            const int d = dist(lx) + dist(ly) + dist(lz);
+            if (d > H - 1 - sw || ((lx + ly + lz) & 1) != c) continue;
+            const uint32_t inner = d == 0 ? 1u : 0u;
+            if (n < NT) row[n] = (uint32_t)((lz * R + ly) * R + lx) | inner << 17 | 1u << 19;
+            ++n;
+          }
+      if (n > NT) std::abort();  // cannot happen for H = 4: at most 980 cells per list
+    }
+  }
 }
 
 hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
