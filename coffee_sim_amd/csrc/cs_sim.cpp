@@ -97,6 +97,10 @@ struct cs_sim {
   bool fuse_residual = true;         // residual test inside the multi-brick sweeps (CS_RBGS_FUSE)
   double* group_partials = nullptr;  // [2 * 1024]: per-workgroup sums of the fused test
   uint32_t* sweep_table = nullptr;   // per-thread cell lists of the multi-brick sweeps
+  bool use_graphs = true;            // replay the steady-state batch as a hipGraph (CS_GRAPHS)
+  hipGraph_t batch_graph = nullptr;
+  hipGraphExec_t batch_exec = nullptr;
+  uint32_t batch_launches_per_chunk = 0;
   // host state
   double* h_res = nullptr;    // pinned
   float* h_stage = nullptr;   // pinned emission staging
@@ -128,7 +132,8 @@ void free_sim(cs_sim* m) {
   void* dev[] = {m->part,       m->aff,        m->lanes,      m->gv,         m->gv2,    m->sdf,
                  m->cls,        m->bed_id,     m->bed,        m->kind,       m->kf,     m->pres,
                  m->rhs,        m->pack,       m->prev_p,     m->warm,       m->metrics, m->res_partials,
-                 m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,      m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
+                 m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,
+                 m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -138,6 +143,8 @@ void free_sim(cs_sim* m) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : m->chunk_ev)
     if (e) (void)hipEventDestroy(e);
+  if (m->batch_exec) (void)hipGraphExecDestroy(m->batch_exec);
+  if (m->batch_graph) (void)hipGraphDestroy(m->batch_graph);
   if (m->h_stage) (void)hipHostFree(m->h_stage);
   if (m->stage_done) (void)hipEventDestroy(m->stage_done);
   for (auto& e : m->ev)
@@ -314,6 +321,48 @@ int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ite
   return CS_OK;
 }
 
+// Captures one batch of `batch_chunks` chunks of `every` iterations whose
+// residual tests are folded into the sweeps (the steady state of a solve)
+// into m->batch_exec.  Kernel arguments are the current substep's, so the
+// graph is rebuilt by every solve that uses it.  The batch has an even number
+// of launches, so the pressure buffers are back in place after each replay.
+int capture_batch(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t every, FusedCheck first) {
+  hipStream_t st = m->ctx->stream;
+  if (m->batch_exec) {
+    (void)hipGraphExecDestroy(m->batch_exec);
+    m->batch_exec = nullptr;
+  }
+  if (m->batch_graph) {
+    (void)hipGraphDestroy(m->batch_graph);
+    m->batch_graph = nullptr;
+  }
+  CS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int cur = 0;
+  FusedCheck chk = first;
+  int rc = CS_OK;
+  for (uint32_t c = 0; c < m->batch_chunks && rc == CS_OK; ++c) {
+    uint32_t launches = 0;
+    rc = enqueue_sweeps(m, u, nactive, every, &launches, &cur, &chk);
+    chk.iters = every;
+    chk.launches = launches;
+    m->batch_launches_per_chunk = launches;
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(st, &g);
+  if (rc != CS_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ee != hipSuccess) return hip_fail(ee, "hipStreamEndCapture");
+  if (cur != 0) {
+    (void)hipGraphDestroy(g);
+    return fail(CS_ERR_STATE, "captured batch has an odd number of sweep launches");
+  }
+  m->batch_graph = g;
+  CS_HIP(hipGraphInstantiate(&m->batch_exec, g, nullptr, nullptr, 0));
+  return CS_OK;
+}
+
 // Brick-sparse solve (cs_solver.hip) with the convergence test on the device.
 int solve_bricks(cs_sim* m, const cs_params& u) {
   hipStream_t st = m->ctx->stream;
@@ -376,10 +425,10 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     }
     const uint32_t every = m->s.rbgs_check_every ? m->s.rbgs_check_every : 8u;
     const uint32_t cap = m->s.rbgs_max_iters;
-    const size_t pool = m->chunk_ev.size() / 2;  // event pairs, >= 3 batches of chunks
-    uint32_t enq = 0, batch = 0, chunk_id = 0;
-    std::vector<uint32_t> batch_first;  // first chunk id of each batch
-    uint32_t harvested = 0;             // chunks whose sweep time has been added
+    const size_t pool = m->chunk_ev.size() / 2;  // event pairs, one per batch, >= 3 batches in flight
+    uint32_t enq = 0, batch = 0;
+    uint32_t harvested = 0;  // batches whose sweep time has been added
+    std::vector<uint32_t> batch_iter_end;  // iterations enqueued through each batch
     auto harvest = [&](uint32_t upto) {
       for (; harvested < upto; ++harvested) {
         size_t e = (size_t)(harvested % pool) * 2;
@@ -388,35 +437,53 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
         m->kt.rbgs_us += (double)ms * 1000.0;
       }
     };
+    // A batch of full chunks whose residual tests are all folded into the
+    // sweeps is the same launch sequence every time: capture it once per
+    // solve and replay it.
+    const bool graph_ok = m->use_graphs && m->fuse_residual && can_fuse(m, every);
+    bool graph_built = false;
     bool stop = false;
     bool pending = false;  // the last enqueued chunk still awaits its residual test
     FusedCheck prev{0u, 0u, tol};
     while (enq < cap && !stop) {
-      batch_first.push_back(chunk_id);
-      for (uint32_t c = 0; c < m->batch_chunks && enq < cap; ++c) {
-        uint32_t chunk = std::min(every, cap - enq);
-        const bool fuse = pending && m->fuse_residual && can_fuse(m, chunk);
-        if (pending && !fuse) {
-          CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
-                                           m->brick_partials, m->dstate, 1u, prev.iters, prev.launches, tol));
-        }
-        uint32_t launches = 0;
-        size_t e = (size_t)(chunk_id % pool) * 2;
-        if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e], st));
-        int rc = enqueue_sweeps(m, u, nactive, chunk, &launches, &cur, fuse ? &prev : nullptr);
+      const size_t e = (size_t)(batch % pool) * 2;
+      if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e], st));
+      const bool steady = graph_ok && pending && cur == 0 && prev.iters == every &&
+                          cap - enq >= m->batch_chunks * every;
+      if (steady && !graph_built) {
+        int rc = capture_batch(m, u, nactive, every, prev);
         if (rc) return rc;
-        if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e + 1], st));
-        ++chunk_id;
-        pending = true;
-        prev.iters = chunk;
-        prev.launches = launches;
-        enq += chunk;
+        graph_built = true;
       }
+      if (steady) {
+        CS_HIP(hipGraphLaunch(m->batch_exec, st));
+        enq += m->batch_chunks * every;
+        prev.iters = every;
+        prev.launches = m->batch_launches_per_chunk;
+      } else {
+        for (uint32_t c = 0; c < m->batch_chunks && enq < cap; ++c) {
+          uint32_t chunk = std::min(every, cap - enq);
+          const bool fuse = pending && m->fuse_residual && can_fuse(m, chunk);
+          if (pending && !fuse) {
+            CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
+                                             m->brick_partials, m->dstate, 1u, prev.iters, prev.launches, tol));
+          }
+          uint32_t launches = 0;
+          int rc = enqueue_sweeps(m, u, nactive, chunk, &launches, &cur, fuse ? &prev : nullptr);
+          if (rc) return rc;
+          pending = true;
+          prev.iters = chunk;
+          prev.launches = launches;
+          enq += chunk;
+        }
+      }
+      if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e + 1], st));
+      batch_iter_end.push_back(enq);
       CS_HIP(hipMemcpyAsync(&m->h_done[batch & 1u], &m->dstate->done, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
       CS_HIP(hipEventRecord(m->batch_ev[batch & 1u], st));
       if (batch > 0) {
         CS_HIP(hipEventSynchronize(m->batch_ev[(batch - 1) & 1u]));
-        if (m->timing) harvest(batch_first[batch]);
+        if (m->timing) harvest(batch);
         if (m->h_done[(batch - 1) & 1u]) stop = true;
       }
       ++batch;
@@ -428,9 +495,10 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
     CS_HIP(hipStreamSynchronize(st));
     if (m->timing) {
-      // chunks after the converged one returned at once; count only executed ones
-      uint32_t executed = (m->h_state->iterations + every - 1) / every;
-      if (executed > harvested) harvest(std::min(executed, chunk_id));
+      // batches after the converged one returned at once; count the executed ones
+      uint32_t executed = 0;
+      while (executed < batch && (executed == 0 || batch_iter_end[executed - 1] < m->h_state->iterations)) ++executed;
+      if (executed > harvested) harvest(executed);
     }
     const cs_solve_state& hs = *m->h_state;
     S.iterations = hs.iterations;
@@ -680,6 +748,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   }
   if (const char* v = getenv("CS_RBGS_MULTI")) m->multi = atoi(v) != 0;
   if (const char* v = getenv("CS_RBGS_FUSE")) m->fuse_residual = atoi(v) != 0;
+  if (const char* v = getenv("CS_GRAPHS")) m->use_graphs = atoi(v) != 0;
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->ctx->device) == hipSuccess && cus > 0)

@@ -73,13 +73,18 @@ def test_v60_small_one_substep_grid_tolerance_and_bitwise(ctx, orc):
     {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_RBGS_GROUPS": "8"},
     {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_RBGS_GROUPS": "3"},
     {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_RBGS_FUSE": "0"},
+    {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_GRAPHS": "0"},
+    {"CS_RBGS_HALO": "4", "CS_RBGS_MULTI": "1", "CS_RBGS_BATCH": "2"},
 ], ids=["dense", "brick-h1", "brick-h2-t256", "brick-h2-t512", "brick-h4-single-batch1", "brick-h4-multi",
-        "brick-h4-multi-8groups", "brick-h4-multi-3groups", "brick-h4-multi-unfused"])
+        "brick-h4-multi-8groups", "brick-h4-multi-3groups", "brick-h4-multi-unfused", "brick-h4-multi-nographs",
+        "brick-h4-multi-batch2"])
 def test_solver_variants_match_oracle(ctx, orc, monkeypatch, variant):
     """Every pressure-solver layout (one thread per cell over the whole grid;
     brick-sparse with 1, 2 or 4 half sweeps per launch; the multi-brick launch
-    with few workgroups, so each walks several bricks and prefetches) gives
-    the oracle's pressure, iteration count and particles bit for bit."""
+    with few workgroups, so each walks several bricks and prefetches; with and
+    without the residual test folded into the sweeps and the steady-state
+    batch replayed as a graph) gives the oracle's pressure, iteration count
+    and particles bit for bit."""
     for k, v in variant.items():
         monkeypatch.setenv(k, v)
     w = cs.SeededWorkload("v60_working", seed=3, scale=4)
