@@ -197,4 +197,70 @@ __device__ float volume_target_div(const cs_params& u, float rest, float cur) {
   return cs_max(target, over * u.j_alpha);
 }
 
+// ---- P2G per-particle arithmetic (shader.rs:955-1052) ---------------------------
+// Shared by every P2G layout so they compute the same fixed-point terms.
+struct P2GParticle {
+  int bx, by, bz;           // base node floor(g - 0.5)
+  cs_f3 fx;                 // g - base
+  float wx[3], wy[3], wz[3];
+  cs_f3 ac0, ac1, ac2, mv;  // m * C columns, m * v
+  float mass_p, rpv, cpv;   // mass, rest volume, current volume
+};
+
+// Loads particle `pid`; false when it does not scatter (bed phase or below the
+// active-mass threshold, shader.rs:967).
+__device__ __forceinline__ bool p2g_setup(const cs_params& u, const float4* __restrict__ part,
+                                          const float4* __restrict__ aff, uint32_t pid, P2GParticle& q) {
+  const float4 P = part[2 * pid], Vv = part[2 * pid + 1];
+  const float4 A0 = aff[3 * pid], A1 = aff[3 * pid + 1], A2 = aff[3 * pid + 2];
+  const float mass_p = Vv.w, J = P.w, phase = A0.w;
+  if (!(phase < 0.5f) || mass_p <= u.nominal_mass * 0.10f) return false;
+  const cs_f3 gp = (f3(P.x, P.y, P.z) - f3(u.origin[0], u.origin[1], u.origin[2])) * u.inv_dx;
+  q.bx = cs_f2i(floorf(gp.x - 0.5f));
+  q.by = cs_f2i(floorf(gp.y - 0.5f));
+  q.bz = cs_f2i(floorf(gp.z - 0.5f));
+  const cs_f3 fx = f3(gp.x - (float)q.bx, gp.y - (float)q.by, gp.z - (float)q.bz);
+  q.fx = fx;
+  q.wx[0] = 0.5f * (1.5f - fx.x) * (1.5f - fx.x);
+  q.wx[1] = 0.75f - (fx.x - 1.0f) * (fx.x - 1.0f);
+  q.wx[2] = 0.5f * (fx.x - 0.5f) * (fx.x - 0.5f);
+  q.wy[0] = 0.5f * (1.5f - fx.y) * (1.5f - fx.y);
+  q.wy[1] = 0.75f - (fx.y - 1.0f) * (fx.y - 1.0f);
+  q.wy[2] = 0.5f * (fx.y - 0.5f) * (fx.y - 0.5f);
+  q.wz[0] = 0.5f * (1.5f - fx.z) * (1.5f - fx.z);
+  q.wz[1] = 0.75f - (fx.z - 1.0f) * (fx.z - 1.0f);
+  q.wz[2] = 0.5f * (fx.z - 0.5f) * (fx.z - 0.5f);
+  q.ac0 = f3(mass_p * A0.x, mass_p * A0.y, mass_p * A0.z);
+  q.ac1 = f3(mass_p * A1.x, mass_p * A1.y, mass_p * A1.z);
+  q.ac2 = f3(mass_p * A2.x, mass_p * A2.y, mass_p * A2.z);
+  q.mv = f3(Vv.x, Vv.y, Vv.z) * mass_p;
+  q.mass_p = mass_p;
+  q.rpv = u.p_vol * mass_p / cs_max(u.nominal_mass, 1e-6f);
+  q.cpv = q.rpv * cs_clamp(J, 0.40f, 2.00f);
+  return true;
+}
+
+// The six fixed-point terms of node (base + (i, j, k)): mass, momentum x/y/z,
+// rest volume, current volume (shader.rs:1013-1048).  Returns 1 when a term
+// exceeds the overflow probe's limit.
+__device__ __forceinline__ uint32_t p2g_node(const cs_params& u, const P2GParticle& q, int i, int j, int k,
+                                             int32_t v[6]) {
+  const float w = q.wx[i] * q.wy[j] * q.wz[k];
+  const cs_f3 dpos = (f3((float)i, (float)j, (float)k) - q.fx) * u.dx;
+  const cs_f3 mom = (q.mv + f3(dot3(q.ac0, dpos), dot3(q.ac1, dpos), dot3(q.ac2, dpos))) * w;
+  const float fp = u.fp_scale;
+  const float mfp = (w * q.mass_p) * fp, rfp = (w * q.rpv) * fp, cfp = (w * q.cpv) * fp;
+  const float mxf = mom.x * fp, myf = mom.y * fp, mzf = mom.z * fp;
+  const float lim = 1.0e9f;
+  const uint32_t over = (fabsf(mfp) > lim || fabsf(mxf) > lim || fabsf(myf) > lim || fabsf(mzf) > lim ||
+                         fabsf(rfp) > lim || fabsf(cfp) > lim) ? 1u : 0u;
+  v[0] = cs_f2i(mfp);
+  v[1] = cs_f2i(mxf);
+  v[2] = cs_f2i(myf);
+  v[3] = cs_f2i(mzf);
+  v[4] = cs_f2i(rfp);
+  v[5] = cs_f2i(cfp);
+  return over;
+}
+
 }  // namespace

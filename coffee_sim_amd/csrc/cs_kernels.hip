@@ -24,58 +24,27 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, size_t n, int32_t v) {
 }
 
 // ---- P2G (shader.rs:955-1052), one thread per particle, global i32 atomics -------
+// The layout of the reference dispatch; the tiled P2G of cs_p2g.hip is the
+// default and gives the same lanes bit for bit (CS_P2G=atomic selects this).
 __global__ void __launch_bounds__(256) k_p2g_atomic(cs_params u, const float4* __restrict__ part,
                                                     const float4* __restrict__ aff, int32_t* __restrict__ lanes,
                                                     uint32_t* __restrict__ metrics) {
   uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
   if (pid >= u.num_particles) return;
-  float4 P = part[2 * pid], Vv = part[2 * pid + 1];
-  float4 A0 = aff[3 * pid], A1 = aff[3 * pid + 1], A2 = aff[3 * pid + 2];
-  float mass_p = Vv.w, J = P.w, phase = A0.w;
-  if (!(phase < 0.5f) || mass_p <= u.nominal_mass * 0.10f) return;
-  cs_f3 gp = (f3(P.x, P.y, P.z) - f3(u.origin[0], u.origin[1], u.origin[2])) * u.inv_dx;
-  int bx = cs_f2i(floorf(gp.x - 0.5f)), by = cs_f2i(floorf(gp.y - 0.5f)), bz = cs_f2i(floorf(gp.z - 0.5f));
-  cs_f3 fx = f3(gp.x - (float)bx, gp.y - (float)by, gp.z - (float)bz);
-  float wx[3], wy[3], wz[3];
-  wx[0] = 0.5f * (1.5f - fx.x) * (1.5f - fx.x);
-  wx[1] = 0.75f - (fx.x - 1.0f) * (fx.x - 1.0f);
-  wx[2] = 0.5f * (fx.x - 0.5f) * (fx.x - 0.5f);
-  wy[0] = 0.5f * (1.5f - fx.y) * (1.5f - fx.y);
-  wy[1] = 0.75f - (fx.y - 1.0f) * (fx.y - 1.0f);
-  wy[2] = 0.5f * (fx.y - 0.5f) * (fx.y - 0.5f);
-  wz[0] = 0.5f * (1.5f - fx.z) * (1.5f - fx.z);
-  wz[1] = 0.75f - (fx.z - 1.0f) * (fx.z - 1.0f);
-  wz[2] = 0.5f * (fx.z - 0.5f) * (fx.z - 0.5f);
-  cs_f3 ac0 = f3(mass_p * A0.x, mass_p * A0.y, mass_p * A0.z);
-  cs_f3 ac1 = f3(mass_p * A1.x, mass_p * A1.y, mass_p * A1.z);
-  cs_f3 ac2 = f3(mass_p * A2.x, mass_p * A2.y, mass_p * A2.z);
-  cs_f3 mv = f3(Vv.x, Vv.y, Vv.z) * mass_p;
-  float fp = u.fp_scale;
-  float rpv = u.p_vol * mass_p / cs_max(u.nominal_mass, 1e-6f);
-  float cpv = rpv * cs_clamp(J, 0.40f, 2.00f);
+  P2GParticle q;
+  if (!p2g_setup(u, part, aff, pid, q)) return;
   uint32_t nc = u.nc;
   uint32_t overflow = 0;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 3; ++k) {
-        int cx = bx + i, cy = by + j, cz = bz + k;
+        int cx = q.bx + i, cy = q.by + j, cz = q.bz + k;
         if (!on_grid(u, cx, cy, cz)) continue;
-        float w = wx[i] * wy[j] * wz[k];
-        cs_f3 dpos = (f3((float)i, (float)j, (float)k) - fx) * u.dx;
-        cs_f3 mom = (mv + f3(dot3(ac0, dpos), dot3(ac1, dpos), dot3(ac2, dpos))) * w;
-        float mfp = (w * mass_p) * fp, rfp = (w * rpv) * fp, cfp = (w * cpv) * fp;
-        float mxf = mom.x * fp, myf = mom.y * fp, mzf = mom.z * fp;
-        const float lim = 1.0e9f;
-        if (fabsf(mfp) > lim || fabsf(mxf) > lim || fabsf(myf) > lim || fabsf(mzf) > lim || fabsf(rfp) > lim ||
-            fabsf(cfp) > lim)
-          ++overflow;
+        int32_t v[6];
+        overflow += p2g_node(u, q, i, j, k, v);
         uint32_t ci = cell_index(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
-        atomicAdd(&lanes[ci], cs_f2i(mfp));
-        atomicAdd(&lanes[nc + ci], cs_f2i(mxf));
-        atomicAdd(&lanes[2u * nc + ci], cs_f2i(myf));
-        atomicAdd(&lanes[3u * nc + ci], cs_f2i(mzf));
-        atomicAdd(&lanes[4u * nc + ci], cs_f2i(rfp));
-        atomicAdd(&lanes[5u * nc + ci], cs_f2i(cfp));
+#pragma unroll
+        for (int l = 0; l < 6; ++l) atomicAdd(&lanes[(size_t)l * nc + ci], v[l]);
       }
   if (overflow) atomicAdd(&metrics[CS_M_OVERFLOW], overflow);
 }

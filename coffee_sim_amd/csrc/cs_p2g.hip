@@ -1,0 +1,178 @@
+// cs_p2g.hip — tile-binned P2G: particles sorted into 8^3-cell tiles, one
+// workgroup per tile accumulating its node footprint in LDS, one flush to HBM.
+//
+// Restates p2g (shader.rs:955-1052).  The reference adds every particle's 162
+// fixed-point terms (27 nodes x 6 lanes) to the grid with global i32 atomics.
+// Here each active particle is binned by the tile of its base node
+// floor(g - 0.5): a wave-aggregated atomic on the tile counter gives its rank
+// in the tile, a one-block scan turns counts into offsets, and a scatter
+// writes the tile-sorted particle list.  One workgroup per tile then adds the
+// terms of its particles into a 10^3-node x 6-lane LDS block (nodes base-tile
+// origin + [0, 10) in each axis) and flushes the nonzero entries with one
+// global i32 atomic each: neighbouring tiles share two node layers per axis,
+// so the flush adds rather than stores.  Integer addition is associative, so
+// the lanes equal the reference's bit for bit whatever the order.
+#include "cs_common.h"
+#include "cs_kernels.h"
+
+#include "cs_device.h"
+
+namespace {
+
+constexpr int TB = CS_BRICK;      // tile side in cells (the solver's brick)
+constexpr int FP = TB + 2;        // footprint side in nodes
+constexpr int FP3 = FP * FP * FP;  // 1000 nodes
+constexpr uint32_t kNoTile = 0xffffffffu;
+
+// Tile of a particle's base node; bases one cell outside the grid (the
+// clamped domain reaches g = -0.5 .. n + 0.5) go to the edge tile.
+__device__ __forceinline__ uint32_t tile_of(const cs_params& u, int bx, int by, int bz) {
+  const uint32_t tx = (uint32_t)min(max(bx, 0), (int)u.gx - 1) / TB;
+  const uint32_t ty = (uint32_t)min(max(by, 0), (int)u.gy - 1) / TB;
+  const uint32_t tz = (uint32_t)min(max(bz, 0), (int)u.gz - 1) / TB;
+  return (tz * u.nby + ty) * u.nbx + tx;
+}
+
+}  // namespace
+
+// Pass 1: tile and in-tile rank of every particle.  Lanes of a wavefront that
+// share a tile take one atomic between them (the lowest such lane adds the
+// group's count and hands out consecutive ranks).
+__global__ void __launch_bounds__(256) k_p2g_bin(cs_params u, const float4* __restrict__ part,
+                                                 const float4* __restrict__ aff, uint32_t* __restrict__ tile_count,
+                                                 uint2* __restrict__ key) {
+  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = false;
+  uint32_t tile = kNoTile;
+  if (pid < u.num_particles) {
+    P2GParticle q;
+    active = p2g_setup(u, part, aff, pid, q);
+    if (active) tile = tile_of(u, q.bx, q.by, q.bz);
+  }
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+  uint64_t todo = __ballot(active);
+  uint32_t rank = 0;
+  while (todo) {
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
+    const uint32_t lt = __shfl(tile, (int)leader, 64);
+    const uint64_t group = __ballot(active && tile == lt);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&tile_count[lt], (uint32_t)__popcll(group));
+    base = __shfl(base, (int)leader, 64);
+    if (active && tile == lt) rank = base + (uint32_t)__popcll(group & below);
+    todo &= ~group;
+  }
+  if (pid < u.num_particles) key[pid] = make_uint2(tile, rank);
+}
+
+// Pass 2: exclusive scan of the tile counts (one workgroup).
+__global__ void __launch_bounds__(1024) k_p2g_scan(const uint32_t* __restrict__ count, uint32_t ntiles,
+                                                   uint32_t* __restrict__ start) {
+  __shared__ uint32_t s_sum[1024];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (ntiles + 1023u) / 1024u;
+  const uint32_t b0 = min(t * per, ntiles), b1 = min(b0 + per, ntiles);
+  uint32_t c = 0;
+  for (uint32_t b = b0; b < b1; ++b) c += count[b];
+  s_sum[t] = c;
+  __syncthreads();
+  for (uint32_t d = 1; d < 1024u; d <<= 1) {
+    const uint32_t v = t >= d ? s_sum[t - d] : 0u;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = s_sum[t] - c;
+  for (uint32_t b = b0; b < b1; ++b) {
+    start[b] = run;
+    run += count[b];
+  }
+}
+
+// Pass 3: the tile-sorted particle list.
+__global__ void __launch_bounds__(256) k_p2g_scatter(uint32_t n, const uint2* __restrict__ key,
+                                                     const uint32_t* __restrict__ start, uint32_t* __restrict__ sorted) {
+  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= n) return;
+  const uint2 k = key[pid];
+  if (k.x != kNoTile) sorted[start[k.x] + k.y] = pid;
+}
+
+// Pass 4: one workgroup per tile.  The tile's particles are dealt to threads
+// in a strided order (thread t takes list positions t*K, t*K + 1, ...), so the
+// lanes of one wavefront work on particles far apart in the list and rarely
+// hit the same LDS word in one instruction.
+__global__ void __launch_bounds__(256) k_p2g_tile(cs_params u, const float4* __restrict__ part,
+                                                  const float4* __restrict__ aff,
+                                                  const uint32_t* __restrict__ tile_count,
+                                                  const uint32_t* __restrict__ start,
+                                                  const uint32_t* __restrict__ sorted, int32_t* __restrict__ lanes,
+                                                  uint32_t* __restrict__ metrics) {
+  constexpr uint32_t NT = 256;
+  __shared__ int32_t s_acc[6 * FP3];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t n = tile_count[tile];
+  if (n == 0) return;
+  const uint32_t first = start[tile];
+  for (uint32_t i = threadIdx.x; i < 6u * FP3; i += NT) s_acc[i] = 0;
+  const uint32_t tx = tile % u.nbx, ty = (tile / u.nbx) % u.nby, tz = tile / (u.nbx * u.nby);
+  const int ox = (int)(tx * TB), oy = (int)(ty * TB), oz = (int)(tz * TB);
+  __syncthreads();
+  const uint32_t per = (n + NT - 1) / NT;
+  uint32_t overflow = 0;
+  for (uint32_t r = 0; r < per; ++r) {
+    const uint32_t pos = threadIdx.x * per + r;
+    if (pos >= n) break;
+    const uint32_t pid = sorted[first + pos];
+    P2GParticle q;
+    p2g_setup(u, part, aff, pid, q);  // binned particles are active
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          const int cx = q.bx + i, cy = q.by + j, cz = q.bz + k;
+          if (!on_grid(u, cx, cy, cz)) continue;
+          int32_t v[6];
+          overflow += p2g_node(u, q, i, j, k, v);
+          const int lx = cx - ox, ly = cy - oy, lz = cz - oz;
+          if ((uint32_t)lx < (uint32_t)FP && (uint32_t)ly < (uint32_t)FP && (uint32_t)lz < (uint32_t)FP) {
+            const int li = (lz * FP + ly) * FP + lx;
+#pragma unroll
+            for (int l = 0; l < 6; ++l) atomicAdd(&s_acc[l * FP3 + li], v[l]);
+          } else {
+            // a node outside the footprint (a particle far outside the domain): straight to HBM
+            const uint32_t ci = cell_index(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+#pragma unroll
+            for (int l = 0; l < 6; ++l) atomicAdd(&lanes[(size_t)l * u.nc + ci], v[l]);
+          }
+        }
+  }
+  __syncthreads();
+  for (uint32_t li = threadIdx.x; li < (uint32_t)FP3; li += NT) {
+    const int gx = ox + (int)(li % FP), gy = oy + (int)((li / FP) % FP), gz = oz + (int)(li / (FP * FP));
+    if (!on_grid(u, gx, gy, gz)) continue;
+    const uint32_t ci = cell_index(u, (uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      const int32_t v = s_acc[l * FP3 + li];
+      if (v != 0) atomicAdd(&lanes[(size_t)l * u.nc + ci], v);
+    }
+  }
+  if (overflow) atomicAdd(&metrics[CS_M_OVERFLOW], overflow);
+}
+
+hipError_t cs_launch_p2g_tiled(hipStream_t s, const cs_params& u, const float* part, const float* aff,
+                               int32_t* lanes, uint32_t* metrics, uint32_t* tile_count, uint32_t* tile_start,
+                               uint2* key, uint32_t* sorted) {
+  if (u.num_particles == 0) return hipSuccess;
+  const dim3 pb((u.num_particles + 255u) / 256u);
+  hipError_t e = cs_launch_fill_i32(s, reinterpret_cast<int32_t*>(tile_count), u.nbricks, 0);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_p2g_bin, pb, dim3(256), 0, s, u, reinterpret_cast<const float4*>(part),
+                     reinterpret_cast<const float4*>(aff), tile_count, key);
+  hipLaunchKernelGGL(k_p2g_scan, dim3(1), dim3(1024), 0, s, tile_count, u.nbricks, tile_start);
+  hipLaunchKernelGGL(k_p2g_scatter, pb, dim3(256), 0, s, u.num_particles, key, tile_start, sorted);
+  hipLaunchKernelGGL(k_p2g_tile, dim3(u.nbricks), dim3(256), 0, s, u, reinterpret_cast<const float4*>(part),
+                     reinterpret_cast<const float4*>(aff), tile_count, tile_start, sorted, lanes, metrics);
+  return hipGetLastError();
+}

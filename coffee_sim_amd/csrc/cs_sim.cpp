@@ -101,6 +101,12 @@ struct cs_sim {
   hipGraph_t batch_graph = nullptr;
   hipGraphExec_t batch_exec = nullptr;
   uint32_t batch_launches_per_chunk = 0;
+  // tile-binned P2G scratch
+  bool p2g_tiled = true;             // CS_P2G=atomic selects the one-thread-per-particle layout
+  uint32_t* tile_count = nullptr;    // [nbricks]
+  uint32_t* tile_start = nullptr;    // [nbricks]
+  uint2* p2g_key = nullptr;          // [max_particles]
+  uint32_t* p2g_sorted = nullptr;    // [max_particles]
   // host state
   double* h_res = nullptr;    // pinned
   float* h_stage = nullptr;   // pinned emission staging
@@ -133,7 +139,8 @@ void free_sim(cs_sim* m) {
                  m->cls,        m->bed_id,     m->bed,        m->kind,       m->kf,     m->pres,
                  m->rhs,        m->pack,       m->prev_p,     m->warm,       m->metrics, m->res_partials,
                  m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,
-                 m->brick_flag, m->brick_list, m->brick_partials, m->dstate};
+                 m->brick_flag, m->brick_list, m->brick_partials, m->dstate,       m->tile_count,
+                 m->tile_start, m->p2g_key,    m->p2g_sorted};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -547,7 +554,11 @@ int substep(cs_sim* m, float sub_dt) {
   CS_HIP(cs_launch_fill_i32(st, m->lanes, m->nc * 6, 0));
   CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->metrics), CS_METRICS_SLOTS, 0));
   kmark(m, 0);
-  CS_HIP(cs_launch_p2g(st, u, m->part, m->aff, m->lanes, m->metrics));
+  if (m->p2g_tiled)
+    CS_HIP(cs_launch_p2g_tiled(st, u, m->part, m->aff, m->lanes, m->metrics, m->tile_count, m->tile_start,
+                               m->p2g_key, m->p2g_sorted));
+  else
+    CS_HIP(cs_launch_p2g(st, u, m->part, m->aff, m->lanes, m->metrics));
   kmark(m, 1);
   mark(m, P_GRID);
   CS_HIP(cs_launch_grid(st, u, m->lanes, m->gv, m->sdf));
@@ -735,6 +746,10 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   ALLOC(m->wc, m->nc * 4);
   ALLOC(m->group_partials, 2 * 1024);
   ALLOC(m->sweep_table, CS_SWEEP_TABLE_WORDS);
+  ALLOC(m->tile_count, geom.nbricks);
+  ALLOC(m->tile_start, geom.nbricks);
+  ALLOC(m->p2g_key, mp);
+  ALLOC(m->p2g_sorted, mp);
   ALLOC(m->brick_flag, geom.nbricks);
   ALLOC(m->brick_list, geom.nbricks);
   ALLOC(m->brick_partials, 2 * (size_t)geom.nbricks);
@@ -749,6 +764,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (const char* v = getenv("CS_RBGS_MULTI")) m->multi = atoi(v) != 0;
   if (const char* v = getenv("CS_RBGS_FUSE")) m->fuse_residual = atoi(v) != 0;
   if (const char* v = getenv("CS_GRAPHS")) m->use_graphs = atoi(v) != 0;
+  if (const char* v = getenv("CS_P2G")) m->p2g_tiled = strcmp(v, "atomic") != 0;
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->ctx->device) == hipSuccess && cus > 0)

@@ -230,3 +230,38 @@ def test_residual_test_with_clamp_fires_matches_oracle(ctx, orc, monkeypatch, va
         assert sg.converged == so.converged
         assert ref.read_metrics()[cs.abi.METRIC_P_CLAMP] > 0
         compare_state(sim, ref, f"clamp fires {variant} substep {k}")
+
+
+@pytest.mark.parametrize("variant", [{}, {"CS_P2G": "atomic"}], ids=["tiled", "atomic"])
+def test_p2g_lanes_at_domain_edges_match_oracle(ctx, orc, monkeypatch, variant):
+    """P2G over a 24^3 box (3 x 3 x 3 tiles) with particles spread over the
+    whole domain including the half-cell margins the position clamp allows
+    (base nodes -1 and n), random velocities and affine matrices, and bed-phase
+    and below-threshold particles that must not scatter: the six lanes after
+    the scatter, and the whole substep, match the oracle bit for bit."""
+    from scenes import SceneArrays, box_settings
+
+    for k, v in variant.items():
+        monkeypatch.setenv(k, v)
+    s = box_settings(n=24)
+    n = 24
+    side = float(s.bounds_size[0])
+    dx = side / n
+    rng = np.random.default_rng(11)
+    m = 40000
+    pos = rng.uniform(-0.5 * side - 0.5 * dx, 0.5 * side + 0.5 * dx, size=(m, 3))
+    vel = rng.normal(0.0, 2.0, size=(m, 3))
+    mass = np.full(m, 0.25)
+    mass[::97] = 0.01  # below the active-mass threshold
+    part = np.concatenate([pos, np.ones((m, 1)), vel, mass[:, None]], axis=1).astype(np.float32)
+    aff = rng.normal(0.0, 3.0, size=(m, 12)).astype(np.float32)
+    aff[:, 3] = 0.0
+    aff[::53, 3] = 1.0  # bed phase
+    aff[:, 7] = 0.0
+    aff[:, 11] = 0.0
+    scene = SceneArrays(part, aff)
+    sim, ref = _pair(ctx, orc, s, scene.scene)
+    sim.step_frame(1.0 / 600.0)
+    ref.step_frame(1.0 / 600.0)
+    assert_bitwise("p2g lanes", sim.read_p2g_lanes(), ref.read_p2g_lanes())
+    compare_state(sim, ref, f"edge P2G {variant}")
