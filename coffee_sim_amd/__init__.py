@@ -91,6 +91,25 @@ def _bind(lib: C.CDLL) -> None:
         "cs_device_expf": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_uint32]),
         "cs_sim_get_kernel_timing": (C.c_int, [C.c_void_p, P(abi.KernelTiming)]),
         "cs_sim_reset_kernel_timing": (C.c_int, [C.c_void_p]),
+        "cs_unet_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, P(C.c_void_p)]),
+        "cs_unet_destroy": (C.c_int, [C.c_void_p]),
+        "cs_unet_param_count": (C.c_int, [C.c_void_p, P(C.c_uint32)]),
+        "cs_unet_get_params": (C.c_int, [C.c_void_p, P(C.c_float)]),
+        "cs_unet_set_params": (C.c_int, [C.c_void_p, P(C.c_float)]),
+        "cs_unet_set_scales": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
+        "cs_unet_get_scales": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float)]),
+        "cs_unet_forward": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float)]),
+        "cs_unet_gradient": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_uint8), P(C.c_float),
+                                       P(C.c_float)]),
+        "cs_unet_train_step": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_uint8), C.c_float,
+                                         P(C.c_float)]),
+        "cs_unet_capture": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "cs_unet_dataset_size": (C.c_int, [C.c_void_p, P(C.c_uint32)]),
+        "cs_unet_dataset_clear": (C.c_int, [C.c_void_p]),
+        "cs_unet_read_sample": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_float), P(C.c_float), P(C.c_uint8)]),
+        "cs_unet_train_epoch": (C.c_int, [C.c_void_p, C.c_float, C.c_uint64, P(C.c_float)]),
+        "cs_sim_set_warm_net": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "cs_sim_read_warm_start": (C.c_int, [C.c_void_p, P(C.c_float)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -211,6 +230,7 @@ class Context:
     def __init__(self, device: int = 0):
         self._h = C.c_void_p()
         _check(load_library().cs_ctx_create(device, C.byref(self._h)), "cs_ctx_create")
+        self._children: "weakref.WeakSet" = weakref.WeakSet()  # simulations and networks on this context
         _live_ctxs.add(self)
 
     @property
@@ -225,6 +245,9 @@ class Context:
 
     def close(self) -> None:
         if self._h:
+            # everything created on this context goes first: it uses the context's stream
+            for child in list(self._children):
+                child.close()
             load_library().cs_ctx_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -233,6 +256,113 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def _u8ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+class UNet:
+    """The learned pressure warm start (include/coffee_sim.h, cs_unet_*): a 3-D
+    U-Net on a cubic grid of `side` cells predicting the converged pressure
+    (in units of the pressure scale) from four features per cell."""
+
+    def __init__(self, ctx: Context, side: int, seed: int = 0):
+        self.ctx = ctx
+        self.side = int(side)
+        self.nv = self.side ** 3
+        self._h = C.c_void_p()
+        _check(load_library().cs_unet_create(ctx.handle, self.side, seed, C.byref(self._h)), "cs_unet_create")
+        n = C.c_uint32()
+        _check(load_library().cs_unet_param_count(self._h, C.byref(n)), "cs_unet_param_count")
+        self.param_count = int(n.value)
+        _live_sims.add(self)
+        ctx._children.add(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            load_library().cs_unet_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.param_count, np.float32)
+        _check(load_library().cs_unet_get_params(self._h, _fptr(out)), "cs_unet_get_params")
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32).reshape(-1)
+        if p.size != self.param_count:
+            raise ValueError(f"expected {self.param_count} parameters")
+        _check(load_library().cs_unet_set_params(self._h, _fptr(p)), "cs_unet_set_params")
+
+    def set_scales(self, rhs_scale: float, pressure_scale: float) -> None:
+        _check(load_library().cs_unet_set_scales(self._h, rhs_scale, pressure_scale), "cs_unet_set_scales")
+
+    def scales(self) -> tuple[float, float]:
+        a, b = C.c_float(), C.c_float()
+        _check(load_library().cs_unet_get_scales(self._h, C.byref(a), C.byref(b)), "cs_unet_get_scales")
+        return float(a.value), float(b.value)
+
+    def _features(self, f4: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(f4, np.float32).reshape(-1)
+        if f.size != self.nv * 4:
+            raise ValueError("features must hold 4 values per cell")
+        return f
+
+    def forward(self, f4: np.ndarray) -> np.ndarray:
+        out = np.empty(self.nv, np.float32)
+        _check(load_library().cs_unet_forward(self._h, _fptr(self._features(f4)), _fptr(out)), "cs_unet_forward")
+        return out.reshape((self.side,) * 3)
+
+    def gradient(self, f4: np.ndarray, target: np.ndarray, mask: np.ndarray) -> tuple[float, np.ndarray]:
+        t = np.ascontiguousarray(target, np.float32).reshape(-1)
+        m = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+        loss = C.c_float()
+        grad = np.empty(self.param_count, np.float32)
+        _check(load_library().cs_unet_gradient(self._h, _fptr(self._features(f4)), _fptr(t), _u8ptr(m),
+                                               C.byref(loss), _fptr(grad)), "cs_unet_gradient")
+        return float(loss.value), grad
+
+    def train_step(self, f4: np.ndarray, target: np.ndarray, mask: np.ndarray, lr: float) -> float:
+        t = np.ascontiguousarray(target, np.float32).reshape(-1)
+        m = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+        loss = C.c_float()
+        _check(load_library().cs_unet_train_step(self._h, _fptr(self._features(f4)), _fptr(t), _u8ptr(m), lr,
+                                                 C.byref(loss)), "cs_unet_train_step")
+        return float(loss.value)
+
+    def capture(self, sim: "MpmSim3D") -> None:
+        _check(load_library().cs_unet_capture(self._h, sim._h), "cs_unet_capture")
+
+    def dataset_size(self) -> int:
+        n = C.c_uint32()
+        _check(load_library().cs_unet_dataset_size(self._h, C.byref(n)), "cs_unet_dataset_size")
+        return int(n.value)
+
+    def clear_dataset(self) -> None:
+        _check(load_library().cs_unet_dataset_clear(self._h), "cs_unet_dataset_clear")
+
+    def read_sample(self, i: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        f = np.empty((self.nv, 4), np.float32)
+        t = np.empty(self.nv, np.float32)
+        m = np.empty(self.nv, np.uint8)
+        _check(load_library().cs_unet_read_sample(self._h, i, _fptr(f), _fptr(t), _u8ptr(m)), "cs_unet_read_sample")
+        return f, t, m
+
+    def train_epoch(self, lr: float, seed: int = 0) -> float:
+        loss = C.c_float()
+        _check(load_library().cs_unet_train_epoch(self._h, lr, seed, C.byref(loss)), "cs_unet_train_epoch")
+        return float(loss.value)
 
 
 class MpmSim3D:
@@ -248,6 +378,7 @@ class MpmSim3D:
         self.nc = self.settings.num_cells
         self.dims = tuple(int(d) for d in self.settings.grid_dims)
         _live_sims.add(self)
+        ctx._children.add(self)
 
     # ---- lifecycle -----------------------------------------------------------------
     def close(self) -> None:
@@ -361,6 +492,17 @@ class MpmSim3D:
         if p.size != self.nc:
             raise ValueError("warm start must hold one value per cell")
         _check(load_library().cs_sim_set_warm_start(self._h, _fptr(p)), "set_warm_start")
+
+    def set_warm_net(self, net: "UNet | None") -> None:
+        """Network used when the settings ask for CS_WARM_LEARNED."""
+        self._warm_net = net
+        _check(load_library().cs_sim_set_warm_net(self._h, net.handle if net is not None else None),
+               "cs_sim_set_warm_net")
+
+    def read_warm_start(self) -> np.ndarray:
+        out = np.empty(self.nc, np.float32)
+        _check(load_library().cs_sim_read_warm_start(self._h, _fptr(out)), "cs_sim_read_warm_start")
+        return out
 
     def save(self) -> bytes:
         n = C.c_size_t()

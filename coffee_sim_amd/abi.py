@@ -29,6 +29,7 @@ CS_RBGS_TOLERANCE = 1
 CS_WARM_ZERO = 0
 CS_WARM_PREVIOUS = 1
 CS_WARM_EXTERNAL = 2
+CS_WARM_LEARNED = 3
 
 CS_PRESET_DEFAULT_V60 = 0
 CS_PRESET_FREE_STREAM = 1
@@ -129,6 +130,7 @@ class Settings(C.Structure):
         ("rbgs_max_iters", C.c_uint32),
         ("warm_start", C.c_uint32),
         ("has_bed_field", C.c_uint32),
+        ("warm_fallback_residual", C.c_float),
     ]
 
     @property

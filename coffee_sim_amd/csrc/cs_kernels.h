@@ -44,3 +44,8 @@ hipError_t cs_launch_expf(hipStream_t stream, const float* in, float* out, uint3
 
 // Blocks used by the residual reduction (size of the partials buffer / 2).
 #define CS_RESIDUAL_BLOCKS 1024u
+// Features of the learned warm start ([nc][4]) and, with `target`, its training
+// target, loss mask and fluid-cell count.
+hipError_t cs_launch_unet_features(hipStream_t stream, const cs_params& u, const float* kf, const float* rhs,
+                                   const int32_t* pres, const int32_t* bed_id, const cs_bed_cell* bed, float rhs_scale,
+                                   float inv_p_scale, float* f4, float* target, uint8_t* mask, uint32_t* count);

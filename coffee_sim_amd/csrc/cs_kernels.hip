@@ -579,6 +579,38 @@ __global__ void k_expf(const float* __restrict__ in, float* __restrict__ out, ui
   if (i < n) out[i] = cs_expf(in[i]);
 }
 
+// ---- inputs and targets of the learned warm start (no reference counterpart) ---------
+// Four features per cell as the network reads them: rhs * rhs_scale and liquid
+// fill for fluid cells (0 elsewhere), the solid mask, and the bed porosity; with
+// `target`, also the converged pressure / pressure_scale and the fluid mask of
+// the loss, counting fluid cells into `count`.
+__global__ void __launch_bounds__(256) k_unet_features(cs_params u, const float* __restrict__ kf,
+                                                       const float* __restrict__ rhs, const int32_t* __restrict__ pres,
+                                                       const int32_t* __restrict__ bed_id,
+                                                       const cs_bed_cell* __restrict__ bed, float rhs_scale,
+                                                       float inv_p_scale, float4* __restrict__ f4,
+                                                       float* __restrict__ target, uint8_t* __restrict__ mask,
+                                                       uint32_t* __restrict__ count) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  bool fluid = false;
+  if (idx < u.nc) {
+    const float skf = kf[idx];
+    fluid = kf_fluid(skf);
+    const int32_t b = bed_id[idx];
+    const float poro = (b >= 0 && (uint32_t)b < u.num_bed_cells) ? bed[b].porosity : 0.0f;
+    f4[idx] = make_float4(fluid ? rhs[idx] * rhs_scale : 0.0f, fluid ? kf_fill(skf) : 0.0f,
+                          skf == CS_KF_SOLID ? 1.0f : 0.0f, poro);
+    if (target) {
+      target[idx] = fluid ? (float)pres[idx] * u.inv_fp_scale * inv_p_scale : 0.0f;
+      mask[idx] = fluid ? 1u : 0u;
+    }
+  }
+  if (target) {
+    const uint64_t bal = __ballot(fluid);
+    if ((threadIdx.x & 63u) == 0u && bal) atomicAdd(count, (uint32_t)__popcll(bal));
+  }
+}
+
 // ---- launchers -------------------------------------------------------------------------
 namespace {
 constexpr uint32_t kBlock = 256;
@@ -669,5 +701,12 @@ hipError_t cs_launch_g2p(hipStream_t s, const cs_params& u, float* part, float* 
 hipError_t cs_launch_expf(hipStream_t s, const float* in, float* out, uint32_t n) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_expf, blocks_for(n), dim3(kBlock), 0, s, in, out, n);
+  return hipGetLastError();
+}
+hipError_t cs_launch_unet_features(hipStream_t s, const cs_params& u, const float* kf, const float* rhs,
+                                   const int32_t* pres, const int32_t* bed_id, const cs_bed_cell* bed, float rhs_scale,
+                                   float inv_p_scale, float* f4, float* target, uint8_t* mask, uint32_t* count) {
+  hipLaunchKernelGGL(k_unet_features, blocks_for(u.nc), dim3(kBlock), 0, s, u, kf, rhs, pres, bed_id, bed, rhs_scale,
+                     inv_p_scale, reinterpret_cast<float4*>(f4), target, mask, count);
   return hipGetLastError();
 }

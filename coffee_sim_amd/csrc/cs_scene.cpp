@@ -151,6 +151,7 @@ int settings_preset(int preset, cs_settings* s) {
   s->rbgs_max_iters = 20000u;
   s->warm_start = CS_WARM_ZERO;
   s->has_bed_field = 0u;
+  s->warm_fallback_residual = 1.0f;
   if (preset == CS_PRESET_FREE_STREAM) {
     sp.origin[0] = 0.0f;
     sp.origin[1] = 6.8f;
@@ -179,7 +180,8 @@ int validate_settings(const cs_settings& s, std::string* why) {
   if (s.max_particles == 0u || s.max_particles > (1u << 28)) return fail("max_particles out of range");
   if (s.num_obstacles > CS_MAX_OBSTACLES) return fail("too many obstacles");
   if (s.rbgs_mode > CS_RBGS_TOLERANCE) return fail("unknown rbgs_mode");
-  if (s.warm_start > CS_WARM_EXTERNAL) return fail("unknown warm_start");
+  if (s.warm_start > CS_WARM_LEARNED) return fail("unknown warm_start");
+  if (!(s.warm_fallback_residual > 0.0f)) return fail("warm_fallback_residual must be positive");
   if (s.emitter_mode > CS_EMIT_FIXED_COUNT) return fail("unknown emitter_mode");
   return CS_OK;
 }

@@ -12,9 +12,11 @@
 #include <string>
 #include <vector>
 
+#include "cs_ctx.h"
 #include "cs_host.h"
 #include "cs_kernels.h"
 #include "cs_solver.h"
+#include "cs_unet_host.h"
 
 namespace {
 
@@ -45,10 +47,8 @@ enum Pass { P_P2G = 0, P_GRID, P_CLASSIFY, P_SOLVE, P_POST, P_G2P, P_END, P_COUN
 
 }  // namespace
 
-struct cs_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-};
+void cs_set_last_error(const char* msg) { g_error = msg ? msg : ""; }
+
 
 struct cs_sim {
   cs_ctx* ctx = nullptr;
@@ -107,6 +107,7 @@ struct cs_sim {
   uint32_t* tile_start = nullptr;    // [nbricks]
   uint2* p2g_key = nullptr;          // [max_particles]
   uint32_t* p2g_sorted = nullptr;    // [max_particles]
+  cs_unet* warm_net = nullptr;       // network of CS_WARM_LEARNED
   // host state
   double* h_res = nullptr;    // pinned
   float* h_stage = nullptr;   // pinned emission staging
@@ -196,6 +197,16 @@ int residual_now(cs_sim* m, const cs_params& u, double* out) {
   return CS_OK;
 }
 
+// The attached network's pressure prediction from this substep's solve inputs,
+// into m->warm (then injected into the fluid cells like an external start).
+int predict_warm_start(cs_sim* m, const cs_params& u) {
+  if (!m->warm_net) return fail(CS_ERR_STATE, "warm_start CS_WARM_LEARNED needs cs_sim_set_warm_net");
+  cs_unet* net = m->warm_net;
+  CS_HIP(cs_launch_unet_features(m->ctx->stream, u, m->kf, m->rhs, nullptr, m->bed_id, m->bed, cs_unet_rhs_scale(net),
+                                 1.0f, cs_unet_feature_buffer(net), nullptr, nullptr, nullptr));
+  return cs_unet_predict_device(net, m->warm);
+}
+
 // One-thread-per-cell solve over the whole grid (the reference's dispatch
 // shape), kept as a cross-check of the brick solve.
 int solve_dense(cs_sim* m, const cs_params& u) {
@@ -208,6 +219,11 @@ int solve_dense(cs_sim* m, const cs_params& u) {
   if (m->s.warm_start == CS_WARM_PREVIOUS) {
     CS_HIP(cs_launch_warm_previous(st, u, m->kf, m->prev_p, m->pres));
   } else if (m->s.warm_start == CS_WARM_EXTERNAL && m->has_warm) {
+    CS_HIP(cs_launch_warm_external(st, u, m->kf, m->warm, m->pres, m->metrics));
+    external = true;
+  } else if (m->s.warm_start == CS_WARM_LEARNED) {
+    int rc = predict_warm_start(m, u);
+    if (rc) return rc;
     CS_HIP(cs_launch_warm_external(st, u, m->kf, m->warm, m->pres, m->metrics));
     external = true;
   }
@@ -231,7 +247,7 @@ int solve_dense(cs_sim* m, const cs_params& u) {
     double r = 0.0;
     int rc = residual_now(m, u, &r);
     if (rc) return rc;
-    if (external && r > 1.0) {
+    if (external && r > (double)m->s.warm_fallback_residual) {
       // the supplied start is worse than the zero start: fall back
       CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
       S.fell_back = 1;
@@ -385,6 +401,11 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
   } else if (m->s.warm_start == CS_WARM_EXTERNAL && m->has_warm) {
     CS_HIP(cs_launch_warm_external(st, u, m->kf, m->warm, m->pres, m->metrics));
     external = true;
+  } else if (m->s.warm_start == CS_WARM_LEARNED) {
+    int rc = predict_warm_start(m, u);
+    if (rc) return rc;
+    CS_HIP(cs_launch_warm_external(st, u, m->kf, m->warm, m->pres, m->metrics));
+    external = true;
   }
   const double tol = (double)m->s.rbgs_tolerance;
   const bool fixed = m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS;
@@ -422,7 +443,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     if (external) {
       CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
       CS_HIP(hipStreamSynchronize(st));
-      if (m->h_state->initial_residual > 1.0) {
+      if (m->h_state->initial_residual > (double)m->s.warm_fallback_residual) {
         // the supplied start is worse than the zero start: fall back
         CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
         S.fell_back = 1;
@@ -1099,3 +1120,43 @@ int cs_device_expf(cs_ctx* ctx, const float* in, float* out, uint32_t n) {
 }
 
 }  // extern "C"
+
+// ---- learned warm start: attachment and training captures ----------------------------------
+int cs_sim_set_warm_net(cs_sim* m, cs_unet* net) {
+  if (!m) return fail(CS_ERR_INVALID, "null simulation");
+  if (net) {
+    const int n = cs_unet_side(net);
+    if ((uint32_t)n != m->gx || (uint32_t)n != m->gy || (uint32_t)n != m->gz)
+      return fail(CS_ERR_INVALID, "the network's grid side differs from the simulation's grid");
+    if (cs_unet_context(net) != m->ctx) return fail(CS_ERR_INVALID, "the network belongs to another context");
+  }
+  m->warm_net = net;
+  return CS_OK;
+}
+
+int cs_unet_capture(cs_unet* net, cs_sim* m) {
+  if (!net || !m) return fail(CS_ERR_INVALID, "null argument");
+  const int n = cs_unet_side(net);
+  if ((uint32_t)n != m->gx || (uint32_t)n != m->gy || (uint32_t)n != m->gz)
+    return fail(CS_ERR_INVALID, "the network's grid side differs from the simulation's grid");
+  if (cs_unet_context(net) != m->ctx) return fail(CS_ERR_INVALID, "the network belongs to another context");
+  (void)hipSetDevice(m->ctx->device);
+  cs_params u;
+  cs::make_params(m->s, m->num_water, m->num_bed_cells, 0.0f, &u);
+  float *f4 = nullptr, *target = nullptr;
+  uint8_t* mask = nullptr;
+  uint32_t* count = nullptr;
+  int rc = cs_unet_new_sample(net, &f4, &target, &mask, &count);
+  if (rc) return rc;
+  CS_HIP(cs_launch_unet_features(m->ctx->stream, u, m->kf, m->rhs, m->pres, m->bed_id, m->bed, cs_unet_rhs_scale(net),
+                                 1.0f / cs_unet_pressure_scale(net), f4, target, mask, count));
+  return cs_unet_finish_sample(net);
+}
+
+int cs_sim_read_warm_start(cs_sim* m, float* out) {
+  if (!m || !out) return fail(CS_ERR_INVALID, "null argument");
+  (void)hipSetDevice(m->ctx->device);
+  CS_HIP(hipMemcpyAsync(out, m->warm, m->nc * sizeof(float), hipMemcpyDeviceToHost, m->ctx->stream));
+  CS_HIP(hipStreamSynchronize(m->ctx->stream));
+  return CS_OK;
+}

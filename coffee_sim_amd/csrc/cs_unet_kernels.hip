@@ -1,0 +1,523 @@
+// cs_unet_kernels.hip — kernels of the learned pressure warm start (a 3-D U-Net).
+//
+// The reference has no learned component: it restarts the pressure from zero
+// every substep (shader.rs:1187).  This network predicts the converged
+// pressure from the solve's inputs so RBGS can start from it; the plain solve
+// stays the judge (its residual) and the fallback.
+//
+// Activations are bf16, channels-last ([voxel][channel], voxel = (z*s + y)*s + x
+// on a cubic grid of side s), accumulation is fp32.  3x3x3 convolutions are
+// implicit GEMMs on v_mfma_f32_16x16x32_bf16: rows are 16 voxels of an 8^3
+// brick, columns 16 output channels, and each 32-wide K step covers two taps
+// x 16 input channels staged in LDS (a 10^3 halo region per 16-channel chunk).
+// Lane l of an MFMA holds A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16] and
+// D[4(l/16)+i][l%16] (checked on the device against a host matmul).
+#include <hip/hip_runtime.h>
+
+#include "cs_unet.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;  // NaN
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ bool bf_pos(uint16_t h) { return !(h & 0x8000u) && (h & 0x7fffu); }
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// Zero the bf16 entries of `v` whose counterpart in `m` is not positive (the
+// ReLU derivative of a layer's output applied to its gradient).
+__device__ __forceinline__ uint4 relu_mask(uint4 v, uint4 m) {
+  uint32_t a[4] = {v.x, v.y, v.z, v.w}, b[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t lo = bf_pos((uint16_t)(b[i] & 0xffffu)) ? 0x0000ffffu : 0u;
+    const uint32_t hi = bf_pos((uint16_t)(b[i] >> 16)) ? 0xffff0000u : 0u;
+    a[i] &= lo | hi;
+  }
+  return make_uint4(a[0], a[1], a[2], a[3]);
+}
+
+constexpr int HALO = 10;  // 8^3 brick + one voxel each side
+constexpr int HV = HALO * HALO * HALO;
+
+}  // namespace
+
+// ---- 3x3x3 convolution (forward, and input gradient with flipped weights) ----------
+// out[v][co] = act(bias[co] + sum_tap sum_ci in[v + tap][ci] * w[tap][co][ci]), zero
+// padding, act = ReLU when `relu`.  With `mask` the input is used only where the
+// mask (a layer output) is positive.  One workgroup per 8^3 brick of the s^3 grid;
+// each of the 4 wavefronts owns 8 row tiles of 16 voxels.
+template <int COUT>
+__global__ void __launch_bounds__(256) k_unet_conv(const uint16_t* __restrict__ in, int cin, int s,
+                                                   const uint16_t* __restrict__ w, const float* __restrict__ bias,
+                                                   const uint16_t* __restrict__ mask, uint16_t* __restrict__ out,
+                                                   int relu) {
+  constexpr int NTL = COUT / 16;
+  __shared__ uint4 s_in[HV * 2];  // [halo voxel][16 channels] as two 16-byte halves
+  const int nb = s / 8;
+  const int bid = (int)blockIdx.x;
+  const int bx = bid % nb, by = (bid / nb) % nb, bz = bid / (nb * nb);
+  const int x0 = bx * 8 - 1, y0 = by * 8 - 1, z0 = bz * 8 - 1;
+  const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+  const int g = lane >> 4, r16 = lane & 15;
+  f32x4 acc[8][NTL];
+#pragma unroll
+  for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+    for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int c0 = 0; c0 < cin; c0 += 16) {
+    __syncthreads();
+    for (int e = (int)threadIdx.x; e < HV * 2; e += 256) {
+      const int hv = e >> 1, h = e & 1;
+      const int gx = x0 + hv % HALO, gy = y0 + (hv / HALO) % HALO, gz = z0 + hv / (HALO * HALO);
+      uint4 val = make_uint4(0u, 0u, 0u, 0u);
+      if ((unsigned)gx < (unsigned)s && (unsigned)gy < (unsigned)s && (unsigned)gz < (unsigned)s) {
+        const size_t off = (((size_t)gz * s + gy) * s + gx) * (size_t)cin + (size_t)(c0 + 8 * h);
+        val = *reinterpret_cast<const uint4*>(in + off);
+        if (mask) val = relu_mask(val, *reinterpret_cast<const uint4*>(mask + off));
+      }
+      s_in[e] = val;
+    }
+    __syncthreads();
+    for (int ks = 0; ks < 14; ++ks) {
+      const int tap = 2 * ks + (g >> 1);  // K rows 0..15: tap 2ks, 16..31: tap 2ks+1
+      const bool tv = tap < 27;
+      const int tt = tv ? tap : 26;
+      const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
+      const int ci0 = 8 * (g & 1);
+      bf16x8 b[NTL];
+#pragma unroll
+      for (int nt = 0; nt < NTL; ++nt) {
+        const uint4 wvec = *reinterpret_cast<const uint4*>(w + ((size_t)(tt * COUT + nt * 16 + r16) * cin + c0 + ci0));
+        b[nt] = as_bf16x8(tv ? wvec : make_uint4(0u, 0u, 0u, 0u));
+      }
+#pragma unroll
+      for (int mi = 0; mi < 8; ++mi) {
+        const int vb = (wv * 8 + mi) * 16 + r16;
+        const int vx = vb & 7, vy = (vb >> 3) & 7, vz = vb >> 6;
+        const int hv = ((vz + dz + 1) * HALO + (vy + dy + 1)) * HALO + (vx + dx + 1);
+        const uint4 av = s_in[hv * 2 + (g & 1)];
+        const bf16x8 a = as_bf16x8(tv ? av : make_uint4(0u, 0u, 0u, 0u));
+#pragma unroll
+        for (int nt = 0; nt < NTL; ++nt) acc[mi][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mi][nt], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+    for (int nt = 0; nt < NTL; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int vb = (wv * 8 + mi) * 16 + 4 * g + i;
+        const int co = nt * 16 + r16;
+        float v = acc[mi][nt][i] + (bias ? bias[co] : 0.0f);
+        if (relu) v = fmaxf(v, 0.0f);
+        const int gx = bx * 8 + (vb & 7), gy = by * 8 + ((vb >> 3) & 7), gz = bz * 8 + (vb >> 6);
+        out[(((size_t)gz * s + gy) * s + gx) * COUT + co] = f2bf(v);
+      }
+}
+
+// ---- weight gradient ----------------------------------------------------------------
+// dW[tap][ci][co] = sum_v in[v + tap][ci] * gz[v][co], gz = gout masked by the
+// layer output's ReLU derivative.  Grid (slot, dz, chunk): a workgroup sums the
+// 9 taps of one dz for one 16-channel input chunk and one 16-channel output
+// chunk over its run of bricks and writes fp32 partials[slot][tap][ci][co]; a
+// fixed-order reduction over slots follows.  Rows of the MFMA are input
+// channels, columns output channels, K runs over 32 voxels of the brick.
+__global__ void __launch_bounds__(256) k_unet_wgrad(const uint16_t* __restrict__ in, int cin,
+                                                    const uint16_t* __restrict__ gout,
+                                                    const uint16_t* __restrict__ outv, int cout, int s,
+                                                    int bricks_per_slot, float* __restrict__ partial) {
+  __shared__ uint16_t s_in[HV * 16];
+  __shared__ uint16_t s_g[512 * 16];
+  const int slot = (int)blockIdx.x;
+  const int dzg = (int)blockIdx.y;  // dz = dzg - 1
+  const int nco = cout / 16;
+  const int cic = (int)blockIdx.z / nco, coc = (int)blockIdx.z % nco;
+  const int nb = s / 8, nbricks = nb * nb * nb;
+  const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+  const int g = lane >> 4, r16 = lane & 15;
+  f32x4 acc[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int b0 = slot * bricks_per_slot, b1 = min(b0 + bricks_per_slot, nbricks);
+  for (int bid = b0; bid < b1; ++bid) {
+    const int bx = bid % nb, by = (bid / nb) % nb, bz = bid / (nb * nb);
+    __syncthreads();
+    for (int e = (int)threadIdx.x; e < HV * 2; e += 256) {
+      const int hv = e >> 1, h = e & 1;
+      const int gx = bx * 8 - 1 + hv % HALO, gy = by * 8 - 1 + (hv / HALO) % HALO, gz = bz * 8 - 1 + hv / (HALO * HALO);
+      uint4 val = make_uint4(0u, 0u, 0u, 0u);
+      if ((unsigned)gx < (unsigned)s && (unsigned)gy < (unsigned)s && (unsigned)gz < (unsigned)s)
+        val = *reinterpret_cast<const uint4*>(in + (((size_t)gz * s + gy) * s + gx) * (size_t)cin + cic * 16 + 8 * h);
+      reinterpret_cast<uint4*>(s_in)[e] = val;
+    }
+    for (int e = (int)threadIdx.x; e < 512 * 2; e += 256) {
+      const int vb = e >> 1, h = e & 1;
+      const int gx = bx * 8 + (vb & 7), gy = by * 8 + ((vb >> 3) & 7), gz = bz * 8 + (vb >> 6);
+      const size_t off = (((size_t)gz * s + gy) * s + gx) * (size_t)cout + coc * 16 + 8 * h;
+      reinterpret_cast<uint4*>(s_g)[e] =
+          relu_mask(*reinterpret_cast<const uint4*>(gout + off), *reinterpret_cast<const uint4*>(outv + off));
+    }
+    __syncthreads();
+    for (int ks = 0; ks < 16; ++ks) {
+      // K = voxels ks*32 + 8g + j of the brick
+      uint32_t bw[4];
+#pragma unroll
+      for (int j = 0; j < 8; j += 2) {
+        const int v0 = ks * 32 + 8 * g + j;
+        bw[j >> 1] = (uint32_t)s_g[v0 * 16 + r16] | ((uint32_t)s_g[(v0 + 1) * 16 + r16] << 16);
+      }
+      const bf16x8 b = as_bf16x8(make_uint4(bw[0], bw[1], bw[2], bw[3]));
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int t9 = wv + 4 * k;  // this wavefront's taps of the dz slice
+        if (t9 >= 9) break;
+        const int dy = t9 / 3 - 1, dx = t9 % 3 - 1, dz = dzg - 1;
+        uint32_t aw[4];
+#pragma unroll
+        for (int j = 0; j < 8; j += 2) {
+          const int v0 = ks * 32 + 8 * g + j, v1 = v0 + 1;
+          const int h0 = (((v0 >> 6) + dz + 1) * HALO + ((v0 >> 3) & 7) + dy + 1) * HALO + (v0 & 7) + dx + 1;
+          const int h1 = (((v1 >> 6) + dz + 1) * HALO + ((v1 >> 3) & 7) + dy + 1) * HALO + (v1 & 7) + dx + 1;
+          aw[j >> 1] = (uint32_t)s_in[h0 * 16 + r16] | ((uint32_t)s_in[h1 * 16 + r16] << 16);
+        }
+        const bf16x8 a = as_bf16x8(make_uint4(aw[0], aw[1], aw[2], aw[3]));
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[k], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int t9 = wv + 4 * k;
+    if (t9 >= 9) break;
+    const int tap = dzg * 9 + t9;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ci = cic * 16 + 4 * g + i, co = coc * 16 + r16;
+      partial[(((size_t)slot * 27 + tap) * cin + ci) * cout + co] = acc[k][i];
+    }
+  }
+}
+
+// Fixed-order sum over slots: out[e] = sum_slot partial[slot][e].
+__global__ void k_unet_reduce_slots(const float* __restrict__ partial, int slots, size_t n, float* __restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float acc = 0.0f;
+  for (int k = 0; k < slots; ++k) acc += partial[(size_t)k * n + e];
+  out[e] = acc;
+}
+
+// Bias gradient, stage 1: part[rb][co] = sum over the rb-th share of voxels of
+// gz[v][co]; grid (cout / 16, row blocks), threads = 16 channels x 16 rows.
+__global__ void __launch_bounds__(256) k_unet_bias_grad(const uint16_t* __restrict__ gout,
+                                                        const uint16_t* __restrict__ outv, int cout, size_t nv,
+                                                        float* __restrict__ part) {
+  __shared__ float s_sum[256];
+  const int co = (int)blockIdx.x * 16 + (int)(threadIdx.x & 15u);
+  const size_t rows = (size_t)gridDim.y * 16;
+  float acc = 0.0f;
+  for (size_t v = (size_t)blockIdx.y * 16 + (threadIdx.x >> 4); v < nv; v += rows) {
+    const uint16_t o = outv[v * cout + co];
+    if (bf_pos(o)) acc += bf2f(gout[v * cout + co]);
+  }
+  s_sum[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    float t = 0.0f;
+    for (int r = 0; r < 16; ++r) t += s_sum[r * 16 + threadIdx.x];
+    part[(size_t)blockIdx.y * cout + co] = t;
+  }
+}
+
+// ---- pooling, upsampling, concatenation -------------------------------------------------
+// 2x average pool, [s^3][c] -> [(s/2)^3][c].
+__global__ void k_unet_pool(const uint16_t* __restrict__ in, int c, int s, uint16_t* __restrict__ out) {
+  const int h = s / 2;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)h * h * h * c) return;
+  const int ch = (int)(e % c);
+  const size_t vo = e / c;
+  const int x = (int)(vo % h), y = (int)((vo / h) % h), z = (int)(vo / ((size_t)h * h));
+  float acc = 0.0f;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx)
+        acc += bf2f(in[((((size_t)(2 * z + dz)) * s + 2 * y + dy) * s + 2 * x + dx) * c + ch]);
+  out[e] = f2bf(acc * 0.125f);
+}
+
+// Gradient of the pool plus a second contribution read from a channel window of
+// another tensor: g_in[v][ch] = add[v][add_off + ch] + g_out[parent(v)][ch] / 8.
+__global__ void k_unet_pool_back(const uint16_t* __restrict__ g_out, int c, int s, const uint16_t* __restrict__ add,
+                                 int add_stride, int add_off, uint16_t* __restrict__ g_in) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)s * s * s * c) return;
+  const int ch = (int)(e % c);
+  const size_t v = e / c;
+  const int x = (int)(v % s), y = (int)((v / s) % s), z = (int)(v / ((size_t)s * s));
+  const int h = s / 2;
+  const float gp = bf2f(g_out[((((size_t)(z / 2)) * h + y / 2) * h + x / 2) * c + ch]) * 0.125f;
+  const float a = add ? bf2f(add[v * add_stride + add_off + ch]) : 0.0f;
+  g_in[e] = f2bf(a + gp);
+}
+
+// Nearest 2x upsample into channels [off, off + c) of a concatenation with `stride` channels.
+__global__ void k_unet_up(const uint16_t* __restrict__ in, int c, int s, uint16_t* __restrict__ cat, int stride,
+                          int off) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int S = 2 * s;
+  if (e >= (size_t)S * S * S * c) return;
+  const int ch = (int)(e % c);
+  const size_t v = e / c;
+  const int x = (int)(v % S), y = (int)((v / S) % S), z = (int)(v / ((size_t)S * S));
+  cat[v * stride + off + ch] = in[((((size_t)(z / 2)) * s + y / 2) * s + x / 2) * c + ch];
+}
+
+// Gradient of the upsample: g_in[v'][ch] = sum of the 8 children's cat[child][off + ch].
+__global__ void k_unet_up_back(const uint16_t* __restrict__ g_cat, int stride, int off, int c, int s,
+                               uint16_t* __restrict__ g_in) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)s * s * s * c) return;
+  const int ch = (int)(e % c);
+  const size_t vo = e / c;
+  const int x = (int)(vo % s), y = (int)((vo / s) % s), z = (int)(vo / ((size_t)s * s));
+  const int S = 2 * s;
+  float acc = 0.0f;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx)
+        acc += bf2f(g_cat[((((size_t)(2 * z + dz)) * S + 2 * y + dy) * S + 2 * x + dx) * stride + off + ch]);
+  g_in[e] = f2bf(acc);
+}
+
+// Copies a [v][c] tensor into channels [off, off + c) of a concatenation.
+__global__ void k_unet_put(const uint16_t* __restrict__ in, int c, size_t nv, uint16_t* __restrict__ cat, int stride,
+                           int off) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nv * c) return;
+  cat[(e / c) * stride + off + e % c] = in[e];
+}
+
+// ---- head, loss --------------------------------------------------------------------------
+// pred[v] = sum_c x[v][c] * wh[c] + bh, x with 16 channels.
+__global__ void k_unet_head(const uint16_t* __restrict__ x, const float* __restrict__ wh, size_t nv,
+                            float* __restrict__ pred) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  float acc = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) acc += bf2f(x[v * 16 + c]) * wh[c];
+  pred[v] = acc + wh[16];
+}
+
+// Masked mean squared error: per-block partial sums of mask and squared error,
+// and the prediction gradient 2 * mask * (pred - target) / count once `count`
+// (the mask sum, computed on the host's request beforehand) is known.
+__global__ void __launch_bounds__(256) k_unet_loss(const float* __restrict__ pred, const float* __restrict__ target,
+                                                   const uint8_t* __restrict__ mask, size_t nv, float inv_count,
+                                                   float* __restrict__ gpred, float* __restrict__ block_sse) {
+  __shared__ float s_sum[256];
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float se = 0.0f;
+  if (v < nv) {
+    const float d = mask[v] ? pred[v] - target[v] : 0.0f;
+    se = d * d;
+    gpred[v] = 2.0f * d * inv_count;
+  }
+  s_sum[threadIdx.x] = se;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) s_sum[threadIdx.x] += s_sum[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) block_sse[blockIdx.x] = s_sum[0];
+}
+
+// Gradient into the head: g_x[v][c] = gpred[v] * wh[c]; per-block partials of
+// d wh[c] = sum_v gpred[v] x[v][c] and d bh = sum_v gpred[v].
+__global__ void __launch_bounds__(256) k_unet_head_back(const uint16_t* __restrict__ x, const float* __restrict__ wh,
+                                                        const float* __restrict__ gpred, size_t nv,
+                                                        uint16_t* __restrict__ g_x, float* __restrict__ block_part) {
+  __shared__ float s_part[17][256];
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const float gp = v < nv ? gpred[v] : 0.0f;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    const float xc = v < nv ? bf2f(x[v * 16 + c]) : 0.0f;
+    if (v < nv) g_x[v * 16 + c] = f2bf(gp * wh[c]);
+    s_part[c][threadIdx.x] = gp * xc;
+  }
+  s_part[16][threadIdx.x] = gp;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int c = 0; c < 17; ++c) s_part[c][threadIdx.x] += s_part[c][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 17) block_part[blockIdx.x * 17 + threadIdx.x] = s_part[threadIdx.x][0];
+}
+
+// Fixed-order sum of `parts` rows of `width` values.
+__global__ void k_unet_sum_rows(const float* __restrict__ part, int parts, int width, float* __restrict__ out) {
+  const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= width) return;
+  float acc = 0.0f;
+  for (int k = 0; k < parts; ++k) acc += part[(size_t)k * width + c];
+  out[c] = acc;
+}
+
+// ---- parameters ---------------------------------------------------------------------------
+// bf16 copies of one convolution's fp32 weights W[tap][ci][co]: wf[tap][co][ci]
+// for the forward pass, wd[tap][ci][co] = W[26 - tap] for the input gradient.
+__global__ void k_unet_prep_weights(const float* __restrict__ w, int cin, int cout, uint16_t* __restrict__ wf,
+                                    uint16_t* __restrict__ wd) {
+  const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= 27 * cin * cout) return;
+  const int co = e % cout, ci = (e / cout) % cin, tap = e / (cin * cout);
+  const uint16_t h = f2bf(w[e]);
+  wf[((size_t)tap * cout + co) * cin + ci] = h;
+  wd[((size_t)(26 - tap) * cin + ci) * cout + co] = h;
+}
+
+// Adam (Kingma & Ba 2015) with bias correction folded into the step size.
+__global__ void k_unet_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, size_t n, float lr_t, float b1, float b2, float eps) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr_t * mi / (sqrtf(vi) + eps);
+}
+
+// ---- inputs --------------------------------------------------------------------------------
+// The network's input channels from host-side float features [v][4] (the
+// remaining 12 of 16 channels are zero).
+__global__ void k_unet_pack_features(const float* __restrict__ f4, size_t nv, uint16_t* __restrict__ x) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nv * 16) return;
+  const size_t v = e / 16;
+  const int c = (int)(e % 16);
+  x[e] = c < 4 ? f2bf(f4[v * 4 + c]) : (uint16_t)0;
+}
+
+// out[v] = pred[v] * scale.
+__global__ void k_unet_scale(const float* __restrict__ pred, size_t nv, float scale, float* __restrict__ out) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nv) out[v] = pred[v] * scale;
+}
+
+// ---- launchers ------------------------------------------------------------------------------
+namespace {
+inline dim3 grid1(size_t n, int block = 256) { return dim3((unsigned)((n + block - 1) / block)); }
+}  // namespace
+
+hipError_t cs_unet_launch_conv(hipStream_t st, const uint16_t* in, int cin, int s, const uint16_t* w,
+                               const float* bias, const uint16_t* mask, uint16_t* out, int cout, int relu) {
+  const dim3 g((unsigned)((s / 8) * (s / 8) * (s / 8)));
+  switch (cout) {
+    case 16: hipLaunchKernelGGL(k_unet_conv<16>, g, dim3(256), 0, st, in, cin, s, w, bias, mask, out, relu); break;
+    case 32: hipLaunchKernelGGL(k_unet_conv<32>, g, dim3(256), 0, st, in, cin, s, w, bias, mask, out, relu); break;
+    case 48: hipLaunchKernelGGL(k_unet_conv<48>, g, dim3(256), 0, st, in, cin, s, w, bias, mask, out, relu); break;
+    case 64: hipLaunchKernelGGL(k_unet_conv<64>, g, dim3(256), 0, st, in, cin, s, w, bias, mask, out, relu); break;
+    case 96: hipLaunchKernelGGL(k_unet_conv<96>, g, dim3(256), 0, st, in, cin, s, w, bias, mask, out, relu); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int cs_unet_wgrad_slots(int s) {
+  const int nbricks = (s / 8) * (s / 8) * (s / 8);
+  return nbricks < 64 ? nbricks : 64;
+}
+
+hipError_t cs_unet_launch_wgrad(hipStream_t st, const uint16_t* in, int cin, const uint16_t* gout,
+                                const uint16_t* outv, int cout, int s, float* partial, float* bias_part, float* dw,
+                                float* db) {
+  const int nbricks = (s / 8) * (s / 8) * (s / 8);
+  const int slots = cs_unet_wgrad_slots(s);
+  const int per = (nbricks + slots - 1) / slots;
+  hipLaunchKernelGGL(k_unet_wgrad, dim3((unsigned)slots, 3u, (unsigned)((cin / 16) * (cout / 16))), dim3(256), 0, st,
+                     in, cin, gout, outv, cout, s, per, partial);
+  const size_t n = (size_t)27 * cin * cout;
+  hipLaunchKernelGGL(k_unet_reduce_slots, grid1(n), dim3(256), 0, st, partial, slots, n, dw);
+  hipLaunchKernelGGL(k_unet_bias_grad, dim3((unsigned)(cout / 16), (unsigned)CS_UNET_BIAS_ROWS), dim3(256), 0, st,
+                     gout, outv, cout, (size_t)s * s * s, bias_part);
+  hipLaunchKernelGGL(k_unet_sum_rows, grid1((size_t)cout, 64), dim3(64), 0, st, bias_part, CS_UNET_BIAS_ROWS, cout, db);
+  return hipGetLastError();
+}
+
+hipError_t cs_unet_launch_pool(hipStream_t st, const uint16_t* in, int c, int s, uint16_t* out) {
+  const size_t n = (size_t)(s / 2) * (s / 2) * (s / 2) * c;
+  hipLaunchKernelGGL(k_unet_pool, grid1(n), dim3(256), 0, st, in, c, s, out);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_pool_back(hipStream_t st, const uint16_t* g_out, int c, int s, const uint16_t* add,
+                                    int add_stride, int add_off, uint16_t* g_in) {
+  const size_t n = (size_t)s * s * s * c;
+  hipLaunchKernelGGL(k_unet_pool_back, grid1(n), dim3(256), 0, st, g_out, c, s, add, add_stride, add_off, g_in);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_up(hipStream_t st, const uint16_t* in, int c, int s, uint16_t* cat, int stride, int off) {
+  const size_t n = (size_t)(2 * s) * (2 * s) * (2 * s) * c;
+  hipLaunchKernelGGL(k_unet_up, grid1(n), dim3(256), 0, st, in, c, s, cat, stride, off);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_up_back(hipStream_t st, const uint16_t* g_cat, int stride, int off, int c, int s,
+                                  uint16_t* g_in) {
+  const size_t n = (size_t)s * s * s * c;
+  hipLaunchKernelGGL(k_unet_up_back, grid1(n), dim3(256), 0, st, g_cat, stride, off, c, s, g_in);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_put(hipStream_t st, const uint16_t* in, int c, size_t nv, uint16_t* cat, int stride,
+                              int off) {
+  hipLaunchKernelGGL(k_unet_put, grid1(nv * c), dim3(256), 0, st, in, c, nv, cat, stride, off);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_head(hipStream_t st, const uint16_t* x, const float* wh, size_t nv, float* pred) {
+  hipLaunchKernelGGL(k_unet_head, grid1(nv), dim3(256), 0, st, x, wh, nv, pred);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_loss(hipStream_t st, const float* pred, const float* target, const uint8_t* mask, size_t nv,
+                               float inv_count, float* gpred, float* block_sse, float* sse) {
+  const dim3 g = grid1(nv);
+  hipLaunchKernelGGL(k_unet_loss, g, dim3(256), 0, st, pred, target, mask, nv, inv_count, gpred, block_sse);
+  hipLaunchKernelGGL(k_unet_sum_rows, dim3(1), dim3(64), 0, st, block_sse, (int)g.x, 1, sse);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_head_back(hipStream_t st, const uint16_t* x, const float* wh, const float* gpred, size_t nv,
+                                    uint16_t* g_x, float* block_part, float* dwh) {
+  const dim3 g = grid1(nv);
+  hipLaunchKernelGGL(k_unet_head_back, g, dim3(256), 0, st, x, wh, gpred, nv, g_x, block_part);
+  hipLaunchKernelGGL(k_unet_sum_rows, dim3(1), dim3(64), 0, st, block_part, (int)g.x, 17, dwh);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_prep_weights(hipStream_t st, const float* w, int cin, int cout, uint16_t* wf, uint16_t* wd) {
+  hipLaunchKernelGGL(k_unet_prep_weights, grid1((size_t)27 * cin * cout), dim3(256), 0, st, w, cin, cout, wf, wd);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_adam(hipStream_t st, float* p, const float* g, float* m, float* v, size_t n, float lr_t,
+                               float b1, float b2, float eps) {
+  hipLaunchKernelGGL(k_unet_adam, grid1(n), dim3(256), 0, st, p, g, m, v, n, lr_t, b1, b2, eps);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_scale(hipStream_t st, const float* pred, size_t nv, float scale, float* out) {
+  hipLaunchKernelGGL(k_unet_scale, grid1(nv), dim3(256), 0, st, pred, nv, scale, out);
+  return hipGetLastError();
+}
+hipError_t cs_unet_launch_pack_features(hipStream_t st, const float* f4, size_t nv, uint16_t* x) {
+  hipLaunchKernelGGL(k_unet_pack_features, grid1(nv * 16), dim3(256), 0, st, f4, nv, x);
+  return hipGetLastError();
+}

@@ -93,6 +93,7 @@ typedef struct cs_spout {
 #define CS_WARM_ZERO 0
 #define CS_WARM_PREVIOUS 1 /* previous substep's converged pressure                        */
 #define CS_WARM_EXTERNAL 2 /* pressure supplied by cs_sim_set_warm_start (learned predictor) */
+#define CS_WARM_LEARNED 3  /* pressure predicted on the device by an attached cs_unet          */
 
 /* Analytic contact fallbacks that the reference hard-codes in WGSL
  * (shader.rs:744-823, 1778-1779).  Reference mode plugs in its constants;
@@ -143,6 +144,9 @@ typedef struct cs_settings {
   uint32_t rbgs_max_iters;     /* iteration cap (20000)                            */
   uint32_t warm_start;         /* CS_WARM_*                                        */
   uint32_t has_bed_field;      /* a per-cell porous bed is supplied with the scene */
+  float warm_fallback_residual; /* an external or learned start whose relative residual
+                                 * exceeds this restarts from zero (1: no worse than the
+                                 * zero start's residual)                          */
 } cs_settings;
 
 /* Per-cell porous bed ("field mode").  The reference stamps bed-particle ids into
@@ -307,6 +311,49 @@ int cs_sim_reset_kernel_timing(cs_sim* sim);
 /* Verification entry point (no reference counterpart): evaluates the solver's
  * device exp (used by the J update, shader.rs:1853) on `n` inputs. */
 int cs_device_expf(cs_ctx* ctx, const float* in, float* out, uint32_t n);
+
+/* ---- learned warm start (no reference counterpart) ------------------------------
+ * A 3-D U-Net (3 levels, 16/32/64 channels, 3x3x3 kernels, bf16 weights and
+ * activations, fp32 accumulation) that predicts the converged pressure of a solve
+ * from four features per cell: rhs * rhs_scale, liquid fill (0 when not fluid),
+ * solid mask, bed porosity (0 outside the bed).  It replaces the zero pressure
+ * the reference starts every solve from (shader.rs:1187); RBGS then runs to the
+ * tolerance from the prediction and falls back to a zero start when the
+ * prediction's residual exceeds 1, so the plain solve remains the judge.
+ * The grid side must be a multiple of 32.  Parameters are one fp32 vector:
+ * for each convolution e1a, e1b, e2a, e2b, ba, bb, d2a, d2b, d1a, d1b the
+ * weights [27][cin][cout] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) then the bias
+ * [cout]; last the head's 16 weights and bias.  Host arrays: features
+ * [side^3][4], predictions and targets [side^3] in units of pressure_scale,
+ * masks [side^3] (1 = counted in the loss).
+ * Training pairs live on the device: cs_unet_capture appends the last solve
+ * of a simulation (its features and converged pressure). */
+typedef struct cs_unet cs_unet;
+int cs_unet_create(cs_ctx* ctx, uint32_t side, uint64_t seed, cs_unet** out);
+int cs_unet_destroy(cs_unet* net);
+int cs_unet_param_count(const cs_unet* net, uint32_t* count);
+int cs_unet_get_params(cs_unet* net, float* out);
+int cs_unet_set_params(cs_unet* net, const float* in);
+int cs_unet_set_scales(cs_unet* net, float rhs_scale, float pressure_scale);
+int cs_unet_get_scales(const cs_unet* net, float* rhs_scale, float* pressure_scale);
+int cs_unet_forward(cs_unet* net, const float* features4, float* pred);
+/* Loss (masked mean squared error against `target`) and its gradient for every parameter. */
+int cs_unet_gradient(cs_unet* net, const float* features4, const float* target, const uint8_t* mask, float* loss,
+                     float* grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on one sample. */
+int cs_unet_train_step(cs_unet* net, const float* features4, const float* target, const uint8_t* mask, float lr,
+                       float* loss);
+int cs_unet_capture(cs_unet* net, cs_sim* sim);
+int cs_unet_dataset_size(const cs_unet* net, uint32_t* count);
+int cs_unet_dataset_clear(cs_unet* net);
+int cs_unet_read_sample(cs_unet* net, uint32_t index, float* features4, float* target, uint8_t* mask);
+/* One pass over the captured samples in a seeded random order, one Adam step each. */
+int cs_unet_train_epoch(cs_unet* net, float lr, uint64_t seed, float* mean_loss);
+/* The network a simulation uses when its settings ask for CS_WARM_LEARNED (NULL detaches). */
+int cs_sim_set_warm_net(cs_sim* sim, cs_unet* net);
+/* The warm-start pressure (float per cell) the last CS_WARM_EXTERNAL / CS_WARM_LEARNED
+ * solve started from, before it was injected into the fluid cells. */
+int cs_sim_read_warm_start(cs_sim* sim, float* pressure);
 
 #ifdef __cplusplus
 }

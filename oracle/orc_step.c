@@ -639,7 +639,7 @@ static void solve_pressure(orc_sim* S, const orc_params* u) {
     st->converged = st->final_residual <= (double)S->s.rbgs_tolerance;
   } else {
     double r = residual_rows(S, u);
-    if (warmed && S->s.warm_start == CS_WARM_EXTERNAL && r > 1.0) {
+    if (warmed && S->s.warm_start == CS_WARM_EXTERNAL && r > (double)S->s.warm_fallback_residual) {
       /* the predicted start is worse than the zero start: fall back */
       for (size_t i = 0; i < S->nc; ++i) LANE(S, 0, i) = 0;
       st->fell_back = 1;
