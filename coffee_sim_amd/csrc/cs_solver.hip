@@ -584,18 +584,14 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
 // in brick order (a fixed order) and advances the solve state.  (Electing the
 // last workgroup with an arrival counter instead serialises one device-scope
 // atomic per brick on a single address, which costs more than the launch.)
-__global__ void __launch_bounds__(256) k_residual_bricks(cs_params u, const uint32_t* __restrict__ list,
-                                                         const float4* __restrict__ wc, const float* __restrict__ rhs,
-                                                         const int32_t* __restrict__ p, double* __restrict__ partials,
-                                                         const cs_solve_state* __restrict__ st, uint32_t mode) {
-  constexpr int R = BR + 2, R3 = R * R * R, NT = 256, PER = (R3 + NT - 1) / NT;
+// Sums of squared residual and squared right-hand side over one brick's fluid
+// cells (one 256-thread workgroup); valid in s_rr[0], s_bb[0] on return.
+template <int NT>
+__device__ __forceinline__ void brick_residual_sums(const cs_params& u, uint32_t brick, const float4* __restrict__ wc,
+                                                    const float* __restrict__ rhs, const int32_t* __restrict__ p,
+                                                    float4* s_wc, int32_t* s_p, double* s_rr, double* s_bb) {
+  constexpr int R = BR + 2, R3 = R * R * R, PER = (R3 + NT - 1) / NT;
   constexpr int BPER = BR * BR * BR / NT;
-  __shared__ float4 s_wc[R3];
-  __shared__ int32_t s_p[R3];
-  __shared__ double s_rr[NT], s_bb[NT];
-  const uint32_t done = st->done;
-  const uint32_t brick = list[blockIdx.x];
-  if (mode != 0u && done) return;
   int x0, y0, z0;
   brick_origin(u, brick, x0, y0, z0);
   float rb[BPER];  // right-hand side of this thread's brick cells
@@ -651,13 +647,27 @@ __global__ void __launch_bounds__(256) k_residual_bricks(cs_params u, const uint
   s_rr[threadIdx.x] = rr;
   s_bb[threadIdx.x] = bb;
   __syncthreads();
-  for (uint32_t s = NT / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      s_rr[threadIdx.x] += s_rr[threadIdx.x + s];
-      s_bb[threadIdx.x] += s_bb[threadIdx.x + s];
+  for (uint32_t st = NT / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s_rr[threadIdx.x] += s_rr[threadIdx.x + st];
+      s_bb[threadIdx.x] += s_bb[threadIdx.x + st];
     }
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(256) k_residual_bricks(cs_params u, const uint32_t* __restrict__ list,
+                                                         const float4* __restrict__ wc, const float* __restrict__ rhs,
+                                                         const int32_t* __restrict__ p, double* __restrict__ partials,
+                                                         const cs_solve_state* __restrict__ st, uint32_t mode) {
+  constexpr int R3 = (BR + 2) * (BR + 2) * (BR + 2), NT = 256;
+  __shared__ float4 s_wc[R3];
+  __shared__ int32_t s_p[R3];
+  __shared__ double s_rr[NT], s_bb[NT];
+  const uint32_t done = st->done;
+  const uint32_t brick = list[blockIdx.x];
+  if (mode != 0u && done) return;
+  brick_residual_sums<NT>(u, brick, wc, rhs, p, s_wc, s_p, s_rr, s_bb);
   if (threadIdx.x == 0) {
     partials[2 * blockIdx.x] = s_rr[0];
     partials[2 * blockIdx.x + 1] = s_bb[0];

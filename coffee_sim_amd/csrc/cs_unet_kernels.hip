@@ -367,13 +367,21 @@ __global__ void __launch_bounds__(256) k_unet_head_back(const uint16_t* __restri
   if (threadIdx.x < 17) block_part[blockIdx.x * 17 + threadIdx.x] = s_part[threadIdx.x][0];
 }
 
-// Fixed-order sum of `parts` rows of `width` values.
-__global__ void k_unet_sum_rows(const float* __restrict__ part, int parts, int width, float* __restrict__ out) {
-  const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (c >= width) return;
+// Fixed-order sum of `parts` rows of `width` values: one workgroup per column,
+// a strided partial per thread, then a tree over the workgroup.
+__global__ void __launch_bounds__(256) k_unet_sum_rows(const float* __restrict__ part, int parts, int width,
+                                                       float* __restrict__ out) {
+  __shared__ float s_sum[256];
+  const int c = (int)blockIdx.x;
   float acc = 0.0f;
-  for (int k = 0; k < parts; ++k) acc += part[(size_t)k * width + c];
-  out[c] = acc;
+  for (int k = (int)threadIdx.x; k < parts; k += 256) acc += part[(size_t)k * width + c];
+  s_sum[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) s_sum[threadIdx.x] += s_sum[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = s_sum[0];
 }
 
 // ---- parameters ---------------------------------------------------------------------------
@@ -455,7 +463,7 @@ hipError_t cs_unet_launch_wgrad(hipStream_t st, const uint16_t* in, int cin, con
   hipLaunchKernelGGL(k_unet_reduce_slots, grid1(n), dim3(256), 0, st, partial, slots, n, dw);
   hipLaunchKernelGGL(k_unet_bias_grad, dim3((unsigned)(cout / 16), (unsigned)CS_UNET_BIAS_ROWS), dim3(256), 0, st,
                      gout, outv, cout, (size_t)s * s * s, bias_part);
-  hipLaunchKernelGGL(k_unet_sum_rows, grid1((size_t)cout, 64), dim3(64), 0, st, bias_part, CS_UNET_BIAS_ROWS, cout, db);
+  hipLaunchKernelGGL(k_unet_sum_rows, dim3((unsigned)cout), dim3(256), 0, st, bias_part, CS_UNET_BIAS_ROWS, cout, db);
   return hipGetLastError();
 }
 
@@ -494,14 +502,14 @@ hipError_t cs_unet_launch_loss(hipStream_t st, const float* pred, const float* t
                                float inv_count, float* gpred, float* block_sse, float* sse) {
   const dim3 g = grid1(nv);
   hipLaunchKernelGGL(k_unet_loss, g, dim3(256), 0, st, pred, target, mask, nv, inv_count, gpred, block_sse);
-  hipLaunchKernelGGL(k_unet_sum_rows, dim3(1), dim3(64), 0, st, block_sse, (int)g.x, 1, sse);
+  hipLaunchKernelGGL(k_unet_sum_rows, dim3(1), dim3(256), 0, st, block_sse, (int)g.x, 1, sse);
   return hipGetLastError();
 }
 hipError_t cs_unet_launch_head_back(hipStream_t st, const uint16_t* x, const float* wh, const float* gpred, size_t nv,
                                     uint16_t* g_x, float* block_part, float* dwh) {
   const dim3 g = grid1(nv);
   hipLaunchKernelGGL(k_unet_head_back, g, dim3(256), 0, st, x, wh, gpred, nv, g_x, block_part);
-  hipLaunchKernelGGL(k_unet_sum_rows, dim3(1), dim3(64), 0, st, block_part, (int)g.x, 17, dwh);
+  hipLaunchKernelGGL(k_unet_sum_rows, dim3(17), dim3(256), 0, st, block_part, (int)g.x, 17, dwh);
   return hipGetLastError();
 }
 hipError_t cs_unet_launch_prep_weights(hipStream_t st, const float* w, int cin, int cout, uint16_t* wf, uint16_t* wd) {

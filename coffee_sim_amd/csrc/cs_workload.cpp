@@ -112,22 +112,97 @@ void seed_cell(std::vector<float>& out, uint64_t seed, const cs_params& u, uint3
   }
 }
 
+// Everything that differs between the generated scenes.
+struct Spec {
+  bool working;            // pre-wetted slab + pour (else the small cylinder scene)
+  uint32_t n;              // grid side
+  float dx_m;              // cell size in metres
+  uint32_t apex_cells;     // cone apex height
+  uint32_t bed_cells;      // bed depth above the apex
+  uint32_t n_total;        // particles in the run
+  uint32_t n_initial;      // particles at the start
+  uint32_t emit_per_substep;
+  float noise_wavelength;  // porosity noise lattice, cells
+  float noise_amp;         // porosity noise amplitude
+  float kettle_deg;        // spout tilt (informational)
+  float aim_dx, aim_dy;    // spout direction in the x-y plane
+  uint32_t radius_cells;   // small scene: initial cylinder radius
+  uint64_t seed;
+  float frame_dt;
+  uint32_t substeps_total;
+};
+
+int build(const Spec& sp, cs_workload* out);
+
 }  // namespace
 
 extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs_workload* out) {
   if (!out || (workload != CS_WORKLOAD_V60_SMALL && workload != CS_WORKLOAD_V60_WORKING)) return CS_ERR_INVALID;
   if (scale == 0 || scale > 8) return CS_ERR_INVALID;
-  memset(out, 0, sizeof(*out));
   const bool working = workload == CS_WORKLOAD_V60_WORKING;
-  const float supm = cs::sim_units_per_meter();
-  const uint32_t n = (working ? 128u : 64u) / scale;
-  const float dx_m = (working ? 0.001f : 0.002f) * (float)scale;
-  const float dx = dx_m * supm;
-  const uint32_t apex_cells = (working ? 16u : 8u) / scale;
-  const uint32_t bed_cells = (working ? 24u : 12u) / scale;
   const uint32_t s3 = scale * scale * scale;
-  const uint32_t n_total = (working ? 1048576u : 32768u) / s3;
-  const uint32_t n_initial = working ? 734004u / s3 : n_total;
+  Spec sp;
+  sp.working = working;
+  sp.n = (working ? 128u : 64u) / scale;
+  sp.dx_m = (working ? 0.001f : 0.002f) * (float)scale;
+  sp.apex_cells = (working ? 16u : 8u) / scale;
+  sp.bed_cells = (working ? 24u : 12u) / scale;
+  sp.n_total = (working ? 1048576u : 32768u) / s3;
+  sp.n_initial = working ? 734004u / s3 : sp.n_total;
+  sp.emit_per_substep = std::max(1u, 2000u / s3);
+  sp.noise_wavelength = 8.0f / (float)scale;
+  sp.noise_amp = 0.05f;
+  sp.kettle_deg = 45.0f;
+  sp.aim_dx = 1.0f;  // 45 degrees below horizontal
+  sp.aim_dy = -1.0f;
+  sp.radius_cells = std::max(1u, 4u / scale);
+  sp.seed = seed;
+  sp.frame_dt = working ? kWorkingDt : 1.0e-4f;
+  sp.substeps_total = working ? 1000u : 200u;
+  return build(sp, out);
+}
+
+extern "C" int cs_workload_build_pour(const cs_pour_params* p, cs_workload* out) {
+  if (!p || !out) return CS_ERR_INVALID;
+  if (p->side < 32 || p->side > 256 || p->side % 16 != 0) return CS_ERR_INVALID;
+  if (p->particles == 0 || p->pour_per_substep == 0) return CS_ERR_INVALID;
+  if (!(p->kettle_angle_deg > 0.0f && p->kettle_angle_deg < 90.0f)) return CS_ERR_INVALID;
+  Spec sp;
+  sp.working = true;
+  sp.n = p->side;
+  sp.dx_m = 0.128f / (float)p->side;  // the working scene's 128 mm domain
+  sp.apex_cells = p->side / 8u;
+  sp.bed_cells = 3u * p->side / 16u;
+  sp.n_total = p->particles;
+  sp.n_initial = (uint32_t)((double)p->particles * 0.7 + 0.5);
+  sp.emit_per_substep = p->pour_per_substep;
+  sp.noise_wavelength = 8.0f;
+  sp.noise_amp = 0.05f;
+  sp.kettle_deg = p->kettle_angle_deg;
+  const float a = p->kettle_angle_deg * 3.14159265f / 180.0f;
+  sp.aim_dx = cosf(a);
+  sp.aim_dy = -sinf(a);
+  sp.radius_cells = 1u;
+  sp.seed = p->seed;
+  sp.frame_dt = kWorkingDt;
+  sp.substeps_total = 500u;
+  return build(sp, out);
+}
+
+namespace {
+
+int build(const Spec& sp, cs_workload* out) {
+  memset(out, 0, sizeof(*out));
+  const bool working = sp.working;
+  const uint64_t seed = sp.seed;
+  const float supm = cs::sim_units_per_meter();
+  const uint32_t n = sp.n;
+  const float dx_m = sp.dx_m;
+  const float dx = dx_m * supm;
+  const uint32_t apex_cells = sp.apex_cells;
+  const uint32_t bed_cells = sp.bed_cells;
+  const uint32_t n_total = sp.n_total;
+  const uint32_t n_initial = sp.n_initial;
   const uint32_t n_pour = n_total - n_initial;
 
   cs_settings& s = out->settings;
@@ -157,7 +232,7 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
   s.obstacles[0].top_y = half;
   s.obstacles[0].top_radius = (half - apex_y) * tan60;
   s.has_bed_field = 1u;
-  s.initial_kettle_angle_deg = working ? 45.0f : 0.0f;
+  s.initial_kettle_angle_deg = working ? sp.kettle_deg : 0.0f;
   s.emitter_mode = CS_EMIT_FIXED_COUNT;
   s.emit_per_substep = 0u;
   s.emit_total = 0u;
@@ -169,7 +244,7 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
     // 2,000 particles per substep (scaled with the particle count) from a
     // spout tilted 45 degrees, at half the speed cap; the jet radius keeps
     // the poured column at 8 particles per cell: N = 8 pi R^2 (v dt / dx).
-    s.emit_per_substep = std::max(1u, 2000u / s3);
+    s.emit_per_substep = sp.emit_per_substep;
     s.emit_total = n_pour;
     s.emit_speed = 0.5f * cs::max_velocity();
     const float dt = kWorkingDt;
@@ -178,7 +253,7 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
     s.spout.origin[0] = -0.30f * side;
     s.spout.origin[1] = 0.30f * side;
     s.spout.origin[2] = 0.0f;
-    cs::spout_aim_at(&s.spout, s.spout.origin[0] + 1.0f, s.spout.origin[1] - 1.0f, 0.0f);
+    cs::spout_aim_at(&s.spout, s.spout.origin[0] + sp.aim_dx, s.spout.origin[1] + sp.aim_dy, 0.0f);
   } else {
     s.rbgs_mode = CS_RBGS_FIXED_PAIRS;
     s.pressure_rbgs_pairs = 40u;
@@ -207,8 +282,8 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
         memset(&r, 0, sizeof(r));
         float ccy = u.origin[1] + ((float)iy + 0.5f) * dx;
         if (working) {
-          float nz = value_noise(seed + 1, (float)ix + 0.5f, (float)iy + 0.5f, (float)iz + 0.5f, 8.0f / scale);
-          r.porosity = 0.4f + 0.05f * nz;
+          float nz = value_noise(seed + 1, (float)ix + 0.5f, (float)iy + 0.5f, (float)iz + 0.5f, sp.noise_wavelength);
+          r.porosity = 0.4f + sp.noise_amp * nz;
           r.permeability = cs::kozeny_carman_permeability(d_um, r.porosity);
         } else {
           r.porosity = 0.4f;
@@ -243,7 +318,7 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
     // widened to the smallest radius whose column fits below the top margin
     const uint32_t y0 = apex_cells + bed_cells + 1u;
     const uint32_t top = n - 3u;
-    uint32_t r_cells = std::max(1u, 4u / scale);
+    uint32_t r_cells = sp.radius_cells;
     for (;; ++r_cells) {
       uint32_t per_layer = 0;
       float rr = (float)r_cells;
@@ -277,12 +352,14 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
   sc.particles = own->particles.data();
   sc.affine = own->affine.data();
   sc.num_particles = placed;
-  out->frame_dt = working ? kWorkingDt : 1.0e-4f;
-  out->substeps_total = working ? 1000u : 200u;
+  out->frame_dt = sp.frame_dt;
+  out->substeps_total = sp.substeps_total;
   out->total_particles = placed + n_pour;
   out->_owner = own.release();
   return CS_OK;
 }
+
+}  // namespace
 
 extern "C" int cs_workload_free(cs_workload* w) {
   if (!w) return CS_ERR_INVALID;

@@ -212,6 +212,20 @@ typedef struct cs_workload {
   void* _owner;            /* library-owned storage                            */
 } cs_workload;
 int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs_workload* out);
+/* A pour scene of the working kind with its own size (no reference counterpart):
+ * a cubic grid of `side` cells over the same 128 mm domain (side a multiple of 16,
+ * 32..256), cone apex side/8 and bed 3*side/16 cells up, porosity noise as in
+ * V60_WORKING (8-cell lattice, +-0.05), 70% of `particles` as the pre-wetted slab
+ * and the rest poured `pour_per_substep` at a time from a spout tilted
+ * `kettle_angle_deg` below horizontal; RBGS to 1e-4 every 8 iterations. */
+typedef struct cs_pour_params {
+  uint32_t side;
+  uint32_t particles;
+  uint32_t pour_per_substep;
+  float kettle_angle_deg;
+  uint64_t seed;
+} cs_pour_params;
+int cs_workload_build_pour(const cs_pour_params* p, cs_workload* out);
 int cs_workload_free(cs_workload* w);
 
 /* Scene generators exposed for verification (state.rs:250-344). */
