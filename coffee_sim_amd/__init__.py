@@ -110,6 +110,14 @@ def _bind(lib: C.CDLL) -> None:
         "cs_unet_train_epoch": (C.c_int, [C.c_void_p, C.c_float, C.c_uint64, P(C.c_float)]),
         "cs_sim_set_warm_net": (C.c_int, [C.c_void_p, C.c_void_p]),
         "cs_sim_read_warm_start": (C.c_int, [C.c_void_p, P(C.c_float)]),
+        "cs_workload_build_pour": (C.c_int, [P(abi.PourParams), P(Workload)]),
+        "cs_batch_default_config": (C.c_int, [P(abi.BatchConfig)]),
+        "cs_batch_create": (C.c_int, [C.c_void_p, P(C.c_void_p), C.c_uint32, P(abi.BatchConfig), P(C.c_void_p)]),
+        "cs_batch_destroy": (C.c_int, [C.c_void_p]),
+        "cs_batch_set_scene_p2g": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int]),
+        "cs_batch_step_frame": (C.c_int, [C.c_void_p, C.c_float]),
+        "cs_batch_get_stats": (C.c_int, [C.c_void_p, P(abi.BatchStats)]),
+        "cs_batch_reset_stats": (C.c_int, [C.c_void_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -159,8 +167,22 @@ class SeededWorkload:
     def __init__(self, name: str, seed: int = 1, scale: int = 1):
         self._w = Workload()
         self.name = name
+        if name == "pour":  # built by SeededWorkload.pour
+            return
         _check(load_library().cs_workload_build(abi.WORKLOADS[name], seed, scale, C.byref(self._w)),
                "cs_workload_build")
+
+    @classmethod
+    def pour(cls, side: int, particles: int, pour_per_substep: int, kettle_angle_deg: float,
+             seed: int = 1) -> "SeededWorkload":
+        """A pour scene on a cubic grid of `side` cells (a multiple of 16 in
+        32..256) over the working scene's 128 mm domain: 70% of `particles` as a
+        pre-wetted slab in the bed, the rest poured at `pour_per_substep` from a
+        spout tilted `kettle_angle_deg` below horizontal."""
+        w = cls("pour")
+        p = abi.PourParams(side, particles, pour_per_substep, kettle_angle_deg, seed)
+        _check(load_library().cs_workload_build_pour(C.byref(p), C.byref(w._w)), "cs_workload_build_pour")
+        return w
 
     @property
     def settings(self) -> Settings:
@@ -530,6 +552,54 @@ class MpmSim3D:
 
     def reset_kernel_timing(self) -> None:
         _check(load_library().cs_sim_reset_kernel_timing(self._h), "reset_kernel_timing")
+
+
+class Batch:
+    """Many simulations advanced together, one launch per pass over every
+    scene (cs_batch_*).  The simulations stay owned by the caller and are not
+    stepped on their own while the batch exists."""
+
+    def __init__(self, ctx: Context, sims: list, config: abi.BatchConfig | None = None):
+        self.ctx = ctx
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[sm._h.value for sm in self.sims])
+        self._h = C.c_void_p()
+        cfg = C.byref(config) if config is not None else None
+        _check(load_library().cs_batch_create(ctx.handle, arr, len(self.sims), cfg, C.byref(self._h)),
+               "cs_batch_create")
+        _live_sims.add(self)
+        ctx._children.add(self)
+
+    @staticmethod
+    def default_config() -> abi.BatchConfig:
+        c = abi.BatchConfig()
+        _check(load_library().cs_batch_default_config(C.byref(c)), "cs_batch_default_config")
+        return c
+
+    def set_scene_p2g(self, scene: int, tiled: bool) -> None:
+        _check(load_library().cs_batch_set_scene_p2g(self._h, scene, 1 if tiled else 0), "cs_batch_set_scene_p2g")
+
+    def step_frame(self, dt: float) -> None:
+        _check(load_library().cs_batch_step_frame(self._h, dt), "cs_batch_step_frame")
+
+    def stats(self) -> abi.BatchStats:
+        st = abi.BatchStats()
+        _check(load_library().cs_batch_get_stats(self._h, C.byref(st)), "cs_batch_get_stats")
+        return st
+
+    def reset_stats(self) -> None:
+        _check(load_library().cs_batch_reset_stats(self._h), "cs_batch_reset_stats")
+
+    def close(self) -> None:
+        if self._h:
+            _check(load_library().cs_batch_destroy(self._h), "cs_batch_destroy")
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def generate_sdf(settings: Settings) -> np.ndarray:

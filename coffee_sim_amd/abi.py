@@ -209,3 +209,43 @@ class Workload(C.Structure):
         ("total_particles", C.c_uint32),
         ("_owner", C.c_void_p),
     ]
+
+
+class PourParams(C.Structure):
+    """cs_pour_params: one seeded pour scene on a cubic grid of `side` cells."""
+
+    _fields_ = [
+        ("side", C.c_uint32),
+        ("particles", C.c_uint32),
+        ("pour_per_substep", C.c_uint32),
+        ("kettle_angle_deg", C.c_float),
+        ("seed", C.c_uint64),
+    ]
+
+
+class BatchConfig(C.Structure):
+    """cs_batch_config: launch shapes of a batch (zeros pick the defaults)."""
+
+    _fields_ = [
+        ("sweep_groups", C.c_uint32),
+        ("residual_groups", C.c_uint32),
+        ("check_batches", C.c_uint32),
+        ("use_graphs", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("substeps", C.c_uint64),
+        ("particle_substeps", C.c_uint64),
+        ("sweep_launches", C.c_uint64),
+        ("residual_checks", C.c_uint64),
+        ("max_iterations", C.c_uint64),
+        ("substep_us", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}

@@ -263,4 +263,28 @@ __device__ __forceinline__ uint32_t p2g_node(const cs_params& u, const P2GPartic
   return over;
 }
 
+// p2g (shader.rs:955-1052) for one particle: its 162 terms added to the grid
+// lanes with global i32 atomics.
+__device__ __forceinline__ void p2g_atomic_particle(const cs_params& u, const float4* __restrict__ part,
+                                                    const float4* __restrict__ aff, int32_t* __restrict__ lanes,
+                                                    uint32_t* __restrict__ metrics, uint32_t pid) {
+  if (pid >= u.num_particles) return;
+  P2GParticle q;
+  if (!p2g_setup(u, part, aff, pid, q)) return;
+  uint32_t nc = u.nc;
+  uint32_t overflow = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) {
+        int cx = q.bx + i, cy = q.by + j, cz = q.bz + k;
+        if (!on_grid(u, cx, cy, cz)) continue;
+        int32_t v[6];
+        overflow += p2g_node(u, q, i, j, k, v);
+        uint32_t ci = cell_index(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+#pragma unroll
+        for (int l = 0; l < 6; ++l) atomicAdd(&lanes[(size_t)l * nc + ci], v[l]);
+      }
+  if (overflow) atomicAdd(&metrics[CS_M_OVERFLOW], overflow);
+}
+
 }  // namespace

@@ -14,6 +14,7 @@
 #include "cs_common.h"
 #include "cs_kernels.h"
 
+#include "cs_batch.h"
 #include "cs_device.h"
 
 // ---------------------------------------------------------------------------------
@@ -29,30 +30,12 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, size_t n, int32_t v) {
 __global__ void __launch_bounds__(256) k_p2g_atomic(cs_params u, const float4* __restrict__ part,
                                                     const float4* __restrict__ aff, int32_t* __restrict__ lanes,
                                                     uint32_t* __restrict__ metrics) {
-  uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pid >= u.num_particles) return;
-  P2GParticle q;
-  if (!p2g_setup(u, part, aff, pid, q)) return;
-  uint32_t nc = u.nc;
-  uint32_t overflow = 0;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      for (int k = 0; k < 3; ++k) {
-        int cx = q.bx + i, cy = q.by + j, cz = q.bz + k;
-        if (!on_grid(u, cx, cy, cz)) continue;
-        int32_t v[6];
-        overflow += p2g_node(u, q, i, j, k, v);
-        uint32_t ci = cell_index(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
-#pragma unroll
-        for (int l = 0; l < 6; ++l) atomicAdd(&lanes[(size_t)l * nc + ci], v[l]);
-      }
-  if (overflow) atomicAdd(&metrics[CS_M_OVERFLOW], overflow);
+  p2g_atomic_particle(u, part, aff, lanes, metrics, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- grid_update + boundary_project (shader.rs:1056-1082, 1630-1679) -------------
-__global__ void __launch_bounds__(256) k_grid(cs_params u, const int32_t* __restrict__ lanes,
-                                              float4* __restrict__ gvel, const float* __restrict__ sdf) {
-  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void grid_cell(const cs_params& u, const int32_t* __restrict__ lanes,
+                                              float4* __restrict__ gvel, const float* __restrict__ sdf, uint32_t idx) {
   if (idx >= u.nc) return;
   float inv_fp = u.inv_fp_scale;
   float mass = (float)lanes[idx] * inv_fp;
@@ -69,20 +52,23 @@ __global__ void __launch_bounds__(256) k_grid(cs_params u, const int32_t* __rest
   v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
   gvel[idx] = make_float4(v.x, v.y, v.z, mass);
 }
+__global__ void __launch_bounds__(256) k_grid(cs_params u, const int32_t* __restrict__ lanes,
+                                              float4* __restrict__ gvel, const float* __restrict__ sdf) {
+  grid_cell(u, lanes, gvel, sdf, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // ---- classify_cells (shader.rs:1181-1367) + packing_prepare (shader.rs:1566-1583) ----
 // Writes kind, the kind/fill word, pressure 0, the right-hand side c = dx*dx*rhs
 // of the pressure update (shader.rs:1436-1437, from the quantised divergence)
 // and the packing pressure.
-__global__ void __launch_bounds__(256) k_classify(cs_params u, const float4* __restrict__ gvel,
+__device__ __forceinline__ void classify_cell(const cs_params& u, const float4* __restrict__ gvel,
                                                   const uint8_t* __restrict__ cls, const int32_t* __restrict__ lanes,
                                                   const int32_t* __restrict__ bed_id,
                                                   const cs_bed_cell* __restrict__ bed, int8_t* __restrict__ kind_out,
                                                   float* __restrict__ kf_out, int32_t* __restrict__ pres,
                                                   float* __restrict__ rhs, int32_t* __restrict__ pack,
                                                   uint32_t* __restrict__ metrics, const float* __restrict__ sdf,
-                                                  uint32_t* __restrict__ brick_flag) {
-  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+                                                  uint32_t* __restrict__ brick_flag, uint32_t idx) {
   if (idx >= u.nc) return;
   float occ = u.nominal_mass * 0.1f;
   float mass = gvel[idx].w;
@@ -173,13 +159,26 @@ __global__ void __launch_bounds__(256) k_classify(cs_params u, const float4* __r
   if (fires_d) atomicAdd(&metrics[CS_M_DIV_CLAMP], 1u);
   if (fires_p) atomicAdd(&metrics[CS_M_P_CLAMP], 1u);
 }
+__global__ void __launch_bounds__(256) k_classify(cs_params u, const float4* __restrict__ gvel,
+                                                  const uint8_t* __restrict__ cls, const int32_t* __restrict__ lanes,
+                                                  const int32_t* __restrict__ bed_id,
+                                                  const cs_bed_cell* __restrict__ bed, int8_t* __restrict__ kind_out,
+                                                  float* __restrict__ kf_out, int32_t* __restrict__ pres,
+                                                  float* __restrict__ rhs, int32_t* __restrict__ pack,
+                                                  uint32_t* __restrict__ metrics, const float* __restrict__ sdf,
+                                                  uint32_t* __restrict__ brick_flag) {
+  classify_cell(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, metrics, sdf, brick_flag, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // ---- warm starts (injected after classify zeroed the pressure) -------------------
-__global__ void k_warm_previous(cs_params u, const float* __restrict__ kf, const int32_t* __restrict__ prev,
-                                int32_t* __restrict__ pres) {
-  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void warm_previous_cell(const cs_params& u, const float* __restrict__ kf, const int32_t* __restrict__ prev,
+                                int32_t* __restrict__ pres, uint32_t idx) {
   if (idx >= u.nc) return;
   if (kf_fluid(kf[idx])) pres[idx] = prev[idx];
+}
+__global__ void k_warm_previous(cs_params u, const float* __restrict__ kf, const int32_t* __restrict__ prev,
+                                int32_t* __restrict__ pres) {
+  warm_previous_cell(u, kf, prev, pres, blockIdx.x * blockDim.x + threadIdx.x);
 }
 __global__ void k_warm_external(cs_params u, const float* __restrict__ kf, const float* __restrict__ warm,
                                 int32_t* __restrict__ pres, uint32_t* __restrict__ metrics) {
@@ -301,10 +300,9 @@ __global__ void k_residual_final(const double* __restrict__ partials, uint32_t n
 }
 
 // ---- project_pressure + boundary_project (shader.rs:1457-1543, 1630-1679) ---------
-__global__ void __launch_bounds__(256) k_project(cs_params u, float4* __restrict__ gvel, const float* __restrict__ kf,
+__device__ __forceinline__ void project_cell(const cs_params& u, float4* __restrict__ gvel, const float* __restrict__ kf,
                                                  const int32_t* __restrict__ pres, const int32_t* __restrict__ bed_id,
-                                                 const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf) {
-  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf, uint32_t idx) {
   if (idx >= u.nc) return;
   float4 gv = gvel[idx];
   if (gv.w < 1e-6f) return;
@@ -347,11 +345,15 @@ __global__ void __launch_bounds__(256) k_project(cs_params u, float4* __restrict
   v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
   gvel[idx] = make_float4(v.x, v.y, v.z, gv.w);
 }
+__global__ void __launch_bounds__(256) k_project(cs_params u, float4* __restrict__ gvel, const float* __restrict__ kf,
+                                                 const int32_t* __restrict__ pres, const int32_t* __restrict__ bed_id,
+                                                 const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf) {
+  project_cell(u, gvel, kf, pres, bed_id, bed, sdf, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // ---- packing_apply + boundary_project (shader.rs:1585-1626, 1630-1679) ------------
-__global__ void __launch_bounds__(256) k_pack(cs_params u, float4* __restrict__ gvel, const float* __restrict__ kf,
-                                              const int32_t* __restrict__ pack, const float* __restrict__ sdf) {
-  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pack_cell(const cs_params& u, float4* __restrict__ gvel, const float* __restrict__ kf,
+                                              const int32_t* __restrict__ pack, const float* __restrict__ sdf, uint32_t idx) {
   if (idx >= u.nc) return;
   float4 gv = gvel[idx];
   if (gv.w < 1e-6f) return;
@@ -380,11 +382,14 @@ __global__ void __launch_bounds__(256) k_pack(cs_params u, float4* __restrict__ 
   v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
   gvel[idx] = make_float4(v.x, v.y, v.z, gv.w);
 }
+__global__ void __launch_bounds__(256) k_pack(cs_params u, float4* __restrict__ gvel, const float* __restrict__ kf,
+                                              const int32_t* __restrict__ pack, const float* __restrict__ sdf) {
+  pack_cell(u, gvel, kf, pack, sdf, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // ---- viscosity_prepare + viscosity_apply + boundary_project (shader.rs:1086-1177) --
-__global__ void __launch_bounds__(256) k_visc(cs_params u, const float4* __restrict__ gin, float4* __restrict__ gout,
-                                              const uint8_t* __restrict__ cls, const float* __restrict__ sdf) {
-  uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void visc_cell(const cs_params& u, const float4* __restrict__ gin, float4* __restrict__ gout,
+                                              const uint8_t* __restrict__ cls, const float* __restrict__ sdf, uint32_t idx) {
   if (idx >= u.nc) return;
   float occ = u.nominal_mass * 0.1f;
   float4 gv = gin[idx];
@@ -426,12 +431,15 @@ __global__ void __launch_bounds__(256) k_visc(cs_params u, const float4* __restr
   v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
   gout[idx] = make_float4(v.x, v.y, v.z, gv.w);
 }
+__global__ void __launch_bounds__(256) k_visc(cs_params u, const float4* __restrict__ gin, float4* __restrict__ gout,
+                                              const uint8_t* __restrict__ cls, const float* __restrict__ sdf) {
+  visc_cell(u, gin, gout, cls, sdf, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // ---- G2P (shader.rs:1683-1881) ------------------------------------------------------
-__global__ void __launch_bounds__(256) k_g2p(cs_params u, float4* __restrict__ part, float4* __restrict__ aff,
+__device__ __forceinline__ void g2p_particle(const cs_params& u, float4* __restrict__ part, float4* __restrict__ aff,
                                              const float4* __restrict__ gvel, const int32_t* __restrict__ bed_id,
-                                             const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf) {
-  uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+                                             const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf, uint32_t pid) {
   if (pid >= u.num_particles) return;
   float4 P = part[2 * pid], Vv = part[2 * pid + 1];
   float phase = aff[3 * pid].w;
@@ -572,6 +580,11 @@ __global__ void __launch_bounds__(256) k_g2p(cs_params u, float4* __restrict__ p
   aff[3 * pid + 1] = make_float4(C1.x, C1.y, C1.z, 0.0f);
   aff[3 * pid + 2] = make_float4(C2.x, C2.y, C2.z, 0.0f);
 }
+__global__ void __launch_bounds__(256) k_g2p(cs_params u, float4* __restrict__ part, float4* __restrict__ aff,
+                                             const float4* __restrict__ gvel, const int32_t* __restrict__ bed_id,
+                                             const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf) {
+  g2p_particle(u, part, aff, gvel, bed_id, bed, sdf, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // ---- exp check kernel (used by the verification entry point) -----------------------
 __global__ void k_expf(const float* __restrict__ in, float* __restrict__ out, uint32_t n) {
@@ -609,6 +622,83 @@ __global__ void __launch_bounds__(256) k_unet_features(cs_params u, const float*
     const uint64_t bal = __ballot(fluid);
     if ((threadIdx.x & 63u) == 0u && bal) atomicAdd(count, (uint32_t)__popcll(bal));
   }
+}
+
+// ---- batched passes: one launch over every scene of a batch ------------------------------
+// Workgroup g of a batched launch runs block g - start[s] of scene s's own
+// launch (cs_batch_scene_of), with the scene's parameter block copied from the
+// descriptor table.
+#define CS_BATCH_SCENE(space)                                                                         \
+  const uint32_t* st_ = starts + (space) * (n + 1u);                                                  \
+  const uint32_t scene_ = cs_batch_scene_of(st_, n, blockIdx.x);                                      \
+  const cs_scene_dev& d = sc[scene_];                                                                 \
+  const uint32_t blk = blockIdx.x - st_[scene_];                                                      \
+  const cs_params u = d.u;                                                                            \
+  const uint32_t tid = blk * 256u + threadIdx.x
+
+// Emitted particles into the particle arrays (affine zeroed), grid lanes,
+// metric slots, P2G tile counts and brick flags to zero: the host-side
+// append and buffer clears at the start of a substep.
+__global__ void __launch_bounds__(256) k_batch_begin(const cs_scene_dev* __restrict__ sc,
+                                                     const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_CELLS);
+  const uint32_t span = (st_[scene_ + 1u] - st_[scene_]) * 256u;
+  for (uint32_t i = tid; i < d.emit_count; i += span) {
+    const float4* src = reinterpret_cast<const float4*>(d.emit_src) + 2u * (size_t)i;
+    float4* dp = reinterpret_cast<float4*>(d.part) + 2u * ((size_t)d.emit_first + i);
+    float4* da = reinterpret_cast<float4*>(d.aff) + 3u * ((size_t)d.emit_first + i);
+    dp[0] = src[0];
+    dp[1] = src[1];
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    da[0] = z;
+    da[1] = z;
+    da[2] = z;
+  }
+  if (tid < u.nc) {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) d.lanes[(size_t)l * u.nc + tid] = 0;
+  }
+  if (tid < u.nbricks) {
+    d.tile_count[tid] = 0u;
+    d.brick_flag[tid] = 0u;
+  }
+  if (tid < CS_METRICS_SLOTS) d.metrics[tid] = 0u;
+}
+
+__global__ void __launch_bounds__(256) k_batch_grid(const cs_scene_dev* __restrict__ sc,
+                                                    const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_CELLS);
+  grid_cell(u, d.lanes, reinterpret_cast<float4*>(d.gv), d.sdf, tid);
+}
+
+__global__ void __launch_bounds__(256) k_batch_classify(const cs_scene_dev* __restrict__ sc,
+                                                        const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_CELLS);
+  classify_cell(u, reinterpret_cast<const float4*>(d.gv), d.cls, d.lanes, d.bed_id, d.bed, d.kind, d.kf, d.pres,
+                d.rhs, d.pack, d.metrics, d.sdf, d.brick_flag, tid);
+}
+
+// project then pack on the same cell: each reads only its own cell's velocity
+// and neighbours' pressure, packing pressure and kinds, none of which the
+// other pass writes, so the pair equals the two separate passes.
+__global__ void __launch_bounds__(256) k_batch_project_pack(const cs_scene_dev* __restrict__ sc,
+                                                            const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_CELLS);
+  project_cell(u, reinterpret_cast<float4*>(d.gv), d.kf, d.pres, d.bed_id, d.bed, d.sdf, tid);
+  pack_cell(u, reinterpret_cast<float4*>(d.gv), d.kf, d.pack, d.sdf, tid);
+}
+
+__global__ void __launch_bounds__(256) k_batch_visc(const cs_scene_dev* __restrict__ sc,
+                                                    const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_CELLS);
+  visc_cell(u, reinterpret_cast<const float4*>(d.gv), reinterpret_cast<float4*>(d.gv2), d.cls, d.sdf, tid);
+}
+
+__global__ void __launch_bounds__(256) k_batch_g2p(const cs_scene_dev* __restrict__ sc,
+                                                   const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_PARTICLES);
+  g2p_particle(u, reinterpret_cast<float4*>(d.part), reinterpret_cast<float4*>(d.aff),
+               reinterpret_cast<const float4*>(d.gv2), d.bed_id, d.bed, d.sdf, tid);
 }
 
 // ---- launchers -------------------------------------------------------------------------
@@ -708,5 +798,30 @@ hipError_t cs_launch_unet_features(hipStream_t s, const cs_params& u, const floa
                                    float inv_p_scale, float* f4, float* target, uint8_t* mask, uint32_t* count) {
   hipLaunchKernelGGL(k_unet_features, blocks_for(u.nc), dim3(kBlock), 0, s, u, kf, rhs, pres, bed_id, bed, rhs_scale,
                      inv_p_scale, reinterpret_cast<float4*>(f4), target, mask, count);
+  return hipGetLastError();
+}
+
+// ---- batched launchers --------------------------------------------------------------------
+#define CS_BATCH_LAUNCH(kernel, space)                                                              \
+  do {                                                                                            \
+    const uint32_t nb_ = cs_batch_blocks(b, space);                                               \
+    if (nb_) hipLaunchKernelGGL(kernel, dim3(nb_), dim3(kBlock), 0, b.s, b.sc, b.starts, b.n);     \
+  } while (0)
+
+hipError_t cs_batch_launch_begin(const cs_batch_launch& b) {
+  CS_BATCH_LAUNCH(k_batch_begin, CS_SPACE_CELLS);
+  return hipGetLastError();
+}
+
+hipError_t cs_batch_launch_grid_classify(const cs_batch_launch& b) {
+  CS_BATCH_LAUNCH(k_batch_grid, CS_SPACE_CELLS);
+  CS_BATCH_LAUNCH(k_batch_classify, CS_SPACE_CELLS);
+  return hipGetLastError();
+}
+
+hipError_t cs_batch_launch_post(const cs_batch_launch& b) {
+  CS_BATCH_LAUNCH(k_batch_project_pack, CS_SPACE_CELLS);
+  CS_BATCH_LAUNCH(k_batch_visc, CS_SPACE_CELLS);
+  CS_BATCH_LAUNCH(k_batch_g2p, CS_SPACE_PARTICLES);
   return hipGetLastError();
 }

@@ -15,6 +15,7 @@
 #include "cs_common.h"
 #include "cs_kernels.h"
 
+#include "cs_batch.h"
 #include "cs_device.h"
 
 namespace {
@@ -38,10 +39,10 @@ __device__ __forceinline__ uint32_t tile_of(const cs_params& u, int bx, int by, 
 // Pass 1: tile and in-tile rank of every particle.  Lanes of a wavefront that
 // share a tile take one atomic between them (the lowest such lane adds the
 // group's count and hands out consecutive ranks).
-__global__ void __launch_bounds__(256) k_p2g_bin(cs_params u, const float4* __restrict__ part,
+// Every lane of the wavefront must call it (the ranks are wave-aggregated).
+__device__ __forceinline__ void p2g_bin_particle(const cs_params& u, const float4* __restrict__ part,
                                                  const float4* __restrict__ aff, uint32_t* __restrict__ tile_count,
-                                                 uint2* __restrict__ key) {
-  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 uint2* __restrict__ key, uint32_t pid) {
   bool active = false;
   uint32_t tile = kNoTile;
   if (pid < u.num_particles) {
@@ -65,11 +66,15 @@ __global__ void __launch_bounds__(256) k_p2g_bin(cs_params u, const float4* __re
   }
   if (pid < u.num_particles) key[pid] = make_uint2(tile, rank);
 }
+__global__ void __launch_bounds__(256) k_p2g_bin(cs_params u, const float4* __restrict__ part,
+                                                 const float4* __restrict__ aff, uint32_t* __restrict__ tile_count,
+                                                 uint2* __restrict__ key) {
+  p2g_bin_particle(u, part, aff, tile_count, key, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // Pass 2: exclusive scan of the tile counts (one workgroup).
-__global__ void __launch_bounds__(1024) k_p2g_scan(const uint32_t* __restrict__ count, uint32_t ntiles,
-                                                   uint32_t* __restrict__ start) {
-  __shared__ uint32_t s_sum[1024];
+__device__ __forceinline__ void p2g_scan_block(const uint32_t* __restrict__ count, uint32_t ntiles,
+                                               uint32_t* __restrict__ start, uint32_t* s_sum) {
   const uint32_t t = threadIdx.x;
   const uint32_t per = (ntiles + 1023u) / 1024u;
   const uint32_t b0 = min(t * per, ntiles), b1 = min(b0 + per, ntiles);
@@ -89,29 +94,37 @@ __global__ void __launch_bounds__(1024) k_p2g_scan(const uint32_t* __restrict__ 
     run += count[b];
   }
 }
+__global__ void __launch_bounds__(1024) k_p2g_scan(const uint32_t* __restrict__ count, uint32_t ntiles,
+                                                   uint32_t* __restrict__ start) {
+  __shared__ uint32_t s_sum[1024];
+  p2g_scan_block(count, ntiles, start, s_sum);
+}
 
 // Pass 3: the tile-sorted particle list.
-__global__ void __launch_bounds__(256) k_p2g_scatter(uint32_t n, const uint2* __restrict__ key,
-                                                     const uint32_t* __restrict__ start, uint32_t* __restrict__ sorted) {
-  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void p2g_scatter_particle(uint32_t n, const uint2* __restrict__ key,
+                                                     const uint32_t* __restrict__ start, uint32_t* __restrict__ sorted,
+                                                     uint32_t pid) {
   if (pid >= n) return;
   const uint2 k = key[pid];
   if (k.x != kNoTile) sorted[start[k.x] + k.y] = pid;
+}
+__global__ void __launch_bounds__(256) k_p2g_scatter(uint32_t n, const uint2* __restrict__ key,
+                                                     const uint32_t* __restrict__ start, uint32_t* __restrict__ sorted) {
+  p2g_scatter_particle(n, key, start, sorted, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Pass 4: one workgroup per tile.  The tile's particles are dealt to threads
 // in a strided order (thread t takes list positions t*K, t*K + 1, ...), so the
 // lanes of one wavefront work on particles far apart in the list and rarely
 // hit the same LDS word in one instruction.
-__global__ void __launch_bounds__(256) k_p2g_tile(cs_params u, const float4* __restrict__ part,
-                                                  const float4* __restrict__ aff,
-                                                  const uint32_t* __restrict__ tile_count,
-                                                  const uint32_t* __restrict__ start,
-                                                  const uint32_t* __restrict__ sorted, int32_t* __restrict__ lanes,
-                                                  uint32_t* __restrict__ metrics) {
-  constexpr uint32_t NT = 256;
-  __shared__ int32_t s_acc[6 * FP3];
-  const uint32_t tile = blockIdx.x;
+constexpr uint32_t kTileThreads = 256;
+__device__ __forceinline__ void p2g_tile_block(const cs_params& u, const float4* __restrict__ part,
+                                               const float4* __restrict__ aff,
+                                               const uint32_t* __restrict__ tile_count,
+                                               const uint32_t* __restrict__ start,
+                                               const uint32_t* __restrict__ sorted, int32_t* __restrict__ lanes,
+                                               uint32_t* __restrict__ metrics, uint32_t tile, int32_t* s_acc) {
+  constexpr uint32_t NT = kTileThreads;
   const uint32_t n = tile_count[tile];
   if (n == 0) return;
   const uint32_t first = start[tile];
@@ -159,6 +172,72 @@ __global__ void __launch_bounds__(256) k_p2g_tile(cs_params u, const float4* __r
     }
   }
   if (overflow) atomicAdd(&metrics[CS_M_OVERFLOW], overflow);
+}
+__global__ void __launch_bounds__(kTileThreads) k_p2g_tile(cs_params u, const float4* __restrict__ part,
+                                                           const float4* __restrict__ aff,
+                                                           const uint32_t* __restrict__ tile_count,
+                                                           const uint32_t* __restrict__ start,
+                                                           const uint32_t* __restrict__ sorted,
+                                                           int32_t* __restrict__ lanes, uint32_t* __restrict__ metrics) {
+  __shared__ int32_t s_acc[6 * FP3];
+  p2g_tile_block(u, part, aff, tile_count, start, sorted, lanes, metrics, blockIdx.x, s_acc);
+}
+
+// ---- batched passes (one launch over every scene of a batch) -------------------------------
+#define CS_BATCH_SCENE(space)                                                                         \
+  const uint32_t* st_ = starts + (space) * (n + 1u);                                                  \
+  const uint32_t scene_ = cs_batch_scene_of(st_, n, blockIdx.x);                                      \
+  const cs_scene_dev& d = sc[scene_];                                                                 \
+  const uint32_t blk = blockIdx.x - st_[scene_];                                                      \
+  const cs_params u = d.u
+
+// Binning for tiled scenes; the whole P2G for the others (k_p2g_atomic).
+__global__ void __launch_bounds__(256) k_batch_p2g_bin(const cs_scene_dev* __restrict__ sc,
+                                                       const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_PARTICLES);
+  const uint32_t pid = blk * 256u + threadIdx.x;
+  if (d.p2g_tiled)
+    p2g_bin_particle(u, reinterpret_cast<const float4*>(d.part), reinterpret_cast<const float4*>(d.aff),
+                     d.tile_count, d.p2g_key, pid);
+  else
+    p2g_atomic_particle(u, reinterpret_cast<const float4*>(d.part), reinterpret_cast<const float4*>(d.aff), d.lanes,
+                        d.metrics, pid);
+}
+
+__global__ void __launch_bounds__(1024) k_batch_p2g_scan(const cs_scene_dev* __restrict__ sc,
+                                                         const uint32_t* __restrict__ starts, uint32_t n) {
+  __shared__ uint32_t s_sum[1024];
+  CS_BATCH_SCENE(CS_SPACE_SCENES);
+  (void)blk;
+  if (!d.p2g_tiled) return;
+  p2g_scan_block(d.tile_count, u.nbricks, d.tile_start, s_sum);
+}
+
+__global__ void __launch_bounds__(256) k_batch_p2g_scatter(const cs_scene_dev* __restrict__ sc,
+                                                           const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_PARTICLES);
+  if (!d.p2g_tiled) return;
+  p2g_scatter_particle(u.num_particles, d.p2g_key, d.tile_start, d.p2g_sorted, blk * 256u + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(kTileThreads) k_batch_p2g_tile(const cs_scene_dev* __restrict__ sc,
+                                                                 const uint32_t* __restrict__ starts, uint32_t n) {
+  __shared__ int32_t s_acc[6 * FP3];
+  CS_BATCH_SCENE(CS_SPACE_BRICKS);
+  if (!d.p2g_tiled) return;
+  p2g_tile_block(u, reinterpret_cast<const float4*>(d.part), reinterpret_cast<const float4*>(d.aff), d.tile_count,
+                 d.tile_start, d.p2g_sorted, d.lanes, d.metrics, blk, s_acc);
+}
+
+hipError_t cs_batch_launch_p2g(const cs_batch_launch& b) {
+  const uint32_t pb = cs_batch_blocks(b, CS_SPACE_PARTICLES);
+  if (pb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_p2g_bin, dim3(pb), dim3(256), 0, b.s, b.sc, b.starts, b.n);
+  hipLaunchKernelGGL(k_batch_p2g_scan, dim3(b.n), dim3(1024), 0, b.s, b.sc, b.starts, b.n);
+  hipLaunchKernelGGL(k_batch_p2g_scatter, dim3(pb), dim3(256), 0, b.s, b.sc, b.starts, b.n);
+  hipLaunchKernelGGL(k_batch_p2g_tile, dim3(cs_batch_blocks(b, CS_SPACE_BRICKS)), dim3(kTileThreads), 0, b.s, b.sc,
+                     b.starts, b.n);
+  return hipGetLastError();
 }
 
 hipError_t cs_launch_p2g_tiled(hipStream_t s, const cs_params& u, const float* part, const float* aff,

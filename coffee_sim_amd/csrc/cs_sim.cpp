@@ -15,6 +15,7 @@
 #include "cs_ctx.h"
 #include "cs_host.h"
 #include "cs_kernels.h"
+#include "cs_sim_impl.h"
 #include "cs_solver.h"
 #include "cs_unet_host.h"
 
@@ -43,94 +44,13 @@ hipError_t dalloc(T** p, size_t count) {
   return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
 }
 
-enum Pass { P_P2G = 0, P_GRID, P_CLASSIFY, P_SOLVE, P_POST, P_G2P, P_END, P_COUNT };
-
 }  // namespace
 
 void cs_set_last_error(const char* msg) { g_error = msg ? msg : ""; }
+int cs_fail(int code, const char* msg) { return fail(code, msg); }
+int cs_hip_fail(hipError_t e, const char* what) { return hip_fail(e, what); }
 
 
-struct cs_sim {
-  cs_ctx* ctx = nullptr;
-  cs_settings s;
-  uint32_t gx = 0, gy = 0, gz = 0;
-  size_t nc = 0;
-  // device state
-  float* part = nullptr;     // 8 floats per particle
-  float* aff = nullptr;      // 12 floats per particle
-  int32_t* lanes = nullptr;  // [6][nc]
-  float* gv = nullptr;       // float4 per cell
-  float* gv2 = nullptr;      // float4 per cell (after viscosity)
-  float* sdf = nullptr;
-  uint8_t* cls = nullptr;
-  int32_t* bed_id = nullptr;
-  cs_bed_cell* bed = nullptr;
-  uint32_t num_bed_cells = 0;
-  int8_t* kind = nullptr;
-  float* kf = nullptr;
-  int32_t* pres = nullptr;
-  float* rhs = nullptr;
-  int32_t* pack = nullptr;
-  int32_t* prev_p = nullptr;
-  float* warm = nullptr;
-  bool has_warm = false;
-  uint32_t* metrics = nullptr;
-  double* res_partials = nullptr;
-  double* res_out = nullptr;
-  // brick-sparse solve
-  int32_t* pres2 = nullptr;          // second pressure buffer
-  float* wc = nullptr;               // float4 per cell: face weights and diagonal
-  uint32_t* brick_flag = nullptr;    // [nbricks]
-  uint32_t* brick_list = nullptr;    // [nbricks]
-  double* brick_partials = nullptr;  // [2 * nbricks]
-  cs_solve_state* dstate = nullptr;
-  cs_solve_state* h_state = nullptr; // pinned
-  uint32_t* h_done = nullptr;        // pinned, one slot per in-flight batch
-  hipEvent_t batch_ev[2] = {};
-  std::vector<hipEvent_t> chunk_ev;  // start/end pairs for sweep timing
-  bool dense_solver = false;         // one-thread-per-cell reference layout (CS_SOLVER=dense)
-  int halo = 4;                      // half sweeps per launch (CS_RBGS_HALO = 1, 2 or 4)
-  int threads = 512;                 // threads per brick workgroup (CS_RBGS_THREADS = 256 or 512)
-  bool multi = true;                 // halo 4: one prefetching workgroup per compute unit (CS_RBGS_MULTI)
-  uint32_t multi_groups = 256;       // workgroups of the multi-brick launch: the compute-unit count
-  uint32_t batch_chunks = 4;         // residual checks enqueued ahead of the host
-  bool fuse_residual = true;         // residual test inside the multi-brick sweeps (CS_RBGS_FUSE)
-  double* group_partials = nullptr;  // [2 * 1024]: per-workgroup sums of the fused test
-  uint32_t* sweep_table = nullptr;   // per-thread cell lists of the multi-brick sweeps
-  bool use_graphs = true;            // replay the steady-state batch as a hipGraph (CS_GRAPHS)
-  hipGraph_t batch_graph = nullptr;
-  hipGraphExec_t batch_exec = nullptr;
-  uint32_t batch_launches_per_chunk = 0;
-  // tile-binned P2G scratch
-  bool p2g_tiled = true;             // CS_P2G=atomic selects the one-thread-per-particle layout
-  uint32_t* tile_count = nullptr;    // [nbricks]
-  uint32_t* tile_start = nullptr;    // [nbricks]
-  uint2* p2g_key = nullptr;          // [max_particles]
-  uint32_t* p2g_sorted = nullptr;    // [max_particles]
-  cs_unet* warm_net = nullptr;       // network of CS_WARM_LEARNED
-  // host state
-  double* h_res = nullptr;    // pinned
-  float* h_stage = nullptr;   // pinned emission staging
-  size_t h_stage_cap = 0;
-  hipEvent_t stage_done = nullptr;
-  std::vector<float> init_particles, init_affine;
-  uint32_t init_count = 0;
-  cs::Inflow inflow;
-  uint32_t num_water = 0;
-  float total_time = 0.0f;
-  float total_emitted_mass = 0.0f;
-  uint32_t total_dropped = 0;
-  cs_solve_stats stats;
-  bool final_residual_pending = false;
-  bool pending_from_state = false;  // lazily read residual lives in h_state (brick solve)
-  bool timing = false;
-  hipEvent_t ev[P_COUNT] = {};
-  float pass_us[7] = {};
-  // per-kernel timing (cs_kernel_timing): [p2g start, end, g2p start, end, rbgs start, end]
-  hipEvent_t kev[6] = {};
-  cs_kernel_timing kt = {};
-  std::vector<float> emit_buf;
-};
 
 namespace {
 

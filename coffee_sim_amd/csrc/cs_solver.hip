@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cstdlib>
 
+#include "cs_batch.h"
 #include "cs_device.h"
 #include "cs_solver.h"
 
@@ -100,9 +101,8 @@ __device__ __forceinline__ int32_t update_cell(const cs_params& u, float4 self, 
 // `count` accumulated over all six faces in the reference's order
 // (shader.rs:1393-1420); .w = -1 marks a cell that is not fluid.  The kinds
 // and fills do not change during a solve, so this runs once per substep.
-__global__ void __launch_bounds__(256) k_face_weights(cs_params u, const float* __restrict__ kf,
-                                                      float4* __restrict__ wc) {
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void face_weights_cell(const cs_params& u, const float* __restrict__ kf,
+                                                  float4* __restrict__ wc, uint32_t idx) {
   if (idx >= u.nc) return;
   const float skf = kf[idx];
   if (!kf_fluid(skf)) {
@@ -132,12 +132,16 @@ __global__ void __launch_bounds__(256) k_face_weights(cs_params u, const float* 
   }
   wc[idx] = make_float4(wplus[0], wplus[1], wplus[2], count);
 }
+__global__ void __launch_bounds__(256) k_face_weights(cs_params u, const float* __restrict__ kf,
+                                                      float4* __restrict__ wc) {
+  face_weights_cell(u, kf, wc, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // ---- active brick list ----------------------------------------------------------------
 // Single block: ascending list of flagged bricks and their count.
-__global__ void __launch_bounds__(1024) k_brick_scan(const uint32_t* __restrict__ flags, uint32_t nbricks,
-                                                     uint32_t* __restrict__ list, cs_solve_state* __restrict__ st) {
-  __shared__ uint32_t s_cnt[1024];
+__device__ __forceinline__ void brick_scan_block(const uint32_t* __restrict__ flags, uint32_t nbricks,
+                                                 uint32_t* __restrict__ list, cs_solve_state* __restrict__ st,
+                                                 uint32_t* s_cnt) {
   const uint32_t t = threadIdx.x;
   const uint32_t per = (nbricks + 1023u) / 1024u;
   const uint32_t b0 = t * per;
@@ -157,6 +161,11 @@ __global__ void __launch_bounds__(1024) k_brick_scan(const uint32_t* __restrict_
   for (uint32_t b = b0; b < b1; ++b)
     if (flags[b]) list[pos++] = b;
   if (t == 1023u) st->active_bricks = s_cnt[1023];
+}
+__global__ void __launch_bounds__(1024) k_brick_scan(const uint32_t* __restrict__ flags, uint32_t nbricks,
+                                                     uint32_t* __restrict__ list, cs_solve_state* __restrict__ st) {
+  __shared__ uint32_t s_cnt[1024];
+  brick_scan_block(flags, nbricks, list, st, s_cnt);
 }
 
 // ---- red-black sweeps over LDS-resident regions --------------------------------------------
@@ -675,12 +684,11 @@ __global__ void __launch_bounds__(256) k_residual_bricks(cs_params u, const uint
 }
 
 // Fixed-order sum of `n` per-brick partials, then the solve-state update.
-__global__ void __launch_bounds__(1024) k_residual_reduce(const double* __restrict__ partials, uint32_t n,
-                                                          cs_solve_state* __restrict__ st, uint32_t mode,
-                                                          uint32_t iters, uint32_t launches, double tol) {
+// Sum of n (rr, bb) partials, thread-strided then a tree over 1024 threads;
+// valid in s_rr[0], s_bb[0].
+__device__ __forceinline__ void reduce_partials_block(const double* __restrict__ partials, uint32_t n, double* s_rr,
+                                                      double* s_bb) {
   constexpr uint32_t NT = 1024;
-  __shared__ double s_rr[NT], s_bb[NT];
-  if (mode != 0u && st->done) return;
   double rr = 0.0, bb = 0.0;
   for (uint32_t i = threadIdx.x; i < n; i += NT) {
     rr += partials[2 * i];
@@ -696,6 +704,13 @@ __global__ void __launch_bounds__(1024) k_residual_reduce(const double* __restri
     }
     __syncthreads();
   }
+}
+__global__ void __launch_bounds__(1024) k_residual_reduce(const double* __restrict__ partials, uint32_t n,
+                                                          cs_solve_state* __restrict__ st, uint32_t mode,
+                                                          uint32_t iters, uint32_t launches, double tol) {
+  __shared__ double s_rr[1024], s_bb[1024];
+  if (mode != 0u && st->done) return;
+  reduce_partials_block(partials, n, s_rr, s_bb);
   if (threadIdx.x == 0) finalize_state(st, s_rr[0], s_bb[0], mode, iters, launches, tol);
 }
 
@@ -734,6 +749,240 @@ __global__ void k_state_init(cs_solve_state* st) {
     st->residual = 0.0;
     st->initial_residual = 0.0;
   }
+}
+
+// ---- batched solve: every scene of a batch in one launch per step ---------------------------
+// The active bricks of the scenes still solving are compacted into one list of
+// entries (scene << 16 | brick), scene by scene in ascending brick order.  The
+// sweep launch walks that list like k_rbgs_multi walks one scene's list, and
+// the residual is formed per brick with the single-scene helper and reduced
+// per scene in the single-scene order, so each scene's solve equals its own
+// unfused brick solve bit for bit.
+#define CS_BATCH_SCENE(space)                                                                         \
+  const uint32_t* st_ = starts + (space) * (n + 1u);                                                  \
+  const uint32_t scene_ = cs_batch_scene_of(st_, n, blockIdx.x);                                      \
+  const cs_scene_dev& d = sc[scene_];                                                                 \
+  const uint32_t blk = blockIdx.x - st_[scene_];                                                      \
+  const cs_params u = d.u
+
+// Solve-state reset and active-brick scan (k_state_init + k_brick_scan), one
+// workgroup per scene.
+__global__ void __launch_bounds__(1024) k_batch_scan(const cs_scene_dev* __restrict__ sc,
+                                                     const uint32_t* __restrict__ starts, uint32_t n) {
+  __shared__ uint32_t s_cnt[1024];
+  CS_BATCH_SCENE(CS_SPACE_SCENES);
+  (void)blk;
+  if (threadIdx.x == 0) {
+    d.st->done = 0u;
+    d.st->iterations = 0u;
+    d.st->launches = 0u;
+    d.st->pending_fires = 0u;
+    d.st->residual = 0.0;
+    d.st->initial_residual = 0.0;
+  }
+  brick_scan_block(d.brick_flag, u.nbricks, d.brick_list, d.st, s_cnt);
+}
+
+// Face weights, and the previous step's pressure on fluid cells for scenes
+// that start from it (k_face_weights + k_warm_previous).
+__global__ void __launch_bounds__(256) k_batch_face_warm(const cs_scene_dev* __restrict__ sc,
+                                                         const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_CELLS);
+  const uint32_t idx = blk * 256u + threadIdx.x;
+  face_weights_cell(u, d.kf, reinterpret_cast<float4*>(d.wc), idx);
+  if (d.warm_previous && idx < u.nc && kf_fluid(d.kf[idx])) d.pres[idx] = d.prev_p[idx];
+}
+
+// One workgroup per scene: this scene's offset in the compacted list (the
+// active-brick counts of the solving scenes before it), then its entries.
+// The last workgroup also records the list length and the solving-scene count.
+__global__ void __launch_bounds__(256) k_batch_compact(const cs_scene_dev* __restrict__ sc, uint32_t n,
+                                                       uint32_t skip_done, uint32_t* __restrict__ entries,
+                                                       uint32_t* __restrict__ first, uint32_t* __restrict__ count,
+                                                       cs_batch_solve* __restrict__ bs) {
+  __shared__ uint32_t s_sum[256], s_live[256];
+  const uint32_t s = blockIdx.x;
+  const bool last = s + 1u == n;
+  uint32_t before = 0, live = 0;
+  for (uint32_t k = threadIdx.x; k < (last ? n : s); k += 256u) {
+    const cs_solve_state* st = sc[k].st;
+    const bool on = !(skip_done && st->done);
+    if (k < s) before += on ? st->active_bricks : 0u;
+    live += on ? 1u : 0u;
+  }
+  s_sum[threadIdx.x] = before;
+  s_live[threadIdx.x] = live;
+  __syncthreads();
+  for (uint32_t h = 128u; h > 0u; h >>= 1) {
+    if (threadIdx.x < h) {
+      s_sum[threadIdx.x] += s_sum[threadIdx.x + h];
+      s_live[threadIdx.x] += s_live[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  const cs_solve_state* mine = sc[s].st;
+  const uint32_t cnt = (skip_done && mine->done) ? 0u : mine->active_bricks;
+  const uint32_t f = s_sum[0];
+  const uint32_t* list = sc[s].brick_list;
+  for (uint32_t i = threadIdx.x; i < cnt; i += 256u) entries[f + i] = s << CS_BATCH_ENTRY_SCENE_SHIFT | list[i];
+  if (threadIdx.x == 0) {
+    first[s] = f;
+    count[s] = cnt;
+    if (last) {
+      bs->total = f + cnt;
+      bs->live = s_live[0];
+    }
+  }
+}
+
+// Where one entry's brick lives: the scene's parameters and buffers.
+struct BatchBrick {
+  cs_params u;
+  const float4* wc;
+  const float* rhs;
+  const int32_t* pin;
+  int32_t* pout;
+  uint32_t* metrics;
+  uint32_t brick;
+};
+
+__device__ __forceinline__ void batch_brick(const cs_scene_dev* __restrict__ sc, uint32_t entry, uint32_t cur,
+                                            BatchBrick& b) {
+  const cs_scene_dev& d = sc[entry >> CS_BATCH_ENTRY_SCENE_SHIFT];
+  b.u = d.u;
+  b.wc = reinterpret_cast<const float4*>(d.wc);
+  b.rhs = d.rhs;
+  b.pin = cur ? d.pres2 : d.pres;
+  b.pout = cur ? d.pres : d.pres2;
+  b.metrics = d.metrics;
+  b.brick = entry & ((1u << CS_BATCH_ENTRY_SCENE_SHIFT) - 1u);
+}
+
+// k_rbgs_multi over the compacted entries of all solving scenes (without the
+// folded residual test): a fixed grid of workgroups, each walking a run of the
+// list and prefetching the next entry's region while it sweeps the current one.
+template <int H, int NT>
+__global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restrict__ sc,
+                                                   const uint32_t* __restrict__ entries,
+                                                   const cs_batch_solve* __restrict__ bs, uint32_t cur,
+                                                   uint32_t color0, const uint32_t* __restrict__ sweep_table) {
+  using Reg = Region<H, NT>;
+  static_assert(H % 2 == 0, "the listed sweeps assume an even region corner");
+  __shared__ float4 s_wc[Reg::R3];
+  __shared__ float s_rhs[Reg::R3];
+  __shared__ int32_t s_p[Reg::R3];
+  const uint32_t total = bs->total;
+  const uint32_t runs = gridDim.x >= 8u ? 8u : 1u;
+  const uint32_t run = blockIdx.x % runs;
+  const uint32_t per_run = gridDim.x / runs;
+  const uint32_t slot = blockIdx.x / runs;
+  const uint32_t run_lo = (uint32_t)(((uint64_t)total * run) / runs);
+  const uint32_t run_hi = (uint32_t)(((uint64_t)total * (run + 1u)) / runs);
+  uint32_t e = run_lo + slot;
+  if (e >= run_hi) return;
+  BatchBrick cb;
+  batch_brick(sc, entries[e], cur, cb);
+  int x0, y0, z0;
+  brick_origin(cb.u, cb.brick, x0, y0, z0);
+  x0 -= H;
+  y0 -= H;
+  z0 -= H;
+  Reg reg;
+  reg.fetch(cb.u, x0, y0, z0, cb.wc, cb.rhs, cb.pin);
+  uint32_t codes[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k)
+    codes[k] = sweep_table[((color0 ^ (uint32_t)(k & 1)) * H + (uint32_t)k) * NT + threadIdx.x];
+  for (;;) {
+    reg.stage(s_wc, s_rhs, s_p);
+    __syncthreads();
+    const uint32_t ne = e + per_run;
+    BatchBrick nbk;
+    int nx0 = 0, ny0 = 0, nz0 = 0;
+    if (ne < run_hi) {
+      batch_brick(sc, entries[ne], cur, nbk);
+      brick_origin(nbk.u, nbk.brick, nx0, ny0, nz0);
+      nx0 -= H;
+      ny0 -= H;
+      nz0 -= H;
+      reg.fetch(nbk.u, nx0, ny0, nz0, nbk.wc, nbk.rhs, nbk.pin);
+    }
+    const uint32_t fires = sweep_listed<H, NT>(cb.u, codes, s_wc, s_rhs, s_p);
+    store_brick<H, NT>(cb.u, x0, y0, z0, s_p, cb.pout);
+    if (fires) atomicAdd(&cb.metrics[CS_M_P_CLAMP], fires);
+    if (ne >= run_hi) break;
+    __syncthreads();  // the store has read the LDS region before it is restaged
+    e = ne;
+    cb = nbk;
+    x0 = nx0;
+    y0 = ny0;
+    z0 = nz0;
+  }
+}
+
+// Per-entry residual partials of buffer `cur` (a fixed grid striding over the list).
+__global__ void __launch_bounds__(256) k_batch_residual(const cs_scene_dev* __restrict__ sc,
+                                                        const uint32_t* __restrict__ entries,
+                                                        const cs_batch_solve* __restrict__ bs, uint32_t cur,
+                                                        double* __restrict__ partials) {
+  constexpr int R3 = (BR + 2) * (BR + 2) * (BR + 2), NT = 256;
+  __shared__ float4 s_wc[R3];
+  __shared__ int32_t s_p[R3];
+  __shared__ double s_rr[NT], s_bb[NT];
+  const uint32_t total = bs->total;
+  for (uint32_t e = blockIdx.x; e < total; e += gridDim.x) {
+    const uint32_t entry = entries[e];
+    const cs_scene_dev& d = sc[entry >> CS_BATCH_ENTRY_SCENE_SHIFT];
+    const cs_params u = d.u;
+    brick_residual_sums<NT>(u, entry & ((1u << CS_BATCH_ENTRY_SCENE_SHIFT) - 1u),
+                            reinterpret_cast<const float4*>(d.wc), d.rhs, cur ? d.pres2 : d.pres, s_wc, s_p, s_rr,
+                            s_bb);
+    if (threadIdx.x == 0) {
+      partials[2 * e] = s_rr[0];
+      partials[2 * e + 1] = s_bb[0];
+    }
+  }
+}
+
+// One workgroup per scene: the scene's partials reduced as k_residual_reduce
+// reduces them, then its solve state advanced.
+__global__ void __launch_bounds__(1024) k_batch_residual_reduce(const cs_scene_dev* __restrict__ sc,
+                                                                const uint32_t* __restrict__ first,
+                                                                const uint32_t* __restrict__ count,
+                                                                const double* __restrict__ partials, uint32_t mode,
+                                                                uint32_t iters, uint32_t launches) {
+  __shared__ double s_rr[1024], s_bb[1024];
+  const uint32_t s = blockIdx.x;
+  cs_solve_state* st = sc[s].st;
+  if (mode != 0u && st->done) return;
+  reduce_partials_block(partials + 2u * (size_t)first[s], count[s], s_rr, s_bb);
+  if (threadIdx.x == 0) finalize_state(st, s_rr[0], s_bb[0], mode, iters, launches, sc[s].tol);
+}
+
+// The solved pressure into pres (from the second buffer when the launch count
+// is odd), and into prev_p for scenes that start the next solve from it
+// (k_select_final + the copy after the single-scene solve).
+__global__ void __launch_bounds__(256) k_batch_finalize(const cs_scene_dev* __restrict__ sc,
+                                                        const uint32_t* __restrict__ starts, uint32_t n,
+                                                        uint32_t by_state, uint32_t odd_all) {
+  CS_BATCH_SCENE(CS_SPACE_CELLS);
+  const uint32_t idx = blk * 256u + threadIdx.x;
+  if (idx >= u.nc) return;
+  const bool odd = by_state ? (d.st->launches & 1u) != 0u : odd_all != 0u;
+  uint32_t ix, iy, iz;
+  decompose(u, idx, ix, iy, iz);
+  const uint32_t brick = ((iz / BR) * u.nby + iy / BR) * u.nbx + ix / BR;
+  int32_t v = d.pres[idx];
+  if (odd && d.brick_flag[brick]) {
+    v = d.pres2[idx];
+    d.pres[idx] = v;
+  }
+  if (d.warm_previous) d.prev_p[idx] = v;
+}
+
+__global__ void k_batch_gather_states(const cs_scene_dev* __restrict__ sc, uint32_t n,
+                                      cs_solve_state* __restrict__ out) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) out[s] = *sc[s].st;
 }
 
 // ---- launchers ---------------------------------------------------------------------------
@@ -823,5 +1072,52 @@ hipError_t cs_launch_select_final(hipStream_t s, const cs_params& u, uint32_t na
                                   int32_t* a, const int32_t* b, const cs_solve_state* st, bool by_state) {
   if (nactive == 0) return hipSuccess;
   hipLaunchKernelGGL(k_select_final, dim3(nactive), dim3(256), 0, s, u, list, a, b, st, by_state ? 1u : 0u);
+  return hipGetLastError();
+}
+
+// ---- batched launchers --------------------------------------------------------------------
+hipError_t cs_batch_launch_solve_setup(const cs_batch_launch& b) {
+  hipLaunchKernelGGL(k_batch_scan, dim3(b.n), dim3(1024), 0, b.s, b.sc, b.starts, b.n);
+  const uint32_t cb = cs_batch_blocks(b, CS_SPACE_CELLS);
+  if (cb) hipLaunchKernelGGL(k_batch_face_warm, dim3(cb), dim3(256), 0, b.s, b.sc, b.starts, b.n);
+  return hipGetLastError();
+}
+
+hipError_t cs_batch_launch_compact(const cs_batch_launch& b, bool skip_done, uint32_t* entries, uint32_t* first,
+                                   uint32_t* count, cs_batch_solve* bs) {
+  hipLaunchKernelGGL(k_batch_compact, dim3(b.n), dim3(256), 0, b.s, b.sc, b.n, skip_done ? 1u : 0u, entries, first,
+                     count, bs);
+  return hipGetLastError();
+}
+
+hipError_t cs_batch_launch_sweeps(const cs_batch_launch& b, const uint32_t* entries, const cs_batch_solve* bs,
+                                  uint32_t groups, uint32_t cur, uint32_t color0, const uint32_t* sweep_table) {
+  if (groups >= 8u) groups &= ~7u;  // whole runs
+  if (groups == 0u || groups > 4096u) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_batch_rbgs<CS_MULTI_HALO, CS_MULTI_THREADS>), dim3(groups), dim3(CS_MULTI_THREADS), 0, b.s,
+                     b.sc, entries, bs, cur, color0, sweep_table);
+  return hipGetLastError();
+}
+
+hipError_t cs_batch_launch_residual(const cs_batch_launch& b, const uint32_t* entries, const uint32_t* first,
+                                    const uint32_t* count, const cs_batch_solve* bs, uint32_t groups, uint32_t cur,
+                                    double* partials, uint32_t mode, uint32_t iters, uint32_t launches) {
+  if (groups == 0u) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_batch_residual, dim3(groups), dim3(256), 0, b.s, b.sc, entries, bs, cur, partials);
+  hipLaunchKernelGGL(k_batch_residual_reduce, dim3(b.n), dim3(1024), 0, b.s, b.sc, first, count, partials, mode, iters,
+                     launches);
+  return hipGetLastError();
+}
+
+hipError_t cs_batch_launch_finalize(const cs_batch_launch& b, bool by_state, bool odd) {
+  const uint32_t cb = cs_batch_blocks(b, CS_SPACE_CELLS);
+  if (cb)
+    hipLaunchKernelGGL(k_batch_finalize, dim3(cb), dim3(256), 0, b.s, b.sc, b.starts, b.n, by_state ? 1u : 0u,
+                       odd ? 1u : 0u);
+  return hipGetLastError();
+}
+
+hipError_t cs_batch_launch_gather_states(const cs_batch_launch& b, cs_solve_state* out) {
+  hipLaunchKernelGGL(k_batch_gather_states, dim3((b.n + 255u) / 256u), dim3(256), 0, b.s, b.sc, b.n, out);
   return hipGetLastError();
 }

@@ -333,7 +333,8 @@ int cs_device_expf(cs_ctx* ctx, const float* in, float* out, uint32_t n);
  * solid mask, bed porosity (0 outside the bed).  It replaces the zero pressure
  * the reference starts every solve from (shader.rs:1187); RBGS then runs to the
  * tolerance from the prediction and falls back to a zero start when the
- * prediction's residual exceeds 1, so the plain solve remains the judge.
+ * prediction's relative residual exceeds the settings' warm_fallback_residual,
+ * so the plain solve remains the judge.
  * The grid side must be a multiple of 32.  Parameters are one fp32 vector:
  * for each convolution e1a, e1b, e2a, e2b, ba, bb, d2a, d2b, d1a, d1b the
  * weights [27][cin][cout] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) then the bias
@@ -368,6 +369,43 @@ int cs_sim_set_warm_net(cs_sim* sim, cs_unet* net);
 /* The warm-start pressure (float per cell) the last CS_WARM_EXTERNAL / CS_WARM_LEARNED
  * solve started from, before it was injected into the fluid cells. */
 int cs_sim_read_warm_start(cs_sim* sim, float* pressure);
+
+/* ---- batched scenes (no reference counterpart) -----------------------------------
+ * Advances many independent simulations together, one launch per pass over
+ * every scene: each workgroup of a batched launch runs one block of one scene's
+ * own launch.  The pressure solves share one launch per four half sweeps over
+ * the active bricks of the scenes still solving, and every scene stops at its
+ * own tolerance.  Each scene's state after a batched step equals, bit for bit,
+ * the state of stepping it alone with the residual test as a pass of its own
+ * (CS_RBGS_FUSE=0).  Requirements: one context; the brick solver; the same
+ * rbgs_mode, substeps, and rbgs_check_every and rbgs_max_iters (tolerance mode,
+ * both even) or pressure_rbgs_pairs (fixed mode, even); warm_start ZERO or
+ * PREVIOUS; at most 1024 scenes and 65535 bricks per scene.  A simulation in a
+ * batch is not stepped on its own until the batch is destroyed. */
+typedef struct cs_batch cs_batch;
+typedef struct cs_batch_config {
+  uint32_t sweep_groups;     /* workgroups of the persistent sweep launch (0: one per compute unit) */
+  uint32_t residual_groups;  /* workgroups of the persistent residual launch (0: 4 per compute unit) */
+  uint32_t check_batches;    /* residual tests enqueued before the host reads a verdict (0: 4)     */
+  uint32_t use_graphs;       /* replay the fixed launch sequences as hipGraphs                      */
+} cs_batch_config;
+int cs_batch_default_config(cs_batch_config* out);
+int cs_batch_create(cs_ctx* ctx, cs_sim* const* sims, uint32_t count, const cs_batch_config* config,
+                    cs_batch** out);
+int cs_batch_destroy(cs_batch* batch);
+/* P2G layout of one scene: 1 = tile-binned (default), 0 = one thread per particle. */
+int cs_batch_set_scene_p2g(cs_batch* batch, uint32_t scene, int tiled);
+int cs_batch_step_frame(cs_batch* batch, float dt);
+typedef struct cs_batch_stats {
+  uint64_t substeps;           /* batched substeps                                */
+  uint64_t particle_substeps;  /* sum over substeps of the scenes' particle counts */
+  uint64_t sweep_launches;     /* batched sweep launches (four half sweeps each)   */
+  uint64_t residual_checks;    /* batched residual tests                           */
+  uint64_t max_iterations;     /* largest per-scene iteration count of a substep, summed */
+  double substep_us;           /* host wall time inside cs_batch_step_frame       */
+} cs_batch_stats;
+int cs_batch_get_stats(cs_batch* batch, cs_batch_stats* out);
+int cs_batch_reset_stats(cs_batch* batch);
 
 #ifdef __cplusplus
 }
