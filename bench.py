@@ -19,6 +19,12 @@ pressure (`--warm-start zero` starts from zero, which at this size needs more
 than the 20,000-iteration cap); one extra untimed substep from a zero start
 reports that iteration count beside the result.
 
+`--workload batch64` measures the batched form instead: 64 independent pour
+scenes (cubic grids of side 48-96, 20k-400k particles, seeded) advanced by
+one launch per pass, each solved to 1e-4, with launch shapes and per-side P2G
+layouts from the autotuner; `value` is the particle-substeps/s of the whole
+batch.
+
 The JSON line also carries
   roofline      for the kernel with the largest share of device time, from
                 HIP events recorded on the solver's stream over the timed region
@@ -160,11 +166,89 @@ def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, gpu_iters: float,
     }
 
 
+def bench_batch(args) -> int:
+    """64 batched pour scenes (the scene set of coffee_sim_amd.batch)."""
+    import coffee_sim_amd as cs
+    from coffee_sim_amd import autotune
+    from coffee_sim_amd.batch import make_batch, make_sims, pour_specs
+
+    ctx = cs.Context(0)
+    specs = pour_specs(64, seed=args.seed)
+    tuned = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuned = autotune.tune(ctx, specs, cache=args.tune_cache)
+        log(f"autotuner: {tuned} ({time.perf_counter() - t0:.1f}s)")
+    works, sims = make_sims(ctx, specs)
+    batch = make_batch(ctx, sims, autotune.config_from(tuned) if tuned else None,
+                       tuned["p2g_tiled"] if tuned else None)
+    dt = works[0].frame_dt
+    log(f"64 scenes, sides {sorted({s.side for s in specs})}, {sum(s.particle_count() for s in sims)} particles "
+        f"at start; {args.warmup} untimed substeps")
+    for _ in range(args.warmup):
+        batch.step_frame(dt)
+    batch.reset_stats()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        batch.step_frame(dt)
+        its = [s.solve_stats().iterations for s in sims]
+        log(f"substep {k + 1}/{args.steps}: iterations max {max(its)} mean {np.mean(its):.0f}, converged "
+            f"{sum(s.solve_stats().converged for s in sims)}/64, elapsed {time.perf_counter() - t0:.1f}s")
+    elapsed = time.perf_counter() - t0
+    st = batch.stats()
+    log(f"batch stats {st.as_dict()}")
+    achieved = 16.0 * st.cell_updates / (st.solve_us * 1e-6) / 1e9 if st.solve_us > 0 else 0.0
+    out = {
+        "metric": "particle-substeps/s of 64 batched pour scenes; RBGS to 1e-4 residual per scene",
+        "value": round(st.particle_substeps / elapsed, 2),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generators)",
+        "config": {
+            "workload": "64 pour scenes: cubic grids of side 48/64/80/96, 20k-400k particles (log-uniform), "
+                        "kettle 30-60 deg, 500-3000 particles/substep, RBGS to 1e-4 from a previous-step start",
+            "scenes": 64,
+            "seed": args.seed,
+            "particle_substeps": int(st.particle_substeps),
+            "max_iterations_per_substep": round(st.max_iterations / max(st.substeps, 1), 1),
+            "tuned": tuned,
+        },
+        "roofline": {
+            "kernel": "batched rbgs sweeps + residual tests",
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "solve_us": round(st.solve_us, 1),
+            "algorithmic_bytes": 16 * int(st.cell_updates),
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    for s in sims:
+        s.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None, help="timed substeps (20; batch64: 5)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed substeps before timing (3; batch64: 1)")
+    ap.add_argument("--workload", choices=("v60_working", "batch64"), default="v60_working")
+    ap.add_argument("--tune-cache", default=None, help="batch64: JSON cache of the autotuner's choices")
+    ap.add_argument("--no-tune", action="store_true", help="batch64: default launch shapes, tiled P2G")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--scale", type=int, default=1, help="divide the grid side (tests only)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle sampling (0 = skip)")
@@ -180,6 +264,12 @@ def main() -> int:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "run with --gpus 1", file=sys.stderr)
         return 2
+    if args.workload == "batch64":
+        args.steps = 5 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        return bench_batch(args)
+    args.steps = 20 if args.steps is None else args.steps
+    args.warmup = 3 if args.warmup is None else args.warmup
 
     import coffee_sim_amd as cs
 

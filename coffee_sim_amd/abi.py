@@ -245,6 +245,8 @@ class BatchStats(C.Structure):
         ("residual_checks", C.c_uint64),
         ("max_iterations", C.c_uint64),
         ("substep_us", C.c_double),
+        ("solve_us", C.c_double),
+        ("cell_updates", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

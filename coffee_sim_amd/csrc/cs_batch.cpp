@@ -45,14 +45,17 @@ struct cs_batch {
   cs_batch_solve* bs = nullptr;
   double* partials = nullptr;
   cs_solve_state* d_states = nullptr;
+  uint32_t* d_fluid = nullptr;
   // host
   cs_scene_dev* h_sc = nullptr;  // pinned
   float* h_emit = nullptr;       // pinned
   std::vector<uint32_t> h_starts;
   cs_solve_state* h_states = nullptr;  // pinned [n]
+  uint32_t* h_fluid = nullptr;         // pinned [n]
   cs_batch_solve* h_bs = nullptr;      // pinned [2]
   hipEvent_t poll_ev[2] = {};
   hipEvent_t upload_done = nullptr;
+  hipEvent_t solve_ev[2] = {};  // around the solve loop
   hipGraph_t graph[3] = {};
   hipGraphExec_t exec[3] = {};  // before the solve loop, a group of chunks, after the solve
   bool captured = false;
@@ -76,15 +79,18 @@ void free_batch(cs_batch* b) {
     if (b->exec[g]) (void)hipGraphExecDestroy(b->exec[g]);
     if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
   }
-  void* dev[] = {b->d_sc, b->d_starts, b->d_emit, b->entries, b->first, b->count, b->bs, b->partials, b->d_states};
+  void* dev[] = {b->d_sc,    b->d_starts, b->d_emit,   b->entries, b->first,
+                 b->count,   b->bs,       b->partials, b->d_states, b->d_fluid};
   for (void* p : dev)
     if (p) (void)hipFree(p);
-  void* host[] = {b->h_sc, b->h_emit, b->h_states, b->h_bs};
+  void* host[] = {b->h_sc, b->h_emit, b->h_states, b->h_fluid, b->h_bs};
   for (void* p : host)
     if (p) (void)hipHostFree(p);
   for (auto& e : b->poll_ev)
     if (e) (void)hipEventDestroy(e);
   if (b->upload_done) (void)hipEventDestroy(b->upload_done);
+  for (auto& e : b->solve_ev)
+    if (e) (void)hipEventDestroy(e);
   delete b;
 }
 
@@ -178,7 +184,7 @@ int enqueue_post(cs_batch* b, uint32_t cur) {
   const cs_batch_launch L = launch_of(b);
   CS_BHIP(cs_batch_launch_finalize(L, !b->fixed, cur != 0u));
   CS_BHIP(cs_batch_launch_post(L));
-  CS_BHIP(cs_batch_launch_gather_states(L, b->d_states));
+  CS_BHIP(cs_batch_launch_gather_states(L, b->d_states, b->d_fluid));
   return CS_OK;
 }
 
@@ -287,11 +293,13 @@ int batch_substep(cs_batch* b, float sub_dt) {
   }
   if (graphs) CS_BHIP(hipGraphLaunch(b->exec[G_PRE], st));
   else if ((rc = enqueue_pre(b))) return rc;
+  CS_BHIP(hipEventRecord(b->solve_ev[0], st));
   uint32_t cur = 0;
   if (b->fixed) {
     if (graphs) CS_BHIP(hipGraphLaunch(b->exec[G_POST], st));
     else {
       if ((rc = enqueue_fixed_solve(b, &cur))) return rc;
+      CS_BHIP(hipEventRecord(b->solve_ev[1], st));
       if ((rc = enqueue_post(b, cur))) return rc;
     }
     b->stats.sweep_launches += b->pairs / 2u;
@@ -318,6 +326,7 @@ int batch_substep(cs_batch* b, float sub_dt) {
       }
       ++batch;
     }
+    CS_BHIP(hipEventRecord(b->solve_ev[1], st));
     // the post sequence was captured for buffers in place; finalize reads
     // each scene's launch parity from its state, so `cur` does not matter
     if (graphs) CS_BHIP(hipGraphLaunch(b->exec[G_POST], st));
@@ -325,7 +334,12 @@ int batch_substep(cs_batch* b, float sub_dt) {
   }
   CS_BHIP(hipMemcpyAsync(b->h_states, b->d_states, (size_t)b->n * sizeof(cs_solve_state), hipMemcpyDeviceToHost,
                          st));
+  CS_BHIP(hipMemcpyAsync(b->h_fluid, b->d_fluid, (size_t)b->n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   CS_BHIP(hipStreamSynchronize(st));
+  if (!(b->fixed && graphs)) {  // the fixed solve sits inside the post graph, untimed
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b->solve_ev[0], b->solve_ev[1]) == hipSuccess) b->stats.solve_us += ms * 1000.0;
+  }
   uint32_t max_it = 0;
   for (uint32_t s = 0; s < b->n; ++s) {
     cs_sim* m = b->sims[s];
@@ -346,6 +360,7 @@ int batch_substep(cs_batch* b, float sub_dt) {
     m->pending_from_state = false;
     m->total_time += sub_dt;
     max_it = std::max(max_it, S.iterations);
+    b->stats.cell_updates += (uint64_t)S.iterations * b->h_fluid[s];
   }
   if (!b->fixed) {
     b->stats.sweep_launches += max_it / 2u;
@@ -467,6 +482,10 @@ int cs_batch_create(cs_ctx* ctx, cs_sim* const* sims, uint32_t count, const cs_b
   ok(balloc(&b->bs, 1));
   ok(balloc(&b->partials, 2 * total_bricks));
   ok(balloc(&b->d_states, count));
+  ok(balloc(&b->d_fluid, count));
+  ok(hipHostMalloc(reinterpret_cast<void**>(&b->h_fluid), (size_t)count * sizeof(uint32_t)));
+  ok(hipEventCreate(&b->solve_ev[0]));
+  ok(hipEventCreate(&b->solve_ev[1]));
   ok(hipHostMalloc(reinterpret_cast<void**>(&b->h_sc), (size_t)count * sizeof(cs_scene_dev)));
   ok(hipHostMalloc(reinterpret_cast<void**>(&b->h_states), (size_t)count * sizeof(cs_solve_state)));
   ok(hipHostMalloc(reinterpret_cast<void**>(&b->h_bs), 2 * sizeof(cs_batch_solve)));

@@ -106,8 +106,8 @@ hipError_t cs_batch_launch_residual(const cs_batch_launch& b, const uint32_t* en
 // always when `odd`), and into prev_p for previous-step starts.
 hipError_t cs_batch_launch_finalize(const cs_batch_launch& b, bool by_state, bool odd);
 hipError_t cs_batch_launch_post(const cs_batch_launch& b);
-// Solve states of all scenes into `out` [n].
-hipError_t cs_batch_launch_gather_states(const cs_batch_launch& b, cs_solve_state* out);
+// Solve states and fluid-cell counts of all scenes into `out` [n] and `fluid` [n].
+hipError_t cs_batch_launch_gather_states(const cs_batch_launch& b, cs_solve_state* out, uint32_t* fluid);
 
 #if defined(__HIPCC__)
 // The scene of workgroup `g` in a space whose block-start table is `start`

@@ -63,33 +63,41 @@ constexpr int NB_DIR[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 
 // A fixed-point pressure as the update reads it (pressure_load, shader.rs:198).
 __device__ __forceinline__ float p_float(const cs_params& u, int32_t p) { return (float)p * u.inv_fp_scale; }
 
-// pressure_update's fixed-point store of a clamped value: |v| <= pressure_clamp
-// = FP_VALUE_LIMIT - 1 (mod.rs:592), so v * fp_scale lies inside the i32
-// range and truncation equals the saturating conversion.
-__device__ __forceinline__ int32_t to_fixed_clamped(const cs_params& u, float v) { return (int32_t)(v * u.fp_scale); }
+// The LDS copy of a swept region holds each fixed-point pressure p as the
+// float the update reads, p_float(p) = f32(p) * 2^-18.  Every pressure in the
+// buffers is the truncation of a float times 2^18 (the store below), so f32(p)
+// is exact and p = p_float(p) * 2^18 recovers it (fp_scale is the power of two
+// 2^18, state.rs:13).
+__device__ __forceinline__ float p_stage(const cs_params& u, int32_t p) { return p_float(u, p); }
+__device__ __forceinline__ int32_t p_unstage(const cs_params& u, float v) { return (int32_t)(v * u.fp_scale); }
 
 // One update of pressure_update (shader.rs:1386-1438) from precomputed face
-// weights.  `self` holds the +x, +y, +z face weights and the diagonal count of
-// the cell; the -x, -y, -z weights are the +x, +y, +z weights of the neighbours
-// on those sides (the face weight is symmetric).  A face the reference skips
-// carries weight 0, whose product +-0 leaves the sum unchanged (the sum starts
-// at +0 and is never -0), so the result is the reference's bit for bit.
-__device__ __forceinline__ int32_t update_cell(const cs_params& u, float4 self, float srhs, float wxm, float wym,
-                                               float wzm, const int32_t* s_p, int li, int sx, int sy, bool& fired) {
-  float sum = 0.0f;
-  sum += wxm * p_float(u, s_p[li - 1]);
-  sum += self.x * p_float(u, s_p[li + 1]);
-  sum += wym * p_float(u, s_p[li - sx]);
-  sum += self.y * p_float(u, s_p[li + sx]);
-  sum += wzm * p_float(u, s_p[li - sy]);
-  sum += self.z * p_float(u, s_p[li + sy]);
+// weights, on pressures held as p_float values.  `self` holds the +x, +y, +z
+// face weights and the diagonal count of the cell; the -x, -y, -z weights are
+// the +x, +y, +z weights of the neighbours on those sides (the face weight is
+// symmetric).  A face the reference skips carries weight 0, whose product +-0
+// leaves the sum unchanged, so the sum is the reference's bit for bit (up to
+// the sign of an all-zero sum, which no stored pressure can tell apart).
+// Returns p_float of the pressure the reference stores: the clamped value
+// (|v| <= pressure_clamp = FP_VALUE_LIMIT - 1, mod.rs:592, so v * 2^18 fits
+// an i32 and truncation equals the saturating conversion) truncated to the
+// fixed-point grid.  The clamp is max-then-min: for the non-NaN bounds
+// -c < c these return the same value as cs_clamp for every input, NaN included.
+__device__ __forceinline__ float update_cell(const cs_params& u, float4 self, float srhs, float wxm, float wym,
+                                             float wzm, const float* s_p, int li, int sx, int sy, bool& fired) {
+  float sum = wxm * s_p[li - 1];
+  sum += self.x * s_p[li + 1];
+  sum += wym * s_p[li - sx];
+  sum += self.y * s_p[li + sx];
+  sum += wzm * s_p[li - sy];
+  sum += self.z * s_p[li + sy];
   // count > 0 selects the quotient, else 0 (shader.rs:1422-1430); the division
   // runs either way so the update has no branch
   const float quot = (sum - srhs) / self.w;
   const float pn = self.w > 0.0f ? quot : 0.0f;
-  const float cl = cs_clamp(pn, -u.pressure_clamp, u.pressure_clamp);
+  const float cl = fminf(fmaxf(pn, -u.pressure_clamp), u.pressure_clamp);
   fired = cl != pn;
-  return to_fixed_clamped(u, cl);
+  return truncf(cl * u.fp_scale) * u.inv_fp_scale;
 }
 
 }  // namespace
@@ -193,14 +201,14 @@ struct Region {
     }
   }
   // Writes the fetched region to LDS; off-grid cells become non-fluid.
-  __device__ __forceinline__ void stage(float4* s_wc, float* s_rhs, int32_t* s_p) const {
+  __device__ __forceinline__ void stage(const cs_params& u, float4* s_wc, float* s_rhs, float* s_p) const {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int i = (int)threadIdx.x + k * NT;
       if (i < R3) {
         s_wc[i] = on[k] ? wv[k] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
         s_rhs[i] = on[k] ? rv[k] : 0.0f;
-        s_p[i] = on[k] ? pv[k] : 0;
+        s_p[i] = on[k] ? p_stage(u, pv[k]) : 0.0f;
       }
     }
   }
@@ -261,7 +269,7 @@ struct SweepMap {
 template <int H, int NT>
 __device__ __forceinline__ uint32_t sweep_region(const cs_params& u, const SweepMap<H, NT>& map, uint32_t color0,
                                                  uint32_t base_par, const float4* s_wc, const float* s_rhs,
-                                                 int32_t* s_p) {
+                                                 float* s_p) {
   constexpr int R = SweepMap<H, NT>::R;
   uint32_t fires = 0;
   int t = 0;
@@ -276,12 +284,11 @@ __device__ __forceinline__ uint32_t sweep_region(const cs_params& u, const Sweep
         const int li = (int)(code & 0xffffu) + (int)par;
         const float4 self = s_wc[li];
         const float srhs = s_rhs[li];
-        const int32_t pold = s_p[li];
         bool fired;
-        const int32_t np = update_cell(u, self, srhs, s_wc[li - 1].x, s_wc[li - R].y, s_wc[li - R * R].z, s_p, li,
-                                       R, R * R, fired);
+        const float np = update_cell(u, self, srhs, s_wc[li - 1].x, s_wc[li - R].y, s_wc[li - R * R].z, s_p, li, R,
+                                     R * R, fired);
         const bool fluid = self.w >= 0.0f;
-        s_p[li] = fluid ? np : pold;
+        if (fluid) s_p[li] = np;
         const uint32_t inner = (code >> (17u + par)) & 1u;
         fires += (fluid && fired) ? inner : 0u;
       }
@@ -300,7 +307,7 @@ __device__ __forceinline__ uint32_t sweep_region(const cs_params& u, const Sweep
 // for this thread's cell of half sweep s (cs_build_sweep_table).
 template <int H, int NT>
 __device__ __forceinline__ uint32_t sweep_listed(const cs_params& u, const uint32_t (&codes)[H], const float4* s_wc,
-                                                 const float* s_rhs, int32_t* s_p) {
+                                                 const float* s_rhs, float* s_p) {
   constexpr int R = BR + 2 * H;
   uint32_t fires = 0;
 #pragma unroll
@@ -310,12 +317,11 @@ __device__ __forceinline__ uint32_t sweep_listed(const cs_params& u, const uint3
       const int li = (int)(code & 0xffffu);
       const float4 self = s_wc[li];
       const float srhs = s_rhs[li];
-      const int32_t pold = s_p[li];
       bool fired;
-      const int32_t np = update_cell(u, self, srhs, s_wc[li - 1].x, s_wc[li - R].y, s_wc[li - R * R].z, s_p, li, R,
-                                     R * R, fired);
+      const float np = update_cell(u, self, srhs, s_wc[li - 1].x, s_wc[li - R].y, s_wc[li - R * R].z, s_p, li, R,
+                                   R * R, fired);
       const bool fluid = self.w >= 0.0f;
-      s_p[li] = fluid ? np : pold;
+      if (fluid) s_p[li] = np;
       fires += (fluid && fired) ? ((code >> 17) & 1u) : 0u;
     }
     __syncthreads();
@@ -325,14 +331,14 @@ __device__ __forceinline__ uint32_t sweep_listed(const cs_params& u, const uint3
 
 // Writes the brick cells of a swept region (corner x0, y0, z0) to `pout`.
 template <int H, int NT>
-__device__ __forceinline__ void store_brick(const cs_params& u, int x0, int y0, int z0, const int32_t* s_p,
+__device__ __forceinline__ void store_brick(const cs_params& u, int x0, int y0, int z0, const float* s_p,
                                             int32_t* __restrict__ pout) {
   constexpr int R = Region<H, NT>::R;
   for (int j = threadIdx.x; j < BR * BR * BR; j += NT) {
     const int lx = H + j % BR, ly = H + (j / BR) % BR, lz = H + j / (BR * BR);
     const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
     if (gx < (int)u.gx && gy < (int)u.gy && gz < (int)u.gz)
-      pout[((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx] = s_p[(lz * R + ly) * R + lx];
+      pout[((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx] = p_unstage(u, s_p[(lz * R + ly) * R + lx]);
   }
 }
 
@@ -346,7 +352,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* 
   using Reg = Region<H, NT>;
   __shared__ float4 s_wc[Reg::R3];
   __shared__ float s_rhs[Reg::R3];
-  __shared__ int32_t s_p[Reg::R3];
+  __shared__ float s_p[Reg::R3];
   // both scalar loads in flight before the first wait
   const uint32_t done = st->done;
   const uint32_t brick = list[blockIdx.x];
@@ -360,7 +366,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* 
   reg.fetch(u, x0, y0, z0, wc, rhs, pin);
   SweepMap<H, NT> map;
   map.build();
-  reg.stage(s_wc, s_rhs, s_p);
+  reg.stage(u, s_wc, s_rhs, s_p);
   __syncthreads();
   const uint32_t fires = sweep_region<H, NT>(u, map, color0, (uint32_t)(x0 + y0 + z0) & 1u, s_wc, s_rhs, s_p);
   store_brick<H, NT>(u, x0, y0, z0, s_p, pout);
@@ -401,7 +407,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // holds two doubles per wavefront.
 template <int H, int NT>
 __device__ __forceinline__ void brick_residual(const cs_params& u, const float4* s_wc, const float* s_rhs,
-                                               const int32_t* s_p, double* s_red, double& rr_out, double& bb_out) {
+                                               const float* s_p, double* s_red, double& rr_out, double& bb_out) {
   constexpr int R = Region<H, NT>::R;
   constexpr int NW = NT / 64;
   double rr = 0.0, bb = 0.0;
@@ -414,9 +420,9 @@ __device__ __forceinline__ void brick_residual(const cs_params& u, const float4*
     const int off[6] = {-1, 1, -R, R, -R * R, R * R};
     double ap_n = 0.0;
 #pragma unroll
-    for (int n = 0; n < 6; ++n) ap_n += (double)wn[n] * (double)p_float(u, s_p[li + off[n]]);
+    for (int n = 0; n < 6; ++n) ap_n += (double)wn[n] * (double)s_p[li + off[n]];
     const double b = -(double)s_rhs[li];
-    const double ap = (double)count * (double)p_float(u, s_p[li]) - ap_n;
+    const double ap = (double)count * (double)s_p[li] - ap_n;
     if (count > 0.0f) {
       rr += (b - ap) * (b - ap);
       bb += b * b;
@@ -475,10 +481,42 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
   static_assert(H % 2 == 0, "the listed sweeps assume an even region corner");
   __shared__ float4 s_wc[Reg::R3];
   __shared__ float s_rhs[Reg::R3];
-  __shared__ int32_t s_p[Reg::R3];
+  __shared__ float s_p[Reg::R3];
   __shared__ double s_red[2 * NW];
+  __shared__ uint32_t s_list[NT];
   __shared__ uint32_t s_stop;
-  if (st->done) return;
+  // this workgroup's bricks: run r = blockIdx.x % 8 of the list, every `per_run`-th from `slot`
+  const uint32_t runs = gridDim.x >= 8u ? 8u : 1u;
+  const uint32_t run = blockIdx.x % runs;
+  const uint32_t per_run = gridDim.x / runs;  // the host launches a multiple of 8 (or fewer than 8) groups
+  const uint32_t slot = blockIdx.x / runs;
+  const uint32_t run_lo = (uint32_t)(((uint64_t)nactive * run) / runs);
+  const uint32_t run_hi = (uint32_t)(((uint64_t)nactive * (run + 1u)) / runs);
+  const uint32_t b0 = run_lo + slot;
+  const uint32_t nmine = b0 < run_hi ? (run_hi - 1u - b0) / per_run + 1u : 0u;
+  // Every load that does not wait on a verdict is issued first: the solve
+  // state, the first brick and its region, and this workgroup's brick list
+  // (one entry per thread, staged to LDS with the first region), so their
+  // latencies overlap one another and the residual decision.
+  const uint32_t done = st->done;
+  uint32_t my_brick = 0;
+  if (threadIdx.x < nmine) my_brick = list[b0 + threadIdx.x * per_run];
+  int x0 = 0, y0 = 0, z0 = 0;
+  Reg reg;
+  if (nmine) {
+    brick_origin(u, list[b0], x0, y0, z0);
+    x0 -= H;
+    y0 -= H;
+    z0 -= H;
+    reg.fetch(u, x0, y0, z0, wc, rhs, pin);
+  }
+  // this thread's cell of each half sweep: colour color0 ^ (s & 1); the region
+  // corner (brick origin - H, H even) has even parity, so local parity is colour
+  uint32_t codes[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k)
+    codes[k] = sweep_table[((color0 ^ (uint32_t)(k & 1)) * H + (uint32_t)k) * NT + threadIdx.x];
+  if (done) return;
   if (res_mode & CS_RES_DECIDE) {
     double rr = 0.0, bb = 0.0;
     for (uint32_t i = threadIdx.x; i < gridDim.x; i += NT) {
@@ -518,44 +556,25 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
     if (s_stop) return;
   }
   const bool compute = (res_mode & CS_RES_COMPUTE) != 0u;
-  // this workgroup's bricks: run r = blockIdx.x % 8 of the list, every `per_run`-th from `slot`
-  const uint32_t runs = gridDim.x >= 8u ? 8u : 1u;
-  const uint32_t run = blockIdx.x % runs;
-  const uint32_t per_run = gridDim.x / runs;  // the host launches a multiple of 8 (or fewer than 8) groups
-  const uint32_t slot = blockIdx.x / runs;
-  const uint32_t run_lo = (uint32_t)(((uint64_t)nactive * run) / runs);
-  const uint32_t run_hi = (uint32_t)(((uint64_t)nactive * (run + 1u)) / runs);
-  uint32_t b = run_lo + slot;
-  if (b >= run_hi) {
+  if (nmine == 0) {
     if (compute && threadIdx.x == 0) {
       partials[2 * blockIdx.x] = 0.0;
       partials[2 * blockIdx.x + 1] = 0.0;
     }
     return;
   }
-  int x0, y0, z0;
-  brick_origin(u, list[b], x0, y0, z0);
-  x0 -= H;
-  y0 -= H;
-  z0 -= H;
-  Reg reg;
-  reg.fetch(u, x0, y0, z0, wc, rhs, pin);
-  // this thread's cell of each half sweep: colour color0 ^ (s & 1); the region
-  // corner (brick origin - H, H even) has even parity, so local parity is colour
-  uint32_t codes[H];
-#pragma unroll
-  for (int k = 0; k < H; ++k)
-    codes[k] = sweep_table[((color0 ^ (uint32_t)(k & 1)) * H + (uint32_t)k) * NT + threadIdx.x];
+  s_list[threadIdx.x] = my_brick;  // read after the first staging barrier
   uint32_t fires = 0;
   double acc_rr = 0.0, acc_bb = 0.0;  // thread 0: this workgroup's residual sums
-  for (;;) {
-    reg.stage(s_wc, s_rhs, s_p);
+  for (uint32_t k = 0;; ++k) {
+    reg.stage(u, s_wc, s_rhs, s_p);
     __syncthreads();
     // prefetch the next brick; its loads stay in flight through the sweeps
-    const uint32_t nb = b + per_run;
+    const bool more = k + 1u < nmine;
     int nx0 = 0, ny0 = 0, nz0 = 0;
-    if (nb < run_hi) {
-      brick_origin(u, list[nb], nx0, ny0, nz0);
+    if (more) {
+      const uint32_t nb = k + 1u < (uint32_t)NT ? s_list[k + 1u] : list[b0 + (k + 1u) * per_run];
+      brick_origin(u, nb, nx0, ny0, nz0);
       nx0 -= H;
       ny0 -= H;
       nz0 -= H;
@@ -572,9 +591,8 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
     }
     fires += sweep_listed<H, NT>(u, codes, s_wc, s_rhs, s_p);
     store_brick<H, NT>(u, x0, y0, z0, s_p, pout);
-    if (nb >= run_hi) break;
+    if (!more) break;
     __syncthreads();  // the store has read the LDS region before it is restaged
-    b = nb;
     x0 = nx0;
     y0 = ny0;
     z0 = nz0;
@@ -870,7 +888,8 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
   static_assert(H % 2 == 0, "the listed sweeps assume an even region corner");
   __shared__ float4 s_wc[Reg::R3];
   __shared__ float s_rhs[Reg::R3];
-  __shared__ int32_t s_p[Reg::R3];
+  __shared__ float s_p[Reg::R3];
+  __shared__ uint32_t s_list[NT];
   const uint32_t total = bs->total;
   const uint32_t runs = gridDim.x >= 8u ? 8u : 1u;
   const uint32_t run = blockIdx.x % runs;
@@ -878,10 +897,15 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
   const uint32_t slot = blockIdx.x / runs;
   const uint32_t run_lo = (uint32_t)(((uint64_t)total * run) / runs);
   const uint32_t run_hi = (uint32_t)(((uint64_t)total * (run + 1u)) / runs);
-  uint32_t e = run_lo + slot;
-  if (e >= run_hi) return;
+  const uint32_t e0 = run_lo + slot;
+  const uint32_t nmine = e0 < run_hi ? (run_hi - 1u - e0) / per_run + 1u : 0u;
+  if (nmine == 0) return;
+  // this workgroup's entries, one per thread, staged to LDS with the first region
+  uint32_t my_entry = 0;
+  if (threadIdx.x < nmine) my_entry = entries[e0 + threadIdx.x * per_run];
+  uint32_t entry = entries[e0];
   BatchBrick cb;
-  batch_brick(sc, entries[e], cur, cb);
+  batch_brick(sc, entry, cur, cb);
   int x0, y0, z0;
   brick_origin(cb.u, cb.brick, x0, y0, z0);
   x0 -= H;
@@ -893,14 +917,22 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
 #pragma unroll
   for (int k = 0; k < H; ++k)
     codes[k] = sweep_table[((color0 ^ (uint32_t)(k & 1)) * H + (uint32_t)k) * NT + threadIdx.x];
-  for (;;) {
-    reg.stage(s_wc, s_rhs, s_p);
+  s_list[threadIdx.x] = my_entry;  // read after the first staging barrier
+  for (uint32_t k = 0;; ++k) {
+    reg.stage(cb.u, s_wc, s_rhs, s_p);
     __syncthreads();
-    const uint32_t ne = e + per_run;
+    const bool more = k + 1u < nmine;
     BatchBrick nbk;
     int nx0 = 0, ny0 = 0, nz0 = 0;
-    if (ne < run_hi) {
-      batch_brick(sc, entries[ne], cur, nbk);
+    if (more) {
+      const uint32_t ne = k + 1u < (uint32_t)NT ? s_list[k + 1u] : entries[e0 + (k + 1u) * per_run];
+      if ((ne >> CS_BATCH_ENTRY_SCENE_SHIFT) == (entry >> CS_BATCH_ENTRY_SCENE_SHIFT)) {
+        nbk = cb;  // same scene: only the brick changes
+        nbk.brick = ne & ((1u << CS_BATCH_ENTRY_SCENE_SHIFT) - 1u);
+      } else {
+        batch_brick(sc, ne, cur, nbk);
+      }
+      entry = ne;
       brick_origin(nbk.u, nbk.brick, nx0, ny0, nz0);
       nx0 -= H;
       ny0 -= H;
@@ -910,9 +942,8 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
     const uint32_t fires = sweep_listed<H, NT>(cb.u, codes, s_wc, s_rhs, s_p);
     store_brick<H, NT>(cb.u, x0, y0, z0, s_p, cb.pout);
     if (fires) atomicAdd(&cb.metrics[CS_M_P_CLAMP], fires);
-    if (ne >= run_hi) break;
+    if (!more) break;
     __syncthreads();  // the store has read the LDS region before it is restaged
-    e = ne;
     cb = nbk;
     x0 = nx0;
     y0 = ny0;
@@ -981,8 +1012,11 @@ __global__ void __launch_bounds__(256) k_batch_finalize(const cs_scene_dev* __re
 }
 
 __global__ void k_batch_gather_states(const cs_scene_dev* __restrict__ sc, uint32_t n,
-                                      cs_solve_state* __restrict__ out) {
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) out[s] = *sc[s].st;
+                                      cs_solve_state* __restrict__ out, uint32_t* __restrict__ fluid) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    out[s] = *sc[s].st;
+    fluid[s] = sc[s].metrics[CS_M_FLUID_CELLS];
+  }
 }
 
 // ---- launchers ---------------------------------------------------------------------------
@@ -1117,7 +1151,7 @@ hipError_t cs_batch_launch_finalize(const cs_batch_launch& b, bool by_state, boo
   return hipGetLastError();
 }
 
-hipError_t cs_batch_launch_gather_states(const cs_batch_launch& b, cs_solve_state* out) {
-  hipLaunchKernelGGL(k_batch_gather_states, dim3((b.n + 255u) / 256u), dim3(256), 0, b.s, b.sc, b.n, out);
+hipError_t cs_batch_launch_gather_states(const cs_batch_launch& b, cs_solve_state* out, uint32_t* fluid) {
+  hipLaunchKernelGGL(k_batch_gather_states, dim3((b.n + 255u) / 256u), dim3(256), 0, b.s, b.sc, b.n, out, fluid);
   return hipGetLastError();
 }

@@ -403,6 +403,9 @@ typedef struct cs_batch_stats {
   uint64_t residual_checks;    /* batched residual tests                           */
   uint64_t max_iterations;     /* largest per-scene iteration count of a substep, summed */
   double substep_us;           /* host wall time inside cs_batch_step_frame       */
+  double solve_us;             /* device time of the solve loops (sweeps and residual tests),
+                                  HIP events on the batch's stream              */
+  uint64_t cell_updates;       /* sum over scenes and substeps of iterations x fluid cells */
 } cs_batch_stats;
 int cs_batch_get_stats(cs_batch* batch, cs_batch_stats* out);
 int cs_batch_reset_stats(cs_batch* batch);
