@@ -241,12 +241,80 @@ def bench_batch(args) -> int:
     return 0
 
 
+# (cin, cout, level) of the ten 3x3x3 convolutions of the warm-start U-Net
+UNET_CONVS = ((16, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2), (96, 32, 1),
+              (32, 32, 1), (48, 16, 0), (16, 16, 0))
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def unet_flops(side: int) -> tuple[float, float]:
+    """Multiply-add FLOPs (x2) of one forward pass and of one backward pass
+    (input gradient + weight gradient of every convolution except the first's
+    input gradient) of the U-Net at `side`^3, as the MFMA tiles compute them
+    (16 input channels at the first convolution)."""
+    fwd = bwd = 0.0
+    for k, (ci, co, lev) in enumerate(UNET_CONVS):
+        f = 2.0 * (side >> lev) ** 3 * 27 * ci * co
+        fwd += f
+        bwd += f * (1 if k == 0 else 2)
+    return fwd, bwd
+
+
+def bench_unet(args) -> int:
+    """Device time of the learned warm start's network at 128^3: one forward
+    pass (the cost added to a solve) and one forward + backward pass (a
+    training step without the optimizer), against the MFMA bf16 peak."""
+    import coffee_sim_amd as cs
+
+    ctx = cs.Context(0)
+    side = 128
+    net = cs.UNet(ctx, side, seed=args.seed)
+    rng = np.random.default_rng(args.seed)
+    f4 = rng.normal(0.0, 1.0, size=(side ** 3, 4)).astype(np.float32)
+    target = rng.normal(0.0, 1.0, size=side ** 3).astype(np.float32)
+    mask = (rng.random(side ** 3) < 0.5).astype(np.uint8)
+    net.gradient(f4, target, mask)  # uploads the input, the target and the mask
+    net.benchmark(max(args.warmup, 1))
+    fwd_us, grad_us = net.benchmark(args.steps)
+    f_fwd, f_bwd = unet_flops(side)
+    fwd_tf = f_fwd / (fwd_us * 1e-6) / 1e12
+    grad_tf = (f_fwd + f_bwd) / (grad_us * 1e-6) / 1e12
+    log(f"forward {fwd_us:.1f} us ({f_fwd / 1e9:.1f} GFLOP, {fwd_tf:.1f} TFLOP/s); forward+backward "
+        f"{grad_us:.1f} us ({(f_fwd + f_bwd) / 1e9:.1f} GFLOP, {grad_tf:.1f} TFLOP/s)")
+    out = {
+        "metric": "U-Net warm-start forward passes/s at 128^3 (bf16 MFMA, fp32 accumulate)",
+        "value": round(1e6 / fwd_us, 3),
+        "unit": "forward passes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fwd_us / 1000.0, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded normal inputs)",
+        "config": {"workload": "3-level 3-D U-Net 16/32/64 channels, 3x3x3, on a 128^3 grid",
+                   "forward_gflop": round(f_fwd / 1e9, 2), "backward_gflop": round(f_bwd / 1e9, 2),
+                   "forward_backward_us": round(grad_us, 1),
+                   "forward_backward_tflops": round(grad_tf, 2)},
+        "roofline": {"kernel": "U-Net forward (all launches)", "bound": "mfma", "achieved": round(fwd_tf, 2),
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(fwd_tf / MFMA_BF16_PEAK_TFLOPS, 5), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    net.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed substeps (20; batch64: 5)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed substeps before timing (3; batch64: 1)")
-    ap.add_argument("--workload", choices=("v60_working", "batch64"), default="v60_working")
+    ap.add_argument("--workload", choices=("v60_working", "batch64", "unet128"), default="v60_working")
     ap.add_argument("--tune-cache", default=None, help="batch64: JSON cache of the autotuner's choices")
     ap.add_argument("--no-tune", action="store_true", help="batch64: default launch shapes, tiled P2G")
     ap.add_argument("--seed", type=int, default=1)
@@ -268,6 +336,8 @@ def main() -> int:
         args.steps = 5 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
         return bench_batch(args)
+    if args.workload == "unet128":
+        return bench_unet(args)
     args.steps = 20 if args.steps is None else args.steps
     args.warmup = 3 if args.warmup is None else args.warmup
 

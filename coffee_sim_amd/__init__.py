@@ -109,6 +109,7 @@ def _bind(lib: C.CDLL) -> None:
         "cs_unet_read_sample": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_float), P(C.c_float), P(C.c_uint8)]),
         "cs_unet_train_epoch": (C.c_int, [C.c_void_p, C.c_float, C.c_uint64, P(C.c_float)]),
         "cs_sim_set_warm_net": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "cs_unet_benchmark": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double), P(C.c_double)]),
         "cs_sim_read_warm_start": (C.c_int, [C.c_void_p, P(C.c_float)]),
         "cs_workload_build_pour": (C.c_int, [P(abi.PourParams), P(Workload)]),
         "cs_batch_default_config": (C.c_int, [P(abi.BatchConfig)]),
@@ -380,6 +381,12 @@ class UNet:
         m = np.empty(self.nv, np.uint8)
         _check(load_library().cs_unet_read_sample(self._h, i, _fptr(f), _fptr(t), _u8ptr(m)), "cs_unet_read_sample")
         return f, t, m
+
+    def benchmark(self, reps: int = 10) -> tuple[float, float]:
+        """Device microseconds of one forward and one forward + backward pass on the last input."""
+        f, g = C.c_double(), C.c_double()
+        _check(load_library().cs_unet_benchmark(self._h, reps, C.byref(f), C.byref(g)), "cs_unet_benchmark")
+        return float(f.value), float(g.value)
 
     def train_epoch(self, lr: float, seed: int = 0) -> float:
         loss = C.c_float()

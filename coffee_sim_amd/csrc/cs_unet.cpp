@@ -419,6 +419,32 @@ int cs_unet_train_step(cs_unet* u, const float* f4, const float* target, const u
   return rc;
 }
 
+int cs_unet_benchmark(cs_unet* u, uint32_t reps, double* forward_us, double* gradient_us) {
+  if (!u || reps == 0) return CS_ERR_INVALID;
+  hipStream_t st = u->ctx->stream;
+  hipEvent_t ev[3] = {};
+  for (auto& e : ev) U_HIP(hipEventCreate(&e));
+  int rc = CS_OK;
+  const float inv = 1.0f / (float)u->nv;
+  U_HIP(hipEventRecord(ev[0], st));
+  for (uint32_t r = 0; r < reps && rc == CS_OK; ++r) rc = forward(u);
+  U_HIP(hipEventRecord(ev[1], st));
+  for (uint32_t r = 0; r < reps && rc == CS_OK; ++r) {
+    rc = forward(u);
+    if (!rc) rc = backward(u, u->target, u->mask, inv);
+  }
+  U_HIP(hipEventRecord(ev[2], st));
+  U_HIP(hipEventSynchronize(ev[2]));
+  float ms[2] = {0.0f, 0.0f};
+  U_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
+  U_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  if (forward_us) *forward_us = 1000.0 * ms[0] / reps;
+  if (gradient_us) *gradient_us = 1000.0 * ms[1] / reps;
+  return CS_OK;
+}
+
 int cs_unet_dataset_size(const cs_unet* u, uint32_t* count) {
   if (!u || !count) return CS_ERR_INVALID;
   *count = (uint32_t)u->data.size();

@@ -364,6 +364,11 @@ int cs_unet_dataset_clear(cs_unet* net);
 int cs_unet_read_sample(cs_unet* net, uint32_t index, float* features4, float* target, uint8_t* mask);
 /* One pass over the captured samples in a seeded random order, one Adam step each. */
 int cs_unet_train_epoch(cs_unet* net, float lr, uint64_t seed, float* mean_loss);
+/* Device time of the network on its last input (cs_unet_forward / _gradient /
+ * _train_step), measured with HIP events on the context's stream over `reps`
+ * repetitions: one forward pass, and one forward + backward pass (loss against
+ * the last target and mask; the parameters are not changed). */
+int cs_unet_benchmark(cs_unet* net, uint32_t reps, double* forward_us, double* gradient_us);
 /* The network a simulation uses when its settings ask for CS_WARM_LEARNED (NULL detaches). */
 int cs_sim_set_warm_net(cs_sim* sim, cs_unet* net);
 /* The warm-start pressure (float per cell) the last CS_WARM_EXTERNAL / CS_WARM_LEARNED
