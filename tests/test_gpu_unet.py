@@ -148,17 +148,24 @@ def test_learned_start_inside_the_solve_matches_the_host_path_and_the_oracle(ctx
 
 def test_trained_network_cuts_rbgs_iterations(ctx):
     """Pairs dumped by the plain solver from six seeded 1/4-size pours (96
-    solves) train the network on the device for 40 epochs; on a held-out seed
-    its prediction reaches the 1e-4 residual in at least 25% fewer RBGS
-    iterations than the zero start (about 2x fewer when this was written).  A
-    prediction is accepted while its initial residual is below 10: its excess
-    over the zero start's residual of 1 is high-frequency error that the first
-    sweeps remove."""
+    solves, each run to 1e-7 or 40,000 iterations so the target carries little
+    of the solver's own smooth error) train the network on the device for 40
+    epochs; on a held-out seed its prediction reaches the 1e-4 residual in at
+    least 25% fewer RBGS iterations than the zero start (1.9x fewer when this
+    was written).  A prediction is accepted while its initial residual is
+    below 10: its excess over the zero start's residual of 1 is high-frequency
+    error that the first sweeps remove."""
     net = cs.UNet(ctx, N, seed=2)
     train_seeds, held_out = (11, 12, 13, 14, 16, 17), 15
     # scales from the data: rms of the right-hand side and of the pressure
+    def pair_settings(w):
+        s = w.settings.copy()
+        s.rbgs_tolerance = 1e-7
+        s.rbgs_max_iters = 40000
+        return s
+
     probe = cs.SeededWorkload("v60_working", seed=train_seeds[0], scale=4)
-    sp = cs.MpmSim3D(ctx, probe.settings, probe.scene)
+    sp = cs.MpmSim3D(ctx, pair_settings(probe), probe.scene)
     sp.step_frame(probe.frame_dt)
     net.capture(sp)
     f4, t, m = net.read_sample(0)
@@ -169,12 +176,12 @@ def test_trained_network_cuts_rbgs_iterations(ctx):
     sp.close()
     for sd in train_seeds:
         wk = cs.SeededWorkload("v60_working", seed=sd, scale=4)
-        sim = cs.MpmSim3D(ctx, wk.settings, wk.scene)
+        sim = cs.MpmSim3D(ctx, pair_settings(wk), wk.scene)
         for _ in range(16):
             sim.step_frame(wk.frame_dt)
             net.capture(sim)
         sim.close()
-    losses = [net.train_epoch(2e-3 if e < 30 else 4e-4, seed=e) for e in range(40)]
+    losses = [net.train_epoch(2e-3 * 0.2 ** (e / 39), seed=e) for e in range(40)]
     assert losses[-1] < 0.05 * losses[0], losses
     wk = cs.SeededWorkload("v60_working", seed=held_out, scale=4)
     plain = cs.MpmSim3D(ctx, wk.settings, wk.scene)
