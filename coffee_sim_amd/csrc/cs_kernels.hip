@@ -253,6 +253,7 @@ __global__ void __launch_bounds__(256) k_residual(cs_params u, const float* __re
       if (kf_fluid(nkf)) nb += (double)w * (double)((float)pres[ni] * u.inv_fp_scale);
     }
     if (count <= 0.0f) continue;
+    if (fabsf((float)pres[idx] * u.inv_fp_scale) == u.pressure_clamp) continue;  // the bound replaces the row
     double ap = (double)count * (double)((float)pres[idx] * u.inv_fp_scale) - nb;
     double b = -(double)rhs[idx];
     rr += (b - ap) * (b - ap);

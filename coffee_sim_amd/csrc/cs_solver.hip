@@ -374,7 +374,9 @@ __global__ void __launch_bounds__(NT) k_rbgs_brick(cs_params u, const uint32_t* 
 }
 
 // ---- residual helpers ---------------------------------------------------------------------
-// Relative residual from the sums of squares (A14: r = |b - Ap| / |b| over fluid cells).
+// Relative residual from the sums of squares (A14: r = |b - Ap| / |b| over fluid
+// cells, leaving out cells held at the pressure clamp, whose row the bound
+// replaces).
 __device__ __forceinline__ double relative_residual(double rr, double bb) {
   if (bb > 0.0) return sqrt(rr / bb);
   return rr > 0.0 ? __builtin_huge_val() : 0.0;
@@ -423,7 +425,7 @@ __device__ __forceinline__ void brick_residual(const cs_params& u, const float4*
     for (int n = 0; n < 6; ++n) ap_n += (double)wn[n] * (double)s_p[li + off[n]];
     const double b = -(double)s_rhs[li];
     const double ap = (double)count * (double)s_p[li] - ap_n;
-    if (count > 0.0f) {
+    if (count > 0.0f && fabsf(s_p[li]) != u.pressure_clamp) {
       rr += (b - ap) * (b - ap);
       bb += b * b;
     }
@@ -660,6 +662,7 @@ __device__ __forceinline__ void brick_residual_sums(const cs_params& u, uint32_t
     const float4 self = s_wc[li];
     const float count = self.w;
     if (count <= 0.0f) continue;  // not fluid (-1), or a fluid cell with no used face
+    if (fabsf(p_float(u, s_p[li])) == u.pressure_clamp) continue;  // held at the bound: not a row of the system
     // residual_rows order: -x, +x, -y, +y, -z, +z; unused faces carry weight 0
     const float wn[6] = {s_wc[li - 1].x, self.x, s_wc[li - R].y, self.y, s_wc[li - R * R].z, self.z};
     const int off[6] = {-1, 1, -R, R, -R * R, R * R};

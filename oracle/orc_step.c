@@ -560,7 +560,11 @@ static void k_rbgs_half(orc_sim* S, const orc_params* u, uint32_t parity) {
 
 /* Relative residual r = ||b - A p||_2 / ||b||_2 over fluid cells with a
  * non-empty stencil, where the row of cell i reads count_i p_i - sum_n w_n p_n = -c_i
- * (the fixed point of pressure_update); summed in ascending cell order. */
+ * (the fixed point of pressure_update); summed in ascending cell order.
+ * A cell whose pressure sits at the clamp (|p| = pressure_clamp, the bound
+ * pressure_update enforces through pressure_store, shader.rs:202-209) is left out: there the bound
+ * replaces the row, so the clamped iteration's fixed point leaves that row
+ * unsatisfied and it would hold the residual above any tolerance. */
 static double residual_rows(orc_sim* S, const orc_params* u) {
   double rr = 0.0, bb = 0.0;
   size_t i0 = 0, i1 = 0;
@@ -569,6 +573,7 @@ static double residual_rows(orc_sim* S, const orc_params* u) {
     if (i1 >= g_nrows[1] || (i0 < g_nrows[0] && g_rows[0][i0].idx < g_rows[1][i1].idx)) r = &g_rows[0][i0++];
     else r = &g_rows[1][i1++];
     if (r->count <= 0.0f) continue;
+    if (fabsf(pressure_load(S, u, r->idx)) == u->pressure_clamp) continue;
     double ap = (double)r->count * (double)pressure_load(S, u, r->idx);
     for (int k = 0; k < 6; ++k)
       if (r->w[k] > 0.0f && r->nf[k]) ap -= (double)r->w[k] * (double)pressure_load(S, u, r->nb[k]);
