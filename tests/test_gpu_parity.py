@@ -210,11 +210,10 @@ def test_v60_working_full_size_fixed_pairs(ctx, orc, monkeypatch, variant):
 @pytest.mark.parametrize("variant", [{}, {"CS_RBGS_GROUPS": "8"}, {"CS_RBGS_FUSE": "0"}],
                          ids=["fused", "fused-8groups", "unfused"])
 def test_residual_test_with_clamp_fires_matches_oracle(ctx, orc, monkeypatch, variant):
-    """A short substep (dt x 0.2) drives the pressure into its clamp while the
-    solve runs to a loose tolerance: the first substep converges after 208
-    iterations with clamp fires in every sweep, so the sweeps the device runs
-    past the converged state must not count; the second stops at the cap.
-    Pressure, iteration counts and the clamp-fire counter match the oracle."""
+    """A short substep (dt x 0.2) fires the clamps while the solve runs to a
+    loose tolerance, so the sweeps the device runs past the converged state
+    must not count.  Pressure, iteration counts and the clamp-fire counter
+    match the oracle."""
     for k, v in variant.items():
         monkeypatch.setenv(k, v)
     w = cs.SeededWorkload("v60_working", seed=3, scale=4)
@@ -265,3 +264,28 @@ def test_p2g_lanes_at_domain_edges_match_oracle(ctx, orc, monkeypatch, variant):
     ref.step_frame(1.0 / 600.0)
     assert_bitwise("p2g lanes", sim.read_p2g_lanes(), ref.read_p2g_lanes())
     compare_state(sim, ref, f"edge P2G {variant}")
+
+
+@pytest.mark.parametrize("variant", [{}, {"CS_RBGS_FUSE": "0"}, {"CS_SOLVER": "dense"}],
+                         ids=["fused", "unfused", "dense"])
+def test_solve_with_cells_held_at_the_clamp_reaches_tolerance(ctx, orc, monkeypatch, variant):
+    """A block falling at 20 units/s into a cup drives cells to the pressure
+    clamp; the residual leaves those cells out, so both solves reach 1e-4 after
+    the same iterations with the same state."""
+    from scenes import SceneArrays, block_particles, box_settings
+
+    for k, v in variant.items():
+        monkeypatch.setenv(k, v)
+    s = box_settings(n=16, cup=True, tolerance=1e-4)
+    s.rbgs_max_iters = 4000
+    sc = SceneArrays(block_particles(s, (4, 3, 4), (12, 9, 12), vel=(0.0, -20.0, 0.0)))
+    sim, ref = _pair(ctx, orc, s, sc.scene)
+    for k in range(3):
+        sim.step_frame(1e-3)
+        ref.step_frame(1e-3)
+        sg, so = sim.solve_stats(), ref.solve_stats()
+        assert (sg.iterations, sg.converged) == (so.iterations, so.converged), (k, sg.as_dict(), so.as_dict())
+        assert so.converged == 1
+        compare_state(sim, ref, f"clamp-held {variant} substep {k}")
+    p, _ = ref.read_pressure()
+    assert np.any(np.abs(p) == 8191 * 262144)

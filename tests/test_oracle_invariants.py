@@ -280,3 +280,46 @@ def test_viscosity_blends_toward_neighbours(orc):
     # the lateral neighbours of a cell deep in the top block move with it
     deep = cell_index(s, 8, 11, 8)
     assert after[deep, 0] > 0.0
+
+
+def test_residual_leaves_out_cells_held_at_the_clamp(orc):
+    """A block falling at 20 units/s into a cup: its impact drives cells to the
+    pressure clamp (|p| = FP_VALUE_LIMIT - 1 = 8191).  The reported residual is
+    ||b - A p|| / ||b|| of the system rebuilt here from the oracle's rows, over
+    the fluid cells whose pressure is not at the clamp, and the solve reaches
+    the 1e-4 tolerance; with the held cells included the residual could not."""
+    s = box_settings(n=16, cup=True, tolerance=1e-4)
+    s.rbgs_max_iters = 4000
+    parts = block_particles(s, (4, 3, 4), (12, 9, 12), vel=(0.0, -20.0, 0.0))
+    sc = SceneArrays(parts)
+    o = orc.OracleSim(s, sc.scene)
+    o.debug_step_until(1e-3, "rbgs")
+    st = o.solve_stats()
+    assert st.converged == 1 and st.iterations < 4000
+    wts, count, c = o.read_system()
+    p_fp, kind = o.read_pressure()
+    p = p_fp.astype(np.float64) / FP
+    fluid = np.nonzero((kind >= 1) & (kind <= 3) & (count > 0))[0]
+    held = fluid[np.abs(p[fluid]) == 8191.0]
+    assert held.size > 10, "the impact should drive cells to the clamp"
+    n = s.grid_dims[0]
+    off = [-1, 1, -n, n, -n * n, n * n]
+    fluid_set = set(int(i) for i in fluid)
+
+    def residual(rows):
+        rr = bb = 0.0
+        for i in rows:
+            ap = float(count[i]) * p[i]
+            for d in range(6):
+                j = int(i) + off[d]
+                if wts[i, d] > 0 and j in fluid_set:
+                    ap -= float(wts[i, d]) * p[j]
+            b = -float(c[i])
+            rr += (b - ap) ** 2
+            bb += b * b
+        return (rr / bb) ** 0.5
+
+    r_free = residual(fluid[np.abs(p[fluid]) != 8191.0])
+    assert abs(r_free - st.final_residual) <= 1e-9 * r_free
+    assert r_free <= 1e-4 < residual(fluid)
+    o.close()
