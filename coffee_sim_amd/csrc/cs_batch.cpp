@@ -11,6 +11,7 @@
 // change), so they are captured as hipGraphs once and replayed every substep.
 #include <hip/hip_runtime_api.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -435,6 +436,9 @@ int cs_batch_create(cs_ctx* ctx, cs_sim* const* sims, uint32_t count, const cs_b
   for (uint32_t s = 0; s < count; ++s) b->p2g_tiled[s] = sims[s]->p2g_tiled ? 1u : 0u;
   cs_batch_default_config(&b->cfg);
   if (config) b->cfg = *config;
+  if (const char* v = getenv("CS_GRAPHS")) {  // as for a single scene: 0 submits every launch eagerly
+    if (atoi(v) == 0) b->cfg.use_graphs = 0u;
+  }
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
     cus = 256;
