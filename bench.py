@@ -309,12 +309,120 @@ def bench_unet(args) -> int:
     return 0
 
 
+def bench_learned(args) -> int:
+    """The learned warm start at the working size (128^3): the network is
+    trained on the device from pairs the plain solver dumps during seeded
+    pours (every second substep, solved to 1e-6 or the cap so the targets
+    carry little solver error), then a held-out pour's post-pour substeps are
+    solved from the learned, the previous-step and the zero start from the
+    same states."""
+    import coffee_sim_amd as cs
+    from coffee_sim_amd.abi import CS_WARM_LEARNED, CS_WARM_PREVIOUS, CS_WARM_ZERO
+
+    ctx = cs.Context(0)
+    t0 = time.perf_counter()
+    net = cs.UNet(ctx, 128, seed=args.seed)
+    scaled = False
+    for k in range(args.train_pours):
+        wk = cs.SeededWorkload("v60_working", seed=1000 + k)
+        st = wk.settings.copy()
+        st.rbgs_tolerance = 1e-6
+        sim = cs.MpmSim3D(ctx, st, wk.scene)
+        pour = (wk.settings.emit_total + wk.settings.emit_per_substep - 1) // wk.settings.emit_per_substep
+        for j in range(pour):
+            sim.step_frame(wk.frame_dt)
+            if j % 2:
+                continue
+            net.capture(sim)
+            if not scaled:  # input and output scales from the first pair
+                f4, tg, m = net.read_sample(0)
+                sel = m.astype(bool)
+                net.set_scales(1.0 / max(float(np.sqrt(np.mean(f4[sel, 0] ** 2))), 1e-6),
+                               max(float(np.sqrt(np.mean(tg[sel] ** 2))), 1e-6))
+                net.clear_dataset()
+                net.capture(sim)
+                scaled = True
+        sim.close()
+        log(f"training pour {k + 1}/{args.train_pours}: {net.dataset_size()} pairs, {time.perf_counter() - t0:.0f}s")
+    losses = []
+    for e in range(args.epochs):
+        losses.append(net.train_epoch(2e-3 * 0.1 ** (e / max(args.epochs - 1, 1)), seed=e))
+    log(f"trained {args.epochs} epochs: loss {losses[0]:.4f} -> {losses[-1]:.4f}, {time.perf_counter() - t0:.0f}s")
+    train_s = time.perf_counter() - t0
+    net.clear_dataset()
+
+    w = cs.SeededWorkload("v60_working", seed=args.seed)
+    sims = {}
+    for name, mode in (("previous", CS_WARM_PREVIOUS), ("learned", CS_WARM_LEARNED), ("zero", CS_WARM_ZERO)):
+        st = w.settings.copy()
+        st.warm_start = mode
+        st.warm_fallback_residual = 20.0
+        sims[name] = cs.MpmSim3D(ctx, st, w.scene)
+    sims["learned"].set_warm_net(net)
+    base = sims["previous"]
+    pour = (w.settings.emit_total + w.settings.emit_per_substep - 1) // w.settings.emit_per_substep
+    for _ in range(pour + args.warmup):
+        base.step_frame(w.frame_dt)
+    blob = base.save()
+    sims["learned"].load(blob)
+    sims["zero"].load(blob)
+    its = {k: [] for k in sims}
+    secs = {k: 0.0 for k in sims}
+    fell = 0
+    for k in range(args.steps):
+        for name, sm in sims.items():
+            sm.synchronize()
+            t1 = time.perf_counter()
+            sm.step_frame(w.frame_dt)
+            sm.synchronize()
+            secs[name] += time.perf_counter() - t1
+            its[name].append(int(sm.solve_stats().iterations))
+        fell += int(sims["learned"].solve_stats().fell_back)
+        log(f"substep {k + 1}/{args.steps}: iterations " + ", ".join(f"{n} {its[n][-1]}" for n in sims))
+    n_part = base.particle_count()
+    tot = {k: sum(v) for k, v in its.items()}
+    out = {
+        "metric": "particle-substeps/s at 1M particles,128³ with the learned warm start; RBGS iters to 1e-4 residual",
+        "value": round(n_part * args.steps / secs["learned"], 2),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * secs["learned"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 solve, bf16 network",
+        "data": "synthetic (seeded generators)",
+        "config": {
+            "workload": "v60_working after the pour: learned warm start (U-Net trained on device from solver "
+                        "pairs) against the previous-step and zero starts from identical states",
+            "train_pours": args.train_pours, "epochs": args.epochs, "train_seconds": round(train_s, 1),
+            "iterations": its, "fell_back": fell,
+            "iterations_ratio_zero_over_learned": round(tot["zero"] / max(tot["learned"], 1), 3),
+            "iterations_ratio_previous_over_learned": round(tot["previous"] / max(tot["learned"], 1), 3),
+            "ms_per_substep": {k: round(1000.0 * v / args.steps, 2) for k, v in secs.items()},
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    for sm in sims.values():
+        sm.close()
+    net.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed substeps (20; batch64: 5)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed substeps before timing (3; batch64: 1)")
-    ap.add_argument("--workload", choices=("v60_working", "batch64", "unet128"), default="v60_working")
+    ap.add_argument("--workload", choices=("v60_working", "batch64", "unet128", "learned128"),
+                    default="v60_working")
+    ap.add_argument("--train-pours", type=int, default=3, help="learned128: pours whose solves train the network")
+    ap.add_argument("--epochs", type=int, default=30, help="learned128: training epochs")
     ap.add_argument("--tune-cache", default=None, help="batch64: JSON cache of the autotuner's choices")
     ap.add_argument("--no-tune", action="store_true", help="batch64: default launch shapes, tiled P2G")
     ap.add_argument("--seed", type=int, default=1)
@@ -338,6 +446,10 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "unet128":
         return bench_unet(args)
+    if args.workload == "learned128":
+        args.steps = 8 if args.steps is None else args.steps
+        args.warmup = 0 if args.warmup is None else args.warmup
+        return bench_learned(args)
     args.steps = 20 if args.steps is None else args.steps
     args.warmup = 3 if args.warmup is None else args.warmup
 
