@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
   static_assert(H % 2 == 0, "the listed sweeps assume an even region corner");
   __shared__ float4 s_wc[Reg::R3];
   __shared__ float s_rhs[Reg::R3];
-  __shared__ float s_p[Reg::R3];
+  __shared__ float s_pp[2][Reg::R3];  // pressure of alternate bricks: a store never races the next staging
   __shared__ double s_red[2 * NW];
   __shared__ uint32_t s_list[NT];
   __shared__ uint32_t s_stop;
@@ -569,6 +569,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
   uint32_t fires = 0;
   double acc_rr = 0.0, acc_bb = 0.0;  // thread 0: this workgroup's residual sums
   for (uint32_t k = 0;; ++k) {
+    float* s_p = s_pp[k & 1u];
     reg.stage(u, s_wc, s_rhs, s_p);
     __syncthreads();
     // prefetch the next brick; its loads stay in flight through the sweeps
@@ -594,7 +595,8 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
     fires += sweep_listed<H, NT>(u, codes, s_wc, s_rhs, s_p);
     store_brick<H, NT>(u, x0, y0, z0, s_p, pout);
     if (!more) break;
-    __syncthreads();  // the store has read the LDS region before it is restaged
+    // no barrier: the next staging writes the other pressure buffer, and the
+    // weights and right-hand side were last read before the final sweep barrier
     x0 = nx0;
     y0 = ny0;
     z0 = nz0;
@@ -891,7 +893,7 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
   static_assert(H % 2 == 0, "the listed sweeps assume an even region corner");
   __shared__ float4 s_wc[Reg::R3];
   __shared__ float s_rhs[Reg::R3];
-  __shared__ float s_p[Reg::R3];
+  __shared__ float s_pp[2][Reg::R3];  // pressure of alternate bricks: a store never races the next staging
   __shared__ uint32_t s_list[NT];
   const uint32_t total = bs->total;
   const uint32_t runs = gridDim.x >= 8u ? 8u : 1u;
@@ -922,6 +924,7 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
     codes[k] = sweep_table[((color0 ^ (uint32_t)(k & 1)) * H + (uint32_t)k) * NT + threadIdx.x];
   s_list[threadIdx.x] = my_entry;  // read after the first staging barrier
   for (uint32_t k = 0;; ++k) {
+    float* s_p = s_pp[k & 1u];
     reg.stage(cb.u, s_wc, s_rhs, s_p);
     __syncthreads();
     const bool more = k + 1u < nmine;
@@ -946,7 +949,7 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
     store_brick<H, NT>(cb.u, x0, y0, z0, s_p, cb.pout);
     if (fires) atomicAdd(&cb.metrics[CS_M_P_CLAMP], fires);
     if (!more) break;
-    __syncthreads();  // the store has read the LDS region before it is restaged
+    // no barrier: the next staging writes the other pressure buffer
     cb = nbk;
     x0 = nx0;
     y0 = ny0;
