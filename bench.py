@@ -28,8 +28,9 @@ batch.
 The JSON line also carries
   roofline      for the kernel with the largest share of device time, from
                 HIP events recorded on the solver's stream over the timed region
-  cpu_baseline  the CPU oracle (single thread) timed on a bounded sample of the
-                same workload state on this machine's host
+  cpu_baseline  the CPU oracle (its OpenMP build on 16 host threads, the GPU's
+                share of the host; --cpu-threads 1 times the serial build)
+                on a bounded sample of the same workload state
 """
 from __future__ import annotations
 
@@ -98,9 +99,11 @@ def roofline(kt) -> dict:
     }
 
 
-def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, gpu_iters: float, budget_s: float) -> dict:
-    """The CPU oracle (single-threaded C restatement of the reference solver) on
-    the same post-pour state.
+def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, gpu_iters: float, budget_s: float,
+                 threads: int) -> dict:
+    """The CPU oracle (the C restatement of the reference solver; with
+    threads > 1 its OpenMP build, whose cell and particle loops run in
+    parallel) on the same post-pour state.
 
     One substep solved to 1e-4 takes thousands of RBGS iterations, minutes of
     CPU time at this size, so the sample is bounded: the oracle's own
@@ -132,7 +135,7 @@ def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, gpu_iters: float,
         s.emit_per_substep = 0
         s.emit_total = 0
         s.rbgs_max_iters = cap
-        ref = orc.OracleSim(s, sc)
+        ref = orc.OracleSim(s, sc, parallel=threads > 1, threads=threads)
         t0 = time.perf_counter()
         ref.step_frame(w.frame_dt)
         el = time.perf_counter() - t0
@@ -157,12 +160,13 @@ def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, gpu_iters: float,
         how = (f"tolerance-mode substeps capped at {it_lo} and {it_hi} iterations ({t_lo:.2f} s, {t_hi:.2f} s): "
                f"{fixed:.3f} s fixed + {1e3 * per_iter:.3f} ms per iteration, evaluated at "
                f"{gpu_iters:.1f} iterations")
+    who = "single-threaded C oracle" if threads == 1 else f"C oracle, OpenMP build on {threads} threads"
     return {
         "value": round(n / per_sub, 2),
         "unit": "particle-substeps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"v60_working after the pour ({n} particles), single-threaded C oracle: {how}",
+        "sample": f"v60_working after the pour ({n} particles), {who}: {how}",
     }
 
 
@@ -428,6 +432,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--scale", type=int, default=1, help="divide the grid side (tests only)")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle sampling (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the CPU baseline (16: this GPU's share of the host; 1: the serial oracle)")
     ap.add_argument("--warm-start", choices=("previous", "zero"), default="previous",
                     help="initial pressure of each substep's solve: the previous substep's solution or zero")
     ap.add_argument("--state", default=None,
@@ -521,7 +527,7 @@ def main() -> int:
     if args.cpu_budget > 0:
         parts = sim.read_particles(0, n_particles)
         aff = sim.read_affine(0, n_particles)
-        cpu = cpu_baseline(w, parts, aff, float(np.mean(iters_timed)), args.cpu_budget)
+        cpu = cpu_baseline(w, parts, aff, float(np.mean(iters_timed)), args.cpu_budget, max(args.cpu_threads, 1))
 
     value = psteps / elapsed
     out = {

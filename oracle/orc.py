@@ -7,6 +7,7 @@ the same methods as coffee_sim_amd.MpmSim3D so a parity test can drive both.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -16,22 +17,26 @@ from coffee_sim_amd.abi import CS_METRICS_SLOTS, PRESETS, Scene, Settings, Solve
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liborc.so"
-_lib = None
+OMP_LIB_PATH = HERE / "liborc_omp.so"  # the same code with parallel loops: CPU timing only
+_libs: dict = {}
 
 
 def build() -> Path:
-    subprocess.run(["make", "-s", "-C", str(HERE), "-B", "liborc.so"], check=True)
-    global _lib
-    _lib = None
+    subprocess.run(["make", "-s", "-C", str(HERE), "-B", "all"], check=True)
+    _libs.clear()
     return LIB_PATH
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
+def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
+    """The checker (serial) or, with `parallel`, the OpenMP timing build run
+    on `threads` threads (OMP_NUM_THREADS, read when it first loads)."""
+    if parallel not in _libs:
+        path = OMP_LIB_PATH if parallel else LIB_PATH
+        if not path.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
+        if parallel and threads:
+            os.environ["OMP_NUM_THREADS"] = str(int(threads))
+        L = C.CDLL(str(path))
         P = C.POINTER
         f32, i32p, f32p = C.c_float, P(C.c_int32), P(C.c_float)
         sigs = {
@@ -77,8 +82,8 @@ def lib() -> C.CDLL:
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+        _libs[parallel] = L
+    return _libs[parallel]
 
 
 def _f(a):
@@ -115,9 +120,11 @@ def generate_sdf_class(s: Settings, sdf: np.ndarray) -> np.ndarray:
 class OracleSim:
     """CPU restatement of MpmSim3D with the same surface as coffee_sim_amd.MpmSim3D."""
 
-    def __init__(self, settings: Settings, scene: Scene | None = None):
+    def __init__(self, settings: Settings, scene: Scene | None = None, parallel: bool = False,
+                 threads: int | None = None):
         self.settings = settings.copy()
-        h = lib().orc_create(C.byref(self.settings), C.byref(scene) if scene is not None else None)
+        self._L = lib(parallel, threads)
+        h = self._L.orc_create(C.byref(self.settings), C.byref(scene) if scene is not None else None)
         if not h:
             raise RuntimeError("orc_create failed")
         self._h = C.c_void_p(h)
@@ -126,7 +133,7 @@ class OracleSim:
 
     def close(self):
         if self._h:
-            lib().orc_destroy(self._h)
+            self._L.orc_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):
@@ -136,39 +143,39 @@ class OracleSim:
             pass
 
     def reset(self):
-        lib().orc_reset(self._h)
+        self._L.orc_reset(self._h)
 
     def step_frame(self, dt: float):
-        lib().orc_step_frame(self._h, dt)
+        self._L.orc_step_frame(self._h, dt)
 
     def debug_step_until(self, sub_dt: float, stop_after: str):
-        lib().orc_debug_step_until(self._h, sub_dt, PASS[stop_after])
+        self._L.orc_debug_step_until(self._h, sub_dt, PASS[stop_after])
 
     def set_kettle_angle(self, a):
-        lib().orc_set_kettle_angle(self._h, a)
+        self._L.orc_set_kettle_angle(self._h, a)
 
     def set_spout_position(self, x, y, z):
-        lib().orc_set_spout_position(self._h, x, y, z)
+        self._L.orc_set_spout_position(self._h, x, y, z)
 
     def set_spout_target(self, x, y, z):
-        lib().orc_set_spout_target(self._h, x, y, z)
+        self._L.orc_set_spout_target(self._h, x, y, z)
 
     def set_warm_start(self, p):
         p = np.ascontiguousarray(p, np.float32).reshape(-1)
-        lib().orc_set_warm_start(self._h, _f(p))
+        self._L.orc_set_warm_start(self._h, _f(p))
 
     def write_particles(self, offset, particles, affine=None):
         p = np.ascontiguousarray(particles, np.float32).reshape(-1, 8)
         a = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(-1, 12)
-        rc = lib().orc_write_particles(self._h, offset, p.shape[0], _f(p), None if a is None else _f(a))
+        rc = self._L.orc_write_particles(self._h, offset, p.shape[0], _f(p), None if a is None else _f(a))
         assert rc == 0
 
     def set_particle_count(self, n):
-        assert lib().orc_set_particle_count(self._h, n) == 0
+        assert self._L.orc_set_particle_count(self._h, n) == 0
 
     def counts(self):
         w, b, m = C.c_uint32(), C.c_uint32(), C.c_uint32()
-        lib().orc_get_counts(self._h, C.byref(w), C.byref(b), C.byref(m))
+        self._L.orc_get_counts(self._h, C.byref(w), C.byref(b), C.byref(m))
         return w.value, b.value, m.value
 
     def particle_count(self):
@@ -177,60 +184,60 @@ class OracleSim:
 
     def times(self):
         t, m, d = C.c_float(), C.c_float(), C.c_uint32()
-        lib().orc_get_time(self._h, C.byref(t), C.byref(m), C.byref(d))
+        self._L.orc_get_time(self._h, C.byref(t), C.byref(m), C.byref(d))
         return t.value, m.value, d.value
 
     def inflow(self):
         a, f, e = C.c_float(), C.c_float(), C.c_float()
-        lib().orc_get_inflow(self._h, C.byref(a), C.byref(f), C.byref(e))
+        self._L.orc_get_inflow(self._h, C.byref(a), C.byref(f), C.byref(e))
         return a.value, f.value, e.value
 
     def read_particles(self, offset=0, count=None):
         if count is None:
             count = self.particle_count() - offset
         out = np.empty((count, 8), np.float32)
-        lib().orc_read_particles(self._h, offset, count, _f(out))
+        self._L.orc_read_particles(self._h, offset, count, _f(out))
         return out
 
     def read_affine(self, offset=0, count=None):
         if count is None:
             count = self.particle_count() - offset
         out = np.empty((count, 12), np.float32)
-        lib().orc_read_affine(self._h, offset, count, _f(out))
+        self._L.orc_read_affine(self._h, offset, count, _f(out))
         return out
 
     def read_p2g_lanes(self):
         out = np.empty((6, self.nc), np.int32)
-        lib().orc_read_p2g_lanes(self._h, _i(out))
+        self._L.orc_read_p2g_lanes(self._h, _i(out))
         return out
 
     def read_grid_vel(self):
         out = np.empty((self.nc, 4), np.float32)
-        lib().orc_read_grid_vel(self._h, _f(out))
+        self._L.orc_read_grid_vel(self._h, _f(out))
         return out
 
     def read_pressure(self):
         p = np.empty(self.nc, np.int32)
         k = np.empty(self.nc, np.int32)
-        lib().orc_read_pressure(self._h, _i(p), _i(k))
+        self._L.orc_read_pressure(self._h, _i(p), _i(k))
         return p, k
 
     def read_metrics(self):
         out = np.empty(CS_METRICS_SLOTS, np.uint32)
-        lib().orc_read_metrics(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        self._L.orc_read_metrics(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
         return out
 
     def solve_stats(self) -> SolveStats:
         st = SolveStats()
-        lib().orc_get_solve_stats(self._h, C.byref(st))
+        self._L.orc_get_solve_stats(self._h, C.byref(st))
         return st
 
     def residual(self) -> float:
-        return float(lib().orc_residual(self._h))
+        return float(self._L.orc_residual(self._h))
 
     def read_system(self):
         w = np.empty((self.nc, 6), np.float32)
         cnt = np.empty(self.nc, np.float32)
         c = np.empty(self.nc, np.float32)
-        lib().orc_read_system(self._h, _f(w), _f(cnt), _f(c))
+        self._L.orc_read_system(self._h, _f(w), _f(cnt), _f(c))
         return w, cnt, c

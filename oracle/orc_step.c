@@ -22,6 +22,29 @@ static inline void decompose(const orc_params* u, size_t idx, uint32_t* ix, uint
   *ix = (uint32_t)(rem % u->gx);
 }
 
+/* Built with -fopenmp (liborc_omp.so, the CPU timing baseline) the cell and
+ * particle loops run in parallel: every pass writes only its own cell or
+ * particle except the P2G scatter and the metric counters, which then use
+ * atomics (integer addition, so the lanes and counts are the same).  Without
+ * OpenMP (liborc.so, the checker) the same code runs serially. */
+#ifdef _OPENMP
+#define ORC_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#define ORC_COUNT(p) __atomic_fetch_add((p), 1u, __ATOMIC_RELAXED)
+#define ORC_LANE_ADD(p, v) __atomic_fetch_add((uint32_t*)(p), (uint32_t)(v), __ATOMIC_RELAXED)
+static inline void orc_max_u32(uint32_t* p, uint32_t v) {
+  uint32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+  while (v > cur && !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+  }
+}
+#else
+#define ORC_PARALLEL_FOR
+#define ORC_COUNT(p) (*(p) += 1u)
+#define ORC_LANE_ADD(p, v) (*(p) = o_wrap_add(*(p), (v)))
+static inline void orc_max_u32(uint32_t* p, uint32_t v) {
+  if (v > *p) *p = v;
+}
+#endif
+
 /* lane helpers (shader.rs:152-164) */
 #define LANE(S, l, c) ((S)->grid[(size_t)(l) * (S)->nc + (c)])
 static inline int32_t kind_load(orc_sim* S, size_t c) { return LANE(S, 3, c); }
@@ -32,7 +55,7 @@ static inline float pressure_load(orc_sim* S, const orc_params* u, size_t c) {
 static inline void pressure_store(orc_sim* S, const orc_params* u, size_t c, float v) {
   float lim = u->pressure_clamp;
   float cl = o_clamp(v, -lim, lim);
-  if (cl != v) S->metrics[O_M_P_CLAMP] += 1u;
+  if (cl != v) ORC_COUNT(&S->metrics[O_M_P_CLAMP]);
   LANE(S, 0, c) = o_f2i(cl * u->fp_scale);
 }
 static inline float packing_load(orc_sim* S, const orc_params* u, size_t c) {
@@ -41,7 +64,7 @@ static inline float packing_load(orc_sim* S, const orc_params* u, size_t c) {
 /* shader.rs:215-222 */
 static inline void packing_store(orc_sim* S, const orc_params* u, size_t c, float v) {
   float cl = o_clamp(v, 0.0f, u->pressure_clamp);
-  if (cl != v) S->metrics[O_M_P_CLAMP] += 1u;
+  if (cl != v) ORC_COUNT(&S->metrics[O_M_P_CLAMP]);
   LANE(S, 2, c) = o_f2i(cl * u->fp_scale);
 }
 static inline float div_load(orc_sim* S, const orc_params* u, size_t c) {
@@ -51,7 +74,7 @@ static inline float div_load(orc_sim* S, const orc_params* u, size_t c) {
 static inline void div_store(orc_sim* S, const orc_params* u, size_t c, float v) {
   float lim = u->div_clamp;
   float cl = o_clamp(v, -lim, lim);
-  if (cl != v) S->metrics[O_M_DIV_CLAMP] += 1u;
+  if (cl != v) ORC_COUNT(&S->metrics[O_M_DIV_CLAMP]);
   LANE(S, 1, c) = o_f2i(cl * u->fp_scale);
 }
 static inline float rest_vol(orc_sim* S, const orc_params* u, size_t c) {
@@ -299,6 +322,7 @@ static float bed_pore_target_div(orc_sim* S, const orc_params* u, int32_t id, v3
 static void k_p2g(orc_sim* S, const orc_params* u) {
   uint32_t np = u->num_water + u->num_bed_particles;
   float thr = u->nominal_mass * 0.10f;
+  ORC_PARALLEL_FOR
   for (uint32_t pid = 0; pid < np; ++pid) {
     const float* p = &S->particles[(size_t)pid * 8];
     const float* a = &S->affine[(size_t)pid * 12];
@@ -340,19 +364,20 @@ static void k_p2g(orc_sim* S, const orc_params* u) {
           float mxf = mom.x * fp, myf = mom.y * fp, mzf = mom.z * fp;
           if (fabsf(mfp) > lim || fabsf(mxf) > lim || fabsf(myf) > lim || fabsf(mzf) > lim ||
               fabsf(rfp) > lim || fabsf(cfp) > lim)
-            S->metrics[O_M_OVERFLOW] += 1u;
-          LANE(S, 0, ci) = o_wrap_add(LANE(S, 0, ci), o_f2i(mfp));
-          LANE(S, 1, ci) = o_wrap_add(LANE(S, 1, ci), o_f2i(mxf));
-          LANE(S, 2, ci) = o_wrap_add(LANE(S, 2, ci), o_f2i(myf));
-          LANE(S, 3, ci) = o_wrap_add(LANE(S, 3, ci), o_f2i(mzf));
-          LANE(S, 4, ci) = o_wrap_add(LANE(S, 4, ci), o_f2i(rfp));
-          LANE(S, 5, ci) = o_wrap_add(LANE(S, 5, ci), o_f2i(cfp));
+            ORC_COUNT(&S->metrics[O_M_OVERFLOW]);
+          ORC_LANE_ADD(&LANE(S, 0, ci), o_f2i(mfp));
+          ORC_LANE_ADD(&LANE(S, 1, ci), o_f2i(mxf));
+          ORC_LANE_ADD(&LANE(S, 2, ci), o_f2i(myf));
+          ORC_LANE_ADD(&LANE(S, 3, ci), o_f2i(mzf));
+          ORC_LANE_ADD(&LANE(S, 4, ci), o_f2i(rfp));
+          ORC_LANE_ADD(&LANE(S, 5, ci), o_f2i(cfp));
         }
   }
 }
 
 /* ---- grid_update (shader.rs:1056-1082) ------------------------------------ */
 static void k_grid_update(orc_sim* S, const orc_params* u) {
+  ORC_PARALLEL_FOR
   for (size_t idx = 0; idx < S->nc; ++idx) {
     float inv_fp = u->inv_fp_scale;
     float mass = (float)LANE(S, 0, idx) * inv_fp;
@@ -369,6 +394,7 @@ static void k_grid_update(orc_sim* S, const orc_params* u) {
 
 /* ---- boundary_project (shader.rs:1630-1679) -------------------------------- */
 static void k_boundary(orc_sim* S, const orc_params* u) {
+  ORC_PARALLEL_FOR
   for (size_t idx = 0; idx < S->nc; ++idx) {
     float* g = &S->grid_vel[idx * 4];
     if (g[3] < 1e-6f) continue;
@@ -409,6 +435,7 @@ static const int32_t OFF6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0},
 
 static void k_classify(orc_sim* S, const orc_params* u) {
   float occ = u->nominal_mass * 0.1f;
+  ORC_PARALLEL_FOR
   for (size_t idx = 0; idx < S->nc; ++idx) {
     float mass = S->grid_vel[idx * 4 + 3];
     pressure_store(S, u, idx, 0.0f);
@@ -476,8 +503,8 @@ static void k_classify(orc_sim* S, const orc_params* u) {
     div_store(S, u, idx, div - target);
     float abs_div = fabsf(div);
     uint32_t fpd = o_f2u(o_clamp(abs_div * u->metrics_div_fp_scale, 0.0f, 2147483648.0f));
-    if (fpd > S->metrics[O_M_MAX_ABS_DIV]) S->metrics[O_M_MAX_ABS_DIV] = fpd;
-    S->metrics[O_M_FLUID_CELLS] += 1u;
+    orc_max_u32(&S->metrics[O_M_MAX_ABS_DIV], fpd);
+    ORC_COUNT(&S->metrics[O_M_FLUID_CELLS]);
   }
 }
 
@@ -547,6 +574,7 @@ static void build_rows(orc_sim* S, const orc_params* u) {
 static void k_rbgs_half(orc_sim* S, const orc_params* u, uint32_t parity) {
   sys_row* rows = g_rows[parity];
   size_t n = g_nrows[parity];
+  ORC_PARALLEL_FOR
   for (size_t i = 0; i < n; ++i) {
     const sys_row* r = &rows[i];
     float sum = 0.0f;
@@ -565,22 +593,37 @@ static void k_rbgs_half(orc_sim* S, const orc_params* u, uint32_t parity) {
  * pressure_update enforces through pressure_store, shader.rs:202-209) is left out: there the bound
  * replaces the row, so the clamped iteration's fixed point leaves that row
  * unsatisfied and it would hold the residual above any tolerance. */
+/* One row's contribution to the residual sums (nothing for a row the clamp holds). */
+static inline void row_residual(orc_sim* S, const orc_params* u, const sys_row* r, double* rr, double* bb) {
+  if (r->count <= 0.0f) return;
+  if (fabsf(pressure_load(S, u, r->idx)) == u->pressure_clamp) return;
+  double ap = (double)r->count * (double)pressure_load(S, u, r->idx);
+  for (int k = 0; k < 6; ++k)
+    if (r->w[k] > 0.0f && r->nf[k]) ap -= (double)r->w[k] * (double)pressure_load(S, u, r->nb[k]);
+  double b = -(double)r->c;
+  *rr += (b - ap) * (b - ap);
+  *bb += b * b;
+}
+
 static double residual_rows(orc_sim* S, const orc_params* u) {
   double rr = 0.0, bb = 0.0;
+#ifdef _OPENMP
+  /* the timing build sums each colour's rows in parallel (another order) */
+  for (int col = 0; col < 2; ++col) {
+    const sys_row* rows = g_rows[col];
+    const size_t n = g_nrows[col];
+#pragma omp parallel for reduction(+ : rr, bb) schedule(static)
+    for (size_t i = 0; i < n; ++i) row_residual(S, u, &rows[i], &rr, &bb);
+  }
+#else
   size_t i0 = 0, i1 = 0;
   while (i0 < g_nrows[0] || i1 < g_nrows[1]) {
     const sys_row* r;
     if (i1 >= g_nrows[1] || (i0 < g_nrows[0] && g_rows[0][i0].idx < g_rows[1][i1].idx)) r = &g_rows[0][i0++];
     else r = &g_rows[1][i1++];
-    if (r->count <= 0.0f) continue;
-    if (fabsf(pressure_load(S, u, r->idx)) == u->pressure_clamp) continue;
-    double ap = (double)r->count * (double)pressure_load(S, u, r->idx);
-    for (int k = 0; k < 6; ++k)
-      if (r->w[k] > 0.0f && r->nf[k]) ap -= (double)r->w[k] * (double)pressure_load(S, u, r->nb[k]);
-    double b = -(double)r->c;
-    rr += (b - ap) * (b - ap);
-    bb += b * b;
+    row_residual(S, u, r, &rr, &bb);
   }
+#endif
   if (bb > 0.0) return sqrt(rr / bb);
   return rr > 0.0 ? INFINITY : 0.0;
 }
@@ -681,6 +724,7 @@ static float pressure_weighted_or_mirror(orc_sim* S, const orc_params* u, i3 c, 
   return o_mix(mirror, np, fw);
 }
 static void k_project(orc_sim* S, const orc_params* u) {
+  ORC_PARALLEL_FOR
   for (size_t idx = 0; idx < S->nc; ++idx) {
     float* g = &S->grid_vel[idx * 4];
     if (g[3] < 1e-6f) continue;
@@ -728,6 +772,7 @@ static float packing_or_mirror(orc_sim* S, const orc_params* u, i3 c, float mirr
   return packing_load(S, u, ci);
 }
 static void k_packing_prepare(orc_sim* S, const orc_params* u) {
+  ORC_PARALLEL_FOR
   for (size_t idx = 0; idx < S->nc; ++idx) {
     int32_t kind = kind_load(S, idx);
     if (!is_fluid(kind) || kind == O_CELL_BED) { packing_store(S, u, idx, 0.0f); continue; }
@@ -740,6 +785,7 @@ static void k_packing_prepare(orc_sim* S, const orc_params* u) {
 }
 static void k_packing_apply(orc_sim* S, const orc_params* u) {
   float occ = u->nominal_mass * 0.1f;
+  ORC_PARALLEL_FOR
   for (size_t idx = 0; idx < S->nc; ++idx) {
     float* g = &S->grid_vel[idx * 4];
     if (g[3] <= occ) continue;
@@ -767,6 +813,7 @@ static void k_packing_apply(orc_sim* S, const orc_params* u) {
 /* ---- viscosity (shader.rs:1086-1177) ----------------------------------------- */
 static void k_viscosity_prepare(orc_sim* S, const orc_params* u) {
   float occ = u->nominal_mass * 0.1f, support = u->nominal_mass * 2.0f;
+  ORC_PARALLEL_FOR
   for (size_t idx = 0; idx < S->nc; ++idx) {
     const float* gv = &S->grid_vel[idx * 4];
     v3 vh = V(gv[0], gv[1], gv[2]);
@@ -797,6 +844,7 @@ static void k_viscosity_prepare(orc_sim* S, const orc_params* u) {
 }
 static void k_viscosity_apply(orc_sim* S, const orc_params* u) {
   float occ = u->nominal_mass * 0.1f;
+  ORC_PARALLEL_FOR
   for (size_t idx = 0; idx < S->nc; ++idx) {
     float* g = &S->grid_vel[idx * 4];
     float mass = g[3];
@@ -810,6 +858,7 @@ static void k_viscosity_apply(orc_sim* S, const orc_params* u) {
 static void k_g2p(orc_sim* S, const orc_params* u) {
   uint32_t np = u->num_water + u->num_bed_particles;
   float thr = u->nominal_mass * 0.10f;
+  ORC_PARALLEL_FOR
   for (uint32_t pid = 0; pid < np; ++pid) {
     float* p = &S->particles[(size_t)pid * 8];
     float* a = &S->affine[(size_t)pid * 12];
