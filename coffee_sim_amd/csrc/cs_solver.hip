@@ -212,6 +212,41 @@ struct Region {
       }
     }
   }
+
+  // Whether local cell i lies within L1 distance H of the brick: the cells
+  // the listed sweeps, the residual and the store ever read (2656 of 4096
+  // for H = 4).  The others are neither fetched nor staged.
+  __device__ __forceinline__ static bool in_ball(int i) {
+    const int lx = i % R, ly = (i / R) % R, lz = i / (R * R);
+    auto d = [](int l) { return max(0, max(H - l, l - (H + BR - 1))); };
+    return d(lx) + d(ly) + d(lz) <= H;
+  }
+  // fetch/stage restricted to the ball.
+  __device__ __forceinline__ void fetch_ball(const cs_params& u, int x0, int y0, int z0,
+                                             const float4* __restrict__ wc, const float* __restrict__ rhs,
+                                             const int32_t* __restrict__ pin) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = (int)threadIdx.x + k * NT;
+      const uint32_t idx = region_cell<R>(u, min(i, R3 - 1), x0, y0, z0, on[k]);
+      if (i < R3 && in_ball(i)) {
+        wv[k] = wc[idx];
+        rv[k] = rhs[idx];
+        pv[k] = pin[idx];
+      }
+    }
+  }
+  __device__ __forceinline__ void stage_ball(const cs_params& u, float4* s_wc, float* s_rhs, float* s_p) const {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (i < R3 && in_ball(i)) {
+        s_wc[i] = on[k] ? wv[k] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        s_rhs[i] = on[k] ? rv[k] : 0.0f;
+        s_p[i] = on[k] ? p_stage(u, pv[k]) : 0.0f;
+      }
+    }
+  }
 };
 
 // The cells a thread updates in each half sweep, fixed for the whole launch.
@@ -510,7 +545,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
     x0 -= H;
     y0 -= H;
     z0 -= H;
-    reg.fetch(u, x0, y0, z0, wc, rhs, pin);
+    reg.fetch_ball(u, x0, y0, z0, wc, rhs, pin);
   }
   // this thread's cell of each half sweep: colour color0 ^ (s & 1); the region
   // corner (brick origin - H, H even) has even parity, so local parity is colour
@@ -570,7 +605,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
   double acc_rr = 0.0, acc_bb = 0.0;  // thread 0: this workgroup's residual sums
   for (uint32_t k = 0;; ++k) {
     float* s_p = s_pp[k & 1u];
-    reg.stage(u, s_wc, s_rhs, s_p);
+    reg.stage_ball(u, s_wc, s_rhs, s_p);
     __syncthreads();
     // prefetch the next brick; its loads stay in flight through the sweeps
     const bool more = k + 1u < nmine;
@@ -581,7 +616,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
       nx0 -= H;
       ny0 -= H;
       nz0 -= H;
-      reg.fetch(u, nx0, ny0, nz0, wc, rhs, pin);
+      reg.fetch_ball(u, nx0, ny0, nz0, wc, rhs, pin);
     }
     if (compute) {
       double rr = 0.0, bb = 0.0;
@@ -917,7 +952,7 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
   y0 -= H;
   z0 -= H;
   Reg reg;
-  reg.fetch(cb.u, x0, y0, z0, cb.wc, cb.rhs, cb.pin);
+  reg.fetch_ball(cb.u, x0, y0, z0, cb.wc, cb.rhs, cb.pin);
   uint32_t codes[H];
 #pragma unroll
   for (int k = 0; k < H; ++k)
@@ -925,7 +960,7 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
   s_list[threadIdx.x] = my_entry;  // read after the first staging barrier
   for (uint32_t k = 0;; ++k) {
     float* s_p = s_pp[k & 1u];
-    reg.stage(cb.u, s_wc, s_rhs, s_p);
+    reg.stage_ball(cb.u, s_wc, s_rhs, s_p);
     __syncthreads();
     const bool more = k + 1u < nmine;
     BatchBrick nbk;
@@ -943,7 +978,7 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
       nx0 -= H;
       ny0 -= H;
       nz0 -= H;
-      reg.fetch(nbk.u, nx0, ny0, nz0, nbk.wc, nbk.rhs, nbk.pin);
+      reg.fetch_ball(nbk.u, nx0, ny0, nz0, nbk.wc, nbk.rhs, nbk.pin);
     }
     const uint32_t fires = sweep_listed<H, NT>(cb.u, codes, s_wc, s_rhs, s_p);
     store_brick<H, NT>(cb.u, x0, y0, z0, s_p, cb.pout);
