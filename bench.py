@@ -55,6 +55,18 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel: str) -> int | None:
+    """HBM bytes per launch of `kernel` from its rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary under profiles/ (calibrated against a copy kernel there), or None."""
+    path = ROOT / "profiles" / f"{kernel}_traffic.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
+
+
 def roofline(kt) -> dict:
     """Achieved algorithmic bandwidth of the dominant kernel.
 
@@ -92,7 +104,7 @@ def roofline(kt) -> dict:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": measured_traffic(name),
         "avg_launch_us": round(avg_us, 3),
         "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
         "device_time_share_us": {k: round(v, 1) for k, v in shares.items()},
