@@ -82,6 +82,7 @@ def _bind(lib: C.CDLL) -> None:
         "cs_sim_read_grid_vel": (C.c_int, [C.c_void_p, P(C.c_float)]),
         "cs_sim_read_pressure": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
         "cs_sim_read_metrics": (C.c_int, [C.c_void_p, P(C.c_uint32)]),
+        "cs_sim_pack_render": (C.c_int, [C.c_void_p, P(C.c_float)]),
         "cs_sim_get_solve_stats": (C.c_int, [C.c_void_p, P(SolveStats)]),
         "cs_sim_set_warm_start": (C.c_int, [C.c_void_p, P(C.c_float)]),
         "cs_sim_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, P(C.c_size_t)]),
@@ -509,6 +510,12 @@ class MpmSim3D:
         out = np.empty(abi.CS_METRICS_SLOTS, np.uint32)
         _check(load_library().cs_sim_read_metrics(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32))),
                "read_metrics")
+        return out
+
+    def pack_render(self) -> np.ndarray:
+        """prepare_render's RenderParticle per particle in use: (n, 8) float32."""
+        out = np.empty((self.particle_count(), 8), np.float32)
+        _check(load_library().cs_sim_pack_render(self._h, _fptr(out)), "cs_sim_pack_render")
         return out
 
     def solve_stats(self) -> SolveStats:

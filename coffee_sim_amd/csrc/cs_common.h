@@ -26,6 +26,11 @@
 #define CS_METRICS_DIV_FP_SCALE 1024.0f
 #define CS_SDF_NO_CONSTRAINT 999.0f
 
+// Render sample radii in cells (DEFAULT_BREW, brew_config.rs:50-51; read by
+// water_particle_radius / bed_particle_radius, shader.rs:127-128).
+#define CS_WATER_SAMPLE_RADIUS_DX 0.18f
+#define CS_BED_SAMPLE_RADIUS_DX 0.62f
+
 // Cell kinds (shader.rs:174-178).
 #define CS_CELL_AIR 0
 #define CS_CELL_SURFACE 1

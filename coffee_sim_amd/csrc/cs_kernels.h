@@ -40,6 +40,9 @@ hipError_t cs_launch_visc(hipStream_t stream, const cs_params& u, const float* g
                           const float* sdf);
 hipError_t cs_launch_g2p(hipStream_t stream, const cs_params& u, float* part, float* aff, const float* gvel,
                          const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf);
+// prepare_render (shader.rs:2242-2276): 8 floats per particle into `out`.
+hipError_t cs_launch_pack_render(hipStream_t stream, const cs_params& u, const float* part, const float* aff,
+                                 uint32_t num_bed, float* out);
 hipError_t cs_launch_expf(hipStream_t stream, const float* in, float* out, uint32_t n);
 
 // Blocks used by the residual reduction (size of the partials buffer / 2).

@@ -587,6 +587,36 @@ __global__ void __launch_bounds__(256) k_g2p(cs_params u, float4* __restrict__ p
   g2p_particle(u, part, aff, gvel, bed_id, bed, sdf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// ---- prepare_render (shader.rs:2242-2276) --------------------------------------------
+// RenderParticle (shader.rs:40-43) per particle: (pos.xyz, colour) and (radius,
+// 0, 0, 0); an inactive particle is parked below the scene.  Bed particles
+// (pid < num_bed) would colour by saturation; field-mode beds have none, so
+// their saturation term is 0.
+__global__ void __launch_bounds__(256) k_pack_render(cs_params u, const float4* __restrict__ part,
+                                                     const float4* __restrict__ aff, uint32_t num_bed,
+                                                     float4* __restrict__ out) {
+  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= u.num_particles) return;
+  const float4 P = part[2 * pid], Vv = part[2 * pid + 1];
+  const float phase = aff[3 * pid].w;
+  if (Vv.w <= u.nominal_mass * 0.10f) {
+    out[2 * pid] = make_float4(0.0f, -1e6f, 0.0f, -999.0f);
+    out[2 * pid + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  float color_t, radius = u.dx * CS_WATER_SAMPLE_RADIUS_DX;
+  if (phase < 0.5f) {
+    const float speed = len3(f3(Vv.x, Vv.y, Vv.z));
+    color_t = cs_clamp(speed / 10.0f, 0.0f, 2.0f);
+  } else {
+    float sat = 0.0f;
+    if (pid < num_bed) radius = u.dx * CS_BED_SAMPLE_RADIUS_DX;
+    color_t = -1.0f - sat;
+  }
+  out[2 * pid] = make_float4(P.x, P.y, P.z, color_t);
+  out[2 * pid + 1] = make_float4(radius, 0.0f, 0.0f, 0.0f);
+}
+
 // ---- exp check kernel (used by the verification entry point) -----------------------
 __global__ void k_expf(const float* __restrict__ in, float* __restrict__ out, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -787,6 +817,14 @@ hipError_t cs_launch_g2p(hipStream_t s, const cs_params& u, float* part, float* 
   if (u.num_particles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_g2p, blocks_for(u.num_particles), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(part),
                      reinterpret_cast<float4*>(aff), reinterpret_cast<const float4*>(gvel), bed_id, bed, sdf);
+  return hipGetLastError();
+}
+hipError_t cs_launch_pack_render(hipStream_t s, const cs_params& u, const float* part, const float* aff,
+                                 uint32_t num_bed, float* out) {
+  if (u.num_particles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_render, blocks_for(u.num_particles), dim3(kBlock), 0, s, u,
+                     reinterpret_cast<const float4*>(part), reinterpret_cast<const float4*>(aff), num_bed,
+                     reinterpret_cast<float4*>(out));
   return hipGetLastError();
 }
 hipError_t cs_launch_expf(hipStream_t s, const float* in, float* out, uint32_t n) {

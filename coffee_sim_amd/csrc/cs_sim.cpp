@@ -61,7 +61,7 @@ void free_sim(cs_sim* m) {
                  m->rhs,        m->pack,       m->prev_p,     m->warm,       m->metrics, m->res_partials,
                  m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,
                  m->brick_flag, m->brick_list, m->brick_partials, m->dstate,       m->tile_count,
-                 m->tile_start, m->p2g_key,    m->p2g_sorted};
+                 m->tile_start, m->p2g_key,    m->p2g_sorted, m->render};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -909,6 +909,16 @@ int cs_sim_read_metrics(cs_sim* m, uint32_t* slots) {
   if (!m) return fail(CS_ERR_INVALID, "null sim");
   return read_dev(m, slots, m->metrics, CS_METRICS_SLOTS * 4);
 }
+int cs_sim_pack_render(cs_sim* m, float* out) {
+  if (!m || !out) return fail(CS_ERR_INVALID, "null argument");
+  (void)hipSetDevice(m->ctx->device);
+  if (!m->render) CS_HIP(dalloc(&m->render, (size_t)m->s.max_particles * 8));
+  cs_params u;
+  cs::make_params(m->s, m->num_water, m->num_bed_cells, 1.0e-4f, &u);  // dt does not enter the packing
+  CS_HIP(cs_launch_pack_render(m->ctx->stream, u, m->part, m->aff, 0u, m->render));
+  return read_dev(m, out, m->render, (size_t)m->num_water * 32);
+}
+
 int cs_sim_get_solve_stats(cs_sim* m, cs_solve_stats* out) {
   if (!m || !out) return fail(CS_ERR_INVALID, "null argument");
   if (m->final_residual_pending) {

@@ -36,6 +36,7 @@ struct cs_sim {
   int32_t* pack = nullptr;
   int32_t* prev_p = nullptr;
   float* warm = nullptr;
+  float* render = nullptr;   // RenderParticle per particle slot, packed on demand
   bool has_warm = false;
   uint32_t* metrics = nullptr;
   double* res_partials = nullptr;

@@ -289,6 +289,12 @@ int cs_sim_read_grid_vel(cs_sim* sim, float* vel4);
 int cs_sim_read_pressure(cs_sim* sim, int32_t* pressure_fp, int32_t* kind);
 /* Metric slots (state.rs:22-28). */
 int cs_sim_read_metrics(cs_sim* sim, uint32_t* slots8);
+/* prepare_render (shader.rs:2242-2276, run by step_frame at mod.rs:410-412 and
+ * exposed through render_buffer, mod.rs:500-502): the RenderParticle layout
+ * (shader.rs:40-43), 8 floats per particle, (pos.xyz, colour) and (radius, 0,
+ * 0, 0), for every particle slot in use.  Packed on demand from the current
+ * particle state; `render8` holds particle_count x 8 floats. */
+int cs_sim_pack_render(cs_sim* sim, float* render8);
 int cs_sim_get_solve_stats(cs_sim* sim, cs_solve_stats* out);
 
 /* Warm start for the next pressure solve (CS_WARM_EXTERNAL): float pressure per cell. */

@@ -70,6 +70,8 @@ void orc_read_p2g_lanes(orc_sim* sim, int32_t* lanes6);
 void orc_read_grid_vel(orc_sim* sim, float* vel4);
 void orc_read_pressure(orc_sim* sim, int32_t* p, int32_t* kind);
 void orc_read_metrics(orc_sim* sim, uint32_t* slots8);
+/* prepare_render (shader.rs:2242-2276): RenderParticle, 8 floats per particle in use. */
+void orc_pack_render(orc_sim* sim, float* render8);
 void orc_get_solve_stats(orc_sim* sim, cs_solve_stats* out);
 /* relative residual of the current pressure (definition shared with the product) */
 double orc_residual(orc_sim* sim);

@@ -73,6 +73,7 @@ def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
             "orc_read_grid_vel": (None, [C.c_void_p, f32p]),
             "orc_read_pressure": (None, [C.c_void_p, i32p, i32p]),
             "orc_read_metrics": (None, [C.c_void_p, P(C.c_uint32)]),
+            "orc_pack_render": (None, [C.c_void_p, f32p]),
             "orc_get_solve_stats": (None, [C.c_void_p, P(SolveStats)]),
             "orc_residual": (C.c_double, [C.c_void_p]),
             "orc_read_system": (None, [C.c_void_p, f32p, f32p, f32p]),
@@ -225,6 +226,11 @@ class OracleSim:
     def read_metrics(self):
         out = np.empty(CS_METRICS_SLOTS, np.uint32)
         self._L.orc_read_metrics(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return out
+
+    def pack_render(self) -> np.ndarray:
+        out = np.empty((self.particle_count(), 8), np.float32)
+        self._L.orc_pack_render(self._h, _f(out))
         return out
 
     def solve_stats(self) -> SolveStats:

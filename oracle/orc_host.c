@@ -537,6 +537,39 @@ void orc_read_affine(orc_sim* S, uint32_t off, uint32_t n, float* a12) {
 void orc_read_p2g_lanes(orc_sim* S, int32_t* l) { memcpy(l, S->p2g_lanes, S->nc * 6 * sizeof(int32_t)); }
 void orc_read_grid_vel(orc_sim* S, float* v) { memcpy(v, S->grid_vel, S->nc * 4 * sizeof(float)); }
 void orc_read_metrics(orc_sim* S, uint32_t* m) { memcpy(m, S->metrics, sizeof(S->metrics)); }
+
+/* prepare_render (shader.rs:2242-2276) on the current particle state: an
+ * inactive particle (mass <= inactive_mass_threshold, shader.rs:116) is parked
+ * at y = -1e6 with colour -999; water colours by speed / 10 clamped to
+ * [0, 2]; other phases by -1 - saturation, where a bed particle
+ * (pid < num_bed) also takes the bed radius.  Radii are dx times the
+ * DEFAULT_BREW sample radii (brew_config.rs:50-51, shader.rs:127-128). */
+void orc_pack_render(orc_sim* S, float* out) {
+  orc_params u;
+  orc_write_uniforms(S, 1.0e-4f, &u);  /* dt does not enter the packing */
+  const uint32_t n = u.num_water + u.num_bed_particles;
+  for (uint32_t pid = 0; pid < n; ++pid) {
+    const float* p = &S->particles[(size_t)pid * 8];
+    const float phase = S->affine[(size_t)pid * 12 + 3];
+    float* o = &out[(size_t)pid * 8];
+    if (p[7] <= u.nominal_mass * 0.10f) {
+      const float park[8] = {0.0f, -1e6f, 0.0f, -999.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      memcpy(o, park, sizeof(park));
+      continue;
+    }
+    float color_t, radius = u.dx * 0.18f;
+    if (phase < 0.5f) {
+      const float speed = sqrtf(p[4] * p[4] + p[5] * p[5] + p[6] * p[6]);
+      color_t = o_clamp(speed / 10.0f, 0.0f, 2.0f);
+    } else {
+      float sat = 0.0f;  /* no bed particles, hence no extraction state, in field mode */
+      if (pid < u.num_bed_particles) radius = u.dx * 0.62f;
+      color_t = -1.0f - sat;
+    }
+    o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = color_t;
+    o[4] = radius; o[5] = 0.0f; o[6] = 0.0f; o[7] = 0.0f;
+  }
+}
 void orc_get_solve_stats(orc_sim* S, cs_solve_stats* o) { *o = S->stats; }
 
 /* ---- exp from Cody-Waite range reduction + degree-7 Taylor polynomial ------- */
