@@ -363,7 +363,7 @@ def bench_learned(args) -> int:
     losses = []
     for e in range(args.epochs):
         losses.append(net.train_epoch(2e-3 * 0.1 ** (e / max(args.epochs - 1, 1)), seed=e))
-    log(f"trained {args.epochs} epochs: loss {losses[0]:.4f} -> {losses[-1]:.4f}, {time.perf_counter() - t0:.0f}s")
+        log(f"epoch {e + 1}/{args.epochs}: loss {losses[-1]:.4f}, {time.perf_counter() - t0:.0f}s")
     train_s = time.perf_counter() - t0
     net.clear_dataset()
 
