@@ -368,23 +368,31 @@ def bench_learned(args) -> int:
     net.clear_dataset()
 
     w = cs.SeededWorkload("v60_working", seed=args.seed)
+    rescales = [int(v) for v in args.rescale_after.split(",") if v.strip() and int(v) > 0]
+    starts = [("previous", CS_WARM_PREVIOUS, 0), ("learned", CS_WARM_LEARNED, 0), ("zero", CS_WARM_ZERO, 0)]
+    for m in rescales:
+        starts += [(f"learned+rescale{m}", CS_WARM_LEARNED, m), (f"previous+rescale{m}", CS_WARM_PREVIOUS, m),
+                   (f"zero+rescale{m}", CS_WARM_ZERO, m)]
     sims = {}
-    for name, mode in (("previous", CS_WARM_PREVIOUS), ("learned", CS_WARM_LEARNED), ("zero", CS_WARM_ZERO)):
+    for name, mode, m in starts:
         st = w.settings.copy()
         st.warm_start = mode
         st.warm_fallback_residual = 20.0
+        st.rbgs_rescale_after = m
         sims[name] = cs.MpmSim3D(ctx, st, w.scene)
-    sims["learned"].set_warm_net(net)
+        if mode == CS_WARM_LEARNED:
+            sims[name].set_warm_net(net)
     base = sims["previous"]
     pour = (w.settings.emit_total + w.settings.emit_per_substep - 1) // w.settings.emit_per_substep
     for _ in range(pour + args.warmup):
         base.step_frame(w.frame_dt)
     blob = base.save()
-    sims["learned"].load(blob)
-    sims["zero"].load(blob)
+    for name, sm in sims.items():
+        if sm is not base:
+            sm.load(blob)
     its = {k: [] for k in sims}
     secs = {k: 0.0 for k in sims}
-    fell = 0
+    fell = {k: 0 for k in sims if "learned" in k}
     for k in range(args.steps):
         for name, sm in sims.items():
             sm.synchronize()
@@ -393,18 +401,21 @@ def bench_learned(args) -> int:
             sm.synchronize()
             secs[name] += time.perf_counter() - t1
             its[name].append(int(sm.solve_stats().iterations))
-        fell += int(sims["learned"].solve_stats().fell_back)
+            if name in fell:
+                fell[name] += int(sm.solve_stats().fell_back)
         log(f"substep {k + 1}/{args.steps}: iterations " + ", ".join(f"{n} {its[n][-1]}" for n in sims))
     n_part = base.particle_count()
     tot = {k: sum(v) for k, v in its.items()}
+    head = "learned"
+    ratios = {f"{other}/{head}": round(tot[other] / max(tot[head], 1), 3) for other in tot if other != head}
     out = {
         "metric": "particle-substeps/s at 1M particles,128³ with the learned warm start; RBGS iters to 1e-4 residual",
-        "value": round(n_part * args.steps / secs["learned"], 2),
+        "value": round(n_part * args.steps / secs[head], 2),
         "unit": "particle-substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * secs["learned"] / args.steps, 4),
+        "ms_per_step": round(1000.0 * secs[head] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -414,9 +425,8 @@ def bench_learned(args) -> int:
             "workload": "v60_working after the pour: learned warm start (U-Net trained on device from solver "
                         "pairs) against the previous-step and zero starts from identical states",
             "train_pours": args.train_pours, "epochs": args.epochs, "train_seconds": round(train_s, 1),
-            "iterations": its, "fell_back": fell,
-            "iterations_ratio_zero_over_learned": round(tot["zero"] / max(tot["learned"], 1), 3),
-            "iterations_ratio_previous_over_learned": round(tot["previous"] / max(tot["learned"], 1), 3),
+            "timed_start": head, "iterations": its, "iterations_total": tot, "fell_back": fell,
+            "iterations_ratio_over_timed_start": ratios,
             "ms_per_substep": {k: round(1000.0 * v / args.steps, 2) for k, v in secs.items()},
         },
         "roofline": None,
@@ -439,6 +449,9 @@ def main() -> int:
                     default="v60_working")
     ap.add_argument("--train-pours", type=int, default=3, help="learned128: pours whose solves train the network")
     ap.add_argument("--epochs", type=int, default=30, help="learned128: training epochs")
+    ap.add_argument("--rescale-after", default="96",
+                    help="learned128: comma-separated rbgs_rescale_after values whose starts are also measured "
+                         "(0 or empty: none)")
     ap.add_argument("--tune-cache", default=None, help="batch64: JSON cache of the autotuner's choices")
     ap.add_argument("--no-tune", action="store_true", help="batch64: default launch shapes, tiled P2G")
     ap.add_argument("--seed", type=int, default=1)

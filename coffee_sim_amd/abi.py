@@ -131,6 +131,7 @@ class Settings(C.Structure):
         ("warm_start", C.c_uint32),
         ("has_bed_field", C.c_uint32),
         ("warm_fallback_residual", C.c_float),
+        ("rbgs_rescale_after", C.c_uint32),
     ]
 
     @property

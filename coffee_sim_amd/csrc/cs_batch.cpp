@@ -402,6 +402,8 @@ int cs_batch_create(cs_ctx* ctx, cs_sim* const* sims, uint32_t count, const cs_b
     if (m->dense_solver) return cs_fail(CS_ERR_INVALID, "batched scenes need the brick solver");
     if (m->s.warm_start != CS_WARM_ZERO && m->s.warm_start != CS_WARM_PREVIOUS)
       return cs_fail(CS_ERR_INVALID, "batched scenes start their solves from zero or the previous pressure");
+    if (m->s.rbgs_rescale_after)
+      return cs_fail(CS_ERR_INVALID, "batched scenes do not take rbgs_rescale_after");
     if ((m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS) != fixed)
       return cs_fail(CS_ERR_INVALID, "every scene of a batch must use the same rbgs_mode");
     if (std::max(m->s.substeps, 1u) != std::max(m0->s.substeps, 1u))

@@ -152,6 +152,7 @@ int settings_preset(int preset, cs_settings* s) {
   s->warm_start = CS_WARM_ZERO;
   s->has_bed_field = 0u;
   s->warm_fallback_residual = 1.0f;
+  s->rbgs_rescale_after = 0u;
   if (preset == CS_PRESET_FREE_STREAM) {
     sp.origin[0] = 0.0f;
     sp.origin[1] = 6.8f;
@@ -182,6 +183,14 @@ int validate_settings(const cs_settings& s, std::string* why) {
   if (s.rbgs_mode > CS_RBGS_TOLERANCE) return fail("unknown rbgs_mode");
   if (s.warm_start > CS_WARM_LEARNED) return fail("unknown warm_start");
   if (!(s.warm_fallback_residual > 0.0f)) return fail("warm_fallback_residual must be positive");
+  if (s.rbgs_rescale_after) {
+    const uint32_t every = s.rbgs_check_every ? s.rbgs_check_every : 8u;
+    if (s.rbgs_mode != CS_RBGS_TOLERANCE) return fail("rbgs_rescale_after needs the residual-driven rbgs_mode");
+    if (s.rbgs_rescale_after % 4u != 0u || s.rbgs_rescale_after % every != 0u)
+      return fail("rbgs_rescale_after must be a multiple of 4 and of rbgs_check_every");
+    if ((uint64_t)s.rbgs_rescale_after + every > s.rbgs_max_iters)
+      return fail("rbgs_rescale_after leaves fewer than rbgs_check_every iterations under rbgs_max_iters");
+  }
   if (s.emitter_mode > CS_EMIT_FIXED_COUNT) return fail("unknown emitter_mode");
   return CS_OK;
 }

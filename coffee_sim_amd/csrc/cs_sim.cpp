@@ -130,6 +130,7 @@ int predict_warm_start(cs_sim* m, const cs_params& u) {
 // One-thread-per-cell solve over the whole grid (the reference's dispatch
 // shape), kept as a cross-check of the brick solve.
 int solve_dense(cs_sim* m, const cs_params& u) {
+  if (m->s.rbgs_rescale_after) return fail(CS_ERR_INVALID, "rbgs_rescale_after needs the brick solver");
   hipStream_t st = m->ctx->stream;
   cs_solve_stats& S = m->stats;
   memset(&S, 0, sizeof(S));
@@ -374,7 +375,12 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     const uint32_t every = m->s.rbgs_check_every ? m->s.rbgs_check_every : 8u;
     const uint32_t cap = m->s.rbgs_max_iters;
     const size_t pool = m->chunk_ev.size() / 2;  // event pairs, one per batch, >= 3 batches in flight
-    uint32_t enq = 0, batch = 0;
+    // rbgs_rescale_after: that many untested iterations (an even number of
+    // launches, so the pressure is back in m->pres), then the rescale; a start
+    // already within the tolerance skips both (the kernels read the state).
+    const uint32_t pre = m->s.rbgs_rescale_after;
+    uint32_t pre_launches = 0;
+    uint32_t enq = pre, batch = 0;
     uint32_t harvested = 0;  // batches whose sweep time has been added
     std::vector<uint32_t> batch_iter_end;  // iterations enqueued through each batch
     auto harvest = [&](uint32_t upto) {
@@ -402,6 +408,13 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
         int rc = capture_batch(m, u, nactive, every, prev);
         if (rc) return rc;
         graph_built = true;
+      }
+      if (batch == 0 && pre > 0) {
+        int rc = enqueue_sweeps(m, u, nactive, pre, &pre_launches, &cur);
+        if (rc) return rc;
+        if (cur != 0) return fail(CS_ERR_STATE, "rescale sweeps left the pressure in the second buffer");
+        CS_HIP(cs_launch_rescale(st, u, nactive, m->brick_list, m->wc, m->rhs, m->kf, m->pres, m->brick_partials,
+                                 m->dstate, m->metrics));
       }
       if (steady) {
         CS_HIP(hipGraphLaunch(m->batch_exec, st));
@@ -442,22 +455,26 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
                                        m->brick_partials, m->dstate, 1u, prev.iters, prev.launches, tol));
     CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
     CS_HIP(hipStreamSynchronize(st));
+    const cs_solve_state& hs = *m->h_state;
+    // the untested iterations before the rescale ran unless the start was within the tolerance
+    const bool pre_ran = pre > 0 && !(hs.initial_residual <= tol);
+    const uint32_t total_iters = hs.iterations + (pre_ran ? pre : 0u);
     if (m->timing) {
       // batches after the converged one returned at once; count the executed ones
       uint32_t executed = 0;
-      while (executed < batch && (executed == 0 || batch_iter_end[executed - 1] < m->h_state->iterations)) ++executed;
+      while (executed < batch && (executed == 0 || batch_iter_end[executed - 1] < total_iters)) ++executed;
       if (executed > harvested) harvest(executed);
     }
-    const cs_solve_state& hs = *m->h_state;
-    S.iterations = hs.iterations;
+    S.iterations = total_iters;
     S.initial_residual = hs.initial_residual;
     S.final_residual = hs.residual;
     S.converged = hs.done;
     // the solved pressure is in pres2 when an odd number of launches ran
     CS_HIP(cs_launch_select_final(st, u, nactive, m->brick_list, m->pres, m->pres2, m->dstate, true));
     if (m->timing) {
-      m->kt.rbgs_launches += hs.launches;
-      m->kt.rbgs_cells_swept += (uint64_t)hs.launches * nactive * CS_BRICK * CS_BRICK * CS_BRICK;
+      const uint32_t ran = hs.launches + (pre_ran ? pre_launches : 0u);
+      m->kt.rbgs_launches += ran;
+      m->kt.rbgs_cells_swept += (uint64_t)ran * nactive * CS_BRICK * CS_BRICK * CS_BRICK;
     }
   }
   if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_copy_i32(st, m->pres, m->prev_p, m->nc));

@@ -13,7 +13,7 @@ struct cs_solve_state {
   uint32_t launches;       // sweep launches executed (parity says which buffer is current)
   uint32_t active_bricks;  // bricks holding a fluid cell
   uint32_t pending_fires;   // clamp fires of a residual-computing launch, committed by the deciding one
-  uint32_t pad;
+  float rescale;           // Galerkin factor of the one-time rescale (rbgs_rescale_after)
   double residual;         // latest relative residual
   double initial_residual; // before the first sweep
 };
@@ -49,6 +49,11 @@ hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t
                                      const float* wc, const float* rhs, const int32_t* p, double* partials,
                                      cs_solve_state* st, uint32_t mode, uint32_t iters, uint32_t launches,
                                      double tol);
+// The one-time rescale p <- alpha p (alpha = <b,p>/<p,Ap> over the residual's rows) of the
+// fluid cells of `p`, skipped when the state says done; alpha is left in st->rescale.
+hipError_t cs_launch_rescale(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                             const float* wc, const float* rhs, const float* kf, int32_t* p, double* partials,
+                             cs_solve_state* st, uint32_t* metrics);
 // Brick cells of `b` into `a`: when the state's launch count is odd (`by_state`) or always.
 hipError_t cs_launch_select_final(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
                                   int32_t* a, const int32_t* b, const cs_solve_state* st, bool by_state);

@@ -651,8 +651,9 @@ __global__ void __launch_bounds__(NT) k_rbgs_multi(cs_params u, const uint32_t* 
 // last workgroup with an arrival counter instead serialises one device-scope
 // atomic per brick on a single address, which costs more than the launch.)
 // Sums of squared residual and squared right-hand side over one brick's fluid
-// cells (one 256-thread workgroup); valid in s_rr[0], s_bb[0] on return.
-template <int NT>
+// cells (one 256-thread workgroup); valid in s_rr[0], s_bb[0] on return.  With
+// ENERGY the same rows give <p, A p> and <p, b> instead (the Galerkin rescale).
+template <int NT, bool ENERGY = false>
 __device__ __forceinline__ void brick_residual_sums(const cs_params& u, uint32_t brick, const float4* __restrict__ wc,
                                                     const float* __restrict__ rhs, const int32_t* __restrict__ p,
                                                     float4* s_wc, int32_t* s_p, double* s_rr, double* s_bb) {
@@ -707,9 +708,15 @@ __device__ __forceinline__ void brick_residual_sums(const cs_params& u, uint32_t
 #pragma unroll
     for (int n = 0; n < 6; ++n) ap_n += (double)wn[n] * (double)p_float(u, s_p[li + off[n]]);
     const double b = -(double)rb[k];
-    const double ap = (double)count * (double)p_float(u, s_p[li]) - ap_n;
-    rr += (b - ap) * (b - ap);
-    bb += b * b;
+    const double pi = (double)p_float(u, s_p[li]);
+    const double ap = (double)count * pi - ap_n;
+    if (ENERGY) {
+      rr += pi * ap;
+      bb += pi * b;
+    } else {
+      rr += (b - ap) * (b - ap);
+      bb += b * b;
+    }
   }
   s_rr[threadIdx.x] = rr;
   s_bb[threadIdx.x] = bb;
@@ -778,6 +785,55 @@ __global__ void k_residual_empty(cs_solve_state* __restrict__ st, uint32_t mode,
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (mode != 0u && st->done) return;
   finalize_state(st, 0.0, 0.0, mode, iters, launches, tol);
+}
+
+// ---- one-time Galerkin rescale ----------------------------------------------------------------
+// After rbgs_rescale_after iterations the error left is mostly smooth, and a
+// start that carried a few-per-cent scale error still carries it.  The factor
+// alpha = <b, p> / <p, A p> minimises the energy norm of the error of alpha p
+// over alpha; it is formed over the residual's rows (fluid, non-empty
+// stencil, not held at the clamp), rounded to a multiple of 2^-20 so that the
+// summation order of the double sums cannot change it in practice, and kept in
+// [1/16, 16] (1 when the sums are degenerate).
+__device__ __forceinline__ float rescale_factor(double pap, double pb) {
+  if (!(pap > 0.0)) return 1.0f;
+  const double a = pb / pap;
+  if (!(a >= 0.0625 && a <= 16.0)) return a > 16.0 ? 16.0f : (a < 0.0625 ? 0.0625f : 1.0f);
+  return (float)(rint(a * 1048576.0) * (1.0 / 1048576.0));
+}
+__global__ void __launch_bounds__(256) k_rescale_bricks(cs_params u, const uint32_t* __restrict__ list,
+                                                        const float4* __restrict__ wc, const float* __restrict__ rhs,
+                                                        const int32_t* __restrict__ p, double* __restrict__ partials,
+                                                        const cs_solve_state* __restrict__ st) {
+  constexpr int R3 = (BR + 2) * (BR + 2) * (BR + 2), NT = 256;
+  __shared__ float4 s_wc[R3];
+  __shared__ int32_t s_p[R3];
+  __shared__ double s_rr[NT], s_bb[NT];
+  if (st->done) return;
+  brick_residual_sums<NT, true>(u, list[blockIdx.x], wc, rhs, p, s_wc, s_p, s_rr, s_bb);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = s_rr[0];
+    partials[2 * blockIdx.x + 1] = s_bb[0];
+  }
+}
+__global__ void __launch_bounds__(1024) k_rescale_reduce(const double* __restrict__ partials, uint32_t n,
+                                                         cs_solve_state* __restrict__ st) {
+  __shared__ double s_rr[1024], s_bb[1024];
+  if (st->done) return;
+  reduce_partials_block(partials, n, s_rr, s_bb);
+  if (threadIdx.x == 0) st->rescale = rescale_factor(s_rr[0], s_bb[0]);
+}
+// p <- alpha p in every fluid cell, clamped and quantised as pressure_update stores it.
+__global__ void __launch_bounds__(256) k_rescale_apply(cs_params u, const float* __restrict__ kf,
+                                                       int32_t* __restrict__ p, const cs_solve_state* __restrict__ st,
+                                                       uint32_t* __restrict__ metrics) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= u.nc || st->done) return;
+  if (!kf_fluid(kf[idx])) return;
+  const float v = st->rescale * p_float(u, p[idx]);
+  const float cl = cs_clamp(v, -u.pressure_clamp, u.pressure_clamp);
+  if (cl != v) atomicAdd(&metrics[CS_M_P_CLAMP], 1u);
+  p[idx] = cs_f2i(cl * u.fp_scale);
 }
 
 // Copies the brick cells of the second buffer back to the first: when an odd
@@ -1140,6 +1196,17 @@ hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t
                        tol);
   } else
     hipLaunchKernelGGL(k_residual_empty, dim3(1), dim3(64), 0, s, st, mode, iters, launches, tol);
+  return hipGetLastError();
+}
+
+hipError_t cs_launch_rescale(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                             const float* wc, const float* rhs, const float* kf, int32_t* p, double* partials,
+                             cs_solve_state* st, uint32_t* metrics) {
+  if (nactive == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rescale_bricks, dim3(nactive), dim3(256), 0, s, u, list, reinterpret_cast<const float4*>(wc),
+                     rhs, p, partials, st);
+  hipLaunchKernelGGL(k_rescale_reduce, dim3(1), dim3(1024), 0, s, partials, nactive, st);
+  hipLaunchKernelGGL(k_rescale_apply, dim3((u.nc + 255u) / 256u), dim3(256), 0, s, u, kf, p, st, metrics);
   return hipGetLastError();
 }
 

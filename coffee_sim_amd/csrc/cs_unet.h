@@ -5,7 +5,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
-#define CS_UNET_BIAS_ROWS 256  // row blocks of the bias-gradient reduction
+#define CS_UNET_BIAS_ROWS 1024  // workgroups (partial rows) of the bias-gradient reduction
 
 // 3x3x3 'same' convolution over an s^3 grid (s a multiple of 8), bf16 [voxel][channel]:
 // w is [27][cout][cin]; bias may be null; with `mask` the input counts only where the

@@ -217,25 +217,51 @@ __global__ void k_unet_reduce_slots(const float* __restrict__ partial, int slots
   out[e] = acc;
 }
 
-// Bias gradient, stage 1: part[rb][co] = sum over the rb-th share of voxels of
-// gz[v][co]; grid (cout / 16, row blocks), threads = 16 channels x 16 rows.
+// Bias gradient, stage 1: part[g][co] = sum over the g-th share of voxels of
+// gz[v][co] (gout where the layer output is positive).  Each thread reads
+// 8 channels of a voxel as one 16-byte vector of gout and one of the output
+// (lanes = cout / 8 threads per voxel, rows = 256 / lanes voxels per pass,
+// two voxels in flight per thread); the rows are then summed per channel in
+// row order through LDS.
 __global__ void __launch_bounds__(256) k_unet_bias_grad(const uint16_t* __restrict__ gout,
                                                         const uint16_t* __restrict__ outv, int cout, size_t nv,
                                                         float* __restrict__ part) {
-  __shared__ float s_sum[256];
-  const int co = (int)blockIdx.x * 16 + (int)(threadIdx.x & 15u);
-  const size_t rows = (size_t)gridDim.y * 16;
-  float acc = 0.0f;
-  for (size_t v = (size_t)blockIdx.y * 16 + (threadIdx.x >> 4); v < nv; v += rows) {
-    const uint16_t o = outv[v * cout + co];
-    if (bf_pos(o)) acc += bf2f(gout[v * cout + co]);
+  __shared__ float s_sum[256 * 8];
+  const int lanes = cout / 8, rows = 256 / lanes;
+  const int t = (int)threadIdx.x, row = t / lanes, lane = t % lanes;
+  float acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0f;
+  if (row < rows) {
+    const size_t stride = (size_t)gridDim.x * rows;
+    auto add = [&](const uint4 g, const uint4 o) {
+      const uint32_t gw[4] = {g.x, g.y, g.z, g.w}, ow[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (bf_pos((uint16_t)(ow[q] & 0xffffu))) acc[2 * q] += bf2f((uint16_t)(gw[q] & 0xffffu));
+        if (bf_pos((uint16_t)(ow[q] >> 16))) acc[2 * q + 1] += bf2f((uint16_t)(gw[q] >> 16));
+      }
+    };
+    size_t v = (size_t)blockIdx.x * rows + row;
+    for (; v + stride < nv; v += 2 * stride) {
+      const size_t e0 = v * cout + 8 * lane, e1 = (v + stride) * cout + 8 * lane;
+      const uint4 g0 = *reinterpret_cast<const uint4*>(gout + e0), o0 = *reinterpret_cast<const uint4*>(outv + e0);
+      const uint4 g1 = *reinterpret_cast<const uint4*>(gout + e1), o1 = *reinterpret_cast<const uint4*>(outv + e1);
+      add(g0, o0);
+      add(g1, o1);
+    }
+    if (v < nv) {
+      const size_t e0 = v * cout + 8 * lane;
+      add(*reinterpret_cast<const uint4*>(gout + e0), *reinterpret_cast<const uint4*>(outv + e0));
+    }
   }
-  s_sum[threadIdx.x] = acc;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s_sum[t * 8 + j] = acc[j];
   __syncthreads();
-  if (threadIdx.x < 16) {
-    float t = 0.0f;
-    for (int r = 0; r < 16; ++r) t += s_sum[r * 16 + threadIdx.x];
-    part[(size_t)blockIdx.y * cout + co] = t;
+  if (t < cout) {
+    float sum = 0.0f;
+    for (int r = 0; r < rows; ++r) sum += s_sum[(r * lanes + t / 8) * 8 + t % 8];
+    part[(size_t)blockIdx.x * cout + t] = sum;
   }
 }
 
@@ -454,6 +480,8 @@ int cs_unet_wgrad_slots(int s) {
 hipError_t cs_unet_launch_wgrad(hipStream_t st, const uint16_t* in, int cin, const uint16_t* gout,
                                 const uint16_t* outv, int cout, int s, float* partial, float* bias_part, float* dw,
                                 float* db) {
+  // the bias gradient reads 8 channels per 16-byte vector
+  if (cout % 8 != 0 || cout > 256 || ((uintptr_t)gout | (uintptr_t)outv) % 16u != 0u) return hipErrorInvalidValue;
   const int nbricks = (s / 8) * (s / 8) * (s / 8);
   const int slots = cs_unet_wgrad_slots(s);
   const int per = (nbricks + slots - 1) / slots;
@@ -461,7 +489,7 @@ hipError_t cs_unet_launch_wgrad(hipStream_t st, const uint16_t* in, int cin, con
                      in, cin, gout, outv, cout, s, per, partial);
   const size_t n = (size_t)27 * cin * cout;
   hipLaunchKernelGGL(k_unet_reduce_slots, grid1(n), dim3(256), 0, st, partial, slots, n, dw);
-  hipLaunchKernelGGL(k_unet_bias_grad, dim3((unsigned)(cout / 16), (unsigned)CS_UNET_BIAS_ROWS), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_unet_bias_grad, dim3((unsigned)CS_UNET_BIAS_ROWS), dim3(256), 0, st,
                      gout, outv, cout, (size_t)s * s * s, bias_part);
   hipLaunchKernelGGL(k_unet_sum_rows, dim3((unsigned)cout), dim3(256), 0, st, bias_part, CS_UNET_BIAS_ROWS, cout, db);
   return hipGetLastError();

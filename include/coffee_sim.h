@@ -147,6 +147,15 @@ typedef struct cs_settings {
   float warm_fallback_residual; /* an external or learned start whose relative residual
                                  * exceeds this restarts from zero (1: no worse than the
                                  * zero start's residual)                          */
+  uint32_t rbgs_rescale_after;  /* residual mode, brick solver: after this many iterations
+                                 * the pressure is rescaled once, p <- alpha p with the
+                                 * Galerkin factor alpha = <b,p>/<p,Ap> (rounded to a
+                                 * multiple of 2^-20), then the solve goes on.  0 = off
+                                 * (the reference's plain iteration); otherwise a
+                                 * multiple of 4 and of rbgs_check_every, with at least
+                                 * rbgs_check_every iterations left under rbgs_max_iters.
+                                 * The iterations before the rescale count toward
+                                 * cs_solve_stats.iterations.  No reference counterpart. */
 } cs_settings;
 
 /* Per-cell porous bed ("field mode").  The reference stamps bed-particle ids into

@@ -75,6 +75,11 @@ void orc_pack_render(orc_sim* sim, float* render8);
 void orc_get_solve_stats(orc_sim* sim, cs_solve_stats* out);
 /* relative residual of the current pressure (definition shared with the product) */
 double orc_residual(orc_sim* sim);
+/* Galerkin factor <p,b>/<p,Ap> of the current pressure over the residual's rows,
+ * unrounded (NaN when <p,Ap> <= 0); the solve rounds it as rbgs_rescale_after says. */
+double orc_rescale_factor(orc_sim* sim);
+/* Overwrite the pressure lane (fixed point, one value per cell), for tests. */
+void orc_debug_set_pressure(orc_sim* sim, const int32_t* p);
 
 /* Dense pressure-system pieces after classify, for solver unit tests: per fluid
  * cell the 6 face weights (order -x,+x,-y,+y,-z,+z), their running sum and the

@@ -76,6 +76,8 @@ def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
             "orc_pack_render": (None, [C.c_void_p, f32p]),
             "orc_get_solve_stats": (None, [C.c_void_p, P(SolveStats)]),
             "orc_residual": (C.c_double, [C.c_void_p]),
+            "orc_rescale_factor": (C.c_double, [C.c_void_p]),
+            "orc_debug_set_pressure": (None, [C.c_void_p, i32p]),
             "orc_read_system": (None, [C.c_void_p, f32p, f32p, f32p]),
             "orc_debug_step_until": (C.c_int, [C.c_void_p, f32, C.c_int]),
         }
@@ -240,6 +242,15 @@ class OracleSim:
 
     def residual(self) -> float:
         return float(self._L.orc_residual(self._h))
+
+    def rescale_factor(self) -> float:
+        """Unrounded Galerkin factor <p,b>/<p,Ap> of the current pressure lane."""
+        return float(self._L.orc_rescale_factor(self._h))
+
+    def debug_set_pressure(self, p_fixed: np.ndarray) -> None:
+        p = np.ascontiguousarray(p_fixed, dtype=np.int32).reshape(-1)
+        assert p.size == self.nc
+        self._L.orc_debug_set_pressure(self._h, _i(p))
 
     def read_system(self):
         w = np.empty((self.nc, 6), np.float32)

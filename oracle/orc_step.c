@@ -633,6 +633,56 @@ double orc_residual(orc_sim* S) {
   return residual_rows(S, &S->u);
 }
 
+/* One-time Galerkin rescale (rbgs_rescale_after; no reference counterpart).
+ * Over the residual's rows, <p, A p> and <p, b>; the factor alpha = <p,b>/<p,Ap>
+ * minimises the energy norm of the error of alpha p.  It is rounded to a
+ * multiple of 2^-20 and kept in [1/16, 16] (1 for degenerate sums). */
+static void energy_rows(orc_sim* S, const orc_params* u, double* pap_out, double* pb_out) {
+  double pap = 0.0, pb = 0.0;
+  for (int col = 0; col < 2; ++col) {
+    const sys_row* rows = g_rows[col];
+    const size_t n = g_nrows[col];
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : pap, pb) schedule(static)
+#endif
+    for (size_t i = 0; i < n; ++i) {
+      const sys_row* r = &rows[i];
+      if (r->count <= 0.0f) continue;
+      const float pf = pressure_load(S, u, r->idx);
+      if (fabsf(pf) == u->pressure_clamp) continue;
+      double ap = (double)r->count * (double)pf;
+      for (int k = 0; k < 6; ++k)
+        if (r->w[k] > 0.0f && r->nf[k]) ap -= (double)r->w[k] * (double)pressure_load(S, u, r->nb[k]);
+      pap += (double)pf * ap;
+      pb += (double)pf * -(double)r->c;
+    }
+  }
+  *pap_out = pap;
+  *pb_out = pb;
+}
+static float rescale_factor(double pap, double pb) {
+  if (!(pap > 0.0)) return 1.0f;
+  const double a = pb / pap;
+  if (!(a >= 0.0625 && a <= 16.0)) return a > 16.0 ? 16.0f : (a < 0.0625 ? 0.0625f : 1.0f);
+  return (float)(rint(a * 1048576.0) * (1.0 / 1048576.0));
+}
+static void rescale_pressure(orc_sim* S, const orc_params* u) {
+  double pap, pb;
+  energy_rows(S, u, &pap, &pb);
+  const float a = rescale_factor(pap, pb);
+  for (size_t i = 0; i < S->nc; ++i)
+    if (is_fluid(kind_load(S, i))) pressure_store(S, u, i, a * pressure_load(S, u, i));
+}
+double orc_rescale_factor(orc_sim* S) {
+  double pap, pb;
+  build_rows(S, &S->u);
+  energy_rows(S, &S->u, &pap, &pb);
+  return pap > 0.0 ? pb / pap : NAN;
+}
+void orc_debug_set_pressure(orc_sim* S, const int32_t* p) {
+  for (size_t i = 0; i < S->nc; ++i) LANE(S, 0, i) = p[i];
+}
+
 void orc_read_system(orc_sim* S, float* w6o, float* cnto, float* co) {
   const orc_params* u = &S->u;
   for (size_t idx = 0; idx < S->nc; ++idx) {
@@ -695,6 +745,15 @@ static void solve_pressure(orc_sim* S, const orc_params* u) {
     }
     st->initial_residual = r;
     uint32_t it = 0, every = S->s.rbgs_check_every ? S->s.rbgs_check_every : 8u;
+    if (S->s.rbgs_rescale_after && !(r <= (double)S->s.rbgs_tolerance)) {
+      /* untested iterations, then the rescale; the next test is at it = rescale_after + every */
+      for (uint32_t k = 0; k < S->s.rbgs_rescale_after; ++k) {
+        k_rbgs_half(S, u, 0u);
+        k_rbgs_half(S, u, 1u);
+      }
+      it = S->s.rbgs_rescale_after;
+      rescale_pressure(S, u);
+    }
     while (!(r <= (double)S->s.rbgs_tolerance) && it < S->s.rbgs_max_iters) {
       k_rbgs_half(S, u, 0u);
       k_rbgs_half(S, u, 1u);

@@ -165,6 +165,62 @@ def test_external_warm_start_and_fallback_match_oracle(ctx, orc):
         compare_state(sim, ref, f"external warm start ({label})")
 
 
+@pytest.mark.parametrize("variant", [
+    {},
+    {"CS_RBGS_FUSE": "0"},
+    {"CS_GRAPHS": "0"},
+    {"CS_RBGS_HALO": "2", "CS_RBGS_THREADS": "256"},
+    {"CS_RBGS_HALO": "1"},
+], ids=["brick-h4-multi", "brick-h4-multi-unfused", "brick-h4-multi-nographs", "brick-h2", "brick-h1"])
+def test_rescaled_solve_matches_oracle(ctx, orc, monkeypatch, variant):
+    """rbgs_rescale_after: 32 untested iterations, the Galerkin rescale, then
+    the tested iteration, from the zero, previous-step and an external start
+    (the zero-start solution scaled by 0.95 plus noise), on every brick layout:
+    pressure, iteration count and particles equal the oracle's bit for bit."""
+    for k, v in variant.items():
+        monkeypatch.setenv(k, v)
+    w = cs.SeededWorkload("v60_working", seed=6, scale=4)
+    nc = int(np.prod(list(w.settings.grid_dims)))
+    for start in (cs.abi.CS_WARM_ZERO, cs.abi.CS_WARM_PREVIOUS, cs.abi.CS_WARM_EXTERNAL):
+        s = w.settings.copy()
+        s.warm_start = start
+        s.rbgs_rescale_after = 32
+        s.warm_fallback_residual = 1e30
+        field = None
+        if start == cs.abi.CS_WARM_EXTERNAL:
+            probe = cs.MpmSim3D(ctx, w.settings, w.scene)
+            probe.step_frame(w.frame_dt)
+            p_fixed, _ = probe.read_pressure()
+            probe.close()
+            rng = np.random.default_rng(7)
+            field = (0.95 * p_fixed.astype(np.float64) / cs.abi.FP_SCALE
+                     + rng.normal(0.0, 0.5, nc)).astype(np.float32)
+        sim, ref = _pair(ctx, orc, s, w.scene)
+        for k in range(2 if start == cs.abi.CS_WARM_PREVIOUS else 1):
+            if field is not None:
+                sim.set_warm_start(field)
+                ref.set_warm_start(field)
+            sim.step_frame(w.frame_dt)
+            ref.step_frame(w.frame_dt)
+            sg, so = sim.solve_stats(), ref.solve_stats()
+            assert sg.iterations == so.iterations, (variant, start, k, sg.as_dict(), so.as_dict())
+            assert sg.iterations > 32 and sg.converged == so.converged == 1
+            compare_state(sim, ref, f"rescale {variant} start {start} substep {k}")
+        sim.close()
+        ref.close()
+
+
+def test_rescale_needs_the_brick_solver(ctx, monkeypatch):
+    monkeypatch.setenv("CS_SOLVER", "dense")
+    w = cs.SeededWorkload("v60_working", seed=6, scale=4)
+    s = w.settings.copy()
+    s.rbgs_rescale_after = 32
+    sim = cs.MpmSim3D(ctx, s, w.scene)
+    with pytest.raises(cs.CoffeeSimError, match="brick solver"):
+        sim.step_frame(w.frame_dt)
+    sim.close()
+
+
 def test_save_load_round_trip_continues_bitwise(ctx):
     """A checkpoint taken mid-run and loaded into a fresh simulation continues
     exactly as the original does."""
