@@ -90,6 +90,7 @@ def _bind(lib: C.CDLL) -> None:
         "cs_sim_get_pass_times": (C.c_int, [C.c_void_p, P(C.c_float)]),
         "cs_sim_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
         "cs_device_expf": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), C.c_uint32]),
+        "cs_device_sincosf": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_float), C.c_uint32]),
         "cs_sim_get_kernel_timing": (C.c_int, [C.c_void_p, P(abi.KernelTiming)]),
         "cs_sim_reset_kernel_timing": (C.c_int, [C.c_void_p]),
         "cs_unet_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, P(C.c_void_p)]),
@@ -266,6 +267,13 @@ class Context:
         out = np.empty_like(x)
         _check(load_library().cs_device_expf(self._h, _fptr(x), _fptr(out), x.size), "cs_device_expf")
         return out
+
+    def device_sincosf(self, x: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        sn, cs_ = np.empty_like(x), np.empty_like(x)
+        _check(load_library().cs_device_sincosf(self._h, _fptr(x), _fptr(sn), _fptr(cs_), x.size),
+               "cs_device_sincosf")
+        return sn, cs_
 
     def close(self) -> None:
         if self._h:

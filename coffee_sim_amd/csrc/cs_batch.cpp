@@ -2,9 +2,9 @@
 // per pass over every scene (the batched kernels of cs_kernels.hip,
 // cs_p2g.hip and cs_solver.hip).
 //
-// A substep of the batch: the host emits each scene's particles into one
-// staging buffer and writes each scene's parameter block into the descriptor
-// table (two copies to the device); then the fixed launch sequence up to the
+// A substep of the batch: the host runs each scene's flow model and writes
+// its emission frame and parameter block into the descriptor table (one copy
+// to the device; the begin pass generates the particles); then the fixed launch sequence up to the
 // first residual test, the solve loop, and the fixed sequence after the
 // solve.  The fixed sequences, and a steady-state group of solve chunks, keep
 // the same arguments for the life of the batch (only the table's contents
@@ -38,8 +38,6 @@ struct cs_batch {
   // device
   cs_scene_dev* d_sc = nullptr;
   uint32_t* d_starts = nullptr;
-  float* d_emit = nullptr;
-  size_t emit_cap = 0;  // floats
   uint32_t* entries = nullptr;
   uint32_t* first = nullptr;
   uint32_t* count = nullptr;
@@ -49,7 +47,6 @@ struct cs_batch {
   uint32_t* d_fluid = nullptr;
   // host
   cs_scene_dev* h_sc = nullptr;  // pinned
-  float* h_emit = nullptr;       // pinned
   std::vector<uint32_t> h_starts;
   cs_solve_state* h_states = nullptr;  // pinned [n]
   uint32_t* h_fluid = nullptr;         // pinned [n]
@@ -61,7 +58,6 @@ struct cs_batch {
   hipGraphExec_t exec[3] = {};  // before the solve loop, a group of chunks, after the solve
   bool captured = false;
   cs_batch_stats stats{};
-  std::vector<float> emit_buf;
 };
 
 namespace {
@@ -80,11 +76,11 @@ void free_batch(cs_batch* b) {
     if (b->exec[g]) (void)hipGraphExecDestroy(b->exec[g]);
     if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
   }
-  void* dev[] = {b->d_sc,    b->d_starts, b->d_emit,   b->entries, b->first,
-                 b->count,   b->bs,       b->partials, b->d_states, b->d_fluid};
+  void* dev[] = {b->d_sc, b->d_starts, b->entries,  b->first,    b->count,
+                 b->bs,   b->partials, b->d_states, b->d_fluid};
   for (void* p : dev)
     if (p) (void)hipFree(p);
-  void* host[] = {b->h_sc, b->h_emit, b->h_states, b->h_fluid, b->h_bs};
+  void* host[] = {b->h_sc, b->h_states, b->h_fluid, b->h_bs};
   for (void* p : host)
     if (p) (void)hipHostFree(p);
   for (auto& e : b->poll_ev)
@@ -243,39 +239,19 @@ int capture_all(cs_batch* b) {
   return CS_OK;
 }
 
-// Emission and parameter blocks of every scene, uploaded with two copies.
+// Emission frames and parameter blocks of every scene, uploaded with one copy
+// (the particles themselves are generated by the begin pass).
 int upload_substep(cs_batch* b, float sub_dt) {
   hipStream_t st = b->ctx->stream;
-  CS_BHIP(hipEventSynchronize(b->upload_done));  // the pinned tables are free again
-  b->emit_buf.clear();
-  std::vector<size_t> offs(b->n);
+  CS_BHIP(hipEventSynchronize(b->upload_done));  // the pinned table is free again
   for (uint32_t s = 0; s < b->n; ++s) {
     cs_sim* m = b->sims[s];
-    offs[s] = b->emit_buf.size();
-    const uint32_t emitted = cs::emit(m->s, m->inflow, sub_dt, m->num_water, b->emit_buf, &m->total_dropped);
     cs_scene_dev& d = b->h_sc[s];
-    d.emit_count = emitted;
-    d.emit_first = m->num_water;
+    const uint32_t emitted = cs::emit_plan(m->s, m->inflow, sub_dt, m->num_water, &d.emit, &m->total_dropped);
     m->num_water += emitted;
     m->total_emitted_mass += (float)emitted * (80.0f / 320.0f);
     cs::make_params(m->s, m->num_water, m->num_bed_cells, sub_dt, &d.u);
     b->stats.particle_substeps += m->num_water;
-  }
-  const size_t floats = b->emit_buf.size();
-  if (floats > b->emit_cap) {
-    if (b->d_emit) CS_BHIP(hipFree(b->d_emit));
-    if (b->h_emit) CS_BHIP(hipHostFree(b->h_emit));
-    b->d_emit = nullptr;
-    b->h_emit = nullptr;
-    const size_t cap = std::max<size_t>(floats, 2 * b->emit_cap);
-    CS_BHIP(balloc(&b->d_emit, cap));
-    CS_BHIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_emit), cap * sizeof(float)));
-    b->emit_cap = cap;
-  }
-  for (uint32_t s = 0; s < b->n; ++s) b->h_sc[s].emit_src = b->d_emit + offs[s];
-  if (floats) {
-    memcpy(b->h_emit, b->emit_buf.data(), floats * sizeof(float));
-    CS_BHIP(hipMemcpyAsync(b->d_emit, b->h_emit, floats * sizeof(float), hipMemcpyHostToDevice, st));
   }
   CS_BHIP(hipMemcpyAsync(b->d_sc, b->h_sc, (size_t)b->n * sizeof(cs_scene_dev), hipMemcpyHostToDevice, st));
   CS_BHIP(hipEventRecord(b->upload_done, st));

@@ -41,9 +41,7 @@ struct cs_scene_dev {
   uint32_t* tile_start;
   uint2* p2g_key;
   uint32_t* p2g_sorted;
-  const float* emit_src;   // this substep's emitted particles (8 floats each) in the shared staging buffer
-  uint32_t emit_count;
-  uint32_t emit_first;     // particle index of the first emitted particle
+  cs_emit_frame emit;      // this substep's emission (count, first slot, golden-angle disk frame)
   uint32_t warm_previous;  // seed the solve from prev_p
   uint32_t p2g_tiled;      // tile-binned P2G (else one thread per particle)
   double tol;

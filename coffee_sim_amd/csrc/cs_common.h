@@ -100,6 +100,76 @@ CS_HD float cs_expf(float x) {
   return p * scale;
 }
 
+// sin and cos of a float argument, each rounded once to float; the same
+// double-precision steps on the host and the device, so both give the same
+// bits.  Cody-Waite reduction x = k pi/2 + r with pi/2 = c1 + c2 + c3, where c1
+// and c2 hold 22 significant bits so k c1 and k c2 are exact for |k| < 2^31
+// (|x| < 3.3e9); then the Taylor series of sin r (through r^17) and cos r
+// (through r^18) on |r| <= pi/4 in Horner form (truncation below 1e-18), and the
+// quadrant k mod 4.  Used for the emitter's golden angle theta = sample * 2.399963.
+CS_HD void cs_sincosf(float x, float* sn, float* cs) {
+  const double xd = (double)x;
+  const double k = rint(xd * 0x1.45f306dc9c883p-1);
+  const double r = ((xd - k * 0x1.921fb8p+0) - k * -0x1.5dde98p-23) - k * 0x1.8469898cc5170p-48;
+  const double r2 = r * r;
+  double ps = 0x1.952c77030ad4ap-49;         // 1/17!
+  ps = ps * r2 - 0x1.ae7f3e733b81fp-41;      // 1/15!
+  ps = ps * r2 + 0x1.6124613a86d09p-33;      // 1/13!
+  ps = ps * r2 - 0x1.ae64567f544e4p-26;      // 1/11!
+  ps = ps * r2 + 0x1.71de3a556c734p-19;      // 1/9!
+  ps = ps * r2 - 0x1.a01a01a01a01ap-13;      // 1/7!
+  ps = ps * r2 + 0x1.1111111111111p-7;       // 1/5!
+  ps = ps * r2 - 0x1.5555555555555p-3;       // 1/3!
+  const double sr = r + (r * r2) * ps;
+  double pc = -0x1.6827863b97d97p-53;        // -1/18!
+  pc = pc * r2 + 0x1.ae7f3e733b81fp-45;      // 1/16!
+  pc = pc * r2 - 0x1.93974a8c07c9dp-37;      // 1/14!
+  pc = pc * r2 + 0x1.1eed8eff8d898p-29;      // 1/12!
+  pc = pc * r2 - 0x1.27e4fb7789f5cp-22;      // 1/10!
+  pc = pc * r2 + 0x1.a01a01a01a01ap-16;      // 1/8!
+  pc = pc * r2 - 0x1.6c16c16c16c17p-10;      // 1/6!
+  pc = pc * r2 + 0x1.5555555555555p-5;       // 1/4!
+  pc = pc * r2 - 0.5;
+  const double cr = 1.0 + r2 * pc;
+  const long long q = (long long)k & 3;
+  const double sd = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
+  const double cd = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
+  *sn = (float)sd;
+  *cs = (float)cd;
+}
+
+// One substep's emission (InflowState::emit_particles, inflow.rs:165-219):
+// `count` particles of the golden-angle disk, samples seq0.., written to
+// particle slots first...  The frame (spout tangents, emit origin, velocity)
+// is formed once on the host.
+struct cs_emit_frame {
+  float t1[3], t2[3], origin[3], vel[3];
+  float jet_radius, dt, mass;
+  uint32_t seq0, count, first;
+};
+// Particle i of the frame: pos.xyz, J = 1, vel.xyz, mass (the loop body of inflow.rs:195-214).
+CS_HD void cs_emit_particle(const cs_emit_frame& f, uint32_t i, float* p8) {
+  const float golden = 2.399963f, radial_irr = 0.7548777f;
+  const uint32_t sample = f.seq0 + i;  // wrapping_add
+  const float sf = (float)sample;
+  float ru = (sf + 0.5f) * radial_irr;
+  ru = ru - truncf(ru);  // Rust fract
+  const float r = f.jet_radius * sqrtf(ru);
+  const float theta = sf * golden;
+  const float age = (((float)i + 0.5f) / (float)f.count) * f.dt;
+  float st, ct;
+  cs_sincosf(theta, &st, &ct);
+  for (int c = 0; c < 3; ++c) {
+    const float off = f.t1[c] * r * ct + f.t2[c] * r * st;
+    p8[c] = f.origin[c] + off - f.vel[c] * age;
+  }
+  p8[3] = 1.0f;
+  p8[4] = f.vel[0];
+  p8[5] = f.vel[1];
+  p8[6] = f.vel[2];
+  p8[7] = f.mass;
+}
+
 struct cs_f3 {
   float x, y, z;
 };

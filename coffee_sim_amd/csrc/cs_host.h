@@ -44,9 +44,15 @@ float exit_speed_cap_from_angle(float angle, const cs_spout& sp);
 float exit_speed_from_head(float head, float max_exit_speed);
 float flow_rate_from_speed(float v, float nozzle_radius, float cd, float vol_to_ml, float max_flow);
 
-// InflowState::emit_particles (inflow.rs:122-241).  Appends the emitted
-// particles (8 floats) to `out` (affine of new particles is all zero) and
-// returns how many; `dropped` accumulates particles refused for capacity.
+// InflowState::emit_particles (inflow.rs:122-241) up to the particle loop: the
+// flow model and the capacity check decide how many particles this substep
+// emits (the return value; `dropped` accumulates particles refused for
+// capacity), and `f` receives the disk frame for them at slots current...
+// The inflow state advances as if they were emitted.
+uint32_t emit_plan(const cs_settings& s, Inflow& in, float dt, uint32_t current, cs_emit_frame* f,
+                   uint32_t* dropped);
+// emit_plan, then the particles on the host (8 floats each; the affine of new
+// particles is all zero) appended to `out`.
 uint32_t emit(const cs_settings& s, Inflow& in, float dt, uint32_t current, std::vector<float>& out,
               uint32_t* dropped);
 

@@ -44,6 +44,10 @@ hipError_t cs_launch_g2p(hipStream_t stream, const cs_params& u, float* part, fl
 hipError_t cs_launch_pack_render(hipStream_t stream, const cs_params& u, const float* part, const float* aff,
                                  uint32_t num_bed, float* out);
 hipError_t cs_launch_expf(hipStream_t stream, const float* in, float* out, uint32_t n);
+hipError_t cs_launch_sincosf(hipStream_t stream, const float* in, float* sn, float* cs, uint32_t n);
+// The particles of one substep's emission (f.count of them at slots f.first..)
+// and their zero affine records.
+hipError_t cs_launch_emit(hipStream_t stream, const cs_emit_frame& f, float* part, float* aff);
 
 // Blocks used by the residual reduction (size of the partials buffer / 2).
 #define CS_RESIDUAL_BLOCKS 1024u

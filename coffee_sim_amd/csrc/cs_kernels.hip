@@ -622,6 +622,28 @@ __global__ void k_expf(const float* __restrict__ in, float* __restrict__ out, ui
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = cs_expf(in[i]);
 }
+__global__ void k_sincosf(const float* __restrict__ in, float* __restrict__ sn, float* __restrict__ cs, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cs_sincosf(in[i], &sn[i], &cs[i]);
+}
+
+// ---- inflow emitter on the device (inflow.rs:195-214; the frame and count come from the host) ----
+// One thread per emitted particle: its 32 B particle record from the golden-angle
+// disk and a zero affine record (48 B), at slot f.first + i.
+__global__ void __launch_bounds__(256) k_emit(cs_emit_frame f, float* __restrict__ part, float* __restrict__ aff) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.count) return;
+  float p8[8];
+  cs_emit_particle(f, i, p8);
+  float4* pd = reinterpret_cast<float4*>(part + (size_t)(f.first + i) * 8);
+  pd[0] = make_float4(p8[0], p8[1], p8[2], p8[3]);
+  pd[1] = make_float4(p8[4], p8[5], p8[6], p8[7]);
+  float4* ad = reinterpret_cast<float4*>(aff + (size_t)(f.first + i) * 12);
+  const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ad[0] = z;
+  ad[1] = z;
+  ad[2] = z;
+}
 
 // ---- inputs and targets of the learned warm start (no reference counterpart) ---------
 // Four features per cell as the network reads them: rhs * rhs_scale and liquid
@@ -667,19 +689,20 @@ __global__ void __launch_bounds__(256) k_unet_features(cs_params u, const float*
   const cs_params u = d.u;                                                                            \
   const uint32_t tid = blk * 256u + threadIdx.x
 
-// Emitted particles into the particle arrays (affine zeroed), grid lanes,
-// metric slots, P2G tile counts and brick flags to zero: the host-side
-// append and buffer clears at the start of a substep.
+// Emitted particles from each scene's disk frame into the particle arrays
+// (affine zeroed), grid lanes, metric slots, P2G tile counts and brick flags
+// to zero: the emission and buffer clears at the start of a substep.
 __global__ void __launch_bounds__(256) k_batch_begin(const cs_scene_dev* __restrict__ sc,
                                                      const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_CELLS);
   const uint32_t span = (st_[scene_ + 1u] - st_[scene_]) * 256u;
-  for (uint32_t i = tid; i < d.emit_count; i += span) {
-    const float4* src = reinterpret_cast<const float4*>(d.emit_src) + 2u * (size_t)i;
-    float4* dp = reinterpret_cast<float4*>(d.part) + 2u * ((size_t)d.emit_first + i);
-    float4* da = reinterpret_cast<float4*>(d.aff) + 3u * ((size_t)d.emit_first + i);
-    dp[0] = src[0];
-    dp[1] = src[1];
+  for (uint32_t i = tid; i < d.emit.count; i += span) {
+    float p8[8];
+    cs_emit_particle(d.emit, i, p8);
+    float4* dp = reinterpret_cast<float4*>(d.part) + 2u * ((size_t)d.emit.first + i);
+    float4* da = reinterpret_cast<float4*>(d.aff) + 3u * ((size_t)d.emit.first + i);
+    dp[0] = make_float4(p8[0], p8[1], p8[2], p8[3]);
+    dp[1] = make_float4(p8[4], p8[5], p8[6], p8[7]);
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     da[0] = z;
     da[1] = z;
@@ -830,6 +853,16 @@ hipError_t cs_launch_pack_render(hipStream_t s, const cs_params& u, const float*
 hipError_t cs_launch_expf(hipStream_t s, const float* in, float* out, uint32_t n) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_expf, blocks_for(n), dim3(kBlock), 0, s, in, out, n);
+  return hipGetLastError();
+}
+hipError_t cs_launch_sincosf(hipStream_t s, const float* in, float* sn, float* cs, uint32_t n) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sincosf, blocks_for(n), dim3(kBlock), 0, s, in, sn, cs, n);
+  return hipGetLastError();
+}
+hipError_t cs_launch_emit(hipStream_t s, const cs_emit_frame& f, float* part, float* aff) {
+  if (f.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_emit, dim3((f.count + 255u) / 256u), dim3(256), 0, s, f, part, aff);
   return hipGetLastError();
 }
 hipError_t cs_launch_unet_features(hipStream_t s, const cs_params& u, const float* kf, const float* rhs,

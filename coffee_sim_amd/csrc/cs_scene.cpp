@@ -317,51 +317,38 @@ float flow_rate_from_speed(float v, float nozzle_radius, float cd, float vol_to_
 }
 
 // Golden-angle disk (inflow.rs:165-219).
-static void emit_disk(const cs_spout& sp, Inflow& in, uint32_t count, float dt, float speed, float jet_radius,
-                      float mass, std::vector<float>& out) {
+// The golden-angle disk frame of one substep's emission (inflow.rs:165-193):
+// tangents of the spout direction, the contracted emit origin and the velocity.
+static void disk_frame(const cs_spout& sp, Inflow& in, uint32_t count, float dt, float speed, float jet_radius,
+                       float mass, cs_emit_frame* f) {
   const float* dir = sp.direction;
   float up[3] = {0.0f, 1.0f, 0.0f};
   if (!(fabsf(dir[1]) < 0.9f)) {
     up[0] = 1.0f;
     up[1] = 0.0f;
   }
-  float t1[3], t2[3];
-  rs_cross(dir, up, t1);
-  rs_normalize(t1);
-  rs_cross(dir, t1, t2);
-  rs_normalize(t2);
-  const float vel[3] = {dir[0] * speed, dir[1] * speed, dir[2] * speed};
-  const float eo[3] = {sp.origin[0] + dir[0] * 0.18f, sp.origin[1] + dir[1] * 0.18f, sp.origin[2] + dir[2] * 0.18f};
-  const float golden = 2.399963f, radial_irr = 0.7548777f;
-  size_t base = out.size();
-  out.resize(base + (size_t)count * 8);
-  for (uint32_t i = 0; i < count; ++i) {
-    uint32_t sample = in.emission_sequence + i;
-    float sf = (float)sample;
-    float ru = (sf + 0.5f) * radial_irr;
-    ru = ru - truncf(ru);
-    float r = jet_radius * sqrtf(ru);
-    float theta = sf * golden;
-    float age = (((float)i + 0.5f) / (float)count) * dt;
-    float ct = cosf(theta), st = sinf(theta);
-    float* p = &out[base + (size_t)i * 8];
-    for (int c = 0; c < 3; ++c) {
-      float off = t1[c] * r * ct + t2[c] * r * st;
-      p[c] = eo[c] + off - vel[c] * age;
-    }
-    p[3] = 1.0f;
-    p[4] = vel[0];
-    p[5] = vel[1];
-    p[6] = vel[2];
-    p[7] = mass;
+  rs_cross(dir, up, f->t1);
+  rs_normalize(f->t1);
+  rs_cross(dir, f->t1, f->t2);
+  rs_normalize(f->t2);
+  for (int c = 0; c < 3; ++c) {
+    f->vel[c] = dir[c] * speed;
+    f->origin[c] = sp.origin[c] + dir[c] * 0.18f;
   }
+  f->jet_radius = jet_radius;
+  f->dt = dt;
+  f->mass = mass;
+  f->seq0 = in.emission_sequence;
+  f->count = count;
   in.emission_sequence += count;
 }
 
-uint32_t emit(const cs_settings& s, Inflow& in, float dt, uint32_t current, std::vector<float>& out,
-              uint32_t* dropped) {
+uint32_t emit_plan(const cs_settings& s, Inflow& in, float dt, uint32_t current, cs_emit_frame* f,
+                   uint32_t* dropped) {
   const float mass = 80.0f / 320.0f;  // MASS_UNITS_PER_ML / PARTICLES_PER_ML (mod.rs:278)
   uint32_t avail = s.max_particles > current ? s.max_particles - current : 0u;
+  memset(f, 0, sizeof(*f));
+  f->first = current;
   if (s.emitter_mode == CS_EMIT_FIXED_COUNT) {
     uint32_t want = s.emit_per_substep;
     if (s.emit_total) {
@@ -371,7 +358,7 @@ uint32_t emit(const cs_settings& s, Inflow& in, float dt, uint32_t current, std:
     uint32_t count = std::min(want, avail);
     if (dropped) *dropped += want - count;
     if (count == 0) return 0;
-    emit_disk(s.spout, in, count, dt, s.emit_speed, s.emit_jet_radius, mass, out);
+    disk_frame(s.spout, in, count, dt, s.emit_speed, s.emit_jet_radius, mass, f);
     in.emitted_total += count;
     return count;
   }
@@ -392,7 +379,17 @@ uint32_t emit(const cs_settings& s, Inflow& in, float dt, uint32_t current, std:
   uint32_t count = std::min(requested, avail);
   if (dropped) *dropped += requested - count;
   if (count == 0) return 0;
-  emit_disk(s.spout, in, count, dt, in.exit_speed, s.spout.nozzle_radius * 0.58f, mass, out);
+  disk_frame(s.spout, in, count, dt, in.exit_speed, s.spout.nozzle_radius * 0.58f, mass, f);
+  return count;
+}
+
+uint32_t emit(const cs_settings& s, Inflow& in, float dt, uint32_t current, std::vector<float>& out,
+              uint32_t* dropped) {
+  cs_emit_frame f;
+  const uint32_t count = emit_plan(s, in, dt, current, &f, dropped);
+  const size_t base = out.size();
+  out.resize(base + (size_t)count * 8);
+  for (uint32_t i = 0; i < count; ++i) cs_emit_particle(f, i, &out[base + (size_t)i * 8]);
   return count;
 }
 

@@ -73,8 +73,6 @@ void free_sim(cs_sim* m) {
     if (e) (void)hipEventDestroy(e);
   if (m->batch_exec) (void)hipGraphExecDestroy(m->batch_exec);
   if (m->batch_graph) (void)hipGraphDestroy(m->batch_graph);
-  if (m->h_stage) (void)hipHostFree(m->h_stage);
-  if (m->stage_done) (void)hipEventDestroy(m->stage_done);
   for (auto& e : m->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : m->kev)
@@ -485,22 +483,12 @@ int solve(cs_sim* m, const cs_params& u) { return m->dense_solver ? solve_dense(
 
 int substep(cs_sim* m, float sub_dt) {
   hipStream_t st = m->ctx->stream;
-  // 1. emission on the host (inflow.rs:122-241), appended after the live particles
-  m->emit_buf.clear();
-  uint32_t emitted = cs::emit(m->s, m->inflow, sub_dt, m->num_water, m->emit_buf, &m->total_dropped);
+  // 1. emission (inflow.rs:122-241): the flow model and the disk frame on the
+  // host, the particles on the device, appended after the live particles
+  cs_emit_frame ef;
+  const uint32_t emitted = cs::emit_plan(m->s, m->inflow, sub_dt, m->num_water, &ef, &m->total_dropped);
   if (emitted) {
-    size_t bytes = (size_t)emitted * 32;
-    CS_HIP(hipEventSynchronize(m->stage_done));
-    if (bytes > m->h_stage_cap) {
-      if (m->h_stage) CS_HIP(hipHostFree(m->h_stage));
-      m->h_stage = nullptr;
-      CS_HIP(hipHostMalloc(reinterpret_cast<void**>(&m->h_stage), bytes));
-      m->h_stage_cap = bytes;
-    }
-    memcpy(m->h_stage, m->emit_buf.data(), bytes);
-    CS_HIP(hipMemcpyAsync(m->part + (size_t)m->num_water * 8, m->h_stage, bytes, hipMemcpyHostToDevice, st));
-    CS_HIP(hipMemsetAsync(m->aff + (size_t)m->num_water * 12, 0, (size_t)emitted * 48, st));
-    CS_HIP(hipEventRecord(m->stage_done, st));
+    CS_HIP(cs_launch_emit(st, ef, m->part, m->aff));
     m->num_water += emitted;
     m->total_emitted_mass += (float)emitted * (80.0f / 320.0f);
   }
@@ -745,7 +733,6 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_res), sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_state), sizeof(cs_solve_state));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_done), 2 * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&m->stage_done, hipEventDisableTiming);
   for (auto& ev : m->ev)
     if (e == hipSuccess) e = hipEventCreate(&ev);
   for (auto& ev : m->kev)
@@ -775,7 +762,6 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (e == hipSuccess) e = hipMemsetAsync(m->kind, 0, m->nc, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->lanes, 0, m->nc * 24, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->metrics, 0, CS_METRICS_SLOTS * 4, st);
-  if (e == hipSuccess) e = hipEventRecord(m->stage_done, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     free_sim(m);
@@ -1063,6 +1049,21 @@ int cs_device_expf(cs_ctx* ctx, const float* in, float* out, uint32_t n) {
   (void)hipFree(din);
   if (dout) (void)hipFree(dout);
   if (e != hipSuccess) return hip_fail(e, "cs_device_expf");
+  return CS_OK;
+}
+int cs_device_sincosf(cs_ctx* ctx, const float* in, float* sn, float* cs, uint32_t n) {
+  if (!ctx || (n && (!in || !sn || !cs))) return fail(CS_ERR_INVALID, "null argument");
+  if (n == 0) return CS_OK;
+  CS_HIP(hipSetDevice(ctx->device));
+  float* d = nullptr;
+  CS_HIP(dalloc(&d, (size_t)n * 3));
+  hipError_t e = hipMemcpyAsync(d, in, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = cs_launch_sincosf(ctx->stream, d, d + n, d + 2 * (size_t)n, n);
+  if (e == hipSuccess) e = hipMemcpyAsync(sn, d + n, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(cs, d + 2 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "cs_device_sincosf");
   return CS_OK;
 }
 

@@ -74,9 +74,6 @@ struct cs_sim {
   cs_unet* warm_net = nullptr;       // network of CS_WARM_LEARNED
   // host state
   double* h_res = nullptr;    // pinned
-  float* h_stage = nullptr;   // pinned emission staging
-  size_t h_stage_cap = 0;
-  hipEvent_t stage_done = nullptr;
   std::vector<float> init_particles, init_affine;
   uint32_t init_count = 0;
   cs::Inflow inflow;
@@ -93,7 +90,6 @@ struct cs_sim {
   // per-kernel timing (cs_kernel_timing): [p2g start, end, g2p start, end, rbgs start, end]
   hipEvent_t kev[6] = {};
   cs_kernel_timing kt = {};
-  std::vector<float> emit_buf;
 };
 
 // Sets the message cs_last_error returns and returns `code`.

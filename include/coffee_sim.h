@@ -340,6 +340,10 @@ int cs_sim_reset_kernel_timing(cs_sim* sim);
 /* Verification entry point (no reference counterpart): evaluates the solver's
  * device exp (used by the J update, shader.rs:1853) on `n` inputs. */
 int cs_device_expf(cs_ctx* ctx, const float* in, float* out, uint32_t n);
+/* Verification entry point (no reference counterpart): the emitter's sin and cos
+ * (theta.sin() / theta.cos() of inflow.rs:205-207, evaluated as described at
+ * cs_sincosf in the kernels) on `n` inputs. */
+int cs_device_sincosf(cs_ctx* ctx, const float* in, float* sn, float* cs, uint32_t n);
 
 /* ---- learned warm start (no reference counterpart) ------------------------------
  * A 3-D U-Net (3 levels, 16/32/64 channels, 3x3x3 kernels, bf16 weights and

@@ -47,6 +47,7 @@ float orc_kozeny_carman_permeability(float grind_diameter_um, float porosity);
 
 /* math */
 float orc_expf(float x);
+void orc_sincosf(float x, float* sn, float* cs);
 float orc_smoothstep(float lo, float hi, float x);
 int32_t orc_f2i(float x);
 

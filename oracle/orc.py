@@ -52,6 +52,7 @@ def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
             "orc_spout_translate_origin_to": (None, [P(Spout), f32, f32, f32]),
             "orc_kozeny_carman_permeability": (f32, [f32, f32]),
             "orc_expf": (f32, [f32]),
+            "orc_sincosf": (None, [f32, f32p, f32p]),
             "orc_smoothstep": (f32, [f32, f32, f32]),
             "orc_f2i": (C.c_int32, [f32]),
             "orc_create": (C.c_void_p, [P(Settings), P(Scene)]),

@@ -317,7 +317,8 @@ static void emit_disk(orc_sim* S, uint32_t slot0, uint32_t count, float dt, floa
     float r = jet_radius * sqrtf(radial_u);
     float theta = sf * golden;
     float age = (((float)i + 0.5f) / (float)count) * dt;
-    float ct = cosf(theta), st = sinf(theta);
+    float ct, st;
+    orc_sincosf(theta, &st, &ct);
     float off[3];
     for (int c = 0; c < 3; ++c) off[c] = t1[c] * r * ct + t2[c] * r * st;
     float* p = &S->particles[(size_t)(slot0 + i) * 8];
@@ -595,6 +596,35 @@ float orc_expf(float x) {
   union { uint32_t u; float f; } sc;
   sc.u = (uint32_t)(k + 127) << 23;
   return p * sc.f;
+}
+/* ---- sin and cos for the emitter's golden angle (inflow.rs:205-207) ----------
+ * The reference calls f32::sin / f32::cos (a libm).  The oracle and the product
+ * share one evaluation instead, so host and device agree bit for bit: reduce
+ * x = k pi/2 + r in double with pi/2 split in three (22 + 22 significant bits and
+ * a double remainder, so k*c1 and k*c2 are exact for |k| < 2^31), sum the Taylor
+ * series of sin r and cos r (odd terms through r^17, even through r^18) by
+ * Horner's rule, pick the quadrant k mod 4 and round each once to float. */
+static const double kSinC[8] = {
+  0x1.952c77030ad4ap-49,  -0x1.ae7f3e733b81fp-41, 0x1.6124613a86d09p-33, -0x1.ae64567f544e4p-26,
+  0x1.71de3a556c734p-19,  -0x1.a01a01a01a01ap-13, 0x1.1111111111111p-7,  -0x1.5555555555555p-3};
+static const double kCosC[9] = {
+  -0x1.6827863b97d97p-53, 0x1.ae7f3e733b81fp-45,  -0x1.93974a8c07c9dp-37, 0x1.1eed8eff8d898p-29,
+  -0x1.27e4fb7789f5cp-22, 0x1.a01a01a01a01ap-16,  -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5, -0.5};
+void orc_sincosf(float x, float* sn, float* cs) {
+  const double xd = (double)x;
+  const double k = rint(xd * 0x1.45f306dc9c883p-1); /* nearest multiple of pi/2 */
+  double r = xd - k * 0x1.921fb8p+0;
+  r = r - k * -0x1.5dde98p-23;
+  r = r - k * 0x1.8469898cc5170p-48;
+  const double r2 = r * r;
+  double ps = kSinC[0], pc = kCosC[0];
+  for (int i = 1; i < 8; ++i) ps = ps * r2 + kSinC[i];
+  for (int i = 1; i < 9; ++i) pc = pc * r2 + kCosC[i];
+  const double sr = r + (r * r2) * ps, cr = 1.0 + r2 * pc;
+  const double sv[4] = {sr, cr, -sr, -cr}, cv[4] = {cr, -sr, -cr, sr};
+  const int q = (int)((long long)k & 3);
+  *sn = (float)sv[q];
+  *cs = (float)cv[q];
 }
 float orc_smoothstep(float lo, float hi, float x) { return o_smooth(lo, hi, x); }
 int32_t orc_f2i(float x) { return o_f2i(x); }

@@ -33,6 +33,26 @@ def test_device_exp_matches_oracle_exp(ctx, orc):
     assert_bitwise("expf", got, want)
 
 
+def test_device_sincos_matches_oracle_sincos(ctx, orc):
+    """The emitter's sin/cos (double-precision reduction and series on both
+    sides) on golden angles of up to 2^28 samples and spread arguments."""
+    import ctypes as C
+
+    rng = np.random.default_rng(11)
+    samples = np.concatenate([np.arange(0, 4096), rng.integers(0, 1 << 28, 20000)]).astype(np.uint32)
+    x = np.concatenate([samples.astype(np.float32) * np.float32(2.399963),
+                        rng.uniform(-10.0, 10.0, 4000).astype(np.float32)])
+    sn, cs_ = ctx.device_sincosf(x)
+    L = orc.lib()
+    a, b = C.c_float(), C.c_float()
+    want = np.empty((x.size, 2), np.float32)
+    for i, v in enumerate(x):
+        L.orc_sincosf(float(v), C.byref(a), C.byref(b))
+        want[i] = (a.value, b.value)
+    assert_bitwise("sin", sn, want[:, 0])
+    assert_bitwise("cos", cs_, want[:, 1])
+
+
 def test_free_stream_reference_scene_bitwise(ctx, orc):
     """benchmark_free_stream (mod.rs:145-152): the whole reference substep with
     its own SDF, class mask, emitter and 40 fixed RBGS pairs."""
