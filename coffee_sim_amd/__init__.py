@@ -82,6 +82,7 @@ def _bind(lib: C.CDLL) -> None:
         "cs_sim_read_grid_vel": (C.c_int, [C.c_void_p, P(C.c_float)]),
         "cs_sim_read_pressure": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
         "cs_sim_read_metrics": (C.c_int, [C.c_void_p, P(C.c_uint32)]),
+        "cs_sim_read_sdf": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_uint8)]),
         "cs_sim_pack_render": (C.c_int, [C.c_void_p, P(C.c_float)]),
         "cs_sim_get_solve_stats": (C.c_int, [C.c_void_p, P(SolveStats)]),
         "cs_sim_set_warm_start": (C.c_int, [C.c_void_p, P(C.c_float)]),
@@ -513,6 +514,15 @@ class MpmSim3D:
         k = np.empty(self.nc, np.int32)
         _check(load_library().cs_sim_read_pressure(self._h, _iptr(p), _iptr(k)), "read_pressure")
         return p, k
+
+    def read_sdf(self) -> tuple[np.ndarray, np.ndarray]:
+        """The SDF (sdf_res^3) and class mask (one byte per cell) the simulation holds."""
+        r = int(self.settings.sdf_res)
+        sdf = np.empty(r * r * r, np.float32)
+        cls = np.empty(self.nc, np.uint8)
+        _check(load_library().cs_sim_read_sdf(self._h, _fptr(sdf), cls.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "cs_sim_read_sdf")
+        return sdf, cls
 
     def read_metrics(self) -> np.ndarray:
         out = np.empty(abi.CS_METRICS_SLOTS, np.uint32)

@@ -45,6 +45,10 @@ hipError_t cs_launch_pack_render(hipStream_t stream, const cs_params& u, const f
                                  uint32_t num_bed, float* out);
 hipError_t cs_launch_expf(hipStream_t stream, const float* in, float* out, uint32_t n);
 hipError_t cs_launch_sincosf(hipStream_t stream, const float* in, float* sn, float* cs, uint32_t n);
+// The scene's SDF (sdf_res^3 floats) and class mask (one byte per cell) from the
+// settings' obstacles, as cs_generate_sdf / cs_generate_sdf_class make them on the host.
+hipError_t cs_launch_gen_sdf(hipStream_t stream, const cs_settings& s, float* out);
+hipError_t cs_launch_gen_sdf_class(hipStream_t stream, const cs_settings& s, const float* sdf, uint8_t* out);
 // The particles of one substep's emission (f.count of them at slots f.first..)
 // and their zero affine records.
 hipError_t cs_launch_emit(hipStream_t stream, const cs_emit_frame& f, float* part, float* aff);

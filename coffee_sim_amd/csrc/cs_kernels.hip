@@ -16,6 +16,7 @@
 
 #include "cs_batch.h"
 #include "cs_device.h"
+#include "cs_sdf.h"
 
 // ---------------------------------------------------------------------------------
 __global__ void k_fill_i32(int32_t* __restrict__ p, size_t n, int32_t v) {
@@ -627,6 +628,25 @@ __global__ void k_sincosf(const float* __restrict__ in, float* __restrict__ sn, 
   if (i < n) cs_sincosf(in[i], &sn[i], &cs[i]);
 }
 
+// ---- scene generation on the device (state.rs:206-344) ----------------------------------------
+// One thread per SDF voxel, then one per grid cell of the class mask; the
+// rules are cs_sdf.h's, shared with the host generator.
+__global__ void __launch_bounds__(256) k_gen_sdf(cs_settings s, float* __restrict__ out) {
+  const uint32_t n = s.sdf_res;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * n * n) return;
+  const uint32_t ix = (uint32_t)(idx % n), iy = (uint32_t)((idx / n) % n), iz = (uint32_t)(idx / ((size_t)n * n));
+  out[idx] = cs_sdf_voxel(s, ix, iy, iz);
+}
+__global__ void __launch_bounds__(256) k_gen_sdf_class(cs_settings s, const float* __restrict__ sdf,
+                                                       uint8_t* __restrict__ out) {
+  const uint32_t gx = s.grid_dims[0], gy = s.grid_dims[1], gz = s.grid_dims[2];
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)gx * gy * gz) return;
+  const uint32_t ix = (uint32_t)(idx % gx), iy = (uint32_t)((idx / gx) % gy), iz = (uint32_t)(idx / ((size_t)gx * gy));
+  out[idx] = cs_sdf_class_cell(s, sdf, ix, iy, iz);
+}
+
 // ---- inflow emitter on the device (inflow.rs:195-214; the frame and count come from the host) ----
 // One thread per emitted particle: its 32 B particle record from the golden-angle
 // disk and a zero affine record (48 B), at slot f.first + i.
@@ -858,6 +878,18 @@ hipError_t cs_launch_expf(hipStream_t s, const float* in, float* out, uint32_t n
 hipError_t cs_launch_sincosf(hipStream_t s, const float* in, float* sn, float* cs, uint32_t n) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_sincosf, blocks_for(n), dim3(kBlock), 0, s, in, sn, cs, n);
+  return hipGetLastError();
+}
+hipError_t cs_launch_gen_sdf(hipStream_t st, const cs_settings& s, float* out) {
+  const size_t n = (size_t)s.sdf_res * s.sdf_res * s.sdf_res;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gen_sdf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, out);
+  return hipGetLastError();
+}
+hipError_t cs_launch_gen_sdf_class(hipStream_t st, const cs_settings& s, const float* sdf, uint8_t* out) {
+  const size_t n = (size_t)s.grid_dims[0] * s.grid_dims[1] * s.grid_dims[2];
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gen_sdf_class, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, sdf, out);
   return hipGetLastError();
 }
 hipError_t cs_launch_emit(hipStream_t s, const cs_emit_frame& f, float* part, float* aff) {

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "cs_host.h"
+#include "cs_sdf.h"
 
 namespace cs {
 
@@ -195,96 +196,20 @@ int validate_settings(const cs_settings& s, std::string* why) {
   return CS_OK;
 }
 
-// ---- SDF generation (state.rs:206-276) -----------------------------------------------
-static float sdf_interior(const cs_obstacle& o, float px, float py, float pz) {
-  if (o.kind == CS_OBSTACLE_TRUNCATED_CONE) {
-    float local_y = py - o.center[1];
-    if (local_y > o.top_y || local_y < o.bot_y) return CS_SDF_NO_CONSTRAINT;
-    float height = o.top_y - o.bot_y;
-    if (height < 1e-10f) return CS_SDF_NO_CONSTRAINT;
-    float t = (local_y - o.bot_y) / height;
-    float cone_r = o.bot_radius + (o.top_radius - o.bot_radius) * t;
-    float dx = px - o.center[0], dz = pz - o.center[2];
-    return cone_r - sqrtf(dx * dx + dz * dz);
-  }
-  if (o.kind == CS_OBSTACLE_CYLINDER) {
-    float local_y = py - o.center[1];
-    if (local_y > o.top_y) return CS_SDF_NO_CONSTRAINT;
-    float dx = px - o.center[0], dz = pz - o.center[2];
-    float radial = o.radius - sqrtf(dx * dx + dz * dz);
-    float floor_d = local_y - o.bot_y;
-    return fminf(radial, floor_d);
-  }
-  return CS_SDF_NO_CONSTRAINT;
-}
-
+// ---- SDF generation (state.rs:206-344) -----------------------------------------------
+// The per-voxel and per-cell rules are in cs_sdf.h, shared with the device generator.
 void generate_sdf(const cs_settings& s, float* data) {
   const uint32_t n = s.sdf_res;
-  const size_t total = (size_t)n * n * n;
-  std::fill(data, data + total, CS_SDF_NO_CONSTRAINT);
-  const float bx = s.bounds_size[0], by = s.bounds_size[1], bz = s.bounds_size[2];
-  const float half = s.wall_thickness * 0.5f;
-  for (uint32_t oi = 0; oi < s.num_obstacles; ++oi) {
-    const cs_obstacle& o = s.obstacles[oi];
-    for (uint32_t iz = 0; iz < n; ++iz)
-      for (uint32_t iy = 0; iy < n; ++iy)
-        for (uint32_t ix = 0; ix < n; ++ix) {
-          float px = -bx * 0.5f + ((float)ix + 0.5f) * bx / (float)n;
-          float py = -by * 0.5f + ((float)iy + 0.5f) * by / (float)n;
-          float pz = -bz * 0.5f + ((float)iz + 0.5f) * bz / (float)n;
-          float sd = sdf_interior(o, px, py, pz) - half;
-          size_t idx = ((size_t)iz * n + iy) * n + ix;
-          if (data[idx] >= CS_SDF_NO_CONSTRAINT - 1.0f) data[idx] = sd;
-          else data[idx] = fmaxf(data[idx], sd);
-        }
-  }
+  for (uint32_t iz = 0; iz < n; ++iz)
+    for (uint32_t iy = 0; iy < n; ++iy)
+      for (uint32_t ix = 0; ix < n; ++ix) data[((size_t)iz * n + iy) * n + ix] = cs_sdf_voxel(s, ix, iy, iz);
 }
 
-static float texel(const float* d, uint32_t res, int x, int y, int z) {
-  int mx = (int)res - 1;
-  x = std::clamp(x, 0, mx);
-  y = std::clamp(y, 0, mx);
-  z = std::clamp(z, 0, mx);
-  return d[((size_t)z * res + y) * res + x];
-}
-
-// sample_sdf_from_data (state.rs:287-320): Rust's fract (x - trunc x) and the
-// lerp form c + (d - c) * f.
-static float sample_sdf_cpu(const cs_settings& s, const float* d, float px, float py, float pz) {
-  const float res = (float)s.sdf_res;
-  const float bx = s.bounds_size[0], by = s.bounds_size[1], bz = s.bounds_size[2];
-  float ux = (px + bx * 0.5f) / bx * res - 0.5f;
-  float uy = (py + by * 0.5f) / by * res - 0.5f;
-  float uz = (pz + bz * 0.5f) / bz * res - 0.5f;
-  int b0 = cs_f2i(floorf(ux)), b1 = cs_f2i(floorf(uy)), b2 = cs_f2i(floorf(uz));
-  float fx = ux - truncf(ux), fy = uy - truncf(uy), fz = uz - truncf(uz);
-  const uint32_t r = s.sdf_res;
-  float c000 = texel(d, r, b0, b1, b2), c100 = texel(d, r, b0 + 1, b1, b2);
-  float c010 = texel(d, r, b0, b1 + 1, b2), c110 = texel(d, r, b0 + 1, b1 + 1, b2);
-  float c001 = texel(d, r, b0, b1, b2 + 1), c101 = texel(d, r, b0 + 1, b1, b2 + 1);
-  float c011 = texel(d, r, b0, b1 + 1, b2 + 1), c111 = texel(d, r, b0 + 1, b1 + 1, b2 + 1);
-  float c00 = c000 + (c100 - c000) * fx;
-  float c10 = c010 + (c110 - c010) * fx;
-  float c01 = c001 + (c101 - c001) * fx;
-  float c11 = c011 + (c111 - c011) * fx;
-  float c0 = c00 + (c10 - c00) * fy;
-  float c1 = c01 + (c11 - c01) * fy;
-  return c0 + (c1 - c0) * fz;
-}
-
-// generate_sdf_class_data (state.rs:322-344)
 void generate_sdf_class(const cs_settings& s, const float* sdf, uint8_t* out) {
   const uint32_t gx = s.grid_dims[0], gy = s.grid_dims[1], gz = s.grid_dims[2];
-  const float dx = s.bounds_size[0] / (float)gx;
-  const float ox = -s.bounds_size[0] * 0.5f, oy = -s.bounds_size[1] * 0.5f, oz = -s.bounds_size[2] * 0.5f;
   for (uint32_t iz = 0; iz < gz; ++iz)
     for (uint32_t iy = 0; iy < gy; ++iy)
-      for (uint32_t ix = 0; ix < gx; ++ix) {
-        float cx = ox + ((float)ix + 0.5f) * dx;
-        float cy = oy + ((float)iy + 0.5f) * dx;
-        float cz = oz + ((float)iz + 0.5f) * dx;
-        out[((size_t)iz * gy + iy) * gx + ix] = sample_sdf_cpu(s, sdf, cx, cy, cz) < 0.0f ? 1u : 0u;
-      }
+      for (uint32_t ix = 0; ix < gx; ++ix) out[((size_t)iz * gy + iy) * gx + ix] = cs_sdf_class_cell(s, sdf, ix, iy, iz);
 }
 
 // ---- inflow model (inflow.rs:244-293) -----------------------------------------------

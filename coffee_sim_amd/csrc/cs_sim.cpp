@@ -627,21 +627,10 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   m->gz = s->grid_dims[2];
   m->nc = (size_t)m->gx * m->gy * m->gz;
   const size_t nsdf = (size_t)s->sdf_res * s->sdf_res * s->sdf_res;
-  // scene arrays on the host first (state.rs:178-182)
-  std::vector<float> sdf_h;
+  // scene arrays (state.rs:178-182): uploaded when the scene supplies them,
+  // otherwise generated on the device from the settings
   const float* sdf_src = scene ? scene->sdf : nullptr;
-  if (!sdf_src) {
-    sdf_h.resize(nsdf);
-    cs::generate_sdf(*s, sdf_h.data());
-    sdf_src = sdf_h.data();
-  }
-  std::vector<uint8_t> cls_h;
   const uint8_t* cls_src = scene ? scene->sdf_class : nullptr;
-  if (!cls_src) {
-    cls_h.resize(m->nc);
-    cs::generate_sdf_class(*s, sdf_src, cls_h.data());
-    cls_src = cls_h.data();
-  }
   std::vector<int32_t> bed_index;
   std::vector<cs_bed_cell> bed_table;
   if (s->has_bed_field && scene && scene->bed_lookup && scene->bed_records) {
@@ -750,8 +739,12 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   cs_build_sweep_table(sweep_table.data());
   e = hipMemcpyAsync(m->sweep_table, sweep_table.data(), sweep_table.size() * sizeof(uint32_t),
                      hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(m->sdf, sdf_src, nsdf * sizeof(float), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(m->cls, cls_src, m->nc, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = sdf_src ? hipMemcpyAsync(m->sdf, sdf_src, nsdf * sizeof(float), hipMemcpyHostToDevice, st)
+                : cs_launch_gen_sdf(st, *s, m->sdf);
+  if (e == hipSuccess)
+    e = cls_src ? hipMemcpyAsync(m->cls, cls_src, m->nc, hipMemcpyHostToDevice, st)
+                : cs_launch_gen_sdf_class(st, *s, m->sdf, m->cls);
   if (e == hipSuccess) e = hipMemcpyAsync(m->bed_id, bed_index.data(), m->nc * sizeof(int32_t), hipMemcpyHostToDevice, st);
   if (e == hipSuccess && !bed_table.empty())
     e = hipMemcpyAsync(m->bed, bed_table.data(), bed_table.size() * sizeof(cs_bed_cell), hipMemcpyHostToDevice, st);
@@ -907,6 +900,13 @@ int cs_sim_read_pressure(cs_sim* m, int32_t* p, int32_t* kind) {
   if ((rc = read_dev(m, k.data(), m->kind, m->nc))) return rc;
   for (size_t i = 0; i < m->nc; ++i) kind[i] = k[i];
   return CS_OK;
+}
+int cs_sim_read_sdf(cs_sim* m, float* sdf, uint8_t* sdf_class) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  int rc = CS_OK;
+  if (sdf) rc = read_dev(m, sdf, m->sdf, (size_t)m->s.sdf_res * m->s.sdf_res * m->s.sdf_res * sizeof(float));
+  if (rc == CS_OK && sdf_class) rc = read_dev(m, sdf_class, m->cls, m->nc);
+  return rc;
 }
 int cs_sim_read_metrics(cs_sim* m, uint32_t* slots) {
   if (!m) return fail(CS_ERR_INVALID, "null sim");

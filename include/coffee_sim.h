@@ -296,6 +296,11 @@ int cs_sim_read_p2g_lanes(cs_sim* sim, int32_t* lanes6);
 int cs_sim_read_grid_vel(cs_sim* sim, float* vel4);
 /* Pressure (i32 fixed point, 2^-18) and cell kind (0 air .. 4 solid) of the most recent solve. */
 int cs_sim_read_pressure(cs_sim* sim, int32_t* pressure_fp, int32_t* kind);
+/* The scene's SDF (sdf_res^3 floats, x fastest) and solid-class mask (one byte per
+ * cell) as the simulation holds them: the scene's arrays, or the ones generated
+ * on the device from the settings when the scene gave none (generate_sdf_data
+ * and generate_sdf_class_data, state.rs:206-344).  Either pointer may be NULL. */
+int cs_sim_read_sdf(cs_sim* sim, float* sdf, uint8_t* sdf_class);
 /* Metric slots (state.rs:22-28). */
 int cs_sim_read_metrics(cs_sim* sim, uint32_t* slots8);
 /* prepare_render (shader.rs:2242-2276, run by step_frame at mod.rs:410-412 and

@@ -53,6 +53,28 @@ def test_device_sincos_matches_oracle_sincos(ctx, orc):
     assert_bitwise("cos", cs_, want[:, 1])
 
 
+@pytest.mark.parametrize("name", ["default_v60", "benchmark_free_stream", "benchmark_center_pour", "pour96"])
+def test_device_scene_generation_matches_oracle(ctx, orc, name):
+    """A simulation created without scene arrays generates its SDF and class
+    mask on the device (generate_sdf_data / generate_sdf_class_data,
+    state.rs:206-344): equal bit for bit to the oracle's and to the library's
+    host generator, for the presets and a 96^3 pour grid with a 96^3 field."""
+    if name == "pour96":
+        w = cs.SeededWorkload.pour(96, 20000, 200, 40.0, seed=5)
+        s = w.settings.copy()
+        s.sdf_res = 96
+    else:
+        s = cs.preset(name)
+    sim = cs.MpmSim3D(ctx, s)
+    sdf, cls = sim.read_sdf()
+    sim.close()
+    want_sdf = orc.generate_sdf(s)
+    assert_bitwise("sdf", sdf, want_sdf)
+    assert_bitwise("sdf host", sdf, cs.generate_sdf(s))
+    assert_bitwise("class", cls, orc.generate_sdf_class(s, want_sdf))
+    assert 0 < int(cls.sum()) < cls.size
+
+
 def test_free_stream_reference_scene_bitwise(ctx, orc):
     """benchmark_free_stream (mod.rs:145-152): the whole reference substep with
     its own SDF, class mask, emitter and 40 fixed RBGS pairs."""
