@@ -290,7 +290,7 @@ int cs_unet_create(cs_ctx* ctx, uint32_t n, uint64_t seed, cs_unet** out) {
   const size_t v0 = vox(u, 0), v1 = vox(u, 1), v2 = vox(u, 2);
   size_t wmax = 0;
   for (int k = 0; k < 10; ++k)
-    wmax = std::max(wmax, (size_t)cs_unet_wgrad_slots(side(u, kConvs[k].level)) * 27 * kConvs[k].cin * kConvs[k].cout);
+    wmax = std::max(wmax, cs_unet_wgrad_floats(side(u, kConvs[k].level), kConvs[k].cin, kConvs[k].cout));
   hipError_t e = hipSuccess;
 #define UA(p, cnt) \
   if (e == hipSuccess) e = dalloc(&(p), (cnt));

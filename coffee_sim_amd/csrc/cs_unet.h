@@ -15,8 +15,15 @@ hipError_t cs_unet_launch_conv(hipStream_t st, const uint16_t* in, int cin, int 
                                const float* bias, const uint16_t* mask, uint16_t* out, int cout, int relu);
 // Weight and bias gradient of a convolution whose ReLU output is `outv`, given the
 // gradient `gout` of that output: dw [27][cin][cout], db [cout] (fp32).
-// partial: cs_unet_wgrad_slots(s) * 27 * cin * cout floats; bias_part: CS_UNET_BIAS_ROWS * cout.
-int cs_unet_wgrad_slots(int s);
+// partial: cs_unet_wgrad_floats(s, cin, cout) floats; bias_part: CS_UNET_BIAS_ROWS * cout.
+#define CS_UNET_WGRAD_TARGET 512     // weight-gradient workgroups per launch
+#define CS_UNET_WGRAD_PER_GROUP 16   // slots summed per thread in the first reduction pass
+int cs_unet_wgrad_slots(int s, int cin, int cout);
+// Per-slot partials plus the first reduction pass's group sums.
+inline size_t cs_unet_wgrad_floats(int s, int cin, int cout) {
+  const int slots = cs_unet_wgrad_slots(s, cin, cout);
+  return (size_t)(slots + (slots + CS_UNET_WGRAD_PER_GROUP - 1) / CS_UNET_WGRAD_PER_GROUP) * 27 * cin * cout;
+}
 hipError_t cs_unet_launch_wgrad(hipStream_t st, const uint16_t* in, int cin, const uint16_t* gout,
                                 const uint16_t* outv, int cout, int s, float* partial, float* bias_part, float* dw,
                                 float* db);

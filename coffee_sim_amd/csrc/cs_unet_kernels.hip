@@ -127,93 +127,135 @@ __global__ void __launch_bounds__(256) k_unet_conv(const uint16_t* __restrict__ 
 
 // ---- weight gradient ----------------------------------------------------------------
 // dW[tap][ci][co] = sum_v in[v + tap][ci] * gz[v][co], gz = gout masked by the
-// layer output's ReLU derivative.  Grid (slot, dz, chunk): a workgroup sums the
-// 9 taps of one dz for one 16-channel input chunk and one 16-channel output
-// chunk over its run of bricks and writes fp32 partials[slot][tap][ci][co]; a
-// fixed-order reduction over slots follows.  Rows of the MFMA are input
-// channels, columns output channels, K runs over 32 voxels of the brick.
-__global__ void __launch_bounds__(256) k_unet_wgrad(const uint16_t* __restrict__ in, int cin,
-                                                    const uint16_t* __restrict__ gout,
-                                                    const uint16_t* __restrict__ outv, int cout, int s,
-                                                    int bricks_per_slot, float* __restrict__ partial) {
-  __shared__ uint16_t s_in[HV * 16];
-  __shared__ uint16_t s_g[512 * 16];
+// layer output's ReLU derivative.  Grid (slot, chunk pair): a workgroup of
+// three wavefronts sums all 27 taps (wavefront w owns dz = w - 1) for one
+// 16-channel input chunk and one 16-channel output chunk over its run of
+// bricks, and writes fp32 partials[slot][tap][ci][co]; a fixed-order
+// reduction over slots follows.  Rows of the MFMA are input channels, columns
+// output channels, K runs over 32 voxels (four x-rows of a brick).
+//
+// LDS holds both operands channel-major with x fastest, so a lane's eight K
+// values (eight consecutive x) are one 16-byte read: the input halo as
+// [ci][z 10][y 10][x 16] (x 0..9 used), the masked gradient as [co][512
+// voxels].  The three x-shifted input fragments of a (dz, dy) row come from
+// one 16-byte and one 4-byte read by byte alignment.  Row strides are padded
+// by 16 bytes so the 16 lanes of a read hit distinct banks.
+namespace {
+constexpr int WG_XS = 16;                              // x slots per halo row
+constexpr int WG_CSTR = HALO * HALO * WG_XS + 8;       // u16 per input channel (3216 bytes)
+constexpr int WG_GSTR = 512 + 8;                       // u16 per output channel (1040 bytes)
+constexpr int WG_THREADS = 192;
+}  // namespace
+
+__global__ void __launch_bounds__(WG_THREADS) k_unet_wgrad(const uint16_t* __restrict__ in, int cin,
+                                                           const uint16_t* __restrict__ gout,
+                                                           const uint16_t* __restrict__ outv, int cout, int s,
+                                                           int bricks_per_slot, float* __restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) uint16_t s_in[16 * WG_CSTR];
+  __shared__ __attribute__((aligned(16))) uint16_t s_g[16 * WG_GSTR];
   const int slot = (int)blockIdx.x;
-  const int dzg = (int)blockIdx.y;  // dz = dzg - 1
   const int nco = cout / 16;
-  const int cic = (int)blockIdx.z / nco, coc = (int)blockIdx.z % nco;
+  const int cic = (int)blockIdx.y / nco, coc = (int)blockIdx.y % nco;
   const int nb = s / 8, nbricks = nb * nb * nb;
-  const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & 63, dzi = tid >> 6;  // wavefront = dz + 1
   const int g = lane >> 4, r16 = lane & 15;
-  f32x4 acc[3];
+  f32x4 acc[9];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < 9; ++k) acc[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   const int b0 = slot * bricks_per_slot, b1 = min(b0 + bricks_per_slot, nbricks);
   for (int bid = b0; bid < b1; ++bid) {
     const int bx = bid % nb, by = (bid / nb) % nb, bz = bid / (nb * nb);
     __syncthreads();
-    for (int e = (int)threadIdx.x; e < HV * 2; e += 256) {
-      const int hv = e >> 1, h = e & 1;
-      const int gx = bx * 8 - 1 + hv % HALO, gy = by * 8 - 1 + (hv / HALO) % HALO, gz = bz * 8 - 1 + hv / (HALO * HALO);
-      uint4 val = make_uint4(0u, 0u, 0u, 0u);
-      if ((unsigned)gx < (unsigned)s && (unsigned)gy < (unsigned)s && (unsigned)gz < (unsigned)s)
-        val = *reinterpret_cast<const uint4*>(in + (((size_t)gz * s + gy) * s + gx) * (size_t)cin + cic * 16 + 8 * h);
-      reinterpret_cast<uint4*>(s_in)[e] = val;
+    // input halo: pairs of x-adjacent voxels, 8 channels per task, one dword per channel
+    for (int e = tid; e < HALO * HALO * (HALO / 2) * 2; e += WG_THREADS) {
+      const int h = e & 1, pr = e >> 1;
+      const int hx = 2 * (pr % (HALO / 2)), hy = (pr / (HALO / 2)) % HALO, hz = pr / (HALO * HALO / 2);
+      const int gx = bx * 8 - 1 + hx, gy = by * 8 - 1 + hy, gz = bz * 8 - 1 + hz;
+      uint4 a = make_uint4(0u, 0u, 0u, 0u), c = a;
+      if ((unsigned)gy < (unsigned)s && (unsigned)gz < (unsigned)s) {
+        const size_t row = ((size_t)gz * s + gy) * s;
+        if ((unsigned)gx < (unsigned)s)
+          a = *reinterpret_cast<const uint4*>(in + (row + gx) * (size_t)cin + cic * 16 + 8 * h);
+        if ((unsigned)(gx + 1) < (unsigned)s)
+          c = *reinterpret_cast<const uint4*>(in + (row + gx + 1) * (size_t)cin + cic * 16 + 8 * h);
+      }
+      const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, cw[4] = {c.x, c.y, c.z, c.w};
+      uint16_t* dst = s_in + (8 * h) * WG_CSTR + (hz * HALO + hy) * WG_XS + hx;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t lo = (aw[j >> 1] >> (16 * (j & 1))) & 0xffffu, hi = (cw[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+        *reinterpret_cast<uint32_t*>(dst + j * WG_CSTR) = lo | (hi << 16);
+      }
     }
-    for (int e = (int)threadIdx.x; e < 512 * 2; e += 256) {
-      const int vb = e >> 1, h = e & 1;
-      const int gx = bx * 8 + (vb & 7), gy = by * 8 + ((vb >> 3) & 7), gz = bz * 8 + (vb >> 6);
+    // masked output gradient of the brick, the same way
+    for (int e = tid; e < 256 * 2; e += WG_THREADS) {
+      const int h = e & 1, v = 2 * (e >> 1);
+      const int gx = bx * 8 + (v & 7), gy = by * 8 + ((v >> 3) & 7), gz = bz * 8 + (v >> 6);
       const size_t off = (((size_t)gz * s + gy) * s + gx) * (size_t)cout + coc * 16 + 8 * h;
-      reinterpret_cast<uint4*>(s_g)[e] =
-          relu_mask(*reinterpret_cast<const uint4*>(gout + off), *reinterpret_cast<const uint4*>(outv + off));
+      const uint4 a = relu_mask(*reinterpret_cast<const uint4*>(gout + off), *reinterpret_cast<const uint4*>(outv + off));
+      const uint4 c = relu_mask(*reinterpret_cast<const uint4*>(gout + off + cout),
+                                *reinterpret_cast<const uint4*>(outv + off + cout));
+      const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, cw[4] = {c.x, c.y, c.z, c.w};
+      uint16_t* dst = s_g + (8 * h) * WG_GSTR + v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t lo = (aw[j >> 1] >> (16 * (j & 1))) & 0xffffu, hi = (cw[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+        *reinterpret_cast<uint32_t*>(dst + j * WG_GSTR) = lo | (hi << 16);
+      }
     }
     __syncthreads();
+#pragma unroll 2
     for (int ks = 0; ks < 16; ++ks) {
-      // K = voxels ks*32 + 8g + j of the brick
-      uint32_t bw[4];
+      const int row = ks * 4 + g;  // x-row of the brick: voxels 8 row .. 8 row + 7
+      const int vy = row & 7, vz = row >> 3;
+      const bf16x8 b = as_bf16x8(*reinterpret_cast<const uint4*>(s_g + r16 * WG_GSTR + row * 8));
 #pragma unroll
-      for (int j = 0; j < 8; j += 2) {
-        const int v0 = ks * 32 + 8 * g + j;
-        bw[j >> 1] = (uint32_t)s_g[v0 * 16 + r16] | ((uint32_t)s_g[(v0 + 1) * 16 + r16] << 16);
-      }
-      const bf16x8 b = as_bf16x8(make_uint4(bw[0], bw[1], bw[2], bw[3]));
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int t9 = wv + 4 * k;  // this wavefront's taps of the dz slice
-        if (t9 >= 9) break;
-        const int dy = t9 / 3 - 1, dx = t9 % 3 - 1, dz = dzg - 1;
-        uint32_t aw[4];
-#pragma unroll
-        for (int j = 0; j < 8; j += 2) {
-          const int v0 = ks * 32 + 8 * g + j, v1 = v0 + 1;
-          const int h0 = (((v0 >> 6) + dz + 1) * HALO + ((v0 >> 3) & 7) + dy + 1) * HALO + (v0 & 7) + dx + 1;
-          const int h1 = (((v1 >> 6) + dz + 1) * HALO + ((v1 >> 3) & 7) + dy + 1) * HALO + (v1 & 7) + dx + 1;
-          aw[j >> 1] = (uint32_t)s_in[h0 * 16 + r16] | ((uint32_t)s_in[h1 * 16 + r16] << 16);
-        }
-        const bf16x8 a = as_bf16x8(make_uint4(aw[0], aw[1], aw[2], aw[3]));
-        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[k], 0, 0, 0);
+      for (int dyi = 0; dyi < 3; ++dyi) {
+        const uint16_t* src = s_in + r16 * WG_CSTR + ((vz + dzi) * HALO + vy + dyi) * WG_XS;
+        const uint4 w03 = *reinterpret_cast<const uint4*>(src);
+        const uint32_t w4 = *reinterpret_cast<const uint32_t*>(src + 8);
+        // x shifts -1, 0, +1: halo x = j, j + 1, j + 2 for the eight voxels j
+        const uint4 a1 = make_uint4(__builtin_amdgcn_alignbyte(w03.y, w03.x, 2u),
+                                    __builtin_amdgcn_alignbyte(w03.z, w03.y, 2u),
+                                    __builtin_amdgcn_alignbyte(w03.w, w03.z, 2u),
+                                    __builtin_amdgcn_alignbyte(w4, w03.w, 2u));
+        const uint4 a2 = make_uint4(w03.y, w03.z, w03.w, w4);
+        acc[dyi * 3 + 0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(w03), b, acc[dyi * 3 + 0], 0, 0, 0);
+        acc[dyi * 3 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a1), b, acc[dyi * 3 + 1], 0, 0, 0);
+        acc[dyi * 3 + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a2), b, acc[dyi * 3 + 2], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int t9 = wv + 4 * k;
-    if (t9 >= 9) break;
-    const int tap = dzg * 9 + t9;
+  for (int t = 0; t < 9; ++t) {
+    const int tap = dzi * 9 + t;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ci = cic * 16 + 4 * g + i, co = coc * 16 + r16;
-      partial[(((size_t)slot * 27 + tap) * cin + ci) * cout + co] = acc[k][i];
+      partial[(((size_t)slot * 27 + tap) * cin + ci) * cout + co] = acc[t][i];
     }
   }
 }
 
-// Fixed-order sum over slots: out[e] = sum_slot partial[slot][e].
-__global__ void k_unet_reduce_slots(const float* __restrict__ partial, int slots, size_t n, float* __restrict__ out) {
+// Fixed-order sum over slots, in two passes so that enough loads are in
+// flight: groups[g][e] = sum of slots g*per .. g*per + per - 1 of partial[.][e],
+// then out[e] = sum_g groups[g][e].
+__global__ void __launch_bounds__(256) k_unet_reduce_slots(const float* __restrict__ partial, int slots, int per,
+                                                           size_t n, float* __restrict__ groups) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int k0 = (int)blockIdx.y * per, k1 = min(k0 + per, slots);
+  float acc = 0.0f;
+  for (int k = k0; k < k1; ++k) acc += partial[(size_t)k * n + e];
+  groups[(size_t)blockIdx.y * n + e] = acc;
+}
+__global__ void __launch_bounds__(256) k_unet_reduce_groups(const float* __restrict__ groups, int ngroups, size_t n,
+                                                            float* __restrict__ out) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   float acc = 0.0f;
-  for (int k = 0; k < slots; ++k) acc += partial[(size_t)k * n + e];
+  for (int g = 0; g < ngroups; ++g) acc += groups[(size_t)g * n + e];
   out[e] = acc;
 }
 
@@ -266,72 +308,106 @@ __global__ void __launch_bounds__(256) k_unet_bias_grad(const uint16_t* __restri
 }
 
 // ---- pooling, upsampling, concatenation -------------------------------------------------
+// One thread per voxel and 8-channel group (channel counts, strides and
+// offsets are multiples of 8): 16-byte loads and stores, the per-channel
+// arithmetic in the same order as one channel at a time.
+namespace {
+__device__ __forceinline__ void unpack8(uint4 v, float* f) {
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    f[2 * j] = bf2f((uint16_t)(w[j] & 0xffffu));
+    f[2 * j + 1] = bf2f((uint16_t)(w[j] >> 16));
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float* f) {
+  uint32_t w[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) w[j] = (uint32_t)f2bf(f[2 * j]) | ((uint32_t)f2bf(f[2 * j + 1]) << 16);
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ __forceinline__ uint4 ld8(const uint16_t* p) { return *reinterpret_cast<const uint4*>(p); }
+__device__ __forceinline__ void st8(uint16_t* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
+}  // namespace
+
 // 2x average pool, [s^3][c] -> [(s/2)^3][c].
-__global__ void k_unet_pool(const uint16_t* __restrict__ in, int c, int s, uint16_t* __restrict__ out) {
-  const int h = s / 2;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)h * h * h * c) return;
-  const int ch = (int)(e % c);
-  const size_t vo = e / c;
-  const int x = (int)(vo % h), y = (int)((vo / h) % h), z = (int)(vo / ((size_t)h * h));
-  float acc = 0.0f;
-  for (int dz = 0; dz < 2; ++dz)
-    for (int dy = 0; dy < 2; ++dy)
-      for (int dx = 0; dx < 2; ++dx)
-        acc += bf2f(in[((((size_t)(2 * z + dz)) * s + 2 * y + dy) * s + 2 * x + dx) * c + ch]);
-  out[e] = f2bf(acc * 0.125f);
+__global__ void __launch_bounds__(256) k_unet_pool(const uint16_t* __restrict__ in, int c, int s,
+                                                   uint16_t* __restrict__ out) {
+  const uint32_t h = (uint32_t)s / 2, cq = (uint32_t)c / 8;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= h * h * h * cq) return;
+  const uint32_t q = e % cq, vo = e / cq;
+  const uint32_t x = vo % h, y = (vo / h) % h, z = vo / (h * h);
+  float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, f[8];
+  for (uint32_t dz = 0; dz < 2; ++dz)
+    for (uint32_t dy = 0; dy < 2; ++dy)
+      for (uint32_t dx = 0; dx < 2; ++dx) {
+        unpack8(ld8(in + ((((size_t)(2 * z + dz)) * s + 2 * y + dy) * s + 2 * x + dx) * c + 8 * q), f);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] += f[j];
+      }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] *= 0.125f;
+  st8(out + (size_t)vo * c + 8 * q, pack8(acc));
 }
 
 // Gradient of the pool plus a second contribution read from a channel window of
 // another tensor: g_in[v][ch] = add[v][add_off + ch] + g_out[parent(v)][ch] / 8.
-__global__ void k_unet_pool_back(const uint16_t* __restrict__ g_out, int c, int s, const uint16_t* __restrict__ add,
-                                 int add_stride, int add_off, uint16_t* __restrict__ g_in) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)s * s * s * c) return;
-  const int ch = (int)(e % c);
-  const size_t v = e / c;
-  const int x = (int)(v % s), y = (int)((v / s) % s), z = (int)(v / ((size_t)s * s));
-  const int h = s / 2;
-  const float gp = bf2f(g_out[((((size_t)(z / 2)) * h + y / 2) * h + x / 2) * c + ch]) * 0.125f;
-  const float a = add ? bf2f(add[v * add_stride + add_off + ch]) : 0.0f;
-  g_in[e] = f2bf(a + gp);
+__global__ void __launch_bounds__(256) k_unet_pool_back(const uint16_t* __restrict__ g_out, int c, int s,
+                                                        const uint16_t* __restrict__ add, int add_stride, int add_off,
+                                                        uint16_t* __restrict__ g_in) {
+  const uint32_t cq = (uint32_t)c / 8, us = (uint32_t)s, h = us / 2;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= us * us * us * cq) return;
+  const uint32_t q = e % cq, v = e / cq;
+  const uint32_t x = v % us, y = (v / us) % us, z = v / (us * us);
+  float gp[8], a[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  unpack8(ld8(g_out + ((((size_t)(z / 2)) * h + y / 2) * h + x / 2) * c + 8 * q), gp);
+  if (add) unpack8(ld8(add + (size_t)v * add_stride + add_off + 8 * q), a);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) a[j] = a[j] + gp[j] * 0.125f;
+  st8(g_in + (size_t)v * c + 8 * q, pack8(a));
 }
 
 // Nearest 2x upsample into channels [off, off + c) of a concatenation with `stride` channels.
-__global__ void k_unet_up(const uint16_t* __restrict__ in, int c, int s, uint16_t* __restrict__ cat, int stride,
-                          int off) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int S = 2 * s;
-  if (e >= (size_t)S * S * S * c) return;
-  const int ch = (int)(e % c);
-  const size_t v = e / c;
-  const int x = (int)(v % S), y = (int)((v / S) % S), z = (int)(v / ((size_t)S * S));
-  cat[v * stride + off + ch] = in[((((size_t)(z / 2)) * s + y / 2) * s + x / 2) * c + ch];
+__global__ void __launch_bounds__(256) k_unet_up(const uint16_t* __restrict__ in, int c, int s,
+                                                 uint16_t* __restrict__ cat, int stride, int off) {
+  const uint32_t cq = (uint32_t)c / 8, S = 2u * (uint32_t)s;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= S * S * S * cq) return;
+  const uint32_t q = e % cq, v = e / cq;
+  const uint32_t x = v % S, y = (v / S) % S, z = v / (S * S);
+  st8(cat + (size_t)v * stride + off + 8 * q, ld8(in + ((((size_t)(z / 2)) * s + y / 2) * s + x / 2) * c + 8 * q));
 }
 
 // Gradient of the upsample: g_in[v'][ch] = sum of the 8 children's cat[child][off + ch].
-__global__ void k_unet_up_back(const uint16_t* __restrict__ g_cat, int stride, int off, int c, int s,
-                               uint16_t* __restrict__ g_in) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)s * s * s * c) return;
-  const int ch = (int)(e % c);
-  const size_t vo = e / c;
-  const int x = (int)(vo % s), y = (int)((vo / s) % s), z = (int)(vo / ((size_t)s * s));
-  const int S = 2 * s;
-  float acc = 0.0f;
-  for (int dz = 0; dz < 2; ++dz)
-    for (int dy = 0; dy < 2; ++dy)
-      for (int dx = 0; dx < 2; ++dx)
-        acc += bf2f(g_cat[((((size_t)(2 * z + dz)) * S + 2 * y + dy) * S + 2 * x + dx) * stride + off + ch]);
-  g_in[e] = f2bf(acc);
+__global__ void __launch_bounds__(256) k_unet_up_back(const uint16_t* __restrict__ g_cat, int stride, int off, int c,
+                                                      int s, uint16_t* __restrict__ g_in) {
+  const uint32_t cq = (uint32_t)c / 8, us = (uint32_t)s, S = 2u * us;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= us * us * us * cq) return;
+  const uint32_t q = e % cq, vo = e / cq;
+  const uint32_t x = vo % us, y = (vo / us) % us, z = vo / (us * us);
+  float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, f[8];
+  for (uint32_t dz = 0; dz < 2; ++dz)
+    for (uint32_t dy = 0; dy < 2; ++dy)
+      for (uint32_t dx = 0; dx < 2; ++dx) {
+        unpack8(ld8(g_cat + ((((size_t)(2 * z + dz)) * S + 2 * y + dy) * S + 2 * x + dx) * stride + off + 8 * q), f);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] += f[j];
+      }
+  st8(g_in + (size_t)vo * c + 8 * q, pack8(acc));
 }
 
 // Copies a [v][c] tensor into channels [off, off + c) of a concatenation.
-__global__ void k_unet_put(const uint16_t* __restrict__ in, int c, size_t nv, uint16_t* __restrict__ cat, int stride,
-                           int off) {
+__global__ void __launch_bounds__(256) k_unet_put(const uint16_t* __restrict__ in, int c, size_t nv,
+                                                  uint16_t* __restrict__ cat, int stride, int off) {
+  const uint32_t cq = (uint32_t)c / 8;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nv * c) return;
-  cat[(e / c) * stride + off + e % c] = in[e];
+  if (e >= nv * cq) return;
+  const size_t v = e / cq;
+  const uint32_t q = (uint32_t)(e % cq);
+  st8(cat + v * stride + off + 8 * q, ld8(in + v * c + 8 * q));
 }
 
 // ---- head, loss --------------------------------------------------------------------------
@@ -472,54 +548,74 @@ hipError_t cs_unet_launch_conv(hipStream_t st, const uint16_t* in, int cin, int 
   return hipGetLastError();
 }
 
-int cs_unet_wgrad_slots(int s) {
+int cs_unet_wgrad_slots(int s, int cin, int cout) {
+  // about CS_UNET_WGRAD_TARGET workgroups per launch (two resident per
+  // compute unit); the partials then stay near that many * 27 * 256 floats
   const int nbricks = (s / 8) * (s / 8) * (s / 8);
-  return nbricks < 64 ? nbricks : 64;
+  const int pairs = (cin / 16) * (cout / 16);
+  const int want = pairs >= CS_UNET_WGRAD_TARGET ? 1 : CS_UNET_WGRAD_TARGET / pairs;
+  return nbricks < want ? nbricks : want;
 }
 
 hipError_t cs_unet_launch_wgrad(hipStream_t st, const uint16_t* in, int cin, const uint16_t* gout,
                                 const uint16_t* outv, int cout, int s, float* partial, float* bias_part, float* dw,
                                 float* db) {
-  // the bias gradient reads 8 channels per 16-byte vector
-  if (cout % 8 != 0 || cout > 256 || ((uintptr_t)gout | (uintptr_t)outv) % 16u != 0u) return hipErrorInvalidValue;
+  // 16-byte vectors of 8 channels throughout; whole 8^3 bricks
+  if (cin % 16 != 0 || cout % 16 != 0 || cout > 256 || s % 8 != 0 ||
+      ((uintptr_t)in | (uintptr_t)gout | (uintptr_t)outv) % 16u != 0u)
+    return hipErrorInvalidValue;
   const int nbricks = (s / 8) * (s / 8) * (s / 8);
-  const int slots = cs_unet_wgrad_slots(s);
-  const int per = (nbricks + slots - 1) / slots;
-  hipLaunchKernelGGL(k_unet_wgrad, dim3((unsigned)slots, 3u, (unsigned)((cin / 16) * (cout / 16))), dim3(256), 0, st,
-                     in, cin, gout, outv, cout, s, per, partial);
+  const int slots = cs_unet_wgrad_slots(s, cin, cout);
+  const int bricks_per_slot = (nbricks + slots - 1) / slots;
+  hipLaunchKernelGGL(k_unet_wgrad, dim3((unsigned)slots, (unsigned)((cin / 16) * (cout / 16))), dim3(WG_THREADS), 0,
+                     st, in, cin, gout, outv, cout, s, bricks_per_slot, partial);
   const size_t n = (size_t)27 * cin * cout;
-  hipLaunchKernelGGL(k_unet_reduce_slots, grid1(n), dim3(256), 0, st, partial, slots, n, dw);
+  const int per = CS_UNET_WGRAD_PER_GROUP, ngroups = (slots + per - 1) / per;
+  float* groups = partial + (size_t)slots * n;
+  hipLaunchKernelGGL(k_unet_reduce_slots, dim3(grid1(n).x, (unsigned)ngroups), dim3(256), 0, st, partial, slots, per,
+                     n, groups);
+  hipLaunchKernelGGL(k_unet_reduce_groups, grid1(n), dim3(256), 0, st, groups, ngroups, n, dw);
   hipLaunchKernelGGL(k_unet_bias_grad, dim3((unsigned)CS_UNET_BIAS_ROWS), dim3(256), 0, st,
                      gout, outv, cout, (size_t)s * s * s, bias_part);
   hipLaunchKernelGGL(k_unet_sum_rows, dim3((unsigned)cout), dim3(256), 0, st, bias_part, CS_UNET_BIAS_ROWS, cout, db);
   return hipGetLastError();
 }
 
+// The pooling, upsampling and concatenation kernels move 8 channels per 16-byte access.
+static bool vec8_ok(int c, int stride, int off, const void* a, const void* b) {
+  return c % 8 == 0 && stride % 8 == 0 && off % 8 == 0 && (((uintptr_t)a | (uintptr_t)b) % 16u) == 0u;
+}
 hipError_t cs_unet_launch_pool(hipStream_t st, const uint16_t* in, int c, int s, uint16_t* out) {
-  const size_t n = (size_t)(s / 2) * (s / 2) * (s / 2) * c;
+  if (!vec8_ok(c, 8, 0, in, out)) return hipErrorInvalidValue;
+  const size_t n = (size_t)(s / 2) * (s / 2) * (s / 2) * (c / 8);
   hipLaunchKernelGGL(k_unet_pool, grid1(n), dim3(256), 0, st, in, c, s, out);
   return hipGetLastError();
 }
 hipError_t cs_unet_launch_pool_back(hipStream_t st, const uint16_t* g_out, int c, int s, const uint16_t* add,
                                     int add_stride, int add_off, uint16_t* g_in) {
-  const size_t n = (size_t)s * s * s * c;
+  if (!vec8_ok(c, add ? add_stride : 8, add ? add_off : 0, g_out, g_in) || ((uintptr_t)add % 16u) != 0u)
+    return hipErrorInvalidValue;
+  const size_t n = (size_t)s * s * s * (c / 8);
   hipLaunchKernelGGL(k_unet_pool_back, grid1(n), dim3(256), 0, st, g_out, c, s, add, add_stride, add_off, g_in);
   return hipGetLastError();
 }
 hipError_t cs_unet_launch_up(hipStream_t st, const uint16_t* in, int c, int s, uint16_t* cat, int stride, int off) {
-  const size_t n = (size_t)(2 * s) * (2 * s) * (2 * s) * c;
+  if (!vec8_ok(c, stride, off, in, cat)) return hipErrorInvalidValue;
+  const size_t n = (size_t)(2 * s) * (2 * s) * (2 * s) * (c / 8);
   hipLaunchKernelGGL(k_unet_up, grid1(n), dim3(256), 0, st, in, c, s, cat, stride, off);
   return hipGetLastError();
 }
 hipError_t cs_unet_launch_up_back(hipStream_t st, const uint16_t* g_cat, int stride, int off, int c, int s,
                                   uint16_t* g_in) {
-  const size_t n = (size_t)s * s * s * c;
+  if (!vec8_ok(c, stride, off, g_cat, g_in)) return hipErrorInvalidValue;
+  const size_t n = (size_t)s * s * s * (c / 8);
   hipLaunchKernelGGL(k_unet_up_back, grid1(n), dim3(256), 0, st, g_cat, stride, off, c, s, g_in);
   return hipGetLastError();
 }
 hipError_t cs_unet_launch_put(hipStream_t st, const uint16_t* in, int c, size_t nv, uint16_t* cat, int stride,
                               int off) {
-  hipLaunchKernelGGL(k_unet_put, grid1(nv * c), dim3(256), 0, st, in, c, nv, cat, stride, off);
+  if (!vec8_ok(c, stride, off, in, cat)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_unet_put, grid1(nv * (c / 8)), dim3(256), 0, st, in, c, nv, cat, stride, off);
   return hipGetLastError();
 }
 hipError_t cs_unet_launch_head(hipStream_t st, const uint16_t* x, const float* wh, size_t nv, float* pred) {
