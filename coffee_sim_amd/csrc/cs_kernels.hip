@@ -668,8 +668,9 @@ __global__ void __launch_bounds__(256) k_emit(cs_emit_frame f, float* __restrict
 // ---- inputs and targets of the learned warm start (no reference counterpart) ---------
 // Four features per cell as the network reads them: rhs * rhs_scale and liquid
 // fill for fluid cells (0 elsewhere), the solid mask, and the bed porosity; with
-// `target`, also the converged pressure / pressure_scale and the fluid mask of
-// the loss, counting fluid cells into `count`.
+// `target`, also the converged pressure / pressure_scale and the mask of the
+// loss (fluid cells whose pressure is not held at the clamp), counting the
+// masked cells into `count`.
 __global__ void __launch_bounds__(256) k_unet_features(cs_params u, const float* __restrict__ kf,
                                                        const float* __restrict__ rhs, const int32_t* __restrict__ pres,
                                                        const int32_t* __restrict__ bed_id,
@@ -678,7 +679,7 @@ __global__ void __launch_bounds__(256) k_unet_features(cs_params u, const float*
                                                        float* __restrict__ target, uint8_t* __restrict__ mask,
                                                        uint32_t* __restrict__ count) {
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  bool fluid = false;
+  bool fluid = false, use = false;
   if (idx < u.nc) {
     const float skf = kf[idx];
     fluid = kf_fluid(skf);
@@ -687,12 +688,16 @@ __global__ void __launch_bounds__(256) k_unet_features(cs_params u, const float*
     f4[idx] = make_float4(fluid ? rhs[idx] * rhs_scale : 0.0f, fluid ? kf_fill(skf) : 0.0f,
                           skf == CS_KF_SOLID ? 1.0f : 0.0f, poro);
     if (target) {
-      target[idx] = fluid ? (float)pres[idx] * u.inv_fp_scale * inv_p_scale : 0.0f;
-      mask[idx] = fluid ? 1u : 0u;
+      // a cell held at the pressure bound is not a row of the solved system
+      // (its value is the bound, not a solution), so it is no training target
+      const float p = (float)pres[idx] * u.inv_fp_scale;
+      use = fluid && fabsf(p) != u.pressure_clamp;
+      target[idx] = fluid ? p * inv_p_scale : 0.0f;
+      mask[idx] = use ? 1u : 0u;
     }
   }
   if (target) {
-    const uint64_t bal = __ballot(fluid);
+    const uint64_t bal = __ballot(use);
     if ((threadIdx.x & 63u) == 0u && bal) atomicAdd(count, (uint32_t)__popcll(bal));
   }
 }

@@ -366,7 +366,8 @@ int cs_device_sincosf(cs_ctx* ctx, const float* in, float* sn, float* cs, uint32
  * [side^3][4], predictions and targets [side^3] in units of pressure_scale,
  * masks [side^3] (1 = counted in the loss).
  * Training pairs live on the device: cs_unet_capture appends the last solve
- * of a simulation (its features and converged pressure). */
+ * of a simulation (its features and converged pressure; its mask counts the
+ * fluid cells whose pressure is not held at the clamp). */
 typedef struct cs_unet cs_unet;
 int cs_unet_create(cs_ctx* ctx, uint32_t side, uint64_t seed, cs_unet** out);
 int cs_unet_destroy(cs_unet* net);
