@@ -950,8 +950,11 @@ __global__ void __launch_bounds__(256) k_batch_compact(const cs_scene_dev* __res
 }
 
 // Where one entry's brick lives: the scene's parameters and buffers.
+// The scene of an entry is the same in every lane, and its parameter block
+// is read in place (scalar loads through a uniform pointer): a copy of the
+// whole block per brick would live in scratch memory.
 struct BatchBrick {
-  cs_params u;
+  const cs_params* u;
   const float4* wc;
   const float* rhs;
   const int32_t* pin;
@@ -962,8 +965,9 @@ struct BatchBrick {
 
 __device__ __forceinline__ void batch_brick(const cs_scene_dev* __restrict__ sc, uint32_t entry, uint32_t cur,
                                             BatchBrick& b) {
+  entry = __builtin_amdgcn_readfirstlane(entry);
   const cs_scene_dev& d = sc[entry >> CS_BATCH_ENTRY_SCENE_SHIFT];
-  b.u = d.u;
+  b.u = &d.u;
   b.wc = reinterpret_cast<const float4*>(d.wc);
   b.rhs = d.rhs;
   b.pin = cur ? d.pres2 : d.pres;
@@ -1003,12 +1007,12 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
   BatchBrick cb;
   batch_brick(sc, entry, cur, cb);
   int x0, y0, z0;
-  brick_origin(cb.u, cb.brick, x0, y0, z0);
+  brick_origin(*cb.u, cb.brick, x0, y0, z0);
   x0 -= H;
   y0 -= H;
   z0 -= H;
   Reg reg;
-  reg.fetch_ball(cb.u, x0, y0, z0, cb.wc, cb.rhs, cb.pin);
+  reg.fetch_ball(*cb.u, x0, y0, z0, cb.wc, cb.rhs, cb.pin);
   uint32_t codes[H];
 #pragma unroll
   for (int k = 0; k < H; ++k)
@@ -1016,7 +1020,7 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
   s_list[threadIdx.x] = my_entry;  // read after the first staging barrier
   for (uint32_t k = 0;; ++k) {
     float* s_p = s_pp[k & 1u];
-    reg.stage_ball(cb.u, s_wc, s_rhs, s_p);
+    reg.stage_ball(*cb.u, s_wc, s_rhs, s_p);
     __syncthreads();
     const bool more = k + 1u < nmine;
     BatchBrick nbk;
@@ -1030,14 +1034,14 @@ __global__ void __launch_bounds__(NT) k_batch_rbgs(const cs_scene_dev* __restric
         batch_brick(sc, ne, cur, nbk);
       }
       entry = ne;
-      brick_origin(nbk.u, nbk.brick, nx0, ny0, nz0);
+      brick_origin(*nbk.u, nbk.brick, nx0, ny0, nz0);
       nx0 -= H;
       ny0 -= H;
       nz0 -= H;
-      reg.fetch_ball(nbk.u, nx0, ny0, nz0, nbk.wc, nbk.rhs, nbk.pin);
+      reg.fetch_ball(*nbk.u, nx0, ny0, nz0, nbk.wc, nbk.rhs, nbk.pin);
     }
-    const uint32_t fires = sweep_listed<H, NT>(cb.u, codes, s_wc, s_rhs, s_p);
-    store_brick<H, NT>(cb.u, x0, y0, z0, s_p, cb.pout);
+    const uint32_t fires = sweep_listed<H, NT>(*cb.u, codes, s_wc, s_rhs, s_p);
+    store_brick<H, NT>(*cb.u, x0, y0, z0, s_p, cb.pout);
     if (fires) atomicAdd(&cb.metrics[CS_M_P_CLAMP], fires);
     if (!more) break;
     // no barrier: the next staging writes the other pressure buffer
