@@ -273,8 +273,11 @@ __device__ __forceinline__ void p2g_atomic_particle(const cs_params& u, const fl
   if (!p2g_setup(u, part, aff, pid, q)) return;
   uint32_t nc = u.nc;
   uint32_t overflow = 0;
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int j = 0; j < 3; ++j)
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
         int cx = q.bx + i, cy = q.by + j, cz = q.bz + k;
         if (!on_grid(u, cx, cy, cz)) continue;

@@ -140,8 +140,11 @@ __device__ __forceinline__ void p2g_tile_block(const cs_params& u, const float4*
     const uint32_t pid = sorted[first + pos];
     P2GParticle q;
     p2g_setup(u, part, aff, pid, q);  // binned particles are active
+#pragma unroll
     for (int i = 0; i < 3; ++i)
+#pragma unroll
       for (int j = 0; j < 3; ++j)
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
           const int cx = q.bx + i, cy = q.by + j, cz = q.bz + k;
           if (!on_grid(u, cx, cy, cz)) continue;
