@@ -3,7 +3,7 @@
 // The reference has no learned component: it restarts the pressure from zero
 // every substep (shader.rs:1187).  This network predicts the converged
 // pressure from the solve's inputs so RBGS can start from it; the plain solve
-// stays the judge (its residual) and the fallback.
+// still decides convergence (its residual) and is the fallback.
 //
 // Activations are bf16, channels-last ([voxel][channel], voxel = (z*s + y)*s + x
 // on a cubic grid of side s), accumulation is fp32.  3x3x3 convolutions are

@@ -358,7 +358,7 @@ int cs_device_sincosf(cs_ctx* ctx, const float* in, float* sn, float* cs, uint32
  * the reference starts every solve from (shader.rs:1187); RBGS then runs to the
  * tolerance from the prediction and falls back to a zero start when the
  * prediction's relative residual exceeds the settings' warm_fallback_residual,
- * so the plain solve remains the judge.
+ * so the plain solve still decides the result.
  * The grid side must be a multiple of 32.  Parameters are one fp32 vector:
  * for each convolution e1a, e1b, e2a, e2b, ba, bb, d2a, d2b, d1a, d1b the
  * weights [27][cin][cout] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) then the bias
