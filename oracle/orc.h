@@ -17,7 +17,8 @@
  * IEEE binary32, no FMA contraction, correctly rounded / and sqrt, NaN-avoiding
  * min/max with ties returning the first operand, saturating float->i32 with
  * NaN -> 0, WGSL mix(a,b,t) = a*(1-t) + b*t, exp from Cody-Waite reduction and
- * a degree-7 Taylor polynomial.
+ * a degree-7 Taylor polynomial, and the emitter's sin/cos from a Cody-Waite
+ * reduction by pi/2 in double and Taylor series, rounded once to float.
  */
 #ifndef ORC_H
 #define ORC_H
