@@ -83,6 +83,12 @@ def _bind(lib: C.CDLL) -> None:
         "cs_sim_read_pressure": (C.c_int, [C.c_void_p, P(C.c_int32), P(C.c_int32)]),
         "cs_sim_read_metrics": (C.c_int, [C.c_void_p, P(C.c_uint32)]),
         "cs_sim_read_sdf": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_uint8)]),
+        "cs_sim_read_bed_lookup": (C.c_int, [C.c_void_p, P(C.c_int32)]),
+        "cs_filter_config_default": (None, [P(abi.FilterConfig)]),
+        "cs_bed_config_default": (None, [P(abi.BedConfig)]),
+        "cs_bed_config_seated_in_filter": (None, [P(abi.FilterConfig), P(abi.BedConfig)]),
+        "cs_bed_init": (C.c_int, [P(abi.BedConfig), P(C.c_uint32), P(C.c_float), C.c_uint32, P(C.c_float),
+                                  P(C.c_float), P(C.c_float), P(C.c_int32), P(C.c_uint32)]),
         "cs_sim_pack_render": (C.c_int, [C.c_void_p, P(C.c_float)]),
         "cs_sim_get_solve_stats": (C.c_int, [C.c_void_p, P(SolveStats)]),
         "cs_sim_set_warm_start": (C.c_int, [C.c_void_p, P(C.c_float)]),
@@ -486,15 +492,16 @@ class MpmSim3D:
         _check(load_library().cs_sim_set_particle_count(self._h, n), "set_particle_count")
 
     def read_particles(self, offset: int = 0, count: int | None = None) -> np.ndarray:
+        """Water particles [offset, offset + count) (bed particles live in the scene's own arrays)."""
         if count is None:
-            count = self.particle_count() - offset
+            count = self.water_slots_used() - offset
         out = np.empty((count, 8), np.float32)
         _check(load_library().cs_sim_read_particles(self._h, offset, count, _fptr(out)), "read_particles")
         return out
 
     def read_affine(self, offset: int = 0, count: int | None = None) -> np.ndarray:
         if count is None:
-            count = self.particle_count() - offset
+            count = self.water_slots_used() - offset
         out = np.empty((count, 12), np.float32)
         _check(load_library().cs_sim_read_affine(self._h, offset, count, _fptr(out)), "read_affine")
         return out
@@ -515,6 +522,13 @@ class MpmSim3D:
         _check(load_library().cs_sim_read_pressure(self._h, _iptr(p), _iptr(k)), "read_pressure")
         return p, k
 
+    def read_bed_lookup(self) -> np.ndarray:
+        """Per cell, the highest bed particle that stamped it in the last substep, or -1."""
+        out = np.empty(self.nc, np.int32)
+        _check(load_library().cs_sim_read_bed_lookup(self._h, out.ctypes.data_as(C.POINTER(C.c_int32))),
+               "cs_sim_read_bed_lookup")
+        return out
+
     def read_sdf(self) -> tuple[np.ndarray, np.ndarray]:
         """The SDF (sdf_res^3) and class mask (one byte per cell) the simulation holds."""
         r = int(self.settings.sdf_res)
@@ -532,7 +546,7 @@ class MpmSim3D:
 
     def pack_render(self) -> np.ndarray:
         """prepare_render's RenderParticle per particle in use: (n, 8) float32."""
-        out = np.empty((self.particle_count(), 8), np.float32)
+        out = np.empty((self.water_slots_used(), 8), np.float32)
         _check(load_library().cs_sim_pack_render(self._h, _fptr(out)), "cs_sim_pack_render")
         return out
 
@@ -632,6 +646,60 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def filter_config_default() -> abi.FilterConfig:
+    """FilterConfig::default (filter.rs:14-26)."""
+    f = abi.FilterConfig()
+    load_library().cs_filter_config_default(C.byref(f))
+    return f
+
+
+def bed_config_default() -> abi.BedConfig:
+    """BedConfig::default (bed.rs:20-38)."""
+    b = abi.BedConfig()
+    load_library().cs_bed_config_default(C.byref(b))
+    return b
+
+
+def bed_config_seated_in_filter(filt: abi.FilterConfig | None = None) -> abi.BedConfig:
+    """BedConfig::seated_in_filter (bed.rs:41-81); the default filter when none is given."""
+    f = filt if filt is not None else filter_config_default()
+    b = abi.BedConfig()
+    load_library().cs_bed_config_seated_in_filter(C.byref(f), C.byref(b))
+    return b
+
+
+def bed_init(config: abi.BedConfig, grid_dims, bounds_size) -> dict:
+    """init_bed_particles (bed.rs:107-206) with build_cell_lookup (bed.rs:208-288):
+    'particles' (n, 8), 'affine' (n, 12), 'extracts' (n, 8) and 'cell_lookup' (gx*gy*gz)."""
+    L = load_library()
+    dims = (C.c_uint32 * 3)(*[int(d) for d in grid_dims])
+    bounds = (C.c_float * 3)(*[float(b) for b in bounds_size])
+    n = C.c_uint32()
+    _check(L.cs_bed_init(C.byref(config), dims, bounds, 0, None, None, None, None, C.byref(n)), "cs_bed_init")
+    count = int(n.value)
+    part = np.zeros((count, 8), np.float32)
+    aff = np.zeros((count, 12), np.float32)
+    ext = np.zeros((count, 8), np.float32)
+    lookup = np.empty(int(np.prod([int(d) for d in grid_dims])), np.int32)
+    _check(L.cs_bed_init(C.byref(config), dims, bounds, max(count, 1), _fptr(part), _fptr(aff), _fptr(ext),
+                         lookup.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n)), "cs_bed_init")
+    return {"particles": part, "affine": aff, "extracts": ext, "cell_lookup": lookup}
+
+
+def attach_bed(scene: Scene | None, bed: dict) -> Scene:
+    """A scene carrying `bed` (from bed_init) as its bed of particles; the arrays are
+    kept alive on the returned scene."""
+    sc = scene if scene is not None else Scene()
+    keep = [np.ascontiguousarray(bed["particles"], np.float32), np.ascontiguousarray(bed["affine"], np.float32),
+            np.ascontiguousarray(bed["extracts"], np.float32)]
+    sc.bed_particles = _fptr(keep[0])
+    sc.bed_affine = _fptr(keep[1])
+    sc.bed_extracts = _fptr(keep[2])
+    sc.num_bed_particles = len(keep[0])
+    sc._bed_keep = keep
+    return sc
 
 
 def generate_sdf(settings: Settings) -> np.ndarray:

@@ -165,7 +165,26 @@ class Scene(C.Structure):
         ("particles", C.POINTER(C.c_float)),
         ("affine", C.POINTER(C.c_float)),
         ("num_particles", C.c_uint32),
+        ("bed_particles", C.POINTER(C.c_float)),
+        ("bed_affine", C.POINTER(C.c_float)),
+        ("bed_extracts", C.POINTER(C.c_float)),
+        ("num_bed_particles", C.c_uint32),
     ]
+
+
+class FilterConfig(C.Structure):
+    """FilterConfig (filter.rs:3-26)."""
+    _fields_ = [("center", C.c_float * 3), ("top_y", C.c_float), ("bot_y", C.c_float),
+                ("top_radius", C.c_float), ("bot_radius", C.c_float), ("thickness", C.c_float),
+                ("hole_radius", C.c_float)]
+
+
+class BedConfig(C.Structure):
+    """BedConfig (bed.rs:7-38)."""
+    _fields_ = [("center", C.c_float * 3), ("top_y", C.c_float), ("bot_y", C.c_float),
+                ("top_radius", C.c_float), ("bot_radius", C.c_float), ("num_particles", C.c_uint32),
+                ("initial_porosity", C.c_float), ("initial_permeability", C.c_float),
+                ("extractable_mass", C.c_float)]
 
 
 class SolveStats(C.Structure):

@@ -380,6 +380,8 @@ int cs_batch_create(cs_ctx* ctx, cs_sim* const* sims, uint32_t count, const cs_b
       return cs_fail(CS_ERR_INVALID, "batched scenes start their solves from zero or the previous pressure");
     if (m->s.rbgs_rescale_after)
       return cs_fail(CS_ERR_INVALID, "batched scenes do not take rbgs_rescale_after");
+    if (m->num_bed)
+      return cs_fail(CS_ERR_INVALID, "batched scenes do not take bed particles");
     if ((m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS) != fixed)
       return cs_fail(CS_ERR_INVALID, "every scene of a batch must use the same rbgs_mode");
     if (std::max(m->s.substeps, 1u) != std::max(m0->s.substeps, 1u))

@@ -49,6 +49,12 @@ hipError_t cs_launch_sincosf(hipStream_t stream, const float* in, float* sn, flo
 // settings' obstacles, as cs_generate_sdf / cs_generate_sdf_class make them on the host.
 hipError_t cs_launch_gen_sdf(hipStream_t stream, const cs_settings& s, float* out);
 hipError_t cs_launch_gen_sdf_class(hipStream_t stream, const cs_settings& s, const float* sdf, uint8_t* out);
+// Reference-mode bed: clear the lookup, stamp it from the nbed bed particles
+// (bed_lookup_clear / bed_lookup_scatter) and build the table of the stamped cells
+// (at most `capacity`), mapping each cell to its entry in bed_id.
+hipError_t cs_launch_bed_lookup(hipStream_t st, const cs_params& u, uint32_t nbed, const float* bed_part,
+                                const float* bed_aff, const float* bed_ext, uint32_t capacity, int32_t* lookup,
+                                uint32_t* counter, int32_t* bed_id, cs_bed_cell* table);
 // The particles of one substep's emission (f.count of them at slots f.first..)
 // and their zero affine records.
 hipError_t cs_launch_emit(hipStream_t stream, const cs_emit_frame& f, float* part, float* aff);

@@ -16,6 +16,7 @@
 
 #include "cs_batch.h"
 #include "cs_device.h"
+#include "cs_bed.h"
 #include "cs_sdf.h"
 
 // ---------------------------------------------------------------------------------
@@ -616,6 +617,93 @@ __global__ void __launch_bounds__(256) k_pack_render(cs_params u, const float4* 
   }
   out[2 * pid] = make_float4(P.x, P.y, P.z, color_t);
   out[2 * pid + 1] = make_float4(radius, 0.0f, 0.0f, 0.0f);
+}
+
+// ---- bed of particles: lookup and table (shader.rs:2291-2335, 380-449, 621-666) -------
+// bed_lookup_clear, plus the table slot counter of k_bed_table.
+__global__ void __launch_bounds__(256) k_bed_lookup_clear(uint32_t nc, int32_t* __restrict__ lookup,
+                                                          uint32_t* __restrict__ counter) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *counter = 0u;
+  if (i < nc) lookup[i] = -1;
+}
+
+// bed_lookup_scatter: every non-water bed particle stamps its id into the 3x3x3
+// cells around world_to_cell(pos) (shader.rs:668-671) with atomicMax, so the
+// highest id wins whatever the order.
+__global__ void __launch_bounds__(256) k_bed_lookup_scatter(cs_params u, uint32_t nbed,
+                                                            const float4* __restrict__ bed_part,
+                                                            const float4* __restrict__ bed_aff,
+                                                            int32_t* __restrict__ lookup) {
+  const uint32_t bid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bid >= nbed) return;
+  if (bed_aff[3 * bid].w < 0.5f) return;  // is_water_phase (PHASE_WATER_MAX = 0.5)
+  const float4 P = bed_part[2 * bid];
+  // floor, then i32(): saturating (the clamp keeps the conversion defined)
+  auto cell = [&](float p, float o) {
+    const float g = floorf((p - o) * u.inv_dx);
+    return (int)fminf(fmaxf(g, -2147483648.0f), 2147483520.0f);
+  };
+  const int cx = cell(P.x, u.origin[0]), cy = cell(P.y, u.origin[1]), cz = cell(P.z, u.origin[2]);
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int x = cx + di, y = cy + dj, z = cz + dk;
+        if (x < 0 || y < 0 || z < 0 || x >= (int)u.gx || y >= (int)u.gy || z >= (int)u.gz) continue;
+        atomicMax(&lookup[((uint32_t)z * u.gy + (uint32_t)y) * u.gx + (uint32_t)x], (int32_t)bid);
+      }
+}
+
+// The table entry of every stamped cell, from the stamping particle's record
+// (cs_bed_particle_record), in a slot taken from `counter`; bed_id maps each
+// cell to its slot (-1 elsewhere).  The slots' order varies from run to run;
+// every reader goes through bed_id, so no result does.
+__global__ void __launch_bounds__(256) k_bed_table(cs_params u, const int32_t* __restrict__ lookup,
+                                                   const float* __restrict__ bed_part,
+                                                   const float* __restrict__ bed_ext, uint32_t capacity,
+                                                   uint32_t* __restrict__ counter, int32_t* __restrict__ bed_id,
+                                                   cs_bed_cell* __restrict__ table) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= u.nc) return;
+  const int32_t id = lookup[c];
+  if (id < 0) {
+    bed_id[c] = -1;
+    return;
+  }
+  uint32_t ix, iy, iz;
+  decompose(u, c, ix, iy, iz);
+  const cs_bed_record rec = cs_bed_particle_record(bed_ext + 8 * (size_t)id, bed_part + 8 * (size_t)id);
+  auto at = [&](int x, int y, int z) -> int32_t {
+    if (x < 0 || y < 0 || z < 0 || x >= (int)u.gx || y >= (int)u.gy || z >= (int)u.gz) return -2;
+    return lookup[((uint32_t)z * u.gy + (uint32_t)y) * u.gx + (uint32_t)x];
+  };
+  auto vel = [&](int32_t k, float* v) {
+    v[0] = bed_part[8 * (size_t)k + 4];
+    v[1] = bed_part[8 * (size_t)k + 5];
+    v[2] = bed_part[8 * (size_t)k + 6];
+  };
+  cs_bed_cell t;
+  cs_bed_cell_build(u, u.gx, u.gy, u.gz, (int)ix, (int)iy, (int)iz, rec, at, vel, &t);
+  const uint32_t slot = atomicAdd(counter, 1u);
+  if (slot < capacity) {
+    table[slot] = t;
+    bed_id[c] = (int32_t)slot;
+  } else {
+    bed_id[c] = -1;  // cannot happen: at most 27 stamped cells per bed particle
+  }
+}
+
+hipError_t cs_launch_bed_lookup(hipStream_t st, const cs_params& u, uint32_t nbed, const float* bed_part,
+                                const float* bed_aff, const float* bed_ext, uint32_t capacity, int32_t* lookup,
+                                uint32_t* counter, int32_t* bed_id, cs_bed_cell* table) {
+  const uint32_t nc = (uint32_t)u.nc;
+  hipLaunchKernelGGL(k_bed_lookup_clear, dim3((nc + 255) / 256), dim3(256), 0, st, nc, lookup, counter);
+  if (nbed)
+    hipLaunchKernelGGL(k_bed_lookup_scatter, dim3((nbed + 255) / 256), dim3(256), 0, st, u, nbed,
+                       reinterpret_cast<const float4*>(bed_part), reinterpret_cast<const float4*>(bed_aff), lookup);
+  hipLaunchKernelGGL(k_bed_table, dim3((nc + 255) / 256), dim3(256), 0, st, u, lookup, bed_part, bed_ext, capacity,
+                     counter, bed_id, table);
+  return hipGetLastError();
 }
 
 // ---- exp check kernel (used by the verification entry point) -----------------------

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "cs_host.h"
+#include "cs_bed.h"
 #include "cs_sdf.h"
 
 namespace cs {
@@ -369,17 +370,6 @@ void make_params(const cs_settings& s, uint32_t num_particles, uint32_t num_bed_
 }
 
 // ---- static bed table (field mode of shader.rs:380-449, 621-666) -------------------------
-static float kozeny_factor(float phi_in) {
-  float phi = cs_clamp(phi_in, 0.08f, 0.82f);
-  float solid = cs_max(1.0f - phi, 1e-4f);
-  return (phi * phi * phi) / (solid * solid);
-}
-static float compacted_porosity(const cs_bed_record& b) {
-  float base = cs_clamp(b.porosity, 0.18f, 0.70f);
-  float comp = cs_clamp(b.compaction, 0.0f, 2.25f);
-  return cs_clamp(base - comp * 0.12f, 0.16f, base);
-}
-
 bool build_bed_table(const cs_settings& s, const int32_t* lookup, const cs_bed_record* rec, uint32_t nrec,
                      std::vector<int32_t>& cell_index, std::vector<cs_bed_cell>& table) {
   const uint32_t gx = s.grid_dims[0], gy = s.grid_dims[1], gz = s.grid_dims[2];
@@ -391,94 +381,24 @@ bool build_bed_table(const cs_settings& s, const int32_t* lookup, const cs_bed_r
     if (lookup[i] >= (int32_t)nrec || lookup[i] < -1) return false;
   cs_params u;
   make_params(s, 0, 0, 0.0f, &u);
-  const float dx = u.dx, inv_dx = u.inv_dx;
   auto at = [&](int x, int y, int z) -> int32_t {
     if (x < 0 || y < 0 || z < 0 || x >= (int)gx || y >= (int)gy || z >= (int)gz) return -2;
     return lookup[((size_t)z * gy + y) * gx + x];
   };
-  // bed_matrix_velocity_cell (shader.rs:380-407)
-  auto bmv = [&](int x, int y, int z, float* out) {
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f, w = 0.0f;
-    for (int di = -1; di <= 1; ++di)
-      for (int dj = -1; dj <= 1; ++dj)
-        for (int dk = -1; dk <= 1; ++dk) {
-          int32_t id = at(x + di, y + dj, z + dk);
-          if (id < 0) continue;
-          sx = sx + rec[id].velocity[0];
-          sy = sy + rec[id].velocity[1];
-          sz = sz + rec[id].velocity[2];
-          w += 1.0f;
-        }
-    if (w <= 0.0f) {
-      out[0] = out[1] = out[2] = 0.0f;
-    } else {
-      out[0] = sx / w;
-      out[1] = sy / w;
-      out[2] = sz / w;
-    }
+  auto vel = [&](int32_t id, float* v) {
+    v[0] = rec[id].velocity[0];
+    v[1] = rec[id].velocity[1];
+    v[2] = rec[id].velocity[2];
   };
-  // bed_velocity_load_cell (shader.rs:409-417)
-  auto bvl = [&](int x, int y, int z, float* out) {
-    if (x < 0 || y < 0 || z < 0 || x >= (int)gx || y >= (int)gy || z >= (int)gz) {
-      out[0] = out[1] = out[2] = 0.0f;
-      return;
-    }
-    bmv(x, y, z, out);
-  };
-  const float cv = dx * dx * dx;
   for (uint32_t iz = 0; iz < gz; ++iz)
     for (uint32_t iy = 0; iy < gy; ++iy)
       for (uint32_t ix = 0; ix < gx; ++ix) {
-        size_t c = ((size_t)iz * gy + iy) * gx + ix;
-        int32_t id = lookup[c];
+        const size_t c = ((size_t)iz * gy + iy) * gx + ix;
+        const int32_t id = lookup[c];
         if (id < 0) continue;
-        const cs_bed_record& b = rec[id];
         cs_bed_cell t;
         memset(&t, 0, sizeof(t));
-        // bed_compacted_permeability (shader.rs:442-449)
-        float base_p = cs_clamp(b.porosity, 0.18f, 0.70f);
-        float compacted = compacted_porosity(b);
-        float ratio = kozeny_factor(compacted) / cs_max(kozeny_factor(base_p), 1e-8f);
-        t.k_record = cs_max(b.permeability * ratio, u.min_k);
-        t.v_record[0] = b.velocity[0];
-        t.v_record[1] = b.velocity[1];
-        t.v_record[2] = b.velocity[2];
-        // bed_pore_projection_target_divergence factors (shader.rs:628-662)
-        float ccy = u.origin[1] + ((float)iy + 0.5f) * dx;
-        float base = compacted;
-        float sat = cs_clamp(b.saturation, 0.0f, 1.0f);
-        float comp = cs_clamp(b.compaction, 0.0f, 2.25f);
-        float surface_t = cs_smoothstep(-0.25f, 0.75f, (ccy - b.sample_y) * inv_dx);
-        float open = cs_clamp((1.0f - 0.65f * sat) * (1.0f - 0.12f * comp), 0.20f, 1.0f);
-        float opening = surface_t * cs_clamp(1.0f, 0.0f, 1.0f) * open;  // bed_surface_void_scale = 1
-        float surf_por = cs_min(base + 0.35f, 0.82f);
-        float eff = cs_mix(base, surf_por, opening);
-        float occ = cs_mix(1.0f, 0.35f, opening);
-        float satc = cs_mix(1.0f, 0.32f, sat);
-        float compc = cs_mix(1.0f, 0.58f, cs_clamp(comp / 1.5f, 0.0f, 1.0f));
-        t.pore_capacity = eff * cv * cs_max(1.0f, 0.0f) * satc * compc;  // bed_pore_capacity_scale = 1
-        t.overfill_resp = 18.0f * occ * cs_mix(1.0f, 1.45f, sat);      // bed_pore_overfill_alpha = 18
-        t.porosity = cs_clamp(compacted, 0.08f, 1.0f);
-        t.solid_fraction = 1.0f - t.porosity;
-        // bed_velocity_divergence (shader.rs:419-427)
-        float a[3], bb[3];
-        bvl((int)ix - 1, (int)iy, (int)iz, a);
-        float vxm = a[0];
-        bvl((int)ix + 1, (int)iy, (int)iz, a);
-        float vxp = a[0];
-        bvl((int)ix, (int)iy - 1, (int)iz, a);
-        float vym = a[1];
-        bvl((int)ix, (int)iy + 1, (int)iz, a);
-        float vyp = a[1];
-        bvl((int)ix, (int)iy, (int)iz - 1, a);
-        float vzm = a[2];
-        bvl((int)ix, (int)iy, (int)iz + 1, a);
-        float vzp = a[2];
-        t.solid_div = 0.5f * inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
-        bmv((int)ix, (int)iy, (int)iz, bb);
-        t.bmv[0] = bb[0];
-        t.bmv[1] = bb[1];
-        t.bmv[2] = bb[2];
+        cs_bed_cell_build(u, gx, gy, gz, (int)ix, (int)iy, (int)iz, rec[id], at, vel, &t);
         cell_index[c] = (int32_t)table.size();
         table.push_back(t);
       }

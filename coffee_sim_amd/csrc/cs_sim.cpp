@@ -61,7 +61,8 @@ void free_sim(cs_sim* m) {
                  m->rhs,        m->pack,       m->prev_p,     m->warm,       m->metrics, m->res_partials,
                  m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,
                  m->brick_flag, m->brick_list, m->brick_partials, m->dstate,       m->tile_count,
-                 m->tile_start, m->p2g_key,    m->p2g_sorted, m->render};
+                 m->tile_start, m->p2g_key,    m->p2g_sorted, m->render,     m->bed_part,
+                 m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -486,7 +487,9 @@ int substep(cs_sim* m, float sub_dt) {
   // 1. emission (inflow.rs:122-241): the flow model and the disk frame on the
   // host, the particles on the device, appended after the live particles
   cs_emit_frame ef;
-  const uint32_t emitted = cs::emit_plan(m->s, m->inflow, sub_dt, m->num_water, &ef, &m->total_dropped);
+  // capacity counts the bed particles, which the reference's buffer holds first
+  const uint32_t emitted = cs::emit_plan(m->s, m->inflow, sub_dt, m->num_water + m->num_bed, &ef, &m->total_dropped);
+  ef.first = m->num_water;  // the water's own array
   if (emitted) {
     CS_HIP(cs_launch_emit(st, ef, m->part, m->aff));
     m->num_water += emitted;
@@ -499,6 +502,10 @@ int substep(cs_sim* m, float sub_dt) {
   mark(m, P_P2G);
   CS_HIP(cs_launch_fill_i32(st, m->lanes, m->nc * 6, 0));
   CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->metrics), CS_METRICS_SLOTS, 0));
+  // bed_lookup_clear / bed_lookup_scatter (mod.rs:322-331) and the bed cells' table
+  if (m->num_bed)
+    CS_HIP(cs_launch_bed_lookup(st, u, m->num_bed, m->bed_part, m->bed_aff, m->bed_ext, m->num_bed_cells,
+                                m->bed_lookup, m->bed_counter, m->bed_id, m->bed));
   kmark(m, 0);
   if (m->p2g_tiled)
     CS_HIP(cs_launch_p2g_tiled(st, u, m->part, m->aff, m->lanes, m->metrics, m->tile_count, m->tile_start,
@@ -642,6 +649,24 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
     bed_index.assign(m->nc, -1);
   }
   m->num_bed_cells = (uint32_t)bed_table.size();
+  if (scene && scene->num_bed_particles) {
+    if (!scene->bed_particles || !scene->bed_affine || !scene->bed_extracts) {
+      free_sim(m);
+      return fail(CS_ERR_INVALID, "bed particles need bed_particles, bed_affine and bed_extracts");
+    }
+    if (s->has_bed_field) {
+      free_sim(m);
+      return fail(CS_ERR_INVALID, "a bed field and bed particles are exclusive");
+    }
+    if (scene->num_bed_particles > s->max_particles ||
+        (scene->particles ? scene->num_particles : 0u) > s->max_particles - scene->num_bed_particles) {
+      free_sim(m);
+      return fail(CS_ERR_RANGE, "bed and water particles exceed max_particles");
+    }
+    m->num_bed = scene->num_bed_particles;
+    // each bed particle stamps at most 27 cells
+    m->num_bed_cells = (uint32_t)std::min<size_t>(m->nc, (size_t)27 * m->num_bed);
+  }
   if (scene && scene->particles && scene->num_particles) {
     if (scene->num_particles > s->max_particles) {
       free_sim(m);
@@ -664,7 +689,14 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   ALLOC(m->sdf, nsdf);
   ALLOC(m->cls, m->nc);
   ALLOC(m->bed_id, m->nc);
-  ALLOC(m->bed, bed_table.size());
+  ALLOC(m->bed, m->num_bed ? (size_t)m->num_bed_cells : bed_table.size());
+  if (m->num_bed) {
+    ALLOC(m->bed_part, (size_t)m->num_bed * 8);
+    ALLOC(m->bed_aff, (size_t)m->num_bed * 12);
+    ALLOC(m->bed_ext, (size_t)m->num_bed * 8);
+    ALLOC(m->bed_lookup, m->nc);
+    ALLOC(m->bed_counter, 1);
+  }
   ALLOC(m->kind, m->nc);
   ALLOC(m->kf, m->nc);
   ALLOC(m->pres, m->nc);
@@ -748,6 +780,13 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (e == hipSuccess) e = hipMemcpyAsync(m->bed_id, bed_index.data(), m->nc * sizeof(int32_t), hipMemcpyHostToDevice, st);
   if (e == hipSuccess && !bed_table.empty())
     e = hipMemcpyAsync(m->bed, bed_table.data(), bed_table.size() * sizeof(cs_bed_cell), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && m->num_bed) {
+    e = hipMemcpyAsync(m->bed_part, scene->bed_particles, (size_t)m->num_bed * 32, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(m->bed_aff, scene->bed_affine, (size_t)m->num_bed * 48, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(m->bed_ext, scene->bed_extracts, (size_t)m->num_bed * 32, hipMemcpyHostToDevice, st);
+  }
   if (e == hipSuccess) e = hipMemsetAsync(m->gv, 0, m->nc * 16, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->gv2, 0, m->nc * 16, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->pres, 0, m->nc * 4, st);
@@ -829,7 +868,7 @@ int cs_sim_set_spout_target(cs_sim* m, float x, float y, float z) {
 int cs_sim_get_counts(cs_sim* m, uint32_t* w, uint32_t* b, uint32_t* mp) {
   if (!m) return fail(CS_ERR_INVALID, "null sim");
   if (w) *w = m->num_water;
-  if (b) *b = 0;
+  if (b) *b = m->num_bed;
   if (mp) *mp = m->s.max_particles;
   return CS_OK;
 }
@@ -907,6 +946,14 @@ int cs_sim_read_sdf(cs_sim* m, float* sdf, uint8_t* sdf_class) {
   if (sdf) rc = read_dev(m, sdf, m->sdf, (size_t)m->s.sdf_res * m->s.sdf_res * m->s.sdf_res * sizeof(float));
   if (rc == CS_OK && sdf_class) rc = read_dev(m, sdf_class, m->cls, m->nc);
   return rc;
+}
+int cs_sim_read_bed_lookup(cs_sim* m, int32_t* lookup) {
+  if (!m || !lookup) return fail(CS_ERR_INVALID, "null argument");
+  if (!m->num_bed) {
+    for (size_t i = 0; i < m->nc; ++i) lookup[i] = -1;
+    return CS_OK;
+  }
+  return read_dev(m, lookup, m->bed_lookup, m->nc * sizeof(int32_t));
 }
 int cs_sim_read_metrics(cs_sim* m, uint32_t* slots) {
   if (!m) return fail(CS_ERR_INVALID, "null sim");

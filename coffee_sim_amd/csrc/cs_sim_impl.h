@@ -29,6 +29,13 @@ struct cs_sim {
   int32_t* bed_id = nullptr;
   cs_bed_cell* bed = nullptr;
   uint32_t num_bed_cells = 0;
+  // a bed of particles (reference mode): their own arrays, re-stamped every substep
+  uint32_t num_bed = 0;
+  float* bed_part = nullptr;        // 8 floats per bed particle
+  float* bed_aff = nullptr;         // 12 floats per bed particle
+  float* bed_ext = nullptr;         // BedExtract, 8 floats per bed particle
+  int32_t* bed_lookup = nullptr;    // per cell: highest stamping bed particle, or -1
+  uint32_t* bed_counter = nullptr;  // table slots taken this substep
   int8_t* kind = nullptr;
   float* kf = nullptr;
   int32_t* pres = nullptr;

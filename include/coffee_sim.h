@@ -183,6 +183,21 @@ typedef struct cs_scene {
   const float* particles;    /* 8 floats per particle: pos.xyz, J, vel.xyz, m  */
   const float* affine;       /* 12 floats per particle: C columns, .w phase    */
   uint32_t num_particles;    /* initial water particles                       */
+  /* A bed of particles ("reference mode", exclusive with has_bed_field): the
+   * reference's particle buffer holds these first, then the water (mod.rs:239-249).
+   * Here they sit in their own arrays; p2g and g2p skip them either way
+   * (shader.rs:967, 1693).  Every substep clears and re-stamps the cell lookup
+   * from them (bed_lookup_clear / bed_lookup_scatter, shader.rs:2291-2335) and
+   * rebuilds the bed cells' records from their BedExtracts.  The bed particles
+   * and extracts stay as given: the passes that move them (bed_coupling,
+   * extraction_advect, bed_dynamics) are not part of this library. */
+  const float* bed_particles; /* 8 floats per bed particle, as `particles`     */
+  const float* bed_affine;    /* 12 floats per bed particle, .w of col0 = phase */
+  const float* bed_extracts;  /* 8 floats per bed particle: BedExtract bed.xyzw
+                               * (pore water, porosity, permeability, compaction)
+                               * and extract.xyzw (extractable, dissolved, temp,
+                               * saturation), shader.rs:35-38                   */
+  uint32_t num_bed_particles;
 } cs_scene;
 
 /* Solver statistics of the most recent substep. */
@@ -203,6 +218,35 @@ typedef struct cs_solve_stats {
 #define CS_PRESET_FREE_STREAM 1
 #define CS_PRESET_CENTER_POUR 2
 int cs_settings_preset(int preset, cs_settings* out);
+
+/* FilterConfig (filter.rs:3-26): the paper filter's frustum, y relative to center.y. */
+typedef struct cs_filter_config {
+  float center[3];
+  float top_y, bot_y, top_radius, bot_radius, thickness, hole_radius;
+} cs_filter_config;
+/* BedConfig (bed.rs:7-38): the granular bed's frustum, y relative to center.y. */
+typedef struct cs_bed_config {
+  float center[3];
+  float top_y, bot_y, top_radius, bot_radius;
+  uint32_t num_particles;      /* target count (DEFAULT_BREW.bed_particle_samples) */
+  float initial_porosity;
+  float initial_permeability;  /* m^2, Kozeny-Carman of the default grind */
+  float extractable_mass;
+} cs_bed_config;
+/* FilterConfig::default (filter.rs:14-26) and BedConfig::default (bed.rs:20-38). */
+void cs_filter_config_default(cs_filter_config* out);
+void cs_bed_config_default(cs_bed_config* out);
+/* BedConfig::seated_in_filter (bed.rs:41-81), the bed of MpmSettings::default_v60. */
+void cs_bed_config_seated_in_filter(const cs_filter_config* filter, cs_bed_config* out);
+/* init_bed_particles (bed.rs:107-206) with build_cell_lookup (bed.rs:208-288).
+ * *count receives the bed particle count (at most config->num_particles); the
+ * arrays, when not NULL, receive 8 (particle), 12 (affine, phase 1) and 8
+ * (BedExtract) floats per particle, up to `capacity` particles, and the initial
+ * gx*gy*gz cell lookup.  Returns CS_ERR_RANGE when *count exceeds a non-zero
+ * `capacity` with arrays given.  Host only. */
+int cs_bed_init(const cs_bed_config* config, const uint32_t grid_dims[3], const float bounds_size[3],
+                uint32_t capacity, float* particles, float* affine, float* extracts, int32_t* cell_lookup,
+                uint32_t* count);
 
 /* Seeded synthetic workloads (no reference counterpart; the reference has no RNG).
  *   CS_WORKLOAD_V60_SMALL: 64^3, dx 2 mm, 32,768-particle jittered cylinder over a flat bed.
@@ -301,6 +345,10 @@ int cs_sim_read_pressure(cs_sim* sim, int32_t* pressure_fp, int32_t* kind);
  * on the device from the settings when the scene gave none (generate_sdf_data
  * and generate_sdf_class_data, state.rs:206-344).  Either pointer may be NULL. */
 int cs_sim_read_sdf(cs_sim* sim, float* sdf, uint8_t* sdf_class);
+/* The bed lookup of the most recent substep (gx*gy*gz: the highest stamping bed
+ * particle id, or -1), as bed_lookup_scatter leaves it (shader.rs:2304-2335).
+ * A simulation without bed particles reads all -1. */
+int cs_sim_read_bed_lookup(cs_sim* sim, int32_t* lookup);
 /* Metric slots (state.rs:22-28). */
 int cs_sim_read_metrics(cs_sim* sim, uint32_t* slots8);
 /* prepare_render (shader.rs:2242-2276, run by step_frame at mod.rs:410-412 and

@@ -71,6 +71,9 @@ void orc_read_affine(orc_sim* sim, uint32_t offset, uint32_t count, float* a12);
 void orc_read_p2g_lanes(orc_sim* sim, int32_t* lanes6);
 void orc_read_grid_vel(orc_sim* sim, float* vel4);
 void orc_read_pressure(orc_sim* sim, int32_t* p, int32_t* kind);
+/* The bed lookup: per cell the highest stamping bed particle (reference mode),
+ * the scene's lookup (field mode) or -1. */
+void orc_read_bed_lookup(orc_sim* sim, int32_t* lookup);
 void orc_read_metrics(orc_sim* sim, uint32_t* slots8);
 /* prepare_render (shader.rs:2242-2276): RenderParticle, 8 floats per particle in use. */
 void orc_pack_render(orc_sim* sim, float* render8);

@@ -73,6 +73,7 @@ def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
             "orc_read_p2g_lanes": (None, [C.c_void_p, i32p]),
             "orc_read_grid_vel": (None, [C.c_void_p, f32p]),
             "orc_read_pressure": (None, [C.c_void_p, i32p, i32p]),
+            "orc_read_bed_lookup": (None, [C.c_void_p, i32p]),
             "orc_read_metrics": (None, [C.c_void_p, P(C.c_uint32)]),
             "orc_pack_render": (None, [C.c_void_p, f32p]),
             "orc_get_solve_stats": (None, [C.c_void_p, P(SolveStats)]),
@@ -198,14 +199,14 @@ class OracleSim:
 
     def read_particles(self, offset=0, count=None):
         if count is None:
-            count = self.particle_count() - offset
+            count = self.counts()[0] - offset
         out = np.empty((count, 8), np.float32)
         self._L.orc_read_particles(self._h, offset, count, _f(out))
         return out
 
     def read_affine(self, offset=0, count=None):
         if count is None:
-            count = self.particle_count() - offset
+            count = self.counts()[0] - offset
         out = np.empty((count, 12), np.float32)
         self._L.orc_read_affine(self._h, offset, count, _f(out))
         return out
@@ -218,6 +219,11 @@ class OracleSim:
     def read_grid_vel(self):
         out = np.empty((self.nc, 4), np.float32)
         self._L.orc_read_grid_vel(self._h, _f(out))
+        return out
+
+    def read_bed_lookup(self):
+        out = np.empty(self.nc, np.int32)
+        self._L.orc_read_bed_lookup(self._h, _i(out))
         return out
 
     def read_pressure(self):

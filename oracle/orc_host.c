@@ -334,7 +334,7 @@ static void emit_disk(orc_sim* S, uint32_t slot0, uint32_t count, float dt, floa
 /* InflowState::emit_particles (inflow.rs:122-241) plus the fixed-count mode. */
 void orc_emit(orc_sim* S, float dt) {
   float mass = 80.0f / 320.0f; /* MASS_UNITS_PER_ML / PARTICLES_PER_ML (mod.rs:278) */
-  uint32_t total = S->num_water;
+  uint32_t total = S->num_water + S->bed_n; /* current_water + current_bed (inflow.rs:155) */
   uint32_t avail = S->s.max_particles > total ? S->s.max_particles - total : 0u;
   if (S->s.emitter_mode == CS_EMIT_FIXED_COUNT) {
     uint32_t want = S->s.emit_per_substep;
@@ -345,7 +345,7 @@ void orc_emit(orc_sim* S, float dt) {
     uint32_t count = want < avail ? want : avail;
     S->total_dropped += want - count;
     if (count == 0) return;
-    emit_disk(S, total, count, dt, S->s.emit_speed, S->s.emit_jet_radius, mass);
+    emit_disk(S, S->num_water, count, dt, S->s.emit_speed, S->s.emit_jet_radius, mass); /* the water array */
     S->inflow.emitted_total += count;
     S->num_water += count;
     S->total_emitted_mass += (float)count * mass;
@@ -360,7 +360,7 @@ void orc_emit(orc_sim* S, float dt) {
   S->inflow.accumulator -= (float)requested;
   uint32_t count = requested < avail ? requested : avail;
   if (count == 0) { S->total_dropped += requested; return; }
-  emit_disk(S, total, count, dt, S->inflow.exit_speed, S->s.spout.nozzle_radius * 0.58f, mass);
+  emit_disk(S, S->num_water, count, dt, S->inflow.exit_speed, S->s.spout.nozzle_radius * 0.58f, mass);
   S->num_water += count;
   S->total_emitted_mass += (float)count * mass;
   S->total_dropped += requested - count;
@@ -445,6 +445,17 @@ orc_sim* orc_create(const cs_settings* s, const cs_scene* scene) {
     S->num_bed = scene->num_bed_records;
     S->bed = (cs_bed_record*)malloc((size_t)S->num_bed * sizeof(cs_bed_record) + 1);
     memcpy(S->bed, scene->bed_records, (size_t)S->num_bed * sizeof(cs_bed_record));
+  } else if (scene && scene->num_bed_particles && scene->bed_particles && scene->bed_affine && scene->bed_extracts) {
+    uint32_t n = scene->num_bed_particles;
+    S->bed_n = n;
+    S->num_bed = n;
+    S->bed = (cs_bed_record*)calloc((size_t)n + 1, sizeof(cs_bed_record));
+    S->bed_part = (float*)malloc((size_t)n * 8 * sizeof(float));
+    S->bed_aff = (float*)malloc((size_t)n * 12 * sizeof(float));
+    S->bed_ext = (float*)malloc((size_t)n * 8 * sizeof(float));
+    memcpy(S->bed_part, scene->bed_particles, (size_t)n * 8 * sizeof(float));
+    memcpy(S->bed_aff, scene->bed_affine, (size_t)n * 12 * sizeof(float));
+    memcpy(S->bed_ext, scene->bed_extracts, (size_t)n * 8 * sizeof(float));
   }
   if (scene && scene->particles && scene->num_particles) {
     uint32_t n = scene->num_particles;
@@ -461,7 +472,7 @@ orc_sim* orc_create(const cs_settings* s, const cs_scene* scene) {
 
 void orc_destroy(orc_sim* S) {
   if (!S) return;
-  free(S->sdf); free(S->cls); free(S->bed_lookup); free(S->bed);
+  free(S->sdf); free(S->cls); free(S->bed_lookup); free(S->bed); free(S->bed_part); free(S->bed_aff); free(S->bed_ext);
   free(S->particles); free(S->affine); free(S->init_particles); free(S->init_affine);
   free(S->grid); free(S->p2g_lanes); free(S->grid_vel); free(S->prev_p); free(S->warm);
   free(S->saved_p); free(S->saved_kind);
@@ -517,7 +528,7 @@ int orc_set_particle_count(orc_sim* S, uint32_t n) {
 }
 void orc_get_counts(orc_sim* S, uint32_t* w, uint32_t* b, uint32_t* m) {
   if (w) *w = S->num_water;
-  if (b) *b = 0;
+  if (b) *b = S->bed_n;
   if (m) *m = S->s.max_particles;
 }
 void orc_get_time(orc_sim* S, float* t, float* m, uint32_t* d) {

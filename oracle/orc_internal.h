@@ -100,6 +100,12 @@ struct orc_sim {
   int32_t* bed_lookup;
   cs_bed_record* bed;
   uint32_t num_bed;
+  /* a bed of particles (reference mode): the reference's leading particles,
+   * kept apart; S->bed then holds one record per bed particle */
+  uint32_t bed_n;
+  float* bed_part; /* 8 per bed particle */
+  float* bed_aff;  /* 12 per bed particle */
+  float* bed_ext;  /* BedExtract, 8 per bed particle */
   float* particles; /* 8 per particle */
   float* affine;    /* 12 per particle */
   float* init_particles;

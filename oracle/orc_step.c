@@ -699,6 +699,9 @@ void orc_read_system(orc_sim* S, float* w6o, float* cnto, float* co) {
   }
 }
 
+void orc_read_bed_lookup(orc_sim* S, int32_t* lookup) {
+  memcpy(lookup, S->bed_lookup, S->nc * sizeof(int32_t));
+}
 void orc_read_pressure(orc_sim* S, int32_t* p, int32_t* kind) {
   /* lane 0 keeps the pressure to the end of the substep; the kind lane is
    * overwritten by the viscosity scratch (shader.rs:274-283), so the copy
@@ -1050,6 +1053,45 @@ static void k_g2p(orc_sim* S, const orc_params* u) {
   }
 }
 
+/* ---- bed of particles (reference mode) ------------------------------------- */
+/* bed_lookup_clear and bed_lookup_scatter (shader.rs:2291-2335): every bed
+ * particle that is not water stamps its index into the 3x3x3 cells around
+ * world_to_cell (shader.rs:668-671); the highest index wins (atomicMax).  Then
+ * each particle's record is read from its BedExtract (shader.rs:35-38: porosity,
+ * permeability, compaction in bed.yzw, saturation in extract.w) and from the
+ * particle itself (height, velocity), as the cell readers of shader.rs:361-449
+ * and 621-666 take them. */
+static void bed_restamp(orc_sim* S, const orc_params* u) {
+  for (size_t i = 0; i < S->nc; ++i) S->bed_lookup[i] = -1;
+  for (uint32_t bid = 0; bid < S->bed_n; ++bid) {
+    const float* p = S->bed_part + 8 * (size_t)bid;
+    if (S->bed_aff[12 * (size_t)bid + 3] < 0.5f) continue; /* is_water_phase */
+    int c[3];
+    for (int k = 0; k < 3; ++k) c[k] = (int)floorf((p[k] - u->origin[k]) * u->inv_dx);
+    for (int di = -1; di <= 1; ++di)
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int dk = -1; dk <= 1; ++dk) {
+          int x = c[0] + di, y = c[1] + dj, z = c[2] + dk;
+          if (x < 0 || y < 0 || z < 0 || x >= (int)S->gx || y >= (int)S->gy || z >= (int)S->gz) continue;
+          int32_t* slot = &S->bed_lookup[cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z)];
+          if ((int32_t)bid > *slot) *slot = (int32_t)bid;
+        }
+  }
+  for (uint32_t bid = 0; bid < S->bed_n; ++bid) {
+    const float* p = S->bed_part + 8 * (size_t)bid;
+    const float* e = S->bed_ext + 8 * (size_t)bid;
+    cs_bed_record* r = &S->bed[bid];
+    r->porosity = e[1];
+    r->permeability = e[2];
+    r->compaction = e[3];
+    r->saturation = e[7];
+    r->sample_y = p[1];
+    r->velocity[0] = p[4];
+    r->velocity[1] = p[5];
+    r->velocity[2] = p[6];
+  }
+}
+
 /* ---- one substep in step_frame order (mod.rs:271-418) ---------------------- */
 static int substep(orc_sim* S, float sub_dt, int stop_after, int emit) {
   if (emit) orc_emit(S, sub_dt);
@@ -1058,6 +1100,7 @@ static int substep(orc_sim* S, float sub_dt, int stop_after, int emit) {
   memset(S->grid, 0, S->nc * 6 * sizeof(int32_t));
   memset(S->grid_vel, 0, S->nc * 4 * sizeof(float));
   memset(S->metrics, 0, sizeof(S->metrics));
+  if (S->bed_n) bed_restamp(S, u); /* mod.rs:322-331 */
   k_p2g(S, u);
   memcpy(S->p2g_lanes, S->grid, S->nc * 6 * sizeof(int32_t));
   if (stop_after == ORC_PASS_P2G) return 0;
