@@ -545,8 +545,8 @@ class MpmSim3D:
         return out
 
     def pack_render(self) -> np.ndarray:
-        """prepare_render's RenderParticle per particle in use: (n, 8) float32."""
-        out = np.empty((self.water_slots_used(), 8), np.float32)
+        """prepare_render's RenderParticle per particle in use, bed particles first: (n, 8) float32."""
+        out = np.empty((self.particle_count(), 8), np.float32)
         _check(load_library().cs_sim_pack_render(self._h, _fptr(out)), "cs_sim_pack_render")
         return out
 

@@ -40,9 +40,10 @@ hipError_t cs_launch_visc(hipStream_t stream, const cs_params& u, const float* g
                           const float* sdf);
 hipError_t cs_launch_g2p(hipStream_t stream, const cs_params& u, float* part, float* aff, const float* gvel,
                          const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf);
-// prepare_render (shader.rs:2242-2276): 8 floats per particle into `out`.
+// prepare_render (shader.rs:2242-2276): 8 floats per particle into `out`; particles
+// below num_bed are bed particles, with their BedExtracts in bed_ext (may be null).
 hipError_t cs_launch_pack_render(hipStream_t stream, const cs_params& u, const float* part, const float* aff,
-                                 uint32_t num_bed, float* out);
+                                 uint32_t num_bed, const float* bed_ext, float* out);
 hipError_t cs_launch_expf(hipStream_t stream, const float* in, float* out, uint32_t n);
 hipError_t cs_launch_sincosf(hipStream_t stream, const float* in, float* sn, float* cs, uint32_t n);
 // The scene's SDF (sdf_res^3 floats) and class mask (one byte per cell) from the

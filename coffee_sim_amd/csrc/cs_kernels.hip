@@ -591,11 +591,12 @@ __global__ void __launch_bounds__(256) k_g2p(cs_params u, float4* __restrict__ p
 
 // ---- prepare_render (shader.rs:2242-2276) --------------------------------------------
 // RenderParticle (shader.rs:40-43) per particle: (pos.xyz, colour) and (radius,
-// 0, 0, 0); an inactive particle is parked below the scene.  Bed particles
-// (pid < num_bed) would colour by saturation; field-mode beds have none, so
-// their saturation term is 0.
+// 0, 0, 0); an inactive particle is parked below the scene.  A bed particle
+// (pid < num_bed) takes the bed radius and colours by -1 - saturation, the
+// saturation read from its BedExtract (extract.w) when `bed_ext` is given.
 __global__ void __launch_bounds__(256) k_pack_render(cs_params u, const float4* __restrict__ part,
                                                      const float4* __restrict__ aff, uint32_t num_bed,
+                                                     const float* __restrict__ bed_ext,
                                                      float4* __restrict__ out) {
   const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
   if (pid >= u.num_particles) return;
@@ -612,7 +613,10 @@ __global__ void __launch_bounds__(256) k_pack_render(cs_params u, const float4* 
     color_t = cs_clamp(speed / 10.0f, 0.0f, 2.0f);
   } else {
     float sat = 0.0f;
-    if (pid < num_bed) radius = u.dx * CS_BED_SAMPLE_RADIUS_DX;
+    if (pid < num_bed) {
+      if (bed_ext) sat = bed_ext[8 * (size_t)pid + 7];
+      radius = u.dx * CS_BED_SAMPLE_RADIUS_DX;
+    }
     color_t = -1.0f - sat;
   }
   out[2 * pid] = make_float4(P.x, P.y, P.z, color_t);
@@ -956,10 +960,10 @@ hipError_t cs_launch_g2p(hipStream_t s, const cs_params& u, float* part, float* 
   return hipGetLastError();
 }
 hipError_t cs_launch_pack_render(hipStream_t s, const cs_params& u, const float* part, const float* aff,
-                                 uint32_t num_bed, float* out) {
+                                 uint32_t num_bed, const float* bed_ext, float* out) {
   if (u.num_particles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_pack_render, blocks_for(u.num_particles), dim3(kBlock), 0, s, u,
-                     reinterpret_cast<const float4*>(part), reinterpret_cast<const float4*>(aff), num_bed,
+                     reinterpret_cast<const float4*>(part), reinterpret_cast<const float4*>(aff), num_bed, bed_ext,
                      reinterpret_cast<float4*>(out));
   return hipGetLastError();
 }

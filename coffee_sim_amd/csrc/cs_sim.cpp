@@ -964,9 +964,15 @@ int cs_sim_pack_render(cs_sim* m, float* out) {
   (void)hipSetDevice(m->ctx->device);
   if (!m->render) CS_HIP(dalloc(&m->render, (size_t)m->s.max_particles * 8));
   cs_params u;
-  cs::make_params(m->s, m->num_water, m->num_bed_cells, 1.0e-4f, &u);  // dt does not enter the packing
-  CS_HIP(cs_launch_pack_render(m->ctx->stream, u, m->part, m->aff, 0u, m->render));
-  return read_dev(m, out, m->render, (size_t)m->num_water * 32);
+  // the reference's order: bed particles first, then the water (mod.rs:239-249)
+  if (m->num_bed) {
+    cs::make_params(m->s, m->num_bed, m->num_bed_cells, 1.0e-4f, &u);  // dt does not enter the packing
+    CS_HIP(cs_launch_pack_render(m->ctx->stream, u, m->bed_part, m->bed_aff, m->num_bed, m->bed_ext, m->render));
+  }
+  cs::make_params(m->s, m->num_water, m->num_bed_cells, 1.0e-4f, &u);
+  CS_HIP(cs_launch_pack_render(m->ctx->stream, u, m->part, m->aff, 0u, nullptr,
+                               m->render + (size_t)m->num_bed * 8));
+  return read_dev(m, out, m->render, ((size_t)m->num_bed + m->num_water) * 32);
 }
 
 int cs_sim_get_solve_stats(cs_sim* m, cs_solve_stats* out) {

@@ -355,7 +355,8 @@ int cs_sim_read_metrics(cs_sim* sim, uint32_t* slots8);
  * exposed through render_buffer, mod.rs:500-502): the RenderParticle layout
  * (shader.rs:40-43), 8 floats per particle, (pos.xyz, colour) and (radius, 0,
  * 0, 0), for every particle slot in use.  Packed on demand from the current
- * particle state; `render8` holds particle_count x 8 floats. */
+ * particle state; `render8` holds particle_count x 8 floats, a bed of
+ * particles first (as the reference's buffer holds it), then the water. */
 int cs_sim_pack_render(cs_sim* sim, float* render8);
 int cs_sim_get_solve_stats(cs_sim* sim, cs_solve_stats* out);
 

@@ -560,10 +560,13 @@ void orc_read_metrics(orc_sim* S, uint32_t* m) { memcpy(m, S->metrics, sizeof(S-
 void orc_pack_render(orc_sim* S, float* out) {
   orc_params u;
   orc_write_uniforms(S, 1.0e-4f, &u);  /* dt does not enter the packing */
-  const uint32_t n = u.num_water + u.num_bed_particles;
+  /* the reference's particle order: a bed of particles first, then the water */
+  const uint32_t nb = S->bed_n;
+  const uint32_t n = nb + u.num_water + u.num_bed_particles;
   for (uint32_t pid = 0; pid < n; ++pid) {
-    const float* p = &S->particles[(size_t)pid * 8];
-    const float phase = S->affine[(size_t)pid * 12 + 3];
+    const int bed = pid < nb;
+    const float* p = bed ? &S->bed_part[(size_t)pid * 8] : &S->particles[(size_t)(pid - nb) * 8];
+    const float phase = bed ? S->bed_aff[(size_t)pid * 12 + 3] : S->affine[(size_t)(pid - nb) * 12 + 3];
     float* o = &out[(size_t)pid * 8];
     if (p[7] <= u.nominal_mass * 0.10f) {
       const float park[8] = {0.0f, -1e6f, 0.0f, -999.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -575,8 +578,13 @@ void orc_pack_render(orc_sim* S, float* out) {
       const float speed = sqrtf(p[4] * p[4] + p[5] * p[5] + p[6] * p[6]);
       color_t = o_clamp(speed / 10.0f, 0.0f, 2.0f);
     } else {
-      float sat = 0.0f;  /* no bed particles, hence no extraction state, in field mode */
-      if (pid < u.num_bed_particles) radius = u.dx * 0.62f;
+      float sat = 0.0f;  /* bed_extract[pid].extract.w; a field bed has no extraction state */
+      if (bed) {
+        sat = S->bed_ext[(size_t)pid * 8 + 7];
+        radius = u.dx * 0.62f;
+      } else if (pid < u.num_bed_particles) {
+        radius = u.dx * 0.62f;
+      }
       color_t = -1.0f - sat;
     }
     o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = color_t;

@@ -203,6 +203,24 @@ def brute_force_stamp(s, bed):
     return look.reshape(-1).astype(np.int32)
 
 
+def test_oracle_render_packs_the_bed_first(orc):
+    """prepare_render over the reference's particle order (sh:2242-2276): the
+    bed particles first, at radius 0.62 dx with colour -1 - saturation, then
+    the water."""
+    s, sc, bed = bed_scene(substeps=1, n_water=500)
+    bed["extracts"][:, 7] = np.linspace(0.0, 1.0, len(bed["extracts"]), dtype=np.float32)
+    cs.attach_bed(sc, bed)
+    ref = orc.OracleSim(s, sc)
+    out = ref.pack_render()
+    nb = len(bed["particles"])
+    assert out.shape == (nb + 500, 8)
+    dx = np.float32(np.float32(s.bounds_size[0]) / np.float32(s.grid_dims[0]))
+    assert_bitwise("bed positions", out[:nb, 0:3], bed["particles"][:, 0:3])
+    assert_bitwise("bed colour", out[:nb, 3], (np.float32(-1.0) - bed["extracts"][:, 7]).astype(np.float32))
+    assert (out[:nb, 4] == np.float32(dx * np.float32(0.62))).all()
+    assert (out[nb:, 3] >= 0.0).all() and (out[nb:, 3] <= 2.0).all()
+
+
 def test_oracle_stamp_matches_brute_force(orc):
     s, sc, bed = bed_scene(substeps=1)
     ref = orc.OracleSim(s, sc)
@@ -259,6 +277,12 @@ def test_bed_particles_match_oracle(ctx, orc):
         ref.step_frame(1.0 / 300.0)
         assert_bitwise(f"bed lookup frame {k}", sim.read_bed_lookup(), ref.read_bed_lookup())
         compare_state(sim, ref, f"seated bed frame {k}")
+    # prepare_render: bed particles first, radius 0.62 dx, colour -1 - saturation
+    rg, ro = sim.pack_render(), ref.pack_render()
+    assert rg.shape == (sim.counts()[0] + len(bed["particles"]), 8) == (sim.particle_count(), 8)
+    assert_bitwise("render", rg, ro)
+    nb = len(bed["particles"])
+    assert (rg[:nb, 3] == -1.0).all() and (rg[nb:, 3] >= 0.0).all()
     _, kind = sim.read_pressure()
     assert (kind == 3).sum() > 100
 
