@@ -84,6 +84,7 @@ def _bind(lib: C.CDLL) -> None:
         "cs_sim_read_metrics": (C.c_int, [C.c_void_p, P(C.c_uint32)]),
         "cs_sim_read_sdf": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_uint8)]),
         "cs_sim_read_bed_lookup": (C.c_int, [C.c_void_p, P(C.c_int32)]),
+        "cs_sim_read_bed_extracts": (C.c_int, [C.c_void_p, P(C.c_float)]),
         "cs_filter_config_default": (None, [P(abi.FilterConfig)]),
         "cs_bed_config_default": (None, [P(abi.BedConfig)]),
         "cs_bed_config_seated_in_filter": (None, [P(abi.FilterConfig), P(abi.BedConfig)]),
@@ -527,6 +528,12 @@ class MpmSim3D:
         out = np.empty(self.nc, np.int32)
         _check(load_library().cs_sim_read_bed_lookup(self._h, out.ctypes.data_as(C.POINTER(C.c_int32))),
                "cs_sim_read_bed_lookup")
+        return out
+
+    def read_bed_extracts(self) -> np.ndarray:
+        """The bed particles' BedExtracts after the last substep: (n_bed, 8) float32."""
+        out = np.zeros((self.counts()[1], 8), np.float32)
+        _check(load_library().cs_sim_read_bed_extracts(self._h, _fptr(out)), "cs_sim_read_bed_extracts")
         return out
 
     def read_sdf(self) -> tuple[np.ndarray, np.ndarray]:

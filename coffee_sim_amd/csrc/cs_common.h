@@ -30,6 +30,10 @@
 // water_particle_radius / bed_particle_radius, shader.rs:127-128).
 #define CS_WATER_SAMPLE_RADIUS_DX 0.18f
 #define CS_BED_SAMPLE_RADIUS_DX 0.62f
+// DEFAULT_BREW (brew_config.rs:59-60) and extraction_params.x (mod.rs:645)
+#define CS_BED_ABSORPTION_RATE 1.6f
+#define CS_TARGET_BED_RETENTION_ML 42.0f
+#define CS_EXTRACTION_RATE 0.01f
 
 // Cell kinds (shader.rs:174-178).
 #define CS_CELL_AIR 0

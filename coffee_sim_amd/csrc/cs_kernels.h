@@ -56,6 +56,10 @@ hipError_t cs_launch_gen_sdf_class(hipStream_t stream, const cs_settings& s, con
 hipError_t cs_launch_bed_lookup(hipStream_t st, const cs_params& u, uint32_t nbed, const float* bed_part,
                                 const float* bed_aff, const float* bed_ext, uint32_t capacity, int32_t* lookup,
                                 uint32_t* counter, int32_t* bed_id, cs_bed_cell* table);
+// bed_coupling over the water particles, then extraction_advect over the nbed bed
+// particles (shader.rs:1885-1987); dw holds one i32 water credit per bed particle.
+hipError_t cs_launch_bed_coupling(hipStream_t st, const cs_params& u, float* part, const float* aff,
+                                  const int32_t* lookup, uint32_t nbed, float* bed_ext, int32_t* dw);
 // The particles of one substep's emission (f.count of them at slots f.first..)
 // and their zero affine records.
 hipError_t cs_launch_emit(hipStream_t stream, const cs_emit_frame& f, float* part, float* aff);

@@ -697,6 +697,116 @@ __global__ void __launch_bounds__(256) k_bed_table(cs_params u, const int32_t* _
   }
 }
 
+// ---- bed_coupling and extraction_advect (shader.rs:1885-1987) -----------------------------
+// bed_water_delta_add (shader.rs:455-460): fixed-point water credit of one bed particle.
+__device__ __forceinline__ void bed_water_add(const cs_params& u, int32_t* __restrict__ dw, int32_t idx, float amount) {
+  if (amount <= 0.0f) return;
+  atomicAdd(&dw[idx], cs_f2i(amount * u.fp_scale));
+}
+
+// bed_coupling over the water: a particle in a stamped cell gives part of its
+// mass to the stamping bed particle's pore water, 35 % of it shared among the
+// distinct bed particles of the six face neighbours (deposit_absorbed_bed_water,
+// shader.rs:524-576).  The filter-paper branch (shader.rs:1904-1911) needs
+// filter_absorption_rate > 0; the reference sets it to 0 (brew_config.rs:68),
+// so it is not carried.  Every credit is an integer atomic and each particle
+// reads only the bed state of the previous extraction pass, so the result does
+// not depend on the order.
+__global__ void __launch_bounds__(256) k_bed_coupling(cs_params u, float4* __restrict__ part,
+                                                      const float4* __restrict__ aff,
+                                                      const int32_t* __restrict__ lookup, uint32_t nbed,
+                                                      const float* __restrict__ bed_ext, float max_sat,
+                                                      int32_t* __restrict__ dw) {
+  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= u.num_particles) return;
+  if (!(aff[3 * pid].w < 0.5f)) return;  // is_water_phase
+  const float4 P = part[2 * pid];
+  float4 V = part[2 * pid + 1];
+  auto cell = [&](float p, float o) {
+    const float g = floorf((p - o) * u.inv_dx);
+    return (int)fminf(fmaxf(g, -2147483648.0f), 2147483520.0f);
+  };
+  const int cx = cell(P.x, u.origin[0]), cy = cell(P.y, u.origin[1]), cz = cell(P.z, u.origin[2]);
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)u.gx || cy >= (int)u.gy || cz >= (int)u.gz) return;
+  const float mass_p = V.w;
+  const int32_t home = lookup[((uint32_t)cz * u.gy + (uint32_t)cy) * u.gx + (uint32_t)cx];
+  if (home < 0 || (uint32_t)home >= nbed) return;
+  const float pore = bed_ext[8 * (size_t)home + 0], saturation = bed_ext[8 * (size_t)home + 7];
+  const float capacity = cs_max(max_sat - pore, 0.0f);
+  if (capacity <= 1e-6f) return;
+  const float abs_rate = CS_BED_ABSORPTION_RATE * (1.0f - saturation) * u.dt;
+  const float speed = len3(f3(V.x, V.y, V.z));
+  const float inactive = u.nominal_mass * 0.10f;
+  float absorbed = cs_min(cs_min(mass_p * cs_clamp(abs_rate, 0.0f, 0.25f), mass_p * 0.5f), capacity);
+  const float remaining_after_partial = mass_p - absorbed;
+  if (remaining_after_partial > 0.0f && remaining_after_partial <= u.nominal_mass * 0.22f) {
+    absorbed = cs_min(mass_p, capacity);
+  } else if (saturation > 0.55f && speed < 1.35f) {
+    const float almost = cs_min(mass_p, capacity);
+    if (mass_p - almost <= u.nominal_mass * 0.35f) absorbed = almost;
+  }
+  if (mass_p - absorbed <= inactive && absorbed < mass_p) absorbed = cs_max(mass_p - inactive * 1.05f, 0.0f);
+  if (absorbed <= 1e-6f) return;
+  const float remaining = mass_p - absorbed;
+  part[2 * pid + 1] = remaining <= inactive ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
+                                            : make_float4(V.x, V.y, V.z, remaining);
+  // deposit_absorbed_bed_water
+  const float neighbor_share = absorbed * 0.35f;
+  const float home_share = absorbed - neighbor_share;
+  bed_water_add(u, dw, home, home_share);
+  int32_t ids[6];
+  uint32_t n = 0;
+  const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  for (int k = 0; k < 6; ++k) {
+    const int x = cx + off[k][0], y = cy + off[k][1], z = cz + off[k][2];
+    if (x < 0 || y < 0 || z < 0 || x >= (int)u.gx || y >= (int)u.gy || z >= (int)u.gz) continue;
+    const int32_t id = lookup[((uint32_t)z * u.gy + (uint32_t)y) * u.gx + (uint32_t)x];
+    if (id < 0 || id == home || (uint32_t)id >= nbed) continue;
+    bool dup = false;
+    for (uint32_t j = 0; j < n; ++j) dup = dup || ids[j] == id;
+    if (!dup) ids[n++] = id;
+  }
+  if (n == 0) {
+    bed_water_add(u, dw, home, neighbor_share);
+    return;
+  }
+  const float each = neighbor_share / (float)n;
+  for (uint32_t j = 0; j < n; ++j) bed_water_add(u, dw, ids[j], each);
+}
+
+// extraction_advect: each bed particle takes (and clears) its water credit into
+// its pore water and saturation, then moves extractable to dissolved mass.
+__global__ void __launch_bounds__(256) k_extraction_advect(cs_params u, uint32_t nbed, float max_sat,
+                                                           int32_t* __restrict__ dw, float* __restrict__ bed_ext) {
+  const uint32_t bid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bid >= nbed) return;
+  float* be = bed_ext + 8 * (size_t)bid;
+  const float absorbed = (float)dw[bid] * u.inv_fp_scale;
+  dw[bid] = 0;
+  if (absorbed > 0.0f) {
+    be[0] += absorbed;
+    be[7] = cs_clamp(be[0] / cs_max(max_sat, 1e-6f), 0.0f, 1.0f);
+  }
+  const float sat = be[7];
+  if (sat > 0.01f) {
+    const float flux = CS_EXTRACTION_RATE * be[4] * sat * u.dt;
+    be[4] = cs_max(be[4] - flux, 0.0f);
+    be[5] += flux;
+  }
+}
+
+hipError_t cs_launch_bed_coupling(hipStream_t st, const cs_params& u, float* part, const float* aff,
+                                  const int32_t* lookup, uint32_t nbed, float* bed_ext, int32_t* dw) {
+  if (nbed == 0) return hipSuccess;
+  const float max_sat = CS_TARGET_BED_RETENTION_ML * 80.0f / (float)nbed;  // mod.rs:577-581
+  if (u.num_particles)
+    hipLaunchKernelGGL(k_bed_coupling, dim3((u.num_particles + 255) / 256), dim3(256), 0, st, u,
+                       reinterpret_cast<float4*>(part), reinterpret_cast<const float4*>(aff), lookup, nbed, bed_ext,
+                       max_sat, dw);
+  hipLaunchKernelGGL(k_extraction_advect, dim3((nbed + 255) / 256), dim3(256), 0, st, u, nbed, max_sat, dw, bed_ext);
+  return hipGetLastError();
+}
+
 hipError_t cs_launch_bed_lookup(hipStream_t st, const cs_params& u, uint32_t nbed, const float* bed_part,
                                 const float* bed_aff, const float* bed_ext, uint32_t capacity, int32_t* lookup,
                                 uint32_t* counter, int32_t* bed_id, cs_bed_cell* table) {

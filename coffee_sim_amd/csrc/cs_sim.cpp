@@ -62,7 +62,7 @@ void free_sim(cs_sim* m) {
                  m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,
                  m->brick_flag, m->brick_list, m->brick_partials, m->dstate,       m->tile_count,
                  m->tile_start, m->p2g_key,    m->p2g_sorted, m->render,     m->bed_part,
-                 m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter};
+                 m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -90,6 +90,11 @@ int upload_initial(cs_sim* m) {
     CS_HIP(hipMemcpyAsync(m->aff, m->init_affine.data(), (size_t)m->init_count * 48, hipMemcpyHostToDevice, st));
   }
   CS_HIP(hipMemsetAsync(m->prev_p, 0, m->nc * sizeof(int32_t), st));
+  if (m->num_bed) {  // the bed's extraction state starts over; the bed particles never move here
+    CS_HIP(hipMemcpyAsync(m->bed_ext, m->init_bed_ext.data(), m->init_bed_ext.size() * sizeof(float),
+                          hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemsetAsync(m->bed_dw, 0, (size_t)m->num_bed * sizeof(int32_t), st));
+  }
   CS_HIP(hipStreamSynchronize(st));
   return CS_OK;
 }
@@ -528,6 +533,9 @@ int substep(cs_sim* m, float sub_dt) {
   mark(m, P_G2P);
   kmark(m, 2);
   CS_HIP(cs_launch_g2p(st, u, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf));
+  // bed_coupling and extraction_advect (mod.rs:391-402)
+  if (m->num_bed)
+    CS_HIP(cs_launch_bed_coupling(st, u, m->part, m->aff, m->bed_lookup, m->num_bed, m->bed_ext, m->bed_dw));
   kmark(m, 3);
   mark(m, P_END);
   if (m->timing) {
@@ -696,6 +704,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
     ALLOC(m->bed_ext, (size_t)m->num_bed * 8);
     ALLOC(m->bed_lookup, m->nc);
     ALLOC(m->bed_counter, 1);
+    ALLOC(m->bed_dw, m->num_bed);
   }
   ALLOC(m->kind, m->nc);
   ALLOC(m->kf, m->nc);
@@ -786,6 +795,8 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
       e = hipMemcpyAsync(m->bed_aff, scene->bed_affine, (size_t)m->num_bed * 48, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
       e = hipMemcpyAsync(m->bed_ext, scene->bed_extracts, (size_t)m->num_bed * 32, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(m->bed_dw, 0, (size_t)m->num_bed * sizeof(int32_t), st);
+    m->init_bed_ext.assign(scene->bed_extracts, scene->bed_extracts + (size_t)m->num_bed * 8);
   }
   if (e == hipSuccess) e = hipMemsetAsync(m->gv, 0, m->nc * 16, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->gv2, 0, m->nc * 16, st);
@@ -954,6 +965,11 @@ int cs_sim_read_bed_lookup(cs_sim* m, int32_t* lookup) {
     return CS_OK;
   }
   return read_dev(m, lookup, m->bed_lookup, m->nc * sizeof(int32_t));
+}
+int cs_sim_read_bed_extracts(cs_sim* m, float* extracts) {
+  if (!m || !extracts) return fail(CS_ERR_INVALID, "null argument");
+  if (!m->num_bed) return CS_OK;
+  return read_dev(m, extracts, m->bed_ext, (size_t)m->num_bed * 32);
 }
 int cs_sim_read_metrics(cs_sim* m, uint32_t* slots) {
   if (!m) return fail(CS_ERR_INVALID, "null sim");

@@ -36,6 +36,8 @@ struct cs_sim {
   float* bed_ext = nullptr;         // BedExtract, 8 floats per bed particle
   int32_t* bed_lookup = nullptr;    // per cell: highest stamping bed particle, or -1
   uint32_t* bed_counter = nullptr;  // table slots taken this substep
+  int32_t* bed_dw = nullptr;        // per bed particle: fixed-point water credit of bed_coupling
+  std::vector<float> init_bed_ext;  // the scene's BedExtracts, restored by reset
   int8_t* kind = nullptr;
   float* kf = nullptr;
   int32_t* pres = nullptr;

@@ -349,6 +349,12 @@ int cs_sim_read_sdf(cs_sim* sim, float* sdf, uint8_t* sdf_class);
  * particle id, or -1), as bed_lookup_scatter leaves it (shader.rs:2304-2335).
  * A simulation without bed particles reads all -1. */
 int cs_sim_read_bed_lookup(cs_sim* sim, int32_t* lookup);
+/* The bed particles' BedExtracts (8 floats each) as bed_coupling and
+ * extraction_advect (shader.rs:1885-1987) leave them after the most recent
+ * substep: pore water and saturation grow as the bed absorbs water, extractable
+ * mass moves to dissolved.  Nothing is written for a simulation without bed
+ * particles. */
+int cs_sim_read_bed_extracts(cs_sim* sim, float* extracts8);
 /* Metric slots (state.rs:22-28). */
 int cs_sim_read_metrics(cs_sim* sim, uint32_t* slots8);
 /* prepare_render (shader.rs:2242-2276, run by step_frame at mod.rs:410-412 and

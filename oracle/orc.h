@@ -74,6 +74,8 @@ void orc_read_pressure(orc_sim* sim, int32_t* p, int32_t* kind);
 /* The bed lookup: per cell the highest stamping bed particle (reference mode),
  * the scene's lookup (field mode) or -1. */
 void orc_read_bed_lookup(orc_sim* sim, int32_t* lookup);
+/* The bed particles' BedExtracts, 8 floats each (reference mode). */
+void orc_read_bed_extracts(orc_sim* sim, float* ext);
 void orc_read_metrics(orc_sim* sim, uint32_t* slots8);
 /* prepare_render (shader.rs:2242-2276): RenderParticle, 8 floats per particle in use. */
 void orc_pack_render(orc_sim* sim, float* render8);

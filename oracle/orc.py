@@ -74,6 +74,7 @@ def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
             "orc_read_grid_vel": (None, [C.c_void_p, f32p]),
             "orc_read_pressure": (None, [C.c_void_p, i32p, i32p]),
             "orc_read_bed_lookup": (None, [C.c_void_p, i32p]),
+            "orc_read_bed_extracts": (None, [C.c_void_p, f32p]),
             "orc_read_metrics": (None, [C.c_void_p, P(C.c_uint32)]),
             "orc_pack_render": (None, [C.c_void_p, f32p]),
             "orc_get_solve_stats": (None, [C.c_void_p, P(SolveStats)]),
@@ -219,6 +220,11 @@ class OracleSim:
     def read_grid_vel(self):
         out = np.empty((self.nc, 4), np.float32)
         self._L.orc_read_grid_vel(self._h, _f(out))
+        return out
+
+    def read_bed_extracts(self):
+        out = np.zeros((self.counts()[1], 8), np.float32)
+        self._L.orc_read_bed_extracts(self._h, _f(out))
         return out
 
     def read_bed_lookup(self):

@@ -456,6 +456,9 @@ orc_sim* orc_create(const cs_settings* s, const cs_scene* scene) {
     memcpy(S->bed_part, scene->bed_particles, (size_t)n * 8 * sizeof(float));
     memcpy(S->bed_aff, scene->bed_affine, (size_t)n * 12 * sizeof(float));
     memcpy(S->bed_ext, scene->bed_extracts, (size_t)n * 8 * sizeof(float));
+    S->bed_ext0 = (float*)malloc((size_t)n * 8 * sizeof(float));
+    memcpy(S->bed_ext0, scene->bed_extracts, (size_t)n * 8 * sizeof(float));
+    S->bed_dw = (int32_t*)calloc((size_t)n, sizeof(int32_t));
   }
   if (scene && scene->particles && scene->num_particles) {
     uint32_t n = scene->num_particles;
@@ -472,7 +475,7 @@ orc_sim* orc_create(const cs_settings* s, const cs_scene* scene) {
 
 void orc_destroy(orc_sim* S) {
   if (!S) return;
-  free(S->sdf); free(S->cls); free(S->bed_lookup); free(S->bed); free(S->bed_part); free(S->bed_aff); free(S->bed_ext);
+  free(S->sdf); free(S->cls); free(S->bed_lookup); free(S->bed); free(S->bed_part); free(S->bed_aff); free(S->bed_ext); free(S->bed_ext0); free(S->bed_dw);
   free(S->particles); free(S->affine); free(S->init_particles); free(S->init_affine);
   free(S->grid); free(S->p2g_lanes); free(S->grid_vel); free(S->prev_p); free(S->warm);
   free(S->saved_p); free(S->saved_kind);
@@ -495,6 +498,10 @@ int orc_reset(orc_sim* S) {
     S->num_water = S->init_count;
   }
   memset(S->prev_p, 0, S->nc * sizeof(int32_t));
+  if (S->bed_n) {
+    memcpy(S->bed_ext, S->bed_ext0, (size_t)S->bed_n * 8 * sizeof(float));
+    memset(S->bed_dw, 0, (size_t)S->bed_n * sizeof(int32_t));
+  }
   S->has_warm = 0;
   memset(&S->stats, 0, sizeof(S->stats));
   memset(S->metrics, 0, sizeof(S->metrics));

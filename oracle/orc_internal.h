@@ -106,6 +106,8 @@ struct orc_sim {
   float* bed_part; /* 8 per bed particle */
   float* bed_aff;  /* 12 per bed particle */
   float* bed_ext;  /* BedExtract, 8 per bed particle */
+  float* bed_ext0; /* the scene's BedExtracts, restored by reset */
+  int32_t* bed_dw; /* fixed-point water credit per bed particle (bed_delta lane 0) */
   float* particles; /* 8 per particle */
   float* affine;    /* 12 per particle */
   float* init_particles;

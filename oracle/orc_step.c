@@ -699,6 +699,9 @@ void orc_read_system(orc_sim* S, float* w6o, float* cnto, float* co) {
   }
 }
 
+void orc_read_bed_extracts(orc_sim* S, float* ext) {
+  if (S->bed_n) memcpy(ext, S->bed_ext, (size_t)S->bed_n * 8 * sizeof(float));
+}
 void orc_read_bed_lookup(orc_sim* S, int32_t* lookup) {
   memcpy(lookup, S->bed_lookup, S->nc * sizeof(int32_t));
 }
@@ -1092,6 +1095,96 @@ static void bed_restamp(orc_sim* S, const orc_params* u) {
   }
 }
 
+/* bed_water_delta_add (shader.rs:455-460) */
+static void bed_water_add(orc_sim* S, const orc_params* u, int32_t idx, float amount) {
+  if (amount <= 0.0f) return;
+  S->bed_dw[idx] += o_f2i(amount * u->fp_scale);
+}
+
+/* bed_coupling (shader.rs:1885-1957) with deposit_absorbed_bed_water
+ * (shader.rs:524-576).  The filter-paper branch (shader.rs:1904-1911) runs
+ * only for filter_absorption_rate > 0, which DEFAULT_BREW sets to 0
+ * (brew_config.rs:68); it is left out. */
+static void bed_coupling(orc_sim* S, const orc_params* u) {
+  const uint32_t nbed = S->bed_n;
+  const float max_sat = 42.0f * 80.0f / (float)nbed; /* mod.rs:577-581 */
+  const float inactive = u->nominal_mass * 0.10f;
+  for (uint32_t pid = 0; pid < u->num_water; ++pid) {
+    float* p = S->particles + 8 * (size_t)pid;
+    if (!(S->affine[12 * (size_t)pid + 3] < 0.5f)) continue;
+    int c[3];
+    for (int k = 0; k < 3; ++k) c[k] = (int)floorf((p[k] - u->origin[k]) * u->inv_dx);
+    if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= (int)S->gx || c[1] >= (int)S->gy || c[2] >= (int)S->gz) continue;
+    const float mass_p = p[7];
+    const int32_t home = S->bed_lookup[cidx(u, (uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2])];
+    if (home < 0 || (uint32_t)home >= nbed) continue;
+    const float* be = S->bed_ext + 8 * (size_t)home;
+    const float saturation = be[7];
+    const float capacity = o_max(max_sat - be[0], 0.0f);
+    if (capacity <= 1e-6f) continue;
+    const float abs_rate = 1.6f * (1.0f - saturation) * u->dt; /* bed_absorption_rate, brew_config.rs:59 */
+    const float speed = sqrtf(p[4] * p[4] + p[5] * p[5] + p[6] * p[6]);
+    float absorbed = o_min(o_min(mass_p * o_clamp(abs_rate, 0.0f, 0.25f), mass_p * 0.5f), capacity);
+    const float rest = mass_p - absorbed;
+    if (rest > 0.0f && rest <= u->nominal_mass * 0.22f) {
+      absorbed = o_min(mass_p, capacity);
+    } else if (saturation > 0.55f && speed < 1.35f) {
+      const float almost = o_min(mass_p, capacity);
+      if (mass_p - almost <= u->nominal_mass * 0.35f) absorbed = almost;
+    }
+    if (mass_p - absorbed <= inactive && absorbed < mass_p) absorbed = o_max(mass_p - inactive * 1.05f, 0.0f);
+    if (absorbed <= 1e-6f) continue;
+    const float remaining = mass_p - absorbed;
+    if (remaining <= inactive) {
+      p[4] = p[5] = p[6] = p[7] = 0.0f;
+    } else {
+      p[7] = remaining;
+    }
+    const float neighbor_share = absorbed * 0.35f;
+    bed_water_add(S, u, home, absorbed - neighbor_share);
+    static const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    int32_t ids[6];
+    uint32_t n = 0;
+    for (int k = 0; k < 6; ++k) {
+      int x = c[0] + off[k][0], y = c[1] + off[k][1], z = c[2] + off[k][2];
+      if (x < 0 || y < 0 || z < 0 || x >= (int)S->gx || y >= (int)S->gy || z >= (int)S->gz) continue;
+      int32_t id = S->bed_lookup[cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z)];
+      if (id < 0 || id == home || (uint32_t)id >= nbed) continue;
+      int dup = 0;
+      for (uint32_t j = 0; j < n; ++j) if (ids[j] == id) dup = 1;
+      if (!dup) ids[n++] = id;
+    }
+    if (n == 0) {
+      bed_water_add(S, u, home, neighbor_share);
+      continue;
+    }
+    const float each = neighbor_share / (float)n;
+    for (uint32_t j = 0; j < n; ++j) bed_water_add(S, u, ids[j], each);
+  }
+}
+
+/* extraction_advect (shader.rs:1961-1987): the water credit into pore water and
+ * saturation, then extractable to dissolved mass at extraction_params.x = 0.01
+ * (mod.rs:645). */
+static void extraction_advect(orc_sim* S, const orc_params* u) {
+  const float max_sat = 42.0f * 80.0f / (float)S->bed_n;
+  for (uint32_t bid = 0; bid < S->bed_n; ++bid) {
+    float* be = S->bed_ext + 8 * (size_t)bid;
+    const float absorbed = (float)S->bed_dw[bid] * u->inv_fp_scale;
+    S->bed_dw[bid] = 0;
+    if (absorbed > 0.0f) {
+      be[0] += absorbed;
+      be[7] = o_clamp(be[0] / o_max(max_sat, 1e-6f), 0.0f, 1.0f);
+    }
+    const float sat = be[7];
+    if (sat > 0.01f) {
+      const float flux = 0.01f * be[4] * sat * u->dt;
+      be[4] = o_max(be[4] - flux, 0.0f);
+      be[5] += flux;
+    }
+  }
+}
+
 /* ---- one substep in step_frame order (mod.rs:271-418) ---------------------- */
 static int substep(orc_sim* S, float sub_dt, int stop_after, int emit) {
   if (emit) orc_emit(S, sub_dt);
@@ -1123,6 +1216,11 @@ static int substep(orc_sim* S, float sub_dt, int stop_after, int emit) {
   k_boundary(S, u);
   if (stop_after == ORC_PASS_VISCOSITY) return 0;
   k_g2p(S, u);
+  if (stop_after == ORC_PASS_G2P) return 0;
+  if (S->bed_n) { /* mod.rs:391-402 */
+    bed_coupling(S, u);
+    extraction_advect(S, u);
+  }
   S->total_time += sub_dt;
   return 0;
 }

@@ -221,6 +221,44 @@ def test_oracle_render_packs_the_bed_first(orc):
     assert (out[nb:, 3] >= 0.0).all() and (out[nb:, 3] <= 2.0).all()
 
 
+def test_oracle_bed_coupling_conserves_water(orc):
+    """bed_coupling moves water mass from the particles into pore water and
+    extraction_advect credits it (shader.rs:1885-1987): over one substep of a
+    pour onto the seated bed, the mass the particles lose after G2P equals the
+    pore water gained, short only of the fixed-point truncation of each credit
+    (at most 7 credits of 2^-18 per absorbing particle); extractable plus
+    dissolved mass stays 0.15 per bed particle; saturation is pore water over
+    the per-particle capacity 42 ml * 80 / n_bed, clamped to 1."""
+    s, sc, bed = bed_scene(substeps=1)
+    s.emitter_mode = cs.abi.CS_EMIT_FIXED_COUNT  # no emission: the stopped copy emits nothing
+    s.emit_per_substep = 0
+    sub_dt = 1.0 / 600.0
+    a, b = orc.OracleSim(s, sc), orc.OracleSim(s, sc)
+    for k in range(3):  # both advance together; a stops before the coupling in the last substep
+        if k < 2:
+            a.step_frame(sub_dt)
+        b.step_frame(sub_dt)
+    e0 = a.read_bed_extracts()
+    a.debug_step_until(sub_dt, "g2p")
+    m_after_g2p = a.read_particles()[:, 7].astype(np.float64)
+    assert_bitwise("state before coupling", a.read_bed_extracts(), e0)
+    e1 = b.read_bed_extracts()
+    m_after = b.read_particles()[:, 7].astype(np.float64)
+    lost = m_after_g2p - m_after
+    absorbing = int((lost > 0).sum())
+    gained = float((e1[:, 0].astype(np.float64) - e0[:, 0]).sum())
+    assert absorbing > 50 and (lost >= 0).all()
+    slack = absorbing * 7 * 2.0 ** -18
+    assert 0.0 <= lost.sum() - gained <= slack + 1e-6 * lost.sum(), (lost.sum(), gained, slack)
+    n = len(e1)
+    cap = np.float32(np.float32(42.0 * 80.0) / np.float32(n))
+    wet = e1[:, 0] > 0
+    want_sat = np.clip(e1[wet, 0] / cap, 0.0, 1.0).astype(np.float32)
+    assert np.abs(e1[wet, 7] - want_sat).max() <= 1e-6
+    assert np.abs(e1[:, 4].astype(np.float64) + e1[:, 5] - 0.15).max() <= 1e-6
+    assert (e1[wet, 5] > 0).any()  # extraction started where the bed is wet
+
+
 def test_oracle_stamp_matches_brute_force(orc):
     s, sc, bed = bed_scene(substeps=1)
     ref = orc.OracleSim(s, sc)
@@ -232,12 +270,13 @@ def test_oracle_stamp_matches_brute_force(orc):
 
 
 def test_oracle_reference_mode_equals_field_mode_with_the_same_records(orc):
-    """A bed of particles is a field bed whose lookup is the stamp and whose
-    records are the BedExtracts' porosity, permeability, compaction and
-    saturation with the particles' height and velocity."""
+    """Up to G2P, a bed of particles is a field bed whose lookup is the stamp
+    and whose records are the BedExtracts' porosity, permeability, compaction
+    and saturation with the particles' height and velocity (bed_coupling,
+    which only particles have, follows G2P)."""
     s, sc, bed = bed_scene(substeps=1)
     ref = orc.OracleSim(s, sc)
-    ref.step_frame(1.0 / 600.0)
+    ref.debug_step_until(1.0 / 600.0, "g2p")
     e, p = bed["extracts"], bed["particles"]
     rec = np.zeros((len(p), 8), np.float32)
     rec[:, 0], rec[:, 1], rec[:, 2], rec[:, 3] = e[:, 1], e[:, 2], e[:, 3], e[:, 7]
@@ -252,7 +291,7 @@ def test_oracle_reference_mode_equals_field_mode_with_the_same_records(orc):
     sc2.bed_records = rec.ctypes.data_as(C.POINTER(cs.BedRecord))
     sc2.num_bed_records = len(rec)
     field = orc.OracleSim(s2, sc2)
-    field.step_frame(1.0 / 600.0)
+    field.debug_step_until(1.0 / 600.0, "g2p")
     for name in ("read_particles", "read_affine", "read_grid_vel", "read_p2g_lanes"):
         assert_bitwise(name, getattr(ref, name)(), getattr(field, name)())
     pr, kr = ref.read_pressure()
@@ -277,12 +316,16 @@ def test_bed_particles_match_oracle(ctx, orc):
         ref.step_frame(1.0 / 300.0)
         assert_bitwise(f"bed lookup frame {k}", sim.read_bed_lookup(), ref.read_bed_lookup())
         compare_state(sim, ref, f"seated bed frame {k}")
+        assert_bitwise(f"bed extracts frame {k}", sim.read_bed_extracts(), ref.read_bed_extracts())
+    assert (sim.read_bed_extracts()[:, 0] > 0).sum() > 100  # the bed took up water
     # prepare_render: bed particles first, radius 0.62 dx, colour -1 - saturation
     rg, ro = sim.pack_render(), ref.pack_render()
     assert rg.shape == (sim.counts()[0] + len(bed["particles"]), 8) == (sim.particle_count(), 8)
     assert_bitwise("render", rg, ro)
     nb = len(bed["particles"])
-    assert (rg[:nb, 3] == -1.0).all() and (rg[nb:, 3] >= 0.0).all()
+    sat = sim.read_bed_extracts()[:, 7]
+    assert_bitwise("bed colour", rg[:nb, 3], (np.float32(-1.0) - sat).astype(np.float32))
+    assert (sat > 0).any() and (rg[nb:, 3] >= 0.0).all()
     _, kind = sim.read_pressure()
     assert (kind == 3).sum() > 100
 
