@@ -6,7 +6,9 @@
  * (P2G, grid update, boundary projection, classify, red-black Gauss-Seidel,
  * projection, packing, viscosity, G2P), the pass order of MpmSim3D::step_frame
  * (mod.rs:264-430), the uniform values of write_uniforms (mod.rs:569-670), the
- * scene generators of state.rs:206-344 and the emitter of inflow.rs:109-293.
+ * scene generators of state.rs:206-344, the emitter of inflow.rs:109-293 and,
+ * for a bed of particles, bed_lookup_clear / bed_lookup_scatter, bed_coupling
+ * and extraction_advect (shader.rs:1885-1987, 2291-2335).
  * Every function cites the reference lines it follows.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
