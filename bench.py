@@ -362,7 +362,7 @@ def bench_learned(args) -> int:
         log(f"training pour {k + 1}/{args.train_pours}: {net.dataset_size()} pairs, {time.perf_counter() - t0:.0f}s")
     losses = []
     for e in range(args.epochs):
-        losses.append(net.train_epoch(2e-3 * 0.1 ** (e / max(args.epochs - 1, 1)), seed=e))
+        losses.append(net.train_epoch(args.lr * 0.1 ** (e / max(args.epochs - 1, 1)), seed=e))
         log(f"epoch {e + 1}/{args.epochs}: loss {losses[-1]:.4f}, {time.perf_counter() - t0:.0f}s")
     train_s = time.perf_counter() - t0
     net.clear_dataset()
@@ -424,7 +424,7 @@ def bench_learned(args) -> int:
         "config": {
             "workload": "v60_working after the pour: learned warm start (U-Net trained on device from solver "
                         "pairs) against the previous-step and zero starts from identical states",
-            "train_pours": args.train_pours, "epochs": args.epochs, "train_seconds": round(train_s, 1),
+            "train_pours": args.train_pours, "epochs": args.epochs, "lr": args.lr, "train_seconds": round(train_s, 1),
             "timed_start": head, "iterations": its, "iterations_total": tot, "fell_back": fell,
             "iterations_ratio_over_timed_start": ratios,
             "ms_per_substep": {k: round(1000.0 * v / args.steps, 2) for k, v in secs.items()},
@@ -449,6 +449,8 @@ def main() -> int:
                     default="v60_working")
     ap.add_argument("--train-pours", type=int, default=3, help="learned128: pours whose solves train the network")
     ap.add_argument("--epochs", type=int, default=30, help="learned128: training epochs")
+    ap.add_argument("--lr", type=float, default=2e-3,
+                    help="learned128: Adam's first-epoch rate, decayed 10x over the epochs")
     ap.add_argument("--rescale-after", default="96",
                     help="learned128: comma-separated rbgs_rescale_after values whose starts are also measured "
                          "(0 or empty: none)")
