@@ -111,25 +111,19 @@ def roofline(kt) -> dict:
     }
 
 
-def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, gpu_iters: float, budget_s: float,
+def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, prev_p: np.ndarray, budget_s: float,
                  threads: int) -> dict:
     """The CPU oracle (the C restatement of the reference solver; with
     threads > 1 its OpenMP build, whose cell and particle loops run in
-    parallel) on the same post-pour state.
-
-    One substep solved to 1e-4 takes thousands of RBGS iterations, minutes of
-    CPU time at this size, so the sample is bounded: the oracle's own
-    tolerance-mode substep is timed from the same state with the iteration cap
-    at two values that it reaches before converging.  Substep time is linear in
-    the iteration count (a fixed cost for P2G, grid passes and G2P plus a cost
-    per iteration that includes the residual checks), and the two samples give
-    both terms; the baseline is that line evaluated at the GPU's measured mean
-    iterations per substep (identical inputs give identical iteration counts).
-    """
+    parallel) continuing the GPU run: the same particles and the same
+    previous-step pressure, whole tolerance-mode substeps (RBGS to 1e-4 from
+    the previous-step start, as the GPU runs them) until about `budget_s`
+    seconds have passed.  The value is particles x substeps / elapsed time of
+    that measured window; nothing is extrapolated."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import orc  # checker only: times the CPU restatement beside the GPU
 
-    from coffee_sim_amd.abi import Scene
+    from coffee_sim_amd.abi import CS_WARM_PREVIOUS, Scene
 
     n = particles.shape[0]
     p = np.ascontiguousarray(particles, np.float32)
@@ -141,44 +135,32 @@ def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, gpu_iters: float,
     sc.particles = p.ctypes.data_as(C.POINTER(C.c_float))
     sc.affine = a.ctypes.data_as(C.POINTER(C.c_float))
     sc.num_particles = n
-
-    def timed(cap: int) -> tuple[float, int]:
-        s = w.settings.copy()
-        s.emit_per_substep = 0
-        s.emit_total = 0
-        s.rbgs_max_iters = cap
-        ref = orc.OracleSim(s, sc, parallel=threads > 1, threads=threads)
-        t0 = time.perf_counter()
+    s = w.settings.copy()
+    s.emit_per_substep = 0
+    s.emit_total = 0
+    s.warm_start = CS_WARM_PREVIOUS
+    ref = orc.OracleSim(s, sc, parallel=threads > 1, threads=threads)
+    ref.set_previous_pressure(prev_p)
+    iters = []
+    t0 = time.perf_counter()
+    el = 0.0
+    while el < budget_s and len(iters) < 40:
         ref.step_frame(w.frame_dt)
         el = time.perf_counter() - t0
-        it = ref.solve_stats().iterations
-        ref.close()
-        log(f"cpu oracle substep with iteration cap {cap}: {el:.2f}s, {it} iterations")
-        return el, it
-
-    lo_cap = 16
-    t_lo, it_lo = timed(lo_cap)
-    # spend about the budget on the second sample
-    per_iter_guess = max(t_lo / (lo_cap + 8), 1e-4)
-    hi_cap = int(min(max(lo_cap + 64, budget_s / per_iter_guess), 4096)) // 8 * 8
-    t_hi, it_hi = timed(hi_cap)
-    if it_hi <= it_lo:
-        # converged inside the first cap: the sample is a whole substep
-        per_sub, how = t_lo, f"one whole substep ({it_lo} iterations)"
-    else:
-        per_iter = (t_hi - t_lo) / (it_hi - it_lo)
-        fixed = t_lo - it_lo * per_iter
-        per_sub = fixed + gpu_iters * per_iter
-        how = (f"tolerance-mode substeps capped at {it_lo} and {it_hi} iterations ({t_lo:.2f} s, {t_hi:.2f} s): "
-               f"{fixed:.3f} s fixed + {1e3 * per_iter:.3f} ms per iteration, evaluated at "
-               f"{gpu_iters:.1f} iterations")
+        st = ref.solve_stats()
+        iters.append(int(st.iterations))
+        log(f"cpu oracle substep {len(iters)}: {st.iterations} iterations, converged {bool(st.converged)}, "
+            f"{el:.2f}s so far")
+    ref.close()
     who = "single-threaded C oracle" if threads == 1 else f"C oracle, OpenMP build on {threads} threads"
     return {
-        "value": round(n / per_sub, 2),
+        "value": round(n * len(iters) / el, 2),
         "unit": "particle-substeps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"v60_working after the pour ({n} particles), {who}: {how}",
+        "sample": (f"v60_working after the timed GPU substeps ({n} particles), {who}: {len(iters)} whole "
+                   f"substeps from the GPU run's state and previous-step pressure, RBGS to 1e-4 "
+                   f"({np.mean(iters):.0f} iterations per substep on average), {el:.1f} s measured"),
     }
 
 
@@ -458,7 +440,8 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="batch64: default launch shapes, tiled P2G")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--scale", type=int, default=1, help="divide the grid side (tests only)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle sampling (0 = skip)")
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of whole CPU-oracle substeps to time (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="host threads of the CPU baseline (16: this GPU's share of the host; 1: the serial oracle)")
     ap.add_argument("--warm-start", choices=("previous", "zero"), default="previous",
@@ -554,7 +537,8 @@ def main() -> int:
     if args.cpu_budget > 0:
         parts = sim.read_particles(0, n_particles)
         aff = sim.read_affine(0, n_particles)
-        cpu = cpu_baseline(w, parts, aff, float(np.mean(iters_timed)), args.cpu_budget, max(args.cpu_threads, 1))
+        prev_p, _ = sim.read_pressure()
+        cpu = cpu_baseline(w, parts, aff, prev_p, args.cpu_budget, max(args.cpu_threads, 1))
 
     value = psteps / elapsed
     out = {

@@ -7,6 +7,7 @@ arithmetic.  Output: coffee_sim_amd/lib/libcoffee_sim.so.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -50,15 +51,17 @@ def build(verbose: bool = False) -> Path:
     obj_dir = HERE / "_build"
     obj_dir.mkdir(exist_ok=True)
     LIB_DIR.mkdir(exist_ok=True)
-    objs = []
+    jobs = []
     for src in HOST_SOURCES:
-        obj = obj_dir / (src + ".o")
-        _run(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
-        objs.append(obj)
+        jobs.append(["g++", *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(obj_dir / (src + ".o"))])
     for src in DEVICE_SOURCES:
-        obj = obj_dir / (src + ".o")
-        _run([str(ROCM / "bin" / "hipcc"), *DEVICE_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
-        objs.append(obj)
+        jobs.append([str(ROCM / "bin" / "hipcc"), *DEVICE_FLAGS, "-c", str(CSRC / src), "-o",
+                     str(obj_dir / (src + ".o"))])
+    objs = [obj_dir / (src + ".o") for src in HOST_SOURCES + DEVICE_SOURCES]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "8"))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(_run, cmd) for cmd in jobs]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
     _run([str(ROCM / "bin" / "hipcc"), f"--offload-arch={ARCH}", "-shared", "-fPIC",
           *[str(o) for o in objs], "-o", str(tmp), f"-L{ROCM / 'lib'}", "-lamdhip64",

@@ -114,6 +114,7 @@ void fill_static(const cs_batch* b, uint32_t s, cs_scene_dev* d) {
   d->pres2 = m->pres2;
   d->rhs = m->rhs;
   d->pack = m->pack;
+  d->tgt = m->tgt;
   d->prev_p = m->prev_p;
   d->metrics = m->metrics;
   d->wc = m->wc;

@@ -31,6 +31,7 @@ struct cs_scene_dev {
   int32_t* pres2;
   float* rhs;
   int32_t* pack;
+  float* tgt;  // classify's target divergence per cell
   int32_t* prev_p;
   uint32_t* metrics;
   float* wc;

@@ -55,6 +55,11 @@
 #define CS_M_DIV_CLAMP 2
 #define CS_M_P_CLAMP 3
 #define CS_M_OVERFLOW 4
+// Slots the reference leaves free: max|div| and max|div - target| over the fluid
+// cells after project_pressure + boundary_project, and max|div - target| before it.
+#define CS_M_POST_MAX_ABS_DIV 5
+#define CS_M_POST_MAX_ABS_RES 6
+#define CS_M_PRE_MAX_ABS_RES 7
 
 CS_HD float cs_min(float a, float b) { return (a != a) ? b : ((b < a) ? b : a); }
 CS_HD float cs_max(float a, float b) { return (a != a) ? b : ((b > a) ? b : a); }

@@ -22,7 +22,7 @@ hipError_t cs_launch_grid(hipStream_t stream, const cs_params& u, const int32_t*
                           const float* sdf);
 hipError_t cs_launch_classify(hipStream_t stream, const cs_params& u, const float* gvel, const uint8_t* cls,
                               const int32_t* lanes, const int32_t* bed_id, const cs_bed_cell* bed, int8_t* kind,
-                              float* kf, int32_t* pres, float* rhs, int32_t* pack, uint32_t* metrics,
+                              float* kf, int32_t* pres, float* rhs, int32_t* pack, float* tgt, uint32_t* metrics,
                               const float* sdf, uint32_t* brick_flag);
 hipError_t cs_launch_warm_previous(hipStream_t stream, const cs_params& u, const float* kf, const int32_t* prev,
                                    int32_t* pres);
@@ -34,6 +34,10 @@ hipError_t cs_launch_residual(hipStream_t stream, const cs_params& u, const floa
                               const int32_t* pres, double* partials, uint32_t max_blocks, double* out);
 hipError_t cs_launch_project(hipStream_t stream, const cs_params& u, float* gvel, const float* kf,
                              const int32_t* pres, const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf);
+// The divergence of the projected velocity over the fluid cells (metric slots 5
+// and 6), from classify's per-cell target `tgt`.
+hipError_t cs_launch_post_div(hipStream_t stream, const cs_params& u, const float* gvel, const float* kf,
+                              const uint8_t* cls, const float* sdf, const float* tgt, uint32_t* metrics);
 hipError_t cs_launch_pack(hipStream_t stream, const cs_params& u, float* gvel, const float* kf, const int32_t* pack,
                           const float* sdf);
 hipError_t cs_launch_visc(hipStream_t stream, const cs_params& u, const float* gin, float* gout, const uint8_t* cls,

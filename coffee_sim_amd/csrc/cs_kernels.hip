@@ -59,6 +59,29 @@ __global__ void __launch_bounds__(256) k_grid(cs_params u, const int32_t* __rest
   grid_cell(u, lanes, gvel, sdf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// The central-difference divergence of grid_vel at a cell with the ghost
+// mirror on solid and off-grid faces (shader.rs:1278-1335); solidity from the
+// cached class mask or live SDF samples, as use_sdf_cache selects.
+__device__ __forceinline__ float cell_divergence(const cs_params& u, const float4* __restrict__ gvel,
+                                                 const uint8_t* __restrict__ cls, const float* __restrict__ sdf,
+                                                 uint32_t idx, uint32_t ix, uint32_t iy, uint32_t iz, cs_f3 cc) {
+  const bool cache = u.use_sdf_cache != 0u;
+  float4 sv = gvel[idx];
+  float vxm = -sv.x, vxp = -sv.x, vym = -sv.y, vyp = -sv.y, vzm = -sv.z, vzp = -sv.z;
+  uint32_t plane = u.gx * u.gy;
+  float d = u.dx;
+#define CS_OPEN(nidx, px, py, pz) (cache ? !cls[nidx] : sample_sdf(u, sdf, f3(px, py, pz)) >= 0.0f)
+  if (ix > 0u && CS_OPEN(idx - 1u, cc.x + -d, cc.y + 0.0f, cc.z + 0.0f)) vxm = gvel[idx - 1u].x;
+  if (ix + 1u < u.gx && CS_OPEN(idx + 1u, cc.x + d, cc.y + 0.0f, cc.z + 0.0f)) vxp = gvel[idx + 1u].x;
+  if (iy > 0u && CS_OPEN(idx - u.gx, cc.x + 0.0f, cc.y + -d, cc.z + 0.0f)) vym = gvel[idx - u.gx].y;
+  if (iy + 1u < u.gy && CS_OPEN(idx + u.gx, cc.x + 0.0f, cc.y + d, cc.z + 0.0f)) vyp = gvel[idx + u.gx].y;
+  if (iz > 0u && CS_OPEN(idx - plane, cc.x + 0.0f, cc.y + 0.0f, cc.z + -d)) vzm = gvel[idx - plane].z;
+  if (iz + 1u < u.gz && CS_OPEN(idx + plane, cc.x + 0.0f, cc.y + 0.0f, cc.z + d)) vzp = gvel[idx + plane].z;
+#undef CS_OPEN
+  float div = 0.5f * u.inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
+  return div;
+}
+
 // ---- classify_cells (shader.rs:1181-1367) + packing_prepare (shader.rs:1566-1583) ----
 // Writes kind, the kind/fill word, pressure 0, the right-hand side c = dx*dx*rhs
 // of the pressure update (shader.rs:1436-1437, from the quantised divergence)
@@ -69,6 +92,7 @@ __device__ __forceinline__ void classify_cell(const cs_params& u, const float4* 
                                                   const cs_bed_cell* __restrict__ bed, int8_t* __restrict__ kind_out,
                                                   float* __restrict__ kf_out, int32_t* __restrict__ pres,
                                                   float* __restrict__ rhs, int32_t* __restrict__ pack,
+                                                  float* __restrict__ tgt,
                                                   uint32_t* __restrict__ metrics, const float* __restrict__ sdf,
                                                   uint32_t* __restrict__ brick_flag, uint32_t idx) {
   if (idx >= u.nc) return;
@@ -101,19 +125,7 @@ __device__ __forceinline__ void classify_cell(const cs_params& u, const float4* 
     }
     kind = air ? CS_CELL_SURFACE : CS_CELL_INTERIOR;
   }
-  float4 sv = gvel[idx];
-  float vxm = -sv.x, vxp = -sv.x, vym = -sv.y, vyp = -sv.y, vzm = -sv.z, vzp = -sv.z;
-  uint32_t plane = u.gx * u.gy;
-  float d = u.dx;
-#define CS_OPEN(nidx, px, py, pz) (cache ? !cls[nidx] : sample_sdf(u, sdf, f3(px, py, pz)) >= 0.0f)
-  if (ix > 0u && CS_OPEN(idx - 1u, cc.x + -d, cc.y + 0.0f, cc.z + 0.0f)) vxm = gvel[idx - 1u].x;
-  if (ix + 1u < u.gx && CS_OPEN(idx + 1u, cc.x + d, cc.y + 0.0f, cc.z + 0.0f)) vxp = gvel[idx + 1u].x;
-  if (iy > 0u && CS_OPEN(idx - u.gx, cc.x + 0.0f, cc.y + -d, cc.z + 0.0f)) vym = gvel[idx - u.gx].y;
-  if (iy + 1u < u.gy && CS_OPEN(idx + u.gx, cc.x + 0.0f, cc.y + d, cc.z + 0.0f)) vyp = gvel[idx + u.gx].y;
-  if (iz > 0u && CS_OPEN(idx - plane, cc.x + 0.0f, cc.y + 0.0f, cc.z + -d)) vzm = gvel[idx - plane].z;
-  if (iz + 1u < u.gz && CS_OPEN(idx + plane, cc.x + 0.0f, cc.y + 0.0f, cc.z + d)) vzp = gvel[idx + plane].z;
-#undef CS_OPEN
-  float div = 0.5f * u.inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
+  float div = cell_divergence(u, gvel, cls, sdf, idx, ix, iy, iz, cc);
   float rest = (float)lanes[4u * u.nc + idx] * u.inv_fp_scale;
   float cur = (float)lanes[5u * u.nc + idx] * u.inv_fp_scale;
   float target = volume_target_div(u, rest, cur);
@@ -128,6 +140,7 @@ __device__ __forceinline__ void classify_cell(const cs_params& u, const float4* 
     target = (pore - b.solid_fraction * b.solid_div) / b.porosity;
   }
   float dval = div - target;
+  tgt[idx] = target;
   float dcl = cs_clamp(dval, -u.div_clamp, u.div_clamp);
   uint32_t fires_d = dcl != dval ? 1u : 0u;
   int32_t dfp = cs_f2i(dcl * u.fp_scale);
@@ -156,6 +169,7 @@ __device__ __forceinline__ void classify_cell(const cs_params& u, const float4* 
   }
   uint32_t fpd = cs_f2u(cs_clamp(fabsf(div) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f));
   atomicMax(&metrics[CS_M_MAX_ABS_DIV], fpd);
+  atomicMax(&metrics[CS_M_PRE_MAX_ABS_RES], cs_f2u(cs_clamp(fabsf(dval) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f)));
   atomicAdd(&metrics[CS_M_FLUID_CELLS], 1u);
   brick_flag[((iz / CS_BRICK) * u.nby + iy / CS_BRICK) * u.nbx + ix / CS_BRICK] = 1u;
   if (fires_d) atomicAdd(&metrics[CS_M_DIV_CLAMP], 1u);
@@ -167,9 +181,11 @@ __global__ void __launch_bounds__(256) k_classify(cs_params u, const float4* __r
                                                   const cs_bed_cell* __restrict__ bed, int8_t* __restrict__ kind_out,
                                                   float* __restrict__ kf_out, int32_t* __restrict__ pres,
                                                   float* __restrict__ rhs, int32_t* __restrict__ pack,
+                                                  float* __restrict__ tgt,
                                                   uint32_t* __restrict__ metrics, const float* __restrict__ sdf,
                                                   uint32_t* __restrict__ brick_flag) {
-  classify_cell(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, metrics, sdf, brick_flag, blockIdx.x * blockDim.x + threadIdx.x);
+  classify_cell(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, tgt, metrics, sdf, brick_flag,
+                blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- warm starts (injected after classify zeroed the pressure) -------------------
@@ -352,6 +368,38 @@ __global__ void __launch_bounds__(256) k_project(cs_params u, float4* __restrict
                                                  const int32_t* __restrict__ pres, const int32_t* __restrict__ bed_id,
                                                  const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf) {
   project_cell(u, gvel, kf, pres, bed_id, bed, sdf, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---- post-projection divergence (no reference pass) ---------------------------------
+// The classify stencil on the projected velocity of the fluid cells: max|div|
+// and max|div - target| into metric slots 5 and 6.
+__device__ __forceinline__ void post_div_cell(const cs_params& u, const float4* __restrict__ gvel,
+                                              const float* __restrict__ kf, const uint8_t* __restrict__ cls,
+                                              const float* __restrict__ sdf, const float* __restrict__ tgt,
+                                              uint32_t* __restrict__ metrics, uint32_t idx) {
+  uint32_t fd = 0u, fr = 0u;
+  if (idx < u.nc && kf_fluid(kf[idx])) {
+    uint32_t ix, iy, iz;
+    decompose(u, idx, ix, iy, iz);
+    const float div = cell_divergence(u, gvel, cls, sdf, idx, ix, iy, iz, cell_center(u, ix, iy, iz));
+    fd = cs_f2u(cs_clamp(fabsf(div) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f));
+    fr = cs_f2u(cs_clamp(fabsf(div - tgt[idx]) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f));
+  }
+  // every lane of the wavefront takes part: one atomic per wavefront and slot
+  for (int off = 32; off > 0; off >>= 1) {
+    fd = max(fd, (uint32_t)__shfl_xor((int)fd, off));
+    fr = max(fr, (uint32_t)__shfl_xor((int)fr, off));
+  }
+  if ((threadIdx.x & 63u) == 0u) {
+    if (fd) atomicMax(&metrics[CS_M_POST_MAX_ABS_DIV], fd);
+    if (fr) atomicMax(&metrics[CS_M_POST_MAX_ABS_RES], fr);
+  }
+}
+__global__ void __launch_bounds__(256) k_post_div(cs_params u, const float4* __restrict__ gvel,
+                                                  const float* __restrict__ kf, const uint8_t* __restrict__ cls,
+                                                  const float* __restrict__ sdf, const float* __restrict__ tgt,
+                                                  uint32_t* __restrict__ metrics) {
+  post_div_cell(u, gvel, kf, cls, sdf, tgt, metrics, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- packing_apply + boundary_project (shader.rs:1585-1626, 1630-1679) ------------
@@ -956,16 +1004,26 @@ __global__ void __launch_bounds__(256) k_batch_classify(const cs_scene_dev* __re
                                                         const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_CELLS);
   classify_cell(u, reinterpret_cast<const float4*>(d.gv), d.cls, d.lanes, d.bed_id, d.bed, d.kind, d.kf, d.pres,
-                d.rhs, d.pack, d.metrics, d.sdf, d.brick_flag, tid);
+                d.rhs, d.pack, d.tgt, d.metrics, d.sdf, d.brick_flag, tid);
 }
 
-// project then pack on the same cell: each reads only its own cell's velocity
-// and neighbours' pressure, packing pressure and kinds, none of which the
-// other pass writes, so the pair equals the two separate passes.
-__global__ void __launch_bounds__(256) k_batch_project_pack(const cs_scene_dev* __restrict__ sc,
-                                                            const uint32_t* __restrict__ starts, uint32_t n) {
+__global__ void __launch_bounds__(256) k_batch_project(const cs_scene_dev* __restrict__ sc,
+                                                       const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_CELLS);
   project_cell(u, reinterpret_cast<float4*>(d.gv), d.kf, d.pres, d.bed_id, d.bed, d.sdf, tid);
+}
+
+// the post-projection metric reads the neighbours' projected velocities, so
+// packing (which changes them) waits for it
+__global__ void __launch_bounds__(256) k_batch_post_div(const cs_scene_dev* __restrict__ sc,
+                                                        const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_CELLS);
+  post_div_cell(u, reinterpret_cast<const float4*>(d.gv), d.kf, d.cls, d.sdf, d.tgt, d.metrics, tid);
+}
+
+__global__ void __launch_bounds__(256) k_batch_pack(const cs_scene_dev* __restrict__ sc,
+                                                    const uint32_t* __restrict__ starts, uint32_t n) {
+  CS_BATCH_SCENE(CS_SPACE_CELLS);
   pack_cell(u, reinterpret_cast<float4*>(d.gv), d.kf, d.pack, d.sdf, tid);
 }
 
@@ -1013,12 +1071,12 @@ hipError_t cs_launch_grid(hipStream_t s, const cs_params& u, const int32_t* lane
 }
 hipError_t cs_launch_classify(hipStream_t s, const cs_params& u, const float* gvel, const uint8_t* cls,
                               const int32_t* lanes, const int32_t* bed_id, const cs_bed_cell* bed, int8_t* kind,
-                              float* kf, int32_t* pres, float* rhs, int32_t* pack, uint32_t* metrics,
+                              float* kf, int32_t* pres, float* rhs, int32_t* pack, float* tgt, uint32_t* metrics,
                               const float* sdf, uint32_t* brick_flag) {
   hipLaunchKernelGGL(k_fill_i32, dim3((u.nbricks + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                      reinterpret_cast<int32_t*>(brick_flag), (size_t)u.nbricks, 0);
   hipLaunchKernelGGL(k_classify, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<const float4*>(gvel), cls,
-                     lanes, bed_id, bed, kind, kf, pres, rhs, pack, metrics, sdf, brick_flag);
+                     lanes, bed_id, bed, kind, kf, pres, rhs, pack, tgt, metrics, sdf, brick_flag);
   return hipGetLastError();
 }
 hipError_t cs_launch_warm_previous(hipStream_t s, const cs_params& u, const float* kf, const int32_t* prev,
@@ -1049,6 +1107,12 @@ hipError_t cs_launch_project(hipStream_t s, const cs_params& u, float* gvel, con
                              const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf) {
   hipLaunchKernelGGL(k_project, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(gvel), kf, pres,
                      bed_id, bed, sdf);
+  return hipGetLastError();
+}
+hipError_t cs_launch_post_div(hipStream_t s, const cs_params& u, const float* gvel, const float* kf, const uint8_t* cls,
+                              const float* sdf, const float* tgt, uint32_t* metrics) {
+  hipLaunchKernelGGL(k_post_div, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<const float4*>(gvel), kf,
+                     cls, sdf, tgt, metrics);
   return hipGetLastError();
 }
 hipError_t cs_launch_pack(hipStream_t s, const cs_params& u, float* gvel, const float* kf, const int32_t* pack,
@@ -1131,7 +1195,9 @@ hipError_t cs_batch_launch_grid_classify(const cs_batch_launch& b) {
 }
 
 hipError_t cs_batch_launch_post(const cs_batch_launch& b) {
-  CS_BATCH_LAUNCH(k_batch_project_pack, CS_SPACE_CELLS);
+  CS_BATCH_LAUNCH(k_batch_project, CS_SPACE_CELLS);
+  CS_BATCH_LAUNCH(k_batch_post_div, CS_SPACE_CELLS);
+  CS_BATCH_LAUNCH(k_batch_pack, CS_SPACE_CELLS);
   CS_BATCH_LAUNCH(k_batch_visc, CS_SPACE_CELLS);
   CS_BATCH_LAUNCH(k_batch_g2p, CS_SPACE_PARTICLES);
   return hipGetLastError();

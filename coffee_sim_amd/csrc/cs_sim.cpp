@@ -62,7 +62,7 @@ void free_sim(cs_sim* m) {
                  m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,
                  m->brick_flag, m->brick_list, m->brick_partials, m->dstate,       m->tile_count,
                  m->tile_start, m->p2g_key,    m->p2g_sorted, m->render,     m->bed_part,
-                 m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw};
+                 m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -93,7 +93,7 @@ int upload_initial(cs_sim* m) {
   if (m->num_bed) {  // the bed's extraction state starts over; the bed particles never move here
     CS_HIP(hipMemcpyAsync(m->bed_ext, m->init_bed_ext.data(), m->init_bed_ext.size() * sizeof(float),
                           hipMemcpyHostToDevice, st));
-    CS_HIP(hipMemsetAsync(m->bed_dw, 0, (size_t)m->num_bed * sizeof(int32_t), st));
+    CS_HIP(hipMemsetAsync(m->bed_dw, 0, (size_t)m->num_bed * 4 * sizeof(int32_t), st));
   }
   CS_HIP(hipStreamSynchronize(st));
   return CS_OK;
@@ -522,21 +522,22 @@ int substep(cs_sim* m, float sub_dt) {
   CS_HIP(cs_launch_grid(st, u, m->lanes, m->gv, m->sdf));
   mark(m, P_CLASSIFY);
   CS_HIP(cs_launch_classify(st, u, m->gv, m->cls, m->lanes, m->bed_id, m->bed, m->kind, m->kf, m->pres, m->rhs,
-                            m->pack, m->metrics, m->sdf, m->brick_flag));
+                            m->pack, m->tgt, m->metrics, m->sdf, m->brick_flag));
   mark(m, P_SOLVE);
   int rc = solve(m, u);
   if (rc) return rc;
   mark(m, P_POST);
   CS_HIP(cs_launch_project(st, u, m->gv, m->kf, m->pres, m->bed_id, m->bed, m->sdf));
+  CS_HIP(cs_launch_post_div(st, u, m->gv, m->kf, m->cls, m->sdf, m->tgt, m->metrics));
   CS_HIP(cs_launch_pack(st, u, m->gv, m->kf, m->pack, m->sdf));
   CS_HIP(cs_launch_visc(st, u, m->gv, m->gv2, m->cls, m->sdf));
   mark(m, P_G2P);
   kmark(m, 2);
   CS_HIP(cs_launch_g2p(st, u, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf));
-  // bed_coupling and extraction_advect (mod.rs:391-402)
+  kmark(m, 3);
+  // bed_coupling and extraction_advect (mod.rs:391-402), outside the G2P timing
   if (m->num_bed)
     CS_HIP(cs_launch_bed_coupling(st, u, m->part, m->aff, m->bed_lookup, m->num_bed, m->bed_ext, m->bed_dw));
-  kmark(m, 3);
   mark(m, P_END);
   if (m->timing) {
     CS_HIP(hipEventSynchronize(m->ev[P_END]));
@@ -704,13 +705,14 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
     ALLOC(m->bed_ext, (size_t)m->num_bed * 8);
     ALLOC(m->bed_lookup, m->nc);
     ALLOC(m->bed_counter, 1);
-    ALLOC(m->bed_dw, m->num_bed);
+    ALLOC(m->bed_dw, (size_t)m->num_bed * 4);  // bed_delta: water lane, then impulse x/y/z
   }
   ALLOC(m->kind, m->nc);
   ALLOC(m->kf, m->nc);
   ALLOC(m->pres, m->nc);
   ALLOC(m->rhs, m->nc);
   ALLOC(m->pack, m->nc);
+  ALLOC(m->tgt, m->nc);
   ALLOC(m->prev_p, m->nc);
   ALLOC(m->warm, m->nc);
   ALLOC(m->metrics, CS_METRICS_SLOTS);
@@ -795,7 +797,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
       e = hipMemcpyAsync(m->bed_aff, scene->bed_affine, (size_t)m->num_bed * 48, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
       e = hipMemcpyAsync(m->bed_ext, scene->bed_extracts, (size_t)m->num_bed * 32, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemsetAsync(m->bed_dw, 0, (size_t)m->num_bed * sizeof(int32_t), st);
+    if (e == hipSuccess) e = hipMemsetAsync(m->bed_dw, 0, (size_t)m->num_bed * 4 * sizeof(int32_t), st);
     m->init_bed_ext.assign(scene->bed_extracts, scene->bed_extracts + (size_t)m->num_bed * 8);
   }
   if (e == hipSuccess) e = hipMemsetAsync(m->gv, 0, m->nc * 16, st);
@@ -910,7 +912,9 @@ int cs_sim_write_particles(cs_sim* m, uint32_t off, uint32_t count, const float*
 }
 int cs_sim_set_particle_count(cs_sim* m, uint32_t n) {
   if (!m) return fail(CS_ERR_INVALID, "null sim");
-  if (n > m->s.max_particles) return fail(CS_ERR_RANGE, "count past max_particles");
+  // the reference's buffer holds the bed first, then the water (mod.rs:239-249),
+  // so the water may use what the bed leaves of max_particles
+  if (n > m->s.max_particles - m->num_bed) return fail(CS_ERR_RANGE, "count past max_particles less the bed");
   m->num_water = n;
   return CS_OK;
 }
@@ -1017,25 +1021,33 @@ int cs_sim_set_warm_start(cs_sim* m, const float* p) {
   return CS_OK;
 }
 
-// Checkpoint layout: magic, version, counts, inflow, counters, particles, affine, previous pressure.
+// Checkpoint layout: header, then water particles, water affine, previous
+// pressure, then (for a bed of particles) the bed particles, their affine
+// state and BedExtracts, which bed_coupling / extraction_advect / bed_dynamics
+// change every substep.
 struct SaveHeader {
   uint32_t magic, version, num_water, max_particles;
   uint64_t nc;
   cs::Inflow inflow;
   float total_time, total_emitted_mass;
-  uint32_t total_dropped, pad;
+  uint32_t total_dropped, num_bed;
 };
 static const uint32_t kMagic = 0x43534d31u;  // "CSM1"
+static const uint32_t kSaveVersion = 2u;
+
+static size_t save_size(const cs_sim* m, uint32_t num_water) {
+  return sizeof(SaveHeader) + (size_t)num_water * 80 + m->nc * 4 + (size_t)m->num_bed * (32 + 48 + 32);
+}
 
 int cs_sim_save(cs_sim* m, void* buf, size_t cap, size_t* written) {
   if (!m || !written) return fail(CS_ERR_INVALID, "null argument");
-  size_t need = sizeof(SaveHeader) + (size_t)m->num_water * 80 + m->nc * 4;
+  size_t need = save_size(m, m->num_water);
   *written = need;
   if (!buf) return CS_OK;  // size query
   if (cap < need) return fail(CS_ERR_RANGE, "checkpoint buffer too small");
   SaveHeader h{};
   h.magic = kMagic;
-  h.version = 1;
+  h.version = kSaveVersion;
   h.num_water = m->num_water;
   h.max_particles = m->s.max_particles;
   h.nc = m->nc;
@@ -1043,6 +1055,7 @@ int cs_sim_save(cs_sim* m, void* buf, size_t cap, size_t* written) {
   h.total_time = m->total_time;
   h.total_emitted_mass = m->total_emitted_mass;
   h.total_dropped = m->total_dropped;
+  h.num_bed = m->num_bed;
   char* p = static_cast<char*>(buf);
   memcpy(p, &h, sizeof(h));
   p += sizeof(h);
@@ -1051,7 +1064,16 @@ int cs_sim_save(cs_sim* m, void* buf, size_t cap, size_t* written) {
   p += (size_t)m->num_water * 32;
   if ((rc = read_dev(m, p, m->aff, (size_t)m->num_water * 48))) return rc;
   p += (size_t)m->num_water * 48;
-  return read_dev(m, p, m->prev_p, m->nc * 4);
+  if ((rc = read_dev(m, p, m->prev_p, m->nc * 4))) return rc;
+  p += m->nc * 4;
+  if (m->num_bed) {
+    if ((rc = read_dev(m, p, m->bed_part, (size_t)m->num_bed * 32))) return rc;
+    p += (size_t)m->num_bed * 32;
+    if ((rc = read_dev(m, p, m->bed_aff, (size_t)m->num_bed * 48))) return rc;
+    p += (size_t)m->num_bed * 48;
+    if ((rc = read_dev(m, p, m->bed_ext, (size_t)m->num_bed * 32))) return rc;
+  }
+  return CS_OK;
 }
 
 int cs_sim_load(cs_sim* m, const void* buf, size_t size) {
@@ -1059,11 +1081,11 @@ int cs_sim_load(cs_sim* m, const void* buf, size_t size) {
   if (size < sizeof(SaveHeader)) return fail(CS_ERR_INVALID, "checkpoint truncated");
   SaveHeader h;
   memcpy(&h, buf, sizeof(h));
-  if (h.magic != kMagic || h.version != 1) return fail(CS_ERR_INVALID, "not a checkpoint");
-  if (h.nc != m->nc || h.max_particles != m->s.max_particles || h.num_water > m->s.max_particles)
-    return fail(CS_ERR_INVALID, "checkpoint does not match this simulation's settings");
-  size_t need = sizeof(SaveHeader) + (size_t)h.num_water * 80 + m->nc * 4;
-  if (size < need) return fail(CS_ERR_INVALID, "checkpoint truncated");
+  if (h.magic != kMagic || h.version != kSaveVersion) return fail(CS_ERR_INVALID, "not a checkpoint of this version");
+  if (h.nc != m->nc || h.max_particles != m->s.max_particles || h.num_bed != m->num_bed)
+    return fail(CS_ERR_INVALID, "checkpoint does not match this simulation's settings or bed");
+  if (h.num_water > m->s.max_particles - m->num_bed) return fail(CS_ERR_RANGE, "checkpoint water count past capacity");
+  if (size < save_size(m, h.num_water)) return fail(CS_ERR_INVALID, "checkpoint truncated");
   const char* p = static_cast<const char*>(buf) + sizeof(h);
   hipStream_t st = m->ctx->stream;
   CS_HIP(hipMemsetAsync(m->part, 0, (size_t)m->s.max_particles * 32, st));
@@ -1073,6 +1095,16 @@ int cs_sim_load(cs_sim* m, const void* buf, size_t size) {
   CS_HIP(hipMemcpyAsync(m->aff, p, (size_t)h.num_water * 48, hipMemcpyHostToDevice, st));
   p += (size_t)h.num_water * 48;
   CS_HIP(hipMemcpyAsync(m->prev_p, p, m->nc * 4, hipMemcpyHostToDevice, st));
+  p += m->nc * 4;
+  if (m->num_bed) {
+    CS_HIP(hipMemcpyAsync(m->bed_part, p, (size_t)m->num_bed * 32, hipMemcpyHostToDevice, st));
+    p += (size_t)m->num_bed * 32;
+    CS_HIP(hipMemcpyAsync(m->bed_aff, p, (size_t)m->num_bed * 48, hipMemcpyHostToDevice, st));
+    p += (size_t)m->num_bed * 48;
+    CS_HIP(hipMemcpyAsync(m->bed_ext, p, (size_t)m->num_bed * 32, hipMemcpyHostToDevice, st));
+    // water credits and reaction impulses are consumed within a substep
+    CS_HIP(cs_launch_fill_i32(st, m->bed_dw, (size_t)m->num_bed * 4, 0));
+  }
   CS_HIP(hipStreamSynchronize(st));
   m->num_water = h.num_water;
   m->inflow = h.inflow;

@@ -43,6 +43,7 @@ struct cs_sim {
   int32_t* pres = nullptr;
   float* rhs = nullptr;
   int32_t* pack = nullptr;
+  float* tgt = nullptr;      // classify's target divergence per fluid cell
   int32_t* prev_p = nullptr;
   float* warm = nullptr;
   float* render = nullptr;   // RenderParticle per particle slot, packed on demand

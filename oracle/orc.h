@@ -105,6 +105,22 @@ int orc_debug_step_until(orc_sim* sim, float sub_dt, int stop_after_pass);
 #define ORC_PASS_PACKING 6
 #define ORC_PASS_VISCOSITY 7
 #define ORC_PASS_G2P 8
+#define ORC_PASS_BOUNDARY 9
+#define ORC_PASS_POST_DIVERGENCE 10
+/* One pass on the current state (uniforms of a substep of sub_dt, nothing
+ * cleared): ORC_PASS_RBGS runs one red+black pair, PACKING and VISCOSITY their
+ * prepare + apply, PROJECT the projection alone. */
+int orc_debug_run_pass(orc_sim* sim, float sub_dt, int pass);
+/* The pressure a CS_WARM_PREVIOUS solve starts from (fixed point, one value per
+ * cell), e.g. a GPU run's last solve, so the oracle continues that run. */
+void orc_set_previous_pressure(orc_sim* sim, const int32_t* p);
+/* Overwrite / read the working grid state, for the pass tests: grid_vel (v.xyz,
+ * mass per cell) and the six i32 lanes as the passes see them (after grid
+ * update: pressure, divergence, packing pressure, kind, rest and current volume). */
+void orc_debug_set_grid_vel(orc_sim* sim, const float* vel4);
+void orc_debug_set_grid_lanes(orc_sim* sim, const int32_t* lanes6);
+void orc_debug_read_grid_lanes(orc_sim* sim, int32_t* lanes6);
+void orc_debug_clear_metrics(orc_sim* sim);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,12 @@ def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
             "orc_debug_set_pressure": (None, [C.c_void_p, i32p]),
             "orc_read_system": (None, [C.c_void_p, f32p, f32p, f32p]),
             "orc_debug_step_until": (C.c_int, [C.c_void_p, f32, C.c_int]),
+            "orc_debug_run_pass": (C.c_int, [C.c_void_p, f32, C.c_int]),
+            "orc_debug_set_grid_vel": (None, [C.c_void_p, f32p]),
+            "orc_debug_set_grid_lanes": (None, [C.c_void_p, i32p]),
+            "orc_debug_read_grid_lanes": (None, [C.c_void_p, i32p]),
+            "orc_debug_clear_metrics": (None, [C.c_void_p]),
+            "orc_set_previous_pressure": (None, [C.c_void_p, i32p]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -100,7 +106,8 @@ def _i(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
 
 
-PASS = {"p2g": 1, "grid": 2, "classify": 3, "rbgs": 4, "project": 5, "packing": 6, "viscosity": 7, "g2p": 8}
+PASS = {"p2g": 1, "grid": 2, "classify": 3, "rbgs": 4, "project": 5, "packing": 6, "viscosity": 7, "g2p": 8,
+        "boundary": 9, "post_divergence": 10}
 
 
 def preset(name: str) -> Settings:
@@ -156,6 +163,33 @@ class OracleSim:
 
     def debug_step_until(self, sub_dt: float, stop_after: str):
         self._L.orc_debug_step_until(self._h, sub_dt, PASS[stop_after])
+
+    def debug_run_pass(self, sub_dt: float, name: str):
+        """One pass on the current state (nothing cleared): the pass tests' hook."""
+        assert self._L.orc_debug_run_pass(self._h, sub_dt, PASS[name]) == 0
+
+    def debug_set_grid_vel(self, vel4: np.ndarray) -> None:
+        v = np.ascontiguousarray(vel4, np.float32).reshape(-1)
+        assert v.size == 4 * self.nc
+        self._L.orc_debug_set_grid_vel(self._h, _f(v))
+
+    def debug_set_grid_lanes(self, lanes6: np.ndarray) -> None:
+        a = np.ascontiguousarray(lanes6, np.int32).reshape(-1)
+        assert a.size == 6 * self.nc
+        self._L.orc_debug_set_grid_lanes(self._h, _i(a))
+
+    def debug_read_grid_lanes(self) -> np.ndarray:
+        out = np.empty((6, self.nc), np.int32)
+        self._L.orc_debug_read_grid_lanes(self._h, _i(out))
+        return out
+
+    def set_previous_pressure(self, p_fixed: np.ndarray) -> None:
+        p = np.ascontiguousarray(p_fixed, dtype=np.int32).reshape(-1)
+        assert p.size == self.nc
+        self._L.orc_set_previous_pressure(self._h, _i(p))
+
+    def debug_clear_metrics(self) -> None:
+        self._L.orc_debug_clear_metrics(self._h)
 
     def set_kettle_angle(self, a):
         self._L.orc_set_kettle_angle(self._h, a)

@@ -430,7 +430,8 @@ orc_sim* orc_create(const cs_settings* s, const cs_scene* scene) {
   S->warm = (float*)calloc(S->nc, sizeof(float));
   S->saved_p = (int32_t*)calloc(S->nc, sizeof(int32_t));
   S->saved_kind = (int32_t*)calloc(S->nc, sizeof(int32_t));
-  if (!S->saved_p || !S->saved_kind ||!S->sdf || !S->cls || !S->bed_lookup || !S->particles || !S->affine || !S->grid ||
+  S->tgt = (float*)calloc(S->nc, sizeof(float));
+  if (!S->tgt || !S->saved_p || !S->saved_kind ||!S->sdf || !S->cls || !S->bed_lookup || !S->particles || !S->affine || !S->grid ||
       !S->p2g_lanes || !S->grid_vel || !S->prev_p || !S->warm) {
     orc_destroy(S);
     return NULL;
@@ -458,7 +459,7 @@ orc_sim* orc_create(const cs_settings* s, const cs_scene* scene) {
     memcpy(S->bed_ext, scene->bed_extracts, (size_t)n * 8 * sizeof(float));
     S->bed_ext0 = (float*)malloc((size_t)n * 8 * sizeof(float));
     memcpy(S->bed_ext0, scene->bed_extracts, (size_t)n * 8 * sizeof(float));
-    S->bed_dw = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+    S->bed_dw = (int32_t*)calloc((size_t)n * 4, sizeof(int32_t)); /* bed_delta lanes: water, impulse xyz */
   }
   if (scene && scene->particles && scene->num_particles) {
     uint32_t n = scene->num_particles;
@@ -478,7 +479,7 @@ void orc_destroy(orc_sim* S) {
   free(S->sdf); free(S->cls); free(S->bed_lookup); free(S->bed); free(S->bed_part); free(S->bed_aff); free(S->bed_ext); free(S->bed_ext0); free(S->bed_dw);
   free(S->particles); free(S->affine); free(S->init_particles); free(S->init_affine);
   free(S->grid); free(S->p2g_lanes); free(S->grid_vel); free(S->prev_p); free(S->warm);
-  free(S->saved_p); free(S->saved_kind);
+  free(S->saved_p); free(S->saved_kind); free(S->tgt);
   free(S);
 }
 
@@ -500,7 +501,7 @@ int orc_reset(orc_sim* S) {
   memset(S->prev_p, 0, S->nc * sizeof(int32_t));
   if (S->bed_n) {
     memcpy(S->bed_ext, S->bed_ext0, (size_t)S->bed_n * 8 * sizeof(float));
-    memset(S->bed_dw, 0, (size_t)S->bed_n * sizeof(int32_t));
+    memset(S->bed_dw, 0, (size_t)S->bed_n * 4 * sizeof(int32_t));
   }
   S->has_warm = 0;
   memset(&S->stats, 0, sizeof(S->stats));

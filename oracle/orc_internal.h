@@ -27,6 +27,12 @@
 #define O_M_DIV_CLAMP 2
 #define O_M_P_CLAMP 3
 #define O_M_OVERFLOW 4
+/* slots the reference leaves free, used by this build: the projection's
+ * outcome, max|div| and max|div - target| over the fluid cells after
+ * project_pressure + boundary_project, and max|div - target| before it */
+#define O_M_POST_MAX_ABS_DIV 5
+#define O_M_POST_MAX_ABS_RES 6
+#define O_M_PRE_MAX_ABS_RES 7
 
 /* units.rs:13-27 evaluated in binary32 like Rust's const evaluation. */
 float orc_sim_units_per_meter(void);
@@ -119,6 +125,7 @@ struct orc_sim {
   int32_t* prev_p;  /* previous solve, CS_WARM_PREVIOUS */
   int32_t* saved_p; /* pressure and kind lanes at the end of the last solve */
   int32_t* saved_kind;
+  float* tgt;       /* classify's target divergence per fluid cell (post-projection metric) */
   float* warm;      /* CS_WARM_EXTERNAL */
   int has_warm;
   uint32_t metrics[8];

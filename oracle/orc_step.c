@@ -433,6 +433,32 @@ static void k_boundary(orc_sim* S, const orc_params* u) {
 /* ---- classify_cells (shader.rs:1181-1367) ---------------------------------- */
 static const int32_t OFF6[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
 
+/* The central-difference divergence of grid_vel at a fluid cell, with the
+ * ghost-mirror rule on solid and off-grid faces (shader.rs:1278-1335). */
+static float cell_divergence(orc_sim* S, const orc_params* u, uint32_t ix, uint32_t iy, uint32_t iz, v3 cc) {
+  size_t idx = cidx(u, ix, iy, iz);
+  const float* sv = &S->grid_vel[idx * 4];
+  float dxv = u->dx;
+  float vxm = -sv[0], vxp = -sv[0], vym = -sv[1], vyp = -sv[1], vzm = -sv[2], vzp = -sv[2];
+#define OPEN(cond_sdf_pos, cx, cy, cz) \
+  (u->use_sdf_cache ? !class_solid(S, u, (cx), (cy), (cz)) : (sample_sdf(S, u, cond_sdf_pos) >= 0.0f))
+  if (ix > 0u && OPEN(V(cc.x + -dxv, cc.y + 0.0f, cc.z + 0.0f), (int32_t)ix - 1, (int32_t)iy, (int32_t)iz))
+    vxm = S->grid_vel[cidx(u, ix - 1u, iy, iz) * 4 + 0];
+  if (ix + 1u < u->gx && OPEN(V(cc.x + dxv, cc.y + 0.0f, cc.z + 0.0f), (int32_t)ix + 1, (int32_t)iy, (int32_t)iz))
+    vxp = S->grid_vel[cidx(u, ix + 1u, iy, iz) * 4 + 0];
+  if (iy > 0u && OPEN(V(cc.x + 0.0f, cc.y + -dxv, cc.z + 0.0f), (int32_t)ix, (int32_t)iy - 1, (int32_t)iz))
+    vym = S->grid_vel[cidx(u, ix, iy - 1u, iz) * 4 + 1];
+  if (iy + 1u < u->gy && OPEN(V(cc.x + 0.0f, cc.y + dxv, cc.z + 0.0f), (int32_t)ix, (int32_t)iy + 1, (int32_t)iz))
+    vyp = S->grid_vel[cidx(u, ix, iy + 1u, iz) * 4 + 1];
+  if (iz > 0u && OPEN(V(cc.x + 0.0f, cc.y + 0.0f, cc.z + -dxv), (int32_t)ix, (int32_t)iy, (int32_t)iz - 1))
+    vzm = S->grid_vel[cidx(u, ix, iy, iz - 1u) * 4 + 2];
+  if (iz + 1u < u->gz && OPEN(V(cc.x + 0.0f, cc.y + 0.0f, cc.z + dxv), (int32_t)ix, (int32_t)iy, (int32_t)iz + 1))
+    vzp = S->grid_vel[cidx(u, ix, iy, iz + 1u) * 4 + 2];
+#undef OPEN
+  float div = 0.5f * u->inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
+  return div;
+}
+
 static void k_classify(orc_sim* S, const orc_params* u) {
   float occ = u->nominal_mass * 0.1f;
   ORC_PARALLEL_FOR
@@ -468,25 +494,7 @@ static void k_classify(orc_sim* S, const orc_params* u) {
     }
     int32_t kind = kind_load(S, idx);
     if (!is_fluid(kind)) { div_store(S, u, idx, 0.0f); continue; }
-    const float* sv = &S->grid_vel[idx * 4];
-    float dxv = u->dx;
-    float vxm = -sv[0], vxp = -sv[0], vym = -sv[1], vyp = -sv[1], vzm = -sv[2], vzp = -sv[2];
-#define OPEN(cond_sdf_pos, cx, cy, cz) \
-  (u->use_sdf_cache ? !class_solid(S, u, (cx), (cy), (cz)) : (sample_sdf(S, u, cond_sdf_pos) >= 0.0f))
-    if (ix > 0u && OPEN(V(cc.x + -dxv, cc.y + 0.0f, cc.z + 0.0f), (int32_t)ix - 1, (int32_t)iy, (int32_t)iz))
-      vxm = S->grid_vel[cidx(u, ix - 1u, iy, iz) * 4 + 0];
-    if (ix + 1u < u->gx && OPEN(V(cc.x + dxv, cc.y + 0.0f, cc.z + 0.0f), (int32_t)ix + 1, (int32_t)iy, (int32_t)iz))
-      vxp = S->grid_vel[cidx(u, ix + 1u, iy, iz) * 4 + 0];
-    if (iy > 0u && OPEN(V(cc.x + 0.0f, cc.y + -dxv, cc.z + 0.0f), (int32_t)ix, (int32_t)iy - 1, (int32_t)iz))
-      vym = S->grid_vel[cidx(u, ix, iy - 1u, iz) * 4 + 1];
-    if (iy + 1u < u->gy && OPEN(V(cc.x + 0.0f, cc.y + dxv, cc.z + 0.0f), (int32_t)ix, (int32_t)iy + 1, (int32_t)iz))
-      vyp = S->grid_vel[cidx(u, ix, iy + 1u, iz) * 4 + 1];
-    if (iz > 0u && OPEN(V(cc.x + 0.0f, cc.y + 0.0f, cc.z + -dxv), (int32_t)ix, (int32_t)iy, (int32_t)iz - 1))
-      vzm = S->grid_vel[cidx(u, ix, iy, iz - 1u) * 4 + 2];
-    if (iz + 1u < u->gz && OPEN(V(cc.x + 0.0f, cc.y + 0.0f, cc.z + dxv), (int32_t)ix, (int32_t)iy, (int32_t)iz + 1))
-      vzp = S->grid_vel[cidx(u, ix, iy, iz + 1u) * 4 + 2];
-#undef OPEN
-    float div = 0.5f * u->inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
+    float div = cell_divergence(S, u, ix, iy, iz, cc);
     float rest = rest_vol(S, u, idx), cur = cur_vol(S, u, idx);
     float target = volume_target_div(u, rest, cur);
     if (kind == O_CELL_BED) {
@@ -501,6 +509,9 @@ static void k_classify(orc_sim* S, const orc_params* u) {
       }
     }
     div_store(S, u, idx, div - target);
+    S->tgt[idx] = target;
+    uint32_t fpr = o_f2u(o_clamp(fabsf(div - target) * u->metrics_div_fp_scale, 0.0f, 2147483648.0f));
+    orc_max_u32(&S->metrics[O_M_PRE_MAX_ABS_RES], fpr);
     float abs_div = fabsf(div);
     uint32_t fpd = o_f2u(o_clamp(abs_div * u->metrics_div_fp_scale, 0.0f, 2147483648.0f));
     orc_max_u32(&S->metrics[O_M_MAX_ABS_DIV], fpd);
@@ -824,6 +835,27 @@ static void k_project(orc_sim* S, const orc_params* u) {
     float speed = vlen(v);
     if (speed > u->vel_cap) v = vmul(v, u->vel_cap / speed);
     g[0] = v.x; g[1] = v.y; g[2] = v.z;
+  }
+}
+
+/* ---- post-projection divergence (no reference pass) ------------------------
+ * The A7 divergence stencil (shader.rs:1278-1335) evaluated on grid_vel after
+ * project_pressure and its boundary_project, over the cells classify made
+ * fluid: max|div| (encoded like shader.rs:1363-1365) and max|div - target|
+ * (the quantity the projection drives to zero) into the free metric slots. */
+static void k_post_divergence(orc_sim* S, const orc_params* u) {
+  ORC_PARALLEL_FOR
+  for (size_t idx = 0; idx < S->nc; ++idx) {
+    if (!is_fluid(kind_load(S, idx))) continue;
+    uint32_t ix, iy, iz;
+    decompose(u, idx, &ix, &iy, &iz);
+    v3 cc = V(u->origin[0] + ((float)ix + 0.5f) * u->dx, u->origin[1] + ((float)iy + 0.5f) * u->dx,
+              u->origin[2] + ((float)iz + 0.5f) * u->dx);
+    float div = cell_divergence(S, u, ix, iy, iz, cc);
+    uint32_t fd = o_f2u(o_clamp(fabsf(div) * u->metrics_div_fp_scale, 0.0f, 2147483648.0f));
+    uint32_t fr = o_f2u(o_clamp(fabsf(div - S->tgt[idx]) * u->metrics_div_fp_scale, 0.0f, 2147483648.0f));
+    orc_max_u32(&S->metrics[O_M_POST_MAX_ABS_DIV], fd);
+    orc_max_u32(&S->metrics[O_M_POST_MAX_ABS_RES], fr);
   }
 }
 
@@ -1206,6 +1238,7 @@ static int substep(orc_sim* S, float sub_dt, int stop_after, int emit) {
   if (stop_after == ORC_PASS_RBGS) return 0;
   k_project(S, u);
   k_boundary(S, u);
+  k_post_divergence(S, u);
   if (stop_after == ORC_PASS_PROJECT) return 0;
   k_packing_prepare(S, u);
   k_packing_apply(S, u);
@@ -1228,6 +1261,32 @@ static int substep(orc_sim* S, float sub_dt, int stop_after, int emit) {
 int orc_debug_step_until(orc_sim* S, float sub_dt, int stop_after) {
   return substep(S, sub_dt, stop_after, 0);
 }
+
+/* One pass on the current state, with the uniforms of a substep of sub_dt and
+ * nothing cleared, for the closed-form tests of the passes. */
+int orc_debug_run_pass(orc_sim* S, float sub_dt, int pass) {
+  orc_write_uniforms(S, sub_dt, &S->u);
+  orc_params* u = &S->u;
+  switch (pass) {
+    case ORC_PASS_P2G: k_p2g(S, u); break;
+    case ORC_PASS_GRID: k_grid_update(S, u); break;
+    case ORC_PASS_BOUNDARY: k_boundary(S, u); break;
+    case ORC_PASS_CLASSIFY: k_classify(S, u); break;
+    case ORC_PASS_RBGS: build_rows(S, u); k_rbgs_half(S, u, 0u); k_rbgs_half(S, u, 1u); break;
+    case ORC_PASS_PROJECT: k_project(S, u); break;
+    case ORC_PASS_PACKING: k_packing_prepare(S, u); k_packing_apply(S, u); break;
+    case ORC_PASS_VISCOSITY: k_viscosity_prepare(S, u); k_viscosity_apply(S, u); break;
+    case ORC_PASS_G2P: k_g2p(S, u); break;
+    case ORC_PASS_POST_DIVERGENCE: k_post_divergence(S, u); break;
+    default: return CS_ERR_INVALID;
+  }
+  return CS_OK;
+}
+void orc_set_previous_pressure(orc_sim* S, const int32_t* p) { memcpy(S->prev_p, p, S->nc * sizeof(int32_t)); }
+void orc_debug_set_grid_vel(orc_sim* S, const float* vel4) { memcpy(S->grid_vel, vel4, S->nc * 4 * sizeof(float)); }
+void orc_debug_set_grid_lanes(orc_sim* S, const int32_t* lanes6) { memcpy(S->grid, lanes6, S->nc * 6 * sizeof(int32_t)); }
+void orc_debug_read_grid_lanes(orc_sim* S, int32_t* lanes6) { memcpy(lanes6, S->grid, S->nc * 6 * sizeof(int32_t)); }
+void orc_debug_clear_metrics(orc_sim* S) { memset(S->metrics, 0, sizeof(S->metrics)); }
 
 /* MpmSim3D::step_frame (mod.rs:264-430) */
 int orc_step_frame(orc_sim* S, float dt) {

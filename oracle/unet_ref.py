@@ -209,3 +209,14 @@ def param_dict(v: np.ndarray) -> dict:
     o += 16
     p["head.b"] = v[o:o + 1]
     return p
+
+
+def adam_step(p: np.ndarray, g: np.ndarray, m: np.ndarray, v: np.ndarray, t: int, lr: float,
+              b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1) in float64, step t >= 1.
+    Returns the new (parameters, first moment, second moment)."""
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** t)
+    vh = v / (1.0 - b2 ** t)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -64,10 +64,31 @@ def test_ctypes_layout_matches_c(tmp_path):
             assert int(c_vals[f"{cname}.{field}"]) == getattr(py, field).offset, f"{cname}.{field}"
 
 
-def test_reference_buffer_layouts():
-    # Particle 32 B, AffineC 48 B, BedExtract 32 B (state.rs:90-104, 146-153)
-    assert 8 * 4 == 32 and 12 * 4 == 48
-    assert C.sizeof(abi.BedRecord) == 32
+def _build_caller(tmp_path) -> Path:
+    """tests/abi_caller.c, a plain C host, linked against the built library."""
+    exe = tmp_path / "abi_caller"
+    libdir = cs.LIB_PATH.parent
+    subprocess.run(["gcc", "-std=c11", "-O1", "-Wall", "-Werror", f"-I{ROOT / 'include'}",
+                    str(ROOT / "tests" / "abi_caller.c"), "-o", str(exe), f"-L{libdir}", "-lcoffee_sim",
+                    f"-Wl,-rpath,{libdir}", "-lm"], check=True)
+    return exe
+
+
+def test_c_program_drives_the_host_entry_points(lib, tmp_path):
+    """A C program built against include/coffee_sim.h alone calls the presets,
+    the seeded workload, the bed lattice and the SDF generator."""
+    out = subprocess.run([str(_build_caller(tmp_path))], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "abi caller ok: host part" in out.stdout
+
+
+@pytest.mark.gpu
+def test_c_program_steps_a_simulation(lib, tmp_path):
+    """The same C program creates a context and a simulation, steps it,
+    checkpoints and restores it, and reads the solve statistics."""
+    out = subprocess.run([str(_build_caller(tmp_path)), "run"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert "fluid cells" in out.stdout, out.stdout
 
 
 def test_preset_errors_are_reported(lib):

@@ -347,3 +347,39 @@ def test_bed_particles_are_validated(ctx):
     plain = cs.MpmSim3D(ctx, s, None)
     plain.step_frame(1.0 / 600.0)
     assert (plain.read_bed_lookup() == -1).all()
+
+
+@pytest.mark.gpu
+def test_seated_bed_checkpoint_resumes_bitwise(ctx):
+    """A checkpoint carries the bed particles, their affine state and their
+    BedExtracts: saved after a frame and loaded into a fresh simulation, the
+    next frame equals the uninterrupted run bit for bit."""
+    s, sc, bed = bed_scene(substeps=2)
+    a = cs.MpmSim3D(ctx, s, sc)
+    a.step_frame(1.0 / 300.0)
+    blob = a.save()
+    b = cs.MpmSim3D(ctx, s, sc)
+    b.step_frame(1.0 / 600.0)  # a different bed state, overwritten by the load
+    b.load(blob)
+    a.step_frame(1.0 / 300.0)
+    b.step_frame(1.0 / 300.0)
+    compare_state(a, b, "resumed seated bed")
+    assert_bitwise("bed extracts", a.read_bed_extracts(), b.read_bed_extracts())
+    # a checkpoint of another bed (none) is rejected
+    plain = cs.MpmSim3D(ctx, s, None)
+    with pytest.raises(cs.CoffeeSimError):
+        plain.load(blob)
+
+
+@pytest.mark.gpu
+def test_water_count_leaves_room_for_the_bed(ctx):
+    """The water may use max_particles less the bed (mod.rs:239-249); a larger
+    count is a range error, returned before any device work."""
+    s, sc, bed = bed_scene(substeps=1, n_water=100)
+    sim = cs.MpmSim3D(ctx, s, sc)
+    nb = len(bed["particles"])
+    with pytest.raises(cs.CoffeeSimError):
+        sim.set_particle_count(s.max_particles)
+    with pytest.raises(cs.CoffeeSimError):
+        sim.set_particle_count(s.max_particles - nb + 1)
+    sim.set_particle_count(s.max_particles - nb)

@@ -93,3 +93,26 @@ def test_network_gradients_match_finite_differences():
         vm[i] -= eps
         fd = (loss_of(vp) - loss_of(vm)) / (2 * eps)
         assert abs(fd - gv[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, gv[i])
+
+
+def test_adam_closed_forms():
+    """Adam's bias correction makes a constant gradient move every parameter by
+    lr * g / (|g| + eps) at every step (the first step included); a gradient
+    that flips sign at step 2 gives m_hat = -g (0.01 / 0.19),
+    v_hat = g^2 (0.001999 / 0.001999) (Kingma & Ba 2015, Sec. 2-3)."""
+    rng = np.random.default_rng(0)
+    g = rng.normal(size=64) * np.logspace(-3, 2, 64)
+    p0 = rng.normal(size=64)
+    lr = 1e-3
+    p, m, v = p0.copy(), np.zeros(64), np.zeros(64)
+    for t in range(1, 6):
+        p1, m, v = U.adam_step(p, g, m, v, t, lr)
+        assert np.allclose(p1 - p, -lr * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-18), t
+        p = p1
+    p, m, v = U.adam_step(p0, g, np.zeros(64), np.zeros(64), 1, lr)
+    p2, _, _ = U.adam_step(p, -g, m, v, 2, lr)
+    mh = -g * 0.01 / (1.0 - 0.81)
+    vh = g * g * (0.999 * 0.001 + 0.001) / (1.0 - 0.999 ** 2)
+    assert np.allclose(p2 - p, -lr * mh / (np.sqrt(vh) + 1e-8), rtol=1e-12)
+    # the step is bounded by lr (here about 0.0526 lr / 1.0)
+    assert np.all(np.abs(p2 - p) <= lr)
