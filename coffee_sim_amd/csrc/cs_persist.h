@@ -1,0 +1,47 @@
+// cs_persist.h — the persistent red-black Gauss-Seidel solve (cs_rbgs_persist.hip):
+// one launch of co-resident workgroups, one active brick each, that runs a
+// whole solve with neighbour-to-neighbour synchronisation.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "cs_common.h"
+#include "cs_solver.h"
+
+#define CS_PERSIST_THREADS 256
+#define CS_PERSIST_FLAG_STRIDE 16u  // u32 words between two workgroups' flags (64 B)
+#define CS_PERSIST_FACE_WORDS (6u * 2u * 32u)  // face-buffer words per brick
+
+struct cs_persist_args {
+  cs_params u;
+  const uint32_t* list;      // active bricks, ascending (cs_launch_brick_scan)
+  const float4* wc;          // face weights and diagonal per cell (cs_launch_face_weights)
+  const float* rhs;          // c = dx^2 * rhs per cell
+  int32_t* p;                // the pressure: the start on entry, the solution on exit
+  uint32_t* metrics;         // clamp fires are added to CS_M_P_CLAMP
+  cs_solve_state* st;        // active_bricks in; iterations, residuals, done, faults out
+  uint64_t* fb;              // [nbricks][6 faces][2 colours][32]: surface cells as
+                             // (epoch << 32 | fixed-point pressure)
+  uint32_t epoch_base;       // epochs of this solve are epoch_base + 1, 2, ... (grow across solves)
+  uint32_t* pflags;          // [max_runs * stride]: residual checks published per brick
+  double* partials;          // [2][max_runs][2]: (rr, bb) per brick and check parity
+  uint32_t max_runs;         // capacity of pflags and partials (the brick count)
+  uint32_t fixed_pairs;      // 0: iterate to `tol`; else exactly this many red+black pairs
+  uint32_t every;            // iterations between residual tests (tolerance mode)
+  uint32_t cap;              // iteration cap (tolerance mode)
+  double tol;
+  uint64_t spin_limit;       // 100 MHz ticks a wait may last before the solve gives up
+};
+
+// Clears the residual-check flags and the failure state.
+hipError_t cs_persist_prepare(hipStream_t s, const cs_params& u, cs_solve_state* st, uint32_t* pflags);
+// Epochs a solve of at most `iters` red+black pairs uses (the next solve's base moves by this).
+inline uint32_t cs_persist_epochs(uint32_t iters) { return 2u * iters + 2u; }
+// Workgroups launched for `nactive` bricks (a multiple of 8, one per XCD slice).
+uint32_t cs_persist_groups(uint32_t nactive);
+// Workgroups of the persistent kernel that fit on the device at once.
+hipError_t cs_persist_max_groups(int device, uint32_t* out);
+// With `cooperative` the launch fails instead of starting workgroups that
+// would wait for ones not yet resident.
+hipError_t cs_persist_launch(hipStream_t s, const cs_persist_args& a, uint32_t groups, bool cooperative);

@@ -1,0 +1,410 @@
+// cs_rbgs_persist.hip — the whole red-black Gauss-Seidel pressure solve of a
+// substep (pressure_update, shader.rs:1371-1453, looped as mod.rs:352-357 or to
+// a residual tolerance) in ONE launch of co-resident workgroups.
+//
+// Why one launch: an iteration of this solve moves a few MB (about 185k fluid
+// cells at the working size), far less than a kernel boundary costs at 8 TB/s,
+// so a solve made of launches is bound by launch and first-load latency, not
+// by HBM.  Device-wide barriers with acquire/release fences are no cheaper on
+// a chip of 8 L2s (each release writes the XCD's L2 back, each acquire
+// invalidates it).  Instead, each workgroup keeps one active 8^3 brick
+// resident for the whole solve and talks only to the (at most six) workgroups
+// of its face-neighbour bricks:
+//
+//  - Registers hold the static part of its two cells' rows (face weights,
+//    diagonal, right-hand side): loaded once.
+//  - LDS holds the brick's pressure with a one-cell halo (10^3 floats).
+//  - Per half sweep of colour c: every thread updates its colour-c cell
+//    exactly as pressure_update does (same operation order, so the result is
+//    the reference's bit for bit).  A cell on the brick's surface is also
+//    published as one 64-bit word (fixed-point pressure, epoch) with a
+//    device-coherent store into the brick's face buffer (per face and colour
+//    32 contiguous words, so a face moves in a few 64-byte segments); a halo
+//    cell is read by polling the neighbour's word until it carries the
+//    current epoch.  The data is its own flag, so a half sweep
+//    costs one store-to-load latency and one barrier.  The next half sweep
+//    reads only colour-c neighbours (red-black), so one word per cell is
+//    enough: a neighbour rewrites colour c only at epoch e+2, after it has
+//    read this brick's epoch-e+1 surface, which this brick publishes only
+//    after its reads of epoch e.  Epochs grow across solves (a per-simulation
+//    base), so words left by an earlier solve are never taken as current.
+//  - Device-coherent here means relaxed agent-scope atomics (global loads and
+//    stores with the device-scope cache policy): no L2 write-back or
+//    invalidate is needed.
+//  - Every `every` iterations (and at the cap) each workgroup sums its cells'
+//    residual rows in double, publishes (rr, bb) and reads everyone's sums in
+//    the same order, so all workgroups reach the same verdict and stop at the
+//    same iteration, exactly where the oracle's loop stops.
+//
+// Every wait is bounded by a wall-clock limit; a workgroup that times out
+// raises an abort flag that every other wait also watches, so the launch
+// always drains.  The host reports the failure (it cannot happen while the
+// launch is co-resident, which the cooperative launch guarantees).
+#include "cs_device.h"
+#include "cs_persist.h"
+
+namespace {
+
+constexpr int BR = CS_BRICK;           // brick side (8)
+constexpr int PR = BR + 2;             // staged side: brick plus one halo cell
+constexpr int PR2 = PR * PR;
+constexpr int PR3 = PR * PR * PR;
+constexpr int NT = CS_PERSIST_THREADS;  // 256: one cell of each colour per thread
+static_assert(NT * 2 == BR * BR * BR, "two cells per thread");
+
+__device__ __forceinline__ uint32_t ld_dev(const uint32_t* p) {
+  return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t ld_dev(const int32_t* p) {
+  return __hip_atomic_load(const_cast<int32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_dev(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(int32_t* p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until every memory operation this wavefront issued has completed (the
+// stores have reached the device-coherent level).  The empty asm statements
+// keep the compiler from moving memory accesses across the wait.
+__device__ __forceinline__ void stores_complete() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  asm volatile("" ::: "memory");
+}
+
+// 100 MHz constant clock
+__device__ __forceinline__ uint64_t now_ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
+// One update of pressure_update (shader.rs:1386-1438) from the precomputed
+// face weights: `self` = (+x, +y, +z weights, diagonal count); the minus-side
+// weights are the neighbours' plus-side ones.  Same order as the reference
+// (-x, +x, -y, +y, -z, +z); an unused face has weight 0 and adds +-0.
+__device__ __forceinline__ float update(const cs_params& u, float4 self, float srhs, float wxm, float wym, float wzm,
+                                        const float* s_p, int li, bool& fired) {
+  float sum = wxm * s_p[li - 1];
+  sum += self.x * s_p[li + 1];
+  sum += wym * s_p[li - PR];
+  sum += self.y * s_p[li + PR];
+  sum += wzm * s_p[li - PR2];
+  sum += self.z * s_p[li + PR2];
+  const float quot = (sum - srhs) / self.w;
+  const float pn = self.w > 0.0f ? quot : 0.0f;
+  const float cl = fminf(fmaxf(pn, -u.pressure_clamp), u.pressure_clamp);
+  fired = cl != pn;
+  return truncf(cl * u.fp_scale) * u.inv_fp_scale;
+}
+
+// Sum over a wavefront in a fixed butterfly order (every lane gets the total).
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+__device__ __forceinline__ double rel_residual(double rr, double bb) {
+  if (bb > 0.0) return sqrt(rr / bb);
+  return rr > 0.0 ? __builtin_huge_val() : 0.0;
+}
+
+// Word k of colour c on face f (-x, +x, -y, +y, -z, +z) of a brick's face buffer.
+__device__ __forceinline__ int face_word(uint32_t brick, int f, int c, int k) {
+  return (int)(((brick * 6u + (uint32_t)f) * 2u + (uint32_t)c) * 32u + (uint32_t)k);
+}
+
+struct Cell {
+  int li;        // index in the staged 10^3 region
+  int32_t g;     // global cell index (-1: past the grid's edge)
+  int fslot[3];  // per axis: its word in the face buffer of the brick face it lies on, or -1
+  float4 self;   // +x, +y, +z weights and diagonal (w < 0: not fluid)
+  float rhs;     // c = dx^2 * rhs of the update
+  float wxm, wym, wzm;
+};
+
+}  // namespace
+
+// Workgroup w serves brick `run` of the active list; runs are dealt so that a
+// contiguous part of the (ascending, hence z-major) list lands on one XCD
+// (workgroup w runs on XCD w % 8).
+__global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
+  const cs_params& u = a.u;
+  const uint32_t t = threadIdx.x;
+  const uint32_t nactive = a.st->active_bricks;
+  const uint32_t per = (gridDim.x + 7u) / 8u;
+  const uint32_t run = (blockIdx.x % 8u) * per + blockIdx.x / 8u;
+  if (run >= nactive) return;
+  const uint32_t brick = a.list[run];
+  const uint32_t bx = brick % u.nbx, by = (brick / u.nbx) % u.nby, bz = brick / (u.nbx * u.nby);
+  const int x0 = (int)bx * BR, y0 = (int)by * BR, z0 = (int)bz * BR;
+
+  __shared__ float s_p[PR3];
+  __shared__ double s_red[2 * (NT / 64)];
+  __shared__ uint32_t s_fail;
+  __shared__ uint32_t s_verdict;  // 1: converged or capped, stop
+
+  if (t == 0) s_fail = 0u;
+  // ---- the two cells of this thread (colour c at x = 2m + ((y + z + c) & 1)) -----------------
+  Cell cell[2];
+  int hl[2];
+  int32_t hg[2];
+  const int m = (int)(t & 3u), ly = (int)((t >> 2) & 7u), lz = (int)(t >> 5);
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int lx = 2 * m + ((ly + lz + c) & 1);  // brick corners are even, so parity = colour
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    Cell& e = cell[c];
+    e.li = ((lz + 1) * PR + ly + 1) * PR + lx + 1;
+    {
+      // the faces this cell lies on; word k = (b2 << 2) | (a1 >> 1) of face f, colour c,
+      // where (a1, b2) are its in-face coordinates (the reader's halo mapping below)
+      const int lc[3] = {lx, ly, lz};
+#pragma unroll
+      for (int axis = 0; axis < 3; ++axis) {
+        const int a1 = lc[axis == 0 ? 1 : 0], b2 = lc[axis == 2 ? 1 : 2];
+        const int k = (b2 << 2) | (a1 >> 1);
+        e.fslot[axis] = lc[axis] == 0        ? face_word(brick, 2 * axis, c, k)
+                        : lc[axis] == BR - 1 ? face_word(brick, 2 * axis + 1, c, k)
+                                             : -1;
+      }
+    }
+    // a brick on the grid's far edge may reach past it: such cells are not fluid
+    const bool on = gx < (int)u.gx && gy < (int)u.gy && gz < (int)u.gz;
+    e.g = on ? (int32_t)(((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx) : -1;
+    e.self = on ? a.wc[e.g] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    e.rhs = on ? a.rhs[e.g] : 0.0f;
+    e.wxm = on && gx > 0 ? a.wc[e.g - 1].x : 0.0f;
+    e.wym = on && gy > 0 ? a.wc[e.g - (int32_t)u.gx].y : 0.0f;
+    e.wzm = on && gz > 0 ? a.wc[e.g - (int32_t)(u.gx * u.gy)].z : 0.0f;
+    // this thread's halo cell of colour c (threads 0..191: 32 per face) when it is
+    // fluid, hence in an active brick whose workgroup publishes it every epoch
+    hl[c] = -1;
+    hg[c] = -1;
+    if (t < 192u) {
+      const int f = (int)(t >> 5), k = (int)(t & 31u);
+      const int axis = f >> 1, side = f & 1;
+      const int fc = side ? BR : -1;   // halo coordinate along the face axis
+      const int b2 = k >> 2;           // second in-face coordinate
+      const int a1 = 2 * (k & 3) + ((c + (fc & 1) + b2) & 1);  // parity (fc + a1 + b2) = c
+      int l[3];
+      l[axis] = fc;
+      l[axis == 0 ? 1 : 0] = a1;
+      l[axis == 2 ? 1 : 2] = b2;
+      hl[c] = ((l[2] + 1) * PR + l[1] + 1) * PR + l[0] + 1;
+      const int hx = x0 + l[0], hy = y0 + l[1], hz = z0 + l[2];
+      if (hx >= 0 && hy >= 0 && hz >= 0 && hx < (int)u.gx && hy < (int)u.gy && hz < (int)u.gz) {
+        const int32_t hgi = (int32_t)(((uint32_t)hz * u.gy + (uint32_t)hy) * u.gx + (uint32_t)hx);
+        if (a.wc[hgi].w >= 0.0f) {
+          // the neighbour across face f publishes this cell on its opposite face, same word
+          const uint32_t nb = ((uint32_t)(hz / BR) * u.nby + (uint32_t)(hy / BR)) * u.nbx + (uint32_t)(hx / BR);
+          hg[c] = face_word(nb, f ^ 1, c, k);
+        }
+      }
+    }
+  }
+  // ---- stage the pressure region (device-coherent loads: other workgroups write it) ----------
+  for (int i = (int)t; i < PR3; i += NT) {
+    const int lx = i % PR - 1, lyy = (i / PR) % PR - 1, lzz = i / PR2 - 1;
+    const int gx = x0 + lx, gy = y0 + lyy, gz = z0 + lzz;
+    float v = 0.0f;
+    if (gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)u.gx && gy < (int)u.gy && gz < (int)u.gz)
+      v = (float)ld_dev(&a.p[((uint32_t)gz * u.gy + (uint32_t)gy) * u.gx + (uint32_t)gx]) * u.inv_fp_scale;
+    s_p[i] = v;
+  }
+  __syncthreads();
+
+  uint32_t fires = 0;
+  uint32_t it = 0;
+  uint32_t epoch = 0;
+  uint32_t checks = 0;
+  const uint32_t limit = a.fixed_pairs ? a.fixed_pairs : a.cap;
+  bool failed = false;
+
+  // Residual of this brick's rows (residual_rows: fluid, non-empty, not held
+  // at the clamp), published and reduced over every brick in run order.
+  auto check = [&](uint32_t iters) -> bool {
+    double rr = 0.0, bb = 0.0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const Cell& e = cell[c];
+      const float p = s_p[e.li];
+      const float wn[6] = {e.wxm, e.self.x, e.wym, e.self.y, e.wzm, e.self.z};
+      const int off[6] = {-1, 1, -PR, PR, -PR2, PR2};
+      double apn = 0.0;
+#pragma unroll
+      for (int n = 0; n < 6; ++n) apn += (double)wn[n] * (double)s_p[e.li + off[n]];
+      const double b = -(double)e.rhs;
+      const double ap = (double)e.self.w * (double)p - apn;
+      if (e.self.w > 0.0f && fabsf(p) != u.pressure_clamp) {
+        rr += (b - ap) * (b - ap);
+        bb += b * b;
+      }
+    }
+    rr = wave_sum_d(rr);
+    bb = wave_sum_d(bb);
+    if ((t & 63u) == 0u) {
+      s_red[2 * (t >> 6)] = rr;
+      s_red[2 * (t >> 6) + 1] = bb;
+    }
+    __syncthreads();
+    const uint32_t slot = checks & 1u;
+    if (t == 0) {
+      double r0 = 0.0, b0 = 0.0;
+      for (int w = 0; w < NT / 64; ++w) {
+        r0 += s_red[2 * w];
+        b0 += s_red[2 * w + 1];
+      }
+      uint64_t* dst = reinterpret_cast<uint64_t*>(a.partials) + ((size_t)slot * a.max_runs + run) * 2;
+      st_dev(dst, __double_as_longlong(r0));
+      st_dev(dst + 1, __double_as_longlong(b0));
+      stores_complete();
+      st_dev(&a.pflags[(size_t)run * CS_PERSIST_FLAG_STRIDE], checks + 1u);
+    }
+    // every brick's sums, in run order within each thread, then a fixed tree
+    double tr = 0.0, tb = 0.0;
+    const uint64_t t0 = now_ticks();
+    for (uint32_t q = t; q < nactive; q += NT) {
+      const uint32_t* pf = &a.pflags[(size_t)q * CS_PERSIST_FLAG_STRIDE];
+      while (ld_dev(pf) < checks + 1u) {
+        if (ld_dev(&a.st->abort) || now_ticks() - t0 > a.spin_limit) {
+          failed = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (failed) break;
+      const uint64_t* src = reinterpret_cast<const uint64_t*>(a.partials) + ((size_t)slot * a.max_runs + q) * 2;
+      tr += __longlong_as_double(ld_dev(src));
+      tb += __longlong_as_double(ld_dev(src + 1));
+    }
+    if (failed) atomicOr(&s_fail, 1u);
+    tr = wave_sum_d(tr);
+    tb = wave_sum_d(tb);
+    __syncthreads();
+    if ((t & 63u) == 0u) {
+      s_red[2 * (t >> 6)] = tr;
+      s_red[2 * (t >> 6) + 1] = tb;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double r0 = 0.0, b0 = 0.0;
+      for (int w = 0; w < NT / 64; ++w) {
+        r0 += s_red[2 * w];
+        b0 += s_red[2 * w + 1];
+      }
+      const double r = rel_residual(r0, b0);
+      const bool stop = a.fixed_pairs ? true : (r <= a.tol || iters >= a.cap);
+      s_verdict = stop ? 1u : 0u;
+      if (run == 0) {
+        if (checks == 0) a.st->initial_residual = r;
+        a.st->residual = r;
+        a.st->iterations = iters;
+        a.st->done = r <= a.tol ? 1u : 0u;
+      }
+    }
+    ++checks;
+    __syncthreads();
+    return s_verdict != 0u || s_fail != 0u;
+  };
+
+  // tolerance mode tests the start (the oracle's initial residual)
+  bool stop = a.fixed_pairs ? false : check(0u);
+  // one half sweep of colour c; returns false when a wait gave up
+  auto half_sweep = [&](const Cell& e, int h_l, int32_t h_g) -> bool {
+    ++epoch;
+    const uint32_t want = a.epoch_base + epoch;
+    bool fired;
+    const float np = update(u, e.self, e.rhs, e.wxm, e.wym, e.wzm, s_p, e.li, fired);
+    if (e.self.w >= 0.0f) {
+      s_p[e.li] = np;
+      fires += fired ? 1u : 0u;
+      const uint64_t word = (uint64_t)want << 32 | (uint32_t)(int32_t)(np * u.fp_scale);
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (e.fslot[j] >= 0) st_dev(&a.fb[e.fslot[j]], word);
+    }
+    // the colour-c halo of this epoch, polled until its word carries the epoch
+    if (h_g >= 0) {
+      uint64_t v = ld_dev(&a.fb[h_g]);
+      if ((uint32_t)(v >> 32) < want) {
+        const uint64_t t0 = now_ticks();
+        do {
+          if (ld_dev(&a.st->abort) || now_ticks() - t0 > a.spin_limit) {
+            atomicOr(&s_fail, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          v = ld_dev(&a.fb[h_g]);
+        } while ((uint32_t)(v >> 32) < want);
+      }
+      s_p[h_l] = (float)(int32_t)(uint32_t)v * u.inv_fp_scale;
+    }
+    __syncthreads();
+    return s_fail == 0u;
+  };
+
+  while (!stop && it < limit) {
+    if (!half_sweep(cell[0], hl[0], hg[0]) || !half_sweep(cell[1], hl[1], hg[1])) break;
+    ++it;
+    if (a.fixed_pairs ? it == a.fixed_pairs : (it % a.every == 0u || it >= a.cap)) stop = check(it);
+  }
+  if (s_fail) {
+    if (t == 0) {
+      st_dev(&a.st->abort, 1u);
+      atomicAdd(&a.st->faults, 1u);
+    }
+    return;
+  }
+  // the solution of this brick's cells (later passes read it with plain loads)
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+    if (cell[c].self.w >= 0.0f) a.p[cell[c].g] = (int32_t)(s_p[cell[c].li] * u.fp_scale);
+#pragma unroll
+  for (int m2 = 32; m2 > 0; m2 >>= 1) fires += (uint32_t)__shfl_xor((int)fires, m2, 64);
+  if ((t & 63u) == 0u && fires) atomicAdd(&a.metrics[CS_M_P_CLAMP], fires);
+}
+
+// Clears the residual-check flags and the failure state of a solve.
+__global__ void __launch_bounds__(256) k_persist_prepare(cs_solve_state* __restrict__ st, uint32_t nbricks,
+                                                         uint32_t* __restrict__ pflags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbricks) pflags[(size_t)i * CS_PERSIST_FLAG_STRIDE] = 0u;
+  if (i == 0) {
+    st->abort = 0u;
+    st->faults = 0u;
+  }
+}
+
+hipError_t cs_persist_prepare(hipStream_t s, const cs_params& u, cs_solve_state* st, uint32_t* pflags) {
+  hipLaunchKernelGGL(k_persist_prepare, dim3((u.nbricks + 255u) / 256u), dim3(256), 0, s, st, u.nbricks, pflags);
+  return hipGetLastError();
+}
+
+uint32_t cs_persist_groups(uint32_t nactive) { return (nactive + 7u) / 8u * 8u; }
+
+hipError_t cs_persist_max_groups(int device, uint32_t* out) {
+  int per_cu = 0, cus = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rbgs_persist, NT, 0);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return e;
+  *out = (uint32_t)per_cu * (uint32_t)cus;
+  return hipSuccess;
+}
+
+hipError_t cs_persist_launch(hipStream_t s, const cs_persist_args& a, uint32_t groups, bool cooperative) {
+  if (cooperative) {
+    cs_persist_args copy = a;
+    void* args[] = {&copy};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_rbgs_persist), dim3(groups), dim3(NT), args, 0,
+                                      s);
+  }
+  hipLaunchKernelGGL(k_rbgs_persist, dim3(groups), dim3(NT), 0, s, a);
+  return hipGetLastError();
+}

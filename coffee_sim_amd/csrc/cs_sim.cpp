@@ -62,7 +62,8 @@ void free_sim(cs_sim* m) {
                  m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,
                  m->brick_flag, m->brick_list, m->brick_partials, m->dstate,       m->tile_count,
                  m->tile_start, m->p2g_key,    m->p2g_sorted, m->render,     m->bed_part,
-                 m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt};
+                 m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt,
+                 m->persist_fb, m->persist_pflags, m->persist_partials};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -311,6 +312,74 @@ int capture_batch(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ever
   return CS_OK;
 }
 
+// The whole solve as one launch of co-resident workgroups (cs_rbgs_persist.hip):
+// same updates, same residual tests, same stopping iteration as the
+// launch-per-chunk solve; the pressure is solved in place in m->pres.
+int solve_persistent(cs_sim* m, const cs_params& u, uint32_t nactive, bool external) {
+  hipStream_t st = m->ctx->stream;
+  cs_solve_stats& S = m->stats;
+  const bool fixed = m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS;
+  const double tol = (double)m->s.rbgs_tolerance;
+  if (!fixed && external) {
+    // a supplied start is judged by its residual before the solve (the fallback)
+    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+                                     m->dstate, 0u, 0u, 0u, tol));
+    CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+    if (m->h_state->initial_residual > (double)m->s.warm_fallback_residual) {
+      CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
+      S.fell_back = 1;
+    }
+  }
+  const uint32_t span = cs_persist_epochs(fixed ? m->s.pressure_rbgs_pairs : m->s.rbgs_max_iters);
+  if ((uint64_t)m->persist_epoch + span >= 0x80000000ull) {
+    // epochs would leave the 31-bit range: forget every published word
+    CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->persist_fb),
+                              2 * (size_t)u.nbricks * CS_PERSIST_FACE_WORDS, 0));
+    m->persist_epoch = 1;
+  }
+  CS_HIP(cs_persist_prepare(st, u, m->dstate, m->persist_pflags));
+  cs_persist_args a{};
+  a.u = u;
+  a.list = m->brick_list;
+  a.wc = reinterpret_cast<const float4*>(m->wc);
+  a.rhs = m->rhs;
+  a.p = m->pres;
+  a.metrics = m->metrics;
+  a.st = m->dstate;
+  a.fb = m->persist_fb;
+  a.epoch_base = m->persist_epoch;
+  a.pflags = m->persist_pflags;
+  a.partials = m->persist_partials;
+  a.max_runs = u.nbricks;
+  a.fixed_pairs = fixed ? m->s.pressure_rbgs_pairs : 0u;
+  a.every = m->s.rbgs_check_every ? m->s.rbgs_check_every : 8u;
+  a.cap = m->s.rbgs_max_iters;
+  a.tol = tol;
+  a.spin_limit = 200000000ull;  // 2 s of the 100 MHz clock: far beyond any legitimate wait
+  if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[0], st));
+  CS_HIP(cs_persist_launch(st, a, cs_persist_groups(nactive), true));
+  if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[1], st));
+  CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
+  CS_HIP(hipStreamSynchronize(st));
+  const cs_solve_state& hs = *m->h_state;
+  if (hs.faults) return fail(CS_ERR_DEVICE, "persistent pressure solve: a workgroup gave up waiting for its neighbours");
+  ++m->persist_solves;
+  m->persist_epoch += span;
+  S.iterations = hs.iterations;
+  S.initial_residual = fixed ? 0.0 : hs.initial_residual;
+  S.final_residual = hs.residual;
+  S.converged = hs.residual <= tol ? 1u : 0u;
+  if (m->timing) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, m->chunk_ev[0], m->chunk_ev[1]);
+    m->kt.rbgs_us += (double)ms * 1000.0;
+    m->kt.rbgs_launches += 1;
+    m->kt.rbgs_cells_swept += (uint64_t)hs.iterations * nactive * CS_BRICK * CS_BRICK * CS_BRICK;
+  }
+  return CS_OK;
+}
+
 // Brick-sparse solve (cs_solver.hip) with the convergence test on the device.
 int solve_bricks(cs_sim* m, const cs_params& u) {
   hipStream_t st = m->ctx->stream;
@@ -338,6 +407,13 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
   CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
   CS_HIP(hipStreamSynchronize(st));
   const uint32_t nactive = m->h_state->active_bricks;
+  if (m->persist && !m->s.rbgs_rescale_after && nactive > 0 && cs_persist_groups(nactive) <= m->persist_max_groups &&
+      (!fixed || m->s.pressure_rbgs_pairs > 0)) {
+    int rc = solve_persistent(m, u, nactive, external);
+    if (rc) return rc;
+    if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_copy_i32(st, m->pres, m->prev_p, m->nc));
+    return CS_OK;
+  }
   int cur = 0;
   if (fixed) {
     uint32_t launches = 0;
@@ -732,8 +808,18 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   ALLOC(m->brick_list, geom.nbricks);
   ALLOC(m->brick_partials, 2 * (size_t)geom.nbricks);
   ALLOC(m->dstate, 1);
+  ALLOC(m->persist_fb, (size_t)geom.nbricks * CS_PERSIST_FACE_WORDS);
+  ALLOC(m->persist_pflags, (size_t)geom.nbricks * CS_PERSIST_FLAG_STRIDE);
+  ALLOC(m->persist_partials, 4 * (size_t)geom.nbricks);
 #undef ALLOC
   // solver variant (benchmarking and cross-checks; both give identical results)
+  // a named layout of the launch-per-chunk solve turns the persistent solve off
+  for (const char* k : {"CS_SOLVER", "CS_RBGS_HALO", "CS_RBGS_MULTI", "CS_RBGS_FUSE", "CS_RBGS_GROUPS",
+                        "CS_RBGS_THREADS", "CS_RBGS_BATCH"})
+    if (getenv(k)) m->persist = false;
+  if (const char* v = getenv("CS_RBGS_PERSIST")) m->persist = atoi(v) != 0;
+  if (m->persist && cs_persist_max_groups(m->ctx->device, &m->persist_max_groups) != hipSuccess)
+    m->persist = false;
   if (const char* v = getenv("CS_SOLVER")) m->dense_solver = strcmp(v, "dense") == 0;
   if (const char* v = getenv("CS_RBGS_HALO")) {
     int h = atoi(v);
@@ -778,10 +864,13 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
     return fail(e == hipErrorOutOfMemory ? CS_ERR_NOMEM : CS_ERR_DEVICE, std::string("allocation: ") + hipGetErrorString(e));
   }
   hipStream_t st = ctx->stream;
+  // no surface word is current before the first persistent solve
+  e = hipMemsetAsync(m->persist_fb, 0, (size_t)geom.nbricks * CS_PERSIST_FACE_WORDS * sizeof(uint64_t), st);
   std::vector<uint32_t> sweep_table(CS_SWEEP_TABLE_WORDS);
   cs_build_sweep_table(sweep_table.data());
-  e = hipMemcpyAsync(m->sweep_table, sweep_table.data(), sweep_table.size() * sizeof(uint32_t),
-                     hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(m->sweep_table, sweep_table.data(), sweep_table.size() * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
     e = sdf_src ? hipMemcpyAsync(m->sdf, sdf_src, nsdf * sizeof(float), hipMemcpyHostToDevice, st)
                 : cs_launch_gen_sdf(st, *s, m->sdf);

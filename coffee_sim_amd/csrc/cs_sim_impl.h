@@ -8,6 +8,7 @@
 
 #include "cs_ctx.h"
 #include "cs_host.h"
+#include "cs_persist.h"
 #include "cs_solver.h"
 #include "cs_unet_host.h"
 
@@ -75,6 +76,14 @@ struct cs_sim {
   hipGraph_t batch_graph = nullptr;
   hipGraphExec_t batch_exec = nullptr;
   uint32_t batch_launches_per_chunk = 0;
+  // persistent solve (cs_rbgs_persist.hip): the whole solve in one launch
+  bool persist = true;               // CS_RBGS_PERSIST (off when another layout is asked for)
+  uint32_t persist_max_groups = 0;   // co-resident workgroups of the persistent kernel
+  uint64_t* persist_fb = nullptr;    // [nbricks * CS_PERSIST_FACE_WORDS] face buffers
+  uint32_t persist_epoch = 1;        // base of the next solve's epochs
+  uint32_t* persist_pflags = nullptr;
+  double* persist_partials = nullptr;  // [2 * nbricks * 2]
+  uint64_t persist_solves = 0;       // solves run by the persistent kernel
   // tile-binned P2G scratch
   bool p2g_tiled = true;             // CS_P2G=atomic selects the one-thread-per-particle layout
   uint32_t* tile_count = nullptr;    // [nbricks]

@@ -16,6 +16,8 @@ struct cs_solve_state {
   float rescale;           // Galerkin factor of the one-time rescale (rbgs_rescale_after)
   double residual;         // latest relative residual
   double initial_residual; // before the first sweep
+  uint32_t abort;          // persistent solve: a workgroup gave up waiting, the others stop
+  uint32_t faults;         // persistent solve: workgroups that gave up
 };
 
 hipError_t cs_launch_brick_scan(hipStream_t s, const cs_params& u, const uint32_t* flags, uint32_t* list,

@@ -212,11 +212,18 @@ def param_dict(v: np.ndarray) -> dict:
 
 
 def adam_step(p: np.ndarray, g: np.ndarray, m: np.ndarray, v: np.ndarray, t: int, lr: float,
-              b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
-    """One Adam step (Kingma & Ba 2015, Algorithm 1) in float64, step t >= 1.
-    Returns the new (parameters, first moment, second moment)."""
+              b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8, folded: bool = False):
+    """One Adam step (Kingma & Ba 2015) in float64, step t >= 1.  Returns the
+    new (parameters, first moment, second moment).  `folded` gives the form of
+    the paper's Section 2 closing remark that the device uses: the bias
+    corrections folded into the step, lr_t = lr sqrt(1 - b2^t) / (1 - b1^t),
+    and p -= lr_t m / (sqrt(v) + eps), which is Algorithm 1 with eps divided
+    by sqrt(1 - b2^t)."""
     m = b1 * m + (1.0 - b1) * g
     v = b2 * v + (1.0 - b2) * g * g
+    if folded:
+        lr_t = lr * np.sqrt(1.0 - b2 ** t) / (1.0 - b1 ** t)
+        return p - lr_t * m / (np.sqrt(v) + eps), m, v
     mh = m / (1.0 - b1 ** t)
     vh = v / (1.0 - b2 ** t)
     return p - lr * mh / (np.sqrt(vh) + eps), m, v

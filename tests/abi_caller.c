@@ -89,11 +89,12 @@ static int device_part(void) {
   free(a);
   free(c);
   free(blob);
+  const uint32_t pairs = w.settings.pressure_rbgs_pairs;
   CHECK(cs_sim_destroy(sim));
   CHECK(cs_ctx_destroy(ctx));
-  CHECK(cs_workload_free(&w));
+  CHECK(cs_workload_free(&w));  /* clears w */
   if (!same) return fprintf(stderr, "restored run differs\n"), 1;
-  if (m[1] == 0 || st.iterations != w.settings.pressure_rbgs_pairs) return fprintf(stderr, "no solve\n"), 1;
+  if (m[1] == 0 || st.iterations != pairs) return fprintf(stderr, "no solve: %u fluid cells, %u iterations\n", m[1], st.iterations), 1;
   printf("abi caller ok: %u particles, %u fluid cells, %u iterations, post-projection max|div| %.3f\n", nw, m[1],
          st.iterations, m[5] / 1024.0);
   return 0;

@@ -108,11 +108,10 @@ def test_training_lowers_the_loss(ctx):
 
 
 def test_device_adam_matches_the_oracle_adam(ctx):
-    """Two train steps on the device against oracle/unet_ref.adam_step applied
-    to the device's own gradient at each step.  The device folds the bias
-    correction into the step size, lr_t = lr sqrt(1 - b2^t) / (1 - b1^t), which
-    leaves eps unscaled; the two forms agree to 1e-4 relative where |g| is far
-    above eps, and every step stays within lr."""
+    """Two train steps on the device against oracle/unet_ref.adam_step (its
+    folded form, which the device implements: the bias corrections in the
+    step size) applied to the device's own gradient at each step; every step
+    stays within lr."""
     net = cs.UNet(ctx, N, seed=7)
     f4, target, mask = _inputs(9)
     lr = 1e-3
@@ -122,13 +121,14 @@ def test_device_adam_matches_the_oracle_adam(ctx):
     for t in (1, 2):
         _, g = net.gradient(f4, target, mask)
         g = g.astype(np.float64)
-        want, m, v = U.adam_step(p, g, m, v, t, lr)
+        want, m, v = U.adam_step(p, g, m, v, t, lr, folded=True)
         net.train_step(f4, target, mask, lr)
         got = net.get_params().astype(np.float64)
-        big = np.abs(g) > 1e-4
+        big = np.abs(g) > 1e-6
         assert big.sum() > 1000
         d_got, d_want = got - p, want - p
-        assert np.abs(d_got[big] - d_want[big]).max() <= 1e-4 * lr + 2e-7 * np.abs(p[big]).max(), t
+        # float32 parameters and moments: a few ulps of p plus 1e-4 of the step
+        assert np.all(np.abs(d_got[big] - d_want[big]) <= 1e-4 * lr + 4e-7 * np.abs(p[big])), t
         assert np.all(np.abs(d_got) <= lr * (1 + 1e-3) + 1e-7 * np.abs(p)), t
         p = got
     net.close()

@@ -116,3 +116,12 @@ def test_adam_closed_forms():
     assert np.allclose(p2 - p, -lr * mh / (np.sqrt(vh) + 1e-8), rtol=1e-12)
     # the step is bounded by lr (here about 0.0526 lr / 1.0)
     assert np.all(np.abs(p2 - p) <= lr)
+    # the folded form is Algorithm 1 with eps divided by sqrt(1 - b2^t)
+    m, v = np.zeros(64), np.zeros(64)
+    ma, va = np.zeros(64), np.zeros(64)
+    pa = pb = p0
+    for t in range(1, 4):
+        gt = g * (1.0 + 0.3 * t)
+        pa, ma, va = U.adam_step(pa, gt, ma, va, t, lr, folded=True)
+        pb, m, v = U.adam_step(pb, gt, m, v, t, lr, eps=1e-8 / np.sqrt(1.0 - 0.999 ** t))
+        assert np.allclose(pa, pb, rtol=1e-13, atol=1e-16), t
