@@ -16,6 +16,8 @@
 struct cs_persist_args {
   cs_params u;
   const uint32_t* list;      // active bricks, ascending (cs_launch_brick_scan)
+  const int32_t* slot;       // per brick: its position in `list` (its run), or -1
+  uint32_t* xcc;             // [max_runs]: 1 + the XCC id each run's workgroup runs on
   const float4* wc;          // face weights and diagonal per cell (cs_launch_face_weights)
   const float* rhs;          // c = dx^2 * rhs per cell
   int32_t* p;                // the pressure: the start on entry, the solution on exit
@@ -34,8 +36,10 @@ struct cs_persist_args {
   uint64_t spin_limit;       // 100 MHz ticks a wait may last before the solve gives up
 };
 
-// Clears the residual-check flags and the failure state.
-hipError_t cs_persist_prepare(hipStream_t s, const cs_params& u, cs_solve_state* st, uint32_t* pflags);
+// Clears the slot map, check flags, XCD table and failure state, then maps each
+// listed brick to its slot (reads the active count from `st`).
+hipError_t cs_persist_prepare(hipStream_t s, const cs_params& u, const uint32_t* list, cs_solve_state* st,
+                              int32_t* slot, uint32_t* pflags, uint32_t* xcc);
 // Epochs a solve of at most `iters` red+black pairs uses (the next solve's base moves by this).
 inline uint32_t cs_persist_epochs(uint32_t iters) { return 2u * iters + 2u; }
 // Workgroups launched for `nactive` bricks (a multiple of 8, one per XCD slice).

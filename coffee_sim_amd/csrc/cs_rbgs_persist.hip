@@ -71,6 +71,20 @@ __device__ __forceinline__ void st_dev(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Coherent within one XCD: a non-temporal store, and a load with the sc0 and
+// nt policies, which misses the CU's L1 and is served by the XCD's shared L2.
+// (A plain sc0 load can hit a stale line in the reading CU's L1.)  Used only
+// between workgroups that read the same XCC id; `r` covers the face buffers.
+__device__ __forceinline__ uint64_t ld_xcd(__amdgpu_buffer_rsrc_t r, int32_t word) {
+  return __builtin_bit_cast(uint64_t, __builtin_amdgcn_raw_buffer_load_b64(r, word * 8, 0, 3 /* sc0 | nt */));
+}
+__device__ __forceinline__ void st_xcd(__amdgpu_buffer_rsrc_t r, int32_t word, uint64_t v) {
+  typedef __attribute__((ext_vector_type(2))) unsigned u2;
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, word * 8, 0, 2 /* nt */);
+}
+// s_getreg of HW_REG_XCC_ID (20), bits [3:0]
+__device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg(20 | (3 << 11)); }
+
 // Waits until every memory operation this wavefront issued has completed (the
 // stores have reached the device-coherent level).  The empty asm statements
 // keep the compiler from moving memory accesses across the wait.
@@ -123,6 +137,8 @@ struct Cell {
   int li;        // index in the staged 10^3 region
   int32_t g;     // global cell index (-1: past the grid's edge)
   int fslot[3];  // per axis: its word in the face buffer of the brick face it lies on, or -1
+  int fface[3];  // that face (0..5), or -1
+  uint32_t floc; // bit j: the neighbour across face fface[j] shares this XCD
   float4 self;   // +x, +y, +z weights and diagonal (w < 0: not fluid)
   float rhs;     // c = dx^2 * rhs of the update
   float wxm, wym, wzm;
@@ -154,6 +170,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
   Cell cell[2];
   int hl[2];
   int32_t hg[2];
+  int hf[2];     // face of the halo cell
   const int m = (int)(t & 3u), ly = (int)((t >> 2) & 7u), lz = (int)(t >> 5);
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
@@ -169,9 +186,8 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
       for (int axis = 0; axis < 3; ++axis) {
         const int a1 = lc[axis == 0 ? 1 : 0], b2 = lc[axis == 2 ? 1 : 2];
         const int k = (b2 << 2) | (a1 >> 1);
-        e.fslot[axis] = lc[axis] == 0        ? face_word(brick, 2 * axis, c, k)
-                        : lc[axis] == BR - 1 ? face_word(brick, 2 * axis + 1, c, k)
-                                             : -1;
+        e.fface[axis] = lc[axis] == 0 ? 2 * axis : lc[axis] == BR - 1 ? 2 * axis + 1 : -1;
+        e.fslot[axis] = e.fface[axis] >= 0 ? face_word(brick, e.fface[axis], c, k) : -1;
       }
     }
     // a brick on the grid's far edge may reach past it: such cells are not fluid
@@ -186,6 +202,8 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
     // fluid, hence in an active brick whose workgroup publishes it every epoch
     hl[c] = -1;
     hg[c] = -1;
+    hf[c] = 0;
+    e.floc = 0u;
     if (t < 192u) {
       const int f = (int)(t >> 5), k = (int)(t & 31u);
       const int axis = f >> 1, side = f & 1;
@@ -204,10 +222,15 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
           // the neighbour across face f publishes this cell on its opposite face, same word
           const uint32_t nb = ((uint32_t)(hz / BR) * u.nby + (uint32_t)(hy / BR)) * u.nbx + (uint32_t)(hx / BR);
           hg[c] = face_word(nb, f ^ 1, c, k);
+          hf[c] = f;
         }
       }
     }
   }
+  // ---- this workgroup's XCD, read by the neighbours after the first check ---------------------
+  const uint32_t my_xcc = xcc_id();
+  if (t == 0) st_dev(&a.xcc[run], my_xcc + 1u);  // 0: not published
+
   // ---- stage the pressure region (device-coherent loads: other workgroups write it) ----------
   for (int i = (int)t; i < PR3; i += NT) {
     const int lx = i % PR - 1, lyy = (i / PR) % PR - 1, lzz = i / PR2 - 1;
@@ -299,7 +322,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
         b0 += s_red[2 * w + 1];
       }
       const double r = rel_residual(r0, b0);
-      const bool stop = a.fixed_pairs ? true : (r <= a.tol || iters >= a.cap);
+      const bool stop = a.fixed_pairs ? iters >= a.fixed_pairs : (r <= a.tol || iters >= a.cap);
       s_verdict = stop ? 1u : 0u;
       if (run == 0) {
         if (checks == 0) a.st->initial_residual = r;
@@ -313,10 +336,35 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
     return s_verdict != 0u || s_fail != 0u;
   };
 
-  // tolerance mode tests the start (the oracle's initial residual)
-  bool stop = a.fixed_pairs ? false : check(0u);
+  // the first test (the oracle's initial residual in tolerance mode) is also the
+  // point after which every workgroup's XCD is published
+  bool stop = check(0u);
+  {
+    uint32_t local = 0u;  // bit f: the brick across face f is served on this XCD
+    const int bc[3] = {(int)bx, (int)by, (int)bz};
+    const int nbd[3] = {(int)u.nbx, (int)u.nby, (int)u.nbz};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      int q[3] = {bc[0], bc[1], bc[2]};
+      q[f >> 1] += (f & 1) ? 1 : -1;
+      if (q[f >> 1] < 0 || q[f >> 1] >= nbd[f >> 1]) continue;
+      const int32_t sl = a.slot[((uint32_t)q[2] * u.nby + (uint32_t)q[1]) * u.nbx + (uint32_t)q[0]];
+      if (sl >= 0 && ld_dev(&a.xcc[sl]) == my_xcc + 1u) local |= 1u << f;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (cell[c].fface[j] >= 0 && ((local >> cell[c].fface[j]) & 1u)) cell[c].floc |= 1u << j;
+      hf[c] = (local >> hf[c]) & 1u;  // from here on: 1 when the halo word is XCD-local
+    }
+  }
+  // the face buffers as a raw buffer (32-bit data format), for the XCD-local words
+  const __amdgpu_buffer_rsrc_t fbr =
+      __builtin_amdgcn_make_buffer_rsrc(a.fb, 0, (int)(u.nbricks * CS_PERSIST_FACE_WORDS * 8u), 0x00020000);
+
   // one half sweep of colour c; returns false when a wait gave up
-  auto half_sweep = [&](const Cell& e, int h_l, int32_t h_g) -> bool {
+  auto half_sweep = [&](const Cell& e, int h_l, int32_t h_g, int h_local) -> bool {
     ++epoch;
     const uint32_t want = a.epoch_base + epoch;
     bool fired;
@@ -326,21 +374,27 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
       fires += fired ? 1u : 0u;
       const uint64_t word = (uint64_t)want << 32 | (uint32_t)(int32_t)(np * u.fp_scale);
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        if (e.fslot[j] >= 0) st_dev(&a.fb[e.fslot[j]], word);
+      for (int j = 0; j < 3; ++j) {
+        if (e.fslot[j] < 0) continue;
+        if ((e.floc >> j) & 1u) st_xcd(fbr, e.fslot[j], word);
+        else st_dev(&a.fb[e.fslot[j]], word);
+      }
     }
     // the colour-c halo of this epoch, polled until its word carries the epoch
     if (h_g >= 0) {
-      uint64_t v = ld_dev(&a.fb[h_g]);
+      uint64_t v = h_local ? ld_xcd(fbr, h_g) : ld_dev(&a.fb[h_g]);
       if ((uint32_t)(v >> 32) < want) {
+        // the abort flag and the clock are looked at every 32nd poll only, so a
+        // retry costs one load latency
         const uint64_t t0 = now_ticks();
+        uint32_t n = 0;
         do {
-          if (ld_dev(&a.st->abort) || now_ticks() - t0 > a.spin_limit) {
+          __builtin_amdgcn_s_sleep(1);
+          v = h_local ? ld_xcd(fbr, h_g) : ld_dev(&a.fb[h_g]);
+          if ((++n & 31u) == 0u && (ld_dev(&a.st->abort) || now_ticks() - t0 > a.spin_limit)) {
             atomicOr(&s_fail, 1u);
             break;
           }
-          __builtin_amdgcn_s_sleep(1);
-          v = ld_dev(&a.fb[h_g]);
         } while ((uint32_t)(v >> 32) < want);
       }
       s_p[h_l] = (float)(int32_t)(uint32_t)v * u.inv_fp_scale;
@@ -350,7 +404,7 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
   };
 
   while (!stop && it < limit) {
-    if (!half_sweep(cell[0], hl[0], hg[0]) || !half_sweep(cell[1], hl[1], hg[1])) break;
+    if (!half_sweep(cell[0], hl[0], hg[0], hf[0]) || !half_sweep(cell[1], hl[1], hg[1], hf[1])) break;
     ++it;
     if (a.fixed_pairs ? it == a.fixed_pairs : (it % a.every == 0u || it >= a.cap)) stop = check(it);
   }
@@ -370,19 +424,36 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
   if ((t & 63u) == 0u && fires) atomicAdd(&a.metrics[CS_M_P_CLAMP], fires);
 }
 
-// Clears the residual-check flags and the failure state of a solve.
+// Clears the slot map, the residual-check flags, the XCD table and the
+// failure state of a solve.
 __global__ void __launch_bounds__(256) k_persist_prepare(cs_solve_state* __restrict__ st, uint32_t nbricks,
-                                                         uint32_t* __restrict__ pflags) {
+                                                         int32_t* __restrict__ slot, uint32_t* __restrict__ pflags,
+                                                         uint32_t* __restrict__ xcc) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nbricks) pflags[(size_t)i * CS_PERSIST_FLAG_STRIDE] = 0u;
+  if (i < nbricks) {
+    slot[i] = -1;
+    pflags[(size_t)i * CS_PERSIST_FLAG_STRIDE] = 0u;
+    xcc[i] = 0u;
+  }
   if (i == 0) {
     st->abort = 0u;
     st->faults = 0u;
   }
 }
 
-hipError_t cs_persist_prepare(hipStream_t s, const cs_params& u, cs_solve_state* st, uint32_t* pflags) {
-  hipLaunchKernelGGL(k_persist_prepare, dim3((u.nbricks + 255u) / 256u), dim3(256), 0, s, st, u.nbricks, pflags);
+// Each listed brick's position in the list (its workgroup's run).
+__global__ void __launch_bounds__(256) k_persist_slots(const uint32_t* __restrict__ list,
+                                                       const cs_solve_state* __restrict__ st,
+                                                       int32_t* __restrict__ slot) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < st->active_bricks) slot[list[i]] = (int32_t)i;
+}
+
+hipError_t cs_persist_prepare(hipStream_t s, const cs_params& u, const uint32_t* list, cs_solve_state* st,
+                              int32_t* slot, uint32_t* pflags, uint32_t* xcc) {
+  const dim3 g((u.nbricks + 255u) / 256u);
+  hipLaunchKernelGGL(k_persist_prepare, g, dim3(256), 0, s, st, u.nbricks, slot, pflags, xcc);
+  hipLaunchKernelGGL(k_persist_slots, g, dim3(256), 0, s, list, st, slot);
   return hipGetLastError();
 }
 

@@ -63,7 +63,7 @@ void free_sim(cs_sim* m) {
                  m->brick_flag, m->brick_list, m->brick_partials, m->dstate,       m->tile_count,
                  m->tile_start, m->p2g_key,    m->p2g_sorted, m->render,     m->bed_part,
                  m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt,
-                 m->persist_fb, m->persist_pflags, m->persist_partials};
+                 m->persist_fb, m->persist_pflags, m->persist_partials, m->persist_slot, m->persist_xcc};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -338,10 +338,12 @@ int solve_persistent(cs_sim* m, const cs_params& u, uint32_t nactive, bool exter
                               2 * (size_t)u.nbricks * CS_PERSIST_FACE_WORDS, 0));
     m->persist_epoch = 1;
   }
-  CS_HIP(cs_persist_prepare(st, u, m->dstate, m->persist_pflags));
+  CS_HIP(cs_persist_prepare(st, u, m->brick_list, m->dstate, m->persist_slot, m->persist_pflags, m->persist_xcc));
   cs_persist_args a{};
   a.u = u;
   a.list = m->brick_list;
+  a.slot = m->persist_slot;
+  a.xcc = m->persist_xcc;
   a.wc = reinterpret_cast<const float4*>(m->wc);
   a.rhs = m->rhs;
   a.p = m->pres;
@@ -809,6 +811,8 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   ALLOC(m->brick_partials, 2 * (size_t)geom.nbricks);
   ALLOC(m->dstate, 1);
   ALLOC(m->persist_fb, (size_t)geom.nbricks * CS_PERSIST_FACE_WORDS);
+  ALLOC(m->persist_slot, geom.nbricks);
+  ALLOC(m->persist_xcc, geom.nbricks);
   ALLOC(m->persist_pflags, (size_t)geom.nbricks * CS_PERSIST_FLAG_STRIDE);
   ALLOC(m->persist_partials, 4 * (size_t)geom.nbricks);
 #undef ALLOC

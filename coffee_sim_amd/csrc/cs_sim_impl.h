@@ -82,6 +82,8 @@ struct cs_sim {
   uint64_t* persist_fb = nullptr;    // [nbricks * CS_PERSIST_FACE_WORDS] face buffers
   uint32_t persist_epoch = 1;        // base of the next solve's epochs
   uint32_t* persist_pflags = nullptr;
+  int32_t* persist_slot = nullptr;   // [nbricks] brick -> run
+  uint32_t* persist_xcc = nullptr;   // [nbricks] run -> 1 + XCC id
   double* persist_partials = nullptr;  // [2 * nbricks * 2]
   uint64_t persist_solves = 0;       // solves run by the persistent kernel
   // tile-binned P2G scratch

@@ -110,8 +110,8 @@ def test_training_lowers_the_loss(ctx):
 def test_device_adam_matches_the_oracle_adam(ctx):
     """Two train steps on the device against oracle/unet_ref.adam_step (its
     folded form, which the device implements: the bias corrections in the
-    step size) applied to the device's own gradient at each step; every step
-    stays within lr."""
+    step size) applied to the device's own gradient at each step; the first
+    step moves no parameter by more than lr."""
     net = cs.UNet(ctx, N, seed=7)
     f4, target, mask = _inputs(9)
     lr = 1e-3
@@ -129,7 +129,8 @@ def test_device_adam_matches_the_oracle_adam(ctx):
         d_got, d_want = got - p, want - p
         # float32 parameters and moments: a few ulps of p plus 1e-4 of the step
         assert np.all(np.abs(d_got[big] - d_want[big]) <= 1e-4 * lr + 4e-7 * np.abs(p[big])), t
-        assert np.all(np.abs(d_got) <= lr * (1 + 1e-3) + 1e-7 * np.abs(p)), t
+        if t == 1:  # lr g / (|g| + eps) in the first step
+            assert np.all(np.abs(d_got) <= lr * (1 + 1e-3) + 1e-7 * np.abs(p))
         p = got
     net.close()
 
