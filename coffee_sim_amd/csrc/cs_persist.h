@@ -12,6 +12,7 @@
 #define CS_PERSIST_THREADS 256
 #define CS_PERSIST_FLAG_STRIDE 16u  // u32 words between two workgroups' flags (64 B)
 #define CS_PERSIST_FACE_WORDS (6u * 2u * 32u)  // face-buffer words per brick
+#define CS_PERSIST_MAX_BRICKS 1024              // active bricks one persistent solve takes
 
 struct cs_persist_args {
   cs_params u;
@@ -34,6 +35,7 @@ struct cs_persist_args {
   uint32_t cap;              // iteration cap (tolerance mode)
   double tol;
   uint64_t spin_limit;       // 100 MHz ticks a wait may last before the solve gives up
+  uint32_t poll_delay;       // s_sleep(1) steps (64 clocks each) before a halo word's first poll
 };
 
 // Clears the slot map, check flags, XCD table and failure state, then maps each

@@ -149,7 +149,7 @@ struct Cell {
 // Workgroup w serves brick `run` of the active list; runs are dealt so that a
 // contiguous part of the (ascending, hence z-major) list lands on one XCD
 // (workgroup w runs on XCD w % 8).
-__global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
+__global__ void __launch_bounds__(NT, 4) k_rbgs_persist(cs_persist_args a) {
   const cs_params& u = a.u;
   const uint32_t t = threadIdx.x;
   const uint32_t nactive = a.st->active_bricks;
@@ -289,47 +289,90 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
       stores_complete();
       st_dev(&a.pflags[(size_t)run * CS_PERSIST_FLAG_STRIDE], checks + 1u);
     }
-    // every brick's sums, in run order within each thread, then a fixed tree
-    double tr = 0.0, tb = 0.0;
-    const uint64_t t0 = now_ticks();
-    for (uint32_t q = t; q < nactive; q += NT) {
-      const uint32_t* pf = &a.pflags[(size_t)q * CS_PERSIST_FLAG_STRIDE];
-      while (ld_dev(pf) < checks + 1u) {
-        if (ld_dev(&a.st->abort) || now_ticks() - t0 > a.spin_limit) {
+    // Run 0's workgroup gathers every brick's sums (all of its loads in flight
+    // together), reduces them in one fixed order and publishes the verdict as
+    // one word (check number << 32 | stop); the others poll that word.
+    const uint32_t want = checks + 1u;
+    if (run == 0) {
+      constexpr int QMAX = CS_PERSIST_MAX_BRICKS / NT;
+      uint32_t fl[QMAX];
+#pragma unroll
+      for (int j = 0; j < QMAX; ++j) {
+        const uint32_t q = t + (uint32_t)j * NT;
+        fl[j] = q < nactive ? ld_dev(&a.pflags[(size_t)q * CS_PERSIST_FLAG_STRIDE]) : want;
+      }
+      const uint64_t t0 = now_ticks();
+      uint32_t n = 0;
+      for (;;) {
+        bool ready = true;
+#pragma unroll
+        for (int j = 0; j < QMAX; ++j) ready &= fl[j] >= want;
+        if (ready) break;
+        if ((++n & 31u) == 0u && (ld_dev(&a.st->abort) || now_ticks() - t0 > a.spin_limit)) {
           failed = true;
           break;
         }
         __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int j = 0; j < QMAX; ++j) {
+          const uint32_t q = t + (uint32_t)j * NT;
+          if (fl[j] < want) fl[j] = ld_dev(&a.pflags[(size_t)q * CS_PERSIST_FLAG_STRIDE]);
+        }
       }
-      if (failed) break;
-      const uint64_t* src = reinterpret_cast<const uint64_t*>(a.partials) + ((size_t)slot * a.max_runs + q) * 2;
-      tr += __longlong_as_double(ld_dev(src));
-      tb += __longlong_as_double(ld_dev(src + 1));
-    }
-    if (failed) atomicOr(&s_fail, 1u);
-    tr = wave_sum_d(tr);
-    tb = wave_sum_d(tb);
-    __syncthreads();
-    if ((t & 63u) == 0u) {
-      s_red[2 * (t >> 6)] = tr;
-      s_red[2 * (t >> 6) + 1] = tb;
-    }
-    __syncthreads();
-    if (t == 0) {
-      double r0 = 0.0, b0 = 0.0;
-      for (int w = 0; w < NT / 64; ++w) {
-        r0 += s_red[2 * w];
-        b0 += s_red[2 * w + 1];
+      double tr = 0.0, tb = 0.0;
+      if (!failed) {
+        uint64_t pr[QMAX], pb[QMAX];
+#pragma unroll
+        for (int j = 0; j < QMAX; ++j) {
+          const uint32_t q = t + (uint32_t)j * NT;
+          const uint64_t* src = reinterpret_cast<const uint64_t*>(a.partials) + ((size_t)slot * a.max_runs + q) * 2;
+          pr[j] = q < nactive ? ld_dev(src) : 0ull;
+          pb[j] = q < nactive ? ld_dev(src + 1) : 0ull;
+        }
+#pragma unroll
+        for (int j = 0; j < QMAX; ++j) {
+          tr += __longlong_as_double(pr[j]);
+          tb += __longlong_as_double(pb[j]);
+        }
       }
-      const double r = rel_residual(r0, b0);
-      const bool stop = a.fixed_pairs ? iters >= a.fixed_pairs : (r <= a.tol || iters >= a.cap);
-      s_verdict = stop ? 1u : 0u;
-      if (run == 0) {
+      if (failed) atomicOr(&s_fail, 1u);
+      tr = wave_sum_d(tr);
+      tb = wave_sum_d(tb);
+      __syncthreads();
+      if ((t & 63u) == 0u) {
+        s_red[2 * (t >> 6)] = tr;
+        s_red[2 * (t >> 6) + 1] = tb;
+      }
+      __syncthreads();
+      if (t == 0) {
+        double r0 = 0.0, b0 = 0.0;
+        for (int w = 0; w < NT / 64; ++w) {
+          r0 += s_red[2 * w];
+          b0 += s_red[2 * w + 1];
+        }
+        const double r = rel_residual(r0, b0);
+        const bool stop = a.fixed_pairs ? iters >= a.fixed_pairs : (r <= a.tol || iters >= a.cap);
+        s_verdict = stop ? 1u : 0u;
         if (checks == 0) a.st->initial_residual = r;
         a.st->residual = r;
         a.st->iterations = iters;
         a.st->done = r <= a.tol ? 1u : 0u;
+        stores_complete();
+        st_dev(&a.st->verdict, (uint64_t)want << 32 | (stop ? 1u : 0u));
       }
+    } else if (t == 0) {
+      uint64_t v = ld_dev(&a.st->verdict);
+      const uint64_t t0 = now_ticks();
+      uint32_t n = 0;
+      while ((uint32_t)(v >> 32) < want) {
+        if ((++n & 31u) == 0u && (ld_dev(&a.st->abort) || now_ticks() - t0 > a.spin_limit)) {
+          atomicOr(&s_fail, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        v = ld_dev(&a.st->verdict);
+      }
+      s_verdict = (uint32_t)v & 1u;
     }
     ++checks;
     __syncthreads();
@@ -382,6 +425,9 @@ __global__ void __launch_bounds__(NT) k_rbgs_persist(cs_persist_args a) {
     }
     // the colour-c halo of this epoch, polled until its word carries the epoch
     if (h_g >= 0) {
+      // the neighbour stores its word about when this one is stored: a first
+      // poll issued at once would mostly come back stale and cost a round trip
+      for (uint32_t d = 0; d < a.poll_delay; ++d) __builtin_amdgcn_s_sleep(1);
       uint64_t v = h_local ? ld_xcd(fbr, h_g) : ld_dev(&a.fb[h_g]);
       if ((uint32_t)(v >> 32) < want) {
         // the abort flag and the clock are looked at every 32nd poll only, so a
@@ -438,6 +484,7 @@ __global__ void __launch_bounds__(256) k_persist_prepare(cs_solve_state* __restr
   if (i == 0) {
     st->abort = 0u;
     st->faults = 0u;
+    st->verdict = 0ull;
   }
 }
 

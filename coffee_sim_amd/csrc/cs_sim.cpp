@@ -358,7 +358,8 @@ int solve_persistent(cs_sim* m, const cs_params& u, uint32_t nactive, bool exter
   a.every = m->s.rbgs_check_every ? m->s.rbgs_check_every : 8u;
   a.cap = m->s.rbgs_max_iters;
   a.tol = tol;
-  a.spin_limit = 200000000ull;  // 2 s of the 100 MHz clock: far beyond any legitimate wait
+  a.spin_limit = 200000000ull;
+  a.poll_delay = m->persist_poll_delay;  // 2 s of the 100 MHz clock: far beyond any legitimate wait
   if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[0], st));
   CS_HIP(cs_persist_launch(st, a, cs_persist_groups(nactive), true));
   if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[1], st));
@@ -409,7 +410,8 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
   CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
   CS_HIP(hipStreamSynchronize(st));
   const uint32_t nactive = m->h_state->active_bricks;
-  if (m->persist && !m->s.rbgs_rescale_after && nactive > 0 && cs_persist_groups(nactive) <= m->persist_max_groups &&
+  if (m->persist && !m->s.rbgs_rescale_after && nactive > 0 && nactive <= CS_PERSIST_MAX_BRICKS &&
+      cs_persist_groups(nactive) <= m->persist_max_groups &&
       (!fixed || m->s.pressure_rbgs_pairs > 0)) {
     int rc = solve_persistent(m, u, nactive, external);
     if (rc) return rc;
@@ -822,6 +824,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
                         "CS_RBGS_THREADS", "CS_RBGS_BATCH"})
     if (getenv(k)) m->persist = false;
   if (const char* v = getenv("CS_RBGS_PERSIST")) m->persist = atoi(v) != 0;
+  if (const char* v = getenv("CS_PERSIST_POLL_DELAY")) m->persist_poll_delay = (uint32_t)std::min(std::max(atoi(v), 0), 256);
   if (m->persist && cs_persist_max_groups(m->ctx->device, &m->persist_max_groups) != hipSuccess)
     m->persist = false;
   if (const char* v = getenv("CS_SOLVER")) m->dense_solver = strcmp(v, "dense") == 0;

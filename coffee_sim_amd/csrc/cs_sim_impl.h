@@ -86,6 +86,7 @@ struct cs_sim {
   uint32_t* persist_xcc = nullptr;   // [nbricks] run -> 1 + XCC id
   double* persist_partials = nullptr;  // [2 * nbricks * 2]
   uint64_t persist_solves = 0;       // solves run by the persistent kernel
+  uint32_t persist_poll_delay = 8;   // CS_PERSIST_POLL_DELAY: about 0.25 us, measured best of 0-24
   // tile-binned P2G scratch
   bool p2g_tiled = true;             // CS_P2G=atomic selects the one-thread-per-particle layout
   uint32_t* tile_count = nullptr;    // [nbricks]

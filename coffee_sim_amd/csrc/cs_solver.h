@@ -18,6 +18,7 @@ struct cs_solve_state {
   double initial_residual; // before the first sweep
   uint32_t abort;          // persistent solve: a workgroup gave up waiting, the others stop
   uint32_t faults;         // persistent solve: workgroups that gave up
+  uint64_t verdict;        // persistent solve: (residual test number << 32) | stop
 };
 
 hipError_t cs_launch_brick_scan(hipStream_t s, const cs_params& u, const uint32_t* flags, uint32_t* list,
