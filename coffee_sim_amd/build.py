@@ -21,7 +21,8 @@ ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = os.environ.get("CS_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["cs_scene.cpp", "cs_bedinit.cpp", "cs_workload.cpp", "cs_sim.cpp", "cs_unet.cpp", "cs_batch.cpp"]
-DEVICE_SOURCES = ["cs_kernels.hip", "cs_solver.hip", "cs_rbgs_persist.hip", "cs_p2g.hip", "cs_unet_kernels.hip"]
+DEVICE_SOURCES = ["cs_kernels.hip", "cs_solver.hip", "cs_rbgs_persist.hip", "cs_p2g.hip", "cs_unet_kernels.hip",
+                  "cs_noalloc.hip"]
 HEADERS = ["cs_common.h", "cs_bed.h", "cs_device.h", "cs_host.h", "cs_kernels.h", "cs_persist.h", "cs_solver.h"]
 
 HOST_FLAGS = [

@@ -364,6 +364,13 @@ int cs_sim_read_metrics(cs_sim* sim, uint32_t* slots8);
  * particle state; `render8` holds particle_count x 8 floats, a bed of
  * particles first (as the reference's buffer holds it), then the water. */
 int cs_sim_pack_render(cs_sim* sim, float* render8);
+/* cs_sim_pack_render with no device allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under stream capture, tight frame
+ * loops, hosts that forbid allocation).  The vertices are staged in a fixed
+ * device array inside the library (at most 262144 particles, bed plus water,
+ * 8 MiB per GPU, held until the process ends; CS_ERR_RANGE beyond that).  The
+ * array is shared by every call on a GPU: calls that overlap on one GPU race. */
+int cs_sim_pack_render_noalloc(cs_sim* sim, float* render8);
 int cs_sim_get_solve_stats(cs_sim* sim, cs_solve_stats* out);
 
 /* Warm start for the next pressure solve (CS_WARM_EXTERNAL): float pressure per cell. */
