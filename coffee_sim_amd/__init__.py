@@ -85,6 +85,7 @@ def _bind(lib: C.CDLL) -> None:
         "cs_sim_read_sdf": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_uint8)]),
         "cs_sim_read_bed_lookup": (C.c_int, [C.c_void_p, P(C.c_int32)]),
         "cs_sim_read_bed_extracts": (C.c_int, [C.c_void_p, P(C.c_float)]),
+        "cs_sim_read_bed_particles": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float)]),
         "cs_filter_config_default": (None, [P(abi.FilterConfig)]),
         "cs_bed_config_default": (None, [P(abi.BedConfig)]),
         "cs_bed_config_seated_in_filter": (None, [P(abi.FilterConfig), P(abi.BedConfig)]),
@@ -535,6 +536,14 @@ class MpmSim3D:
         out = np.zeros((self.counts()[1], 8), np.float32)
         _check(load_library().cs_sim_read_bed_extracts(self._h, _fptr(out)), "cs_sim_read_bed_extracts")
         return out
+
+    def read_bed_particles(self) -> tuple[np.ndarray, np.ndarray]:
+        """The bed grains and their affine rows after the last substep: (n_bed, 8) and (n_bed, 12) float32."""
+        nb = self.counts()[1]
+        part = np.zeros((nb, 8), np.float32)
+        aff = np.zeros((nb, 12), np.float32)
+        _check(load_library().cs_sim_read_bed_particles(self._h, _fptr(part), _fptr(aff)), "cs_sim_read_bed_particles")
+        return part, aff
 
     def read_sdf(self) -> tuple[np.ndarray, np.ndarray]:
         """The SDF (sdf_res^3) and class mask (one byte per cell) the simulation holds."""

@@ -94,6 +94,12 @@ class Geometry(C.Structure):
         ("cup_top_y", C.c_float),
         ("cup_radius", C.c_float),
         ("cup_bottom_y", C.c_float),
+        ("filter_enabled", C.c_uint32),
+        ("filter_center_y", C.c_float),
+        ("filter_top_y", C.c_float),
+        ("filter_bot_y", C.c_float),
+        ("filter_top_radius", C.c_float),
+        ("filter_thickness", C.c_float),
     ]
 
 

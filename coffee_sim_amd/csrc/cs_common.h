@@ -218,6 +218,19 @@ struct cs_params {
   cs_geometry geo;
 };
 
+// A bed of particles as the reaction impulses see it (shader.rs:458-515):
+// the momentum the bed takes from the water goes to the grains stamped in
+// the cells around, through bed_delta lanes 1-3.  n = 0: no grains (a field
+// bed or none), no reaction.
+struct cs_grains {
+  const int32_t* lookup;  // per cell: the stamping bed particle, or -1
+  const float* aff;       // 12 floats per bed particle (.w of col0 = phase)
+  int32_t* delta;         // bed_delta [4][n]: water credit, impulse x, y, z (i32 fixed point)
+  uint32_t n;
+};
+#define CS_BED_REACTION_ALPHA 0.04f        // shader.rs:76
+#define CS_BED_REACTION_IMPULSE_CAP 0.012f // shader.rs:77
+
 // Static per-bed-cell record derived once from cs_bed_record (field mode of
 // shader.rs:361-666): everything of the bed formulas that does not depend on
 // the particles.

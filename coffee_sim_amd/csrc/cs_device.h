@@ -102,7 +102,7 @@ __device__ __forceinline__ cs_f3 cap_speed(const cs_params& u, cs_f3 v) {
   return v;
 }
 
-// ---- contact for particles (shader.rs:717-903, water only) -------------------
+// ---- contact for particles (shader.rs:717-903) ---------------------------------
 struct contact {
   cs_f3 pos, vel;
 };
@@ -122,7 +122,20 @@ __device__ contact radial_barrier(const cs_params& u, cs_f3 pos, cs_f3 vel, floa
   }
   return contact{pos, vel};
 }
-__device__ contact scene_obstacles(const cs_params& u, cs_f3 pos, cs_f3 vel) {
+// bed_particle_radius (shader.rs:128): dx * bed_sample_radius_dx
+__device__ __forceinline__ float bed_particle_radius(const cs_params& u) { return u.dx * CS_BED_SAMPLE_RADIUS_DX; }
+// coffee_filter_floor_y (shader.rs:136-146) of the geometry's paper filter
+__device__ __forceinline__ float coffee_filter_floor_y(const cs_params& u) {
+  const cs_geometry& g = u.geo;
+  const float fb = g.filter_center_y + g.filter_bot_y, ft = g.filter_center_y + g.filter_top_y;
+  const float bco = cs_max(u.contact_offset, bed_particle_radius(u));
+  const float fh = cs_max(ft - fb, 1e-6f);
+  const float slope = g.filter_top_radius / fh;
+  return fb + (bco + g.filter_thickness) / cs_max(slope, 1e-6f);
+}
+// resolve_scene_obstacles (shader.rs:744-823): the cone, then for bed grains the
+// paper filter and its apex floor, then the cup.
+__device__ contact scene_obstacles(const cs_params& u, cs_f3 pos, cs_f3 vel, bool is_bed) {
   const cs_geometry& g = u.geo;
   if (g.cone_enabled && pos.y <= g.cone_top_y && pos.y >= g.cone_bot_y) {
     float t = cs_clamp((pos.y - g.cone_bot_y) / (g.cone_top_y - g.cone_bot_y), 0.0f, 1.0f);
@@ -130,6 +143,31 @@ __device__ contact scene_obstacles(const cs_params& u, cs_f3 pos, cs_f3 vel) {
     contact c = radial_barrier(u, pos, vel, cone_r);
     pos = c.pos;
     vel = c.vel;
+  }
+  if (is_bed && g.filter_enabled) {
+    const float ft = g.filter_center_y + g.filter_top_y, fb = g.filter_center_y + g.filter_bot_y;
+    const float bco = cs_max(u.contact_offset, bed_particle_radius(u));
+    const float floor_y = coffee_filter_floor_y(u);
+    if (pos.y <= ft && pos.y >= fb) {
+      const float t = cs_clamp((pos.y - fb) / (ft - fb), 0.0f, 1.0f);
+      const float outer_r = cs_mix(0.0f, g.filter_top_radius, t);
+      const float fr = cs_max(outer_r - g.filter_thickness, 0.0f) - bco;
+      contact c = radial_barrier(u, pos, vel, cs_max(fr, 0.0f));
+      pos = c.pos;
+      vel = c.vel;
+    }
+    if (pos.y <= floor_y) {
+      pos.y = floor_y;
+      pos.x = 0.0f;
+      pos.z = 0.0f;
+      if (vel.y < 0.0f) {
+        vel.y = 0.0f;
+        vel.x *= 1.0f - u.friction * 0.55f;
+        vel.z *= 1.0f - u.friction * 0.55f;
+      }
+      vel.x *= 1.0f - u.friction * 0.55f;
+      vel.z *= 1.0f - u.friction * 0.55f;
+    }
   }
   if (g.cup_enabled && pos.y <= g.cup_top_y) {
     float cup_r = g.cup_radius - u.wall_half - u.contact_offset;
@@ -148,7 +186,8 @@ __device__ contact scene_obstacles(const cs_params& u, cs_f3 pos, cs_f3 vel) {
   }
   return contact{pos, vel};
 }
-__device__ contact sdf_contact(const cs_params& u, const float* __restrict__ sdf, cs_f3 pos, cs_f3 vel) {
+__device__ contact sdf_contact(const cs_params& u, const float* __restrict__ sdf, cs_f3 pos, cs_f3 vel,
+                               bool is_bed = false) {
   float sv = sample_sdf(u, sdf, pos);
   if (sv < u.contact_offset) {
     cs_f3 n = sdf_gradient(u, sdf, pos);
@@ -166,7 +205,7 @@ __device__ contact sdf_contact(const cs_params& u, const float* __restrict__ sdf
       }
     }
   }
-  return scene_obstacles(u, pos, vel);
+  return scene_obstacles(u, pos, vel, is_bed);
 }
 
 // ---- kind/fill word ------------------------------------------------------------

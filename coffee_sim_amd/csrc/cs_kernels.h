@@ -33,17 +33,20 @@ hipError_t cs_launch_rbgs_pair(hipStream_t stream, const cs_params& u, const flo
 hipError_t cs_launch_residual(hipStream_t stream, const cs_params& u, const float* kf, const float* rhs,
                               const int32_t* pres, double* partials, uint32_t max_blocks, double* out);
 hipError_t cs_launch_project(hipStream_t stream, const cs_params& u, float* gvel, const float* kf,
-                             const int32_t* pres, const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf);
+                             const int32_t* pres, const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf,
+                             const cs_grains& grains);
 // The divergence of the projected velocity over the fluid cells (metric slots 5
 // and 6), from classify's per-cell target `tgt`.
 hipError_t cs_launch_post_div(hipStream_t stream, const cs_params& u, const float* gvel, const float* kf,
                               const uint8_t* cls, const float* sdf, const float* tgt, uint32_t* metrics);
 hipError_t cs_launch_pack(hipStream_t stream, const cs_params& u, float* gvel, const float* kf, const int32_t* pack,
                           const float* sdf);
+// alias4 (may be null): per cell 1 when the viscosity scratch's z lane holds
+// CELL_SOLID, the lane bed_dynamics reads as the kind (shader.rs:158-164, 238-259).
 hipError_t cs_launch_visc(hipStream_t stream, const cs_params& u, const float* gin, float* gout, const uint8_t* cls,
-                          const float* sdf);
+                          const float* sdf, uint8_t* alias4);
 hipError_t cs_launch_g2p(hipStream_t stream, const cs_params& u, float* part, float* aff, const float* gvel,
-                         const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf);
+                         const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf, const cs_grains& grains);
 // prepare_render (shader.rs:2242-2276): 8 floats per particle into `out`; particles
 // below num_bed are bed particles, with their BedExtracts in bed_ext (may be null).
 hipError_t cs_launch_pack_render(hipStream_t stream, const cs_params& u, const float* part, const float* aff,
@@ -64,6 +67,12 @@ hipError_t cs_launch_bed_lookup(hipStream_t st, const cs_params& u, uint32_t nbe
 // particles (shader.rs:1885-1987); dw holds one i32 water credit per bed particle.
 hipError_t cs_launch_bed_coupling(hipStream_t st, const cs_params& u, float* part, const float* aff,
                                   const int32_t* lookup, uint32_t nbed, float* bed_ext, int32_t* dw);
+// bed_dynamics (shader.rs:1991-2238) over the nbed grains: snapshots their
+// positions into `snap` (4 floats each), then moves them; reads the final grid
+// velocity, the solved pressure and the viscosity pass's alias4 flags.
+hipError_t cs_launch_bed_dynamics(hipStream_t st, const cs_params& u, uint32_t nbed, float* bed_part, float* bed_aff,
+                                  float* bed_ext, int32_t* dw, float* snap, const int32_t* lookup, const float* gvel,
+                                  const int32_t* pres, const uint8_t* alias4, const float* sdf);
 // The particles of one substep's emission (f.count of them at slots f.first..)
 // and their zero affine records.
 hipError_t cs_launch_emit(hipStream_t stream, const cs_emit_frame& f, float* part, float* aff);

@@ -318,10 +318,45 @@ __global__ void k_residual_final(const double* __restrict__ partials, uint32_t n
   }
 }
 
+// ---- bed reaction impulses (shader.rs:458-515) ------------------------------------------
+__device__ __forceinline__ bool grain_solid(const cs_grains& gr, int32_t id) {  // is_valid_bed_solid_idx
+  return id >= 0 && (uint32_t)id < gr.n && !(gr.aff[12 * (size_t)id + 3] < 0.5f);
+}
+// bed_impulse_delta_add_neighborhood: one share per stamped cell of the 3x3x3
+// block, each share clamped per component and added in fixed point
+__device__ void grain_impulse(const cs_params& u, const cs_grains& gr, int cx, int cy, int cz, cs_f3 imp) {
+  float count = 0.0f;
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int x = cx + di, y = cy + dj, z = cz + dk;
+        if (!on_grid(u, x, y, z)) continue;
+        if (grain_solid(gr, gr.lookup[cell_index(u, x, y, z)])) count += 1.0f;
+      }
+  if (count <= 0.0f) return;
+  const cs_f3 share = f3(imp.x / count, imp.y / count, imp.z / count);
+  const float cap = u.nominal_mass * u.vel_cap * CS_BED_REACTION_IMPULSE_CAP;
+  const int32_t ix = cs_f2i(cs_clamp(share.x, -cap, cap) * u.fp_scale);
+  const int32_t iy = cs_f2i(cs_clamp(share.y, -cap, cap) * u.fp_scale);
+  const int32_t iz = cs_f2i(cs_clamp(share.z, -cap, cap) * u.fp_scale);
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int x = cx + di, y = cy + dj, z = cz + dk;
+        if (!on_grid(u, x, y, z)) continue;
+        const int32_t id = gr.lookup[cell_index(u, x, y, z)];
+        if (!grain_solid(gr, id)) continue;
+        atomicAdd(&gr.delta[1u * gr.n + (uint32_t)id], ix);
+        atomicAdd(&gr.delta[2u * gr.n + (uint32_t)id], iy);
+        atomicAdd(&gr.delta[3u * gr.n + (uint32_t)id], iz);
+      }
+}
+
 // ---- project_pressure + boundary_project (shader.rs:1457-1543, 1630-1679) ---------
 __device__ __forceinline__ void project_cell(const cs_params& u, float4* __restrict__ gvel, const float* __restrict__ kf,
                                                  const int32_t* __restrict__ pres, const int32_t* __restrict__ bed_id,
-                                                 const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf, uint32_t idx) {
+                                                 const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf,
+                                                 const cs_grains& gr, uint32_t idx) {
   if (idx >= u.nc) return;
   float4 gv = gvel[idx];
   if (gv.w < 1e-6f) return;
@@ -356,7 +391,12 @@ __device__ __forceinline__ void project_cell(const cs_params& u, float4* __restr
         float rate = u.water_nu / b.k_record;
         float damping = 1.0f / (1.0f + cs_max(rate * u.dt, 0.0f));
         cs_f3 bv = f3(b.bmv[0], b.bmv[1], b.bmv[2]);
+        const cs_f3 before = v;
         v = bv + (v - bv) * damping;
+        if (gr.n) {  // the reaction on the grains (shader.rs:1533-1535)
+          const cs_f3 wi = (v - before) * gv.w;
+          grain_impulse(u, gr, (int)ix, (int)iy, (int)iz, f3(-wi.x, -wi.y, -wi.z) * CS_BED_REACTION_ALPHA);
+        }
       }
     }
     v = cap_speed(u, v);
@@ -366,8 +406,9 @@ __device__ __forceinline__ void project_cell(const cs_params& u, float4* __restr
 }
 __global__ void __launch_bounds__(256) k_project(cs_params u, float4* __restrict__ gvel, const float* __restrict__ kf,
                                                  const int32_t* __restrict__ pres, const int32_t* __restrict__ bed_id,
-                                                 const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf) {
-  project_cell(u, gvel, kf, pres, bed_id, bed, sdf, blockIdx.x * blockDim.x + threadIdx.x);
+                                                 const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf,
+                                                 cs_grains gr) {
+  project_cell(u, gvel, kf, pres, bed_id, bed, sdf, gr, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- post-projection divergence (no reference pass) ---------------------------------
@@ -440,19 +481,16 @@ __global__ void __launch_bounds__(256) k_pack(cs_params u, float4* __restrict__ 
 
 // ---- viscosity_prepare + viscosity_apply + boundary_project (shader.rs:1086-1177) --
 __device__ __forceinline__ void visc_cell(const cs_params& u, const float4* __restrict__ gin, float4* __restrict__ gout,
-                                              const uint8_t* __restrict__ cls, const float* __restrict__ sdf, uint32_t idx) {
+                                              const uint8_t* __restrict__ cls, const float* __restrict__ sdf,
+                                              uint8_t* __restrict__ alias4, uint32_t idx) {
   if (idx >= u.nc) return;
   float occ = u.nominal_mass * 0.1f;
   float4 gv = gin[idx];
-  if (gv.w <= occ) {
-    gout[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return;
-  }
-  uint32_t ix, iy, iz;
-  decompose(u, idx, ix, iy, iz);
   cs_f3 vh = f3(gv.x, gv.y, gv.z);
   cs_f3 vs = vh;
-  if (u.viscosity > 0.0f && gv.w > u.nominal_mass * 2.0f) {
+  if (gv.w > occ && u.viscosity > 0.0f && gv.w > u.nominal_mass * 2.0f) {
+    uint32_t ix, iy, iz;
+    decompose(u, idx, ix, iy, iz);
     float alpha = cs_clamp(u.viscosity * u.dt / cs_max(u.dx * u.dx, 1e-6f), 0.0f, 0.14f);
     cs_f3 nsum = f3(0.0f, 0.0f, 0.0f);
     float wsum = 0.0f;
@@ -477,20 +515,33 @@ __device__ __forceinline__ void visc_cell(const cs_params& u, const float4* __re
   }
   // velocity_scratch_store / load round trip (shader.rs:266-283)
   vs = cap_speed(u, vs);
+  const int32_t lz = cs_f2i(vs.z * u.fp_scale);
+  // The scratch's z lane is the lane the reference reads as the cell kind
+  // until the next clear (shader.rs:158-164): bed_dynamics' pressure gradient
+  // takes a cell as solid when it holds CELL_SOLID.
+  if (alias4) alias4[idx] = lz == CS_CELL_SOLID ? 1u : 0u;
+  if (gv.w <= occ) {
+    gout[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  uint32_t ix, iy, iz;
+  decompose(u, idx, ix, iy, iz);
   cs_f3 v = f3((float)cs_f2i(vs.x * u.fp_scale) * u.inv_fp_scale, (float)cs_f2i(vs.y * u.fp_scale) * u.inv_fp_scale,
-               (float)cs_f2i(vs.z * u.fp_scale) * u.inv_fp_scale);
+               (float)lz * u.inv_fp_scale);
   v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
   gout[idx] = make_float4(v.x, v.y, v.z, gv.w);
 }
 __global__ void __launch_bounds__(256) k_visc(cs_params u, const float4* __restrict__ gin, float4* __restrict__ gout,
-                                              const uint8_t* __restrict__ cls, const float* __restrict__ sdf) {
-  visc_cell(u, gin, gout, cls, sdf, blockIdx.x * blockDim.x + threadIdx.x);
+                                              const uint8_t* __restrict__ cls, const float* __restrict__ sdf,
+                                              uint8_t* __restrict__ alias4) {
+  visc_cell(u, gin, gout, cls, sdf, alias4, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- G2P (shader.rs:1683-1881) ------------------------------------------------------
 __device__ __forceinline__ void g2p_particle(const cs_params& u, float4* __restrict__ part, float4* __restrict__ aff,
                                              const float4* __restrict__ gvel, const int32_t* __restrict__ bed_id,
-                                             const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf, uint32_t pid) {
+                                             const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf,
+                                             const cs_grains& gr, uint32_t pid) {
   if (pid >= u.num_particles) return;
   float4 P = part[2 * pid], Vv = part[2 * pid + 1];
   float phase = aff[3 * pid].w;
@@ -602,7 +653,14 @@ __device__ __forceinline__ void g2p_particle(const cs_params& u, float4* __restr
     float dd = 1.0f / (1.0f + cs_max(rate * u.dt * porous, 0.0f));
     float rs = len3(rel);
     float idd = 1.0f / (1.0f + cs_max(rs * u.dt * u.inv_dx * porous * 0.35f, 0.0f));
+    const cs_f3 before = nv;
     nv = pbv + (rel * dd) * idd;
+    if (gr.n) {  // the reaction on the grains (shader.rs:1837-1843)
+      const cs_f3 wi = (nv - before) * mass_p;
+      grain_impulse(u, gr, cs_f2i(floorf((xp.x - u.origin[0]) * u.inv_dx)),
+                    cs_f2i(floorf((xp.y - u.origin[1]) * u.inv_dx)), cs_f2i(floorf((xp.z - u.origin[2]) * u.inv_dx)),
+                    f3(-wi.x, -wi.y, -wi.z) * CS_BED_REACTION_ALPHA);
+    }
     float ad = cs_mix(1.0f, dd * idd, porous);
     C0 = C0 * ad;
     C1 = C1 * ad;
@@ -633,8 +691,9 @@ __device__ __forceinline__ void g2p_particle(const cs_params& u, float4* __restr
 }
 __global__ void __launch_bounds__(256) k_g2p(cs_params u, float4* __restrict__ part, float4* __restrict__ aff,
                                              const float4* __restrict__ gvel, const int32_t* __restrict__ bed_id,
-                                             const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf) {
-  g2p_particle(u, part, aff, gvel, bed_id, bed, sdf, blockIdx.x * blockDim.x + threadIdx.x);
+                                             const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf,
+                                             cs_grains gr) {
+  g2p_particle(u, part, aff, gvel, bed_id, bed, sdf, gr, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- prepare_render (shader.rs:2242-2276) --------------------------------------------
@@ -855,6 +914,265 @@ hipError_t cs_launch_bed_coupling(hipStream_t st, const cs_params& u, float* par
   return hipGetLastError();
 }
 
+// ---- bed_dynamics (shader.rs:1991-2238) ------------------------------------------------
+// pressure_gradient_at_cell (shader.rs:238-259) as bed_dynamics sees it: the
+// solved pressure, with a neighbour taken as solid (mirror) when the lane the
+// reference reads as its kind holds CELL_SOLID; by now that lane holds the
+// viscosity scratch (alias4, written by the viscosity pass).
+__device__ __forceinline__ float grad_p_or_mirror(const cs_params& u, const int32_t* __restrict__ pres,
+                                                  const uint8_t* __restrict__ alias4, int x, int y, int z, float mirror) {
+  if (!on_grid(u, x, y, z)) return mirror;
+  const uint32_t ci = cell_index(u, x, y, z);
+  if (alias4[ci]) return mirror;
+  return (float)pres[ci] * u.inv_fp_scale;
+}
+__device__ cs_f3 pressure_gradient_at_cell(const cs_params& u, const int32_t* __restrict__ pres,
+                                           const uint8_t* __restrict__ alias4, int x, int y, int z) {
+  if (!on_grid(u, x, y, z)) return f3(0.0f, 0.0f, 0.0f);
+  const float ph = (float)pres[cell_index(u, x, y, z)] * u.inv_fp_scale;
+  const float pxm = grad_p_or_mirror(u, pres, alias4, x - 1, y, z, ph);
+  const float pxp = grad_p_or_mirror(u, pres, alias4, x + 1, y, z, ph);
+  const float pym = grad_p_or_mirror(u, pres, alias4, x, y - 1, z, ph);
+  const float pyp = grad_p_or_mirror(u, pres, alias4, x, y + 1, z, ph);
+  const float pzm = grad_p_or_mirror(u, pres, alias4, x, y, z - 1, ph);
+  const float pzp = grad_p_or_mirror(u, pres, alias4, x, y, z + 1, ph);
+  const float h = 0.5f * u.inv_dx;
+  return f3(h * (pxp - pxm), h * (pyp - pym), h * (pzp - pzm));
+}
+
+__device__ __forceinline__ int floor_cell(const cs_params& u, float p, int axis) {
+  return cs_f2i(floorf((p - u.origin[axis]) * u.inv_dx));
+}
+
+// resolve_coffee_particle_packing (shader.rs:825-877) against the grains'
+// positions from before the pass (`snap`): the reference reads them while
+// they move, which depends on thread scheduling.
+__device__ contact coffee_packing(const cs_params& u, const int32_t* __restrict__ lookup, uint32_t nbed,
+                                  const float4* __restrict__ snap, cs_f3 pos, cs_f3 vel, uint32_t self) {
+  cs_f3 cd = f3(0.0f, 0.0f, 0.0f);
+  float closest = 1e9f;
+  bool found = false;
+  const int cx = floor_cell(u, pos.x, 0), cy = floor_cell(u, pos.y, 1), cz = floor_cell(u, pos.z, 2);
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int x = cx + di, y = cy + dj, z = cz + dk;
+        if (!on_grid(u, x, y, z)) continue;
+        const int32_t o = lookup[cell_index(u, x, y, z)];
+        if (o < 0 || (uint32_t)o == self || (uint32_t)o >= nbed) continue;
+        const float4 q = snap[o];
+        const cs_f3 d = pos - f3(q.x, q.y, q.z);
+        const float dist = len3(d);
+        if (dist < closest) {
+          closest = dist;
+          cd = d;
+          found = true;
+        }
+      }
+  const float min_sep = bed_particle_radius(u) * 1.78f;
+  if (found && closest < min_sep) {
+    cs_f3 n = f3(0.0f, 1.0f, 0.0f);
+    if (closest > 1e-5f) n = cd / closest;
+    const float pen = min_sep - closest;
+    pos = pos + n * cs_min(pen * 0.65f, u.dx * 0.22f);
+    const float vn = dot3(vel, n);
+    if (vn < 0.0f) vel = vel - n * vn;
+  }
+  return contact{pos, vel};
+}
+
+__global__ void __launch_bounds__(256) k_bed_snapshot(uint32_t nbed, const float4* __restrict__ bed_part,
+                                                      float4* __restrict__ snap) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nbed) snap[b] = bed_part[2 * b];
+}
+
+// One grain: its reaction impulse, then either the suspended drift with the
+// water or the anchored, plastically compacting matrix motion around its rest
+// position; bed_damping 8.5, bed_compaction_rate 5.5, bed_impact 8.0
+// (mod.rs:644-649, brew_config.rs:69-70).
+__global__ void __launch_bounds__(256) k_bed_dynamics(cs_params u, uint32_t nbed, float4* __restrict__ bed_part,
+                                                      float4* __restrict__ bed_aff, float* __restrict__ bed_ext,
+                                                      int32_t* __restrict__ dw, const float4* __restrict__ snap,
+                                                      const int32_t* __restrict__ lookup,
+                                                      const float4* __restrict__ gvel, const int32_t* __restrict__ pres,
+                                                      const uint8_t* __restrict__ alias4,
+                                                      const float* __restrict__ sdf) {
+  const uint32_t bid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bid >= nbed) return;
+  const float bed_damping = 8.5f, bed_compaction = 5.5f, bed_impact = 8.0f;
+  const float dx = u.dx, dt = u.dt;
+  const float floor_y = coffee_filter_floor_y(u);
+  const float4 P = bed_part[2 * bid], Vv = bed_part[2 * bid + 1];
+  const float4 A0 = bed_aff[3 * bid], A1 = bed_aff[3 * bid + 1];
+  float* be = bed_ext + 8 * (size_t)bid;
+  float phase = A0.w;
+  cs_f3 rest = f3(A1.x, A1.y, A1.z);
+  const cs_f3 pos = f3(P.x, P.y, P.z);
+  const float mass_p = Vv.w;
+  // bed_impulse_delta_exchange (shader.rs:517-522)
+  cs_f3 brv = f3((float)dw[1u * nbed + bid] * u.inv_fp_scale, (float)dw[2u * nbed + bid] * u.inv_fp_scale,
+                 (float)dw[3u * nbed + bid] * u.inv_fp_scale);
+  dw[1u * nbed + bid] = 0;
+  dw[2u * nbed + bid] = 0;
+  dw[3u * nbed + bid] = 0;
+  brv = brv / cs_max(mass_p, u.nominal_mass * 0.25f);
+  const float max_rs = cs_min(u.vel_cap * 0.012f, dx / cs_max(dt, 1e-6f) * 0.02f);
+  const float rs = len3(brv);
+  if (rs > max_rs && rs > 1e-6f) brv = brv * (max_rs / rs);
+  // the water around the grain
+  const cs_f3 gp = (pos - f3(u.origin[0], u.origin[1], u.origin[2])) * u.inv_dx;
+  const int bx = cs_f2i(floorf(gp.x - 0.5f)), by = cs_f2i(floorf(gp.y - 0.5f)), bz = cs_f2i(floorf(gp.z - 0.5f));
+  const cs_f3 fx = f3(gp.x - (float)bx, gp.y - (float)by, gp.z - (float)bz);
+  float wx[3], wy[3], wz[3];
+  wx[0] = 0.5f * (1.5f - fx.x) * (1.5f - fx.x);
+  wx[1] = 0.75f - (fx.x - 1.0f) * (fx.x - 1.0f);
+  wx[2] = 0.5f * (fx.x - 0.5f) * (fx.x - 0.5f);
+  wy[0] = 0.5f * (1.5f - fx.y) * (1.5f - fx.y);
+  wy[1] = 0.75f - (fx.y - 1.0f) * (fx.y - 1.0f);
+  wy[2] = 0.5f * (fx.y - 0.5f) * (fx.y - 0.5f);
+  wz[0] = 0.5f * (1.5f - fx.z) * (1.5f - fx.z);
+  wz[1] = 0.75f - (fx.z - 1.0f) * (fx.z - 1.0f);
+  wz[2] = 0.5f * (fx.z - 0.5f) * (fx.z - 0.5f);
+  cs_f3 water_v = f3(0.0f, 0.0f, 0.0f);
+  float water_mass = 0.0f, asw = 0.0f, lasw = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int cx = bx + i, cy = by + j, cz = bz + k;
+        if (!on_grid(u, cx, cy, cz)) continue;
+        const float w = wx[i] * wy[j] * wz[k];
+        const uint32_t ci = cell_index(u, cx, cy, cz);
+        const float4 g = gvel[ci];
+        water_v = water_v + f3(g.x, g.y, g.z) * w;
+        water_mass += w * g.w;
+        const int32_t sb = lookup[ci];
+        if (sb >= 0 && (uint32_t)sb != bid) {
+          asw += w;
+          const float ccy = u.origin[1] + ((float)cy + 0.5f) * dx;
+          if (ccy <= pos.y + dx * 0.25f) lasw += w;
+        }
+      }
+  if (phase >= 1.5f) {  // suspended grain (shader.rs:2065-2105)
+    const float ws = cs_clamp(water_mass / cs_max(u.nominal_mass * 2.0f, 1e-6f), 0.0f, 1.0f);
+    const float drag = cs_clamp((8.0f + 18.0f * ws) * dt, 0.0f, 1.0f);
+    const cs_f3 unsup = (f3(Vv.x, Vv.y, Vv.z) + brv) + f3(0.0f, u.gravity * dt * 0.85f, 0.0f);
+    const float settle = cs_mix(1.10f, 0.32f, ws);
+    const cs_f3 carried = water_v + f3(0.0f, -settle, 0.0f);
+    cs_f3 sv = mix3(unsup, carried, drag);
+    const float sp = len3(sv);
+    if (sp > u.vel_cap) sv = sv * (u.vel_cap / sp);
+    cs_f3 spos = pos + sv * dt;
+    const contact c = sdf_contact(u, sdf, spos, sv, true);
+    spos = c.pos;
+    sv = c.vel;
+    bed_part[2 * bid] = make_float4(spos.x, spos.y, spos.z, P.w);
+    bed_part[2 * bid + 1] = make_float4(sv.x, sv.y, sv.z, mass_p);
+    const bool anchored = lasw > 0.06f || asw > 0.20f;
+    const bool dense = asw > 0.42f;
+    const bool redeposit = (spos.y <= floor_y + dx * 0.35f && ws < 0.35f) ||
+                           (anchored && sv.y <= 0.20f && (ws < 0.55f || dense));
+    if (redeposit) phase = 1.0f;
+    bed_aff[3 * bid] = make_float4(A0.x, A0.y, A0.z, phase);
+    bed_aff[3 * bid + 1] = make_float4(spos.x, spos.y, spos.z, 0.0f);
+    be[3] *= 0.985f;
+    return;
+  }
+  // anchored grain (shader.rs:2107-2237)
+  const float sat = be[7];
+  const float wl = cs_clamp(water_mass / cs_max(u.nominal_mass * 1.5f, 1e-6f), 0.0f, 1.0f);
+  const float wet = 1.0f + sat * 0.40f;
+  const float mob = cs_clamp((0.30f + wl * 0.62f) * wet, 0.0f, 1.0f);
+  const float sf = cs_clamp((rest.y + 3.0f) / 3.5f, 0.12f, 1.0f);
+  const float damping = cs_clamp(1.0f - bed_damping * dt, 0.0f, 1.0f);
+  cs_f3 vel = f3(Vv.x, Vv.y, Vv.z) + brv;
+  const float sr = cs_clamp(1.0f - asw / 0.55f, 0.0f, 1.0f);
+  const float ed = cs_clamp((2.0f + 5.0f * sat) * wl * sr * dt, 0.0f, 0.08f);
+  vel = mix3(vel, water_v + f3(0.0f, -cs_mix(0.95f, 0.35f, wl), 0.0f), ed);
+  const float ls = sqrtf(water_v.x * water_v.x + water_v.z * water_v.z);
+  const float ldx = ls > 1e-6f ? water_v.x / ls : 0.0f, ldz = ls > 1e-6f ? water_v.z / ls : 0.0f;
+  const float lss = cs_min(ls, fabsf(cs_min(water_v.y, 0.0f)) * 0.18f);
+  const cs_f3 impact = f3(ldx * lss * 0.08f, cs_min(water_v.y, 0.0f) * 0.62f, ldz * lss * 0.08f);
+  vel = vel + (((impact * bed_impact) * mob) * sf) * dt;
+  const cs_f3 pg = pressure_gradient_at_cell(u, pres, alias4, floor_cell(u, pos.x, 0), floor_cell(u, pos.y, 1),
+                                             floor_cell(u, pos.z, 2));
+  const float sfx = -pg.x, sfz = -pg.z;
+  const float sps = sqrtf(sfx * sfx + sfz * sfz);
+  const float sdx = sps > 1e-6f ? sfx / sps : 0.0f, sdz = sps > 1e-6f ? sfz / sps : 0.0f;
+  const float drive = cs_min(sps, u.vel_cap / cs_max(dt, 1e-6f));
+  const float pl = cs_clamp(water_mass * (0.25f + sat), 0.0f, 1.0f);
+  const float slx = sdx * drive * pl * 0.08f, slz = sdz * drive * pl * 0.08f;
+  vel.x += slx * bed_impact * mob * sf * dt;
+  vel.z += slz * bed_impact * mob * sf * dt;
+  vel = vel * damping;
+  cs_f3 np = pos + vel * dt;
+  cs_f3 off = np - rest;
+  const float ll = sqrtf(off.x * off.x + off.z * off.z);
+  const float lpt = dx * 0.22f;
+  if (ll > lpt && ll > 1e-6f) {
+    const float ox = off.x / ll, oz = off.z / ll;
+    const float excess = ll - lpt;
+    const float lp = cs_clamp((0.04f + sat * 0.35f + mob * 0.45f + pl * 0.25f) * sf * bed_compaction * dt, 0.0f, 0.18f);
+    rest.x += ox * excess * lp;
+    rest.z += oz * excess * lp;
+    off = np - rest;
+  }
+  const float cll = sqrtf(off.x * off.x + off.z * off.z);
+  const float ml = dx * 0.9f * sf;
+  if (cll > ml && cll > 1e-6f) {
+    const float ox = off.x / cll, oz = off.z / cll;
+    off.x = ox * ml;
+    off.z = oz * ml;
+    vel.x *= 0.4f;
+    vel.z *= 0.4f;
+  }
+  off.y = cs_clamp(off.y, -dx * (1.75f * sf + 0.2f), dx * 0.04f);
+  np = rest + off;
+  const contact pc = coffee_packing(u, lookup, nbed, snap, np, vel, bid);
+  np = pc.pos;
+  vel = pc.vel;
+  const float comp = cs_max(rest.y - np.y, 0.0f);
+  const float pth = dx * 0.18f;
+  if (comp > pth) {
+    const float ex = comp - pth;
+    const float plast = cs_clamp((0.05f + sat * 0.35f + mob * 0.55f) * sf * bed_compaction * dt, 0.0f, 0.18f);
+    rest.y -= ex * plast;
+  }
+  const bool detach = sat > 0.82f && pl > 0.65f && wl > 0.65f && sr > 0.35f && (comp > dx * 0.65f || cll > dx * 0.65f);
+  const bool entrained = sat > 0.58f && wl > 0.74f && sr > 0.55f && len3(water_v) > 0.95f && np.y > floor_y + dx * 0.70f;
+  const bool isolated = sat > 0.55f && water_mass < u.nominal_mass * 0.75f && asw < 0.04f && lasw < 0.03f &&
+                        np.y > floor_y + dx * 0.50f;
+  if (detach || entrained || isolated) {
+    phase = 2.0f;
+    rest = np;
+  }
+  const contact c = sdf_contact(u, sdf, np, vel, true);
+  np = c.pos;
+  vel = c.vel;
+  bed_part[2 * bid] = make_float4(np.x, np.y, np.z, P.w);
+  bed_part[2 * bid + 1] = make_float4(vel.x, vel.y, vel.z, mass_p);
+  bed_aff[3 * bid] = make_float4(A0.x, A0.y, A0.z, phase);
+  bed_aff[3 * bid + 1] = make_float4(rest.x, rest.y, rest.z, 0.0f);
+  const float gc = cs_max((rest.y - np.y) / cs_max(dx, 1e-6f), 0.0f);
+  be[3] = cs_max(be[3] * 0.995f, gc);
+}
+
+hipError_t cs_launch_bed_dynamics(hipStream_t st, const cs_params& u, uint32_t nbed, float* bed_part, float* bed_aff,
+                                  float* bed_ext, int32_t* dw, float* snap, const int32_t* lookup, const float* gvel,
+                                  const int32_t* pres, const uint8_t* alias4, const float* sdf) {
+  if (nbed == 0) return hipSuccess;
+  const dim3 g((nbed + 255u) / 256u);
+  hipLaunchKernelGGL(k_bed_snapshot, g, dim3(256), 0, st, nbed, reinterpret_cast<const float4*>(bed_part),
+                     reinterpret_cast<float4*>(snap));
+  hipLaunchKernelGGL(k_bed_dynamics, g, dim3(256), 0, st, u, nbed, reinterpret_cast<float4*>(bed_part),
+                     reinterpret_cast<float4*>(bed_aff), bed_ext, dw, reinterpret_cast<const float4*>(snap), lookup,
+                     reinterpret_cast<const float4*>(gvel), pres, alias4, sdf);
+  return hipGetLastError();
+}
+
 hipError_t cs_launch_bed_lookup(hipStream_t st, const cs_params& u, uint32_t nbed, const float* bed_part,
                                 const float* bed_aff, const float* bed_ext, uint32_t capacity, int32_t* lookup,
                                 uint32_t* counter, int32_t* bed_id, cs_bed_cell* table) {
@@ -1010,7 +1328,7 @@ __global__ void __launch_bounds__(256) k_batch_classify(const cs_scene_dev* __re
 __global__ void __launch_bounds__(256) k_batch_project(const cs_scene_dev* __restrict__ sc,
                                                        const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_CELLS);
-  project_cell(u, reinterpret_cast<float4*>(d.gv), d.kf, d.pres, d.bed_id, d.bed, d.sdf, tid);
+  project_cell(u, reinterpret_cast<float4*>(d.gv), d.kf, d.pres, d.bed_id, d.bed, d.sdf, cs_grains{}, tid);
 }
 
 // the post-projection metric reads the neighbours' projected velocities, so
@@ -1030,14 +1348,14 @@ __global__ void __launch_bounds__(256) k_batch_pack(const cs_scene_dev* __restri
 __global__ void __launch_bounds__(256) k_batch_visc(const cs_scene_dev* __restrict__ sc,
                                                     const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_CELLS);
-  visc_cell(u, reinterpret_cast<const float4*>(d.gv), reinterpret_cast<float4*>(d.gv2), d.cls, d.sdf, tid);
+  visc_cell(u, reinterpret_cast<const float4*>(d.gv), reinterpret_cast<float4*>(d.gv2), d.cls, d.sdf, nullptr, tid);
 }
 
 __global__ void __launch_bounds__(256) k_batch_g2p(const cs_scene_dev* __restrict__ sc,
                                                    const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_PARTICLES);
   g2p_particle(u, reinterpret_cast<float4*>(d.part), reinterpret_cast<float4*>(d.aff),
-               reinterpret_cast<const float4*>(d.gv2), d.bed_id, d.bed, d.sdf, tid);
+               reinterpret_cast<const float4*>(d.gv2), d.bed_id, d.bed, d.sdf, cs_grains{}, tid);
 }
 
 // ---- launchers -------------------------------------------------------------------------
@@ -1104,9 +1422,9 @@ hipError_t cs_launch_residual(hipStream_t s, const cs_params& u, const float* kf
   return hipGetLastError();
 }
 hipError_t cs_launch_project(hipStream_t s, const cs_params& u, float* gvel, const float* kf, const int32_t* pres,
-                             const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf) {
+                             const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf, const cs_grains& gr) {
   hipLaunchKernelGGL(k_project, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(gvel), kf, pres,
-                     bed_id, bed, sdf);
+                     bed_id, bed, sdf, gr);
   return hipGetLastError();
 }
 hipError_t cs_launch_post_div(hipStream_t s, const cs_params& u, const float* gvel, const float* kf, const uint8_t* cls,
@@ -1121,16 +1439,16 @@ hipError_t cs_launch_pack(hipStream_t s, const cs_params& u, float* gvel, const 
   return hipGetLastError();
 }
 hipError_t cs_launch_visc(hipStream_t s, const cs_params& u, const float* gin, float* gout, const uint8_t* cls,
-                          const float* sdf) {
+                          const float* sdf, uint8_t* alias4) {
   hipLaunchKernelGGL(k_visc, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<const float4*>(gin),
-                     reinterpret_cast<float4*>(gout), cls, sdf);
+                     reinterpret_cast<float4*>(gout), cls, sdf, alias4);
   return hipGetLastError();
 }
 hipError_t cs_launch_g2p(hipStream_t s, const cs_params& u, float* part, float* aff, const float* gvel,
-                         const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf) {
+                         const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf, const cs_grains& gr) {
   if (u.num_particles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_g2p, blocks_for(u.num_particles), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(part),
-                     reinterpret_cast<float4*>(aff), reinterpret_cast<const float4*>(gvel), bed_id, bed, sdf);
+                     reinterpret_cast<float4*>(aff), reinterpret_cast<const float4*>(gvel), bed_id, bed, sdf, gr);
   return hipGetLastError();
 }
 hipError_t cs_launch_pack_render(hipStream_t s, const cs_params& u, const float* part, const float* aff,

@@ -146,6 +146,12 @@ int settings_preset(int preset, cs_settings* s) {
   s->geometry.cup_top_y = -3.5f;
   s->geometry.cup_radius = 3.0f;
   s->geometry.cup_bottom_y = -8.0f;
+  s->geometry.filter_enabled = 1u;
+  s->geometry.filter_center_y = -0.35f;
+  s->geometry.filter_top_y = 2.75f;
+  s->geometry.filter_bot_y = -3.02f;
+  s->geometry.filter_top_radius = 4.10f;
+  s->geometry.filter_thickness = 0.08f;
   s->emitter_mode = CS_EMIT_FLOW_MODEL;
   s->rbgs_mode = CS_RBGS_FIXED_PAIRS;
   s->rbgs_tolerance = 1.0e-4f;

@@ -63,6 +63,7 @@ void free_sim(cs_sim* m) {
                  m->brick_flag, m->brick_list, m->brick_partials, m->dstate,       m->tile_count,
                  m->tile_start, m->p2g_key,    m->p2g_sorted, m->render,     m->bed_part,
                  m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt,
+                 m->bed_snap,   m->alias4,
                  m->persist_fb, m->persist_pflags, m->persist_partials, m->persist_slot, m->persist_xcc};
   for (void* p : dev)
     if (p) (void)hipFree(p);
@@ -91,7 +92,11 @@ int upload_initial(cs_sim* m) {
     CS_HIP(hipMemcpyAsync(m->aff, m->init_affine.data(), (size_t)m->init_count * 48, hipMemcpyHostToDevice, st));
   }
   CS_HIP(hipMemsetAsync(m->prev_p, 0, m->nc * sizeof(int32_t), st));
-  if (m->num_bed) {  // the bed's extraction state starts over; the bed particles never move here
+  if (m->num_bed) {  // the bed's grains and extraction state start over
+    CS_HIP(hipMemcpyAsync(m->bed_part, m->init_bed_part.data(), m->init_bed_part.size() * sizeof(float),
+                          hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(m->bed_aff, m->init_bed_aff.data(), m->init_bed_aff.size() * sizeof(float),
+                          hipMemcpyHostToDevice, st));
     CS_HIP(hipMemcpyAsync(m->bed_ext, m->init_bed_ext.data(), m->init_bed_ext.size() * sizeof(float),
                           hipMemcpyHostToDevice, st));
     CS_HIP(hipMemsetAsync(m->bed_dw, 0, (size_t)m->num_bed * 4 * sizeof(int32_t), st));
@@ -607,17 +612,25 @@ int substep(cs_sim* m, float sub_dt) {
   int rc = solve(m, u);
   if (rc) return rc;
   mark(m, P_POST);
-  CS_HIP(cs_launch_project(st, u, m->gv, m->kf, m->pres, m->bed_id, m->bed, m->sdf));
+  // the bed particles' grains take the reaction of the water's velocity change
+  // (bed_impulse_add, shader.rs:1484-1491 and 1870-1878)
+  const cs_grains gr = m->num_bed ? cs_grains{m->bed_lookup, m->bed_aff, m->bed_dw, m->num_bed} : cs_grains{};
+  CS_HIP(cs_launch_project(st, u, m->gv, m->kf, m->pres, m->bed_id, m->bed, m->sdf, gr));
   CS_HIP(cs_launch_post_div(st, u, m->gv, m->kf, m->cls, m->sdf, m->tgt, m->metrics));
   CS_HIP(cs_launch_pack(st, u, m->gv, m->kf, m->pack, m->sdf));
-  CS_HIP(cs_launch_visc(st, u, m->gv, m->gv2, m->cls, m->sdf));
+  CS_HIP(cs_launch_visc(st, u, m->gv, m->gv2, m->cls, m->sdf, m->alias4));
   mark(m, P_G2P);
   kmark(m, 2);
-  CS_HIP(cs_launch_g2p(st, u, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf));
+  CS_HIP(cs_launch_g2p(st, u, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf, gr));
   kmark(m, 3);
   // bed_coupling and extraction_advect (mod.rs:391-402), outside the G2P timing
   if (m->num_bed)
     CS_HIP(cs_launch_bed_coupling(st, u, m->part, m->aff, m->bed_lookup, m->num_bed, m->bed_ext, m->bed_dw));
+  // bed_dynamics (mod.rs:404-409): the grains move under the water's drag,
+  // the pressure gradient, packing contacts and the reaction impulses
+  if (m->num_bed)
+    CS_HIP(cs_launch_bed_dynamics(st, u, m->num_bed, m->bed_part, m->bed_aff, m->bed_ext, m->bed_dw, m->bed_snap,
+                                  m->bed_lookup, m->gv2, m->pres, m->alias4, m->sdf));
   mark(m, P_END);
   if (m->timing) {
     CS_HIP(hipEventSynchronize(m->ev[P_END]));
@@ -786,6 +799,8 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
     ALLOC(m->bed_lookup, m->nc);
     ALLOC(m->bed_counter, 1);
     ALLOC(m->bed_dw, (size_t)m->num_bed * 4);  // bed_delta: water lane, then impulse x/y/z
+    ALLOC(m->bed_snap, (size_t)m->num_bed * 4);
+    ALLOC(m->alias4, m->nc);
   }
   ALLOC(m->kind, m->nc);
   ALLOC(m->kf, m->nc);
@@ -895,6 +910,8 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
       e = hipMemcpyAsync(m->bed_ext, scene->bed_extracts, (size_t)m->num_bed * 32, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(m->bed_dw, 0, (size_t)m->num_bed * 4 * sizeof(int32_t), st);
     m->init_bed_ext.assign(scene->bed_extracts, scene->bed_extracts + (size_t)m->num_bed * 8);
+    m->init_bed_part.assign(scene->bed_particles, scene->bed_particles + (size_t)m->num_bed * 8);
+    m->init_bed_aff.assign(scene->bed_affine, scene->bed_affine + (size_t)m->num_bed * 12);
   }
   if (e == hipSuccess) e = hipMemsetAsync(m->gv, 0, m->nc * 16, st);
   if (e == hipSuccess) e = hipMemsetAsync(m->gv2, 0, m->nc * 16, st);
@@ -1070,6 +1087,16 @@ int cs_sim_read_bed_extracts(cs_sim* m, float* extracts) {
   if (!m || !extracts) return fail(CS_ERR_INVALID, "null argument");
   if (!m->num_bed) return CS_OK;
   return read_dev(m, extracts, m->bed_ext, (size_t)m->num_bed * 32);
+}
+int cs_sim_read_bed_particles(cs_sim* m, float* particles, float* affine) {
+  if (!m) return fail(CS_ERR_INVALID, "null sim");
+  if (!m->num_bed) return CS_OK;
+  if (particles) {
+    const int rc = read_dev(m, particles, m->bed_part, (size_t)m->num_bed * 32);
+    if (rc) return rc;
+  }
+  if (affine) return read_dev(m, affine, m->bed_aff, (size_t)m->num_bed * 48);
+  return CS_OK;
 }
 int cs_sim_read_metrics(cs_sim* m, uint32_t* slots) {
   if (!m) return fail(CS_ERR_INVALID, "null sim");

@@ -37,8 +37,12 @@ struct cs_sim {
   float* bed_ext = nullptr;         // BedExtract, 8 floats per bed particle
   int32_t* bed_lookup = nullptr;    // per cell: highest stamping bed particle, or -1
   uint32_t* bed_counter = nullptr;  // table slots taken this substep
-  int32_t* bed_dw = nullptr;        // per bed particle: fixed-point water credit of bed_coupling
+  int32_t* bed_dw = nullptr;        // per bed particle: [4][num_bed] water credit, then reaction impulse x/y/z (fixed point)
+  float* bed_snap = nullptr;        // per bed particle: position and radius before bed_dynamics moves it
+  uint8_t* alias4 = nullptr;        // per cell: the viscosity scratch's z lane reads as the solid kind
   std::vector<float> init_bed_ext;  // the scene's BedExtracts, restored by reset
+  std::vector<float> init_bed_part; // the scene's bed particles and affine rows, restored by reset
+  std::vector<float> init_bed_aff;
   int8_t* kind = nullptr;
   float* kf = nullptr;
   int32_t* pres = nullptr;

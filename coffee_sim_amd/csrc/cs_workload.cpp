@@ -220,6 +220,7 @@ int build(const Spec& sp, cs_workload* out) {
   s.wall_thickness = 2.0f * dx;          // SDF zero level one cell inside the cone
   s.geometry.cone_enabled = 0u;          // the reference's analytic V60/cup fallbacks
   s.geometry.cup_enabled = 0u;           // describe its own scene, not this one
+  s.geometry.filter_enabled = 0u;
   s.sdf_res = 128u;
   const float half = side * 0.5f;
   const float apex_y = -half + (float)apex_cells * dx;

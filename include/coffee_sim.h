@@ -107,6 +107,16 @@ typedef struct cs_geometry {
   float cup_top_y;                    /* -3.5 (shader.rs:805)                  */
   float cup_radius;                   /* 3.0  (shader.rs:806)                  */
   float cup_bottom_y;                 /* -8.0 (shader.rs:107)                  */
+  /* The paper filter that holds a bed of particles (resolve_scene_obstacles with
+   * is_bed, shader.rs:765-801; FilterConfig::default, filter.rs:14-26): bed
+   * grains stay inside the paper cone and rest no lower than its apex floor
+   * (coffee_filter_floor_y, shader.rs:136-146).  Water passes through. */
+  uint32_t filter_enabled;
+  float filter_center_y;              /* -0.35                                  */
+  float filter_top_y;                 /* 2.75 above the centre                  */
+  float filter_bot_y;                 /* -3.02 below the centre                 */
+  float filter_top_radius;            /* 4.10                                   */
+  float filter_thickness;             /* 0.08                                   */
 } cs_geometry;
 
 /* Superset of `MpmSettings` (mpm/mod.rs:87-104).  Fields after `initial_kettle_angle_deg`
@@ -188,9 +198,11 @@ typedef struct cs_scene {
    * Here they sit in their own arrays; p2g and g2p skip them either way
    * (shader.rs:967, 1693).  Every substep clears and re-stamps the cell lookup
    * from them (bed_lookup_clear / bed_lookup_scatter, shader.rs:2291-2335) and
-   * rebuilds the bed cells' records from their BedExtracts.  The bed particles
-   * and extracts stay as given: the passes that move them (bed_coupling,
-   * extraction_advect, bed_dynamics) are not part of this library. */
+   * rebuilds the bed cells' records from their BedExtracts.  After G2P,
+   * bed_coupling and extraction_advect move water into the bed's extracts and
+   * bed_dynamics moves the grains (mod.rs:391-409); the water's reaction on
+   * them is gathered in project_pressure and G2P (shader.rs:458-522).
+   * cs_sim_reset restores the particles, affine rows and extracts given here. */
   const float* bed_particles; /* 8 floats per bed particle, as `particles`     */
   const float* bed_affine;    /* 12 floats per bed particle, .w of col0 = phase */
   const float* bed_extracts;  /* 8 floats per bed particle: BedExtract bed.xyzw
@@ -355,6 +367,13 @@ int cs_sim_read_bed_lookup(cs_sim* sim, int32_t* lookup);
  * mass moves to dissolved.  Nothing is written for a simulation without bed
  * particles. */
 int cs_sim_read_bed_extracts(cs_sim* sim, float* extracts8);
+/* The bed particles (8 floats each, as cs_scene.bed_particles) and their
+ * affine rows (12 floats each: .w of column 0 is the phase, 1 anchored or 2
+ * suspended; column 1 xyz is the anchored rest position) as bed_dynamics
+ * (shader.rs:1991-2238) leaves them after the most recent substep.  Either
+ * pointer may be null; nothing is written for a simulation without bed
+ * particles. */
+int cs_sim_read_bed_particles(cs_sim* sim, float* particles8, float* affine12);
 /* Metric slots (state.rs:22-28). */
 int cs_sim_read_metrics(cs_sim* sim, uint32_t* slots8);
 /* prepare_render (shader.rs:2242-2276, run by step_frame at mod.rs:410-412 and

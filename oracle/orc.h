@@ -78,6 +78,8 @@ void orc_read_pressure(orc_sim* sim, int32_t* p, int32_t* kind);
 void orc_read_bed_lookup(orc_sim* sim, int32_t* lookup);
 /* The bed particles' BedExtracts, 8 floats each (reference mode). */
 void orc_read_bed_extracts(orc_sim* sim, float* ext);
+/* bed grains (8 floats) and affine rows (12 floats) after bed_dynamics; either may be null */
+void orc_read_bed_particles(orc_sim* sim, float* part, float* aff);
 void orc_read_metrics(orc_sim* sim, uint32_t* slots8);
 /* prepare_render (shader.rs:2242-2276): RenderParticle, 8 floats per particle in use. */
 void orc_pack_render(orc_sim* sim, float* render8);
@@ -107,6 +109,8 @@ int orc_debug_step_until(orc_sim* sim, float sub_dt, int stop_after_pass);
 #define ORC_PASS_G2P 8
 #define ORC_PASS_BOUNDARY 9
 #define ORC_PASS_POST_DIVERGENCE 10
+#define ORC_PASS_BED_STAMP 11    /* bed_lookup_clear + bed_lookup_scatter and the bed cells' records */
+#define ORC_PASS_BED_DYNAMICS 12 /* bed_dynamics alone */
 /* One pass on the current state (uniforms of a substep of sub_dt, nothing
  * cleared): ORC_PASS_RBGS runs one red+black pair, PACKING and VISCOSITY their
  * prepare + apply, PROJECT the projection alone. */
@@ -121,6 +125,10 @@ void orc_debug_set_grid_vel(orc_sim* sim, const float* vel4);
 void orc_debug_set_grid_lanes(orc_sim* sim, const int32_t* lanes6);
 void orc_debug_read_grid_lanes(orc_sim* sim, int32_t* lanes6);
 void orc_debug_clear_metrics(orc_sim* sim);
+/* The bed particles' bed_delta lanes, [4][n_bed] i32: water credit, then the
+ * reaction impulse x, y, z (fixed point). */
+void orc_debug_read_bed_delta(orc_sim* sim, int32_t* delta4);
+void orc_debug_set_bed_delta(orc_sim* sim, const int32_t* delta4);
 
 #ifdef __cplusplus
 }

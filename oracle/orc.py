@@ -75,6 +75,7 @@ def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
             "orc_read_pressure": (None, [C.c_void_p, i32p, i32p]),
             "orc_read_bed_lookup": (None, [C.c_void_p, i32p]),
             "orc_read_bed_extracts": (None, [C.c_void_p, f32p]),
+            "orc_read_bed_particles": (None, [C.c_void_p, f32p, f32p]),
             "orc_read_metrics": (None, [C.c_void_p, P(C.c_uint32)]),
             "orc_pack_render": (None, [C.c_void_p, f32p]),
             "orc_get_solve_stats": (None, [C.c_void_p, P(SolveStats)]),
@@ -88,6 +89,8 @@ def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
             "orc_debug_set_grid_lanes": (None, [C.c_void_p, i32p]),
             "orc_debug_read_grid_lanes": (None, [C.c_void_p, i32p]),
             "orc_debug_clear_metrics": (None, [C.c_void_p]),
+            "orc_debug_read_bed_delta": (None, [C.c_void_p, i32p]),
+            "orc_debug_set_bed_delta": (None, [C.c_void_p, i32p]),
             "orc_set_previous_pressure": (None, [C.c_void_p, i32p]),
         }
         for name, (res, args) in sigs.items():
@@ -107,7 +110,7 @@ def _i(a):
 
 
 PASS = {"p2g": 1, "grid": 2, "classify": 3, "rbgs": 4, "project": 5, "packing": 6, "viscosity": 7, "g2p": 8,
-        "boundary": 9, "post_divergence": 10}
+        "boundary": 9, "post_divergence": 10, "bed_stamp": 11, "bed_dynamics": 12}
 
 
 def preset(name: str) -> Settings:
@@ -188,6 +191,16 @@ class OracleSim:
         assert p.size == self.nc
         self._L.orc_set_previous_pressure(self._h, _i(p))
 
+    def debug_read_bed_delta(self) -> np.ndarray:
+        out = np.zeros((4, self.counts()[1]), np.int32)
+        self._L.orc_debug_read_bed_delta(self._h, _i(out))
+        return out
+
+    def debug_set_bed_delta(self, d4: np.ndarray) -> None:
+        a = np.ascontiguousarray(d4, np.int32).reshape(-1)
+        assert a.size == 4 * self.counts()[1]
+        self._L.orc_debug_set_bed_delta(self._h, _i(a))
+
     def debug_clear_metrics(self) -> None:
         self._L.orc_debug_clear_metrics(self._h)
 
@@ -260,6 +273,13 @@ class OracleSim:
         out = np.zeros((self.counts()[1], 8), np.float32)
         self._L.orc_read_bed_extracts(self._h, _f(out))
         return out
+
+    def read_bed_particles(self):
+        nb = self.counts()[1]
+        part = np.zeros((nb, 8), np.float32)
+        aff = np.zeros((nb, 12), np.float32)
+        self._L.orc_read_bed_particles(self._h, _f(part), _f(aff))
+        return part, aff
 
     def read_bed_lookup(self):
         out = np.empty(self.nc, np.int32)

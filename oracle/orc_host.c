@@ -126,6 +126,12 @@ int orc_settings_preset(int preset, cs_settings* s) {
   s->geometry.cup_top_y = -3.5f;
   s->geometry.cup_radius = 3.0f;
   s->geometry.cup_bottom_y = -8.0f;
+  s->geometry.filter_enabled = 1u;
+  s->geometry.filter_center_y = -0.35f;
+  s->geometry.filter_top_y = 2.75f;
+  s->geometry.filter_bot_y = -3.02f;
+  s->geometry.filter_top_radius = 4.10f;
+  s->geometry.filter_thickness = 0.08f;
   s->emitter_mode = CS_EMIT_FLOW_MODEL;
   s->rbgs_mode = CS_RBGS_FIXED_PAIRS;
   s->rbgs_tolerance = 1.0e-4f;
@@ -459,6 +465,11 @@ orc_sim* orc_create(const cs_settings* s, const cs_scene* scene) {
     memcpy(S->bed_ext, scene->bed_extracts, (size_t)n * 8 * sizeof(float));
     S->bed_ext0 = (float*)malloc((size_t)n * 8 * sizeof(float));
     memcpy(S->bed_ext0, scene->bed_extracts, (size_t)n * 8 * sizeof(float));
+    S->bed_part0 = (float*)malloc((size_t)n * 8 * sizeof(float));
+    S->bed_aff0 = (float*)malloc((size_t)n * 12 * sizeof(float));
+    memcpy(S->bed_part0, scene->bed_particles, (size_t)n * 8 * sizeof(float));
+    memcpy(S->bed_aff0, scene->bed_affine, (size_t)n * 12 * sizeof(float));
+    S->bed_snap = (float*)malloc((size_t)n * 3 * sizeof(float));
     S->bed_dw = (int32_t*)calloc((size_t)n * 4, sizeof(int32_t)); /* bed_delta lanes: water, impulse xyz */
   }
   if (scene && scene->particles && scene->num_particles) {
@@ -477,6 +488,7 @@ orc_sim* orc_create(const cs_settings* s, const cs_scene* scene) {
 void orc_destroy(orc_sim* S) {
   if (!S) return;
   free(S->sdf); free(S->cls); free(S->bed_lookup); free(S->bed); free(S->bed_part); free(S->bed_aff); free(S->bed_ext); free(S->bed_ext0); free(S->bed_dw);
+  free(S->bed_part0); free(S->bed_aff0); free(S->bed_snap);
   free(S->particles); free(S->affine); free(S->init_particles); free(S->init_affine);
   free(S->grid); free(S->p2g_lanes); free(S->grid_vel); free(S->prev_p); free(S->warm);
   free(S->saved_p); free(S->saved_kind); free(S->tgt);
@@ -500,6 +512,9 @@ int orc_reset(orc_sim* S) {
   }
   memset(S->prev_p, 0, S->nc * sizeof(int32_t));
   if (S->bed_n) {
+    /* the bed as the scene gave it (the reference re-runs init_bed, mod.rs:432-446) */
+    memcpy(S->bed_part, S->bed_part0, (size_t)S->bed_n * 8 * sizeof(float));
+    memcpy(S->bed_aff, S->bed_aff0, (size_t)S->bed_n * 12 * sizeof(float));
     memcpy(S->bed_ext, S->bed_ext0, (size_t)S->bed_n * 8 * sizeof(float));
     memset(S->bed_dw, 0, (size_t)S->bed_n * 4 * sizeof(int32_t));
   }

@@ -113,7 +113,10 @@ struct orc_sim {
   float* bed_aff;  /* 12 per bed particle */
   float* bed_ext;  /* BedExtract, 8 per bed particle */
   float* bed_ext0; /* the scene's BedExtracts, restored by reset */
-  int32_t* bed_dw; /* fixed-point water credit per bed particle (bed_delta lane 0) */
+  float* bed_part0; /* the scene's bed particles and affine state, restored by reset */
+  float* bed_aff0;
+  float* bed_snap; /* bed positions before bed_dynamics (3 per bed particle) */
+  int32_t* bed_dw; /* bed_delta (shader.rs:451-453): [4][bed_n] lanes water, impulse x, y, z */
   float* particles; /* 8 per particle */
   float* affine;    /* 12 per particle */
   float* init_particles;

@@ -182,14 +182,50 @@ static contact_t radial_barrier(const orc_params* u, v3 pos, v3 vel, float cx, f
   contact_t c = {pos, vel};
   return c;
 }
-/* resolve_scene_obstacles with is_bed = false (water) */
-static contact_t scene_obstacles(const orc_params* u, v3 pos, v3 vel) {
+/* bed_particle_radius (shader.rs:128): dx * bed_sample_radius_dx (brew_config.rs:51) */
+static float bed_particle_radius(const orc_params* u) { return u->dx * 0.62f; }
+/* coffee_filter_floor_y (shader.rs:136-146) with the filter of the geometry */
+static float coffee_filter_floor_y(const orc_params* u) {
+  const cs_geometry* g = &u->geo;
+  float fb = g->filter_center_y + g->filter_bot_y, ft = g->filter_center_y + g->filter_top_y;
+  float bco = o_max(u->contact_offset, bed_particle_radius(u));
+  float fh = o_max(ft - fb, 1e-6f);
+  float slope = g->filter_top_radius / fh;
+  return fb + (bco + g->filter_thickness) / o_max(slope, 1e-6f);
+}
+/* resolve_scene_obstacles (shader.rs:744-823): the cone, then for bed grains the
+ * paper filter and its apex floor, then the cup */
+static contact_t scene_obstacles(const orc_params* u, v3 pos, v3 vel, int is_bed) {
   const cs_geometry* g = &u->geo;
   if (g->cone_enabled && pos.y <= g->cone_top_y && pos.y >= g->cone_bot_y) {
     float t = o_clamp((pos.y - g->cone_bot_y) / (g->cone_top_y - g->cone_bot_y), 0.0f, 1.0f);
     float cone_r = o_mix(g->cone_bot_radius, g->cone_top_radius, t) - u->contact_offset;
     contact_t c = radial_barrier(u, pos, vel, 0.0f, 0.0f, cone_r);
     pos = c.pos; vel = c.vel;
+  }
+  if (is_bed && g->filter_enabled) {
+    float ft = g->filter_center_y + g->filter_top_y, fb = g->filter_center_y + g->filter_bot_y;
+    float bco = o_max(u->contact_offset, bed_particle_radius(u));
+    float floor_y = coffee_filter_floor_y(u);
+    if (pos.y <= ft && pos.y >= fb) {
+      float t = o_clamp((pos.y - fb) / (ft - fb), 0.0f, 1.0f);
+      float outer_r = o_mix(0.0f, g->filter_top_radius, t);
+      float fr = o_max(outer_r - g->filter_thickness, 0.0f) - bco;
+      contact_t c = radial_barrier(u, pos, vel, 0.0f, 0.0f, o_max(fr, 0.0f));
+      pos = c.pos; vel = c.vel;
+    }
+    if (pos.y <= floor_y) {
+      pos.y = floor_y;
+      pos.x = 0.0f;
+      pos.z = 0.0f;
+      if (vel.y < 0.0f) {
+        vel.y = 0.0f;
+        vel.x *= 1.0f - u->friction * 0.55f;
+        vel.z *= 1.0f - u->friction * 0.55f;
+      }
+      vel.x *= 1.0f - u->friction * 0.55f;
+      vel.z *= 1.0f - u->friction * 0.55f;
+    }
   }
   if (g->cup_enabled && pos.y <= g->cup_top_y) {
     float cup_r = g->cup_radius - u->wall_half - u->contact_offset;
@@ -208,7 +244,8 @@ static contact_t scene_obstacles(const orc_params* u, v3 pos, v3 vel) {
   contact_t c = {pos, vel};
   return c;
 }
-static contact_t sdf_contact(orc_sim* S, const orc_params* u, v3 pos, v3 vel) {
+/* resolve_sdf_contact (shader.rs:879-903) */
+static contact_t sdf_contact_as(orc_sim* S, const orc_params* u, v3 pos, v3 vel, int is_bed) {
   float sv = sample_sdf(S, u, pos);
   if (sv < u->contact_offset) {
     v3 n = sdf_gradient(S, u, pos);
@@ -226,7 +263,10 @@ static contact_t sdf_contact(orc_sim* S, const orc_params* u, v3 pos, v3 vel) {
       }
     }
   }
-  return scene_obstacles(u, pos, vel);
+  return scene_obstacles(u, pos, vel, is_bed);
+}
+static contact_t sdf_contact(orc_sim* S, const orc_params* u, v3 pos, v3 vel) {
+  return sdf_contact_as(S, u, pos, vel, 0);
 }
 
 /* ---- bed record access (field mode of shader.rs:361-449) ------------------- */
@@ -278,6 +318,51 @@ static float bed_velocity_divergence(orc_sim* S, const orc_params* u, i3 c) { /*
   a = c; a.z -= 1; float vzm = bed_velocity_load_cell(S, u, a).z;
   a = c; a.z += 1; float vzp = bed_velocity_load_cell(S, u, a).z;
   return 0.5f * u->inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
+}
+
+/* ---- bed reaction impulses (shader.rs:458-515) ------------------------------
+ * The momentum a bed removes from the water (grid Darcy in project_pressure,
+ * particle drag in G2P) is handed back to the bed grains through bed_delta
+ * lanes 1-3 (i32 fixed point, wrapping adds), read and cleared by
+ * bed_dynamics.  Only a bed of particles has grains to receive it. */
+#define O_BED_REACTION_ALPHA 0.04f      /* shader.rs:76 */
+#define O_BED_REACTION_IMPULSE_CAP 0.012f /* shader.rs:77 */
+static int bed_solid_id(orc_sim* S, int32_t id) { /* is_valid_bed_solid_idx, shader.rs:365-370 */
+  return id >= 0 && (uint32_t)id < S->bed_n && !(S->bed_aff[12 * (size_t)id + 3] < 0.5f);
+}
+static void bed_impulse_add(orc_sim* S, const orc_params* u, int32_t bid, v3 imp) { /* shader.rs:462-477 */
+  float cap = u->nominal_mass * u->vel_cap * O_BED_REACTION_IMPULSE_CAP;
+  float cx = o_clamp(imp.x, -cap, cap), cy = o_clamp(imp.y, -cap, cap), cz = o_clamp(imp.z, -cap, cap);
+  ORC_LANE_ADD(&S->bed_dw[1u * S->bed_n + (uint32_t)bid], o_f2i(cx * u->fp_scale));
+  ORC_LANE_ADD(&S->bed_dw[2u * S->bed_n + (uint32_t)bid], o_f2i(cy * u->fp_scale));
+  ORC_LANE_ADD(&S->bed_dw[3u * S->bed_n + (uint32_t)bid], o_f2i(cz * u->fp_scale));
+}
+/* bed_impulse_delta_add_neighborhood (shader.rs:479-515): split over the grains
+ * stamped in the 3x3x3 cells around `cell` (one share per stamped cell) */
+static void bed_impulse_neighborhood(orc_sim* S, const orc_params* u, i3 cell, v3 imp) {
+  float count = 0.0f;
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        int32_t x = cell.x + di, y = cell.y + dj, z = cell.z + dk;
+        if (!on_grid(u, x, y, z)) continue;
+        if (bed_solid_id(S, S->bed_lookup[cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z)])) count += 1.0f;
+      }
+  if (count <= 0.0f) return;
+  v3 share = V(imp.x / count, imp.y / count, imp.z / count);
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        int32_t x = cell.x + di, y = cell.y + dj, z = cell.z + dk;
+        if (!on_grid(u, x, y, z)) continue;
+        int32_t id = S->bed_lookup[cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z)];
+        if (bed_solid_id(S, id)) bed_impulse_add(S, u, id, share);
+      }
+}
+static i3 world_to_cell(const orc_params* u, v3 p) { /* shader.rs:668-671 */
+  i3 c = {o_f2i(floorf((p.x - u->origin[0]) * u->inv_dx)), o_f2i(floorf((p.y - u->origin[1]) * u->inv_dx)),
+          o_f2i(floorf((p.z - u->origin[2]) * u->inv_dx))};
+  return c;
 }
 
 /* shader.rs:597-619 */
@@ -713,6 +798,11 @@ void orc_read_system(orc_sim* S, float* w6o, float* cnto, float* co) {
 void orc_read_bed_extracts(orc_sim* S, float* ext) {
   if (S->bed_n) memcpy(ext, S->bed_ext, (size_t)S->bed_n * 8 * sizeof(float));
 }
+void orc_read_bed_particles(orc_sim* S, float* part, float* aff) {
+  if (!S->bed_n) return;
+  if (part) memcpy(part, S->bed_part, (size_t)S->bed_n * 8 * sizeof(float));
+  if (aff) memcpy(aff, S->bed_aff, (size_t)S->bed_n * 12 * sizeof(float));
+}
 void orc_read_bed_lookup(orc_sim* S, int32_t* lookup) {
   memcpy(lookup, S->bed_lookup, S->nc * sizeof(int32_t));
 }
@@ -827,9 +917,13 @@ static void k_project(orc_sim* S, const orc_params* u) {
         float rate = u->water_nu / k;
         float damping = 1.0f / (1.0f + o_max(rate * u->dt, 0.0f));
         v3 bv = bed_matrix_velocity_cell(S, u, c);
+        v3 before = v;
         v = vadd(bv, vmul(vsub(v, bv), damping));
-        /* the bed reaction impulse (shader.rs:1534-1535) feeds only the bed
-         * particle passes, which a field bed does not have */
+        /* the reaction on the grains (shader.rs:1533-1535); a field bed has none */
+        if (S->bed_n) {
+          v3 wi = vmul(vsub(v, before), g[3]);
+          bed_impulse_neighborhood(S, u, c, vmul(V(-wi.x, -wi.y, -wi.z), O_BED_REACTION_ALPHA));
+        }
       }
     }
     float speed = vlen(v);
@@ -1059,8 +1153,13 @@ static void k_g2p(orc_sim* S, const orc_params* u) {
       float dd = 1.0f / (1.0f + o_max(rate * u->dt * porous, 0.0f));
       float rs = vlen(rel);
       float idd = 1.0f / (1.0f + o_max(rs * u->dt * u->inv_dx * porous * 0.35f, 0.0f));
+      v3 before = nv;
       nv = vadd(pbv, vmul(vmul(rel, dd), idd));
-      /* bed reaction impulse (shader.rs:1839-1843) feeds only bed particles */
+      /* the reaction on the grains (shader.rs:1837-1843); a field bed has none */
+      if (S->bed_n) {
+        v3 wi = vmul(vsub(nv, before), mass_p);
+        bed_impulse_neighborhood(S, u, world_to_cell(u, xp), vmul(V(-wi.x, -wi.y, -wi.z), O_BED_REACTION_ALPHA));
+      }
       float ad = o_mix(1.0f, dd * idd, porous);
       C0 = vmul(C0, ad); C1 = vmul(C1, ad); C2 = vmul(C2, ad);
       j_support = o_max(j_support, porous);
@@ -1217,6 +1316,236 @@ static void extraction_advect(orc_sim* S, const orc_params* u) {
   }
 }
 
+/* pressure_gradient_at_cell (shader.rs:238-259) as bed_dynamics sees it: lane 0
+ * still holds the solved pressure, but lane 3, which the reference reads as the
+ * cell kind, holds viscosity_prepare's velocity scratch z by then (shader.rs:
+ * 158-164, 274-283), so a neighbour counts as solid (mirror) exactly when that
+ * scratch is 4 in fixed point.  The oracle keeps the reference's lane layout,
+ * so reading the lanes reproduces this. */
+static float grad_pressure_or_mirror(orc_sim* S, const orc_params* u, i3 c, float mirror) {
+  if (!on_grid(u, c.x, c.y, c.z)) return mirror;
+  size_t ci = cidx(u, (uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z);
+  if (is_solid(kind_load(S, ci))) return mirror;
+  return pressure_load(S, u, ci);
+}
+static v3 pressure_gradient_at_cell(orc_sim* S, const orc_params* u, i3 c) {
+  if (!on_grid(u, c.x, c.y, c.z)) return V(0.0f, 0.0f, 0.0f);
+  float ph = pressure_load(S, u, cidx(u, (uint32_t)c.x, (uint32_t)c.y, (uint32_t)c.z));
+  i3 a;
+  a = c; a.x -= 1; float pxm = grad_pressure_or_mirror(S, u, a, ph);
+  a = c; a.x += 1; float pxp = grad_pressure_or_mirror(S, u, a, ph);
+  a = c; a.y -= 1; float pym = grad_pressure_or_mirror(S, u, a, ph);
+  a = c; a.y += 1; float pyp = grad_pressure_or_mirror(S, u, a, ph);
+  a = c; a.z -= 1; float pzm = grad_pressure_or_mirror(S, u, a, ph);
+  a = c; a.z += 1; float pzp = grad_pressure_or_mirror(S, u, a, ph);
+  float h = 0.5f * u->inv_dx;
+  return V(h * (pxp - pxm), h * (pyp - pym), h * (pzp - pzm));
+}
+
+/* resolve_coffee_particle_packing (shader.rs:825-877): push a grain out of its
+ * nearest stamped neighbour closer than 1.78 grain radii.  The reference reads
+ * the neighbours' positions while bed_dynamics moves them, so what it sees
+ * depends on thread scheduling; here every grain reads the positions from
+ * before the pass (`snap`), which the device pass reproduces in any order. */
+static contact_t coffee_packing(orc_sim* S, const orc_params* u, v3 pos, v3 vel, uint32_t self, const float* snap) {
+  v3 cd = V(0.0f, 0.0f, 0.0f);
+  float closest = 1e9f;
+  int found = 0;
+  i3 cc = world_to_cell(u, pos);
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        int32_t x = cc.x + di, y = cc.y + dj, z = cc.z + dk;
+        if (!on_grid(u, x, y, z)) continue;
+        int32_t o = S->bed_lookup[cidx(u, (uint32_t)x, (uint32_t)y, (uint32_t)z)];
+        if (o < 0 || (uint32_t)o == self || (uint32_t)o >= S->bed_n) continue;
+        v3 d = vsub(pos, V(snap[3 * (size_t)o], snap[3 * (size_t)o + 1], snap[3 * (size_t)o + 2]));
+        float dist = vlen(d);
+        if (dist < closest) { closest = dist; cd = d; found = 1; }
+      }
+  float min_sep = bed_particle_radius(u) * 1.78f;
+  if (found && closest < min_sep) {
+    v3 n = V(0.0f, 1.0f, 0.0f);
+    if (closest > 1e-5f) n = vdiv(cd, closest);
+    float pen = min_sep - closest;
+    pos = vadd(pos, vmul(n, o_min(pen * 0.65f, u->dx * 0.22f)));
+    float vn = vdot(vel, n);
+    if (vn < 0.0f) vel = vsub(vel, vmul(n, vn));
+  }
+  contact_t c = {pos, vel};
+  return c;
+}
+
+/* bed_dynamics (shader.rs:1991-2238): each grain takes its reaction impulse,
+ * then either drifts with the water (suspended phase 2) or moves as an
+ * overdamped, plastically compacting porous matrix around its rest position
+ * (anchored phase 1), with packing against its neighbours and the paper
+ * filter's contact.  bed_damping 8.5, bed_compaction_rate 5.5, bed_impact 8.0
+ * (mod.rs:644-649, brew_config.rs:69-70). */
+static void bed_dynamics(orc_sim* S, const orc_params* u) {
+  const uint32_t nbed = S->bed_n;
+  const float bed_damping = 8.5f, bed_compaction = 5.5f, bed_impact = 8.0f;
+  const float floor_y = coffee_filter_floor_y(u);
+  const float dx = u->dx, dt = u->dt;
+  for (uint32_t b = 0; b < nbed; ++b) {
+    S->bed_snap[3 * (size_t)b] = S->bed_part[8 * (size_t)b];
+    S->bed_snap[3 * (size_t)b + 1] = S->bed_part[8 * (size_t)b + 1];
+    S->bed_snap[3 * (size_t)b + 2] = S->bed_part[8 * (size_t)b + 2];
+  }
+  ORC_PARALLEL_FOR
+  for (uint32_t bid = 0; bid < nbed; ++bid) {
+    float* p = S->bed_part + 8 * (size_t)bid;
+    float* a = S->bed_aff + 12 * (size_t)bid;
+    float* be = S->bed_ext + 8 * (size_t)bid;
+    float phase = a[3];
+    v3 rest = V(a[4], a[5], a[6]);
+    v3 pos = V(p[0], p[1], p[2]);
+    const float mass_p = p[7];
+    /* bed_impulse_delta_exchange (shader.rs:517-522) */
+    v3 imp = V((float)S->bed_dw[1u * nbed + bid] * u->inv_fp_scale, (float)S->bed_dw[2u * nbed + bid] * u->inv_fp_scale,
+               (float)S->bed_dw[3u * nbed + bid] * u->inv_fp_scale);
+    S->bed_dw[1u * nbed + bid] = S->bed_dw[2u * nbed + bid] = S->bed_dw[3u * nbed + bid] = 0;
+    v3 brv = vdiv(imp, o_max(mass_p, u->nominal_mass * 0.25f));
+    const float max_rs = o_min(u->vel_cap * 0.012f, dx / o_max(dt, 1e-6f) * 0.02f);
+    const float rs = vlen(brv);
+    if (rs > max_rs && rs > 1e-6f) brv = vmul(brv, max_rs / rs);
+    /* the water around the grain: quadratic B-spline gather of grid_vel */
+    v3 gp = vmul(vsub(pos, V(u->origin[0], u->origin[1], u->origin[2])), u->inv_dx);
+    int32_t bx = o_f2i(floorf(gp.x - 0.5f)), by = o_f2i(floorf(gp.y - 0.5f)), bz = o_f2i(floorf(gp.z - 0.5f));
+    v3 fx = V(gp.x - (float)bx, gp.y - (float)by, gp.z - (float)bz);
+    float wx[3], wy[3], wz[3];
+    wx[0] = 0.5f * (1.5f - fx.x) * (1.5f - fx.x);
+    wx[1] = 0.75f - (fx.x - 1.0f) * (fx.x - 1.0f);
+    wx[2] = 0.5f * (fx.x - 0.5f) * (fx.x - 0.5f);
+    wy[0] = 0.5f * (1.5f - fx.y) * (1.5f - fx.y);
+    wy[1] = 0.75f - (fx.y - 1.0f) * (fx.y - 1.0f);
+    wy[2] = 0.5f * (fx.y - 0.5f) * (fx.y - 0.5f);
+    wz[0] = 0.5f * (1.5f - fx.z) * (1.5f - fx.z);
+    wz[1] = 0.75f - (fx.z - 1.0f) * (fx.z - 1.0f);
+    wz[2] = 0.5f * (fx.z - 0.5f) * (fx.z - 0.5f);
+    v3 water_v = V(0.0f, 0.0f, 0.0f);
+    float water_mass = 0.0f, asw = 0.0f, lasw = 0.0f;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          int32_t cx = bx + i, cy = by + j, cz = bz + k;
+          if (!on_grid(u, cx, cy, cz)) continue;
+          float w = wx[i] * wy[j] * wz[k];
+          size_t ci = cidx(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+          const float* g = &S->grid_vel[ci * 4];
+          water_v = vadd(water_v, vmul(V(g[0], g[1], g[2]), w));
+          water_mass += w * g[3];
+          int32_t sb = S->bed_lookup[ci];
+          if (sb >= 0 && (uint32_t)sb != bid) {
+            asw += w;
+            float ccy = u->origin[1] + ((float)cy + 0.5f) * dx;
+            if (ccy <= pos.y + dx * 0.25f) lasw += w;
+          }
+        }
+    if (phase >= 1.5f) { /* suspended grain (shader.rs:2065-2105) */
+      float ws = o_clamp(water_mass / o_max(u->nominal_mass * 2.0f, 1e-6f), 0.0f, 1.0f);
+      float drag = o_clamp((8.0f + 18.0f * ws) * dt, 0.0f, 1.0f);
+      v3 unsup = vadd(vadd(V(p[4], p[5], p[6]), brv), V(0.0f, u->gravity * dt * 0.85f, 0.0f));
+      float settle = o_mix(1.10f, 0.32f, ws);
+      v3 carried = vadd(water_v, V(0.0f, -settle, 0.0f));
+      v3 sv = V(o_mix(unsup.x, carried.x, drag), o_mix(unsup.y, carried.y, drag), o_mix(unsup.z, carried.z, drag));
+      float sp = vlen(sv);
+      if (sp > u->vel_cap) sv = vmul(sv, u->vel_cap / sp);
+      v3 spos = vadd(pos, vmul(sv, dt));
+      contact_t c = sdf_contact_as(S, u, spos, sv, 1);
+      spos = c.pos; sv = c.vel;
+      p[0] = spos.x; p[1] = spos.y; p[2] = spos.z;
+      p[4] = sv.x; p[5] = sv.y; p[6] = sv.z; p[7] = mass_p;
+      int anchored = lasw > 0.06f || asw > 0.20f;
+      int dense = asw > 0.42f;
+      int redeposit = (spos.y <= floor_y + dx * 0.35f && ws < 0.35f) ||
+                      (anchored && sv.y <= 0.20f && (ws < 0.55f || dense));
+      if (redeposit) phase = 1.0f;
+      a[3] = phase;
+      a[4] = spos.x; a[5] = spos.y; a[6] = spos.z; a[7] = 0.0f;
+      be[3] *= 0.985f;
+      continue;
+    }
+    /* anchored grain (shader.rs:2107-2237) */
+    const float sat = be[7];
+    const float wl = o_clamp(water_mass / o_max(u->nominal_mass * 1.5f, 1e-6f), 0.0f, 1.0f);
+    const float wet = 1.0f + sat * 0.40f;
+    const float mob = o_clamp((0.30f + wl * 0.62f) * wet, 0.0f, 1.0f);
+    const float sf = o_clamp((rest.y + 3.0f) / 3.5f, 0.12f, 1.0f);
+    const float damping = o_clamp(1.0f - bed_damping * dt, 0.0f, 1.0f);
+    v3 vel = vadd(V(p[4], p[5], p[6]), brv);
+    const float sr = o_clamp(1.0f - asw / 0.55f, 0.0f, 1.0f);
+    const float ed = o_clamp((2.0f + 5.0f * sat) * wl * sr * dt, 0.0f, 0.08f);
+    v3 target = vadd(water_v, V(0.0f, -o_mix(0.95f, 0.35f, wl), 0.0f));
+    vel = V(o_mix(vel.x, target.x, ed), o_mix(vel.y, target.y, ed), o_mix(vel.z, target.z, ed));
+    const float ls = sqrtf(water_v.x * water_v.x + water_v.z * water_v.z);
+    const float ldx = ls > 1e-6f ? water_v.x / ls : 0.0f, ldz = ls > 1e-6f ? water_v.z / ls : 0.0f;
+    const float lss = o_min(ls, fabsf(o_min(water_v.y, 0.0f)) * 0.18f);
+    v3 impact = V(ldx * lss * 0.08f, o_min(water_v.y, 0.0f) * 0.62f, ldz * lss * 0.08f);
+    vel = vadd(vel, vmul(vmul(vmul(vmul(impact, bed_impact), mob), sf), dt));
+    v3 pg = pressure_gradient_at_cell(S, u, world_to_cell(u, pos));
+    const float sfx = -pg.x, sfz = -pg.z;
+    const float sps = sqrtf(sfx * sfx + sfz * sfz);
+    const float sdx = sps > 1e-6f ? sfx / sps : 0.0f, sdz = sps > 1e-6f ? sfz / sps : 0.0f;
+    const float drive = o_min(sps, u->vel_cap / o_max(dt, 1e-6f));
+    const float pl = o_clamp(water_mass * (0.25f + sat), 0.0f, 1.0f);
+    const float slx = sdx * drive * pl * 0.08f, slz = sdz * drive * pl * 0.08f;
+    vel.x += slx * bed_impact * mob * sf * dt;
+    vel.z += slz * bed_impact * mob * sf * dt;
+    vel = vmul(vel, damping);
+    v3 np = vadd(pos, vmul(vel, dt));
+    v3 off = vsub(np, rest);
+    const float ll = sqrtf(off.x * off.x + off.z * off.z);
+    const float lpt = dx * 0.22f;
+    if (ll > lpt && ll > 1e-6f) {
+      const float ox = off.x / ll, oz = off.z / ll;
+      const float excess = ll - lpt;
+      const float lp = o_clamp((0.04f + sat * 0.35f + mob * 0.45f + pl * 0.25f) * sf * bed_compaction * dt, 0.0f, 0.18f);
+      rest.x += ox * excess * lp;
+      rest.z += oz * excess * lp;
+      off = vsub(np, rest);
+    }
+    const float cll = sqrtf(off.x * off.x + off.z * off.z);
+    const float ml = dx * 0.9f * sf;
+    if (cll > ml && cll > 1e-6f) {
+      const float ox = off.x / cll, oz = off.z / cll;
+      off.x = ox * ml;
+      off.z = oz * ml;
+      vel.x *= 0.4f;
+      vel.z *= 0.4f;
+    }
+    off.y = o_clamp(off.y, -dx * (1.75f * sf + 0.2f), dx * 0.04f);
+    np = vadd(rest, off);
+    contact_t pc = coffee_packing(S, u, np, vel, bid, S->bed_snap);
+    np = pc.pos; vel = pc.vel;
+    const float comp = o_max(rest.y - np.y, 0.0f);
+    const float pth = dx * 0.18f;
+    if (comp > pth) {
+      const float ex = comp - pth;
+      const float plast = o_clamp((0.05f + sat * 0.35f + mob * 0.55f) * sf * bed_compaction * dt, 0.0f, 0.18f);
+      rest.y -= ex * plast;
+    }
+    const int detach = sat > 0.82f && pl > 0.65f && wl > 0.65f && sr > 0.35f &&
+                       (comp > dx * 0.65f || cll > dx * 0.65f);
+    const int entrained = sat > 0.58f && wl > 0.74f && sr > 0.55f && vlen(water_v) > 0.95f &&
+                          np.y > floor_y + dx * 0.70f;
+    const int isolated = sat > 0.55f && water_mass < u->nominal_mass * 0.75f && asw < 0.04f && lasw < 0.03f &&
+                         np.y > floor_y + dx * 0.50f;
+    if (detach || entrained || isolated) {
+      phase = 2.0f;
+      rest = np;
+    }
+    contact_t c = sdf_contact_as(S, u, np, vel, 1);
+    np = c.pos; vel = c.vel;
+    p[0] = np.x; p[1] = np.y; p[2] = np.z;
+    p[4] = vel.x; p[5] = vel.y; p[6] = vel.z; p[7] = mass_p;
+    a[3] = phase;
+    a[4] = rest.x; a[5] = rest.y; a[6] = rest.z; a[7] = 0.0f;
+    const float gc = o_max((rest.y - np.y) / o_max(dx, 1e-6f), 0.0f);
+    be[3] = o_max(be[3] * 0.995f, gc);
+  }
+}
+
 /* ---- one substep in step_frame order (mod.rs:271-418) ---------------------- */
 static int substep(orc_sim* S, float sub_dt, int stop_after, int emit) {
   if (emit) orc_emit(S, sub_dt);
@@ -1250,9 +1579,10 @@ static int substep(orc_sim* S, float sub_dt, int stop_after, int emit) {
   if (stop_after == ORC_PASS_VISCOSITY) return 0;
   k_g2p(S, u);
   if (stop_after == ORC_PASS_G2P) return 0;
-  if (S->bed_n) { /* mod.rs:391-402 */
+  if (S->bed_n) { /* mod.rs:391-406 */
     bed_coupling(S, u);
     extraction_advect(S, u);
+    bed_dynamics(S, u);
   }
   S->total_time += sub_dt;
   return 0;
@@ -1278,6 +1608,8 @@ int orc_debug_run_pass(orc_sim* S, float sub_dt, int pass) {
     case ORC_PASS_VISCOSITY: k_viscosity_prepare(S, u); k_viscosity_apply(S, u); break;
     case ORC_PASS_G2P: k_g2p(S, u); break;
     case ORC_PASS_POST_DIVERGENCE: k_post_divergence(S, u); break;
+    case ORC_PASS_BED_STAMP: if (S->bed_n) bed_restamp(S, u); break;
+    case ORC_PASS_BED_DYNAMICS: if (S->bed_n) bed_dynamics(S, u); break;
     default: return CS_ERR_INVALID;
   }
   return CS_OK;
@@ -1287,6 +1619,12 @@ void orc_debug_set_grid_vel(orc_sim* S, const float* vel4) { memcpy(S->grid_vel,
 void orc_debug_set_grid_lanes(orc_sim* S, const int32_t* lanes6) { memcpy(S->grid, lanes6, S->nc * 6 * sizeof(int32_t)); }
 void orc_debug_read_grid_lanes(orc_sim* S, int32_t* lanes6) { memcpy(lanes6, S->grid, S->nc * 6 * sizeof(int32_t)); }
 void orc_debug_clear_metrics(orc_sim* S) { memset(S->metrics, 0, sizeof(S->metrics)); }
+void orc_debug_read_bed_delta(orc_sim* S, int32_t* d) {
+  if (S->bed_n) memcpy(d, S->bed_dw, (size_t)S->bed_n * 4 * sizeof(int32_t));
+}
+void orc_debug_set_bed_delta(orc_sim* S, const int32_t* d) {
+  if (S->bed_n) memcpy(S->bed_dw, d, (size_t)S->bed_n * 4 * sizeof(int32_t));
+}
 
 /* MpmSim3D::step_frame (mod.rs:264-430) */
 int orc_step_frame(orc_sim* S, float dt) {

@@ -26,6 +26,7 @@ def box_settings(n: int = 16, cup: bool = False, rbgs_pairs: int = 40, tolerance
     s.particle_vol_factor = pvol
     s.geometry.cone_enabled = 0
     s.geometry.cup_enabled = 0
+    s.geometry.filter_enabled = 0
     s.emitter_mode = abi.CS_EMIT_FIXED_COUNT
     s.emit_per_substep = 0
     s.sdf_res = 2 * n

@@ -8,7 +8,8 @@ reference-mode oracle step against a field-mode oracle step handed the same
 lookup and the records that the BedExtracts and particles give.
 
 GPU: the device stamp, bed table and the whole substep with a seated bed and
-water soaking into it equal the oracle bit for bit.
+water soaking into it -- including bed_coupling, extraction_advect and the
+grains' own motion (bed_dynamics) -- equal the oracle bit for bit.
 """
 from __future__ import annotations
 
@@ -317,7 +318,11 @@ def test_bed_particles_match_oracle(ctx, orc):
         assert_bitwise(f"bed lookup frame {k}", sim.read_bed_lookup(), ref.read_bed_lookup())
         compare_state(sim, ref, f"seated bed frame {k}")
         assert_bitwise(f"bed extracts frame {k}", sim.read_bed_extracts(), ref.read_bed_extracts())
+        for name, g, o in zip(("bed grains", "bed affine"), sim.read_bed_particles(), ref.read_bed_particles()):
+            assert_bitwise(f"{name} frame {k}", g, o)
     assert (sim.read_bed_extracts()[:, 0] > 0).sum() > 100  # the bed took up water
+    moved = np.abs(sim.read_bed_particles()[0][:, 0:3] - bed["particles"][:, 0:3]).max(axis=1)
+    assert (moved > 0).sum() > 100  # bed_dynamics moved the grains
     # prepare_render: bed particles first, radius 0.62 dx, colour -1 - saturation
     rg, ro = sim.pack_render(), ref.pack_render()
     assert rg.shape == (sim.counts()[0] + len(bed["particles"]), 8) == (sim.particle_count(), 8)
@@ -328,6 +333,29 @@ def test_bed_particles_match_oracle(ctx, orc):
     assert (sat > 0).any() and (rg[nb:, 3] >= 0.0).all()
     _, kind = sim.read_pressure()
     assert (kind == 3).sum() > 100
+
+
+@pytest.mark.gpu
+def test_bed_grains_follow_the_oracle_over_a_pour(ctx, orc):
+    """Six frames of a pour onto the seated bed: the grains' positions,
+    velocities, phases and rest positions, the extracts and the water equal the
+    oracle bit for bit at every frame, and a reset restores the scene's bed."""
+    s, sc, bed = bed_scene(seed=3, n_water=30000, substeps=2)
+    sim, ref = cs.MpmSim3D(ctx, s, sc), orc.OracleSim(s, sc)
+    for k in range(6):
+        sim.step_frame(1.0 / 200.0)
+        ref.step_frame(1.0 / 200.0)
+        for name, g, o in zip(("bed grains", "bed affine"), sim.read_bed_particles(), ref.read_bed_particles()):
+            assert_bitwise(f"{name} frame {k}", g, o)
+        assert_bitwise(f"bed extracts frame {k}", sim.read_bed_extracts(), ref.read_bed_extracts())
+        compare_state(sim, ref, f"pour frame {k}")
+    part, aff = sim.read_bed_particles()
+    assert np.abs(part[:, 0:3] - bed["particles"][:, 0:3]).max() > 1e-4
+    sim.reset()
+    part, aff = sim.read_bed_particles()
+    assert_bitwise("reset grains", part, bed["particles"])
+    assert_bitwise("reset affine", aff, bed["affine"])
+    assert_bitwise("reset extracts", sim.read_bed_extracts(), bed["extracts"])
 
 
 @pytest.mark.gpu
@@ -365,6 +393,8 @@ def test_seated_bed_checkpoint_resumes_bitwise(ctx):
     b.step_frame(1.0 / 300.0)
     compare_state(a, b, "resumed seated bed")
     assert_bitwise("bed extracts", a.read_bed_extracts(), b.read_bed_extracts())
+    for name, g, o in zip(("bed grains", "bed affine"), a.read_bed_particles(), b.read_bed_particles()):
+        assert_bitwise(name, g, o)
     # a checkpoint of another bed (none) is rejected
     plain = cs.MpmSim3D(ctx, s, None)
     with pytest.raises(cs.CoffeeSimError):
