@@ -449,6 +449,9 @@ def main() -> int:
     ap.add_argument("--state", default=None,
                     help="checkpoint of the post-pour state: loaded when the file exists (the pour is skipped), "
                          "written after the pour otherwise")
+    ap.add_argument("--mg-cycles", type=int, default=0,
+                    help="multigrid W-cycles correcting each solve's start before the red-black iteration "
+                         "(rbgs_mg_cycles; 0 = the plain iteration)")
     ap.add_argument("--no-zero-probe", action="store_true",
                     help="skip the one untimed substep from a zero start that reports its iteration count")
     args = ap.parse_args()
@@ -476,6 +479,7 @@ def main() -> int:
     w = cs.SeededWorkload("v60_working", seed=args.seed, scale=args.scale)
     settings = w.settings.copy()
     settings.warm_start = CS_WARM_PREVIOUS if args.warm_start == "previous" else CS_WARM_ZERO
+    settings.rbgs_mg_cycles = max(args.mg_cycles, 0)
     ctx = cs.Context(0)
     sim = cs.MpmSim3D(ctx, settings, w.scene)
     dt = w.frame_dt
@@ -524,6 +528,7 @@ def main() -> int:
         # the same state solved from a zero start, one substep, untimed
         zs = settings.copy()
         zs.warm_start = CS_WARM_ZERO
+        zs.rbgs_mg_cycles = 0
         sim0 = cs.MpmSim3D(ctx, zs, w.scene)
         sim0.load(sim.save())
         sim0.step_frame(dt)

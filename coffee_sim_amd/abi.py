@@ -138,6 +138,7 @@ class Settings(C.Structure):
         ("has_bed_field", C.c_uint32),
         ("warm_fallback_residual", C.c_float),
         ("rbgs_rescale_after", C.c_uint32),
+        ("rbgs_mg_cycles", C.c_uint32),
     ]
 
     @property
@@ -220,6 +221,8 @@ class KernelTiming(C.Structure):
         ("rbgs_iterations", C.c_uint64),
         ("rbgs_cell_updates", C.c_uint64),
         ("rbgs_cells_swept", C.c_uint64),
+        ("mg_us", C.c_double),
+        ("mg_starts", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

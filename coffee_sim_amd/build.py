@@ -22,8 +22,8 @@ ARCH = os.environ.get("CS_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["cs_scene.cpp", "cs_bedinit.cpp", "cs_workload.cpp", "cs_sim.cpp", "cs_unet.cpp", "cs_batch.cpp"]
 DEVICE_SOURCES = ["cs_kernels.hip", "cs_solver.hip", "cs_rbgs_persist.hip", "cs_p2g.hip", "cs_unet_kernels.hip",
-                  "cs_noalloc.hip"]
-HEADERS = ["cs_common.h", "cs_bed.h", "cs_device.h", "cs_host.h", "cs_kernels.h", "cs_persist.h", "cs_solver.h"]
+                  "cs_noalloc.hip", "cs_mg.hip"]
+HEADERS = ["cs_common.h", "cs_bed.h", "cs_device.h", "cs_host.h", "cs_kernels.h", "cs_mg.h", "cs_persist.h", "cs_solver.h"]
 
 HOST_FLAGS = [
     "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",

@@ -1,0 +1,79 @@
+// cs_mg.h — the multigrid start of the pressure solve (rbgs_mg_cycles).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "cs_common.h"
+#include "cs_solver.h"
+
+#define CS_MG_NU 2u              // red-black smoothing passes before and after each coarse correction
+#define CS_MG_GAMMA 2u           // coarse corrections per visit (W-cycle)
+#define CS_MG_SCALE 1.5f         // weight of a prolonged correction
+#define CS_MG_COARSE 10u         // passes on the coarsest level
+#define CS_MG_COARSEST_SIDE 8u   // levels are added until no side exceeds this
+#define CS_MG_MAX_LEVELS 12
+#define CS_MG_WG_CELLS 4096u     // levels this small run inside one workgroup
+#define CS_MG_WG_THREADS 1024
+#define CS_MG_MAX_OPS 4096u
+
+// One level: a dense X*Y*Z grid of cells with the Galerkin operator (diagonal
+// D, coupling to the +x/+y/+z neighbour), the right-hand side R and the
+// correction E.  Level 0 is the fine grid, restricted to the free cells.
+struct cs_mg_level {
+  uint32_t X, Y, Z;
+  float* D;
+  float* Wx;
+  float* Wy;
+  float* Wz;
+  float* R;
+  float* E;
+};
+
+// An operation of the single-workgroup interpreter: kind, the level it acts on.
+enum : uint8_t { CS_MG_OP_SMOOTH0 = 0, CS_MG_OP_SMOOTH1 = 1, CS_MG_OP_RESTRICT = 2, CS_MG_OP_ZERO = 3, CS_MG_OP_PROLONG = 4 };
+struct cs_mg_op {
+  uint8_t kind;
+  uint8_t level;  // RESTRICT: from level to level+1; ZERO: E of level; PROLONG: level+1 into level
+};
+
+// The level dimensions the start uses on a grid of the given size.
+std::vector<cs_mg_level> cs_mg_shape(uint32_t gx, uint32_t gy, uint32_t gz);
+// The operations of one W-cycle visit of `level` over levels [level, nlevels).
+void cs_mg_visit_ops(int level, int nlevels, std::vector<cs_mg_op>* out);
+
+// Fine level (over the active bricks): operator, residual b - A p and zero
+// correction of the free cells; smoothing, restriction to / prolongation from
+// level 1; and the final p <- clamp(p + e) in fixed point.  Every kernel returns
+// at once when *skip is set.
+hipError_t cs_mg_fine_setup(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                            const float* wc, const float* rhs, const int32_t* p, cs_mg_level L0, const uint32_t* skip);
+hipError_t cs_mg_fine_smooth(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                             cs_mg_level L0, uint32_t colour, const uint32_t* skip);
+hipError_t cs_mg_fine_restrict(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                               cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
+hipError_t cs_mg_fine_build(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                            cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
+hipError_t cs_mg_fine_prolong(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                              cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
+hipError_t cs_mg_fine_apply(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                            cs_mg_level L0, int32_t* p, const uint32_t* skip);
+// Dense levels.
+hipError_t cs_mg_smooth(hipStream_t s, cs_mg_level L, uint32_t colour, const uint32_t* skip);
+hipError_t cs_mg_restrict(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32_t* skip);
+hipError_t cs_mg_build(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32_t* skip);
+hipError_t cs_mg_prolong(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32_t* skip);
+hipError_t cs_mg_zero(hipStream_t s, cs_mg_level L, const uint32_t* skip);
+// The whole visit of a small level in one workgroup: `ops` (device) over the
+// levels `levels` (device array of CS_MG_MAX_LEVELS).
+hipError_t cs_mg_wg_visit(hipStream_t s, const cs_mg_level* levels, const cs_mg_op* ops, uint32_t nops,
+                          const uint32_t* skip);
+// After a residual test (mode 0, into st->initial_residual): flags[0] = skip
+// the next cycle, flags[1] = the start before any cycle was within the
+// tolerance (`first` sets it).  Every MG kernel reads flags[0].
+hipError_t cs_mg_gate(hipStream_t s, const cs_solve_state* st, double tol, uint32_t* flags, bool first);
+// Clears st->done unless flags[1]: after a multigrid start the iteration runs
+// at least one chunk before its first test.
+hipError_t cs_mg_finish(hipStream_t s, cs_solve_state* st, const uint32_t* flags);

@@ -36,6 +36,8 @@ struct cs_persist_args {
   double tol;
   uint64_t spin_limit;       // 100 MHz ticks a wait may last before the solve gives up
   uint32_t poll_delay;       // s_sleep(1) steps (64 clocks each) before a halo word's first poll
+  const uint32_t* min_gate;  // when non-null and *min_gate == 0 (a multigrid start ran), the
+  uint32_t min_iters;        //   tolerance stops the solve only from min_iters iterations on
 };
 
 // Clears the slot map, check flags, XCD table and failure state, then maps each

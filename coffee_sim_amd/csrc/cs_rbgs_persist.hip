@@ -247,6 +247,7 @@ __global__ void __launch_bounds__(NT, 4) k_rbgs_persist(cs_persist_args a) {
   uint32_t epoch = 0;
   uint32_t checks = 0;
   const uint32_t limit = a.fixed_pairs ? a.fixed_pairs : a.cap;
+  const uint32_t minit = (a.min_gate && *a.min_gate == 0u) ? a.min_iters : 0u;
   bool failed = false;
 
   // Residual of this brick's rows (residual_rows: fluid, non-empty, not held
@@ -351,7 +352,7 @@ __global__ void __launch_bounds__(NT, 4) k_rbgs_persist(cs_persist_args a) {
           b0 += s_red[2 * w + 1];
         }
         const double r = rel_residual(r0, b0);
-        const bool stop = a.fixed_pairs ? iters >= a.fixed_pairs : (r <= a.tol || iters >= a.cap);
+        const bool stop = a.fixed_pairs ? iters >= a.fixed_pairs : ((r <= a.tol && iters >= minit) || iters >= a.cap);
         s_verdict = stop ? 1u : 0u;
         if (checks == 0) a.st->initial_residual = r;
         a.st->residual = r;

@@ -161,6 +161,7 @@ int settings_preset(int preset, cs_settings* s) {
   s->has_bed_field = 0u;
   s->warm_fallback_residual = 1.0f;
   s->rbgs_rescale_after = 0u;
+  s->rbgs_mg_cycles = 0u;
   if (preset == CS_PRESET_FREE_STREAM) {
     sp.origin[0] = 0.0f;
     sp.origin[1] = 6.8f;
@@ -198,6 +199,11 @@ int validate_settings(const cs_settings& s, std::string* why) {
       return fail("rbgs_rescale_after must be a multiple of 4 and of rbgs_check_every");
     if ((uint64_t)s.rbgs_rescale_after + every > s.rbgs_max_iters)
       return fail("rbgs_rescale_after leaves fewer than rbgs_check_every iterations under rbgs_max_iters");
+  }
+  if (s.rbgs_mg_cycles) {
+    if (s.rbgs_mode != CS_RBGS_TOLERANCE) return fail("rbgs_mg_cycles needs the residual-driven rbgs_mode");
+    if (s.rbgs_rescale_after) return fail("rbgs_mg_cycles and rbgs_rescale_after exclude each other");
+    if (s.rbgs_mg_cycles > 64u) return fail("rbgs_mg_cycles must be at most 64");
   }
   if (s.emitter_mode > CS_EMIT_FIXED_COUNT) return fail("unknown emitter_mode");
   return CS_OK;

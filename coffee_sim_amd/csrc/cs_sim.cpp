@@ -64,7 +64,8 @@ void free_sim(cs_sim* m) {
                  m->tile_start, m->p2g_key,    m->p2g_sorted, m->render,     m->bed_part,
                  m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt,
                  m->bed_snap,   m->alias4,
-                 m->persist_fb, m->persist_pflags, m->persist_partials, m->persist_slot, m->persist_xcc};
+                 m->persist_fb, m->persist_pflags, m->persist_partials, m->persist_slot, m->persist_xcc,
+                 m->mg_buf,     m->mg_flags,   m->mg_ops};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -79,6 +80,8 @@ void free_sim(cs_sim* m) {
   for (auto& e : m->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : m->kev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : m->mg_ev)
     if (e) (void)hipEventDestroy(e);
   delete m;
 }
@@ -141,6 +144,7 @@ int predict_warm_start(cs_sim* m, const cs_params& u) {
 // shape), kept as a cross-check of the brick solve.
 int solve_dense(cs_sim* m, const cs_params& u) {
   if (m->s.rbgs_rescale_after) return fail(CS_ERR_INVALID, "rbgs_rescale_after needs the brick solver");
+  if (m->s.rbgs_mg_cycles) return fail(CS_ERR_INVALID, "rbgs_mg_cycles needs the brick solver");
   hipStream_t st = m->ctx->stream;
   cs_solve_stats& S = m->stats;
   memset(&S, 0, sizeof(S));
@@ -317,6 +321,134 @@ int capture_batch(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ever
   return CS_OK;
 }
 
+// ---- multigrid start (cs_mg.hip) ---------------------------------------------------------
+int ensure_mg(cs_sim* m) {
+  if (m->mg_buf) return CS_OK;
+  std::vector<cs_mg_level> lv = cs_mg_shape(m->s.grid_dims[0], m->s.grid_dims[1], m->s.grid_dims[2]);
+  size_t total = 0;
+  for (const auto& l : lv) total += 6 * (size_t)l.X * l.Y * l.Z;
+  CS_HIP(dalloc(&m->mg_buf, total));
+  CS_HIP(dalloc(&m->mg_flags, 2));
+  size_t off = 0;
+  for (auto& l : lv) {
+    const size_t n = (size_t)l.X * l.Y * l.Z;
+    float* b = m->mg_buf + off;
+    l.D = b;
+    l.Wx = b + n;
+    l.Wy = b + 2 * n;
+    l.Wz = b + 3 * n;
+    l.R = b + 4 * n;
+    l.E = b + 5 * n;
+    off += 6 * n;
+  }
+  m->mg_floats = total;
+  const int nl = (int)lv.size();
+  // the first coarse level small enough for one workgroup; its visits run whole
+  m->mg_wg_first = nl;
+  for (int l = 1; l < nl; ++l)
+    if ((uint64_t)lv[l].X * lv[l].Y * lv[l].Z <= CS_MG_WG_CELLS) {
+      m->mg_wg_first = l;
+      break;
+    }
+  if (m->mg_wg_first < nl) {
+    std::vector<cs_mg_op> ops;
+    cs_mg_visit_ops(m->mg_wg_first, nl, &ops);
+    if (ops.size() > CS_MG_MAX_OPS) return fail(CS_ERR_STATE, "multigrid: the coarse visit has too many operations");
+    CS_HIP(dalloc(&m->mg_ops, ops.size()));
+    CS_HIP(hipMemcpy(m->mg_ops, ops.data(), ops.size() * sizeof(cs_mg_op), hipMemcpyHostToDevice));
+    m->mg_nops = (uint32_t)ops.size();
+  }
+  for (auto& e : m->mg_ev) CS_HIP(hipEventCreate(&e));
+  m->mg_levels = lv;
+  return CS_OK;
+}
+
+// One W-cycle visit of level l; the levels from mg_wg_first down run in one
+// workgroup, the finer ones as launches (level 0 over the active bricks).
+int mg_visit(cs_sim* m, const cs_params& u, uint32_t nactive, int l) {
+  hipStream_t st = m->ctx->stream;
+  const int nl = (int)m->mg_levels.size();
+  const uint32_t* skip = m->mg_flags;
+  if (l >= m->mg_wg_first) {
+    cs_mg_level lv[CS_MG_MAX_LEVELS] = {};
+    for (int k = 0; k < nl; ++k) lv[k] = m->mg_levels[k];
+    CS_HIP(cs_mg_wg_visit(st, lv, m->mg_ops, m->mg_nops, skip));
+    return CS_OK;
+  }
+  const cs_mg_level& L = m->mg_levels[l];
+  auto passes = [&](uint32_t n) -> int {
+    for (uint32_t k = 0; k < n; ++k)
+      for (uint32_t c = 0; c < 2u; ++c) {
+        if (l == 0) CS_HIP(cs_mg_fine_smooth(st, u, nactive, m->brick_list, L, c, skip));
+        else CS_HIP(cs_mg_smooth(st, L, c, skip));
+      }
+    return CS_OK;
+  };
+  if (l == nl - 1) return passes(CS_MG_COARSE);
+  int rc = passes(CS_MG_NU);
+  if (rc) return rc;
+  const cs_mg_level& C = m->mg_levels[l + 1];
+  for (uint32_t g = 0; g < CS_MG_GAMMA; ++g) {
+    if (l == 0) CS_HIP(cs_mg_fine_restrict(st, u, nactive, m->brick_list, L, C, skip));
+    else CS_HIP(cs_mg_restrict(st, L, C, skip));
+    CS_HIP(cs_mg_zero(st, C, skip));
+    rc = mg_visit(m, u, nactive, l + 1);
+    if (rc) return rc;
+    if (l == 0) CS_HIP(cs_mg_fine_prolong(st, u, nactive, m->brick_list, L, C, skip));
+    else CS_HIP(cs_mg_prolong(st, L, C, skip));
+    rc = passes(CS_MG_NU);
+    if (rc) return rc;
+  }
+  return CS_OK;
+}
+
+// rbgs_mg_cycles W-cycles on the current start in m->pres, skipped on the
+// device when the start is already within the tolerance.
+int mg_start(cs_sim* m, const cs_params& u, uint32_t nactive) {
+  int rc = ensure_mg(m);
+  if (rc) return rc;
+  hipStream_t st = m->ctx->stream;
+  const double tol = (double)m->s.rbgs_tolerance;
+  if (m->timing) CS_HIP(hipEventRecord(m->mg_ev[0], st));
+  CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+                                   m->dstate, 0u, 0u, 0u, tol));
+  CS_HIP(cs_mg_gate(st, m->dstate, tol, m->mg_flags, true));
+  // kernels touch only the active bricks' fine cells: the fine correction and
+  // every coarse level start from zero everywhere else
+  const cs_mg_level& L0 = m->mg_levels[0];
+  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(L0.E), m->nc, 0));
+  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->mg_buf + 6 * m->nc), m->mg_floats - 6 * m->nc, 0));
+  const int nl = (int)m->mg_levels.size();
+  const uint32_t* skip = m->mg_flags;
+  for (uint32_t c = 0; c < m->s.rbgs_mg_cycles; ++c) {
+    CS_HIP(cs_mg_fine_setup(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, L0, skip));
+    if (nl > 1) CS_HIP(cs_mg_fine_build(st, u, nactive, m->brick_list, L0, m->mg_levels[1], skip));
+    for (int l = 1; l + 1 < nl; ++l) CS_HIP(cs_mg_build(st, m->mg_levels[l], m->mg_levels[l + 1], skip));
+    rc = mg_visit(m, u, nactive, 0);
+    if (rc) return rc;
+    CS_HIP(cs_mg_fine_apply(st, u, nactive, m->brick_list, L0, m->pres, skip));
+    // the cycle's residual test: later cycles return at once within the tolerance
+    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+                                     m->dstate, 0u, 0u, 0u, tol));
+    CS_HIP(cs_mg_gate(st, m->dstate, tol, m->mg_flags, false));
+  }
+  CS_HIP(cs_mg_finish(st, m->dstate, m->mg_flags));
+  if (m->timing) {
+    CS_HIP(hipEventRecord(m->mg_ev[1], st));
+    m->mg_timed = true;
+  }
+  return CS_OK;
+}
+void mg_harvest(cs_sim* m) {
+  if (!m->mg_timed) return;
+  float ms = 0.0f;
+  if (hipEventSynchronize(m->mg_ev[1]) == hipSuccess && hipEventElapsedTime(&ms, m->mg_ev[0], m->mg_ev[1]) == hipSuccess) {
+    m->kt.mg_us += (double)ms * 1000.0;
+    m->kt.mg_starts += 1;
+  }
+  m->mg_timed = false;
+}
+
 // The whole solve as one launch of co-resident workgroups (cs_rbgs_persist.hip):
 // same updates, same residual tests, same stopping iteration as the
 // launch-per-chunk solve; the pressure is solved in place in m->pres.
@@ -335,6 +467,10 @@ int solve_persistent(cs_sim* m, const cs_params& u, uint32_t nactive, bool exter
       CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
       S.fell_back = 1;
     }
+  }
+  if (!fixed && m->s.rbgs_mg_cycles) {
+    int rc = mg_start(m, u, nactive);
+    if (rc) return rc;
   }
   const uint32_t span = cs_persist_epochs(fixed ? m->s.pressure_rbgs_pairs : m->s.rbgs_max_iters);
   if ((uint64_t)m->persist_epoch + span >= 0x80000000ull) {
@@ -365,6 +501,9 @@ int solve_persistent(cs_sim* m, const cs_params& u, uint32_t nactive, bool exter
   a.tol = tol;
   a.spin_limit = 200000000ull;
   a.poll_delay = m->persist_poll_delay;  // 2 s of the 100 MHz clock: far beyond any legitimate wait
+  // after a multigrid start the first residual test comes after one check interval
+  a.min_gate = (!fixed && m->s.rbgs_mg_cycles) ? m->mg_flags + 1 : nullptr;
+  a.min_iters = a.every;
   if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[0], st));
   CS_HIP(cs_persist_launch(st, a, cs_persist_groups(nactive), true));
   if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[1], st));
@@ -420,6 +559,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
       (!fixed || m->s.pressure_rbgs_pairs > 0)) {
     int rc = solve_persistent(m, u, nactive, external);
     if (rc) return rc;
+    mg_harvest(m);
     if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_copy_i32(st, m->pres, m->prev_p, m->nc));
     return CS_OK;
   }
@@ -488,6 +628,21 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     bool stop = false;
     bool pending = false;  // the last enqueued chunk still awaits its residual test
     FusedCheck prev{0u, 0u, tol};
+    if (m->s.rbgs_mg_cycles) {
+      // the multigrid start leaves the state's initial residual and `done` as
+      // the oracle's loop sees them; its first chunk runs before any test (the
+      // sweeps return at once when the start was within the tolerance)
+      int rc = mg_start(m, u, nactive);
+      if (rc) return rc;
+      const uint32_t chunk = std::min(every, cap);
+      uint32_t launches = 0;
+      rc = enqueue_sweeps(m, u, nactive, chunk, &launches, &cur);
+      if (rc) return rc;
+      pending = true;
+      prev.iters = chunk;
+      prev.launches = launches;
+      enq = chunk;
+    }
     while (enq < cap && !stop) {
       const size_t e = (size_t)(batch % pool) * 2;
       if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[e], st));
@@ -566,6 +721,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
       m->kt.rbgs_cells_swept += (uint64_t)ran * nactive * CS_BRICK * CS_BRICK * CS_BRICK;
     }
   }
+  mg_harvest(m);
   if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_copy_i32(st, m->pres, m->prev_p, m->nc));
   return CS_OK;
 }

@@ -8,6 +8,7 @@
 
 #include "cs_ctx.h"
 #include "cs_host.h"
+#include "cs_mg.h"
 #include "cs_persist.h"
 #include "cs_solver.h"
 #include "cs_unet_host.h"
@@ -91,6 +92,16 @@ struct cs_sim {
   double* persist_partials = nullptr;  // [2 * nbricks * 2]
   uint64_t persist_solves = 0;       // solves run by the persistent kernel
   uint32_t persist_poll_delay = 8;   // CS_PERSIST_POLL_DELAY: about 0.25 us, measured best of 0-24
+  // multigrid start (cs_mg.hip), allocated on first use
+  std::vector<cs_mg_level> mg_levels;  // level 0 = the fine grid
+  float* mg_buf = nullptr;             // every level's D, Wx, Wy, Wz, R, E
+  size_t mg_floats = 0;
+  uint32_t* mg_flags = nullptr;        // [0] skip the next cycle, [1] no cycle ran (start within the tolerance)
+  cs_mg_op* mg_ops = nullptr;          // one visit of level mg_wg_first, run by one workgroup
+  uint32_t mg_nops = 0;
+  int mg_wg_first = 0;
+  hipEvent_t mg_ev[2] = {nullptr, nullptr};
+  bool mg_timed = false;               // mg_ev holds an unharvested start
   // tile-binned P2G scratch
   bool p2g_tiled = true;             // CS_P2G=atomic selects the one-thread-per-particle layout
   uint32_t* tile_count = nullptr;    // [nbricks]

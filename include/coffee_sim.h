@@ -166,6 +166,15 @@ typedef struct cs_settings {
                                  * rbgs_check_every iterations left under rbgs_max_iters.
                                  * The iterations before the rescale count toward
                                  * cs_solve_stats.iterations.  No reference counterpart. */
+  uint32_t rbgs_mg_cycles;      /* residual mode, brick solver: before the iteration, this
+                                 * many multigrid W-cycles correct the start (the cells at
+                                 * the pressure clamp held fixed; 2x2x2 Galerkin coarsening
+                                 * down to 8^3; 2 pre- and post-smoothing red-black passes
+                                 * per level; corrections scaled by 1.5; 10 passes on the
+                                 * coarsest grid).  The iteration then runs to the tolerance
+                                 * as before and counts only its own passes.  0 = off.
+                                 * Exclusive with rbgs_rescale_after.  No reference
+                                 * counterpart. */
 } cs_settings;
 
 /* Per-cell porous bed ("field mode").  The reference stamps bed-particle ids into
@@ -419,6 +428,8 @@ typedef struct cs_kernel_timing {
   uint64_t rbgs_iterations;    /* red+black pairs                               */
   uint64_t rbgs_cell_updates;  /* fluid-cell updates (iterations x fluid cells)  */
   uint64_t rbgs_cells_swept;   /* cells the sweep launches visited              */
+  double mg_us;                /* multigrid starts (rbgs_mg_cycles), whole     */
+  uint64_t mg_starts;          /* solves whose start the multigrid corrected    */
 } cs_kernel_timing;
 int cs_sim_get_kernel_timing(cs_sim* sim, cs_kernel_timing* out);
 int cs_sim_reset_kernel_timing(cs_sim* sim);

@@ -111,6 +111,7 @@ int orc_debug_step_until(orc_sim* sim, float sub_dt, int stop_after_pass);
 #define ORC_PASS_POST_DIVERGENCE 10
 #define ORC_PASS_BED_STAMP 11    /* bed_lookup_clear + bed_lookup_scatter and the bed cells' records */
 #define ORC_PASS_BED_DYNAMICS 12 /* bed_dynamics alone */
+#define ORC_PASS_MG 13           /* the multigrid start (rbgs_mg_cycles W-cycles) on the current pressure */
 /* One pass on the current state (uniforms of a substep of sub_dt, nothing
  * cleared): ORC_PASS_RBGS runs one red+black pair, PACKING and VISCOSITY their
  * prepare + apply, PROJECT the projection alone. */
@@ -125,10 +126,17 @@ void orc_debug_set_grid_vel(orc_sim* sim, const float* vel4);
 void orc_debug_set_grid_lanes(orc_sim* sim, const int32_t* lanes6);
 void orc_debug_read_grid_lanes(orc_sim* sim, int32_t* lanes6);
 void orc_debug_clear_metrics(orc_sim* sim);
+/* rbgs_mg_cycles of the oracle's own settings (the ORC_PASS_MG pass and later solves) */
+void orc_set_mg_cycles(orc_sim* sim, uint32_t cycles);
 /* The bed particles' bed_delta lanes, [4][n_bed] i32: water credit, then the
  * reaction impulse x, y, z (fixed point). */
 void orc_debug_read_bed_delta(orc_sim* sim, int32_t* delta4);
 void orc_debug_set_bed_delta(orc_sim* sim, const int32_t* delta4);
+/* The multigrid start's operator on `level` (0: the fine level over the free
+ * cells) for the current pressure and kinds: diagonal and +x/+y/+z couplings,
+ * one value per cell of that level's grid.  Returns the level count, 0 when
+ * `level` does not exist. */
+int orc_debug_mg_operator(orc_sim* sim, float sub_dt, int level, float* D, float* Wx, float* Wy, float* Wz);
 
 #ifdef __cplusplus
 }

@@ -89,7 +89,9 @@ def lib(parallel: bool = False, threads: int | None = None) -> C.CDLL:
             "orc_debug_set_grid_lanes": (None, [C.c_void_p, i32p]),
             "orc_debug_read_grid_lanes": (None, [C.c_void_p, i32p]),
             "orc_debug_clear_metrics": (None, [C.c_void_p]),
+            "orc_set_mg_cycles": (None, [C.c_void_p, C.c_uint32]),
             "orc_debug_read_bed_delta": (None, [C.c_void_p, i32p]),
+            "orc_debug_mg_operator": (C.c_int, [C.c_void_p, C.c_float, C.c_int, f32p, f32p, f32p, f32p]),
             "orc_debug_set_bed_delta": (None, [C.c_void_p, i32p]),
             "orc_set_previous_pressure": (None, [C.c_void_p, i32p]),
         }
@@ -110,7 +112,7 @@ def _i(a):
 
 
 PASS = {"p2g": 1, "grid": 2, "classify": 3, "rbgs": 4, "project": 5, "packing": 6, "viscosity": 7, "g2p": 8,
-        "boundary": 9, "post_divergence": 10, "bed_stamp": 11, "bed_dynamics": 12}
+        "boundary": 9, "post_divergence": 10, "bed_stamp": 11, "bed_dynamics": 12, "mg": 13}
 
 
 def preset(name: str) -> Settings:
@@ -191,6 +193,17 @@ class OracleSim:
         assert p.size == self.nc
         self._L.orc_set_previous_pressure(self._h, _i(p))
 
+    def debug_mg_operator(self, sub_dt: float, level: int):
+        """(D, Wx, Wy, Wz) of multigrid level `level`, each shaped (Z, Y, X), or None."""
+        X, Y, Z = self.dims
+        for _ in range(level):
+            X, Y, Z = (X + 1) // 2, (Y + 1) // 2, (Z + 1) // 2
+        out = [np.zeros(X * Y * Z, np.float32) for _ in range(4)]
+        nl = self._L.orc_debug_mg_operator(self._h, sub_dt, level, *[_f(a) for a in out])
+        if nl == 0:
+            return None
+        return tuple(a.reshape(Z, Y, X) for a in out)
+
     def debug_read_bed_delta(self) -> np.ndarray:
         out = np.zeros((4, self.counts()[1]), np.int32)
         self._L.orc_debug_read_bed_delta(self._h, _i(out))
@@ -200,6 +213,10 @@ class OracleSim:
         a = np.ascontiguousarray(d4, np.int32).reshape(-1)
         assert a.size == 4 * self.counts()[1]
         self._L.orc_debug_set_bed_delta(self._h, _i(a))
+
+    def set_mg_cycles(self, cycles: int) -> None:
+        self._L.orc_set_mg_cycles(self._h, cycles)
+        self.settings.rbgs_mg_cycles = cycles
 
     def debug_clear_metrics(self) -> None:
         self._L.orc_debug_clear_metrics(self._h)

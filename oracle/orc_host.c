@@ -141,6 +141,7 @@ int orc_settings_preset(int preset, cs_settings* s) {
   s->has_bed_field = 0u;
   s->warm_fallback_residual = 1.0f;
   s->rbgs_rescale_after = 0u;
+  s->rbgs_mg_cycles = 0u;
   if (preset == CS_PRESET_FREE_STREAM) {           /* mod.rs:145-152 */
     s->spout.origin[0] = 0.0f; s->spout.origin[1] = 6.8f; s->spout.origin[2] = 0.0f;
     orc_spout_aim_at(&s->spout, 0.0f, -6.8f, 0.0f);

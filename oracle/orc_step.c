@@ -814,6 +814,251 @@ void orc_read_pressure(orc_sim* S, int32_t* p, int32_t* kind) {
   if (kind) memcpy(kind, S->saved_kind, S->nc * sizeof(int32_t));
 }
 
+/* ---- multigrid start (rbgs_mg_cycles; no reference counterpart) --------------
+ * Before the red-black iteration, W-cycles of a truncated aggregation multigrid
+ * correct the start.  The free cells are the fluid cells with a non-empty row
+ * whose pressure is inside the clamp; a cell at the clamp keeps its pressure
+ * (its correction is held at zero, as a Dirichlet cell).  Level 0 is the fine
+ * grid's error equation A e = b - A p over the free cells; level l+1 has one
+ * cell per 2x2x2 block of level l (the grid halved, rounded up) and the
+ * Galerkin operator of piecewise-constant aggregation: a coarse cell's diagonal
+ * is its children's diagonals less twice their internal couplings, a coarse
+ * face weight is the sum of the fine couplings across that face.  Levels are
+ * added until no side exceeds 8.  A visit of level l (W-cycle, gamma 2):
+ * MG_NU red-black smoothing passes, then twice { restrict the residual to
+ * level l+1, zero its correction, visit it, add MG_SCALE times the prolonged
+ * (piecewise-constant) correction, MG_NU passes }; the coarsest level gets
+ * MG_COARSE passes.  Every per-cell sum runs in a fixed order (faces -x, +x,
+ * -y, +y, -z, +z; children dz, dy, dx ascending), so the device reproduces it
+ * bit for bit; a cycle ends with p <- clamp(p + e) stored in fixed point. */
+#define MG_NU 2u
+#define MG_GAMMA 2u
+#define MG_SCALE 1.5f
+#define MG_COARSE 10u
+#define MG_COARSEST_SIDE 8u
+#define MG_MAX_LEVELS 12
+
+typedef struct {
+  uint32_t X, Y, Z;
+  size_t n;
+  float *D, *Wx, *Wy, *Wz, *R, *E;
+} mg_level;
+
+static inline size_t mg_idx(const mg_level* L, uint32_t x, uint32_t y, uint32_t z) {
+  return ((size_t)z * L->Y + y) * L->X + x;
+}
+/* the six neighbour terms of cell (x,y,z) against values v, in face order */
+static float mg_offsum(const mg_level* L, const float* v, uint32_t x, uint32_t y, uint32_t z) {
+  const size_t i = mg_idx(L, x, y, z);
+  float t0 = x > 0 ? L->Wx[i - 1] * v[i - 1] : 0.0f;
+  float t1 = x + 1 < L->X ? L->Wx[i] * v[i + 1] : 0.0f;
+  float t2 = y > 0 ? L->Wy[i - L->X] * v[i - L->X] : 0.0f;
+  float t3 = y + 1 < L->Y ? L->Wy[i] * v[i + L->X] : 0.0f;
+  const size_t sl = (size_t)L->X * L->Y;
+  float t4 = z > 0 ? L->Wz[i - sl] * v[i - sl] : 0.0f;
+  float t5 = z + 1 < L->Z ? L->Wz[i] * v[i + sl] : 0.0f;
+  float s = t0;
+  s += t1;
+  s += t2;
+  s += t3;
+  s += t4;
+  s += t5;
+  return s;
+}
+static void mg_smooth(mg_level* L, uint32_t col) {
+  for (uint32_t z = 0; z < L->Z; ++z)
+    for (uint32_t y = 0; y < L->Y; ++y)
+      for (uint32_t x = 0; x < L->X; ++x) {
+        if (((x + y + z) & 1u) != col) continue;
+        const size_t i = mg_idx(L, x, y, z);
+        if (!(L->D[i] > 0.0f)) continue;
+        L->E[i] = (L->R[i] + mg_offsum(L, L->E, x, y, z)) / L->D[i];
+      }
+}
+/* children of coarse cell (X,Y,Z) in a level of (fx,fy,fz) cells */
+#define MG_CHILDREN(FX, FY, FZ, CX, CY, CZ, BODY)                                              \
+  for (uint32_t dz_ = 0; dz_ < 2u; ++dz_)                                                      \
+    for (uint32_t dy_ = 0; dy_ < 2u; ++dy_)                                                    \
+      for (uint32_t dx_ = 0; dx_ < 2u; ++dx_) {                                                \
+        const uint32_t x = 2u * (CX) + dx_, y = 2u * (CY) + dy_, z = 2u * (CZ) + dz_;         \
+        if (x >= (FX) || y >= (FY) || z >= (FZ)) continue;                                     \
+        BODY                                                                                   \
+      }
+/* residual of level f restricted to level c (children summed in order) */
+static void mg_restrict(const mg_level* f, mg_level* c) {
+  for (uint32_t Z = 0; Z < c->Z; ++Z)
+    for (uint32_t Y = 0; Y < c->Y; ++Y)
+      for (uint32_t X = 0; X < c->X; ++X) {
+        float acc = 0.0f;
+        MG_CHILDREN(f->X, f->Y, f->Z, X, Y, Z, {
+          const size_t i = mg_idx(f, x, y, z);
+          if (f->D[i] > 0.0f) {
+            const float s = mg_offsum(f, f->E, x, y, z);
+            acc += f->R[i] - (f->D[i] * f->E[i] - s);
+          }
+        })
+        c->R[mg_idx(c, X, Y, Z)] = acc;
+      }
+}
+/* Galerkin operator of level c from level f */
+static void mg_build(const mg_level* f, mg_level* c) {
+  for (uint32_t Z = 0; Z < c->Z; ++Z)
+    for (uint32_t Y = 0; Y < c->Y; ++Y)
+      for (uint32_t X = 0; X < c->X; ++X) {
+        float d = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
+        MG_CHILDREN(f->X, f->Y, f->Z, X, Y, Z, { d += f->D[mg_idx(f, x, y, z)]; })
+        MG_CHILDREN(f->X, f->Y, f->Z, X, Y, Z, {
+          const size_t i = mg_idx(f, x, y, z);
+          if (dx_ == 0u && x + 1 < f->X) d -= 2.0f * f->Wx[i];
+          if (dy_ == 0u && y + 1 < f->Y) d -= 2.0f * f->Wy[i];
+          if (dz_ == 0u && z + 1 < f->Z) d -= 2.0f * f->Wz[i];
+          if (dx_ == 1u) wx += f->Wx[i];
+          if (dy_ == 1u) wy += f->Wy[i];
+          if (dz_ == 1u) wz += f->Wz[i];
+        })
+        const size_t I = mg_idx(c, X, Y, Z);
+        c->D[I] = d > 0.0f ? d : 0.0f;
+        c->Wx[I] = wx;
+        c->Wy[I] = wy;
+        c->Wz[I] = wz;
+      }
+}
+static void mg_prolong(mg_level* f, const mg_level* c) {
+  for (uint32_t z = 0; z < f->Z; ++z)
+    for (uint32_t y = 0; y < f->Y; ++y)
+      for (uint32_t x = 0; x < f->X; ++x) {
+        const size_t i = mg_idx(f, x, y, z);
+        if (f->D[i] > 0.0f) f->E[i] = f->E[i] + MG_SCALE * c->E[mg_idx(c, x >> 1, y >> 1, z >> 1)];
+      }
+}
+static void mg_visit(mg_level* L, int l, int nl) {
+  if (l == nl - 1) {
+    for (uint32_t k = 0; k < MG_COARSE; ++k) { mg_smooth(&L[l], 0u); mg_smooth(&L[l], 1u); }
+    return;
+  }
+  for (uint32_t k = 0; k < MG_NU; ++k) { mg_smooth(&L[l], 0u); mg_smooth(&L[l], 1u); }
+  for (uint32_t g = 0; g < MG_GAMMA; ++g) {
+    mg_restrict(&L[l], &L[l + 1]);
+    memset(L[l + 1].E, 0, L[l + 1].n * sizeof(float));
+    mg_visit(L, l + 1, nl);
+    mg_prolong(&L[l], &L[l + 1]);
+    for (uint32_t k = 0; k < MG_NU; ++k) { mg_smooth(&L[l], 0u); mg_smooth(&L[l], 1u); }
+  }
+}
+/* Level 0 over the free cells: D = the row's count, face weights toward free
+ * neighbours, R = b - A p, E = 0.  The pressure sum is pressure_update's. */
+static void mg_fine_level(orc_sim* S, const orc_params* u, mg_level* L0, uint8_t* fluid) {
+  const float lim = u->pressure_clamp;
+  memset(L0->D, 0, S->nc * sizeof(float));
+  memset(L0->Wx, 0, S->nc * sizeof(float));
+  memset(L0->Wy, 0, S->nc * sizeof(float));
+  memset(L0->Wz, 0, S->nc * sizeof(float));
+  memset(L0->R, 0, S->nc * sizeof(float));
+  memset(L0->E, 0, S->nc * sizeof(float));
+  uint8_t* fr = (uint8_t*)calloc(S->nc, 1);
+  float* sums = (float*)calloc(S->nc, sizeof(float));
+  for (int col = 0; col < 2; ++col)
+    for (size_t k = 0; k < g_nrows[col]; ++k) {
+      const sys_row* r = &g_rows[col][k];
+      float t[6];
+      for (int n = 0; n < 6; ++n) t[n] = (r->w[n] > 0.0f && r->nf[n]) ? r->w[n] * pressure_load(S, u, r->nb[n]) : 0.0f;
+      float sum = t[0];
+      sum += t[1];
+      sum += t[2];
+      sum += t[3];
+      sum += t[4];
+      sum += t[5];
+      sums[r->idx] = sum;
+      if (!(r->count > 0.0f)) continue;
+      const float p = pressure_load(S, u, r->idx);
+      if (fabsf(p) < lim) {
+        fr[r->idx] = 1u;
+      } else {
+        /* at the clamp: free when pressure_update would move it back inside */
+        const float v = (sum - r->c) / r->count;
+        fr[r->idx] = (p > 0.0f ? v < lim : v > -lim) ? 1u : 0u;
+      }
+    }
+  for (int col = 0; col < 2; ++col)
+    for (size_t k = 0; k < g_nrows[col]; ++k) {
+      const sys_row* r = &g_rows[col][k];
+      const size_t i = r->idx;
+      fluid[i] = 1u;
+      if (!fr[i]) continue;
+      /* +x, +y, +z couplings to free neighbours (faces 1, 3, 5) */
+      if (r->nf[1] && fr[r->nb[1]]) L0->Wx[i] = r->w[1];
+      if (r->nf[3] && fr[r->nb[3]]) L0->Wy[i] = r->w[3];
+      if (r->nf[5] && fr[r->nb[5]]) L0->Wz[i] = r->w[5];
+      L0->D[i] = r->count;
+      L0->R[i] = (sums[i] - r->c) - r->count * pressure_load(S, u, i);
+    }
+  free(sums);
+  free(fr);
+}
+static int mg_alloc(orc_sim* S, mg_level* L) {
+  int nl = 0;
+  uint32_t X = S->gx, Y = S->gy, Z = S->gz;
+  for (;;) {
+    mg_level* l = &L[nl++];
+    l->X = X; l->Y = Y; l->Z = Z;
+    l->n = (size_t)X * Y * Z;
+    float* buf = (float*)calloc(6 * l->n, sizeof(float));
+    l->D = buf; l->Wx = buf + l->n; l->Wy = buf + 2 * l->n; l->Wz = buf + 3 * l->n;
+    l->R = buf + 4 * l->n; l->E = buf + 5 * l->n;
+    if ((X <= MG_COARSEST_SIDE && Y <= MG_COARSEST_SIDE && Z <= MG_COARSEST_SIDE) || nl == MG_MAX_LEVELS) break;
+    X = (X + 1u) / 2u; Y = (Y + 1u) / 2u; Z = (Z + 1u) / 2u;
+  }
+  return nl;
+}
+static double residual_rows(orc_sim* S, const orc_params* u);
+/* Up to `cycles` W-cycles, each followed by the residual test; returns the last
+ * residual (r when no cycle runs). */
+static double mg_start(orc_sim* S, const orc_params* u, uint32_t cycles, double r) {
+  mg_level L[MG_MAX_LEVELS];
+  const int nl = mg_alloc(S, L);
+  uint8_t* fluid = (uint8_t*)calloc(S->nc, 1);
+  const float lim = u->pressure_clamp;
+  for (uint32_t c = 0; c < cycles && !(r <= (double)S->s.rbgs_tolerance); ++c) {
+    mg_fine_level(S, u, &L[0], fluid);
+    for (int l = 0; l + 1 < nl; ++l) mg_build(&L[l], &L[l + 1]);
+    if (nl > 1) mg_visit(L, 0, nl);
+    else for (uint32_t k = 0; k < MG_COARSE; ++k) { mg_smooth(&L[0], 0u); mg_smooth(&L[0], 1u); }
+    for (size_t i = 0; i < S->nc; ++i) {
+      if (!(L[0].D[i] > 0.0f)) continue;
+      const float pn = pressure_load(S, u, i) + L[0].E[i];
+      LANE(S, 0, i) = o_f2i(o_clamp(pn, -lim, lim) * u->fp_scale);
+    }
+    r = residual_rows(S, u);
+  }
+  free(fluid);
+  for (int l = 0; l < nl; ++l) free(L[l].D);
+  return r;
+}
+/* The multigrid operator of `level` (0 = the fine level over the free cells)
+ * for the current pressure and kinds; returns the level count, or 0 when
+ * `level` is beyond the last. */
+int orc_debug_mg_operator(orc_sim* S, float sub_dt, int level, float* D, float* Wx, float* Wy, float* Wz) {
+  orc_write_uniforms(S, sub_dt, &S->u);
+  const orc_params* u = &S->u;
+  build_rows(S, u);
+  mg_level L[MG_MAX_LEVELS];
+  const int nl = mg_alloc(S, L);
+  uint8_t* fluid = (uint8_t*)calloc(S->nc, 1);
+  mg_fine_level(S, u, &L[0], fluid);
+  for (int l = 0; l + 1 < nl && l < level; ++l) mg_build(&L[l], &L[l + 1]);
+  int ok = level >= 0 && level < nl;
+  if (ok) {
+    const mg_level* l = &L[level];
+    memcpy(D, l->D, l->n * sizeof(float));
+    memcpy(Wx, l->Wx, l->n * sizeof(float));
+    memcpy(Wy, l->Wy, l->n * sizeof(float));
+    memcpy(Wz, l->Wz, l->n * sizeof(float));
+  }
+  free(fluid);
+  for (int l = 0; l < nl; ++l) free(L[l].D);
+  return ok ? nl : 0;
+}
+
 /* ---- pressure solve driver ------------------------------------------------- */
 static void solve_pressure(orc_sim* S, const orc_params* u) {
   cs_solve_stats* st = &S->stats;
@@ -850,8 +1095,15 @@ static void solve_pressure(orc_sim* S, const orc_params* u) {
       st->fell_back = 1;
       r = residual_rows(S, u);
     }
-    st->initial_residual = r;
     uint32_t it = 0, every = S->s.rbgs_check_every ? S->s.rbgs_check_every : 8u;
+    /* the multigrid start; after it the first residual test comes after
+     * `every` iterations, so the clamp's projected updates always run */
+    uint32_t min_it = 0;
+    if (S->s.rbgs_mg_cycles && !(r <= (double)S->s.rbgs_tolerance)) {
+      r = mg_start(S, u, S->s.rbgs_mg_cycles, r);
+      min_it = every;
+    }
+    st->initial_residual = r;
     if (S->s.rbgs_rescale_after && !(r <= (double)S->s.rbgs_tolerance)) {
       /* untested iterations, then the rescale; the next test is at it = rescale_after + every */
       for (uint32_t k = 0; k < S->s.rbgs_rescale_after; ++k) {
@@ -861,7 +1113,7 @@ static void solve_pressure(orc_sim* S, const orc_params* u) {
       it = S->s.rbgs_rescale_after;
       rescale_pressure(S, u);
     }
-    while (!(r <= (double)S->s.rbgs_tolerance) && it < S->s.rbgs_max_iters) {
+    while ((it < min_it || !(r <= (double)S->s.rbgs_tolerance)) && it < S->s.rbgs_max_iters) {
       k_rbgs_half(S, u, 0u);
       k_rbgs_half(S, u, 1u);
       ++it;
@@ -1610,6 +1862,7 @@ int orc_debug_run_pass(orc_sim* S, float sub_dt, int pass) {
     case ORC_PASS_POST_DIVERGENCE: k_post_divergence(S, u); break;
     case ORC_PASS_BED_STAMP: if (S->bed_n) bed_restamp(S, u); break;
     case ORC_PASS_BED_DYNAMICS: if (S->bed_n) bed_dynamics(S, u); break;
+    case ORC_PASS_MG: build_rows(S, u); (void)mg_start(S, u, S->s.rbgs_mg_cycles, INFINITY); break;
     default: return CS_ERR_INVALID;
   }
   return CS_OK;
@@ -1619,6 +1872,7 @@ void orc_debug_set_grid_vel(orc_sim* S, const float* vel4) { memcpy(S->grid_vel,
 void orc_debug_set_grid_lanes(orc_sim* S, const int32_t* lanes6) { memcpy(S->grid, lanes6, S->nc * 6 * sizeof(int32_t)); }
 void orc_debug_read_grid_lanes(orc_sim* S, int32_t* lanes6) { memcpy(lanes6, S->grid, S->nc * 6 * sizeof(int32_t)); }
 void orc_debug_clear_metrics(orc_sim* S) { memset(S->metrics, 0, sizeof(S->metrics)); }
+void orc_set_mg_cycles(orc_sim* S, uint32_t cycles) { S->s.rbgs_mg_cycles = cycles; }
 void orc_debug_read_bed_delta(orc_sim* S, int32_t* d) {
   if (S->bed_n) memcpy(d, S->bed_dw, (size_t)S->bed_n * 4 * sizeof(int32_t));
 }
