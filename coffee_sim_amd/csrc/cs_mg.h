@@ -24,6 +24,7 @@
 // correction E.  Level 0 is the fine grid, restricted to the free cells.
 struct cs_mg_level {
   uint32_t X, Y, Z;
+  uint32_t mX, mY;  // floor((2^32 - 1) / X) and / Y: division by X and Y as a multiply-high
   float* D;
   float* Wx;
   float* Wy;
@@ -33,10 +34,13 @@ struct cs_mg_level {
 };
 
 // An operation of the single-workgroup interpreter: kind, the level it acts on.
-enum : uint8_t { CS_MG_OP_SMOOTH0 = 0, CS_MG_OP_SMOOTH1 = 1, CS_MG_OP_RESTRICT = 2, CS_MG_OP_ZERO = 3, CS_MG_OP_PROLONG = 4 };
+// SMOOTH0 / SMOOTH1: a red / black pass of `level`; RESTRICT: its residual into
+// level+1 (whose correction is zeroed); SMOOTH0P: level+1's correction
+// prolonged into `level` together with its red pass.
+enum : uint8_t { CS_MG_OP_SMOOTH0 = 0, CS_MG_OP_SMOOTH1 = 1, CS_MG_OP_RESTRICT = 2, CS_MG_OP_SMOOTH0P = 3 };
 struct cs_mg_op {
   uint8_t kind;
-  uint8_t level;  // RESTRICT: from level to level+1; ZERO: E of level; PROLONG: level+1 into level
+  uint8_t level;
 };
 
 // The level dimensions the start uses on a grid of the given size.
@@ -56,20 +60,22 @@ hipError_t cs_mg_fine_restrict(hipStream_t s, const cs_params& u, uint32_t nacti
                                cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
 hipError_t cs_mg_fine_build(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
                             cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
-hipError_t cs_mg_fine_prolong(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
-                              cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
+hipError_t cs_mg_fine_smooth_prolonged(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                                       cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
 hipError_t cs_mg_fine_apply(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
                             cs_mg_level L0, int32_t* p, const uint32_t* skip);
 // Dense levels.
 hipError_t cs_mg_smooth(hipStream_t s, cs_mg_level L, uint32_t colour, const uint32_t* skip);
 hipError_t cs_mg_restrict(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32_t* skip);
 hipError_t cs_mg_build(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32_t* skip);
-hipError_t cs_mg_prolong(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32_t* skip);
-hipError_t cs_mg_zero(hipStream_t s, cs_mg_level L, const uint32_t* skip);
-// The whole visit of a small level in one workgroup: `ops` (device) over the
-// levels `levels` (device array of CS_MG_MAX_LEVELS).
-hipError_t cs_mg_wg_visit(hipStream_t s, const cs_mg_level* levels, const cs_mg_op* ops, uint32_t nops,
-                          const uint32_t* skip);
+// The coarse correction prolonged into f together with f's red pass.
+hipError_t cs_mg_smooth_prolonged(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32_t* skip);
+// The whole visit of level `first` in one workgroup, levels first..nl-1 staged
+// in LDS: `ops` (device) as cs_mg_visit_ops lists them.
+size_t cs_mg_wg_lds_bytes(const cs_mg_level* levels, int first, int nl);
+hipError_t cs_mg_wg_prepare(size_t lds_bytes);
+hipError_t cs_mg_wg_visit(hipStream_t s, const cs_mg_level* levels, int first, int nl, const cs_mg_op* ops,
+                          uint32_t nops, const uint32_t* skip);
 // After a residual test (mode 0, into st->initial_residual): flags[0] = skip
 // the next cycle, flags[1] = the start before any cycle was within the
 // tolerance (`first` sets it).  Every MG kernel reads flags[0].

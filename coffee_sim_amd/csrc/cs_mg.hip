@@ -37,20 +37,65 @@ namespace {
 
 constexpr int BR = CS_BRICK;
 
-__device__ __forceinline__ size_t lidx(const cs_mg_level& L, uint32_t x, uint32_t y, uint32_t z) {
+// A level held in LDS: the same fields as cs_mg_level with LDS pointers, so
+// every access compiles to a ds load or store rather than a flat one.
+typedef __attribute__((address_space(3))) float sfloat;
+struct slevel {
+  uint32_t X, Y, Z, mX, mY;
+  uint32_t HX, mHX;  // cells of one colour per row at most, ceil(X / 2), and its division magic
+  sfloat* D;
+  sfloat* Wx;
+  sfloat* Wy;
+  sfloat* Wz;
+  sfloat* R;
+  sfloat* E;
+};
+
+// The per-cell steps below take either kind of level (cs_mg_level or slevel).
+template <class LV>
+__device__ __forceinline__ size_t lidx(const LV& L, uint32_t x, uint32_t y, uint32_t z) {
   return ((size_t)z * L.Y + y) * L.X + x;
 }
 
-// The six neighbour terms of a cell against values v, summed in face order.
-__device__ __forceinline__ float offsum(const cs_mg_level& L, const float* v, uint32_t x, uint32_t y, uint32_t z,
-                                        size_t i) {
-  const size_t sy = L.X, sz = (size_t)L.X * L.Y;
-  const float t0 = x > 0u ? L.Wx[i - 1] * v[i - 1] : 0.0f;
-  const float t1 = x + 1u < L.X ? L.Wx[i] * v[i + 1] : 0.0f;
-  const float t2 = y > 0u ? L.Wy[i - sy] * v[i - sy] : 0.0f;
-  const float t3 = y + 1u < L.Y ? L.Wy[i] * v[i + sy] : 0.0f;
-  const float t4 = z > 0u ? L.Wz[i - sz] * v[i - sz] : 0.0f;
-  const float t5 = z + 1u < L.Z ? L.Wz[i] * v[i + sz] : 0.0f;
+// The six neighbour terms of a cell against the corrections, summed in face order.
+// Neighbour indices clamped onto the grid (the cell itself past an edge): every
+// load is issued unconditionally and only the product is selected, so a cell's
+// loads are in flight together instead of one branch-and-wait at a time.
+struct nbr6 {
+  size_t xm, xp, ym, yp, zm, zp;
+  bool bxm, bxp, bym, byp, bzm, bzp;
+};
+__device__ __forceinline__ nbr6 neighbours(uint32_t X, uint32_t Y, uint32_t Z, uint32_t x, uint32_t y, uint32_t z,
+                                           size_t i) {
+  const size_t sy = X, sz = (size_t)X * Y;
+  nbr6 n;
+  n.bxm = x > 0u;
+  n.bxp = x + 1u < X;
+  n.bym = y > 0u;
+  n.byp = y + 1u < Y;
+  n.bzm = z > 0u;
+  n.bzp = z + 1u < Z;
+  n.xm = n.bxm ? i - 1 : i;
+  n.xp = n.bxp ? i + 1 : i;
+  n.ym = n.bym ? i - sy : i;
+  n.yp = n.byp ? i + sy : i;
+  n.zm = n.bzm ? i - sz : i;
+  n.zp = n.bzp ? i + sz : i;
+  return n;
+}
+
+template <class LV>
+__device__ __forceinline__ float offsum(const LV& L, uint32_t x, uint32_t y, uint32_t z, size_t i) {
+  const auto v = L.E;
+  const nbr6 n = neighbours(L.X, L.Y, L.Z, x, y, z, i);
+  const float w0 = L.Wx[n.xm], w1 = L.Wx[i], w2 = L.Wy[n.ym], w3 = L.Wy[i], w4 = L.Wz[n.zm], w5 = L.Wz[i];
+  const float e0 = v[n.xm], e1 = v[n.xp], e2 = v[n.ym], e3 = v[n.yp], e4 = v[n.zm], e5 = v[n.zp];
+  const float t0 = n.bxm ? w0 * e0 : 0.0f;
+  const float t1 = n.bxp ? w1 * e1 : 0.0f;
+  const float t2 = n.bym ? w2 * e2 : 0.0f;
+  const float t3 = n.byp ? w3 * e3 : 0.0f;
+  const float t4 = n.bzm ? w4 * e4 : 0.0f;
+  const float t5 = n.bzp ? w5 * e5 : 0.0f;
   float s = t0;
   s += t1;
   s += t2;
@@ -60,18 +105,18 @@ __device__ __forceinline__ float offsum(const cs_mg_level& L, const float* v, ui
   return s;
 }
 
-__device__ __forceinline__ void smooth_cell(const cs_mg_level& L, uint32_t x, uint32_t y, uint32_t z,
-                                            uint32_t colour) {
+template <class LV>
+__device__ __forceinline__ void smooth_cell(const LV& L, uint32_t x, uint32_t y, uint32_t z, uint32_t colour) {
   if (((x + y + z) & 1u) != colour) return;
   const size_t i = lidx(L, x, y, z);
   const float d = L.D[i];
   if (!(d > 0.0f)) return;
-  L.E[i] = (L.R[i] + offsum(L, L.E, x, y, z, i)) / d;
+  L.E[i] = (L.R[i] + offsum(L, x, y, z, i)) / d;
 }
 
 // Residual of level f summed over the children of coarse cell (X, Y, Z).
-__device__ __forceinline__ void restrict_cell(const cs_mg_level& f, const cs_mg_level& c, uint32_t X, uint32_t Y,
-                                              uint32_t Z) {
+template <class F, class C>
+__device__ __forceinline__ void restrict_cell(const F& f, const C& c, uint32_t X, uint32_t Y, uint32_t Z) {
   float acc = 0.0f;
   for (uint32_t dz = 0; dz < 2u; ++dz)
     for (uint32_t dy = 0; dy < 2u; ++dy)
@@ -80,11 +125,13 @@ __device__ __forceinline__ void restrict_cell(const cs_mg_level& f, const cs_mg_
         if (x >= f.X || y >= f.Y || z >= f.Z) continue;
         const size_t i = lidx(f, x, y, z);
         if (f.D[i] > 0.0f) {
-          const float s = offsum(f, f.E, x, y, z, i);
+          const float s = offsum(f, x, y, z, i);
           acc += f.R[i] - (f.D[i] * f.E[i] - s);
         }
       }
-  c.R[lidx(c, X, Y, Z)] = acc;
+  const size_t I = lidx(c, X, Y, Z);
+  c.R[I] = acc;
+  c.E[I] = 0.0f;  // the coarse visit starts from a zero correction
 }
 
 // Galerkin operator of coarse cell (X, Y, Z) from level f.
@@ -118,17 +165,64 @@ __device__ __forceinline__ void build_cell(const cs_mg_level& f, const cs_mg_lev
   c.Wz[I] = wz;
 }
 
-__device__ __forceinline__ void prolong_cell(const cs_mg_level& f, const cs_mg_level& c, uint32_t x, uint32_t y,
-                                             uint32_t z) {
+// The correction of fine cell (x, y, z) after adding CS_MG_SCALE times the
+// coarse correction (cells outside the operator keep theirs).
+template <class F, class C>
+__device__ __forceinline__ float prolonged(const F& f, const C& c, uint32_t x, uint32_t y, uint32_t z, size_t i) {
+  const float e = f.E[i];
+  const float d = f.D[i];
+  const float ec = c.E[lidx(c, x >> 1, y >> 1, z >> 1)];
+  return d > 0.0f ? e + CS_MG_SCALE * ec : e;
+}
+// Prolongation and the red pass in one: a red cell's update reads its
+// neighbours' prolonged corrections.  The black cells' own prolonged values are
+// read by nothing else before the black pass overwrites them, so the result
+// equals prolonging every cell and then running the red pass.
+template <class F, class C>
+__device__ __forceinline__ void smooth_prolonged_cell(const F& f, const C& c, uint32_t x, uint32_t y, uint32_t z) {
+  if (((x + y + z) & 1u) != 0u) return;
   const size_t i = lidx(f, x, y, z);
-  if (f.D[i] > 0.0f) f.E[i] = f.E[i] + CS_MG_SCALE * c.E[lidx(c, x >> 1, y >> 1, z >> 1)];
+  const float d = f.D[i];
+  if (!(d > 0.0f)) return;
+  const nbr6 n = neighbours(f.X, f.Y, f.Z, x, y, z, i);
+  const float w0 = f.Wx[n.xm], w1 = f.Wx[i], w2 = f.Wy[n.ym], w3 = f.Wy[i], w4 = f.Wz[n.zm], w5 = f.Wz[i];
+  // neighbour coordinates clamped like the indices (a clamped term is discarded)
+  const float e0 = prolonged(f, c, n.bxm ? x - 1u : x, y, z, n.xm);
+  const float e1 = prolonged(f, c, n.bxp ? x + 1u : x, y, z, n.xp);
+  const float e2 = prolonged(f, c, x, n.bym ? y - 1u : y, z, n.ym);
+  const float e3 = prolonged(f, c, x, n.byp ? y + 1u : y, z, n.yp);
+  const float e4 = prolonged(f, c, x, y, n.bzm ? z - 1u : z, n.zm);
+  const float e5 = prolonged(f, c, x, y, n.bzp ? z + 1u : z, n.zp);
+  const float t0 = n.bxm ? w0 * e0 : 0.0f;
+  const float t1 = n.bxp ? w1 * e1 : 0.0f;
+  const float t2 = n.bym ? w2 * e2 : 0.0f;
+  const float t3 = n.byp ? w3 * e3 : 0.0f;
+  const float t4 = n.bzm ? w4 * e4 : 0.0f;
+  const float t5 = n.bzp ? w5 * e5 : 0.0f;
+  float s = t0;
+  s += t1;
+  s += t2;
+  s += t3;
+  s += t4;
+  s += t5;
+  f.E[i] = (f.R[i] + s) / d;
 }
 
-__device__ __forceinline__ void level_cell(const cs_mg_level& L, uint32_t n, uint32_t& x, uint32_t& y, uint32_t& z) {
-  x = n % L.X;
-  const uint32_t r = n / L.X;
-  y = r % L.Y;
-  z = r / L.Y;
+// k / d with m = floor((2^32 - 1) / d): the multiply-high is floor(k / d) or one
+// less for k < 2^31, and one correction step settles it; r = k mod d.
+__device__ __forceinline__ uint32_t div_magic(uint32_t k, uint32_t d, uint32_t m, uint32_t& r) {
+  uint32_t q = __umulhi(k, m);
+  r = k - q * d;
+  if (r >= d) {
+    ++q;
+    r -= d;
+  }
+  return q;
+}
+template <class LV>
+__device__ __forceinline__ void level_cell(const LV& L, uint32_t n, uint32_t& x, uint32_t& y, uint32_t& z) {
+  const uint32_t r = div_magic(n, L.X, L.mX, x);
+  z = div_magic(r, L.Y, L.mY, y);
 }
 
 __device__ __forceinline__ void brick_origin(const cs_params& u, uint32_t b, uint32_t& x0, uint32_t& y0, uint32_t& z0) {
@@ -143,13 +237,17 @@ __device__ __forceinline__ void brick_origin(const cs_params& u, uint32_t b, uin
 __device__ __forceinline__ float fine_sum(const cs_params& u, const float4* __restrict__ wc,
                                           const int32_t* __restrict__ p, uint32_t x, uint32_t y, uint32_t z, size_t i,
                                           float4 w) {
-  const size_t sy = u.gx, sz = (size_t)u.gx * u.gy;
-  const float t0 = x > 0u ? wc[i - 1].x * ((float)p[i - 1] * u.inv_fp_scale) : 0.0f;
-  const float t1 = x + 1u < u.gx ? w.x * ((float)p[i + 1] * u.inv_fp_scale) : 0.0f;
-  const float t2 = y > 0u ? wc[i - sy].y * ((float)p[i - sy] * u.inv_fp_scale) : 0.0f;
-  const float t3 = y + 1u < u.gy ? w.y * ((float)p[i + sy] * u.inv_fp_scale) : 0.0f;
-  const float t4 = z > 0u ? wc[i - sz].z * ((float)p[i - sz] * u.inv_fp_scale) : 0.0f;
-  const float t5 = z + 1u < u.gz ? w.z * ((float)p[i + sz] * u.inv_fp_scale) : 0.0f;
+  const nbr6 n = neighbours(u.gx, u.gy, u.gz, x, y, z, i);
+  const float w0 = wc[n.xm].x, w2 = wc[n.ym].y, w4 = wc[n.zm].z;
+  const float p0 = (float)p[n.xm] * u.inv_fp_scale, p1 = (float)p[n.xp] * u.inv_fp_scale;
+  const float p2 = (float)p[n.ym] * u.inv_fp_scale, p3 = (float)p[n.yp] * u.inv_fp_scale;
+  const float p4 = (float)p[n.zm] * u.inv_fp_scale, p5 = (float)p[n.zp] * u.inv_fp_scale;
+  const float t0 = n.bxm ? w0 * p0 : 0.0f;
+  const float t1 = n.bxp ? w.x * p1 : 0.0f;
+  const float t2 = n.bym ? w2 * p2 : 0.0f;
+  const float t3 = n.byp ? w.y * p3 : 0.0f;
+  const float t4 = n.bzm ? w4 * p4 : 0.0f;
+  const float t5 = n.bzp ? w.z * p5 : 0.0f;
   float sum = t0;
   sum += t1;
   sum += t2;
@@ -236,15 +334,17 @@ __global__ void __launch_bounds__(64) k_mg_fine_build(cs_params u, const uint32_
   if (X < c.X && Y < c.Y && Z < c.Z) build_cell(f, c, X, Y, Z);
 }
 
-__global__ void __launch_bounds__(256) k_mg_fine_prolong(cs_params u, const uint32_t* __restrict__ list, cs_mg_level f,
-                                                         cs_mg_level c, const uint32_t* __restrict__ skip) {
+__global__ void __launch_bounds__(256) k_mg_fine_smooth_prolonged(cs_params u, const uint32_t* __restrict__ list,
+                                                                  cs_mg_level f, cs_mg_level c,
+                                                                  const uint32_t* __restrict__ skip) {
   if (*skip) return;
   uint32_t x0, y0, z0;
   brick_origin(u, list[blockIdx.x], x0, y0, z0);
-  for (uint32_t k = threadIdx.x; k < (uint32_t)(BR * BR * BR); k += 256u) {
-    const uint32_t x = x0 + (k & 7u), y = y0 + ((k >> 3) & 7u), z = z0 + (k >> 6);
-    if (x < u.gx && y < u.gy && z < u.gz) prolong_cell(f, c, x, y, z);
-  }
+  const uint32_t t = threadIdx.x;
+  const uint32_t row = t >> 2, q = t & 3u;
+  const uint32_t y = y0 + (row & 7u), z = z0 + (row >> 3);
+  const uint32_t x = x0 + 2u * q + ((y + z + x0) & 1u);
+  if (x < u.gx && y < u.gy && z < u.gz) smooth_prolonged_cell(f, c, x, y, z);
 }
 
 __global__ void __launch_bounds__(256) k_mg_fine_apply(cs_params u, const uint32_t* __restrict__ list, cs_mg_level L,
@@ -288,62 +388,140 @@ __global__ void __launch_bounds__(256) k_mg_build(cs_mg_level f, cs_mg_level c, 
   level_cell(c, n, X, Y, Z);
   build_cell(f, c, X, Y, Z);
 }
-__global__ void __launch_bounds__(256) k_mg_prolong(cs_mg_level f, cs_mg_level c, const uint32_t* __restrict__ skip) {
+__global__ void __launch_bounds__(256) k_mg_smooth_prolonged(cs_mg_level f, cs_mg_level c,
+                                                             const uint32_t* __restrict__ skip) {
   if (*skip) return;
   const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= f.X * f.Y * f.Z) return;
   uint32_t x, y, z;
   level_cell(f, n, x, y, z);
-  prolong_cell(f, c, x, y, z);
+  smooth_prolonged_cell(f, c, x, y, z);
 }
-__global__ void __launch_bounds__(256) k_mg_zero(cs_mg_level L, const uint32_t* __restrict__ skip) {
-  if (*skip) return;
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < L.X * L.Y * L.Z) L.E[n] = 0.0f;
+
+// ---- passes inside one workgroup (LDS levels, 32-bit indices) -----------------------------
+// A pass enumerates only the cells of its colour: slot k of row (y, z) is
+// x = 2k + ((y + z + colour) & 1).  Same arithmetic per cell as smooth_cell and
+// smooth_prolonged_cell.
+__device__ __forceinline__ float wg_term(bool on, float w, float e) { return on ? w * e : 0.0f; }
+
+template <bool PROLONG>
+__device__ __forceinline__ void wg_pass(const slevel& L, const slevel& C, uint32_t colour, uint32_t t) {
+  const uint32_t X = L.X, Y = L.Y, Z = L.Z;
+  const uint32_t total = L.HX * Y * Z;
+  for (uint32_t k = t; k < total; k += CS_MG_WG_THREADS) {
+    uint32_t s, y;
+    const uint32_t row = div_magic(k, L.HX, L.mHX, s);
+    const uint32_t z = div_magic(row, Y, L.mY, y);
+    const uint32_t x = 2u * s + ((y + z + colour) & 1u);
+    if (x >= X) continue;
+    const uint32_t i = (z * Y + y) * X + x;
+    const float d = L.D[i];
+    if (!(d > 0.0f)) continue;
+    const uint32_t sy = X, sz = X * Y;
+    const bool bxm = x > 0u, bxp = x + 1u < X, bym = y > 0u, byp = y + 1u < Y, bzm = z > 0u, bzp = z + 1u < Z;
+    const uint32_t xm = bxm ? i - 1u : i, xp = bxp ? i + 1u : i, ym = bym ? i - sy : i, yp = byp ? i + sy : i;
+    const uint32_t zm = bzm ? i - sz : i, zp = bzp ? i + sz : i;
+    const float w0 = L.Wx[xm], w1 = L.Wx[i], w2 = L.Wy[ym], w3 = L.Wy[i], w4 = L.Wz[zm], w5 = L.Wz[i];
+    float e0 = L.E[xm], e1 = L.E[xp], e2 = L.E[ym], e3 = L.E[yp], e4 = L.E[zm], e5 = L.E[zp];
+    if (PROLONG) {
+      // the neighbours' corrections after adding CS_MG_SCALE times their parents'
+      const uint32_t cX = C.X, cXY = C.X * C.Y;
+      const uint32_t px = x >> 1, py = y >> 1, pz = z >> 1;
+      const uint32_t pc = (pz * C.Y + py) * cX + px;
+      const uint32_t pxm = ((bxm ? x - 1u : x) >> 1) + (pz * C.Y + py) * cX;
+      const uint32_t pxp = ((bxp ? x + 1u : x) >> 1) + (pz * C.Y + py) * cX;
+      const uint32_t pym = pc - py * cX + ((bym ? y - 1u : y) >> 1) * cX;
+      const uint32_t pyp = pc - py * cX + ((byp ? y + 1u : y) >> 1) * cX;
+      const uint32_t pzm = pc - pz * cXY + ((bzm ? z - 1u : z) >> 1) * cXY;
+      const uint32_t pzp = pc - pz * cXY + ((bzp ? z + 1u : z) >> 1) * cXY;
+      const float d0 = L.D[xm], d1 = L.D[xp], d2 = L.D[ym], d3 = L.D[yp], d4 = L.D[zm], d5 = L.D[zp];
+      const float c0 = C.E[pxm], c1 = C.E[pxp], c2 = C.E[pym], c3 = C.E[pyp], c4 = C.E[pzm], c5 = C.E[pzp];
+      e0 = d0 > 0.0f ? e0 + CS_MG_SCALE * c0 : e0;
+      e1 = d1 > 0.0f ? e1 + CS_MG_SCALE * c1 : e1;
+      e2 = d2 > 0.0f ? e2 + CS_MG_SCALE * c2 : e2;
+      e3 = d3 > 0.0f ? e3 + CS_MG_SCALE * c3 : e3;
+      e4 = d4 > 0.0f ? e4 + CS_MG_SCALE * c4 : e4;
+      e5 = d5 > 0.0f ? e5 + CS_MG_SCALE * c5 : e5;
+    }
+    float sum = wg_term(bxm, w0, e0);
+    sum += wg_term(bxp, w1, e1);
+    sum += wg_term(bym, w2, e2);
+    sum += wg_term(byp, w3, e3);
+    sum += wg_term(bzm, w4, e4);
+    sum += wg_term(bzp, w5, e5);
+    L.E[i] = (L.R[i] + sum) / d;
+  }
 }
 
 // ---- one workgroup runs a whole visit of the small levels --------------------------------
+// Levels first..nl-1 (each at most CS_MG_WG_CELLS cells) are staged in LDS:
+// operator, right-hand side of the first level (its parent's restriction) and
+// corrections; the first level's correction goes back to global memory for the
+// parent's prolongation.
 struct mg_levels {
   cs_mg_level l[CS_MG_MAX_LEVELS];
 };
-__global__ void __launch_bounds__(CS_MG_WG_THREADS) k_mg_wg_visit(mg_levels lv, const cs_mg_op* __restrict__ ops,
-                                                                  uint32_t nops, const uint32_t* __restrict__ skip) {
+__global__ void __launch_bounds__(CS_MG_WG_THREADS) k_mg_wg_visit(mg_levels g, int first, int nl,
+                                                                  const cs_mg_op* __restrict__ ops, uint32_t nops,
+                                                                  const uint32_t* __restrict__ skip) {
   if (*skip) return;
+  extern __shared__ float lds_raw[];
+  sfloat* const lds = (sfloat*)lds_raw;
+  __shared__ slevel lv[CS_MG_MAX_LEVELS];  // descriptors indexed at run time: in LDS, not scratch
+  __shared__ cs_mg_op s_ops[CS_MG_MAX_OPS];     // the op list, so no op waits on a global load
   const uint32_t t = threadIdx.x;
+  for (uint32_t o = t; o < nops; o += CS_MG_WG_THREADS) s_ops[o] = ops[o];
+  size_t off = 0;
+  for (int j = first; j < nl; ++j) {
+    const cs_mg_level G = g.l[j];
+    const uint32_t n = G.X * G.Y * G.Z;
+    slevel L;
+    L.X = G.X;
+    L.Y = G.Y;
+    L.Z = G.Z;
+    L.mX = G.mX;
+    L.mY = G.mY;
+    L.HX = (G.X + 1u) >> 1;
+    L.mHX = (uint32_t)(0xFFFFFFFFull / L.HX);
+    L.D = lds + off;
+    L.Wx = L.D + n;
+    L.Wy = L.D + 2 * n;
+    L.Wz = L.D + 3 * n;
+    L.R = L.D + 4 * n;
+    L.E = L.D + 5 * n;
+    off += 6 * (size_t)n;
+    if (t == 0) lv[j] = L;
+    for (uint32_t k = t; k < n; k += CS_MG_WG_THREADS) {
+      L.D[k] = G.D[k];
+      L.Wx[k] = G.Wx[k];
+      L.Wy[k] = G.Wy[k];
+      L.Wz[k] = G.Wz[k];
+      L.R[k] = j == first ? G.R[k] : 0.0f;
+      L.E[k] = 0.0f;
+    }
+  }
+  __syncthreads();
   for (uint32_t o = 0; o < nops; ++o) {
-    const cs_mg_op op = ops[o];
-    const cs_mg_level& L = lv.l[op.level];
+    const cs_mg_op op = s_ops[o];
+    const slevel L = lv[op.level];
     if (op.kind == CS_MG_OP_SMOOTH0 || op.kind == CS_MG_OP_SMOOTH1) {
-      const uint32_t n = L.X * L.Y * L.Z;
-      for (uint32_t k = t; k < n; k += CS_MG_WG_THREADS) {
-        uint32_t x, y, z;
-        level_cell(L, k, x, y, z);
-        smooth_cell(L, x, y, z, op.kind == CS_MG_OP_SMOOTH0 ? 0u : 1u);
-      }
+      wg_pass<false>(L, L, op.kind == CS_MG_OP_SMOOTH0 ? 0u : 1u, t);
     } else if (op.kind == CS_MG_OP_RESTRICT) {
-      const cs_mg_level& C = lv.l[op.level + 1];
-      const uint32_t n = C.X * C.Y * C.Z;
-      for (uint32_t k = t; k < n; k += CS_MG_WG_THREADS) {
+      const slevel C = lv[op.level + 1];
+      const uint32_t nc = C.X * C.Y * C.Z;
+      for (uint32_t k = t; k < nc; k += CS_MG_WG_THREADS) {
         uint32_t X, Y, Z;
         level_cell(C, k, X, Y, Z);
         restrict_cell(L, C, X, Y, Z);
       }
-    } else if (op.kind == CS_MG_OP_ZERO) {
-      const uint32_t n = L.X * L.Y * L.Z;
-      for (uint32_t k = t; k < n; k += CS_MG_WG_THREADS) L.E[k] = 0.0f;
-    } else {  // CS_MG_OP_PROLONG
-      const cs_mg_level& C = lv.l[op.level + 1];
-      const uint32_t n = L.X * L.Y * L.Z;
-      for (uint32_t k = t; k < n; k += CS_MG_WG_THREADS) {
-        uint32_t x, y, z;
-        level_cell(L, k, x, y, z);
-        prolong_cell(L, C, x, y, z);
-      }
+    } else {  // CS_MG_OP_SMOOTH0P
+      wg_pass<true>(L, lv[op.level + 1], 0u, t);
     }
-    // the next operation reads what this one wrote (global memory, one workgroup)
-    __threadfence_block();
     __syncthreads();
   }
+  const slevel F = lv[first];
+  const uint32_t n = F.X * F.Y * F.Z;
+  for (uint32_t k = t; k < n; k += CS_MG_WG_THREADS) g.l[first].E[k] = F.E[k];
 }
 
 // flags[0]: skip the next cycle; flags[1]: the start was within the tolerance
@@ -373,6 +551,8 @@ std::vector<cs_mg_level> cs_mg_shape(uint32_t gx, uint32_t gy, uint32_t gz) {
     l.X = X;
     l.Y = Y;
     l.Z = Z;
+    l.mX = (uint32_t)(0xFFFFFFFFull / X);
+    l.mY = (uint32_t)(0xFFFFFFFFull / Y);
     out.push_back(l);
     if ((X <= CS_MG_COARSEST_SIDE && Y <= CS_MG_COARSEST_SIDE && Z <= CS_MG_COARSEST_SIDE) ||
         out.size() == (size_t)CS_MG_MAX_LEVELS)
@@ -396,10 +576,10 @@ void cs_mg_visit_ops(int level, int nlevels, std::vector<cs_mg_op>* out) {
   for (uint32_t k = 0; k < CS_MG_NU; ++k) pass(level);
   for (uint32_t g = 0; g < CS_MG_GAMMA; ++g) {
     out->push_back(cs_mg_op{CS_MG_OP_RESTRICT, (uint8_t)level});
-    out->push_back(cs_mg_op{CS_MG_OP_ZERO, (uint8_t)(level + 1)});
     cs_mg_visit_ops(level + 1, nlevels, out);
-    out->push_back(cs_mg_op{CS_MG_OP_PROLONG, (uint8_t)level});
-    for (uint32_t k = 0; k < CS_MG_NU; ++k) pass(level);
+    out->push_back(cs_mg_op{CS_MG_OP_SMOOTH0P, (uint8_t)level});
+    out->push_back(cs_mg_op{CS_MG_OP_SMOOTH1, (uint8_t)level});
+    for (uint32_t k = 1; k < CS_MG_NU; ++k) pass(level);
   }
 }
 
@@ -428,10 +608,10 @@ hipError_t cs_mg_fine_build(hipStream_t s, const cs_params& u, uint32_t nactive,
   hipLaunchKernelGGL(k_mg_fine_build, dim3(nactive), dim3(64), 0, s, u, list, L0, L1, skip);
   return hipGetLastError();
 }
-hipError_t cs_mg_fine_prolong(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
-                              cs_mg_level L0, cs_mg_level L1, const uint32_t* skip) {
+hipError_t cs_mg_fine_smooth_prolonged(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                                       cs_mg_level L0, cs_mg_level L1, const uint32_t* skip) {
   if (!nactive) return hipSuccess;
-  hipLaunchKernelGGL(k_mg_fine_prolong, dim3(nactive), dim3(256), 0, s, u, list, L0, L1, skip);
+  hipLaunchKernelGGL(k_mg_fine_smooth_prolonged, dim3(nactive), dim3(256), 0, s, u, list, L0, L1, skip);
   return hipGetLastError();
 }
 hipError_t cs_mg_fine_apply(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
@@ -452,19 +632,25 @@ hipError_t cs_mg_build(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32
   hipLaunchKernelGGL(k_mg_build, cells_grid(cells(c)), dim3(256), 0, s, f, c, skip);
   return hipGetLastError();
 }
-hipError_t cs_mg_prolong(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32_t* skip) {
-  hipLaunchKernelGGL(k_mg_prolong, cells_grid(cells(f)), dim3(256), 0, s, f, c, skip);
+hipError_t cs_mg_smooth_prolonged(hipStream_t s, cs_mg_level f, cs_mg_level c, const uint32_t* skip) {
+  hipLaunchKernelGGL(k_mg_smooth_prolonged, cells_grid(cells(f)), dim3(256), 0, s, f, c, skip);
   return hipGetLastError();
 }
-hipError_t cs_mg_zero(hipStream_t s, cs_mg_level L, const uint32_t* skip) {
-  hipLaunchKernelGGL(k_mg_zero, cells_grid(cells(L)), dim3(256), 0, s, L, skip);
-  return hipGetLastError();
+size_t cs_mg_wg_lds_bytes(const cs_mg_level* levels, int first, int nl) {
+  size_t floats = 0;
+  for (int j = first; j < nl; ++j) floats += 6 * (size_t)cells(levels[j]);
+  return floats * sizeof(float);
 }
-hipError_t cs_mg_wg_visit(hipStream_t s, const cs_mg_level* levels, const cs_mg_op* ops, uint32_t nops,
-                          const uint32_t* skip) {
+hipError_t cs_mg_wg_prepare(size_t lds_bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_wg_visit),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+hipError_t cs_mg_wg_visit(hipStream_t s, const cs_mg_level* levels, int first, int nl, const cs_mg_op* ops,
+                          uint32_t nops, const uint32_t* skip) {
   mg_levels lv{};
   for (int l = 0; l < CS_MG_MAX_LEVELS; ++l) lv.l[l] = levels[l];
-  hipLaunchKernelGGL(k_mg_wg_visit, dim3(1), dim3(CS_MG_WG_THREADS), 0, s, lv, ops, nops, skip);
+  hipLaunchKernelGGL(k_mg_wg_visit, dim3(1), dim3(CS_MG_WG_THREADS), cs_mg_wg_lds_bytes(levels, first, nl), s, lv,
+                     first, nl, ops, nops, skip);
   return hipGetLastError();
 }
 hipError_t cs_mg_gate(hipStream_t s, const cs_solve_state* st, double tol, uint32_t* flags, bool first) {

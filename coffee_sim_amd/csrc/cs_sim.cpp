@@ -351,6 +351,7 @@ int ensure_mg(cs_sim* m) {
       break;
     }
   if (m->mg_wg_first < nl) {
+    CS_HIP(cs_mg_wg_prepare(cs_mg_wg_lds_bytes(lv.data(), m->mg_wg_first, nl)));
     std::vector<cs_mg_op> ops;
     cs_mg_visit_ops(m->mg_wg_first, nl, &ops);
     if (ops.size() > CS_MG_MAX_OPS) return fail(CS_ERR_STATE, "multigrid: the coarse visit has too many operations");
@@ -372,7 +373,7 @@ int mg_visit(cs_sim* m, const cs_params& u, uint32_t nactive, int l) {
   if (l >= m->mg_wg_first) {
     cs_mg_level lv[CS_MG_MAX_LEVELS] = {};
     for (int k = 0; k < nl; ++k) lv[k] = m->mg_levels[k];
-    CS_HIP(cs_mg_wg_visit(st, lv, m->mg_ops, m->mg_nops, skip));
+    CS_HIP(cs_mg_wg_visit(st, lv, m->mg_wg_first, nl, m->mg_ops, m->mg_nops, skip));
     return CS_OK;
   }
   const cs_mg_level& L = m->mg_levels[l];
@@ -389,14 +390,20 @@ int mg_visit(cs_sim* m, const cs_params& u, uint32_t nactive, int l) {
   if (rc) return rc;
   const cs_mg_level& C = m->mg_levels[l + 1];
   for (uint32_t g = 0; g < CS_MG_GAMMA; ++g) {
+    // restriction (zeroes the coarse correction), the coarse visit, then the
+    // prolongation folded into the first red pass
     if (l == 0) CS_HIP(cs_mg_fine_restrict(st, u, nactive, m->brick_list, L, C, skip));
     else CS_HIP(cs_mg_restrict(st, L, C, skip));
-    CS_HIP(cs_mg_zero(st, C, skip));
     rc = mg_visit(m, u, nactive, l + 1);
     if (rc) return rc;
-    if (l == 0) CS_HIP(cs_mg_fine_prolong(st, u, nactive, m->brick_list, L, C, skip));
-    else CS_HIP(cs_mg_prolong(st, L, C, skip));
-    rc = passes(CS_MG_NU);
+    if (l == 0) {
+      CS_HIP(cs_mg_fine_smooth_prolonged(st, u, nactive, m->brick_list, L, C, skip));
+      CS_HIP(cs_mg_fine_smooth(st, u, nactive, m->brick_list, L, 1u, skip));
+    } else {
+      CS_HIP(cs_mg_smooth_prolonged(st, L, C, skip));
+      CS_HIP(cs_mg_smooth(st, L, 1u, skip));
+    }
+    rc = passes(CS_MG_NU - 1u);
     if (rc) return rc;
   }
   return CS_OK;
