@@ -16,7 +16,7 @@
 #define CS_MG_COARSEST_SIDE 8u   // levels are added until no side exceeds this
 #define CS_MG_MAX_LEVELS 12
 #define CS_MG_WG_CELLS 4096u     // levels this small run inside one workgroup
-#define CS_MG_WG_THREADS 1024
+#define CS_MG_WG_THREADS 512
 #define CS_MG_MAX_OPS 4096u
 
 // One level: a dense X*Y*Z grid of cells with the Galerkin operator (diagonal

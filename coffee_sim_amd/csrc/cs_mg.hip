@@ -399,24 +399,32 @@ __global__ void __launch_bounds__(256) k_mg_smooth_prolonged(cs_mg_level f, cs_m
 }
 
 // ---- passes inside one workgroup (LDS levels, 32-bit indices) -----------------------------
-// A pass enumerates only the cells of its colour: slot k of row (y, z) is
-// x = 2k + ((y + z + colour) & 1).  Same arithmetic per cell as smooth_cell and
-// smooth_prolonged_cell.
+// Each staged level keeps the cells of its operator (D > 0) in two lists, one
+// per colour, packed x | y << 10 | z << 20.  A pass of one colour reads only
+// the other colour, so the order of a list (filled by LDS atomics) changes
+// nothing; the cells off the lists keep a zero correction.  Same arithmetic
+// per cell as smooth_cell / smooth_prolonged_cell / restrict_cell.
+typedef __attribute__((address_space(3))) uint32_t suint;
+struct slist {
+  suint* c0;      // colour-0 cells, from the front of the level's list segment
+  suint* c1end;   // colour-1 cells, from the back: c1end[-1 - k]
+  uint32_t n0, n1;
+};
+
 __device__ __forceinline__ float wg_term(bool on, float w, float e) { return on ? w * e : 0.0f; }
+__device__ __forceinline__ uint32_t wg_entry(const slist& l, uint32_t colour, uint32_t k) {
+  return colour == 0u ? l.c0[k] : l.c1end[-1 - (int)k];
+}
 
 template <bool PROLONG>
-__device__ __forceinline__ void wg_pass(const slevel& L, const slevel& C, uint32_t colour, uint32_t t) {
+__device__ __forceinline__ void wg_pass(const slevel& L, const slevel& C, const slist& A, uint32_t colour, uint32_t t) {
   const uint32_t X = L.X, Y = L.Y, Z = L.Z;
-  const uint32_t total = L.HX * Y * Z;
-  for (uint32_t k = t; k < total; k += CS_MG_WG_THREADS) {
-    uint32_t s, y;
-    const uint32_t row = div_magic(k, L.HX, L.mHX, s);
-    const uint32_t z = div_magic(row, Y, L.mY, y);
-    const uint32_t x = 2u * s + ((y + z + colour) & 1u);
-    if (x >= X) continue;
+  const uint32_t count = colour == 0u ? A.n0 : A.n1;
+  for (uint32_t k = t; k < count; k += CS_MG_WG_THREADS) {
+    const uint32_t e = wg_entry(A, colour, k);
+    const uint32_t x = e & 1023u, y = (e >> 10) & 1023u, z = e >> 20;
     const uint32_t i = (z * Y + y) * X + x;
     const float d = L.D[i];
-    if (!(d > 0.0f)) continue;
     const uint32_t sy = X, sz = X * Y;
     const bool bxm = x > 0u, bxp = x + 1u < X, bym = y > 0u, byp = y + 1u < Y, bzm = z > 0u, bzp = z + 1u < Z;
     const uint32_t xm = bxm ? i - 1u : i, xp = bxp ? i + 1u : i, ym = bym ? i - sy : i, yp = byp ? i + sy : i;
@@ -453,11 +461,21 @@ __device__ __forceinline__ void wg_pass(const slevel& L, const slevel& C, uint32
   }
 }
 
+// Restriction into the coarse cells of the operator (the others' right-hand
+// side is never read and their correction stays zero).
+__device__ __forceinline__ void wg_restrict(const slevel& F, const slevel& C, const slist& A, uint32_t t) {
+  const uint32_t total = A.n0 + A.n1;
+  for (uint32_t k = t; k < total; k += CS_MG_WG_THREADS) {
+    const uint32_t e = k < A.n0 ? wg_entry(A, 0u, k) : wg_entry(A, 1u, k - A.n0);
+    restrict_cell(F, C, e & 1023u, (e >> 10) & 1023u, e >> 20);
+  }
+}
+
 // ---- one workgroup runs a whole visit of the small levels --------------------------------
 // Levels first..nl-1 (each at most CS_MG_WG_CELLS cells) are staged in LDS:
-// operator, right-hand side of the first level (its parent's restriction) and
-// corrections; the first level's correction goes back to global memory for the
-// parent's prolongation.
+// operator, right-hand side of the first level (its parent's restriction),
+// corrections and the active lists; the first level's correction goes back to
+// global memory for the parent's prolongation.
 struct mg_levels {
   cs_mg_level l[CS_MG_MAX_LEVELS];
 };
@@ -468,10 +486,18 @@ __global__ void __launch_bounds__(CS_MG_WG_THREADS) k_mg_wg_visit(mg_levels g, i
   extern __shared__ float lds_raw[];
   sfloat* const lds = (sfloat*)lds_raw;
   __shared__ slevel lv[CS_MG_MAX_LEVELS];  // descriptors indexed at run time: in LDS, not scratch
-  __shared__ cs_mg_op s_ops[CS_MG_MAX_OPS];     // the op list, so no op waits on a global load
+  __shared__ slist al[CS_MG_MAX_LEVELS];
+  __shared__ uint32_t cnt[CS_MG_MAX_LEVELS][2];
+  __shared__ cs_mg_op s_ops[CS_MG_MAX_OPS];  // the op list, so no op waits on a global load
   const uint32_t t = threadIdx.x;
   for (uint32_t o = t; o < nops; o += CS_MG_WG_THREADS) s_ops[o] = ops[o];
+  if (t < (uint32_t)(2 * CS_MG_MAX_LEVELS)) cnt[t >> 1][t & 1u] = 0u;
   size_t off = 0;
+  for (int j = first; j < nl; ++j) off += 6 * (size_t)(g.l[j].X * g.l[j].Y * g.l[j].Z);
+  suint* const lst = (suint*)(lds + off);  // one list segment of n entries per level
+  __syncthreads();
+  off = 0;
+  size_t loff = 0;
   for (int j = first; j < nl; ++j) {
     const cs_mg_level G = g.l[j];
     const uint32_t n = G.X * G.Y * G.Z;
@@ -490,32 +516,47 @@ __global__ void __launch_bounds__(CS_MG_WG_THREADS) k_mg_wg_visit(mg_levels g, i
     L.R = L.D + 4 * n;
     L.E = L.D + 5 * n;
     off += 6 * (size_t)n;
+    suint* const seg = lst + loff;
+    loff += n;
     if (t == 0) lv[j] = L;
     for (uint32_t k = t; k < n; k += CS_MG_WG_THREADS) {
-      L.D[k] = G.D[k];
+      const float d = G.D[k];
+      L.D[k] = d;
       L.Wx[k] = G.Wx[k];
       L.Wy[k] = G.Wy[k];
       L.Wz[k] = G.Wz[k];
       L.R[k] = j == first ? G.R[k] : 0.0f;
       L.E[k] = 0.0f;
+      if (d > 0.0f) {
+        uint32_t x, y, z;
+        level_cell(G, k, x, y, z);
+        const uint32_t c = (x + y + z) & 1u;
+        const uint32_t pos = atomicAdd(&cnt[j][c], 1u);
+        const uint32_t e = x | (y << 10) | (z << 20);
+        if (c == 0u) seg[pos] = e;
+        else seg[n - 1u - pos] = e;
+      }
     }
+    if (t == 0) {
+      al[j].c0 = seg;
+      al[j].c1end = seg + n;
+    }
+  }
+  __syncthreads();
+  if (t < (uint32_t)CS_MG_MAX_LEVELS) {
+    al[t].n0 = cnt[t][0];
+    al[t].n1 = cnt[t][1];
   }
   __syncthreads();
   for (uint32_t o = 0; o < nops; ++o) {
     const cs_mg_op op = s_ops[o];
     const slevel L = lv[op.level];
     if (op.kind == CS_MG_OP_SMOOTH0 || op.kind == CS_MG_OP_SMOOTH1) {
-      wg_pass<false>(L, L, op.kind == CS_MG_OP_SMOOTH0 ? 0u : 1u, t);
+      wg_pass<false>(L, L, al[op.level], op.kind == CS_MG_OP_SMOOTH0 ? 0u : 1u, t);
     } else if (op.kind == CS_MG_OP_RESTRICT) {
-      const slevel C = lv[op.level + 1];
-      const uint32_t nc = C.X * C.Y * C.Z;
-      for (uint32_t k = t; k < nc; k += CS_MG_WG_THREADS) {
-        uint32_t X, Y, Z;
-        level_cell(C, k, X, Y, Z);
-        restrict_cell(L, C, X, Y, Z);
-      }
+      wg_restrict(L, lv[op.level + 1], al[op.level + 1], t);
     } else {  // CS_MG_OP_SMOOTH0P
-      wg_pass<true>(L, lv[op.level + 1], 0u, t);
+      wg_pass<true>(L, lv[op.level + 1], al[op.level], 0u, t);
     }
     __syncthreads();
   }
@@ -637,9 +678,9 @@ hipError_t cs_mg_smooth_prolonged(hipStream_t s, cs_mg_level f, cs_mg_level c, c
   return hipGetLastError();
 }
 size_t cs_mg_wg_lds_bytes(const cs_mg_level* levels, int first, int nl) {
-  size_t floats = 0;
-  for (int j = first; j < nl; ++j) floats += 6 * (size_t)cells(levels[j]);
-  return floats * sizeof(float);
+  size_t words = 0;  // per level: 6 arrays and the active lists, one word per cell each
+  for (int j = first; j < nl; ++j) words += 7 * (size_t)cells(levels[j]);
+  return words * sizeof(float);
 }
 hipError_t cs_mg_wg_prepare(size_t lds_bytes) {
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_wg_visit),
