@@ -8,7 +8,10 @@ red-black Gauss-Seidel pressure solve run to a 1e-4 relative residual every
 substep.  Inputs are synthetic (seeded generators); nothing is read from disk.
 
 A "step" is one substep of the hot path (P2G -> grid update -> classify ->
-RBGS to tolerance -> projection -> packing -> viscosity -> G2P).  The scene is
+pressure solve -> projection -> packing -> viscosity -> G2P).  The pressure
+solve is the reference's red-black Gauss-Seidel run to the tolerance; by
+default (`--mg-cycles 3`) up to three multigrid W-cycles first correct its
+start (the iteration still decides convergence, on the same residual).  The scene is
 first advanced through its pour (about 158 substeps, untimed; `--state` keeps
 that state in a checkpoint file for later runs) so every timed substep runs
 with all 1,048,576 particles in the domain; then `--warmup` untimed substeps,
@@ -68,7 +71,7 @@ def measured_traffic(kernel: str) -> int | None:
 
 
 def roofline(kt) -> dict:
-    """Achieved algorithmic bandwidth of the dominant kernel.
+    """Achieved algorithmic bandwidth of the dominant kernel (family).
 
     Algorithmic bytes (per unit, counted once per launch):
       P2G   80 B per particle read (pos, vel, C) + 24 B per touched cell written
@@ -77,10 +80,27 @@ def roofline(kt) -> dict:
       G2P   128 B per particle (48 read + 80 written) + 20 B per cell read.
       RBGS  16 B per fluid cell per red+black iteration: read packed kind/fill
             4 B, read rhs 4 B, read p 4 B (as a neighbour), write p 4 B.
+      MG    the multigrid start of one solve, all its launches together
+            ("launch" = one start), per W-cycle run on the device: on the fine
+            level six red-black passes (two before and two after each of the
+            two coarse corrections) at the RBGS rate of 16 B per fluid cell,
+            plus the level's set-up (read row 16 B, rhs 4 B, pressure 4 B;
+            write operator, residual, correction 24 B), restriction (24 B),
+            Galerkin build (16 B) and the final p += e (12 B): 204 B per fluid
+            cell; each coarser level has 1/8 of the cells and is visited twice
+            as often, so the whole cycle is 4/3 of that, 272 B per fluid cell.
+            The start is bound by launch and synchronisation latency, not by
+            HBM: the fraction says so.
     """
-    shares = {"p2g": kt.p2g_us, "g2p": kt.g2p_us, "rbgs": kt.rbgs_us}
+    shares = {"p2g": kt.p2g_us, "g2p": kt.g2p_us, "rbgs": kt.rbgs_us, "mg_start": kt.mg_us}
     dom = max(shares, key=shares.get)
-    if dom == "rbgs":
+    if dom == "mg_start":
+        launches = max(kt.mg_starts, 1)
+        fluid = kt.rbgs_cell_updates / max(kt.rbgs_iterations, 1)
+        bytes_per_launch = 272.0 * fluid * kt.mg_cycles / launches
+        avg_us = kt.mg_us / launches
+        name = "mg_start"
+    elif dom == "rbgs":
         launches = max(kt.rbgs_launches, 1)
         # rbgs_cell_updates = iterations x fluid cells; 16 B per fluid cell per iteration
         bytes_per_launch = 16.0 * kt.rbgs_cell_updates / launches
@@ -112,7 +132,7 @@ def roofline(kt) -> dict:
 
 
 def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, prev_p: np.ndarray, budget_s: float,
-                 threads: int) -> dict:
+                 threads: int, mg_cycles: int = 0) -> dict:
     """The CPU oracle (the C restatement of the reference solver; with
     threads > 1 its OpenMP build, whose cell and particle loops run in
     parallel) continuing the GPU run: the same particles and the same
@@ -139,6 +159,7 @@ def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, prev_p: np.ndarra
     s.emit_per_substep = 0
     s.emit_total = 0
     s.warm_start = CS_WARM_PREVIOUS
+    s.rbgs_mg_cycles = mg_cycles
     ref = orc.OracleSim(s, sc, parallel=threads > 1, threads=threads)
     ref.set_previous_pressure(prev_p)
     iters = []
@@ -159,8 +180,9 @@ def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, prev_p: np.ndarra
         "cores": threads,
         "kind": "port",
         "sample": (f"v60_working after the timed GPU substeps ({n} particles), {who}: {len(iters)} whole "
-                   f"substeps from the GPU run's state and previous-step pressure, RBGS to 1e-4 "
-                   f"({np.mean(iters):.0f} iterations per substep on average), {el:.1f} s measured"),
+                   f"substeps from the GPU run's state and previous-step pressure, "
+                   + (f"{mg_cycles}-cycle multigrid start, " if mg_cycles else "")
+                   + f"RBGS to 1e-4 ({np.mean(iters):.0f} iterations per substep on average), {el:.1f} s measured"),
     }
 
 
@@ -449,9 +471,13 @@ def main() -> int:
     ap.add_argument("--state", default=None,
                     help="checkpoint of the post-pour state: loaded when the file exists (the pour is skipped), "
                          "written after the pour otherwise")
-    ap.add_argument("--mg-cycles", type=int, default=0,
+    ap.add_argument("--mg-cycles", type=int, default=3,
                     help="multigrid W-cycles correcting each solve's start before the red-black iteration "
-                         "(rbgs_mg_cycles; 0 = the plain iteration)")
+                         "(rbgs_mg_cycles; 0 = the plain iteration from the previous-step start)")
+    ap.add_argument("--skip-teardown", action="store_true",
+                    help="return after printing the result without closing the simulation and context, "
+                         "leaving them to the interpreter's shutdown (profiled runs: rocprofv3 has failed "
+                         "in the explicit teardown before writing its results)")
     ap.add_argument("--no-zero-probe", action="store_true",
                     help="skip the one untimed substep from a zero start that reports its iteration count")
     args = ap.parse_args()
@@ -524,6 +550,7 @@ def main() -> int:
     log(f"kernel timing {kt.as_dict()}")
 
     zero_probe = None
+    plain_probe = None
     if not args.no_zero_probe:
         # the same state solved from a zero start, one substep, untimed
         zs = settings.copy()
@@ -537,13 +564,26 @@ def main() -> int:
                       "final_residual": float(st0.final_residual)}
         sim0.close()
         log(f"zero-start probe: {zero_probe}")
+        if settings.rbgs_mg_cycles:
+            # the same state from the previous-step start without the multigrid start
+            ps = settings.copy()
+            ps.rbgs_mg_cycles = 0
+            sim1 = cs.MpmSim3D(ctx, ps, w.scene)
+            sim1.load(sim.save())
+            sim1.step_frame(dt)
+            st1 = sim1.solve_stats()
+            plain_probe = {"iterations": int(st1.iterations), "converged": bool(st1.converged),
+                           "final_residual": float(st1.final_residual)}
+            sim1.close()
+            log(f"previous-start probe without the multigrid start: {plain_probe}")
 
     cpu = None
     if args.cpu_budget > 0:
         parts = sim.read_particles(0, n_particles)
         aff = sim.read_affine(0, n_particles)
         prev_p, _ = sim.read_pressure()
-        cpu = cpu_baseline(w, parts, aff, prev_p, args.cpu_budget, max(args.cpu_threads, 1))
+        cpu = cpu_baseline(w, parts, aff, prev_p, args.cpu_budget, max(args.cpu_threads, 1),
+                           settings.rbgs_mg_cycles)
 
     value = psteps / elapsed
     out = {
@@ -561,25 +601,33 @@ def main() -> int:
         "data": "synthetic (seeded generators)",
         "config": {
             "workload": "v60_working: 128^3 grid, dx 1 mm, 1,048,576 particles, value-noise porous bed, "
-                        f"RBGS to 1e-4 relative residual from a {args.warm_start}-step start"
-                        if args.warm_start == "previous" else
-                        "v60_working: 128^3 grid, dx 1 mm, 1,048,576 particles, value-noise porous bed, "
-                        "RBGS to 1e-4 relative residual from a zero start",
+                        "RBGS to 1e-4 relative residual from a "
+                        + ("previous-step" if args.warm_start == "previous" else "zero")
+                        + (f" start corrected by a {settings.rbgs_mg_cycles}-cycle multigrid start"
+                           if settings.rbgs_mg_cycles else " start"),
             "grid": list(int(d) for d in w.settings.grid_dims),
             "particles": int(n_particles),
             "rbgs_tolerance": float(w.settings.rbgs_tolerance),
             "rbgs_warm_start": args.warm_start,
+            "rbgs_mg_cycles": int(settings.rbgs_mg_cycles),
             "rbgs_iterations_per_substep": round(float(np.mean(iters_timed)), 1),
             "rbgs_zero_start_probe": zero_probe,
+            "rbgs_previous_start_probe_without_mg": plain_probe,
             "scale": args.scale,
         },
         "roofline": roofline(kt),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.skip_teardown:
+        _LEFT_OPEN.extend([sim, ctx])  # released by the interpreter's own shutdown
+        return 0
     sim.close()
     ctx.close()
     return 0
+
+
+_LEFT_OPEN: list = []
 
 
 if __name__ == "__main__":

@@ -223,6 +223,7 @@ class KernelTiming(C.Structure):
         ("rbgs_cells_swept", C.c_uint64),
         ("mg_us", C.c_double),
         ("mg_starts", C.c_uint64),
+        ("mg_cycles", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -566,11 +566,17 @@ __global__ void __launch_bounds__(CS_MG_WG_THREADS) k_mg_wg_visit(mg_levels g, i
 }
 
 // flags[0]: skip the next cycle; flags[1]: the start was within the tolerance
-// (no cycle ran).  The first gate sets both, later ones only flags[0].
+// (no cycle ran); flags[2]: cycles run so far.  The first gate sets all three,
+// later ones count the cycle just run and set flags[0].
 __global__ void k_mg_gate(const cs_solve_state* __restrict__ st, double tol, uint32_t* __restrict__ flags,
                           uint32_t first) {
   const uint32_t within = st->initial_residual <= tol ? 1u : 0u;
-  if (first) flags[1] = within;
+  if (first) {
+    flags[1] = within;
+    flags[2] = 0u;
+  } else if (!flags[0]) {
+    flags[2] += 1u;
+  }
   flags[0] = flags[1] | within;
 }
 // After the cycles: the iteration runs (its first test comes after a chunk)

@@ -328,7 +328,7 @@ int ensure_mg(cs_sim* m) {
   size_t total = 0;
   for (const auto& l : lv) total += 6 * (size_t)l.X * l.Y * l.Z;
   CS_HIP(dalloc(&m->mg_buf, total));
-  CS_HIP(dalloc(&m->mg_flags, 2));
+  CS_HIP(dalloc(&m->mg_flags, 3));
   size_t off = 0;
   for (auto& l : lv) {
     const size_t n = (size_t)l.X * l.Y * l.Z;
@@ -449,9 +449,12 @@ int mg_start(cs_sim* m, const cs_params& u, uint32_t nactive) {
 void mg_harvest(cs_sim* m) {
   if (!m->mg_timed) return;
   float ms = 0.0f;
-  if (hipEventSynchronize(m->mg_ev[1]) == hipSuccess && hipEventElapsedTime(&ms, m->mg_ev[0], m->mg_ev[1]) == hipSuccess) {
+  uint32_t ran = 0;
+  if (hipEventSynchronize(m->mg_ev[1]) == hipSuccess && hipEventElapsedTime(&ms, m->mg_ev[0], m->mg_ev[1]) == hipSuccess &&
+      hipMemcpy(&ran, m->mg_flags + 2, sizeof(uint32_t), hipMemcpyDeviceToHost) == hipSuccess) {
     m->kt.mg_us += (double)ms * 1000.0;
     m->kt.mg_starts += 1;
+    m->kt.mg_cycles += ran;
   }
   m->mg_timed = false;
 }

@@ -429,7 +429,9 @@ typedef struct cs_kernel_timing {
   uint64_t rbgs_cell_updates;  /* fluid-cell updates (iterations x fluid cells)  */
   uint64_t rbgs_cells_swept;   /* cells the sweep launches visited              */
   double mg_us;                /* multigrid starts (rbgs_mg_cycles), whole     */
-  uint64_t mg_starts;          /* solves whose start the multigrid corrected    */
+  uint64_t mg_starts;          /* solves that ran the multigrid start section   */
+  uint64_t mg_cycles;          /* W-cycles those starts ran (the rest returned
+                                * at once once the start met the tolerance)     */
 } cs_kernel_timing;
 int cs_sim_get_kernel_timing(cs_sim* sim, cs_kernel_timing* out);
 int cs_sim_reset_kernel_timing(cs_sim* sim);
