@@ -5,14 +5,11 @@ HIP path with the reference's frame counts and thresholds.
 The reference asserts statistical invariants (mass drift, J bounds, kinetic
 energy decay, viscosity ordering, the effect of the pressure projection on a
 sparse stream), not exact values; the HIP path itself equals the oracle bit
-for bit in the parity tests.  The readback statistics below restate the
-reference's snapshot helpers (physics_tests.rs:61-1083) over the particle
-array: particles whose mass is at most 0.1 nominal mass are inactive, the
-particle volume is dx^3 / 4 per nominal mass.  Scenarios sharing a 120-,
-180- or 300-frame pour start from one checkpoint of it (a loaded checkpoint
-continues bit for bit, as the parity tests check).  The scenarios that need
-bed particles (the reference-mode coffee bed) are not ported: that bed is
-not built."""
+for bit in the parity tests.  The readback statistics (scenario_stats)
+restate the reference's snapshot helpers.  Scenarios sharing a 120-, 180- or
+300-frame pour start from one checkpoint of it (a loaded checkpoint
+continues bit for bit, as the parity tests check).  The center-pour
+scenarios, which need the bed, are in test_gpu_bed_scenarios."""
 from __future__ import annotations
 
 import os
@@ -21,12 +18,10 @@ import numpy as np
 import pytest
 
 import coffee_sim_amd as cs
+from scenario_stats import (FRAME, _dx, diag_snapshot, frames, mass_snapshot, packing_snapshot,
+                            velocity_snapshot, volume_snapshot)
 
 pytestmark = pytest.mark.gpu
-
-FRAME = 1.0 / 60.0
-NOMINAL_MASS = 80.0 / 320.0
-INACTIVE = NOMINAL_MASS * 0.1
 
 
 def free_stream(**overrides) -> cs.abi.Settings:
@@ -34,110 +29,6 @@ def free_stream(**overrides) -> cs.abi.Settings:
     for k, v in overrides.items():
         setattr(s, k, v)
     return s
-
-
-def frames(sim, n: int) -> None:
-    for _ in range(n):
-        sim.step_frame(FRAME)
-
-
-def _water(sim):
-    p = sim.read_particles().astype(np.float64)
-    nw, nb, _ = sim.counts()
-    return p[nb:nb + nw]
-
-
-def _dx(sim) -> float:
-    return float(sim.settings.bounds_size[0]) / int(sim.settings.grid_dims[0])
-
-
-# ---- snapshots (physics_tests.rs:61-1083) -------------------------------------------------
-def mass_snapshot(sim) -> dict:
-    p = sim.read_particles().astype(np.float64)
-    return {"active_particle_mass": float(p[:, 7].sum())}
-
-
-def volume_snapshot(sim) -> dict:
-    w = _water(sim)
-    finite = bool(np.isfinite(w[:, [0, 1, 2, 3, 7]]).all())
-    act = w[w[:, 7] > INACTIVE]
-    pv = _dx(sim) ** 3 * 0.25
-    scale = act[:, 7] / NOMINAL_MASS
-    j = act[:, 3]
-    n = max(len(act), 1)
-    return {"all_finite": finite, "active_count": len(act), "active_mass": float(act[:, 7].sum()),
-            "rest_volume": float((scale * pv).sum()), "current_volume": float((scale * pv * j).sum()),
-            "mean_j": float(j.sum() / n), "min_j": float(j.min()) if len(act) else 0.0,
-            "max_j": float(j.max()) if len(act) else 0.0}
-
-
-def packing_snapshot(sim) -> dict:
-    w = _water(sim)
-    finite = bool(np.isfinite(w[:, [0, 1, 2, 3, 7]]).all())
-    act = w[w[:, 7] > INACTIVE]
-    gx, gy, gz = (int(v) for v in sim.settings.grid_dims)
-    dx = _dx(sim)
-    cell_v = dx ** 3
-    origin = -0.5 * np.array([float(v) for v in sim.settings.bounds_size])
-    g = (act[:, :3] - origin) / dx
-    base = np.floor(g - 0.5)
-    fx = g - base
-    wts = [np.stack([0.5 * (1.5 - fx[:, a]) ** 2, 0.75 - (fx[:, a] - 1.0) ** 2, 0.5 * (fx[:, a] - 0.5) ** 2])
-           for a in range(3)]
-    rest_p = cell_v * 0.25 * act[:, 7] / NOMINAL_MASS
-    cur_p = rest_p * np.clip(act[:, 3], 0.40, 2.00)
-    rest = np.zeros(gx * gy * gz)
-    cur = np.zeros(gx * gy * gz)
-    b = base.astype(np.int64)
-    for i in range(3):
-        for j in range(3):
-            for k in range(3):
-                cx, cy, cz = b[:, 0] + i, b[:, 1] + j, b[:, 2] + k
-                ok = (cx >= 0) & (cy >= 0) & (cz >= 0) & (cx < gx) & (cy < gy) & (cz < gz)
-                wgt = wts[0][i] * wts[1][j] * wts[2][k]
-                cell = (cz * gy + cy) * gx + cx
-                np.add.at(rest, cell[ok], (wgt * rest_p)[ok])
-                np.add.at(cur, cell[ok], (wgt * cur_p)[ok])
-    rf, cf = rest / cell_v, cur / cell_v
-    packed = np.maximum(rf, cf)
-    frac = (packed > 0.0) & (packed < 1.0)
-    over = packed > 1.20
-    return {"all_finite": finite, "active_count": len(act), "deposited_rest_volume": float(rest.sum()),
-            "deposited_current_volume": float(cur.sum()), "max_rest_fraction": float(rf.max()),
-            "max_current_fraction": float(cf.max()), "max_packed_fraction": float(packed.max()),
-            "fractional_cell_count": int(frac.sum()),
-            "max_fractional_fraction": float(packed[frac].max()) if frac.any() else 0.0,
-            "overpacked_cell_count": int(over.sum()),
-            "overpacked_volume_fraction": float((packed[over] - 1.20).sum())}
-
-
-def velocity_snapshot(sim, y_min: float = -np.inf, y_max: float = np.inf) -> dict:
-    w = _water(sim)
-    finite = bool(np.isfinite(w[:, [1, 4, 5, 6, 7]]).all())
-    act = w[(w[:, 7] > INACTIVE) & (w[:, 1] >= y_min) & (w[:, 1] <= y_max)]
-    m, v = act[:, 7], act[:, 4:7]
-    sp2 = (v ** 2).sum(1)
-    mass = float(m.sum())
-    n = max(len(act), 1)
-    return {"all_finite": finite, "active_count": len(act), "active_mass": mass,
-            "kinetic_energy": float((0.5 * m * sp2).sum()),
-            "rms_speed": float(np.sqrt((m * sp2).sum() / max(mass, 1e-6))),
-            "mean_speed": float(np.sqrt(sp2).sum() / n),
-            "lateral_rms_speed": float(np.sqrt((m * (v[:, 0] ** 2 + v[:, 2] ** 2)).sum() / max(mass, 1e-6))),
-            "max_speed": float(np.sqrt(sp2).max()) if len(act) else 0.0,
-            "momentum": (m[:, None] * v).sum(0)}
-
-
-def diag_snapshot(sim) -> dict:
-    p = sim.read_particles().astype(np.float64)
-    finite = bool(np.isfinite(p[:, [0, 1, 2, 3, 7]]).all())
-    act = p[p[:, 7] > INACTIVE]
-    y, j = act[:, 1], act[:, 3]
-    n = max(len(act), 1)
-    return {"all_finite": finite, "total_mass": float(act[:, 7].sum()), "active_count": len(act),
-            "y_extent": float(y.max() - y.min()) if len(act) else 0.0, "y_mean": float(y.sum() / n),
-            "mean_j": float(j.sum() / n), "min_j": float(j.min()) if len(act) else 0.0,
-            "max_j": float(j.max()) if len(act) else 0.0}
 
 
 # ---- shared pours ---------------------------------------------------------------------------
