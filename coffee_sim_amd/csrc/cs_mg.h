@@ -76,6 +76,24 @@ size_t cs_mg_wg_lds_bytes(const cs_mg_level* levels, int first, int nl);
 hipError_t cs_mg_wg_prepare(size_t lds_bytes);
 hipError_t cs_mg_wg_visit(hipStream_t s, const cs_mg_level* levels, int first, int nl, const cs_mg_op* ops,
                           uint32_t nops, const uint32_t* skip);
+// The same visit with each operator cell's row held in registers, for the
+// usual case of exactly two small levels (A = levels[first], B the coarsest):
+// k_mg_rows lists the rows once per cycle (after the Galerkin builds), by level
+// and colour: A's colour c at rows[c * |A|], B's at rows[2|A| + c * |B|], the
+// four counts in `counts`.  Same results as cs_mg_wg_visit, bit for bit.
+#define CS_MG_ROWS_THREADS 1024
+#define CS_MG_ROWS_A 4096u  // A's cells at most: two list entries per thread and colour
+#define CS_MG_ROWS_B 512u   // B's cells at most: one list entry per thread and colour
+struct cs_mg_row {
+  uint32_t xyz;  // x | y << 10 | z << 20
+  float d;
+  float w[6];    // face weights -x, +x, -y, +y, -z, +z (minus sides read from the neighbours)
+};
+bool cs_mg_rows_fit(const cs_mg_level* levels, int first, int nl);
+hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, cs_mg_row* rows, uint32_t* counts,
+                      const uint32_t* skip);
+hipError_t cs_mg_rows_visit(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
+                            const uint32_t* counts, const uint32_t* skip);
 // After a residual test (mode 0, into st->initial_residual): flags[0] = skip
 // the next cycle, flags[1] = the start before any cycle was within the
 // tolerance (`first` sets it).  Every MG kernel reads flags[0].

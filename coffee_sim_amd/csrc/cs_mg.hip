@@ -565,6 +565,207 @@ __global__ void __launch_bounds__(CS_MG_WG_THREADS) k_mg_wg_visit(mg_levels g, i
   for (uint32_t k = t; k < n; k += CS_MG_WG_THREADS) g.l[first].E[k] = F.E[k];
 }
 
+// ---- the two smallest levels' visit with the rows held in registers -------------------------
+// When the levels from the first one small enough for a workgroup are exactly
+// two (A, then the coarsest B), a visit of A is a fixed sequence and every
+// operation touches only cells of the operator, so each thread keeps the same
+// cells for the whole visit.  k_mg_rows writes, once per cycle, every operator
+// cell's row (coordinates, diagonal, the six face weights with the minus-side
+// ones read from the neighbours) into per-level, per-colour lists; the visit
+// loads its rows into registers and holds only the corrections in LDS, so a
+// pass is one LDS round trip for the six neighbour corrections and a barrier.
+// Same per-cell arithmetic as smooth_cell, restrict_cell and prolonged:
+//  - restriction: every A cell of the operator stores its residual, then each
+//    B cell sums its children's in the fixed order; a child outside the
+//    operator holds +0.0, which leaves the sum's bits as skipping it would
+//    (the sum starts at +0.0 and an exact zero sum of floats is +0.0);
+//  - prolongation as its own step (E += CS_MG_SCALE * parent's E for A's
+//    operator cells) before the red pass: the same values the folded form
+//    reads.
+__device__ __forceinline__ void mg_rows_build(const cs_mg_level& L, cs_mg_row* __restrict__ out, uint32_t* cnt,
+                                              uint32_t t) {
+  const uint32_t n = L.X * L.Y * L.Z, sy = L.X, sz = L.X * L.Y;
+  for (uint32_t k = t; k < n; k += CS_MG_ROWS_THREADS) {
+    const float d = L.D[k];
+    if (!(d > 0.0f)) continue;
+    uint32_t x, y, z;
+    level_cell(L, k, x, y, z);
+    const uint32_t c = (x + y + z) & 1u;
+    cs_mg_row r;
+    r.xyz = x | (y << 10) | (z << 20);
+    r.d = d;
+    r.w[0] = x > 0u ? L.Wx[k - 1] : 0.0f;
+    r.w[1] = L.Wx[k];
+    r.w[2] = y > 0u ? L.Wy[k - sy] : 0.0f;
+    r.w[3] = L.Wy[k];
+    r.w[4] = z > 0u ? L.Wz[k - sz] : 0.0f;
+    r.w[5] = L.Wz[k];
+    out[(size_t)c * n + atomicAdd(&cnt[c], 1u)] = r;
+  }
+}
+__global__ void __launch_bounds__(CS_MG_ROWS_THREADS) k_mg_rows(cs_mg_level A, cs_mg_level B,
+                                                                cs_mg_row* __restrict__ rows,
+                                                                uint32_t* __restrict__ counts,
+                                                                const uint32_t* __restrict__ skip) {
+  if (*skip) return;
+  __shared__ uint32_t cnt[4];
+  const uint32_t t = threadIdx.x;
+  if (t < 4u) cnt[t] = 0u;
+  __syncthreads();
+  const uint32_t nA = A.X * A.Y * A.Z;
+  mg_rows_build(A, rows, cnt, t);
+  mg_rows_build(B, rows + 2 * (size_t)nA, cnt + 2, t);
+  __syncthreads();
+  if (t < 4u) counts[t] = cnt[t];
+}
+
+// One operator cell of a thread: its row in registers.
+struct vcell {
+  uint32_t i;    // index in the level
+  uint32_t aux;  // A: the parent's index in B; B: packed coordinates
+  uint32_t fl;   // bit f set: the neighbour across face f (-x, +x, -y, +y, -z, +z) is on the grid
+  float d, w0, w1, w2, w3, w4, w5, r;
+  bool on;
+};
+__device__ __forceinline__ vcell mg_load_cell(const cs_mg_row* __restrict__ list, uint32_t count, uint32_t k,
+                                              const cs_mg_level& L, const cs_mg_level* P) {
+  vcell v{};
+  v.on = k < count;
+  if (!v.on) return v;
+  const cs_mg_row row = list[k];
+  const uint32_t x = row.xyz & 1023u, y = (row.xyz >> 10) & 1023u, z = row.xyz >> 20;
+  v.i = (z * L.Y + y) * L.X + x;
+  v.aux = P ? ((z >> 1) * P->Y + (y >> 1)) * P->X + (x >> 1) : row.xyz;
+  v.fl = (x > 0u ? 1u : 0u) | (x + 1u < L.X ? 2u : 0u) | (y > 0u ? 4u : 0u) | (y + 1u < L.Y ? 8u : 0u) |
+         (z > 0u ? 16u : 0u) | (z + 1u < L.Z ? 32u : 0u);
+  v.d = row.d;
+  v.w0 = row.w[0];
+  v.w1 = row.w[1];
+  v.w2 = row.w[2];
+  v.w3 = row.w[3];
+  v.w4 = row.w[4];
+  v.w5 = row.w[5];
+  return v;
+}
+// offsum against the corrections E (same terms and order as offsum)
+__device__ __forceinline__ float mg_vsum(const vcell& v, const sfloat* E, uint32_t sy, uint32_t sz) {
+  const uint32_t i = v.i;
+  const bool bxm = v.fl & 1u, bxp = v.fl & 2u, bym = v.fl & 4u, byp = v.fl & 8u, bzm = v.fl & 16u,
+             bzp = v.fl & 32u;
+  const float e0 = E[bxm ? i - 1u : i], e1 = E[bxp ? i + 1u : i], e2 = E[bym ? i - sy : i];
+  const float e3 = E[byp ? i + sy : i], e4 = E[bzm ? i - sz : i], e5 = E[bzp ? i + sz : i];
+  float s = wg_term(bxm, v.w0, e0);
+  s += wg_term(bxp, v.w1, e1);
+  s += wg_term(bym, v.w2, e2);
+  s += wg_term(byp, v.w3, e3);
+  s += wg_term(bzm, v.w4, e4);
+  s += wg_term(bzp, v.w5, e5);
+  return s;
+}
+__device__ __forceinline__ void mg_vsmooth(const vcell& v, sfloat* E, uint32_t sy, uint32_t sz) {
+  if (!v.on) return;
+  const float s = mg_vsum(v, E, sy, sz);
+  E[v.i] = (v.r + s) / v.d;
+}
+
+__global__ void __launch_bounds__(CS_MG_ROWS_THREADS) k_mg_rows_visit(cs_mg_level A, cs_mg_level B,
+                                                                      const cs_mg_row* __restrict__ rows,
+                                                                      const uint32_t* __restrict__ counts,
+                                                                      const uint32_t* __restrict__ skip) {
+  if (*skip) return;
+  __shared__ float s_ea[CS_MG_ROWS_A];
+  __shared__ float s_ra[CS_MG_ROWS_A];
+  __shared__ float s_eb[CS_MG_ROWS_B];
+  sfloat* const EA = (sfloat*)s_ea;
+  sfloat* const RA = (sfloat*)s_ra;
+  sfloat* const EB = (sfloat*)s_eb;
+  constexpr uint32_t NT = CS_MG_ROWS_THREADS;
+  const uint32_t t = threadIdx.x;
+  const uint32_t nA = A.X * A.Y * A.Z, nB = B.X * B.Y * B.Z;
+  const uint32_t ay = A.X, az = A.X * A.Y, by = B.X, bz = B.X * B.Y;
+  const cs_mg_row* const rowsB = rows + 2 * (size_t)nA;
+  // A: colour c, slot s at a[2c + s] (list entry t + s * NT); B: colour c at b[c]
+  vcell a[4], b[2];
+#pragma unroll
+  for (uint32_t c = 0; c < 2u; ++c) {
+#pragma unroll
+    for (uint32_t s = 0; s < 2u; ++s) {
+      a[2 * c + s] = mg_load_cell(rows + (size_t)c * nA, counts[c], t + s * NT, A, &B);
+      if (a[2 * c + s].on) a[2 * c + s].r = A.R[a[2 * c + s].i];  // the parent's restriction
+    }
+    b[c] = mg_load_cell(rowsB + (size_t)c * nB, counts[2 + c], t, B, nullptr);
+  }
+  for (uint32_t k = t; k < nA; k += NT) {
+    EA[k] = 0.0f;
+    RA[k] = 0.0f;  // cells outside the operator restrict as +0.0
+  }
+  for (uint32_t k = t; k < nB; k += NT) EB[k] = 0.0f;
+  __syncthreads();
+  auto passA = [&](uint32_t c) {
+    if (c == 0u) {
+      mg_vsmooth(a[0], EA, ay, az);
+      mg_vsmooth(a[1], EA, ay, az);
+    } else {
+      mg_vsmooth(a[2], EA, ay, az);
+      mg_vsmooth(a[3], EA, ay, az);
+    }
+    __syncthreads();
+  };
+  auto passB = [&](uint32_t c) {
+    mg_vsmooth(c == 0u ? b[0] : b[1], EB, by, bz);
+    __syncthreads();
+  };
+  for (uint32_t k = 0; k < CS_MG_NU; ++k) {
+    passA(0u);
+    passA(1u);
+  }
+  for (uint32_t g = 0; g < CS_MG_GAMMA; ++g) {
+    // residuals of A's operator cells, then their sums into B's cells
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const vcell& v = a[q];
+      if (v.on) {
+        const float s = mg_vsum(v, EA, ay, az);
+        RA[v.i] = v.r - (v.d * EA[v.i] - s);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      vcell& v = b[c];
+      if (v.on) {
+        const uint32_t X = v.aux & 1023u, Y = (v.aux >> 10) & 1023u, Z = v.aux >> 20;
+        float acc = 0.0f;
+        for (uint32_t dz = 0; dz < 2u; ++dz)
+          for (uint32_t dy = 0; dy < 2u; ++dy)
+            for (uint32_t dx = 0; dx < 2u; ++dx) {
+              const uint32_t x = 2u * X + dx, y = 2u * Y + dy, z = 2u * Z + dz;
+              if (x >= A.X || y >= A.Y || z >= A.Z) continue;
+              acc += RA[(z * A.Y + y) * A.X + x];
+            }
+        v.r = acc;
+        EB[v.i] = 0.0f;
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < CS_MG_COARSE; ++k) {
+      passB(0u);
+      passB(1u);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const vcell& v = a[q];
+      if (v.on) EA[v.i] = EA[v.i] + CS_MG_SCALE * EB[v.aux];
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < CS_MG_NU; ++k) {
+      passA(0u);
+      passA(1u);
+    }
+  }
+  for (uint32_t k = t; k < nA; k += NT) A.E[k] = EA[k];
+}
+
 // flags[0]: skip the next cycle; flags[1]: the start was within the tolerance
 // (no cycle ran); flags[2]: cycles run so far.  The first gate sets all three,
 // later ones count the cycle just run and set flags[0].
@@ -698,6 +899,24 @@ hipError_t cs_mg_wg_visit(hipStream_t s, const cs_mg_level* levels, int first, i
   for (int l = 0; l < CS_MG_MAX_LEVELS; ++l) lv.l[l] = levels[l];
   hipLaunchKernelGGL(k_mg_wg_visit, dim3(1), dim3(CS_MG_WG_THREADS), cs_mg_wg_lds_bytes(levels, first, nl), s, lv,
                      first, nl, ops, nops, skip);
+  return hipGetLastError();
+}
+bool cs_mg_rows_fit(const cs_mg_level* levels, int first, int nl) {
+  if (nl - first != 2) return false;
+  const cs_mg_level& A = levels[first];
+  const cs_mg_level& B = levels[first + 1];
+  if (A.X >= 1024u || A.Y >= 1024u || A.Z >= 1024u) return false;
+  // every colour list within two entries per thread (A) and one (B)
+  return cells(A) <= CS_MG_ROWS_A && cells(B) <= CS_MG_ROWS_B;
+}
+hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, cs_mg_row* rows, uint32_t* counts,
+                      const uint32_t* skip) {
+  hipLaunchKernelGGL(k_mg_rows, dim3(1), dim3(CS_MG_ROWS_THREADS), 0, s, A, B, rows, counts, skip);
+  return hipGetLastError();
+}
+hipError_t cs_mg_rows_visit(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
+                            const uint32_t* counts, const uint32_t* skip) {
+  hipLaunchKernelGGL(k_mg_rows_visit, dim3(1), dim3(CS_MG_ROWS_THREADS), 0, s, A, B, rows, counts, skip);
   return hipGetLastError();
 }
 hipError_t cs_mg_gate(hipStream_t s, const cs_solve_state* st, double tol, uint32_t* flags, bool first) {

@@ -65,7 +65,7 @@ void free_sim(cs_sim* m) {
                  m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt,
                  m->bed_snap,   m->alias4,
                  m->persist_fb, m->persist_pflags, m->persist_partials, m->persist_slot, m->persist_xcc,
-                 m->mg_buf,     m->mg_flags,   m->mg_ops};
+                 m->mg_buf,     m->mg_flags,   m->mg_ops,     m->mg_row_buf, m->mg_row_counts};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -358,6 +358,13 @@ int ensure_mg(cs_sim* m) {
     CS_HIP(dalloc(&m->mg_ops, ops.size()));
     CS_HIP(hipMemcpy(m->mg_ops, ops.data(), ops.size() * sizeof(cs_mg_op), hipMemcpyHostToDevice));
     m->mg_nops = (uint32_t)ops.size();
+    m->mg_rows = m->mg_rows_allowed && cs_mg_rows_fit(lv.data(), m->mg_wg_first, nl);
+    if (m->mg_rows) {
+      const size_t rows = 2 * ((size_t)lv[m->mg_wg_first].X * lv[m->mg_wg_first].Y * lv[m->mg_wg_first].Z +
+                               (size_t)lv[nl - 1].X * lv[nl - 1].Y * lv[nl - 1].Z);
+      CS_HIP(dalloc(&m->mg_row_buf, rows));
+      CS_HIP(dalloc(&m->mg_row_counts, 4));
+    }
   }
   for (auto& e : m->mg_ev) CS_HIP(hipEventCreate(&e));
   m->mg_levels = lv;
@@ -370,6 +377,10 @@ int mg_visit(cs_sim* m, const cs_params& u, uint32_t nactive, int l) {
   hipStream_t st = m->ctx->stream;
   const int nl = (int)m->mg_levels.size();
   const uint32_t* skip = m->mg_flags;
+  if (l >= m->mg_wg_first && m->mg_rows) {
+    CS_HIP(cs_mg_rows_visit(st, m->mg_levels[l], m->mg_levels[l + 1], m->mg_row_buf, m->mg_row_counts, skip));
+    return CS_OK;
+  }
   if (l >= m->mg_wg_first) {
     cs_mg_level lv[CS_MG_MAX_LEVELS] = {};
     for (int k = 0; k < nl; ++k) lv[k] = m->mg_levels[k];
@@ -431,6 +442,9 @@ int mg_start(cs_sim* m, const cs_params& u, uint32_t nactive) {
     CS_HIP(cs_mg_fine_setup(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, L0, skip));
     if (nl > 1) CS_HIP(cs_mg_fine_build(st, u, nactive, m->brick_list, L0, m->mg_levels[1], skip));
     for (int l = 1; l + 1 < nl; ++l) CS_HIP(cs_mg_build(st, m->mg_levels[l], m->mg_levels[l + 1], skip));
+    if (m->mg_rows)
+      CS_HIP(cs_mg_rows(st, m->mg_levels[m->mg_wg_first], m->mg_levels[nl - 1], m->mg_row_buf, m->mg_row_counts,
+                        skip));
     rc = mg_visit(m, u, nactive, 0);
     if (rc) return rc;
     CS_HIP(cs_mg_fine_apply(st, u, nactive, m->brick_list, L0, m->pres, skip));
@@ -1017,6 +1031,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (const char* v = getenv("CS_RBGS_FUSE")) m->fuse_residual = atoi(v) != 0;
   if (const char* v = getenv("CS_GRAPHS")) m->use_graphs = atoi(v) != 0;
   if (const char* v = getenv("CS_P2G")) m->p2g_tiled = strcmp(v, "atomic") != 0;
+  if (const char* v = getenv("CS_MG_ROWS")) m->mg_rows_allowed = atoi(v) != 0;
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->ctx->device) == hipSuccess && cus > 0)

@@ -101,6 +101,10 @@ struct cs_sim {
   cs_mg_op* mg_ops = nullptr;          // one visit of level mg_wg_first, run by one workgroup
   uint32_t mg_nops = 0;
   int mg_wg_first = 0;
+  bool mg_rows_allowed = true;         // CS_MG_ROWS=0: always the operation-list visit (cross-check)
+  bool mg_rows = false;                // the small levels' visit keeps its rows in registers (cs_mg_rows_visit)
+  cs_mg_row* mg_row_buf = nullptr;     // their row lists, rebuilt every cycle
+  uint32_t* mg_row_counts = nullptr;   // [4]
   hipEvent_t mg_ev[2] = {nullptr, nullptr};
   bool mg_timed = false;               // mg_ev holds an unharvested start
   // tile-binned P2G scratch

@@ -11,9 +11,10 @@ CPU, against what the mathematics fixes:
 - on the 64^3 pour, the start cuts the iterations to the 1e-4 tolerance by
   more than 10x.
 GPU: the device start (fine level over the active bricks, whole-grid middle
-levels, the small levels inside one workgroup) equals the oracle's bit for
-bit: iteration counts and the whole state, from zero, previous-step and
-external starts, on the persistent and the launch-per-chunk solve."""
+levels, the small levels inside one workgroup, with their rows in registers
+or from the operation list) equals the oracle's bit for bit: iteration counts
+and the whole state, from zero, previous-step and external starts, on the
+persistent and the launch-per-chunk solve."""
 from __future__ import annotations
 
 import numpy as np
@@ -207,11 +208,16 @@ def _pair(ctx, orc, s, scene):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("visit", ["rows", "oplist"])
 @pytest.mark.parametrize("layout", ["persistent", "launches"])
 @pytest.mark.parametrize("scale", [4, 2])
-def test_mg_start_matches_oracle(ctx, orc, monkeypatch, layout, scale):
+def test_mg_start_matches_oracle(ctx, orc, monkeypatch, layout, scale, visit):
+    """visit: the small levels' visit with its rows in registers (the default
+    on these grids), or the operation-list visit (CS_MG_ROWS=0)."""
     if layout == "launches":
         monkeypatch.setenv("CS_RBGS_PERSIST", "0")
+    if visit == "oplist":
+        monkeypatch.setenv("CS_MG_ROWS", "0")
     w = cs.SeededWorkload("v60_working", seed=6, scale=scale)
     nc = int(np.prod(list(w.settings.grid_dims)))
     starts = (cs.abi.CS_WARM_ZERO, cs.abi.CS_WARM_PREVIOUS, cs.abi.CS_WARM_EXTERNAL) if scale == 4 else \
@@ -233,9 +239,9 @@ def test_mg_start_matches_oracle(ctx, orc, monkeypatch, layout, scale):
             sim.step_frame(w.frame_dt)
             ref.step_frame(w.frame_dt)
             sg, so = sim.solve_stats(), ref.solve_stats()
-            assert sg.iterations == so.iterations, (layout, scale, start, k, sg.as_dict(), so.as_dict())
+            assert sg.iterations == so.iterations, (layout, visit, scale, start, k, sg.as_dict(), so.as_dict())
             assert sg.converged == so.converged == 1
-            compare_state(sim, ref, f"mg start {layout} scale {scale} start {start} substep {k}")
+            compare_state(sim, ref, f"mg start {layout} {visit} scale {scale} start {start} substep {k}")
         sim.close()
         ref.close()
 
