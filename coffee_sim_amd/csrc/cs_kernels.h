@@ -39,6 +39,10 @@ hipError_t cs_launch_project(hipStream_t stream, const cs_params& u, float* gvel
 // and 6), from classify's per-cell target `tgt`.
 hipError_t cs_launch_post_div(hipStream_t stream, const cs_params& u, const float* gvel, const float* kf,
                               const uint8_t* cls, const float* sdf, const float* tgt, uint32_t* metrics);
+// The same over the `nactive` bricks of `list` (every fluid cell lies in an active brick).
+hipError_t cs_launch_post_div_bricks(hipStream_t stream, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                                     const float* gvel, const float* kf, const uint8_t* cls, const float* sdf,
+                                     const float* tgt, uint32_t* metrics);
 hipError_t cs_launch_pack(hipStream_t stream, const cs_params& u, float* gvel, const float* kf, const int32_t* pack,
                           const float* sdf);
 // alias4 (may be null): per cell 1 when the viscosity scratch's z lane holds

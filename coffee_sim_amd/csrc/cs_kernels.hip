@@ -69,20 +69,42 @@ __device__ __forceinline__ float cell_divergence(const cs_params& u, const float
   float4 sv = gvel[idx];
   float vxm = -sv.x, vxp = -sv.x, vym = -sv.y, vyp = -sv.y, vzm = -sv.z, vzp = -sv.z;
   uint32_t plane = u.gx * u.gy;
-  float d = u.dx;
-#define CS_OPEN(nidx, px, py, pz) (cache ? !cls[nidx] : sample_sdf(u, sdf, f3(px, py, pz)) >= 0.0f)
-  if (ix > 0u && CS_OPEN(idx - 1u, cc.x + -d, cc.y + 0.0f, cc.z + 0.0f)) vxm = gvel[idx - 1u].x;
-  if (ix + 1u < u.gx && CS_OPEN(idx + 1u, cc.x + d, cc.y + 0.0f, cc.z + 0.0f)) vxp = gvel[idx + 1u].x;
-  if (iy > 0u && CS_OPEN(idx - u.gx, cc.x + 0.0f, cc.y + -d, cc.z + 0.0f)) vym = gvel[idx - u.gx].y;
-  if (iy + 1u < u.gy && CS_OPEN(idx + u.gx, cc.x + 0.0f, cc.y + d, cc.z + 0.0f)) vyp = gvel[idx + u.gx].y;
-  if (iz > 0u && CS_OPEN(idx - plane, cc.x + 0.0f, cc.y + 0.0f, cc.z + -d)) vzm = gvel[idx - plane].z;
-  if (iz + 1u < u.gz && CS_OPEN(idx + plane, cc.x + 0.0f, cc.y + 0.0f, cc.z + d)) vzp = gvel[idx + plane].z;
+  const bool bxm = ix > 0u, bxp = ix + 1u < u.gx, bym = iy > 0u, byp = iy + 1u < u.gy, bzm = iz > 0u,
+             bzp = iz + 1u < u.gz;
+  if (cache) {
+    // every neighbour's class byte and velocity component loaded at once (indices
+    // clamped onto the cell itself off the grid), then selected
+    const uint32_t nxm = bxm ? idx - 1u : idx, nxp = bxp ? idx + 1u : idx, nym = bym ? idx - u.gx : idx;
+    const uint32_t nyp = byp ? idx + u.gx : idx, nzm = bzm ? idx - plane : idx, nzp = bzp ? idx + plane : idx;
+    const float* gf = reinterpret_cast<const float*>(gvel);
+    const uint8_t cxm = cls[nxm], cxp = cls[nxp], cym = cls[nym], cyp = cls[nyp], czm = cls[nzm], czp = cls[nzp];
+    const float gxm = gf[4 * (size_t)nxm], gxp = gf[4 * (size_t)nxp], gym = gf[4 * (size_t)nym + 1];
+    const float gyp = gf[4 * (size_t)nyp + 1], gzm = gf[4 * (size_t)nzm + 2], gzp = gf[4 * (size_t)nzp + 2];
+    if (bxm && !cxm) vxm = gxm;
+    if (bxp && !cxp) vxp = gxp;
+    if (bym && !cym) vym = gym;
+    if (byp && !cyp) vyp = gyp;
+    if (bzm && !czm) vzm = gzm;
+    if (bzp && !czp) vzp = gzp;
+  } else {
+    float d = u.dx;
+#define CS_OPEN(px, py, pz) (sample_sdf(u, sdf, f3(px, py, pz)) >= 0.0f)
+    if (bxm && CS_OPEN(cc.x + -d, cc.y + 0.0f, cc.z + 0.0f)) vxm = gvel[idx - 1u].x;
+    if (bxp && CS_OPEN(cc.x + d, cc.y + 0.0f, cc.z + 0.0f)) vxp = gvel[idx + 1u].x;
+    if (bym && CS_OPEN(cc.x + 0.0f, cc.y + -d, cc.z + 0.0f)) vym = gvel[idx - u.gx].y;
+    if (byp && CS_OPEN(cc.x + 0.0f, cc.y + d, cc.z + 0.0f)) vyp = gvel[idx + u.gx].y;
+    if (bzm && CS_OPEN(cc.x + 0.0f, cc.y + 0.0f, cc.z + -d)) vzm = gvel[idx - plane].z;
+    if (bzp && CS_OPEN(cc.x + 0.0f, cc.y + 0.0f, cc.z + d)) vzp = gvel[idx + plane].z;
 #undef CS_OPEN
+  }
   float div = 0.5f * u.inv_dx * ((vxp - vxm) + (vyp - vym) + (vzp - vzm));
   return div;
 }
 
 // ---- classify_cells (shader.rs:1181-1367) + packing_prepare (shader.rs:1566-1583) ----
+struct cs_cls_metrics {
+  uint32_t max_div, max_res, fluid, div_clamp, p_clamp;
+};
 // Writes kind, the kind/fill word, pressure 0, the right-hand side c = dx*dx*rhs
 // of the pressure update (shader.rs:1436-1437, from the quantised divergence)
 // and the packing pressure.
@@ -94,7 +116,7 @@ __device__ __forceinline__ void classify_cell(const cs_params& u, const float4* 
                                                   float* __restrict__ rhs, int32_t* __restrict__ pack,
                                                   float* __restrict__ tgt,
                                                   uint32_t* __restrict__ metrics, const float* __restrict__ sdf,
-                                                  uint32_t* __restrict__ brick_flag, uint32_t idx) {
+                                                  uint32_t* __restrict__ brick_flag, uint32_t idx, cs_cls_metrics& cm) {
   if (idx >= u.nc) return;
   float occ = u.nominal_mass * 0.1f;
   float mass = gvel[idx].w;
@@ -114,16 +136,19 @@ __device__ __forceinline__ void classify_cell(const cs_params& u, const float4* 
     return;
   }
   int kind;
-  if (bed_id[idx] >= 0) {
-    kind = CS_CELL_BED;
-  } else {
-    bool air = false;
-    for (int n = 0; n < 6; ++n) {
-      int x = (int)ix + c_off6[n][0], y = (int)iy + c_off6[n][1], z = (int)iz + c_off6[n][2];
-      if (!on_grid(u, x, y, z)) { air = true; break; }
-      if (gvel[cell_index(u, x, y, z)].w <= occ) { air = true; break; }
-    }
-    kind = air ? CS_CELL_SURFACE : CS_CELL_INTERIOR;
+  const int32_t my_bed = bed_id[idx];
+  {
+    // a face off the grid or toward a low-mass neighbour makes a surface cell:
+    // the six neighbour masses loaded at once (indices clamped onto the cell)
+    const uint32_t plane = u.gx * u.gy;
+    const bool bxm = ix > 0u, bxp = ix + 1u < u.gx, bym = iy > 0u, byp = iy + 1u < u.gy, bzm = iz > 0u,
+               bzp = iz + 1u < u.gz;
+    const float m0 = gvel[bxm ? idx - 1u : idx].w, m1 = gvel[bxp ? idx + 1u : idx].w;
+    const float m2 = gvel[bym ? idx - u.gx : idx].w, m3 = gvel[byp ? idx + u.gx : idx].w;
+    const float m4 = gvel[bzm ? idx - plane : idx].w, m5 = gvel[bzp ? idx + plane : idx].w;
+    const bool air = !bxm || !bxp || !bym || !byp || !bzm || !bzp || m0 <= occ || m1 <= occ || m2 <= occ ||
+                     m3 <= occ || m4 <= occ || m5 <= occ;
+    kind = my_bed >= 0 ? CS_CELL_BED : (air ? CS_CELL_SURFACE : CS_CELL_INTERIOR);
   }
   float div = cell_divergence(u, gvel, cls, sdf, idx, ix, iy, iz, cc);
   float rest = (float)lanes[4u * u.nc + idx] * u.inv_fp_scale;
@@ -131,7 +156,7 @@ __device__ __forceinline__ void classify_cell(const cs_params& u, const float4* 
   float target = volume_target_div(u, rest, cur);
   float cv = u.dx * u.dx * u.dx;
   if (kind == CS_CELL_BED) {
-    const cs_bed_cell& b = bed[bed_id[idx]];
+    const cs_bed_cell& b = bed[my_bed];
     // bed_pore_projection_target_divergence (shader.rs:621-666) with the
     // particle-independent factors taken from the bed table
     float pore = target;
@@ -167,13 +192,59 @@ __device__ __forceinline__ void classify_cell(const cs_params& u, const float4* 
     fires_p = pcl != pv ? 1u : 0u;
     pack[idx] = cs_f2i(pcl * u.fp_scale);
   }
-  uint32_t fpd = cs_f2u(cs_clamp(fabsf(div) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f));
-  atomicMax(&metrics[CS_M_MAX_ABS_DIV], fpd);
-  atomicMax(&metrics[CS_M_PRE_MAX_ABS_RES], cs_f2u(cs_clamp(fabsf(dval) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f)));
-  atomicAdd(&metrics[CS_M_FLUID_CELLS], 1u);
+  cm.max_div = cs_f2u(cs_clamp(fabsf(div) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f));
+  cm.max_res = cs_f2u(cs_clamp(fabsf(dval) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f));
+  cm.fluid = 1u;
+  cm.div_clamp = fires_d;
+  cm.p_clamp = fires_p;
   brick_flag[((iz / CS_BRICK) * u.nby + iy / CS_BRICK) * u.nbx + ix / CS_BRICK] = 1u;
-  if (fires_d) atomicAdd(&metrics[CS_M_DIV_CLAMP], 1u);
-  if (fires_p) atomicAdd(&metrics[CS_M_P_CLAMP], 1u);
+}
+// classify_cell for one cell per lane, then its metrics (max |div|, max
+// |div - target|, fluid cells, clamp fires) reduced over the wavefront and
+// added with one atomic per slot and workgroup (of at most 256 threads): the
+// slots are a max and counts, so the result does not depend on how the cells
+// were grouped.  Every thread of the workgroup must call it.
+__device__ __forceinline__ void classify_wave(const cs_params& u, const float4* __restrict__ gvel,
+                                              const uint8_t* __restrict__ cls, const int32_t* __restrict__ lanes,
+                                              const int32_t* __restrict__ bed_id,
+                                              const cs_bed_cell* __restrict__ bed, int8_t* __restrict__ kind_out,
+                                              float* __restrict__ kf_out, int32_t* __restrict__ pres,
+                                              float* __restrict__ rhs, int32_t* __restrict__ pack,
+                                              float* __restrict__ tgt, uint32_t* __restrict__ metrics,
+                                              const float* __restrict__ sdf, uint32_t* __restrict__ brick_flag,
+                                              uint32_t idx) {
+  cs_cls_metrics cm{};
+  classify_cell(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, tgt, metrics, sdf, brick_flag,
+                idx, cm);
+  for (int off = 32; off > 0; off >>= 1) {
+    cm.max_div = max(cm.max_div, (uint32_t)__shfl_xor((int)cm.max_div, off));
+    cm.max_res = max(cm.max_res, (uint32_t)__shfl_xor((int)cm.max_res, off));
+    cm.fluid += (uint32_t)__shfl_xor((int)cm.fluid, off);
+    cm.div_clamp += (uint32_t)__shfl_xor((int)cm.div_clamp, off);
+    cm.p_clamp += (uint32_t)__shfl_xor((int)cm.p_clamp, off);
+  }
+  // then over the workgroup's wavefronts: atomics on one address are
+  // serialised device-wide, so one per workgroup and slot
+  __shared__ cs_cls_metrics s_cm[4];
+  const uint32_t wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0u) s_cm[wave] = cm;
+  __syncthreads();
+  if (threadIdx.x == 0u) {
+    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
+      cm.max_div = max(cm.max_div, s_cm[w].max_div);
+      cm.max_res = max(cm.max_res, s_cm[w].max_res);
+      cm.fluid += s_cm[w].fluid;
+      cm.div_clamp += s_cm[w].div_clamp;
+      cm.p_clamp += s_cm[w].p_clamp;
+    }
+    if (cm.fluid) {
+      atomicMax(&metrics[CS_M_MAX_ABS_DIV], cm.max_div);
+      atomicMax(&metrics[CS_M_PRE_MAX_ABS_RES], cm.max_res);
+      atomicAdd(&metrics[CS_M_FLUID_CELLS], cm.fluid);
+      if (cm.div_clamp) atomicAdd(&metrics[CS_M_DIV_CLAMP], cm.div_clamp);
+      if (cm.p_clamp) atomicAdd(&metrics[CS_M_P_CLAMP], cm.p_clamp);
+    }
+  }
 }
 __global__ void __launch_bounds__(256) k_classify(cs_params u, const float4* __restrict__ gvel,
                                                   const uint8_t* __restrict__ cls, const int32_t* __restrict__ lanes,
@@ -184,7 +255,7 @@ __global__ void __launch_bounds__(256) k_classify(cs_params u, const float4* __r
                                                   float* __restrict__ tgt,
                                                   uint32_t* __restrict__ metrics, const float* __restrict__ sdf,
                                                   uint32_t* __restrict__ brick_flag) {
-  classify_cell(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, tgt, metrics, sdf, brick_flag,
+  classify_wave(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, tgt, metrics, sdf, brick_flag,
                 blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -417,21 +488,34 @@ __global__ void __launch_bounds__(256) k_project(cs_params u, float4* __restrict
 __device__ __forceinline__ void post_div_cell(const cs_params& u, const float4* __restrict__ gvel,
                                               const float* __restrict__ kf, const uint8_t* __restrict__ cls,
                                               const float* __restrict__ sdf, const float* __restrict__ tgt,
-                                              uint32_t* __restrict__ metrics, uint32_t idx) {
-  uint32_t fd = 0u, fr = 0u;
+                                              uint32_t idx, uint32_t& fd, uint32_t& fr) {
   if (idx < u.nc && kf_fluid(kf[idx])) {
     uint32_t ix, iy, iz;
     decompose(u, idx, ix, iy, iz);
     const float div = cell_divergence(u, gvel, cls, sdf, idx, ix, iy, iz, cell_center(u, ix, iy, iz));
-    fd = cs_f2u(cs_clamp(fabsf(div) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f));
-    fr = cs_f2u(cs_clamp(fabsf(div - tgt[idx]) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f));
+    fd = max(fd, cs_f2u(cs_clamp(fabsf(div) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f)));
+    fr = max(fr, cs_f2u(cs_clamp(fabsf(div - tgt[idx]) * u.metrics_div_fp_scale, 0.0f, 2147483648.0f)));
   }
-  // every lane of the wavefront takes part: one atomic per wavefront and slot
+}
+// The maxima of the workgroup (of at most 256 threads) into metric slots 5 and
+// 6, one atomic each.  Every thread of the workgroup must call it.
+__device__ __forceinline__ void post_div_flush(uint32_t* __restrict__ metrics, uint32_t fd, uint32_t fr) {
   for (int off = 32; off > 0; off >>= 1) {
     fd = max(fd, (uint32_t)__shfl_xor((int)fd, off));
     fr = max(fr, (uint32_t)__shfl_xor((int)fr, off));
   }
+  __shared__ uint32_t s_f[2][4];
+  const uint32_t wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63u) == 0u) {
+    s_f[0][wave] = fd;
+    s_f[1][wave] = fr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0u) {
+    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
+      fd = max(fd, s_f[0][w]);
+      fr = max(fr, s_f[1][w]);
+    }
     if (fd) atomicMax(&metrics[CS_M_POST_MAX_ABS_DIV], fd);
     if (fr) atomicMax(&metrics[CS_M_POST_MAX_ABS_RES], fr);
   }
@@ -440,7 +524,25 @@ __global__ void __launch_bounds__(256) k_post_div(cs_params u, const float4* __r
                                                   const float* __restrict__ kf, const uint8_t* __restrict__ cls,
                                                   const float* __restrict__ sdf, const float* __restrict__ tgt,
                                                   uint32_t* __restrict__ metrics) {
-  post_div_cell(u, gvel, kf, cls, sdf, tgt, metrics, blockIdx.x * blockDim.x + threadIdx.x);
+  uint32_t fd = 0u, fr = 0u;
+  post_div_cell(u, gvel, kf, cls, sdf, tgt, blockIdx.x * blockDim.x + threadIdx.x, fd, fr);
+  post_div_flush(metrics, fd, fr);
+}
+// The same over the active bricks only (every fluid cell lies in one): one
+// workgroup of 256 per brick, two cells a thread.
+__global__ void __launch_bounds__(256) k_post_div_bricks(cs_params u, const uint32_t* __restrict__ list,
+                                                         const float4* __restrict__ gvel, const float* __restrict__ kf,
+                                                         const uint8_t* __restrict__ cls, const float* __restrict__ sdf,
+                                                         const float* __restrict__ tgt, uint32_t* __restrict__ metrics) {
+  const uint32_t b = list[blockIdx.x];
+  const uint32_t x0 = (b % u.nbx) * CS_BRICK, y0 = ((b / u.nbx) % u.nby) * CS_BRICK, z0 = (b / (u.nbx * u.nby)) * CS_BRICK;
+  uint32_t fd = 0u, fr = 0u;
+  for (uint32_t k = threadIdx.x; k < (uint32_t)(CS_BRICK * CS_BRICK * CS_BRICK); k += 256u) {
+    const uint32_t x = x0 + (k % CS_BRICK), y = y0 + ((k / CS_BRICK) % CS_BRICK), z = z0 + k / (CS_BRICK * CS_BRICK);
+    const uint32_t idx = x < u.gx && y < u.gy && z < u.gz ? cell_index(u, x, y, z) : u.nc;  // u.nc: no cell
+    post_div_cell(u, gvel, kf, cls, sdf, tgt, idx, fd, fr);
+  }
+  post_div_flush(metrics, fd, fr);
 }
 
 // ---- packing_apply + boundary_project (shader.rs:1585-1626, 1630-1679) ------------
@@ -1321,7 +1423,7 @@ __global__ void __launch_bounds__(256) k_batch_grid(const cs_scene_dev* __restri
 __global__ void __launch_bounds__(256) k_batch_classify(const cs_scene_dev* __restrict__ sc,
                                                         const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_CELLS);
-  classify_cell(u, reinterpret_cast<const float4*>(d.gv), d.cls, d.lanes, d.bed_id, d.bed, d.kind, d.kf, d.pres,
+  classify_wave(u, reinterpret_cast<const float4*>(d.gv), d.cls, d.lanes, d.bed_id, d.bed, d.kind, d.kf, d.pres,
                 d.rhs, d.pack, d.tgt, d.metrics, d.sdf, d.brick_flag, tid);
 }
 
@@ -1336,7 +1438,9 @@ __global__ void __launch_bounds__(256) k_batch_project(const cs_scene_dev* __res
 __global__ void __launch_bounds__(256) k_batch_post_div(const cs_scene_dev* __restrict__ sc,
                                                         const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_CELLS);
-  post_div_cell(u, reinterpret_cast<const float4*>(d.gv), d.kf, d.cls, d.sdf, d.tgt, d.metrics, tid);
+  uint32_t fd = 0u, fr = 0u;
+  post_div_cell(u, reinterpret_cast<const float4*>(d.gv), d.kf, d.cls, d.sdf, d.tgt, tid, fd, fr);
+  post_div_flush(d.metrics, fd, fr);
 }
 
 __global__ void __launch_bounds__(256) k_batch_pack(const cs_scene_dev* __restrict__ sc,
@@ -1431,6 +1535,14 @@ hipError_t cs_launch_post_div(hipStream_t s, const cs_params& u, const float* gv
                               const float* sdf, const float* tgt, uint32_t* metrics) {
   hipLaunchKernelGGL(k_post_div, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<const float4*>(gvel), kf,
                      cls, sdf, tgt, metrics);
+  return hipGetLastError();
+}
+hipError_t cs_launch_post_div_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+                                     const float* gvel, const float* kf, const uint8_t* cls, const float* sdf,
+                                     const float* tgt, uint32_t* metrics) {
+  if (!nactive) return hipSuccess;
+  hipLaunchKernelGGL(k_post_div_bricks, dim3(nactive), dim3(256), 0, s, u, list, reinterpret_cast<const float4*>(gvel),
+                     kf, cls, sdf, tgt, metrics);
   return hipGetLastError();
 }
 hipError_t cs_launch_pack(hipStream_t s, const cs_params& u, float* gvel, const float* kf, const int32_t* pack,

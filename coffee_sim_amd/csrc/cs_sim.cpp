@@ -578,6 +578,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
   CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
   CS_HIP(hipStreamSynchronize(st));
   const uint32_t nactive = m->h_state->active_bricks;
+  m->nactive = nactive;
   if (m->persist && !m->s.rbgs_rescale_after && nactive > 0 && nactive <= CS_PERSIST_MAX_BRICKS &&
       cs_persist_groups(nactive) <= m->persist_max_groups &&
       (!fixed || m->s.pressure_rbgs_pairs > 0)) {
@@ -796,7 +797,11 @@ int substep(cs_sim* m, float sub_dt) {
   // (bed_impulse_add, shader.rs:1484-1491 and 1870-1878)
   const cs_grains gr = m->num_bed ? cs_grains{m->bed_lookup, m->bed_aff, m->bed_dw, m->num_bed} : cs_grains{};
   CS_HIP(cs_launch_project(st, u, m->gv, m->kf, m->pres, m->bed_id, m->bed, m->sdf, gr));
-  CS_HIP(cs_launch_post_div(st, u, m->gv, m->kf, m->cls, m->sdf, m->tgt, m->metrics));
+  if (m->dense_solver)  // no brick list: the whole grid
+    CS_HIP(cs_launch_post_div(st, u, m->gv, m->kf, m->cls, m->sdf, m->tgt, m->metrics));
+  else
+    CS_HIP(cs_launch_post_div_bricks(st, u, m->nactive, m->brick_list, m->gv, m->kf, m->cls, m->sdf, m->tgt,
+                                     m->metrics));
   CS_HIP(cs_launch_pack(st, u, m->gv, m->kf, m->pack, m->sdf));
   CS_HIP(cs_launch_visc(st, u, m->gv, m->gv2, m->cls, m->sdf, m->alias4));
   mark(m, P_G2P);

@@ -101,6 +101,7 @@ struct cs_sim {
   cs_mg_op* mg_ops = nullptr;          // one visit of level mg_wg_first, run by one workgroup
   uint32_t mg_nops = 0;
   int mg_wg_first = 0;
+  uint32_t nactive = 0;                // active bricks of the current substep's solve (brick_list)
   bool mg_rows_allowed = true;         // CS_MG_ROWS=0: always the operation-list visit (cross-check)
   bool mg_rows = false;                // the small levels' visit keeps its rows in registers (cs_mg_rows_visit)
   cs_mg_row* mg_row_buf = nullptr;     // their row lists, rebuilt every cycle
