@@ -4,8 +4,9 @@
 // Restates p2g (shader.rs:955-1052).  The reference adds every particle's 162
 // fixed-point terms (27 nodes x 6 lanes) to the grid with global i32 atomics.
 // Here each active particle is binned by the tile of its base node
-// floor(g - 0.5): a wave-aggregated atomic on the tile counter gives its rank
-// in the tile, a one-block scan turns counts into offsets, and a scatter
+// floor(g - 0.5): counts staged per workgroup in LDS, then one global atomic
+// per workgroup and tile, give its rank in the tile; a one-block scan turns
+// counts into offsets, and a scatter
 // writes the tile-sorted particle list.  One workgroup per tile then adds the
 // terms of its particles into a 10^3-node x 6-lane LDS block (nodes base-tile
 // origin + [0, 10) in each axis) and flushes the nonzero entries with one
@@ -36,40 +37,85 @@ __device__ __forceinline__ uint32_t tile_of(const cs_params& u, int bx, int by, 
 
 }  // namespace
 
-// Pass 1: tile and in-tile rank of every particle.  Lanes of a wavefront that
-// share a tile take one atomic between them (the lowest such lane adds the
-// group's count and hands out consecutive ranks).
-// Every lane of the wavefront must call it (the ranks are wave-aggregated).
-__device__ __forceinline__ void p2g_bin_particle(const cs_params& u, const float4* __restrict__ part,
-                                                 const float4* __restrict__ aff, uint32_t* __restrict__ tile_count,
-                                                 uint2* __restrict__ key, uint32_t pid) {
-  bool active = false;
-  uint32_t tile = kNoTile;
-  if (pid < u.num_particles) {
-    P2GParticle q;
-    active = p2g_setup(u, part, aff, pid, q);
-    if (active) tile = tile_of(u, q.bx, q.by, q.bz);
+// Pass 1: tile and in-tile rank of every particle, R particles per thread
+// (pid = first + r * 256 + thread).  Lanes of a wavefront that share a tile
+// form a group; each group adds to a slot of a small LDS table keyed by
+// tile, then one global atomic per slot reserves the workgroup's run of ranks
+// in that tile (an atomic on one address is serialised device-wide, and
+// neighbouring particles share tiles).  A group that finds the table full takes its ranks
+// from the tile counter directly.  Ranks stay unique and dense per tile.
+constexpr uint32_t kBinSlots = 64;
+template <int R>
+__device__ __forceinline__ void p2g_bin_block(const cs_params& u, const float4* __restrict__ part,
+                                              const float4* __restrict__ aff, uint32_t* __restrict__ tile_count,
+                                              uint2* __restrict__ key, uint32_t first) {
+  __shared__ uint32_t s_tile[kBinSlots], s_cnt[kBinSlots], s_base[kBinSlots];
+  const uint32_t t = threadIdx.x;
+  if (t < kBinSlots) {
+    s_tile[t] = kNoTile;
+    s_cnt[t] = 0u;
   }
-  const uint32_t lane = threadIdx.x & 63u;
+  __syncthreads();
+  const uint32_t lane = t & 63u;
   const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
-  uint64_t todo = __ballot(active);
-  uint32_t rank = 0;
-  while (todo) {
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
-    const uint32_t lt = __shfl(tile, (int)leader, 64);
-    const uint64_t group = __ballot(active && tile == lt);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&tile_count[lt], (uint32_t)__popcll(group));
-    base = __shfl(base, (int)leader, 64);
-    if (active && tile == lt) rank = base + (uint32_t)__popcll(group & below);
-    todo &= ~group;
+  uint32_t tile[R], rank[R];
+  int slot[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t pid = first + (uint32_t)r * 256u + t;
+    bool active = false;
+    tile[r] = kNoTile;
+    if (pid < u.num_particles) {
+      P2GParticle q;
+      active = p2g_setup(u, part, aff, pid, q);
+      if (active) tile[r] = tile_of(u, q.bx, q.by, q.bz);
+    }
+    uint64_t todo = __ballot(active);
+    rank[r] = 0u;
+    slot[r] = -1;
+    while (todo) {
+      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
+      const uint32_t lt = __shfl(tile[r], (int)leader, 64);
+      const uint64_t group = __ballot(active && tile[r] == lt);
+      uint32_t base = 0u;
+      int sl = -1;
+      if (lane == leader) {
+        uint32_t h = (lt * 2654435761u) >> 26;
+        for (uint32_t probe = 0; probe < kBinSlots; ++probe, h = (h + 1u) & (kBinSlots - 1u)) {
+          const uint32_t prev = atomicCAS(&s_tile[h], kNoTile, lt);
+          if (prev == kNoTile || prev == lt) {
+            sl = (int)h;
+            break;
+          }
+        }
+        base = sl >= 0 ? atomicAdd(&s_cnt[sl], (uint32_t)__popcll(group))
+                       : atomicAdd(&tile_count[lt], (uint32_t)__popcll(group));
+      }
+      base = __shfl(base, (int)leader, 64);
+      sl = __shfl(sl, (int)leader, 64);
+      if (active && tile[r] == lt) {
+        rank[r] = base + (uint32_t)__popcll(group & below);
+        slot[r] = sl;
+      }
+      todo &= ~group;
+    }
   }
-  if (pid < u.num_particles) key[pid] = make_uint2(tile, rank);
+  __syncthreads();
+  if (t < kBinSlots && s_tile[t] != kNoTile) s_base[t] = atomicAdd(&tile_count[s_tile[t]], s_cnt[t]);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t pid = first + (uint32_t)r * 256u + t;
+    if (pid >= u.num_particles) continue;
+    const uint32_t rk = (tile[r] != kNoTile && slot[r] >= 0) ? s_base[slot[r]] + rank[r] : rank[r];
+    key[pid] = make_uint2(tile[r], rk);
+  }
 }
+constexpr int kBinPerThread = 4;  // particles per thread of k_p2g_bin: 1024 per workgroup
 __global__ void __launch_bounds__(256) k_p2g_bin(cs_params u, const float4* __restrict__ part,
                                                  const float4* __restrict__ aff, uint32_t* __restrict__ tile_count,
                                                  uint2* __restrict__ key) {
-  p2g_bin_particle(u, part, aff, tile_count, key, blockIdx.x * blockDim.x + threadIdx.x);
+  p2g_bin_block<kBinPerThread>(u, part, aff, tile_count, key, blockIdx.x * (256u * kBinPerThread));
 }
 
 // Pass 2: exclusive scan of the tile counts (one workgroup).
@@ -200,8 +246,8 @@ __global__ void __launch_bounds__(256) k_batch_p2g_bin(const cs_scene_dev* __res
   CS_BATCH_SCENE(CS_SPACE_PARTICLES);
   const uint32_t pid = blk * 256u + threadIdx.x;
   if (d.p2g_tiled)
-    p2g_bin_particle(u, reinterpret_cast<const float4*>(d.part), reinterpret_cast<const float4*>(d.aff),
-                     d.tile_count, d.p2g_key, pid);
+    p2g_bin_block<1>(u, reinterpret_cast<const float4*>(d.part), reinterpret_cast<const float4*>(d.aff),
+                     d.tile_count, d.p2g_key, blk * 256u);
   else
     p2g_atomic_particle(u, reinterpret_cast<const float4*>(d.part), reinterpret_cast<const float4*>(d.aff), d.lanes,
                         d.metrics, pid);
@@ -250,8 +296,9 @@ hipError_t cs_launch_p2g_tiled(hipStream_t s, const cs_params& u, const float* p
   const dim3 pb((u.num_particles + 255u) / 256u);
   hipError_t e = cs_launch_fill_i32(s, reinterpret_cast<int32_t*>(tile_count), u.nbricks, 0);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_p2g_bin, pb, dim3(256), 0, s, u, reinterpret_cast<const float4*>(part),
-                     reinterpret_cast<const float4*>(aff), tile_count, key);
+  const uint32_t per_block = 256u * kBinPerThread;
+  hipLaunchKernelGGL(k_p2g_bin, dim3((u.num_particles + per_block - 1u) / per_block), dim3(256), 0, s, u,
+                     reinterpret_cast<const float4*>(part), reinterpret_cast<const float4*>(aff), tile_count, key);
   hipLaunchKernelGGL(k_p2g_scan, dim3(1), dim3(1024), 0, s, tile_count, u.nbricks, tile_start);
   hipLaunchKernelGGL(k_p2g_scatter, pb, dim3(256), 0, s, u.num_particles, key, tile_start, sorted);
   hipLaunchKernelGGL(k_p2g_tile, dim3(u.nbricks), dim3(256), 0, s, u, reinterpret_cast<const float4*>(part),
