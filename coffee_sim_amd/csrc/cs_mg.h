@@ -79,21 +79,40 @@ hipError_t cs_mg_wg_visit(hipStream_t s, const cs_mg_level* levels, int first, i
 // The same visit with each operator cell's row held in registers, for the
 // usual case of exactly two small levels (A = levels[first], B the coarsest):
 // k_mg_rows lists the rows once per cycle (after the Galerkin builds), by level
-// and colour: A's colour c at rows[c * |A|], B's at rows[2|A| + c * |B|], the
-// four counts in `counts`.  Same results as cs_mg_wg_visit, bit for bit.
+// and colour: A's colour c at rows[c * |A|], B's at rows[2|A| + c * |B|], and
+// with a third level P = levels[first - 1] P's at rows[2(|A| + |B|) + c * |P|];
+// the counts A0, A1, B0, B1, P0, P1 in `counts`.  Same results as
+// cs_mg_wg_visit, bit for bit.
 #define CS_MG_ROWS_THREADS 1024
 #define CS_MG_ROWS_A 4096u  // A's cells at most: two list entries per thread and colour
 #define CS_MG_ROWS_B 512u   // B's cells at most: one list entry per thread and colour
+#define CS_MG_ROWS3_THREADS 512  // threads of the three-level visit
+#define CS_MG_ROWS_PSLOTS 6      // P's list entries per thread and colour held in registers (more are read per pass)
+#define CS_MG_P3_ZERO 8191u      // P's compact id that stays zero (off-grid and non-operator neighbours)
+#define CS_MG_P3_MAX 8191u       // P's operator cells at most for the three-level visit
 struct cs_mg_row {
   uint32_t xyz;  // x | y << 10 | z << 20
   float d;
   float w[6];    // face weights -x, +x, -y, +y, -z, +z (minus sides read from the neighbours)
 };
 bool cs_mg_rows_fit(const cs_mg_level* levels, int first, int nl);
-hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, cs_mg_row* rows, uint32_t* counts,
-                      const uint32_t* skip);
+// The three-level visit's level shapes fit: a level P above A (not the fine
+// level) of at most 65,535 cells with sides below 1024.  Its operator must also
+// have at most CS_MG_P3_MAX cells, which the caller bounds from the active
+// bricks before choosing it.
+bool cs_mg_rows3_fit(const cs_mg_level* levels, int first, int nl);
+// pmap (|P| words), pnb (2|P| entries) and achild (2|A| entries): P's compact
+// ids, P's entries' neighbour ids and A's entries' children ids, written when
+// P is given.
+hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_level* P, cs_mg_row* rows,
+                      uint32_t* counts, uint32_t* pmap, uint4* pnb, uint4* achild, const uint32_t* skip);
 hipError_t cs_mg_rows_visit(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
                             const uint32_t* counts, const uint32_t* skip);
+// A whole visit of P (A's and B's visits inside) in one workgroup, all three
+// levels in LDS; P's corrections written back to P.E at the end.
+hipError_t cs_mg_rows3_prepare();  // once, before the first cs_mg_rows3_visit
+hipError_t cs_mg_rows3_visit(hipStream_t s, cs_mg_level P, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
+                             const uint32_t* counts, const uint4* pnb, const uint4* achild, const uint32_t* skip);
 // After a residual test (mode 0, into st->initial_residual): flags[0] = skip
 // the next cycle, flags[1] = the start before any cycle was within the
 // tolerance (`first` sets it).  Every MG kernel reads flags[0].

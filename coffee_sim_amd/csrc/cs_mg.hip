@@ -30,7 +30,10 @@
 // the oracle's serial loops give the same bits.  The fine level runs over the
 // active bricks; the levels down to 16^3 are whole-grid launches; the smaller
 // levels of a visit run inside one workgroup, from an operation list.
+#include <type_traits>
+
 #include "cs_device.h"
+#include "cs_kernels.h"
 #include "cs_mg.h"
 
 namespace {
@@ -582,41 +585,121 @@ __global__ void __launch_bounds__(CS_MG_WG_THREADS) k_mg_wg_visit(mg_levels g, i
 //  - prolongation as its own step (E += CS_MG_SCALE * parent's E for A's
 //    operator cells) before the red pass: the same values the folded form
 //    reads.
-__device__ __forceinline__ void mg_rows_build(const cs_mg_level& L, cs_mg_row* __restrict__ out, uint32_t* cnt,
-                                              uint32_t t) {
-  const uint32_t n = L.X * L.Y * L.Z, sy = L.X, sz = L.X * L.Y;
-  for (uint32_t k = t; k < n; k += CS_MG_ROWS_THREADS) {
-    const float d = L.D[k];
-    if (!(d > 0.0f)) continue;
+// bit f set: the neighbour across face f (-x, +x, -y, +y, -z, +z) is on the grid
+__device__ __forceinline__ uint32_t mg_face_flags(uint32_t x, uint32_t y, uint32_t z, uint32_t X, uint32_t Y,
+                                                  uint32_t Z) {
+  return (x > 0u ? 1u : 0u) | (x + 1u < X ? 2u : 0u) | (y > 0u ? 4u : 0u) | (y + 1u < Y ? 8u : 0u) |
+         (z > 0u ? 16u : 0u) | (z + 1u < Z ? 32u : 0u);
+}
+// Rows of A, B and (when P.X > 0) P into their colour lists, one thread per
+// cell over the three levels in turn; counts A0, A1, B0, B1, P0, P1 (zero
+// before the launch) reserve each workgroup's run of list positions once.  The
+// order within a list is immaterial: a pass reads only the other colour, and
+// every sum has its fixed order.
+__global__ void __launch_bounds__(256) k_mg_rows_list(cs_mg_level A, cs_mg_level B, cs_mg_level P,
+                                                      cs_mg_row* __restrict__ rows, uint32_t* __restrict__ counts,
+                                                      const uint32_t* __restrict__ skip) {
+  if (*skip) return;
+  __shared__ uint32_t cnt[6], base[6];
+  const uint32_t t = threadIdx.x;
+  if (t < 6u) cnt[t] = 0u;
+  __syncthreads();
+  const uint32_t nA = A.X * A.Y * A.Z, nB = B.X * B.Y * B.Z, nP = P.X * P.Y * P.Z;
+  uint32_t k = blockIdx.x * 256u + t;
+  int lev = -1;
+  if (k < nA) {
+    lev = 0;
+  } else if (k < nA + nB) {
+    lev = 1;
+    k -= nA;
+  } else if (k < nA + nB + nP) {
+    lev = 2;
+    k -= nA + nB;
+  }
+  const cs_mg_level L = lev == 0 ? A : (lev == 1 ? B : P);  // by value: kernel arguments stay off the stack
+  uint32_t slot = 0u, pos = 0u;
+  cs_mg_row r{};
+  if (lev >= 0 && L.D[k] > 0.0f) {
     uint32_t x, y, z;
     level_cell(L, k, x, y, z);
     const uint32_t c = (x + y + z) & 1u;
-    cs_mg_row r;
+    const uint32_t sy = L.X, sz = L.X * L.Y;
     r.xyz = x | (y << 10) | (z << 20);
-    r.d = d;
+    r.d = L.D[k];
     r.w[0] = x > 0u ? L.Wx[k - 1] : 0.0f;
     r.w[1] = L.Wx[k];
     r.w[2] = y > 0u ? L.Wy[k - sy] : 0.0f;
     r.w[3] = L.Wy[k];
     r.w[4] = z > 0u ? L.Wz[k - sz] : 0.0f;
     r.w[5] = L.Wz[k];
-    out[(size_t)c * n + atomicAdd(&cnt[c], 1u)] = r;
+    slot = 1u + (uint32_t)lev * 2u + c;
+    pos = atomicAdd(&cnt[slot - 1u], 1u);
+  }
+  __syncthreads();
+  if (t < 6u && cnt[t]) base[t] = atomicAdd(&counts[t], cnt[t]);
+  __syncthreads();
+  if (slot) {
+    const uint32_t j = slot - 1u, c = j & 1u;
+    const size_t off = j < 2u ? (size_t)c * nA : j < 4u ? 2 * (size_t)nA + (size_t)c * nB
+                                                         : 2 * ((size_t)nA + nB) + (size_t)c * nP;
+    rows[off + base[j] + pos] = r;
   }
 }
-__global__ void __launch_bounds__(CS_MG_ROWS_THREADS) k_mg_rows(cs_mg_level A, cs_mg_level B,
-                                                                cs_mg_row* __restrict__ rows,
-                                                                uint32_t* __restrict__ counts,
-                                                                const uint32_t* __restrict__ skip) {
+// P's compact ids: colour 0's list, then colour 1's, through the dense map.
+__global__ void __launch_bounds__(256) k_mg_rows_map(cs_mg_level A, cs_mg_level B, cs_mg_level P,
+                                                     const cs_mg_row* __restrict__ rows,
+                                                     const uint32_t* __restrict__ counts,
+                                                     uint32_t* __restrict__ pmap, const uint32_t* __restrict__ skip) {
   if (*skip) return;
-  __shared__ uint32_t cnt[4];
-  const uint32_t t = threadIdx.x;
-  if (t < 4u) cnt[t] = 0u;
-  __syncthreads();
-  const uint32_t nA = A.X * A.Y * A.Z;
-  mg_rows_build(A, rows, cnt, t);
-  mg_rows_build(B, rows + 2 * (size_t)nA, cnt + 2, t);
-  __syncthreads();
-  if (t < 4u) counts[t] = cnt[t];
+  const uint32_t nA = A.X * A.Y * A.Z, nB = B.X * B.Y * B.Z, nP = P.X * P.Y * P.Z;
+  const uint32_t c0 = counts[4], c1 = counts[5];
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  if (k >= c0 + c1) return;
+  const cs_mg_row* const lP = rows + 2 * ((size_t)nA + nB);
+  const uint32_t xyz = (k < c0 ? lP[k] : lP[nP + (k - c0)]).xyz;
+  pmap[((xyz >> 20) * P.Y + ((xyz >> 10) & 1023u)) * P.X + (xyz & 1023u)] = k;
+}
+// Each P entry's six neighbours' compact ids and its own (pnb, list order; an
+// off-grid neighbour or one outside the operator as CS_MG_P3_ZERO), and each A
+// entry's eight children's ids in restrict_cell's order (achild; a child off
+// P's grid or outside its operator as CS_MG_P3_ZERO).
+__global__ void __launch_bounds__(256) k_mg_rows_nb(cs_mg_level A, cs_mg_level B, cs_mg_level P,
+                                                    const cs_mg_row* __restrict__ rows,
+                                                    const uint32_t* __restrict__ counts,
+                                                    const uint32_t* __restrict__ pmap, uint4* __restrict__ pnb,
+                                                    uint4* __restrict__ achild, const uint32_t* __restrict__ skip) {
+  if (*skip) return;
+  const uint32_t nA = A.X * A.Y * A.Z, nB = B.X * B.Y * B.Z, nP = P.X * P.Y * P.Z;
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t c0 = counts[4], c1 = counts[5], a0 = counts[0], a1 = counts[1];
+  if (k < c0 + c1) {
+    const uint32_t e = k < c0 ? k : nP + (k - c0);
+    const cs_mg_row& r = rows[2 * ((size_t)nA + nB) + e];
+    const uint32_t x = r.xyz & 1023u, y = (r.xyz >> 10) & 1023u, z = r.xyz >> 20;
+    const uint32_t i = (z * P.Y + y) * P.X + x;
+    const uint32_t fl = mg_face_flags(x, y, z, P.X, P.Y, P.Z);
+    const uint32_t sy = P.X, sz = P.X * P.Y;
+    const uint32_t ni[6] = {i - 1u, i + 1u, i - sy, i + sy, i - sz, i + sz};
+    uint32_t nb[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nb[f] = ((fl >> f) & 1u) && P.D[ni[f]] > 0.0f ? pmap[ni[f]] : CS_MG_P3_ZERO;
+    pnb[e] = make_uint4(nb[0] | (nb[1] << 16), nb[2] | (nb[3] << 16), nb[4] | (nb[5] << 16), k | (i << 16));
+  }
+  if (k < a0 + a1) {
+    const uint32_t e = k < a0 ? k : nA + (k - a0);
+    const uint32_t xyz = rows[e].xyz;
+    const uint32_t X = xyz & 1023u, Y = (xyz >> 10) & 1023u, Z = xyz >> 20;
+    uint32_t ch[8];
+    uint32_t n = 0;
+    for (uint32_t dz = 0; dz < 2u; ++dz)
+      for (uint32_t dy = 0; dy < 2u; ++dy)
+        for (uint32_t dx = 0; dx < 2u; ++dx, ++n) {
+          const uint32_t x = 2u * X + dx, y = 2u * Y + dy, z = 2u * Z + dz;
+          const uint32_t ci = (z * P.Y + y) * P.X + x;
+          ch[n] = x < P.X && y < P.Y && z < P.Z && P.D[ci] > 0.0f ? pmap[ci] : CS_MG_P3_ZERO;
+        }
+    achild[e] = make_uint4(ch[0] | (ch[1] << 16), ch[2] | (ch[3] << 16), ch[4] | (ch[5] << 16), ch[6] | (ch[7] << 16));
+  }
 }
 
 // One operator cell of a thread: its row in registers.
@@ -636,8 +719,7 @@ __device__ __forceinline__ vcell mg_load_cell(const cs_mg_row* __restrict__ list
   const uint32_t x = row.xyz & 1023u, y = (row.xyz >> 10) & 1023u, z = row.xyz >> 20;
   v.i = (z * L.Y + y) * L.X + x;
   v.aux = P ? ((z >> 1) * P->Y + (y >> 1)) * P->X + (x >> 1) : row.xyz;
-  v.fl = (x > 0u ? 1u : 0u) | (x + 1u < L.X ? 2u : 0u) | (y > 0u ? 4u : 0u) | (y + 1u < L.Y ? 8u : 0u) |
-         (z > 0u ? 16u : 0u) | (z + 1u < L.Z ? 32u : 0u);
+  v.fl = mg_face_flags(x, y, z, L.X, L.Y, L.Z);
   v.d = row.d;
   v.w0 = row.w[0];
   v.w1 = row.w[1];
@@ -648,24 +730,110 @@ __device__ __forceinline__ vcell mg_load_cell(const cs_mg_row* __restrict__ list
   return v;
 }
 // offsum against the corrections E (same terms and order as offsum)
-__device__ __forceinline__ float mg_vsum(const vcell& v, const sfloat* E, uint32_t sy, uint32_t sz) {
-  const uint32_t i = v.i;
-  const bool bxm = v.fl & 1u, bxp = v.fl & 2u, bym = v.fl & 4u, byp = v.fl & 8u, bzm = v.fl & 16u,
-             bzp = v.fl & 32u;
+template <class PTR>
+__device__ __forceinline__ float mg_rsum(uint32_t i, uint32_t fl, float w0, float w1, float w2, float w3, float w4,
+                                         float w5, PTR E, uint32_t sy, uint32_t sz) {
+  const bool bxm = fl & 1u, bxp = fl & 2u, bym = fl & 4u, byp = fl & 8u, bzm = fl & 16u, bzp = fl & 32u;
   const float e0 = E[bxm ? i - 1u : i], e1 = E[bxp ? i + 1u : i], e2 = E[bym ? i - sy : i];
   const float e3 = E[byp ? i + sy : i], e4 = E[bzm ? i - sz : i], e5 = E[bzp ? i + sz : i];
-  float s = wg_term(bxm, v.w0, e0);
-  s += wg_term(bxp, v.w1, e1);
-  s += wg_term(bym, v.w2, e2);
-  s += wg_term(byp, v.w3, e3);
-  s += wg_term(bzm, v.w4, e4);
-  s += wg_term(bzp, v.w5, e5);
+  float s = wg_term(bxm, w0, e0);
+  s += wg_term(bxp, w1, e1);
+  s += wg_term(bym, w2, e2);
+  s += wg_term(byp, w3, e3);
+  s += wg_term(bzm, w4, e4);
+  s += wg_term(bzp, w5, e5);
   return s;
+}
+__device__ __forceinline__ float mg_vsum(const vcell& v, const sfloat* E, uint32_t sy, uint32_t sz) {
+  return mg_rsum(v.i, v.fl, v.w0, v.w1, v.w2, v.w3, v.w4, v.w5, E, sy, sz);
 }
 __device__ __forceinline__ void mg_vsmooth(const vcell& v, sfloat* E, uint32_t sy, uint32_t sz) {
   if (!v.on) return;
   const float s = mg_vsum(v, E, sy, sz);
   E[v.i] = (v.r + s) / v.d;
+}
+// A's restriction into B's cell v (children dz, dy, dx ascending; a child outside
+// A's operator holds +0.0 in RA), which also zeroes B's correction.
+__device__ __forceinline__ void mg_vrestrict(vcell& v, const cs_mg_level& A, const sfloat* RA, sfloat* EB) {
+  if (!v.on) return;
+  const uint32_t X = v.aux & 1023u, Y = (v.aux >> 10) & 1023u, Z = v.aux >> 20;
+  float acc = 0.0f;
+  for (uint32_t dz = 0; dz < 2u; ++dz)
+    for (uint32_t dy = 0; dy < 2u; ++dy)
+      for (uint32_t dx = 0; dx < 2u; ++dx) {
+        const uint32_t x = 2u * X + dx, y = 2u * Y + dy, z = 2u * Z + dz;
+        if (x >= A.X || y >= A.Y || z >= A.Z) continue;
+        acc += RA[(z * A.Y + y) * A.X + x];
+      }
+  v.r = acc;
+  EB[v.i] = 0.0f;
+}
+
+// A visit of A (its right-hand side in a[q].r, its correction zero in EA):
+// CS_MG_NU passes, then CS_MG_GAMMA times {residuals into RA, restriction into
+// B, CS_MG_COARSE passes of B, prolongation, CS_MG_NU passes}.  a[2c + s] holds
+// A's colour-c list entry t + s * CS_MG_ROWS_THREADS, b[c] B's entry t.  RA
+// holds +0.0 outside A's operator.
+__device__ __forceinline__ void mg_visit_ab(vcell (&a)[4], vcell (&b)[2], const cs_mg_level& A,
+                                            const cs_mg_level& B, sfloat* EA, sfloat* RA, sfloat* EB) {
+  const uint32_t ay = A.X, az = A.X * A.Y, by = B.X, bz = B.X * B.Y;
+  for (uint32_t k = 0; k < CS_MG_NU; ++k) {
+    mg_vsmooth(a[0], EA, ay, az);
+    mg_vsmooth(a[1], EA, ay, az);
+    __syncthreads();
+    mg_vsmooth(a[2], EA, ay, az);
+    mg_vsmooth(a[3], EA, ay, az);
+    __syncthreads();
+  }
+  for (uint32_t g = 0; g < CS_MG_GAMMA; ++g) {
+    // residuals of A's operator cells, then their sums into B's cells
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const vcell& v = a[q];
+      if (v.on) {
+        const float s = mg_vsum(v, EA, ay, az);
+        RA[v.i] = v.r - (v.d * EA[v.i] - s);
+      }
+    }
+    __syncthreads();
+    mg_vrestrict(b[0], A, RA, EB);
+    mg_vrestrict(b[1], A, RA, EB);
+    __syncthreads();
+    for (uint32_t k = 0; k < CS_MG_COARSE; ++k) {
+      mg_vsmooth(b[0], EB, by, bz);
+      __syncthreads();
+      mg_vsmooth(b[1], EB, by, bz);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const vcell& v = a[q];
+      if (v.on) EA[v.i] = EA[v.i] + CS_MG_SCALE * EB[v.aux];
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < CS_MG_NU; ++k) {
+      mg_vsmooth(a[0], EA, ay, az);
+      mg_vsmooth(a[1], EA, ay, az);
+      __syncthreads();
+      mg_vsmooth(a[2], EA, ay, az);
+      mg_vsmooth(a[3], EA, ay, az);
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ void mg_load_ab(vcell (&a)[4], vcell (&b)[2], const cs_mg_level& A, const cs_mg_level& B,
+                                           const cs_mg_row* __restrict__ rows, const uint32_t* __restrict__ counts,
+                                           uint32_t t) {
+  const uint32_t nA = A.X * A.Y * A.Z, nB = B.X * B.Y * B.Z;
+  const cs_mg_row* const rowsB = rows + 2 * (size_t)nA;
+#pragma unroll
+  for (uint32_t c = 0; c < 2u; ++c) {
+#pragma unroll
+    for (uint32_t s = 0; s < 2u; ++s)
+      a[2 * c + s] = mg_load_cell(rows + (size_t)c * nA, counts[c], t + s * CS_MG_ROWS_THREADS, A, &B);
+    b[c] = mg_load_cell(rowsB + (size_t)c * nB, counts[2 + c], t, B, nullptr);
+  }
 }
 
 __global__ void __launch_bounds__(CS_MG_ROWS_THREADS) k_mg_rows_visit(cs_mg_level A, cs_mg_level B,
@@ -679,91 +847,354 @@ __global__ void __launch_bounds__(CS_MG_ROWS_THREADS) k_mg_rows_visit(cs_mg_leve
   sfloat* const EA = (sfloat*)s_ea;
   sfloat* const RA = (sfloat*)s_ra;
   sfloat* const EB = (sfloat*)s_eb;
-  constexpr uint32_t NT = CS_MG_ROWS_THREADS;
   const uint32_t t = threadIdx.x;
   const uint32_t nA = A.X * A.Y * A.Z, nB = B.X * B.Y * B.Z;
-  const uint32_t ay = A.X, az = A.X * A.Y, by = B.X, bz = B.X * B.Y;
-  const cs_mg_row* const rowsB = rows + 2 * (size_t)nA;
-  // A: colour c, slot s at a[2c + s] (list entry t + s * NT); B: colour c at b[c]
   vcell a[4], b[2];
+  mg_load_ab(a, b, A, B, rows, counts, t);
 #pragma unroll
-  for (uint32_t c = 0; c < 2u; ++c) {
-#pragma unroll
-    for (uint32_t s = 0; s < 2u; ++s) {
-      a[2 * c + s] = mg_load_cell(rows + (size_t)c * nA, counts[c], t + s * NT, A, &B);
-      if (a[2 * c + s].on) a[2 * c + s].r = A.R[a[2 * c + s].i];  // the parent's restriction
-    }
-    b[c] = mg_load_cell(rowsB + (size_t)c * nB, counts[2 + c], t, B, nullptr);
-  }
-  for (uint32_t k = t; k < nA; k += NT) {
+  for (int q = 0; q < 4; ++q)
+    if (a[q].on) a[q].r = A.R[a[q].i];  // the parent's restriction
+  for (uint32_t k = t; k < nA; k += CS_MG_ROWS_THREADS) {
     EA[k] = 0.0f;
     RA[k] = 0.0f;  // cells outside the operator restrict as +0.0
   }
-  for (uint32_t k = t; k < nB; k += NT) EB[k] = 0.0f;
+  for (uint32_t k = t; k < nB; k += CS_MG_ROWS_THREADS) EB[k] = 0.0f;
   __syncthreads();
-  auto passA = [&](uint32_t c) {
-    if (c == 0u) {
-      mg_vsmooth(a[0], EA, ay, az);
-      mg_vsmooth(a[1], EA, ay, az);
-    } else {
-      mg_vsmooth(a[2], EA, ay, az);
-      mg_vsmooth(a[3], EA, ay, az);
-    }
-    __syncthreads();
-  };
-  auto passB = [&](uint32_t c) {
-    mg_vsmooth(c == 0u ? b[0] : b[1], EB, by, bz);
-    __syncthreads();
-  };
-  for (uint32_t k = 0; k < CS_MG_NU; ++k) {
-    passA(0u);
-    passA(1u);
+  mg_visit_ab(a, b, A, B, EA, RA, EB);
+  for (uint32_t k = t; k < nA; k += CS_MG_ROWS_THREADS) A.E[k] = EA[k];
+}
+
+// ---- three levels: P, then A and B, all in LDS ------------------------------------------
+// The level above A (P) joins the visit when its operator has at most
+// CS_MG_P3_MAX cells: its corrections, residuals and right-hand side sit in
+// LDS by compact id, each entry's neighbours as ids (k_mg_rows), so a pass of
+// P is one LDS round trip too.  Registers hold, per thread and colour, the
+// first CS_MG_ROWS_PSLOTS P entries (ids, parent in A, row, right-hand side);
+// further entries read theirs from the lists each time.  A's and B's entries:
+// the first per thread and colour keep only index word and parent (A) or
+// coordinates (B) in registers, their rows in LDS in slot order ([field]
+// [colour][thread]); A's right-hand side lives in LDS for every cell, and A's
+// entries beyond the first decode their row each time.  Summation orders and
+// zero terms are those of the dense passes: a neighbour outside the operator
+// or off the grid contributes w * 0 = +0.0 (the weight is finite and not
+// negative), as the dense pass's skipped term does.
+struct pcell {
+  uint32_t idi;  // compact id | dense index << 16
+  uint32_t par;  // the parent's index in A
+  uint32_t nb01, nb23, nb45;
+  float d, w0, w1, w2, w3, w4, w5, r;
+};
+// The value, opaque to the optimiser: LDS addresses derived from register-held
+// ids are recomputed in each pass instead of being hoisted out of the visit's
+// loops.
+__device__ __forceinline__ uint32_t mg_opaque(uint32_t v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ pcell mg_pcell(const cs_mg_row& row, const uint4& nb, float r, const cs_mg_level& A) {
+  const uint32_t x = row.xyz & 1023u, y = (row.xyz >> 10) & 1023u, z = row.xyz >> 20;
+  pcell c;
+  c.idi = nb.w;
+  c.par = ((z >> 1) * A.Y + (y >> 1)) * A.X + (x >> 1);
+  c.nb01 = nb.x;
+  c.nb23 = nb.y;
+  c.nb45 = nb.z;
+  c.d = row.d;
+  c.w0 = row.w[0];
+  c.w1 = row.w[1];
+  c.w2 = row.w[2];
+  c.w3 = row.w[3];
+  c.w4 = row.w[4];
+  c.w5 = row.w[5];
+  c.r = r;
+  return c;
+}
+// OP 0 smooth, 1 residual into RP, 2 prolongation from A, 3 write the
+// correction back to P.E
+template <int OP>
+__device__ __forceinline__ void mg_p3_cell(const pcell& c, const cs_mg_level& P, sfloat* EP, sfloat* RP,
+                                           const sfloat* EA) {
+  const uint32_t idi = mg_opaque(c.idi);
+  const uint32_t id = idi & 0xffffu;
+  if (OP == 2) {
+    EP[id] = EP[id] + CS_MG_SCALE * EA[mg_opaque(c.par)];
+    return;
   }
+  if (OP == 3) {
+    P.E[idi >> 16] = EP[id];
+    return;
+  }
+  const uint32_t n01 = mg_opaque(c.nb01), n23 = mg_opaque(c.nb23), n45 = mg_opaque(c.nb45);
+  const float e0 = EP[n01 & 0xffffu], e1 = EP[n01 >> 16], e2 = EP[n23 & 0xffffu];
+  const float e3 = EP[n23 >> 16], e4 = EP[n45 & 0xffffu], e5 = EP[n45 >> 16];
+  float s = c.w0 * e0;
+  s += c.w1 * e1;
+  s += c.w2 * e2;
+  s += c.w3 * e3;
+  s += c.w4 * e4;
+  s += c.w5 * e5;
+  if (OP == 0) EP[id] = (c.r + s) / c.d;
+  else RP[id] = c.r - (c.d * EP[id] - s);
+}
+template <int OP>
+__device__ __forceinline__ void mg_p3_list(const pcell (&q)[CS_MG_ROWS_PSLOTS], const cs_mg_row* __restrict__ list,
+                                           const uint4* __restrict__ nbl, uint32_t count, const cs_mg_level& P,
+                                           const cs_mg_level& A, sfloat* EP, sfloat* RP, const sfloat* EA,
+                                           uint32_t t) {
+#pragma unroll
+  for (int s = 0; s < CS_MG_ROWS_PSLOTS; ++s)
+    if (t + (uint32_t)s * CS_MG_ROWS3_THREADS < count) mg_p3_cell<OP>(q[s], P, EP, RP, EA);
+  for (uint32_t k = t + CS_MG_ROWS_PSLOTS * CS_MG_ROWS3_THREADS; k < count; k += CS_MG_ROWS3_THREADS) {
+    const cs_mg_row row = list[k];
+    const uint4 nb = nbl[k];
+    mg_p3_cell<OP>(mg_pcell(row, nb, P.R[nb.w >> 16], A), P, EP, RP, EA);
+  }
+}
+
+struct rowv {
+  float d, w0, w1, w2, w3, w4, w5;
+};
+__device__ __forceinline__ uint32_t mg_iword(uint32_t xyz, const cs_mg_level& L) {
+  const uint32_t x = xyz & 1023u, y = (xyz >> 10) & 1023u, z = xyz >> 20;
+  return ((z * L.Y + y) * L.X + x) | (mg_face_flags(x, y, z, L.X, L.Y, L.Z) << 16) | 0x80000000u;
+}
+__device__ __forceinline__ uint32_t mg_parent(uint32_t xyz, const cs_mg_level& C) {
+  const uint32_t x = xyz & 1023u, y = (xyz >> 10) & 1023u, z = xyz >> 20;
+  return ((z >> 1) * C.Y + (y >> 1)) * C.X + (x >> 1);
+}
+__device__ __forceinline__ rowv mg_rowv(const cs_mg_row& r) {
+  return rowv{r.d, r.w[0], r.w[1], r.w[2], r.w[3], r.w[4], r.w[5]};
+}
+__device__ __forceinline__ rowv mg_rowv_lds(const sfloat* RW, uint32_t slot, uint32_t stride) {
+  return rowv{RW[slot], RW[stride + slot], RW[2 * stride + slot], RW[3 * stride + slot], RW[4 * stride + slot],
+              RW[5 * stride + slot], RW[6 * stride + slot]};
+}
+__device__ __forceinline__ void mg_rowv_store(sfloat* RW, uint32_t slot, uint32_t stride, const rowv& v) {
+  RW[slot] = v.d;
+  RW[stride + slot] = v.w0;
+  RW[2 * stride + slot] = v.w1;
+  RW[3 * stride + slot] = v.w2;
+  RW[4 * stride + slot] = v.w3;
+  RW[5 * stride + slot] = v.w4;
+  RW[6 * stride + slot] = v.w5;
+}
+template <class PTR>
+__device__ __forceinline__ float mg_wsum(uint32_t iw, const rowv& w, PTR E, uint32_t sy, uint32_t sz) {
+  return mg_rsum(iw & 0xffffu, (iw >> 16) & 63u, w.w0, w.w1, w.w2, w.w3, w.w4, w.w5, E, sy, sz);
+}
+// OP 0 smooth, 1 residual, 2 prolongation from B, 3 restriction from P (which
+// zeroes A's correction), on A's cell (iw, par) with row w; ch: its children's
+// compact ids in P (restrict_cell's order; CS_MG_P3_ZERO, whose RP stays +0.0,
+// for a child outside P's operator or grid: adding +0.0 to a sum that starts
+// at +0.0 leaves its bits as skipping the child does)
+template <int OP>
+__device__ __forceinline__ void mg_a3_cell(uint32_t iw, uint32_t par, const rowv& w, const uint4& ch,
+                                           const cs_mg_level& A, sfloat* EA, sfloat* RAR, sfloat* RAS,
+                                           const sfloat* EB, const sfloat* RP) {
+  iw = mg_opaque(iw);
+  const uint32_t i = iw & 0xffffu;
+  if (OP == 0) {
+    const float s = mg_wsum(iw, w, EA, A.X, A.X * A.Y);
+    EA[i] = (RAR[i] + s) / w.d;
+  } else if (OP == 1) {
+    const float s = mg_wsum(iw, w, EA, A.X, A.X * A.Y);
+    RAS[i] = RAR[i] - (w.d * EA[i] - s);
+  } else if (OP == 2) {
+    EA[i] = EA[i] + CS_MG_SCALE * EB[par];
+  } else {
+    const uint32_t cx = mg_opaque(ch.x), cy = mg_opaque(ch.y), cz = mg_opaque(ch.z), cw = mg_opaque(ch.w);
+    const float v0 = RP[cx & 0xffffu], v1 = RP[cx >> 16], v2 = RP[cy & 0xffffu], v3 = RP[cy >> 16];
+    const float v4 = RP[cz & 0xffffu], v5 = RP[cz >> 16], v6 = RP[cw & 0xffffu], v7 = RP[cw >> 16];
+    float acc = 0.0f;
+    acc += v0;
+    acc += v1;
+    acc += v2;
+    acc += v3;
+    acc += v4;
+    acc += v5;
+    acc += v6;
+    acc += v7;
+    RAR[i] = acc;
+    EA[i] = 0.0f;
+  }
+}
+constexpr uint32_t kA3Stride = 2u * CS_MG_ROWS3_THREADS;  // A's register-entry rows in LDS: [field][colour][thread]
+constexpr uint32_t kB3Stride = CS_MG_ROWS_B;             // B's rows in LDS: [field][colour][thread < |B| / 2]
+constexpr uint32_t kB3Half = CS_MG_ROWS_B / 2u;
+constexpr uint32_t kP3Cap = CS_MG_P3_ZERO + 1u;           // P's compact arrays (the last id stays zero)
+constexpr size_t mg_rows3_lds_words() {
+  return 3 * (size_t)CS_MG_ROWS_A + CS_MG_ROWS_B + 7 * (size_t)kA3Stride + 8 * (size_t)kB3Stride + 2 * (size_t)kP3Cap;
+}
+static_assert(mg_rows3_lds_words() * sizeof(float) <= 160u * 1024u, "three-level visit LDS exceeds 160 KiB");
+size_t mg_rows3_lds_bytes() {
+  // EA, RAR, RAS; EB; A's rows; B's rows and right-hand side; EP, RP: 161,792 B of the 160 KiB
+  return sizeof(float) * (3 * (size_t)CS_MG_ROWS_A + CS_MG_ROWS_B + 7 * (size_t)kA3Stride + 8 * (size_t)kB3Stride +
+                          2 * (size_t)kP3Cap);
+}
+__global__ void __launch_bounds__(CS_MG_ROWS3_THREADS) k_mg_rows3_visit(cs_mg_level P, cs_mg_level A, cs_mg_level B,
+                                                                       const cs_mg_row* __restrict__ rows,
+                                                                       const uint32_t* __restrict__ counts,
+                                                                       const uint4* __restrict__ pnb,
+                                                                       const uint4* __restrict__ achild,
+                                                                       const uint32_t* __restrict__ skip) {
+  if (*skip) return;
+  extern __shared__ float lds_raw[];
+  sfloat* const EA = (sfloat*)lds_raw;
+  sfloat* const RAR = EA + CS_MG_ROWS_A;   // A's right-hand side
+  sfloat* const RAS = RAR + CS_MG_ROWS_A;  // A's residuals (+0.0 outside its operator)
+  sfloat* const EB = RAS + CS_MG_ROWS_A;
+  sfloat* const AW = EB + CS_MG_ROWS_B;    // A's register-entry rows
+  sfloat* const BW = AW + 7 * kA3Stride;   // B's rows, then B's right-hand side
+  sfloat* const BR = BW + 7 * kB3Stride;
+  sfloat* const EP = BR + kB3Stride;       // P's corrections and residuals by compact id
+  sfloat* const RP = EP + kP3Cap;
+  constexpr uint32_t NT = CS_MG_ROWS3_THREADS;
+  const uint32_t t = threadIdx.x;
+  const uint32_t nA = A.X * A.Y * A.Z, nB = B.X * B.Y * B.Z, nP = P.X * P.Y * P.Z;
+  const cs_mg_row* const lA[2] = {rows, rows + nA};
+  const cs_mg_row* const lB = rows + 2 * (size_t)nA;
+  const cs_mg_row* const lP0 = rows + 2 * ((size_t)nA + nB);
+  const cs_mg_row* const lP1 = lP0 + nP;
+  const uint4* const nP0 = pnb;
+  const uint4* const nP1 = pnb + nP;
+  const uint32_t cA[2] = {counts[0], counts[1]}, cB[2] = {counts[2], counts[3]};
+  const uint32_t cP0 = counts[4], cP1 = counts[5];
+  uint32_t ia[2], pa[2], ib[2], xb[2];
+  uint4 ac[2];
+#pragma unroll
+  for (uint32_t c = 0; c < 2u; ++c) {
+    ia[c] = pa[c] = ib[c] = xb[c] = 0u;
+    ac[c] = make_uint4(0u, 0u, 0u, 0u);
+    if (t < cA[c]) {
+      const cs_mg_row r = lA[c][t];
+      ia[c] = mg_iword(r.xyz, A);
+      pa[c] = mg_parent(r.xyz, B);
+      ac[c] = achild[(size_t)c * nA + t];
+      mg_rowv_store(AW, c * NT + t, kA3Stride, mg_rowv(r));
+    }
+    if (t < cB[c]) {
+      const cs_mg_row r = lB[(size_t)c * nB + t];
+      ib[c] = mg_iword(r.xyz, B);
+      xb[c] = r.xyz;
+      mg_rowv_store(BW, c * kB3Half + t, kB3Stride, mg_rowv(r));
+    }
+  }
+  pcell q0[CS_MG_ROWS_PSLOTS], q1[CS_MG_ROWS_PSLOTS];
+#pragma unroll
+  for (int s = 0; s < CS_MG_ROWS_PSLOTS; ++s) {
+    const uint32_t k = t + (uint32_t)s * NT;
+    q0[s] = pcell{};
+    q1[s] = pcell{};
+    if (k < cP0) {
+      const uint4 nb = nP0[k];
+      q0[s] = mg_pcell(lP0[k], nb, P.R[nb.w >> 16], A);
+    }
+    if (k < cP1) {
+      const uint4 nb = nP1[k];
+      q1[s] = mg_pcell(lP1[k], nb, P.R[nb.w >> 16], A);
+    }
+  }
+  for (uint32_t k = t; k < nA; k += NT) {
+    EA[k] = 0.0f;
+    RAS[k] = 0.0f;
+  }
+  for (uint32_t k = t; k < nB; k += NT) EB[k] = 0.0f;
+  for (uint32_t k = t; k < kP3Cap; k += NT) EP[k] = 0.0f;  // P.E is zero at the visit's start
+  if (t == 0u) RP[CS_MG_P3_ZERO] = 0.0f;
+  __syncthreads();
+  // A's colour-c list: the register entry, then the rest from the list
+  auto opA = [&](auto op, uint32_t c) {
+    constexpr int OP = decltype(op)::value;
+    if (ia[c]) mg_a3_cell<OP>(ia[c], pa[c], mg_rowv_lds(AW, c * NT + t, kA3Stride), ac[c], A, EA, RAR, RAS, EB, RP);
+    for (uint32_t k = t + NT; k < cA[c]; k += NT) {
+      const cs_mg_row r = lA[c][k];
+      const uint4 ch = OP == 3 ? achild[(size_t)c * nA + k] : make_uint4(0u, 0u, 0u, 0u);
+      mg_a3_cell<OP>(mg_iword(r.xyz, A), mg_parent(r.xyz, B), mg_rowv(r), ch, A, EA, RAR, RAS, EB, RP);
+    }
+  };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  using S2 = std::integral_constant<int, 2>;
+  using S3 = std::integral_constant<int, 3>;
+  auto passA = [&]() {
+    opA(S0{}, 0u);
+    __syncthreads();
+    opA(S0{}, 1u);
+    __syncthreads();
+  };
+  auto smoothB = [&](uint32_t c) {
+    if (!ib[c]) return;
+    const rowv w = mg_rowv_lds(BW, c * kB3Half + t, kB3Stride);
+    const uint32_t iw = mg_opaque(ib[c]);
+    const uint32_t i = iw & 0xffffu;
+    const float s = mg_wsum(iw, w, EB, B.X, B.X * B.Y);
+    EB[i] = (BR[c * kB3Half + t] + s) / w.d;
+  };
+  auto restrictB = [&](uint32_t c) {  // restrict_cell from A's residuals, zeroing B's correction
+    if (!ib[c]) return;
+    const uint32_t X = xb[c] & 1023u, Y = (xb[c] >> 10) & 1023u, Z = xb[c] >> 20;
+    // the eight children's residuals loaded at once (a child off A's grid
+    // reads the cell itself and adds +0.0 instead, which keeps the sum's bits)
+    float v[8];
+    bool in[8];
+    uint32_t n = 0;
+    for (uint32_t dz = 0; dz < 2u; ++dz)
+      for (uint32_t dy = 0; dy < 2u; ++dy)
+        for (uint32_t dx = 0; dx < 2u; ++dx, ++n) {
+          const uint32_t x = 2u * X + dx, y = 2u * Y + dy, z = 2u * Z + dz;
+          in[n] = x < A.X && y < A.Y && z < A.Z;
+          v[n] = RAS[in[n] ? (z * A.Y + y) * A.X + x : 0u];
+        }
+    float acc = 0.0f;
+    for (uint32_t j = 0; j < 8u; ++j) acc += in[j] ? v[j] : 0.0f;
+    BR[c * kB3Half + t] = acc;
+    EB[ib[c] & 0xffffu] = 0.0f;
+  };
+  auto passP = [&]() {
+    mg_p3_list<0>(q0, lP0, nP0, cP0, P, A, EP, RP, EA, t);
+    __syncthreads();
+    mg_p3_list<0>(q1, lP1, nP1, cP1, P, A, EP, RP, EA, t);
+    __syncthreads();
+  };
+  for (uint32_t k = 0; k < CS_MG_NU; ++k) passP();
   for (uint32_t g = 0; g < CS_MG_GAMMA; ++g) {
-    // residuals of A's operator cells, then their sums into B's cells
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const vcell& v = a[q];
-      if (v.on) {
-        const float s = mg_vsum(v, EA, ay, az);
-        RA[v.i] = v.r - (v.d * EA[v.i] - s);
+    // P's residuals, then A's right-hand side from its children (restrict_cell)
+    mg_p3_list<1>(q0, lP0, nP0, cP0, P, A, EP, RP, EA, t);
+    mg_p3_list<1>(q1, lP1, nP1, cP1, P, A, EP, RP, EA, t);
+    __syncthreads();
+    opA(S3{}, 0u);
+    opA(S3{}, 1u);
+    __syncthreads();
+    // A's visit (the steps of mg_visit_ab)
+    for (uint32_t k = 0; k < CS_MG_NU; ++k) passA();
+    for (uint32_t ga = 0; ga < CS_MG_GAMMA; ++ga) {
+      opA(S1{}, 0u);
+      opA(S1{}, 1u);
+      __syncthreads();
+      restrictB(0u);
+      restrictB(1u);
+      __syncthreads();
+      for (uint32_t k = 0; k < CS_MG_COARSE; ++k) {
+        smoothB(0u);
+        __syncthreads();
+        smoothB(1u);
+        __syncthreads();
       }
+      opA(S2{}, 0u);
+      opA(S2{}, 1u);
+      __syncthreads();
+      for (uint32_t k = 0; k < CS_MG_NU; ++k) passA();
     }
+    // prolongation of A's correction into P, then P's passes
+    mg_p3_list<2>(q0, lP0, nP0, cP0, P, A, EP, RP, EA, t);
+    mg_p3_list<2>(q1, lP1, nP1, cP1, P, A, EP, RP, EA, t);
     __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      vcell& v = b[c];
-      if (v.on) {
-        const uint32_t X = v.aux & 1023u, Y = (v.aux >> 10) & 1023u, Z = v.aux >> 20;
-        float acc = 0.0f;
-        for (uint32_t dz = 0; dz < 2u; ++dz)
-          for (uint32_t dy = 0; dy < 2u; ++dy)
-            for (uint32_t dx = 0; dx < 2u; ++dx) {
-              const uint32_t x = 2u * X + dx, y = 2u * Y + dy, z = 2u * Z + dz;
-              if (x >= A.X || y >= A.Y || z >= A.Z) continue;
-              acc += RA[(z * A.Y + y) * A.X + x];
-            }
-        v.r = acc;
-        EB[v.i] = 0.0f;
-      }
-    }
-    __syncthreads();
-    for (uint32_t k = 0; k < CS_MG_COARSE; ++k) {
-      passB(0u);
-      passB(1u);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const vcell& v = a[q];
-      if (v.on) EA[v.i] = EA[v.i] + CS_MG_SCALE * EB[v.aux];
-    }
-    __syncthreads();
-    for (uint32_t k = 0; k < CS_MG_NU; ++k) {
-      passA(0u);
-      passA(1u);
-    }
+    for (uint32_t k = 0; k < CS_MG_NU; ++k) passP();
   }
-  for (uint32_t k = t; k < nA; k += NT) A.E[k] = EA[k];
+  // P's corrections back to P.E for its parent's prolongation (every other
+  // cell of P.E is already zero)
+  mg_p3_list<3>(q0, lP0, nP0, cP0, P, A, EP, RP, EA, t);
+  mg_p3_list<3>(q1, lP1, nP1, cP1, P, A, EP, RP, EA, t);
 }
 
 // flags[0]: skip the next cycle; flags[1]: the start was within the tolerance
@@ -909,14 +1340,39 @@ bool cs_mg_rows_fit(const cs_mg_level* levels, int first, int nl) {
   // every colour list within two entries per thread (A) and one (B)
   return cells(A) <= CS_MG_ROWS_A && cells(B) <= CS_MG_ROWS_B;
 }
-hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, cs_mg_row* rows, uint32_t* counts,
-                      const uint32_t* skip) {
-  hipLaunchKernelGGL(k_mg_rows, dim3(1), dim3(CS_MG_ROWS_THREADS), 0, s, A, B, rows, counts, skip);
+bool cs_mg_rows3_fit(const cs_mg_level* levels, int first, int nl) {
+  if (!cs_mg_rows_fit(levels, first, nl) || first < 2) return false;
+  const cs_mg_level& P = levels[first - 1];
+  return P.X < 1024u && P.Y < 1024u && P.Z < 1024u && cells(P) <= 65535u;
+}
+hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_level* P, cs_mg_row* rows,
+                      uint32_t* counts, uint32_t* pmap, uint4* pnb, uint4* achild, const uint32_t* skip) {
+  cs_mg_level none{};
+  const cs_mg_level PP = P ? *P : none;
+  const uint32_t nA = cells(A), nB = cells(B), nP = P ? cells(*P) : 0u;
+  hipError_t e = cs_launch_fill_i32(s, reinterpret_cast<int32_t*>(counts), 6, 0);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mg_rows_list, cells_grid(nA + nB + nP), dim3(256), 0, s, A, B, PP, rows, counts, skip);
+  if (P) {
+    hipLaunchKernelGGL(k_mg_rows_map, cells_grid(nP), dim3(256), 0, s, A, B, PP, rows, counts, pmap, skip);
+    hipLaunchKernelGGL(k_mg_rows_nb, cells_grid(std::max(nP, nA)), dim3(256), 0, s, A, B, PP, rows, counts, pmap, pnb,
+                       achild, skip);
+  }
   return hipGetLastError();
 }
 hipError_t cs_mg_rows_visit(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
                             const uint32_t* counts, const uint32_t* skip) {
   hipLaunchKernelGGL(k_mg_rows_visit, dim3(1), dim3(CS_MG_ROWS_THREADS), 0, s, A, B, rows, counts, skip);
+  return hipGetLastError();
+}
+hipError_t cs_mg_rows3_prepare() {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_rows3_visit),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)mg_rows3_lds_bytes());
+}
+hipError_t cs_mg_rows3_visit(hipStream_t s, cs_mg_level P, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
+                             const uint32_t* counts, const uint4* pnb, const uint4* achild, const uint32_t* skip) {
+  hipLaunchKernelGGL(k_mg_rows3_visit, dim3(1), dim3(CS_MG_ROWS3_THREADS), mg_rows3_lds_bytes(), s, P, A, B, rows,
+                     counts, pnb, achild, skip);
   return hipGetLastError();
 }
 hipError_t cs_mg_gate(hipStream_t s, const cs_solve_state* st, double tol, uint32_t* flags, bool first) {

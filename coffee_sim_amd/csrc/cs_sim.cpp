@@ -65,7 +65,7 @@ void free_sim(cs_sim* m) {
                  m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt,
                  m->bed_snap,   m->alias4,
                  m->persist_fb, m->persist_pflags, m->persist_partials, m->persist_slot, m->persist_xcc,
-                 m->mg_buf,     m->mg_flags,   m->mg_ops,     m->mg_row_buf, m->mg_row_counts};
+                 m->mg_buf,     m->mg_flags,   m->mg_ops,     m->mg_row_buf, m->mg_row_counts, m->mg_pmap, m->mg_pnb, m->mg_achild};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -358,12 +358,19 @@ int ensure_mg(cs_sim* m) {
     CS_HIP(dalloc(&m->mg_ops, ops.size()));
     CS_HIP(hipMemcpy(m->mg_ops, ops.data(), ops.size() * sizeof(cs_mg_op), hipMemcpyHostToDevice));
     m->mg_nops = (uint32_t)ops.size();
-    m->mg_rows = m->mg_rows_allowed && cs_mg_rows_fit(lv.data(), m->mg_wg_first, nl);
+    m->mg_rows = m->mg_rows_allowed >= 1 && cs_mg_rows_fit(lv.data(), m->mg_wg_first, nl);
+    m->mg_rows3_shape = m->mg_rows && m->mg_rows_allowed >= 2 && cs_mg_rows3_fit(lv.data(), m->mg_wg_first, nl);
     if (m->mg_rows) {
-      const size_t rows = 2 * ((size_t)lv[m->mg_wg_first].X * lv[m->mg_wg_first].Y * lv[m->mg_wg_first].Z +
-                               (size_t)lv[nl - 1].X * lv[nl - 1].Y * lv[nl - 1].Z);
-      CS_HIP(dalloc(&m->mg_row_buf, rows));
-      CS_HIP(dalloc(&m->mg_row_counts, 4));
+      auto cells = [&](int l) { return (size_t)lv[l].X * lv[l].Y * lv[l].Z; };
+      const size_t np = m->mg_rows3_shape ? cells(m->mg_wg_first - 1) : 0;
+      CS_HIP(dalloc(&m->mg_row_buf, 2 * (cells(m->mg_wg_first) + cells(nl - 1) + np)));
+      CS_HIP(dalloc(&m->mg_row_counts, 6));
+      if (m->mg_rows3_shape) {
+        CS_HIP(dalloc(&m->mg_pmap, np));
+        CS_HIP(dalloc(&m->mg_pnb, 2 * np));
+        CS_HIP(dalloc(&m->mg_achild, 2 * cells(m->mg_wg_first)));
+        CS_HIP(cs_mg_rows3_prepare());
+      }
     }
   }
   for (auto& e : m->mg_ev) CS_HIP(hipEventCreate(&e));
@@ -377,6 +384,11 @@ int mg_visit(cs_sim* m, const cs_params& u, uint32_t nactive, int l) {
   hipStream_t st = m->ctx->stream;
   const int nl = (int)m->mg_levels.size();
   const uint32_t* skip = m->mg_flags;
+  if (l == m->mg_wg_first - 1 && m->mg_rows3) {
+    CS_HIP(cs_mg_rows3_visit(st, m->mg_levels[l], m->mg_levels[l + 1], m->mg_levels[l + 2], m->mg_row_buf,
+                             m->mg_row_counts, m->mg_pnb, m->mg_achild, skip));
+    return CS_OK;
+  }
   if (l >= m->mg_wg_first && m->mg_rows) {
     CS_HIP(cs_mg_rows_visit(st, m->mg_levels[l], m->mg_levels[l + 1], m->mg_row_buf, m->mg_row_counts, skip));
     return CS_OK;
@@ -438,13 +450,24 @@ int mg_start(cs_sim* m, const cs_params& u, uint32_t nactive) {
   CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->mg_buf + 6 * m->nc), m->mg_floats - 6 * m->nc, 0));
   const int nl = (int)m->mg_levels.size();
   const uint32_t* skip = m->mg_flags;
+  // the level above the small ones joins their visit when its operator fits
+  // CS_MG_P3_MAX cells for sure: its cells lie in the active bricks' parents
+  m->mg_rows3 = false;
+  if (m->mg_rows3_shape) {
+    const int lp = m->mg_wg_first - 1;
+    const uint64_t side = std::max<uint64_t>(1u, (uint64_t)CS_BRICK >> std::min(lp, 30));
+    const cs_mg_level& P = m->mg_levels[lp];
+    const uint64_t bound = std::min<uint64_t>((uint64_t)P.X * P.Y * P.Z, side * side * side * nactive);
+    m->mg_rows3 = bound <= CS_MG_P3_MAX;
+  }
   for (uint32_t c = 0; c < m->s.rbgs_mg_cycles; ++c) {
     CS_HIP(cs_mg_fine_setup(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, L0, skip));
     if (nl > 1) CS_HIP(cs_mg_fine_build(st, u, nactive, m->brick_list, L0, m->mg_levels[1], skip));
     for (int l = 1; l + 1 < nl; ++l) CS_HIP(cs_mg_build(st, m->mg_levels[l], m->mg_levels[l + 1], skip));
     if (m->mg_rows)
-      CS_HIP(cs_mg_rows(st, m->mg_levels[m->mg_wg_first], m->mg_levels[nl - 1], m->mg_row_buf, m->mg_row_counts,
-                        skip));
+      CS_HIP(cs_mg_rows(st, m->mg_levels[m->mg_wg_first], m->mg_levels[nl - 1],
+                        m->mg_rows3 ? &m->mg_levels[m->mg_wg_first - 1] : nullptr, m->mg_row_buf, m->mg_row_counts,
+                        m->mg_pmap, m->mg_pnb, m->mg_achild, skip));
     rc = mg_visit(m, u, nactive, 0);
     if (rc) return rc;
     CS_HIP(cs_mg_fine_apply(st, u, nactive, m->brick_list, L0, m->pres, skip));
@@ -1036,7 +1059,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (const char* v = getenv("CS_RBGS_FUSE")) m->fuse_residual = atoi(v) != 0;
   if (const char* v = getenv("CS_GRAPHS")) m->use_graphs = atoi(v) != 0;
   if (const char* v = getenv("CS_P2G")) m->p2g_tiled = strcmp(v, "atomic") != 0;
-  if (const char* v = getenv("CS_MG_ROWS")) m->mg_rows_allowed = atoi(v) != 0;
+  if (const char* v = getenv("CS_MG_ROWS")) m->mg_rows_allowed = atoi(v);
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->ctx->device) == hipSuccess && cus > 0)

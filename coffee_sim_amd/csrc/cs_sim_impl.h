@@ -102,10 +102,16 @@ struct cs_sim {
   uint32_t mg_nops = 0;
   int mg_wg_first = 0;
   uint32_t nactive = 0;                // active bricks of the current substep's solve (brick_list)
-  bool mg_rows_allowed = true;         // CS_MG_ROWS=0: always the operation-list visit (cross-check)
+  int mg_rows_allowed = 2;             // CS_MG_ROWS: 0 the operation-list visit, 1 at most the two-level
+                                       // register visit, 2 also the three-level one (cross-checks)
   bool mg_rows = false;                // the small levels' visit keeps its rows in registers (cs_mg_rows_visit)
+  bool mg_rows3_shape = false;         // the level above them can join it (cs_mg_rows3_visit) ...
+  bool mg_rows3 = false;               // ... and does in the current start (its operator fits for sure)
   cs_mg_row* mg_row_buf = nullptr;     // their row lists, rebuilt every cycle
-  uint32_t* mg_row_counts = nullptr;   // [4]
+  uint32_t* mg_row_counts = nullptr;   // [6]
+  uint32_t* mg_pmap = nullptr;         // the three-level visit's top level: dense index -> compact id
+  uint4* mg_pnb = nullptr;             // and each list entry's neighbour ids
+  uint4* mg_achild = nullptr;          // the level below it: each list entry's children's compact ids
   hipEvent_t mg_ev[2] = {nullptr, nullptr};
   bool mg_timed = false;               // mg_ev holds an unharvested start
   // tile-binned P2G scratch

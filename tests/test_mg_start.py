@@ -11,8 +11,9 @@ CPU, against what the mathematics fixes:
 - on the 64^3 pour, the start cuts the iterations to the 1e-4 tolerance by
   more than 10x.
 GPU: the device start (fine level over the active bricks, whole-grid middle
-levels, the small levels inside one workgroup, with their rows in registers
-or from the operation list) equals the oracle's bit for bit: iteration counts
+levels, the small levels inside one workgroup, with their rows in registers,
+with the level above them in the same workgroup, or from the operation list)
+equals the oracle's bit for bit: iteration counts
 and the whole state, from zero, previous-step and external starts, on the
 persistent and the launch-per-chunk solve."""
 from __future__ import annotations
@@ -208,16 +209,17 @@ def _pair(ctx, orc, s, scene):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("visit", ["rows", "oplist"])
+@pytest.mark.parametrize("visit", ["rows3", "rows", "oplist"])
 @pytest.mark.parametrize("layout", ["persistent", "launches"])
 @pytest.mark.parametrize("scale", [4, 2])
 def test_mg_start_matches_oracle(ctx, orc, monkeypatch, layout, scale, visit):
-    """visit: the small levels' visit with its rows in registers (the default
-    on these grids), or the operation-list visit (CS_MG_ROWS=0)."""
+    """visit: the small levels' visit with its rows in registers, joined by the
+    level above them where that is not the fine level (the default; on the
+    32^3 grid the same as "rows"), the two-level register visit
+    (CS_MG_ROWS=1), or the operation-list visit (CS_MG_ROWS=0)."""
     if layout == "launches":
         monkeypatch.setenv("CS_RBGS_PERSIST", "0")
-    if visit == "oplist":
-        monkeypatch.setenv("CS_MG_ROWS", "0")
+    monkeypatch.setenv("CS_MG_ROWS", {"rows3": "2", "rows": "1", "oplist": "0"}[visit])
     w = cs.SeededWorkload("v60_working", seed=6, scale=scale)
     nc = int(np.prod(list(w.settings.grid_dims)))
     starts = (cs.abi.CS_WARM_ZERO, cs.abi.CS_WARM_PREVIOUS, cs.abi.CS_WARM_EXTERNAL) if scale == 4 else \
