@@ -26,7 +26,9 @@ reports that iteration count beside the result.
 scenes (cubic grids of side 48-96, 20k-400k particles, seeded) advanced by
 one launch per pass, each solved to 1e-4, with launch shapes and per-side P2G
 layouts from the autotuner; `value` is the particle-substeps/s of the whole
-batch.
+batch.  `--workload v60_large` runs the 256^3 scene (8,388,608 particles, dx
+0.5 mm) the same way as the headline one and adds each kernel family's
+algorithmic GB/s (`config.kernel_gbs`).
 
 The JSON line also carries
   roofline      for the kernel with the largest share of device time, from
@@ -129,6 +131,29 @@ def roofline(kt) -> dict:
         "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
         "device_time_share_us": {k: round(v, 1) for k, v in shares.items()},
     }
+
+
+WORKLOAD_TEXT = {
+    "v60_working": "v60_working: 128^3 grid, dx 1 mm, 1,048,576 particles, value-noise porous bed",
+    "v60_large": "v60_large: 256^3 grid, dx 0.5 mm, 8,388,608 particles, value-noise porous bed "
+                 "(16-cell wavelength, +-0.08)",
+}
+
+
+def kernel_rates(kt) -> dict:
+    """Algorithmic GB/s of each timed kernel family over the timed region (the
+    byte counts of roofline())."""
+    out = {}
+    if kt.p2g_us > 0:
+        out["p2g"] = round(80.0 * kt.p2g_particles / (kt.p2g_us * 1e-6) / 1e9, 1)
+    if kt.g2p_us > 0:
+        out["g2p"] = round(128.0 * kt.g2p_particles / (kt.g2p_us * 1e-6) / 1e9, 1)
+    if kt.rbgs_us > 0:
+        out["rbgs"] = round(16.0 * kt.rbgs_cell_updates / (kt.rbgs_us * 1e-6) / 1e9, 1)
+    if kt.mg_us > 0:
+        fluid = kt.rbgs_cell_updates / max(kt.rbgs_iterations, 1)
+        out["mg_start"] = round(272.0 * fluid * kt.mg_cycles / (kt.mg_us * 1e-6) / 1e9, 1)
+    return out
 
 
 def cpu_baseline(w, particles: np.ndarray, affine: np.ndarray, prev_p: np.ndarray, budget_s: float,
@@ -449,7 +474,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed substeps (20; batch64: 5)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed substeps before timing (3; batch64: 1)")
-    ap.add_argument("--workload", choices=("v60_working", "batch64", "unet128", "learned128"),
+    ap.add_argument("--workload", choices=("v60_working", "v60_large", "batch64", "unet128", "learned128"),
                     default="v60_working")
     ap.add_argument("--train-pours", type=int, default=3, help="learned128: pours whose solves train the network")
     ap.add_argument("--epochs", type=int, default=30, help="learned128: training epochs")
@@ -495,6 +520,11 @@ def main() -> int:
         args.steps = 8 if args.steps is None else args.steps
         args.warmup = 0 if args.warmup is None else args.warmup
         return bench_learned(args)
+    if args.workload == "v60_large":
+        args.steps = 10 if args.steps is None else args.steps
+        args.warmup = 2 if args.warmup is None else args.warmup
+        if "--cpu-budget" not in sys.argv:
+            args.cpu_budget = 0.0  # the headline line carries the CPU baseline
     args.steps = 20 if args.steps is None else args.steps
     args.warmup = 3 if args.warmup is None else args.warmup
 
@@ -502,7 +532,8 @@ def main() -> int:
 
     from coffee_sim_amd.abi import CS_WARM_PREVIOUS, CS_WARM_ZERO
 
-    w = cs.SeededWorkload("v60_working", seed=args.seed, scale=args.scale)
+    wname = args.workload
+    w = cs.SeededWorkload(wname, seed=args.seed, scale=args.scale)
     settings = w.settings.copy()
     settings.warm_start = CS_WARM_PREVIOUS if args.warm_start == "previous" else CS_WARM_ZERO
     settings.rbgs_mg_cycles = max(args.mg_cycles, 0)
@@ -515,7 +546,7 @@ def main() -> int:
         sim.load(state.read_bytes())
         log(f"loaded the post-pour state from {state}: {sim.particle_count()} particles")
     else:
-        log(f"workload v60_working grid {list(w.settings.grid_dims)} initial {w.scene.num_particles} "
+        log(f"workload {wname} grid {list(w.settings.grid_dims)} initial {w.scene.num_particles} "
             f"total {w.total_particles}; advancing {pour} pour substeps untimed ({args.warm_start} start)")
         t0 = time.perf_counter()
         for k in range(pour):
@@ -600,8 +631,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded generators)",
         "config": {
-            "workload": "v60_working: 128^3 grid, dx 1 mm, 1,048,576 particles, value-noise porous bed, "
-                        "RBGS to 1e-4 relative residual from a "
+            "workload": (WORKLOAD_TEXT[wname] + ", RBGS to 1e-4 relative residual from a ")
                         + ("previous-step" if args.warm_start == "previous" else "zero")
                         + (f" start corrected by a {settings.rbgs_mg_cycles}-cycle multigrid start"
                            if settings.rbgs_mg_cycles else " start"),
@@ -618,6 +648,9 @@ def main() -> int:
         "roofline": roofline(kt),
         "cpu_baseline": cpu,
     }
+    if wname != "v60_working":
+        out["metric"] = f"particle-substeps/s ({wname})"
+        out["config"]["kernel_gbs"] = kernel_rates(kt)
     print(json.dumps(out), flush=True)
     if args.skip_teardown:
         _LEFT_OPEN.extend([sim, ctx])  # released by the interpreter's own shutdown

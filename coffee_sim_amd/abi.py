@@ -42,11 +42,15 @@ PRESETS = {
 
 CS_WORKLOAD_V60_SMALL = 0
 CS_WORKLOAD_V60_WORKING = 1
-WORKLOADS = {"v60_small": CS_WORKLOAD_V60_SMALL, "v60_working": CS_WORKLOAD_V60_WORKING}
+CS_WORKLOAD_V60_LARGE = 2
+WORKLOADS = {"v60_small": CS_WORKLOAD_V60_SMALL, "v60_working": CS_WORKLOAD_V60_WORKING,
+             "v60_large": CS_WORKLOAD_V60_LARGE}
 
 # cell kinds (shader.rs:174-178) and metric slots (shader.rs:67-71)
 CELL_AIR, CELL_SURFACE, CELL_INTERIOR, CELL_BED, CELL_SOLID = range(5)
 METRIC_MAX_ABS_DIV, METRIC_FLUID_CELLS, METRIC_DIV_CLAMP, METRIC_P_CLAMP, METRIC_OVERFLOW = range(5)
+# added slots: max |div| and max |div - target| after projection, max |div - target| before the solve
+METRIC_POST_MAX_ABS_DIV, METRIC_POST_MAX_ABS_RES, METRIC_PRE_MAX_ABS_RES = 5, 6, 7
 FP_SCALE = 262144.0
 METRICS_DIV_FP_SCALE = 1024.0
 
@@ -139,6 +143,7 @@ class Settings(C.Structure):
         ("warm_fallback_residual", C.c_float),
         ("rbgs_rescale_after", C.c_uint32),
         ("rbgs_mg_cycles", C.c_uint32),
+        ("emit_sample_period", C.c_uint32),
     ]
 
     @property

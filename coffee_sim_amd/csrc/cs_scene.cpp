@@ -258,7 +258,7 @@ float flow_rate_from_speed(float v, float nozzle_radius, float cd, float vol_to_
 // The golden-angle disk frame of one substep's emission (inflow.rs:165-193):
 // tangents of the spout direction, the contracted emit origin and the velocity.
 static void disk_frame(const cs_spout& sp, Inflow& in, uint32_t count, float dt, float speed, float jet_radius,
-                       float mass, cs_emit_frame* f) {
+                       float mass, uint32_t period, cs_emit_frame* f) {
   const float* dir = sp.direction;
   float up[3] = {0.0f, 1.0f, 0.0f};
   if (!(fabsf(dir[1]) < 0.9f)) {
@@ -276,7 +276,7 @@ static void disk_frame(const cs_spout& sp, Inflow& in, uint32_t count, float dt,
   f->jet_radius = jet_radius;
   f->dt = dt;
   f->mass = mass;
-  f->seq0 = in.emission_sequence;
+  f->seq0 = period ? in.emission_sequence % period : in.emission_sequence;
   f->count = count;
   in.emission_sequence += count;
 }
@@ -296,7 +296,7 @@ uint32_t emit_plan(const cs_settings& s, Inflow& in, float dt, uint32_t current,
     uint32_t count = std::min(want, avail);
     if (dropped) *dropped += want - count;
     if (count == 0) return 0;
-    disk_frame(s.spout, in, count, dt, s.emit_speed, s.emit_jet_radius, mass, f);
+    disk_frame(s.spout, in, count, dt, s.emit_speed, s.emit_jet_radius, mass, s.emit_sample_period, f);
     in.emitted_total += count;
     return count;
   }
@@ -317,7 +317,7 @@ uint32_t emit_plan(const cs_settings& s, Inflow& in, float dt, uint32_t current,
   uint32_t count = std::min(requested, avail);
   if (dropped) *dropped += requested - count;
   if (count == 0) return 0;
-  disk_frame(s.spout, in, count, dt, in.exit_speed, s.spout.nozzle_radius * 0.58f, mass, f);
+  disk_frame(s.spout, in, count, dt, in.exit_speed, s.spout.nozzle_radius * 0.58f, mass, s.emit_sample_period, f);
   return count;
 }
 

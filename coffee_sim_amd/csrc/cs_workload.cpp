@@ -15,6 +15,16 @@
 //               734,004 particles in a pre-wetted slab filling the cone from
 //               the apex, 314,572 more poured at 2,000 per substep; RBGS to
 //               a 1e-4 relative residual.
+//   V60_LARGE   256^3, dx 0.5 mm, apex 32 cells up, bed 48 cells deep with
+//               porosity 0.4 +- 0.08 from seeded value noise (16-cell
+//               lattice); 5,872,026 particles in the pre-wetted slab,
+//               2,516,582 more poured at 16,000 per substep from the 45 degree
+//               spout (8,388,608 in all); substep 2.5e-4 s, half the working
+//               scene's with half its cell, so the jet crosses the same
+//               fraction of a cell per substep; RBGS to 1e-4.  The emitter's
+//               disk samples restart every 65,536 (emit_sample_period): the
+//               reference's running f32 sample index loses the disk's spread
+//               long before 2.5M emitted particles.
 #include <math.h>
 #include <string.h>
 
@@ -130,6 +140,8 @@ struct Spec {
   uint64_t seed;
   float frame_dt;
   uint32_t substeps_total;
+  uint32_t sdf_res;        // SDF texture side
+  uint32_t emit_sample_period = 0u;  // cs_settings.emit_sample_period
 };
 
 int build(const Spec& sp, cs_workload* out);
@@ -137,8 +149,34 @@ int build(const Spec& sp, cs_workload* out);
 }  // namespace
 
 extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs_workload* out) {
-  if (!out || (workload != CS_WORKLOAD_V60_SMALL && workload != CS_WORKLOAD_V60_WORKING)) return CS_ERR_INVALID;
+  if (!out || (workload != CS_WORKLOAD_V60_SMALL && workload != CS_WORKLOAD_V60_WORKING &&
+               workload != CS_WORKLOAD_V60_LARGE))
+    return CS_ERR_INVALID;
   if (scale == 0 || scale > 8) return CS_ERR_INVALID;
+  if (workload == CS_WORKLOAD_V60_LARGE) {
+    const uint32_t s3 = scale * scale * scale;
+    Spec sp;
+    sp.working = true;
+    sp.n = 256u / scale;
+    sp.dx_m = 0.0005f * (float)scale;
+    sp.apex_cells = 32u / scale;
+    sp.bed_cells = 48u / scale;
+    sp.n_total = 8388608u / s3;
+    sp.n_initial = 5872026u / s3;
+    sp.emit_per_substep = std::max(1u, 16000u / s3);
+    sp.noise_wavelength = 16.0f / (float)scale;
+    sp.noise_amp = 0.08f;
+    sp.kettle_deg = 45.0f;
+    sp.aim_dx = 1.0f;
+    sp.aim_dy = -1.0f;
+    sp.radius_cells = 1u;
+    sp.seed = seed;
+    sp.frame_dt = kWorkingDt * 0.5f * (float)scale;
+    sp.substeps_total = 500u;
+    sp.sdf_res = sp.n;
+    sp.emit_sample_period = 65536u;
+    return build(sp, out);
+  }
   const bool working = workload == CS_WORKLOAD_V60_WORKING;
   const uint32_t s3 = scale * scale * scale;
   Spec sp;
@@ -159,6 +197,7 @@ extern "C" int cs_workload_build(int workload, uint64_t seed, uint32_t scale, cs
   sp.seed = seed;
   sp.frame_dt = working ? kWorkingDt : 1.0e-4f;
   sp.substeps_total = working ? 1000u : 200u;
+  sp.sdf_res = 128u;
   return build(sp, out);
 }
 
@@ -186,6 +225,7 @@ extern "C" int cs_workload_build_pour(const cs_pour_params* p, cs_workload* out)
   sp.seed = p->seed;
   sp.frame_dt = kWorkingDt;
   sp.substeps_total = 500u;
+  sp.sdf_res = 128u;
   return build(sp, out);
 }
 
@@ -221,7 +261,8 @@ int build(const Spec& sp, cs_workload* out) {
   s.geometry.cone_enabled = 0u;          // the reference's analytic V60/cup fallbacks
   s.geometry.cup_enabled = 0u;           // describe its own scene, not this one
   s.geometry.filter_enabled = 0u;
-  s.sdf_res = 128u;
+  s.sdf_res = sp.sdf_res;
+  s.emit_sample_period = sp.emit_sample_period;
   const float half = side * 0.5f;
   const float apex_y = -half + (float)apex_cells * dx;
   const float tan60 = 1.7320508f;
@@ -248,7 +289,7 @@ int build(const Spec& sp, cs_workload* out) {
     s.emit_per_substep = sp.emit_per_substep;
     s.emit_total = n_pour;
     s.emit_speed = 0.5f * cs::max_velocity();
-    const float dt = kWorkingDt;
+    const float dt = sp.frame_dt;
     float cells = sqrtf((float)s.emit_per_substep / (8.0f * 3.14159265f * (s.emit_speed * dt / dx)));
     s.emit_jet_radius = cells * dx;
     s.spout.origin[0] = -0.30f * side;

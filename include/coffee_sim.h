@@ -175,6 +175,13 @@ typedef struct cs_settings {
                                  * as before and counts only its own passes.  0 = off.
                                  * Exclusive with rbgs_rescale_after.  No reference
                                  * counterpart. */
+  uint32_t emit_sample_period;  /* 0: the golden-angle disk's sample index is the running
+                                 * emission count (inflow.rs:195-199).  Otherwise each
+                                 * substep's samples start from that count modulo this period:
+                                 * the index is converted to f32 and multiplied by the golden
+                                 * angle, and past a few million samples the products lose the
+                                 * bits that spread the disk, so emitted particles coincide.
+                                 * No reference counterpart. */
 } cs_settings;
 
 /* Per-cell porous bed ("field mode").  The reference stamps bed-particle ids into
@@ -273,10 +280,14 @@ int cs_bed_init(const cs_bed_config* config, const uint32_t grid_dims[3], const 
  *   CS_WORKLOAD_V60_SMALL: 64^3, dx 2 mm, 32,768-particle jittered cylinder over a flat bed.
  *   CS_WORKLOAD_V60_WORKING: 128^3, dx 1 mm, 1,048,576 particles (70% pre-wetted slab,
  *   30% poured at 2,000 particles/substep), value-noise porosity, Kozeny-Carman permeability.
+ *   CS_WORKLOAD_V60_LARGE: 256^3, dx 0.5 mm, 8,388,608 particles (70% pre-wetted slab, 30% poured
+ *   at 16,000 particles/substep), porosity noise at a 16-cell wavelength and +-0.08 amplitude,
+ *   2.5e-4 s substeps.
  * `scale` (>= 1) divides the grid side and particle counts for quick tests (1 = full size).
  * The arrays are allocated by the library; release them with cs_workload_free. */
 #define CS_WORKLOAD_V60_SMALL 0
 #define CS_WORKLOAD_V60_WORKING 1
+#define CS_WORKLOAD_V60_LARGE 2
 typedef struct cs_workload {
   cs_settings settings;
   cs_scene scene;

@@ -317,8 +317,12 @@ static void emit_disk(orc_sim* S, uint32_t slot0, uint32_t count, float dt, floa
   float eo[3] = {sp->origin[0] + dir[0] * 0.18f, sp->origin[1] + dir[1] * 0.18f,
                  sp->origin[2] + dir[2] * 0.18f};
   const float golden = 2.399963f, radial_irr = 0.7548777f;
+  /* emit_sample_period (no reference counterpart): the substep's samples start
+   * from the running count modulo the period */
+  const uint32_t period = S->s.emit_sample_period;
+  const uint32_t seq0 = period ? S->inflow.emission_sequence % period : S->inflow.emission_sequence;
   for (uint32_t i = 0; i < count; ++i) {
-    uint32_t sample = S->inflow.emission_sequence + i;
+    uint32_t sample = seq0 + i;
     float sf = (float)sample;
     float radial_u = rs_fract((sf + 0.5f) * radial_irr);
     float r = jet_radius * sqrtf(radial_u);
