@@ -372,6 +372,8 @@ def bench_learned(args) -> int:
         wk = cs.SeededWorkload("v60_working", seed=1000 + k)
         st = wk.settings.copy()
         st.rbgs_tolerance = 1e-6
+        st.warm_start = CS_WARM_PREVIOUS
+        st.rbgs_mg_cycles = max(args.mg_cycles, 0)  # the targets' solves only: same solution, fewer iterations
         sim = cs.MpmSim3D(ctx, st, wk.scene)
         pour = (wk.settings.emit_total + wk.settings.emit_per_substep - 1) // wk.settings.emit_per_substep
         for j in range(pour):
@@ -398,16 +400,21 @@ def bench_learned(args) -> int:
 
     w = cs.SeededWorkload("v60_working", seed=args.seed)
     rescales = [int(v) for v in args.rescale_after.split(",") if v.strip() and int(v) > 0]
-    starts = [("previous", CS_WARM_PREVIOUS, 0), ("learned", CS_WARM_LEARNED, 0), ("zero", CS_WARM_ZERO, 0)]
+    starts = [("previous", CS_WARM_PREVIOUS, 0, 0), ("learned", CS_WARM_LEARNED, 0, 0), ("zero", CS_WARM_ZERO, 0, 0)]
     for m in rescales:
-        starts += [(f"learned+rescale{m}", CS_WARM_LEARNED, m), (f"previous+rescale{m}", CS_WARM_PREVIOUS, m),
-                   (f"zero+rescale{m}", CS_WARM_ZERO, m)]
+        starts += [(f"learned+rescale{m}", CS_WARM_LEARNED, m, 0), (f"previous+rescale{m}", CS_WARM_PREVIOUS, m, 0),
+                   (f"zero+rescale{m}", CS_WARM_ZERO, m, 0)]
+    # the same starts corrected by multigrid W-cycles before the iteration
+    for c in [int(v) for v in args.compare_mg.split(",") if v.strip() and int(v) > 0]:
+        starts += [(f"previous+mg{c}", CS_WARM_PREVIOUS, 0, c), (f"learned+mg{c}", CS_WARM_LEARNED, 0, c),
+                   (f"zero+mg{c}", CS_WARM_ZERO, 0, c)]
     sims = {}
-    for name, mode, m in starts:
+    for name, mode, m, c in starts:
         st = w.settings.copy()
         st.warm_start = mode
         st.warm_fallback_residual = 20.0
         st.rbgs_rescale_after = m
+        st.rbgs_mg_cycles = c
         sims[name] = cs.MpmSim3D(ctx, st, w.scene)
         if mode == CS_WARM_LEARNED:
             sims[name].set_warm_net(net)
@@ -437,6 +444,9 @@ def bench_learned(args) -> int:
     tot = {k: sum(v) for k, v in its.items()}
     head = "learned"
     ratios = {f"{other}/{head}": round(tot[other] / max(tot[head], 1), 3) for other in tot if other != head}
+    for c in [int(v) for v in args.compare_mg.split(",") if v.strip() and int(v) > 0]:
+        for other in ("previous", "zero"):
+            ratios[f"{other}+mg{c}/learned+mg{c}"] = round(tot[f"{other}+mg{c}"] / max(tot[f"learned+mg{c}"], 1), 3)
     out = {
         "metric": "particle-substeps/s at 1M particles,128³ with the learned warm start; RBGS iters to 1e-4 residual",
         "value": round(n_part * args.steps / secs[head], 2),
@@ -483,6 +493,9 @@ def main() -> int:
     ap.add_argument("--rescale-after", default="96",
                     help="learned128: comma-separated rbgs_rescale_after values whose starts are also measured "
                          "(0 or empty: none)")
+    ap.add_argument("--compare-mg", default="1,3",
+                    help="learned128: multigrid cycle counts at which the starts are also compared "
+                         "(comma-separated; 0 or empty: none)")
     ap.add_argument("--tune-cache", default=None, help="batch64: JSON cache of the autotuner's choices")
     ap.add_argument("--no-tune", action="store_true", help="batch64: default launch shapes, tiled P2G")
     ap.add_argument("--seed", type=int, default=1)
