@@ -13,11 +13,12 @@ hipError_t cs_launch_fill_i32(hipStream_t stream, int32_t* p, size_t n, int32_t 
 hipError_t cs_launch_copy_i32(hipStream_t stream, const int32_t* src, int32_t* dst, size_t n);
 hipError_t cs_launch_p2g(hipStream_t stream, const cs_params& u, const float* part, const float* aff,
                          int32_t* lanes, uint32_t* metrics);
-// Tile-binned P2G (cs_p2g.hip): scratch tile_count / tile_start [nbricks], key [max_particles]
-// (tile, rank), sorted [max_particles].
+// Tile-binned P2G (cs_p2g.hip): scratch tile_count / tile_start [nbricks], tile_chunk
+// [nbricks + 1] (the chunk prefix of the tile pass), key [max_particles] (tile, rank),
+// sorted [max_particles].
 hipError_t cs_launch_p2g_tiled(hipStream_t stream, const cs_params& u, const float* part, const float* aff,
                                int32_t* lanes, uint32_t* metrics, uint32_t* tile_count, uint32_t* tile_start,
-                               uint2* key, uint32_t* sorted);
+                               uint32_t* tile_chunk, uint2* key, uint32_t* sorted);
 hipError_t cs_launch_grid(hipStream_t stream, const cs_params& u, const int32_t* lanes, float* gvel,
                           const float* sdf);
 hipError_t cs_launch_classify(hipStream_t stream, const cs_params& u, const float* gvel, const uint8_t* cls,

@@ -118,32 +118,47 @@ __global__ void __launch_bounds__(256) k_p2g_bin(cs_params u, const float4* __re
   p2g_bin_block<kBinPerThread>(u, part, aff, tile_count, key, blockIdx.x * (256u * kBinPerThread));
 }
 
-// Pass 2: exclusive scan of the tile counts (one workgroup).
+// Pass 2: exclusive scan of the tile counts (one workgroup).  With `chunk`,
+// also the exclusive scan of the tiles' chunk counts ceil(count / kChunk),
+// chunk[ntiles] holding the total: the work items of the chunked tile pass.
+constexpr uint32_t kChunk = 1024;  // particles per work item of k_p2g_tile_chunks
 __device__ __forceinline__ void p2g_scan_block(const uint32_t* __restrict__ count, uint32_t ntiles,
-                                               uint32_t* __restrict__ start, uint32_t* s_sum) {
+                                               uint32_t* __restrict__ start, uint32_t* __restrict__ chunk,
+                                               uint32_t* s_sum, uint32_t* s_ch) {
   const uint32_t t = threadIdx.x;
   const uint32_t per = (ntiles + 1023u) / 1024u;
   const uint32_t b0 = min(t * per, ntiles), b1 = min(b0 + per, ntiles);
-  uint32_t c = 0;
-  for (uint32_t b = b0; b < b1; ++b) c += count[b];
+  uint32_t c = 0, cc = 0;
+  for (uint32_t b = b0; b < b1; ++b) {
+    c += count[b];
+    cc += (count[b] + kChunk - 1u) / kChunk;
+  }
   s_sum[t] = c;
+  s_ch[t] = cc;
   __syncthreads();
   for (uint32_t d = 1; d < 1024u; d <<= 1) {
     const uint32_t v = t >= d ? s_sum[t - d] : 0u;
+    const uint32_t w = t >= d ? s_ch[t - d] : 0u;
     __syncthreads();
     s_sum[t] += v;
+    s_ch[t] += w;
     __syncthreads();
   }
-  uint32_t run = s_sum[t] - c;
+  uint32_t run = s_sum[t] - c, crun = s_ch[t] - cc;
   for (uint32_t b = b0; b < b1; ++b) {
     start[b] = run;
     run += count[b];
+    if (chunk) {
+      chunk[b] = crun;
+      crun += (count[b] + kChunk - 1u) / kChunk;
+    }
   }
+  if (chunk && t == 1023u) chunk[ntiles] = s_ch[t];
 }
 __global__ void __launch_bounds__(1024) k_p2g_scan(const uint32_t* __restrict__ count, uint32_t ntiles,
-                                                   uint32_t* __restrict__ start) {
-  __shared__ uint32_t s_sum[1024];
-  p2g_scan_block(count, ntiles, start, s_sum);
+                                                   uint32_t* __restrict__ start, uint32_t* __restrict__ chunk) {
+  __shared__ uint32_t s_sum[1024], s_ch[1024];
+  p2g_scan_block(count, ntiles, start, chunk, s_sum, s_ch);
 }
 
 // Pass 3: the tile-sorted particle list.
@@ -163,25 +178,26 @@ __global__ void __launch_bounds__(256) k_p2g_scatter(uint32_t n, const uint2* __
 // in a strided order (thread t takes list positions t*K, t*K + 1, ...), so the
 // lanes of one wavefront work on particles far apart in the list and rarely
 // hit the same LDS word in one instruction.
-constexpr uint32_t kTileThreads = 256;
-__device__ __forceinline__ void p2g_tile_block(const cs_params& u, const float4* __restrict__ part,
-                                               const float4* __restrict__ aff,
-                                               const uint32_t* __restrict__ tile_count,
-                                               const uint32_t* __restrict__ start,
-                                               const uint32_t* __restrict__ sorted, int32_t* __restrict__ lanes,
+constexpr uint32_t kTileThreads = 256;  // the lane dealing below assumes 256
+// The particles [first, first + n) of `tile`'s sorted list into the tile's
+// 10^3-node LDS block, then the block's nonzero entries to the lanes.
+__device__ __forceinline__ void p2g_tile_range(const cs_params& u, const float4* __restrict__ part,
+                                               const float4* __restrict__ aff, const uint32_t* __restrict__ sorted,
+                                               uint32_t first, uint32_t n, int32_t* __restrict__ lanes,
                                                uint32_t* __restrict__ metrics, uint32_t tile, int32_t* s_acc) {
   constexpr uint32_t NT = kTileThreads;
-  const uint32_t n = tile_count[tile];
-  if (n == 0) return;
-  const uint32_t first = start[tile];
   for (uint32_t i = threadIdx.x; i < 6u * FP3; i += NT) s_acc[i] = 0;
   const uint32_t tx = tile % u.nbx, ty = (tile / u.nbx) % u.nby, tz = tile / (u.nbx * u.nby);
   const int ox = (int)(tx * TB), oy = (int)(ty * TB), oz = (int)(tz * TB);
   __syncthreads();
   const uint32_t per = (n + NT - 1) / NT;
+  // thread t takes list positions rev(t) * per + r: the bit-reversed thread
+  // index spreads a wavefront's lanes over the whole range, so they rarely
+  // share a cell and hit the same LDS word in one instruction
+  const uint32_t lead = __brev(threadIdx.x) >> (32 - 8);
   uint32_t overflow = 0;
   for (uint32_t r = 0; r < per; ++r) {
-    const uint32_t pos = threadIdx.x * per + r;
+    const uint32_t pos = lead * per + r;
     if (pos >= n) break;
     const uint32_t pid = sorted[first + pos];
     P2GParticle q;
@@ -222,14 +238,44 @@ __device__ __forceinline__ void p2g_tile_block(const cs_params& u, const float4*
   }
   if (overflow) atomicAdd(&metrics[CS_M_OVERFLOW], overflow);
 }
-__global__ void __launch_bounds__(kTileThreads) k_p2g_tile(cs_params u, const float4* __restrict__ part,
-                                                           const float4* __restrict__ aff,
-                                                           const uint32_t* __restrict__ tile_count,
-                                                           const uint32_t* __restrict__ start,
-                                                           const uint32_t* __restrict__ sorted,
-                                                           int32_t* __restrict__ lanes, uint32_t* __restrict__ metrics) {
+// Pass 4 (batched scenes): one workgroup per tile.
+__device__ __forceinline__ void p2g_tile_block(const cs_params& u, const float4* __restrict__ part,
+                                               const float4* __restrict__ aff,
+                                               const uint32_t* __restrict__ tile_count,
+                                               const uint32_t* __restrict__ start,
+                                               const uint32_t* __restrict__ sorted, int32_t* __restrict__ lanes,
+                                               uint32_t* __restrict__ metrics, uint32_t tile, int32_t* s_acc) {
+  const uint32_t n = tile_count[tile];
+  if (n == 0) return;
+  p2g_tile_range(u, part, aff, sorted, start[tile], n, lanes, metrics, tile, s_acc);
+}
+// Pass 4 (one scene): one workgroup per chunk of at most kChunk particles of a
+// tile's list, so a tile holding several thousand particles is spread over
+// several workgroups (and CUs) instead of serialising on one; each chunk
+// flushes its own block (the lanes are added with i32 atomics, so the result
+// does not depend on the split).  Workgroup w finds its tile as the last one
+// whose chunk prefix is at most w.
+__global__ void __launch_bounds__(kTileThreads) k_p2g_tile_chunks(cs_params u, const float4* __restrict__ part,
+                                                                  const float4* __restrict__ aff,
+                                                                  const uint32_t* __restrict__ tile_count,
+                                                                  const uint32_t* __restrict__ start,
+                                                                  const uint32_t* __restrict__ chunk,
+                                                                  const uint32_t* __restrict__ sorted,
+                                                                  int32_t* __restrict__ lanes,
+                                                                  uint32_t* __restrict__ metrics) {
   __shared__ int32_t s_acc[6 * FP3];
-  p2g_tile_block(u, part, aff, tile_count, start, sorted, lanes, metrics, blockIdx.x, s_acc);
+  const uint32_t w = blockIdx.x;
+  if (w >= chunk[u.nbricks]) return;
+  uint32_t lo = 0, hi = u.nbricks;  // chunk[lo] <= w < chunk[hi]
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (chunk[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t c = w - chunk[lo];
+  const uint32_t n = tile_count[lo];
+  const uint32_t off = c * kChunk;
+  p2g_tile_range(u, part, aff, sorted, start[lo] + off, min(kChunk, n - off), lanes, metrics, lo, s_acc);
 }
 
 // ---- batched passes (one launch over every scene of a batch) -------------------------------
@@ -255,11 +301,11 @@ __global__ void __launch_bounds__(256) k_batch_p2g_bin(const cs_scene_dev* __res
 
 __global__ void __launch_bounds__(1024) k_batch_p2g_scan(const cs_scene_dev* __restrict__ sc,
                                                          const uint32_t* __restrict__ starts, uint32_t n) {
-  __shared__ uint32_t s_sum[1024];
+  __shared__ uint32_t s_sum[1024], s_ch[1024];
   CS_BATCH_SCENE(CS_SPACE_SCENES);
   (void)blk;
   if (!d.p2g_tiled) return;
-  p2g_scan_block(d.tile_count, u.nbricks, d.tile_start, s_sum);
+  p2g_scan_block(d.tile_count, u.nbricks, d.tile_start, nullptr, s_sum, s_ch);
 }
 
 __global__ void __launch_bounds__(256) k_batch_p2g_scatter(const cs_scene_dev* __restrict__ sc,
@@ -291,7 +337,7 @@ hipError_t cs_batch_launch_p2g(const cs_batch_launch& b) {
 
 hipError_t cs_launch_p2g_tiled(hipStream_t s, const cs_params& u, const float* part, const float* aff,
                                int32_t* lanes, uint32_t* metrics, uint32_t* tile_count, uint32_t* tile_start,
-                               uint2* key, uint32_t* sorted) {
+                               uint32_t* tile_chunk, uint2* key, uint32_t* sorted) {
   if (u.num_particles == 0) return hipSuccess;
   const dim3 pb((u.num_particles + 255u) / 256u);
   hipError_t e = cs_launch_fill_i32(s, reinterpret_cast<int32_t*>(tile_count), u.nbricks, 0);
@@ -299,9 +345,11 @@ hipError_t cs_launch_p2g_tiled(hipStream_t s, const cs_params& u, const float* p
   const uint32_t per_block = 256u * kBinPerThread;
   hipLaunchKernelGGL(k_p2g_bin, dim3((u.num_particles + per_block - 1u) / per_block), dim3(256), 0, s, u,
                      reinterpret_cast<const float4*>(part), reinterpret_cast<const float4*>(aff), tile_count, key);
-  hipLaunchKernelGGL(k_p2g_scan, dim3(1), dim3(1024), 0, s, tile_count, u.nbricks, tile_start);
+  hipLaunchKernelGGL(k_p2g_scan, dim3(1), dim3(1024), 0, s, tile_count, u.nbricks, tile_start, tile_chunk);
   hipLaunchKernelGGL(k_p2g_scatter, pb, dim3(256), 0, s, u.num_particles, key, tile_start, sorted);
-  hipLaunchKernelGGL(k_p2g_tile, dim3(u.nbricks), dim3(256), 0, s, u, reinterpret_cast<const float4*>(part),
-                     reinterpret_cast<const float4*>(aff), tile_count, tile_start, sorted, lanes, metrics);
+  // at most one partial chunk per tile beyond the full ones
+  const uint32_t items = u.nbricks + (u.num_particles + kChunk - 1u) / kChunk;
+  hipLaunchKernelGGL(k_p2g_tile_chunks, dim3(items), dim3(kTileThreads), 0, s, u, reinterpret_cast<const float4*>(part),
+                     reinterpret_cast<const float4*>(aff), tile_count, tile_start, tile_chunk, sorted, lanes, metrics);
   return hipGetLastError();
 }

@@ -61,7 +61,7 @@ void free_sim(cs_sim* m) {
                  m->rhs,        m->pack,       m->prev_p,     m->warm,       m->metrics, m->res_partials,
                  m->res_out,    m->pres2,      m->wc,         m->group_partials, m->sweep_table,
                  m->brick_flag, m->brick_list, m->brick_partials, m->dstate,       m->tile_count,
-                 m->tile_start, m->p2g_key,    m->p2g_sorted, m->render,     m->bed_part,
+                 m->tile_start, m->tile_chunk, m->p2g_key,    m->p2g_sorted, m->render,     m->bed_part,
                  m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt,
                  m->bed_snap,   m->alias4,
                  m->persist_fb, m->persist_pflags, m->persist_partials, m->persist_slot, m->persist_xcc,
@@ -802,7 +802,7 @@ int substep(cs_sim* m, float sub_dt) {
                                 m->bed_lookup, m->bed_counter, m->bed_id, m->bed));
   kmark(m, 0);
   if (m->p2g_tiled)
-    CS_HIP(cs_launch_p2g_tiled(st, u, m->part, m->aff, m->lanes, m->metrics, m->tile_count, m->tile_start,
+    CS_HIP(cs_launch_p2g_tiled(st, u, m->part, m->aff, m->lanes, m->metrics, m->tile_count, m->tile_start, m->tile_chunk,
                                m->p2g_key, m->p2g_sorted));
   else
     CS_HIP(cs_launch_p2g(st, u, m->part, m->aff, m->lanes, m->metrics));
@@ -1029,6 +1029,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   ALLOC(m->sweep_table, CS_SWEEP_TABLE_WORDS);
   ALLOC(m->tile_count, geom.nbricks);
   ALLOC(m->tile_start, geom.nbricks);
+  ALLOC(m->tile_chunk, geom.nbricks + 1);
   ALLOC(m->p2g_key, mp);
   ALLOC(m->p2g_sorted, mp);
   ALLOC(m->brick_flag, geom.nbricks);

@@ -118,6 +118,7 @@ struct cs_sim {
   bool p2g_tiled = true;             // CS_P2G=atomic selects the one-thread-per-particle layout
   uint32_t* tile_count = nullptr;    // [nbricks]
   uint32_t* tile_start = nullptr;    // [nbricks]
+  uint32_t* tile_chunk = nullptr;    // [nbricks + 1] the chunked tile pass's work-item prefix
   uint2* p2g_key = nullptr;          // [max_particles]
   uint32_t* p2g_sorted = nullptr;    // [max_particles]
   cs_unet* warm_net = nullptr;       // network of CS_WARM_LEARNED
