@@ -32,7 +32,10 @@ algorithmic GB/s (`config.kernel_gbs`).
 
 The JSON line also carries
   roofline      for the kernel with the largest share of device time, from
-                HIP events recorded on the solver's stream over the timed region
+                HIP events recorded on the solver's stream around each kernel
+                family over a window of substeps right after the timed region
+                (the timed substeps are graph replays, which carry no events;
+                the window launches the same kernels one by one)
   cpu_baseline  the CPU oracle (its OpenMP build on 16 host threads, the GPU's
                 share of the host; --cpu-threads 1 times the serial build)
                 on a bounded sample of the same workload state
@@ -483,6 +486,8 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed substeps (20; batch64: 5)")
+    ap.add_argument("--timing-steps", type=int, default=10,
+                    help="substeps of the per-kernel timing window after the timed region")
     ap.add_argument("--warmup", type=int, default=None, help="untimed substeps before timing (3; batch64: 1)")
     ap.add_argument("--workload", choices=("v60_working", "v60_large", "batch64", "unet128", "learned128"),
                     default="v60_working")
@@ -576,8 +581,9 @@ def main() -> int:
     n_particles = sim.particle_count()
     log(f"warmup done; timing {args.steps} substeps at {n_particles} particles")
 
-    sim.enable_timing(True)
-    sim.reset_kernel_timing()
+    # the timed region: the product path, each substep one replay of the
+    # recorded substep graph (no per-kernel events in it)
+    replayed0 = sim.recorded_counts()[0]
     iters_timed = []
     sim.synchronize()
     t0 = time.perf_counter()
@@ -588,10 +594,26 @@ def main() -> int:
         psteps += sim.particle_count()
     sim.synchronize()
     elapsed = time.perf_counter() - t0
+    replayed = sim.recorded_counts()[0] - replayed0
+    log(f"timed {args.steps} substeps in {elapsed:.3f}s ({replayed} of them graph replays); "
+        f"RBGS iterations {iters_timed}")
+    # per-kernel durations: the same kernels launched one by one with HIP
+    # events around each family, over the substeps right after the timed region
+    timing_steps = min(args.steps, args.timing_steps)
+    sim.enable_timing(True)
+    sim.reset_kernel_timing()
+    sim.synchronize()
+    t1 = time.perf_counter()
+    psteps_timing = 0
+    for k in range(timing_steps):
+        sim.step_frame(dt)
+        sim.solve_stats()
+        psteps_timing += sim.particle_count()
+    sim.synchronize()
+    elapsed_timing = time.perf_counter() - t1
     kt = sim.kernel_timing()
     sim.enable_timing(False)
-    log(f"timed {args.steps} substeps in {elapsed:.3f}s; RBGS iterations {iters_timed}")
-    log(f"kernel timing {kt.as_dict()}")
+    log(f"kernel timing window: {timing_steps} substeps launch by launch in {elapsed_timing:.3f}s; {kt.as_dict()}")
 
     zero_probe = None
     plain_probe = None
@@ -657,8 +679,11 @@ def main() -> int:
             "rbgs_zero_start_probe": zero_probe,
             "rbgs_previous_start_probe_without_mg": plain_probe,
             "scale": args.scale,
+            "substeps_as_graph_replays": int(replayed),
+            "launch_by_launch_value": round(psteps_timing / elapsed_timing, 2) if elapsed_timing > 0 else None,
         },
-        "roofline": roofline(kt),
+        "roofline": dict(roofline(kt), window=f"HIP events over {timing_steps} launch-by-launch substeps after "
+                                              f"the timed region (the replayed kernels, one launch each)"),
         "cpu_baseline": cpu,
     }
     if wname != "v60_working":

@@ -67,6 +67,7 @@ def _bind(lib: C.CDLL) -> None:
         "cs_sim_destroy": (C.c_int, [C.c_void_p]),
         "cs_sim_reset": (C.c_int, [C.c_void_p]),
         "cs_sim_step_frame": (C.c_int, [C.c_void_p, C.c_float]),
+        "cs_sim_get_recorded_counts": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint64)]),
         "cs_sim_synchronize": (C.c_int, [C.c_void_p]),
         "cs_sim_set_kettle_angle": (C.c_int, [C.c_void_p, C.c_float]),
         "cs_sim_set_spout_position": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
@@ -447,6 +448,13 @@ class MpmSim3D:
 
     def synchronize(self) -> None:
         _check(load_library().cs_sim_synchronize(self._h), "cs_sim_synchronize")
+
+    def recorded_counts(self) -> tuple[int, int]:
+        """(substeps run as graph replays, replays finished launch by launch after
+        halting at the brick scan) since creation."""
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        _check(load_library().cs_sim_get_recorded_counts(self._h, C.byref(a), C.byref(b)), "get_recorded_counts")
+        return int(a.value), int(b.value)
 
     # ---- setters / getters (mod.rs:448-538) --------------------------------------------
     def set_kettle_angle(self, angle_deg: float) -> None:

@@ -9,6 +9,7 @@
 #pragma once
 
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/coffee_sim.h"
@@ -216,6 +217,33 @@ struct cs_params {
   float j_alpha, j_expand_alpha, max_rest_fraction;
   float wall_half;
   cs_geometry geo;
+};
+
+// The device-resident parameter block of a single-scene simulation (the
+// reference writes its uniform buffer once per substep, mod.rs:569-670).  The
+// host writes the part before `nactive` with one copy ahead of each substep's
+// launches; the substep's kernels read their parameters here, not from their
+// arguments, so one recorded launch sequence (a graph) serves every substep.
+// The device writes the rest.
+struct cs_dparams {
+  cs_params u;
+  cs_emit_frame ef;     // this substep's emission: ef.count particles from slot ef.first
+  uint32_t epoch_base;  // the persistent solve's epochs are epoch_base + 1, 2, ...
+  uint32_t slot;        // the substep's entry in the simulation's solve records
+  uint32_t cap_bricks;  // active bricks the launches are sized for; a substep with more halts
+  uint32_t reserved;
+  // written on the device
+  uint32_t nactive;  // the substep's active bricks (the brick scan)
+  uint32_t halt;     // set (to slot + 1) by the brick scan when nactive > cap_bricks and then
+                     // kept: every kernel that reads the block returns at once
+};
+#define CS_DPARAMS_HOST_BYTES offsetof(cs_dparams, nactive)
+
+// What a launcher takes: the host copy of the parameters (launch shapes) and
+// the device block the kernels read.
+struct cs_pblock {
+  cs_params u;
+  cs_dparams* d;
 };
 
 // A bed of particles as the reaction impulses see it (shader.rs:458-515):

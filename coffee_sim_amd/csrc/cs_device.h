@@ -5,6 +5,12 @@
 
 #include "cs_common.h"
 
+// A single-scene kernel's parameters, read from the substep's device block.  A
+// halted block makes the kernel return before it reads or writes anything else.
+#define CS_PARAMS_FROM(dp) \
+  if ((dp)->halt) return;  \
+  const cs_params u = (dp)->u
+
 namespace {
 
 __device__ __forceinline__ uint32_t cell_index(const cs_params& u, uint32_t ix, uint32_t iy, uint32_t iz) {

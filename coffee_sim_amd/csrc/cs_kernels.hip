@@ -25,13 +25,21 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, size_t n, int32_t v) {
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
 }
+// The same, skipped when the substep's parameter block is halted.
+__global__ void k_fill_i32_gated(const cs_dparams* __restrict__ dp, int32_t* __restrict__ p, size_t n, int32_t v) {
+  if (dp->halt) return;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
 
 // ---- P2G (shader.rs:955-1052), one thread per particle, global i32 atomics -------
 // The layout of the reference dispatch; the tiled P2G of cs_p2g.hip is the
 // default and gives the same lanes bit for bit (CS_P2G=atomic selects this).
-__global__ void __launch_bounds__(256) k_p2g_atomic(cs_params u, const float4* __restrict__ part,
+__global__ void __launch_bounds__(256) k_p2g_atomic(const cs_dparams* __restrict__ dp, const float4* __restrict__ part,
                                                     const float4* __restrict__ aff, int32_t* __restrict__ lanes,
                                                     uint32_t* __restrict__ metrics) {
+  CS_PARAMS_FROM(dp);
   p2g_atomic_particle(u, part, aff, lanes, metrics, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -54,8 +62,9 @@ __device__ __forceinline__ void grid_cell(const cs_params& u, const int32_t* __r
   v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
   gvel[idx] = make_float4(v.x, v.y, v.z, mass);
 }
-__global__ void __launch_bounds__(256) k_grid(cs_params u, const int32_t* __restrict__ lanes,
+__global__ void __launch_bounds__(256) k_grid(const cs_dparams* __restrict__ dp, const int32_t* __restrict__ lanes,
                                               float4* __restrict__ gvel, const float* __restrict__ sdf) {
+  CS_PARAMS_FROM(dp);
   grid_cell(u, lanes, gvel, sdf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -246,7 +255,7 @@ __device__ __forceinline__ void classify_wave(const cs_params& u, const float4* 
     }
   }
 }
-__global__ void __launch_bounds__(256) k_classify(cs_params u, const float4* __restrict__ gvel,
+__global__ void __launch_bounds__(256) k_classify(const cs_dparams* __restrict__ dp, const float4* __restrict__ gvel,
                                                   const uint8_t* __restrict__ cls, const int32_t* __restrict__ lanes,
                                                   const int32_t* __restrict__ bed_id,
                                                   const cs_bed_cell* __restrict__ bed, int8_t* __restrict__ kind_out,
@@ -255,6 +264,7 @@ __global__ void __launch_bounds__(256) k_classify(cs_params u, const float4* __r
                                                   float* __restrict__ tgt,
                                                   uint32_t* __restrict__ metrics, const float* __restrict__ sdf,
                                                   uint32_t* __restrict__ brick_flag) {
+  CS_PARAMS_FROM(dp);
   classify_wave(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, tgt, metrics, sdf, brick_flag,
                 blockIdx.x * blockDim.x + threadIdx.x);
 }
@@ -265,8 +275,10 @@ __device__ __forceinline__ void warm_previous_cell(const cs_params& u, const flo
   if (idx >= u.nc) return;
   if (kf_fluid(kf[idx])) pres[idx] = prev[idx];
 }
-__global__ void k_warm_previous(cs_params u, const float* __restrict__ kf, const int32_t* __restrict__ prev,
+__global__ void k_warm_previous(const cs_dparams* __restrict__ dp,
+                                const float* __restrict__ kf, const int32_t* __restrict__ prev,
                                 int32_t* __restrict__ pres) {
+  CS_PARAMS_FROM(dp);
   warm_previous_cell(u, kf, prev, pres, blockIdx.x * blockDim.x + threadIdx.x);
 }
 __global__ void k_warm_external(cs_params u, const float* __restrict__ kf, const float* __restrict__ warm,
@@ -280,6 +292,12 @@ __global__ void k_warm_external(cs_params u, const float* __restrict__ kf, const
   pres[idx] = cs_f2i(cl * u.fp_scale);
 }
 __global__ void k_copy_i32(const int32_t* __restrict__ src, int32_t* __restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+__global__ void k_copy_i32_gated(const cs_dparams* __restrict__ dp,
+                                 const int32_t* __restrict__ src, int32_t* __restrict__ dst, size_t n) {
+  if (dp->halt) return;
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
 }
@@ -475,10 +493,12 @@ __device__ __forceinline__ void project_cell(const cs_params& u, float4* __restr
   v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
   gvel[idx] = make_float4(v.x, v.y, v.z, gv.w);
 }
-__global__ void __launch_bounds__(256) k_project(cs_params u, float4* __restrict__ gvel, const float* __restrict__ kf,
+__global__ void __launch_bounds__(256) k_project(const cs_dparams* __restrict__ dp,
+                                                 float4* __restrict__ gvel, const float* __restrict__ kf,
                                                  const int32_t* __restrict__ pres, const int32_t* __restrict__ bed_id,
                                                  const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf,
                                                  cs_grains gr) {
+  CS_PARAMS_FROM(dp);
   project_cell(u, gvel, kf, pres, bed_id, bed, sdf, gr, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -530,10 +550,13 @@ __global__ void __launch_bounds__(256) k_post_div(cs_params u, const float4* __r
 }
 // The same over the active bricks only (every fluid cell lies in one): one
 // workgroup of 256 per brick, two cells a thread.
-__global__ void __launch_bounds__(256) k_post_div_bricks(cs_params u, const uint32_t* __restrict__ list,
+__global__ void __launch_bounds__(256) k_post_div_bricks(const cs_dparams* __restrict__ dp,
+                                                         const uint32_t* __restrict__ list,
                                                          const float4* __restrict__ gvel, const float* __restrict__ kf,
                                                          const uint8_t* __restrict__ cls, const float* __restrict__ sdf,
                                                          const float* __restrict__ tgt, uint32_t* __restrict__ metrics) {
+  CS_PARAMS_FROM(dp);
+  if (blockIdx.x >= dp->nactive) return;
   const uint32_t b = list[blockIdx.x];
   const uint32_t x0 = (b % u.nbx) * CS_BRICK, y0 = ((b / u.nbx) % u.nby) * CS_BRICK, z0 = (b / (u.nbx * u.nby)) * CS_BRICK;
   uint32_t fd = 0u, fr = 0u;
@@ -576,8 +599,10 @@ __device__ __forceinline__ void pack_cell(const cs_params& u, float4* __restrict
   v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
   gvel[idx] = make_float4(v.x, v.y, v.z, gv.w);
 }
-__global__ void __launch_bounds__(256) k_pack(cs_params u, float4* __restrict__ gvel, const float* __restrict__ kf,
+__global__ void __launch_bounds__(256) k_pack(const cs_dparams* __restrict__ dp,
+                                              float4* __restrict__ gvel, const float* __restrict__ kf,
                                               const int32_t* __restrict__ pack, const float* __restrict__ sdf) {
+  CS_PARAMS_FROM(dp);
   pack_cell(u, gvel, kf, pack, sdf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -633,9 +658,11 @@ __device__ __forceinline__ void visc_cell(const cs_params& u, const float4* __re
   v = boundary_apply(u, sdf, v, cell_center(u, ix, iy, iz));
   gout[idx] = make_float4(v.x, v.y, v.z, gv.w);
 }
-__global__ void __launch_bounds__(256) k_visc(cs_params u, const float4* __restrict__ gin, float4* __restrict__ gout,
+__global__ void __launch_bounds__(256) k_visc(const cs_dparams* __restrict__ dp,
+                                              const float4* __restrict__ gin, float4* __restrict__ gout,
                                               const uint8_t* __restrict__ cls, const float* __restrict__ sdf,
                                               uint8_t* __restrict__ alias4) {
+  CS_PARAMS_FROM(dp);
   visc_cell(u, gin, gout, cls, sdf, alias4, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -791,10 +818,12 @@ __device__ __forceinline__ void g2p_particle(const cs_params& u, float4* __restr
   aff[3 * pid + 1] = make_float4(C1.x, C1.y, C1.z, 0.0f);
   aff[3 * pid + 2] = make_float4(C2.x, C2.y, C2.z, 0.0f);
 }
-__global__ void __launch_bounds__(256) k_g2p(cs_params u, float4* __restrict__ part, float4* __restrict__ aff,
+__global__ void __launch_bounds__(256) k_g2p(const cs_dparams* __restrict__ dp,
+                                             float4* __restrict__ part, float4* __restrict__ aff,
                                              const float4* __restrict__ gvel, const int32_t* __restrict__ bed_id,
                                              const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf,
                                              cs_grains gr) {
+  CS_PARAMS_FROM(dp);
   g2p_particle(u, part, aff, gvel, bed_id, bed, sdf, gr, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -834,8 +863,9 @@ __global__ void __launch_bounds__(256) k_pack_render(cs_params u, const float4* 
 
 // ---- bed of particles: lookup and table (shader.rs:2291-2335, 380-449, 621-666) -------
 // bed_lookup_clear, plus the table slot counter of k_bed_table.
-__global__ void __launch_bounds__(256) k_bed_lookup_clear(uint32_t nc, int32_t* __restrict__ lookup,
-                                                          uint32_t* __restrict__ counter) {
+__global__ void __launch_bounds__(256) k_bed_lookup_clear(const cs_dparams* __restrict__ dp, uint32_t nc,
+                                                          int32_t* __restrict__ lookup, uint32_t* __restrict__ counter) {
+  if (dp->halt) return;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *counter = 0u;
   if (i < nc) lookup[i] = -1;
@@ -844,10 +874,11 @@ __global__ void __launch_bounds__(256) k_bed_lookup_clear(uint32_t nc, int32_t* 
 // bed_lookup_scatter: every non-water bed particle stamps its id into the 3x3x3
 // cells around world_to_cell(pos) (shader.rs:668-671) with atomicMax, so the
 // highest id wins whatever the order.
-__global__ void __launch_bounds__(256) k_bed_lookup_scatter(cs_params u, uint32_t nbed,
+__global__ void __launch_bounds__(256) k_bed_lookup_scatter(const cs_dparams* __restrict__ dp, uint32_t nbed,
                                                             const float4* __restrict__ bed_part,
                                                             const float4* __restrict__ bed_aff,
                                                             int32_t* __restrict__ lookup) {
+  CS_PARAMS_FROM(dp);
   const uint32_t bid = blockIdx.x * blockDim.x + threadIdx.x;
   if (bid >= nbed) return;
   if (bed_aff[3 * bid].w < 0.5f) return;  // is_water_phase (PHASE_WATER_MAX = 0.5)
@@ -871,11 +902,13 @@ __global__ void __launch_bounds__(256) k_bed_lookup_scatter(cs_params u, uint32_
 // (cs_bed_particle_record), in a slot taken from `counter`; bed_id maps each
 // cell to its slot (-1 elsewhere).  The slots' order varies from run to run;
 // every reader goes through bed_id, so no result does.
-__global__ void __launch_bounds__(256) k_bed_table(cs_params u, const int32_t* __restrict__ lookup,
+__global__ void __launch_bounds__(256) k_bed_table(const cs_dparams* __restrict__ dp,
+                                                   const int32_t* __restrict__ lookup,
                                                    const float* __restrict__ bed_part,
                                                    const float* __restrict__ bed_ext, uint32_t capacity,
                                                    uint32_t* __restrict__ counter, int32_t* __restrict__ bed_id,
                                                    cs_bed_cell* __restrict__ table) {
+  CS_PARAMS_FROM(dp);
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= u.nc) return;
   const int32_t id = lookup[c];
@@ -921,11 +954,12 @@ __device__ __forceinline__ void bed_water_add(const cs_params& u, int32_t* __res
 // so it is not carried.  Every credit is an integer atomic and each particle
 // reads only the bed state of the previous extraction pass, so the result does
 // not depend on the order.
-__global__ void __launch_bounds__(256) k_bed_coupling(cs_params u, float4* __restrict__ part,
+__global__ void __launch_bounds__(256) k_bed_coupling(const cs_dparams* __restrict__ dp, float4* __restrict__ part,
                                                       const float4* __restrict__ aff,
                                                       const int32_t* __restrict__ lookup, uint32_t nbed,
                                                       const float* __restrict__ bed_ext, float max_sat,
                                                       int32_t* __restrict__ dw) {
+  CS_PARAMS_FROM(dp);
   const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
   if (pid >= u.num_particles) return;
   if (!(aff[3 * pid].w < 0.5f)) return;  // is_water_phase
@@ -985,8 +1019,10 @@ __global__ void __launch_bounds__(256) k_bed_coupling(cs_params u, float4* __res
 
 // extraction_advect: each bed particle takes (and clears) its water credit into
 // its pore water and saturation, then moves extractable to dissolved mass.
-__global__ void __launch_bounds__(256) k_extraction_advect(cs_params u, uint32_t nbed, float max_sat,
+__global__ void __launch_bounds__(256) k_extraction_advect(const cs_dparams* __restrict__ dp,
+                                                           uint32_t nbed, float max_sat,
                                                            int32_t* __restrict__ dw, float* __restrict__ bed_ext) {
+  CS_PARAMS_FROM(dp);
   const uint32_t bid = blockIdx.x * blockDim.x + threadIdx.x;
   if (bid >= nbed) return;
   float* be = bed_ext + 8 * (size_t)bid;
@@ -1004,15 +1040,16 @@ __global__ void __launch_bounds__(256) k_extraction_advect(cs_params u, uint32_t
   }
 }
 
-hipError_t cs_launch_bed_coupling(hipStream_t st, const cs_params& u, float* part, const float* aff,
+hipError_t cs_launch_bed_coupling(hipStream_t st, const cs_pblock& P, float* part, const float* aff,
                                   const int32_t* lookup, uint32_t nbed, float* bed_ext, int32_t* dw) {
+  const cs_params& u = P.u;
   if (nbed == 0) return hipSuccess;
   const float max_sat = CS_TARGET_BED_RETENTION_ML * 80.0f / (float)nbed;  // mod.rs:577-581
   if (u.num_particles)
-    hipLaunchKernelGGL(k_bed_coupling, dim3((u.num_particles + 255) / 256), dim3(256), 0, st, u,
+    hipLaunchKernelGGL(k_bed_coupling, dim3((u.num_particles + 255) / 256), dim3(256), 0, st, P.d,
                        reinterpret_cast<float4*>(part), reinterpret_cast<const float4*>(aff), lookup, nbed, bed_ext,
                        max_sat, dw);
-  hipLaunchKernelGGL(k_extraction_advect, dim3((nbed + 255) / 256), dim3(256), 0, st, u, nbed, max_sat, dw, bed_ext);
+  hipLaunchKernelGGL(k_extraction_advect, dim3((nbed + 255) / 256), dim3(256), 0, st, P.d, nbed, max_sat, dw, bed_ext);
   return hipGetLastError();
 }
 
@@ -1093,13 +1130,15 @@ __global__ void __launch_bounds__(256) k_bed_snapshot(uint32_t nbed, const float
 // water or the anchored, plastically compacting matrix motion around its rest
 // position; bed_damping 8.5, bed_compaction_rate 5.5, bed_impact 8.0
 // (mod.rs:644-649, brew_config.rs:69-70).
-__global__ void __launch_bounds__(256) k_bed_dynamics(cs_params u, uint32_t nbed, float4* __restrict__ bed_part,
+__global__ void __launch_bounds__(256) k_bed_dynamics(const cs_dparams* __restrict__ dp,
+                                                      uint32_t nbed, float4* __restrict__ bed_part,
                                                       float4* __restrict__ bed_aff, float* __restrict__ bed_ext,
                                                       int32_t* __restrict__ dw, const float4* __restrict__ snap,
                                                       const int32_t* __restrict__ lookup,
                                                       const float4* __restrict__ gvel, const int32_t* __restrict__ pres,
                                                       const uint8_t* __restrict__ alias4,
                                                       const float* __restrict__ sdf) {
+  CS_PARAMS_FROM(dp);
   const uint32_t bid = blockIdx.x * blockDim.x + threadIdx.x;
   if (bid >= nbed) return;
   const float bed_damping = 8.5f, bed_compaction = 5.5f, bed_impact = 8.0f;
@@ -1262,28 +1301,30 @@ __global__ void __launch_bounds__(256) k_bed_dynamics(cs_params u, uint32_t nbed
   be[3] = cs_max(be[3] * 0.995f, gc);
 }
 
-hipError_t cs_launch_bed_dynamics(hipStream_t st, const cs_params& u, uint32_t nbed, float* bed_part, float* bed_aff,
+hipError_t cs_launch_bed_dynamics(hipStream_t st, const cs_pblock& P, uint32_t nbed, float* bed_part, float* bed_aff,
                                   float* bed_ext, int32_t* dw, float* snap, const int32_t* lookup, const float* gvel,
                                   const int32_t* pres, const uint8_t* alias4, const float* sdf) {
+  const cs_params& u = P.u;
   if (nbed == 0) return hipSuccess;
   const dim3 g((nbed + 255u) / 256u);
   hipLaunchKernelGGL(k_bed_snapshot, g, dim3(256), 0, st, nbed, reinterpret_cast<const float4*>(bed_part),
                      reinterpret_cast<float4*>(snap));
-  hipLaunchKernelGGL(k_bed_dynamics, g, dim3(256), 0, st, u, nbed, reinterpret_cast<float4*>(bed_part),
+  hipLaunchKernelGGL(k_bed_dynamics, g, dim3(256), 0, st, P.d, nbed, reinterpret_cast<float4*>(bed_part),
                      reinterpret_cast<float4*>(bed_aff), bed_ext, dw, reinterpret_cast<const float4*>(snap), lookup,
                      reinterpret_cast<const float4*>(gvel), pres, alias4, sdf);
   return hipGetLastError();
 }
 
-hipError_t cs_launch_bed_lookup(hipStream_t st, const cs_params& u, uint32_t nbed, const float* bed_part,
+hipError_t cs_launch_bed_lookup(hipStream_t st, const cs_pblock& P, uint32_t nbed, const float* bed_part,
                                 const float* bed_aff, const float* bed_ext, uint32_t capacity, int32_t* lookup,
                                 uint32_t* counter, int32_t* bed_id, cs_bed_cell* table) {
+  const cs_params& u = P.u;
   const uint32_t nc = (uint32_t)u.nc;
-  hipLaunchKernelGGL(k_bed_lookup_clear, dim3((nc + 255) / 256), dim3(256), 0, st, nc, lookup, counter);
+  hipLaunchKernelGGL(k_bed_lookup_clear, dim3((nc + 255) / 256), dim3(256), 0, st, P.d, nc, lookup, counter);
   if (nbed)
-    hipLaunchKernelGGL(k_bed_lookup_scatter, dim3((nbed + 255) / 256), dim3(256), 0, st, u, nbed,
+    hipLaunchKernelGGL(k_bed_lookup_scatter, dim3((nbed + 255) / 256), dim3(256), 0, st, P.d, nbed,
                        reinterpret_cast<const float4*>(bed_part), reinterpret_cast<const float4*>(bed_aff), lookup);
-  hipLaunchKernelGGL(k_bed_table, dim3((nc + 255) / 256), dim3(256), 0, st, u, lookup, bed_part, bed_ext, capacity,
+  hipLaunchKernelGGL(k_bed_table, dim3((nc + 255) / 256), dim3(256), 0, st, P.d, lookup, bed_part, bed_ext, capacity,
                      counter, bed_id, table);
   return hipGetLastError();
 }
@@ -1320,7 +1361,10 @@ __global__ void __launch_bounds__(256) k_gen_sdf_class(cs_settings s, const floa
 // ---- inflow emitter on the device (inflow.rs:195-214; the frame and count come from the host) ----
 // One thread per emitted particle: its 32 B particle record from the golden-angle
 // disk and a zero affine record (48 B), at slot f.first + i.
-__global__ void __launch_bounds__(256) k_emit(cs_emit_frame f, float* __restrict__ part, float* __restrict__ aff) {
+__global__ void __launch_bounds__(256) k_emit(const cs_dparams* __restrict__ dp, float* __restrict__ part,
+                                              float* __restrict__ aff) {
+  if (dp->halt) return;
+  const cs_emit_frame f = dp->ef;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= f.count) return;
   float p8[8];
@@ -1480,30 +1524,46 @@ hipError_t cs_launch_copy_i32(hipStream_t s, const int32_t* src, int32_t* dst, s
   hipLaunchKernelGGL(k_copy_i32, blocks_for(n), dim3(kBlock), 0, s, src, dst, n);
   return hipGetLastError();
 }
-hipError_t cs_launch_p2g(hipStream_t s, const cs_params& u, const float* part, const float* aff, int32_t* lanes,
+hipError_t cs_launch_fill_i32_gated(hipStream_t s, const cs_dparams* d, int32_t* p, size_t n, int32_t v) {
+  if (n == 0) return hipSuccess;
+  size_t nb = (n + kBlock - 1) / kBlock;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(k_fill_i32_gated, dim3((unsigned)nb), dim3(kBlock), 0, s, d, p, n, v);
+  return hipGetLastError();
+}
+hipError_t cs_launch_copy_i32_gated(hipStream_t s, const cs_dparams* d, const int32_t* src, int32_t* dst, size_t n) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_i32_gated, blocks_for(n), dim3(kBlock), 0, s, d, src, dst, n);
+  return hipGetLastError();
+}
+hipError_t cs_launch_p2g(hipStream_t s, const cs_pblock& P, const float* part, const float* aff, int32_t* lanes,
                          uint32_t* metrics) {
+  const cs_params& u = P.u;
   if (u.num_particles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_p2g_atomic, blocks_for(u.num_particles), dim3(kBlock), 0, s, u,
+  hipLaunchKernelGGL(k_p2g_atomic, blocks_for(u.num_particles), dim3(kBlock), 0, s, P.d,
                      reinterpret_cast<const float4*>(part), reinterpret_cast<const float4*>(aff), lanes, metrics);
   return hipGetLastError();
 }
-hipError_t cs_launch_grid(hipStream_t s, const cs_params& u, const int32_t* lanes, float* gvel, const float* sdf) {
-  hipLaunchKernelGGL(k_grid, blocks_for(u.nc), dim3(kBlock), 0, s, u, lanes, reinterpret_cast<float4*>(gvel), sdf);
+hipError_t cs_launch_grid(hipStream_t s, const cs_pblock& P, const int32_t* lanes, float* gvel, const float* sdf) {
+  const cs_params& u = P.u;
+  hipLaunchKernelGGL(k_grid, blocks_for(u.nc), dim3(kBlock), 0, s, P.d, lanes, reinterpret_cast<float4*>(gvel), sdf);
   return hipGetLastError();
 }
-hipError_t cs_launch_classify(hipStream_t s, const cs_params& u, const float* gvel, const uint8_t* cls,
+hipError_t cs_launch_classify(hipStream_t s, const cs_pblock& P, const float* gvel, const uint8_t* cls,
                               const int32_t* lanes, const int32_t* bed_id, const cs_bed_cell* bed, int8_t* kind,
                               float* kf, int32_t* pres, float* rhs, int32_t* pack, float* tgt, uint32_t* metrics,
                               const float* sdf, uint32_t* brick_flag) {
-  hipLaunchKernelGGL(k_fill_i32, dim3((u.nbricks + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+  const cs_params& u = P.u;
+  hipLaunchKernelGGL(k_fill_i32_gated, dim3((u.nbricks + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P.d,
                      reinterpret_cast<int32_t*>(brick_flag), (size_t)u.nbricks, 0);
-  hipLaunchKernelGGL(k_classify, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<const float4*>(gvel), cls,
+  hipLaunchKernelGGL(k_classify, blocks_for(u.nc), dim3(kBlock), 0, s, P.d, reinterpret_cast<const float4*>(gvel), cls,
                      lanes, bed_id, bed, kind, kf, pres, rhs, pack, tgt, metrics, sdf, brick_flag);
   return hipGetLastError();
 }
-hipError_t cs_launch_warm_previous(hipStream_t s, const cs_params& u, const float* kf, const int32_t* prev,
+hipError_t cs_launch_warm_previous(hipStream_t s, const cs_pblock& P, const float* kf, const int32_t* prev,
                                    int32_t* pres) {
-  hipLaunchKernelGGL(k_warm_previous, blocks_for(u.nc), dim3(kBlock), 0, s, u, kf, prev, pres);
+  const cs_params& u = P.u;
+  hipLaunchKernelGGL(k_warm_previous, blocks_for(u.nc), dim3(kBlock), 0, s, P.d, kf, prev, pres);
   return hipGetLastError();
 }
 hipError_t cs_launch_warm_external(hipStream_t s, const cs_params& u, const float* kf, const float* warm,
@@ -1525,9 +1585,10 @@ hipError_t cs_launch_residual(hipStream_t s, const cs_params& u, const float* kf
   hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kBlock), 0, s, partials, (uint32_t)nb, out);
   return hipGetLastError();
 }
-hipError_t cs_launch_project(hipStream_t s, const cs_params& u, float* gvel, const float* kf, const int32_t* pres,
+hipError_t cs_launch_project(hipStream_t s, const cs_pblock& P, float* gvel, const float* kf, const int32_t* pres,
                              const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf, const cs_grains& gr) {
-  hipLaunchKernelGGL(k_project, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(gvel), kf, pres,
+  const cs_params& u = P.u;
+  hipLaunchKernelGGL(k_project, blocks_for(u.nc), dim3(kBlock), 0, s, P.d, reinterpret_cast<float4*>(gvel), kf, pres,
                      bed_id, bed, sdf, gr);
   return hipGetLastError();
 }
@@ -1537,29 +1598,34 @@ hipError_t cs_launch_post_div(hipStream_t s, const cs_params& u, const float* gv
                      cls, sdf, tgt, metrics);
   return hipGetLastError();
 }
-hipError_t cs_launch_post_div_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_launch_post_div_bricks(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                                      const float* gvel, const float* kf, const uint8_t* cls, const float* sdf,
                                      const float* tgt, uint32_t* metrics) {
+  const cs_params& u = P.u;
   if (!nactive) return hipSuccess;
-  hipLaunchKernelGGL(k_post_div_bricks, dim3(nactive), dim3(256), 0, s, u, list, reinterpret_cast<const float4*>(gvel),
+  hipLaunchKernelGGL(k_post_div_bricks, dim3(nactive), dim3(256), 0, s, P.d,
+                     list, reinterpret_cast<const float4*>(gvel),
                      kf, cls, sdf, tgt, metrics);
   return hipGetLastError();
 }
-hipError_t cs_launch_pack(hipStream_t s, const cs_params& u, float* gvel, const float* kf, const int32_t* pack,
+hipError_t cs_launch_pack(hipStream_t s, const cs_pblock& P, float* gvel, const float* kf, const int32_t* pack,
                           const float* sdf) {
-  hipLaunchKernelGGL(k_pack, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(gvel), kf, pack, sdf);
+  const cs_params& u = P.u;
+  hipLaunchKernelGGL(k_pack, blocks_for(u.nc), dim3(kBlock), 0, s, P.d, reinterpret_cast<float4*>(gvel), kf, pack, sdf);
   return hipGetLastError();
 }
-hipError_t cs_launch_visc(hipStream_t s, const cs_params& u, const float* gin, float* gout, const uint8_t* cls,
+hipError_t cs_launch_visc(hipStream_t s, const cs_pblock& P, const float* gin, float* gout, const uint8_t* cls,
                           const float* sdf, uint8_t* alias4) {
-  hipLaunchKernelGGL(k_visc, blocks_for(u.nc), dim3(kBlock), 0, s, u, reinterpret_cast<const float4*>(gin),
+  const cs_params& u = P.u;
+  hipLaunchKernelGGL(k_visc, blocks_for(u.nc), dim3(kBlock), 0, s, P.d, reinterpret_cast<const float4*>(gin),
                      reinterpret_cast<float4*>(gout), cls, sdf, alias4);
   return hipGetLastError();
 }
-hipError_t cs_launch_g2p(hipStream_t s, const cs_params& u, float* part, float* aff, const float* gvel,
+hipError_t cs_launch_g2p(hipStream_t s, const cs_pblock& P, float* part, float* aff, const float* gvel,
                          const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf, const cs_grains& gr) {
+  const cs_params& u = P.u;
   if (u.num_particles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_g2p, blocks_for(u.num_particles), dim3(kBlock), 0, s, u, reinterpret_cast<float4*>(part),
+  hipLaunchKernelGGL(k_g2p, blocks_for(u.num_particles), dim3(kBlock), 0, s, P.d, reinterpret_cast<float4*>(part),
                      reinterpret_cast<float4*>(aff), reinterpret_cast<const float4*>(gvel), bed_id, bed, sdf, gr);
   return hipGetLastError();
 }
@@ -1593,9 +1659,9 @@ hipError_t cs_launch_gen_sdf_class(hipStream_t st, const cs_settings& s, const f
   hipLaunchKernelGGL(k_gen_sdf_class, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s, sdf, out);
   return hipGetLastError();
 }
-hipError_t cs_launch_emit(hipStream_t s, const cs_emit_frame& f, float* part, float* aff) {
-  if (f.count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_emit, dim3((f.count + 255u) / 256u), dim3(256), 0, s, f, part, aff);
+hipError_t cs_launch_emit(hipStream_t s, const cs_dparams* d, uint32_t threads, float* part, float* aff) {
+  if (threads == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_emit, dim3((threads + 255u) / 256u), dim3(256), 0, s, d, part, aff);
   return hipGetLastError();
 }
 hipError_t cs_launch_unet_features(hipStream_t s, const cs_params& u, const float* kf, const float* rhs,

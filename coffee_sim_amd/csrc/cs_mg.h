@@ -52,17 +52,17 @@ void cs_mg_visit_ops(int level, int nlevels, std::vector<cs_mg_op>* out);
 // correction of the free cells; smoothing, restriction to / prolongation from
 // level 1; and the final p <- clamp(p + e) in fixed point.  Every kernel returns
 // at once when *skip is set.
-hipError_t cs_mg_fine_setup(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_setup(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                             const float* wc, const float* rhs, const int32_t* p, cs_mg_level L0, const uint32_t* skip);
-hipError_t cs_mg_fine_smooth(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_smooth(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                              cs_mg_level L0, uint32_t colour, const uint32_t* skip);
-hipError_t cs_mg_fine_restrict(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_restrict(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                                cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
-hipError_t cs_mg_fine_build(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_build(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                             cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
-hipError_t cs_mg_fine_smooth_prolonged(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_smooth_prolonged(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                                        cs_mg_level L0, cs_mg_level L1, const uint32_t* skip);
-hipError_t cs_mg_fine_apply(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_apply(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                             cs_mg_level L0, int32_t* p, const uint32_t* skip);
 // Dense levels.
 hipError_t cs_mg_smooth(hipStream_t s, cs_mg_level L, uint32_t colour, const uint32_t* skip);

@@ -274,10 +274,13 @@ __device__ __forceinline__ bool is_free(const cs_params& u, const float4* __rest
 }
 
 // ---- fine level, one workgroup of 256 per active brick (two cells a thread) ----------
-__global__ void __launch_bounds__(256) k_mg_fine_setup(cs_params u, const uint32_t* __restrict__ list,
+__global__ void __launch_bounds__(256) k_mg_fine_setup(const cs_dparams* __restrict__ dp,
+                                                       const uint32_t* __restrict__ list,
                                                        const float4* __restrict__ wc, const float* __restrict__ rhs,
                                                        const int32_t* __restrict__ p, cs_mg_level L,
                                                        const uint32_t* __restrict__ skip) {
+  CS_PARAMS_FROM(dp);
+  if (blockIdx.x >= dp->nactive) return;
   if (*skip) return;
   uint32_t x0, y0, z0;
   brick_origin(u, list[blockIdx.x], x0, y0, z0);
@@ -304,8 +307,11 @@ __global__ void __launch_bounds__(256) k_mg_fine_setup(cs_params u, const uint32
   }
 }
 
-__global__ void __launch_bounds__(256) k_mg_fine_smooth(cs_params u, const uint32_t* __restrict__ list, cs_mg_level L,
+__global__ void __launch_bounds__(256) k_mg_fine_smooth(const cs_dparams* __restrict__ dp,
+                                                        const uint32_t* __restrict__ list, cs_mg_level L,
                                                         uint32_t colour, const uint32_t* __restrict__ skip) {
+  CS_PARAMS_FROM(dp);
+  if (blockIdx.x >= dp->nactive) return;
   if (*skip) return;
   uint32_t x0, y0, z0;
   brick_origin(u, list[blockIdx.x], x0, y0, z0);
@@ -317,8 +323,11 @@ __global__ void __launch_bounds__(256) k_mg_fine_smooth(cs_params u, const uint3
   if (x < u.gx && y < u.gy && z < u.gz) smooth_cell(L, x, y, z, colour);
 }
 
-__global__ void __launch_bounds__(64) k_mg_fine_restrict(cs_params u, const uint32_t* __restrict__ list, cs_mg_level f,
+__global__ void __launch_bounds__(64) k_mg_fine_restrict(const cs_dparams* __restrict__ dp,
+                                                         const uint32_t* __restrict__ list, cs_mg_level f,
                                                          cs_mg_level c, const uint32_t* __restrict__ skip) {
+  CS_PARAMS_FROM(dp);
+  if (blockIdx.x >= dp->nactive) return;
   if (*skip) return;
   uint32_t x0, y0, z0;
   brick_origin(u, list[blockIdx.x], x0, y0, z0);
@@ -327,8 +336,11 @@ __global__ void __launch_bounds__(64) k_mg_fine_restrict(cs_params u, const uint
   if (X < c.X && Y < c.Y && Z < c.Z) restrict_cell(f, c, X, Y, Z);
 }
 
-__global__ void __launch_bounds__(64) k_mg_fine_build(cs_params u, const uint32_t* __restrict__ list, cs_mg_level f,
+__global__ void __launch_bounds__(64) k_mg_fine_build(const cs_dparams* __restrict__ dp,
+                                                      const uint32_t* __restrict__ list, cs_mg_level f,
                                                       cs_mg_level c, const uint32_t* __restrict__ skip) {
+  CS_PARAMS_FROM(dp);
+  if (blockIdx.x >= dp->nactive) return;
   if (*skip) return;
   uint32_t x0, y0, z0;
   brick_origin(u, list[blockIdx.x], x0, y0, z0);
@@ -337,9 +349,12 @@ __global__ void __launch_bounds__(64) k_mg_fine_build(cs_params u, const uint32_
   if (X < c.X && Y < c.Y && Z < c.Z) build_cell(f, c, X, Y, Z);
 }
 
-__global__ void __launch_bounds__(256) k_mg_fine_smooth_prolonged(cs_params u, const uint32_t* __restrict__ list,
+__global__ void __launch_bounds__(256) k_mg_fine_smooth_prolonged(const cs_dparams* __restrict__ dp,
+                                                                  const uint32_t* __restrict__ list,
                                                                   cs_mg_level f, cs_mg_level c,
                                                                   const uint32_t* __restrict__ skip) {
+  CS_PARAMS_FROM(dp);
+  if (blockIdx.x >= dp->nactive) return;
   if (*skip) return;
   uint32_t x0, y0, z0;
   brick_origin(u, list[blockIdx.x], x0, y0, z0);
@@ -350,8 +365,11 @@ __global__ void __launch_bounds__(256) k_mg_fine_smooth_prolonged(cs_params u, c
   if (x < u.gx && y < u.gy && z < u.gz) smooth_prolonged_cell(f, c, x, y, z);
 }
 
-__global__ void __launch_bounds__(256) k_mg_fine_apply(cs_params u, const uint32_t* __restrict__ list, cs_mg_level L,
+__global__ void __launch_bounds__(256) k_mg_fine_apply(const cs_dparams* __restrict__ dp,
+                                                       const uint32_t* __restrict__ list, cs_mg_level L,
                                                        int32_t* __restrict__ p, const uint32_t* __restrict__ skip) {
+  CS_PARAMS_FROM(dp);
+  if (blockIdx.x >= dp->nactive) return;
   if (*skip) return;
   uint32_t x0, y0, z0;
   brick_origin(u, list[blockIdx.x], x0, y0, z0);
@@ -1262,41 +1280,48 @@ void cs_mg_visit_ops(int level, int nlevels, std::vector<cs_mg_op>* out) {
   }
 }
 
-hipError_t cs_mg_fine_setup(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_setup(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                             const float* wc, const float* rhs, const int32_t* p, cs_mg_level L0, const uint32_t* skip) {
+  const cs_params& u = P.u;
   if (!nactive) return hipSuccess;
-  hipLaunchKernelGGL(k_mg_fine_setup, dim3(nactive), dim3(256), 0, s, u, list, reinterpret_cast<const float4*>(wc), rhs,
+  hipLaunchKernelGGL(k_mg_fine_setup, dim3(nactive), dim3(256), 0, s, P.d,
+                     list, reinterpret_cast<const float4*>(wc), rhs,
                      p, L0, skip);
   return hipGetLastError();
 }
-hipError_t cs_mg_fine_smooth(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_smooth(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                              cs_mg_level L0, uint32_t colour, const uint32_t* skip) {
+  const cs_params& u = P.u;
   if (!nactive) return hipSuccess;
-  hipLaunchKernelGGL(k_mg_fine_smooth, dim3(nactive), dim3(256), 0, s, u, list, L0, colour, skip);
+  hipLaunchKernelGGL(k_mg_fine_smooth, dim3(nactive), dim3(256), 0, s, P.d, list, L0, colour, skip);
   return hipGetLastError();
 }
-hipError_t cs_mg_fine_restrict(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_restrict(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                                cs_mg_level L0, cs_mg_level L1, const uint32_t* skip) {
+  const cs_params& u = P.u;
   if (!nactive) return hipSuccess;
-  hipLaunchKernelGGL(k_mg_fine_restrict, dim3(nactive), dim3(64), 0, s, u, list, L0, L1, skip);
+  hipLaunchKernelGGL(k_mg_fine_restrict, dim3(nactive), dim3(64), 0, s, P.d, list, L0, L1, skip);
   return hipGetLastError();
 }
-hipError_t cs_mg_fine_build(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_build(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                             cs_mg_level L0, cs_mg_level L1, const uint32_t* skip) {
+  const cs_params& u = P.u;
   if (!nactive) return hipSuccess;
-  hipLaunchKernelGGL(k_mg_fine_build, dim3(nactive), dim3(64), 0, s, u, list, L0, L1, skip);
+  hipLaunchKernelGGL(k_mg_fine_build, dim3(nactive), dim3(64), 0, s, P.d, list, L0, L1, skip);
   return hipGetLastError();
 }
-hipError_t cs_mg_fine_smooth_prolonged(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_smooth_prolonged(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                                        cs_mg_level L0, cs_mg_level L1, const uint32_t* skip) {
+  const cs_params& u = P.u;
   if (!nactive) return hipSuccess;
-  hipLaunchKernelGGL(k_mg_fine_smooth_prolonged, dim3(nactive), dim3(256), 0, s, u, list, L0, L1, skip);
+  hipLaunchKernelGGL(k_mg_fine_smooth_prolonged, dim3(nactive), dim3(256), 0, s, P.d, list, L0, L1, skip);
   return hipGetLastError();
 }
-hipError_t cs_mg_fine_apply(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_mg_fine_apply(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                             cs_mg_level L0, int32_t* p, const uint32_t* skip) {
+  const cs_params& u = P.u;
   if (!nactive) return hipSuccess;
-  hipLaunchKernelGGL(k_mg_fine_apply, dim3(nactive), dim3(256), 0, s, u, list, L0, p, skip);
+  hipLaunchKernelGGL(k_mg_fine_apply, dim3(nactive), dim3(256), 0, s, P.d, list, L0, p, skip);
   return hipGetLastError();
 }
 hipError_t cs_mg_smooth(hipStream_t s, cs_mg_level L, uint32_t colour, const uint32_t* skip) {

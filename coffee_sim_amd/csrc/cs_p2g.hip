@@ -112,9 +112,10 @@ __device__ __forceinline__ void p2g_bin_block(const cs_params& u, const float4* 
   }
 }
 constexpr int kBinPerThread = 4;  // particles per thread of k_p2g_bin: 1024 per workgroup
-__global__ void __launch_bounds__(256) k_p2g_bin(cs_params u, const float4* __restrict__ part,
+__global__ void __launch_bounds__(256) k_p2g_bin(const cs_dparams* __restrict__ dp, const float4* __restrict__ part,
                                                  const float4* __restrict__ aff, uint32_t* __restrict__ tile_count,
                                                  uint2* __restrict__ key) {
+  CS_PARAMS_FROM(dp);
   p2g_bin_block<kBinPerThread>(u, part, aff, tile_count, key, blockIdx.x * (256u * kBinPerThread));
 }
 
@@ -169,9 +170,10 @@ __device__ __forceinline__ void p2g_scatter_particle(uint32_t n, const uint2* __
   const uint2 k = key[pid];
   if (k.x != kNoTile) sorted[start[k.x] + k.y] = pid;
 }
-__global__ void __launch_bounds__(256) k_p2g_scatter(uint32_t n, const uint2* __restrict__ key,
+__global__ void __launch_bounds__(256) k_p2g_scatter(const cs_dparams* __restrict__ dp, const uint2* __restrict__ key,
                                                      const uint32_t* __restrict__ start, uint32_t* __restrict__ sorted) {
-  p2g_scatter_particle(n, key, start, sorted, blockIdx.x * blockDim.x + threadIdx.x);
+  CS_PARAMS_FROM(dp);
+  p2g_scatter_particle(u.num_particles, key, start, sorted, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Pass 4: one workgroup per tile.  The tile's particles are dealt to threads
@@ -255,7 +257,8 @@ __device__ __forceinline__ void p2g_tile_block(const cs_params& u, const float4*
 // flushes its own block (the lanes are added with i32 atomics, so the result
 // does not depend on the split).  Workgroup w finds its tile as the last one
 // whose chunk prefix is at most w.
-__global__ void __launch_bounds__(kTileThreads) k_p2g_tile_chunks(cs_params u, const float4* __restrict__ part,
+__global__ void __launch_bounds__(kTileThreads) k_p2g_tile_chunks(const cs_dparams* __restrict__ dp,
+                                                                  const float4* __restrict__ part,
                                                                   const float4* __restrict__ aff,
                                                                   const uint32_t* __restrict__ tile_count,
                                                                   const uint32_t* __restrict__ start,
@@ -263,6 +266,7 @@ __global__ void __launch_bounds__(kTileThreads) k_p2g_tile_chunks(cs_params u, c
                                                                   const uint32_t* __restrict__ sorted,
                                                                   int32_t* __restrict__ lanes,
                                                                   uint32_t* __restrict__ metrics) {
+  CS_PARAMS_FROM(dp);
   __shared__ int32_t s_acc[6 * FP3];
   const uint32_t w = blockIdx.x;
   if (w >= chunk[u.nbricks]) return;
@@ -335,21 +339,23 @@ hipError_t cs_batch_launch_p2g(const cs_batch_launch& b) {
   return hipGetLastError();
 }
 
-hipError_t cs_launch_p2g_tiled(hipStream_t s, const cs_params& u, const float* part, const float* aff,
+hipError_t cs_launch_p2g_tiled(hipStream_t s, const cs_pblock& P, const float* part, const float* aff,
                                int32_t* lanes, uint32_t* metrics, uint32_t* tile_count, uint32_t* tile_start,
                                uint32_t* tile_chunk, uint2* key, uint32_t* sorted) {
+  const cs_params& u = P.u;
   if (u.num_particles == 0) return hipSuccess;
   const dim3 pb((u.num_particles + 255u) / 256u);
   hipError_t e = cs_launch_fill_i32(s, reinterpret_cast<int32_t*>(tile_count), u.nbricks, 0);
   if (e != hipSuccess) return e;
   const uint32_t per_block = 256u * kBinPerThread;
-  hipLaunchKernelGGL(k_p2g_bin, dim3((u.num_particles + per_block - 1u) / per_block), dim3(256), 0, s, u,
+  hipLaunchKernelGGL(k_p2g_bin, dim3((u.num_particles + per_block - 1u) / per_block), dim3(256), 0, s, P.d,
                      reinterpret_cast<const float4*>(part), reinterpret_cast<const float4*>(aff), tile_count, key);
   hipLaunchKernelGGL(k_p2g_scan, dim3(1), dim3(1024), 0, s, tile_count, u.nbricks, tile_start, tile_chunk);
-  hipLaunchKernelGGL(k_p2g_scatter, pb, dim3(256), 0, s, u.num_particles, key, tile_start, sorted);
+  hipLaunchKernelGGL(k_p2g_scatter, pb, dim3(256), 0, s, P.d, key, tile_start, sorted);
   // at most one partial chunk per tile beyond the full ones
   const uint32_t items = u.nbricks + (u.num_particles + kChunk - 1u) / kChunk;
-  hipLaunchKernelGGL(k_p2g_tile_chunks, dim3(items), dim3(kTileThreads), 0, s, u, reinterpret_cast<const float4*>(part),
+  hipLaunchKernelGGL(k_p2g_tile_chunks, dim3(items), dim3(kTileThreads), 0, s, P.d,
+                     reinterpret_cast<const float4*>(part),
                      reinterpret_cast<const float4*>(aff), tile_count, tile_start, tile_chunk, sorted, lanes, metrics);
   return hipGetLastError();
 }

@@ -15,7 +15,7 @@
 #define CS_PERSIST_MAX_BRICKS 1024              // active bricks one persistent solve takes
 
 struct cs_persist_args {
-  cs_params u;
+  const cs_dparams* dp;      // the substep's parameters, epoch base and halt word
   const uint32_t* list;      // active bricks, ascending (cs_launch_brick_scan)
   const int32_t* slot;       // per brick: its position in `list` (its run), or -1
   uint32_t* xcc;             // [max_runs]: 1 + the XCC id each run's workgroup runs on
@@ -26,7 +26,6 @@ struct cs_persist_args {
   cs_solve_state* st;        // active_bricks in; iterations, residuals, done, faults out
   uint64_t* fb;              // [nbricks][6 faces][2 colours][32]: surface cells as
                              // (epoch << 32 | fixed-point pressure)
-  uint32_t epoch_base;       // epochs of this solve are epoch_base + 1, 2, ... (grow across solves)
   uint32_t* pflags;          // [max_runs * stride]: residual checks published per brick
   double* partials;          // [2][max_runs][2]: (rr, bb) per brick and check parity
   uint32_t max_runs;         // capacity of pflags and partials (the brick count)

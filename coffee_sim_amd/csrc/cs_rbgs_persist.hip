@@ -150,12 +150,16 @@ struct Cell {
 // contiguous part of the (ascending, hence z-major) list lands on one XCD
 // (workgroup w runs on XCD w % 8).
 __global__ void __launch_bounds__(NT, 4) k_rbgs_persist(cs_persist_args a) {
-  const cs_params& u = a.u;
+  CS_PARAMS_FROM(a.dp);
   const uint32_t t = threadIdx.x;
   const uint32_t nactive = a.st->active_bricks;
-  const uint32_t per = (gridDim.x + 7u) / 8u;
+  // XCD slices of ceil(nactive / 8) runs; a launch sized for more bricks than
+  // are active leaves its extra workgroups without a run
+  const uint32_t per = (nactive + 7u) / 8u;
+  if (blockIdx.x / 8u >= per) return;
   const uint32_t run = (blockIdx.x % 8u) * per + blockIdx.x / 8u;
   if (run >= nactive) return;
+  const uint32_t epoch_base = a.dp->epoch_base;
   const uint32_t brick = a.list[run];
   const uint32_t bx = brick % u.nbx, by = (brick / u.nbx) % u.nby, bz = brick / (u.nbx * u.nby);
   const int x0 = (int)bx * BR, y0 = (int)by * BR, z0 = (int)bz * BR;
@@ -410,7 +414,7 @@ __global__ void __launch_bounds__(NT, 4) k_rbgs_persist(cs_persist_args a) {
   // one half sweep of colour c; returns false when a wait gave up
   auto half_sweep = [&](const Cell& e, int h_l, int32_t h_g, int h_local) -> bool {
     ++epoch;
-    const uint32_t want = a.epoch_base + epoch;
+    const uint32_t want = epoch_base + epoch;
     bool fired;
     const float np = update(u, e.self, e.rhs, e.wxm, e.wym, e.wzm, s_p, e.li, fired);
     if (e.self.w >= 0.0f) {

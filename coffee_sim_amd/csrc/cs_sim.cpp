@@ -65,12 +65,18 @@ void free_sim(cs_sim* m) {
                  m->bed_aff,    m->bed_ext,    m->bed_lookup, m->bed_counter, m->bed_dw, m->tgt,
                  m->bed_snap,   m->alias4,
                  m->persist_fb, m->persist_pflags, m->persist_partials, m->persist_slot, m->persist_xcc,
-                 m->mg_buf,     m->mg_flags,   m->mg_ops,     m->mg_row_buf, m->mg_row_counts, m->mg_pmap, m->mg_pnb, m->mg_achild};
+                 m->mg_buf,     m->mg_flags,   m->mg_ops,     m->mg_row_buf, m->mg_row_counts, m->mg_pmap, m->mg_pnb,
+                 m->mg_achild,  m->dparams,    m->solve_records};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
   if (m->h_state) (void)hipHostFree(m->h_state);
   if (m->h_done) (void)hipHostFree(m->h_done);
+  if (m->h_pring) (void)hipHostFree(m->h_pring);
+  if (m->h_records) (void)hipHostFree(m->h_records);
+  if (m->h_halt) (void)hipHostFree(m->h_halt);
+  if (m->sg_exec) (void)hipGraphExecDestroy(m->sg_exec);
+  if (m->sg_graph) (void)hipGraphDestroy(m->sg_graph);
   for (auto& e : m->batch_ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : m->chunk_ev)
@@ -142,7 +148,8 @@ int predict_warm_start(cs_sim* m, const cs_params& u) {
 
 // One-thread-per-cell solve over the whole grid (the reference's dispatch
 // shape), kept as a cross-check of the brick solve.
-int solve_dense(cs_sim* m, const cs_params& u) {
+int solve_dense(cs_sim* m, const cs_pblock& P) {
+  const cs_params& u = P.u;
   if (m->s.rbgs_rescale_after) return fail(CS_ERR_INVALID, "rbgs_rescale_after needs the brick solver");
   if (m->s.rbgs_mg_cycles) return fail(CS_ERR_INVALID, "rbgs_mg_cycles needs the brick solver");
   hipStream_t st = m->ctx->stream;
@@ -152,7 +159,7 @@ int solve_dense(cs_sim* m, const cs_params& u) {
   m->final_residual_pending = false;
   bool external = false;
   if (m->s.warm_start == CS_WARM_PREVIOUS) {
-    CS_HIP(cs_launch_warm_previous(st, u, m->kf, m->prev_p, m->pres));
+    CS_HIP(cs_launch_warm_previous(st, P, m->kf, m->prev_p, m->pres));
   } else if (m->s.warm_start == CS_WARM_EXTERNAL && m->has_warm) {
     CS_HIP(cs_launch_warm_external(st, u, m->kf, m->warm, m->pres, m->metrics));
     external = true;
@@ -281,19 +288,13 @@ int enqueue_sweeps(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ite
 
 // Captures one batch of `batch_chunks` chunks of `every` iterations whose
 // residual tests are folded into the sweeps (the steady state of a solve)
-// into m->batch_exec.  Kernel arguments are the current substep's, so the
-// graph is rebuilt by every solve that uses it.  The batch has an even number
-// of launches, so the pressure buffers are back in place after each replay.
+// into m->batch_exec.  Kernel arguments are the current substep's, so every
+// solve that uses it records the batch again and updates the instantiated
+// graph in place (a new instantiation only when the update is refused).  The
+// batch has an even number of launches, so the pressure buffers are back in
+// place after each replay.
 int capture_batch(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t every, FusedCheck first) {
   hipStream_t st = m->ctx->stream;
-  if (m->batch_exec) {
-    (void)hipGraphExecDestroy(m->batch_exec);
-    m->batch_exec = nullptr;
-  }
-  if (m->batch_graph) {
-    (void)hipGraphDestroy(m->batch_graph);
-    m->batch_graph = nullptr;
-  }
   CS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   int cur = 0;
   FusedCheck chk = first;
@@ -316,6 +317,19 @@ int capture_batch(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ever
     (void)hipGraphDestroy(g);
     return fail(CS_ERR_STATE, "captured batch has an odd number of sweep launches");
   }
+  if (m->batch_exec) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult why = hipGraphExecUpdateSuccess;
+    if (hipGraphExecUpdate(m->batch_exec, g, &bad, &why) == hipSuccess && why == hipGraphExecUpdateSuccess) {
+      if (m->batch_graph) (void)hipGraphDestroy(m->batch_graph);
+      m->batch_graph = g;
+      return CS_OK;
+    }
+    (void)hipGetLastError();
+    (void)hipGraphExecDestroy(m->batch_exec);
+    m->batch_exec = nullptr;
+  }
+  if (m->batch_graph) (void)hipGraphDestroy(m->batch_graph);
   m->batch_graph = g;
   CS_HIP(hipGraphInstantiate(&m->batch_exec, g, nullptr, nullptr, 0));
   return CS_OK;
@@ -380,7 +394,7 @@ int ensure_mg(cs_sim* m) {
 
 // One W-cycle visit of level l; the levels from mg_wg_first down run in one
 // workgroup, the finer ones as launches (level 0 over the active bricks).
-int mg_visit(cs_sim* m, const cs_params& u, uint32_t nactive, int l) {
+int mg_visit(cs_sim* m, const cs_pblock& P, uint32_t nactive, int l) {
   hipStream_t st = m->ctx->stream;
   const int nl = (int)m->mg_levels.size();
   const uint32_t* skip = m->mg_flags;
@@ -403,7 +417,7 @@ int mg_visit(cs_sim* m, const cs_params& u, uint32_t nactive, int l) {
   auto passes = [&](uint32_t n) -> int {
     for (uint32_t k = 0; k < n; ++k)
       for (uint32_t c = 0; c < 2u; ++c) {
-        if (l == 0) CS_HIP(cs_mg_fine_smooth(st, u, nactive, m->brick_list, L, c, skip));
+        if (l == 0) CS_HIP(cs_mg_fine_smooth(st, P, nactive, m->brick_list, L, c, skip));
         else CS_HIP(cs_mg_smooth(st, L, c, skip));
       }
     return CS_OK;
@@ -415,13 +429,13 @@ int mg_visit(cs_sim* m, const cs_params& u, uint32_t nactive, int l) {
   for (uint32_t g = 0; g < CS_MG_GAMMA; ++g) {
     // restriction (zeroes the coarse correction), the coarse visit, then the
     // prolongation folded into the first red pass
-    if (l == 0) CS_HIP(cs_mg_fine_restrict(st, u, nactive, m->brick_list, L, C, skip));
+    if (l == 0) CS_HIP(cs_mg_fine_restrict(st, P, nactive, m->brick_list, L, C, skip));
     else CS_HIP(cs_mg_restrict(st, L, C, skip));
-    rc = mg_visit(m, u, nactive, l + 1);
+    rc = mg_visit(m, P, nactive, l + 1);
     if (rc) return rc;
     if (l == 0) {
-      CS_HIP(cs_mg_fine_smooth_prolonged(st, u, nactive, m->brick_list, L, C, skip));
-      CS_HIP(cs_mg_fine_smooth(st, u, nactive, m->brick_list, L, 1u, skip));
+      CS_HIP(cs_mg_fine_smooth_prolonged(st, P, nactive, m->brick_list, L, C, skip));
+      CS_HIP(cs_mg_fine_smooth(st, P, nactive, m->brick_list, L, 1u, skip));
     } else {
       CS_HIP(cs_mg_smooth_prolonged(st, L, C, skip));
       CS_HIP(cs_mg_smooth(st, L, 1u, skip));
@@ -434,13 +448,13 @@ int mg_visit(cs_sim* m, const cs_params& u, uint32_t nactive, int l) {
 
 // rbgs_mg_cycles W-cycles on the current start in m->pres, skipped on the
 // device when the start is already within the tolerance.
-int mg_start(cs_sim* m, const cs_params& u, uint32_t nactive) {
+int mg_start(cs_sim* m, const cs_pblock& P, uint32_t nactive) {
   int rc = ensure_mg(m);
   if (rc) return rc;
   hipStream_t st = m->ctx->stream;
   const double tol = (double)m->s.rbgs_tolerance;
   if (m->timing) CS_HIP(hipEventRecord(m->mg_ev[0], st));
-  CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+  CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
                                    m->dstate, 0u, 0u, 0u, tol));
   CS_HIP(cs_mg_gate(st, m->dstate, tol, m->mg_flags, true));
   // kernels touch only the active bricks' fine cells: the fine correction and
@@ -461,18 +475,18 @@ int mg_start(cs_sim* m, const cs_params& u, uint32_t nactive) {
     m->mg_rows3 = bound <= CS_MG_P3_MAX;
   }
   for (uint32_t c = 0; c < m->s.rbgs_mg_cycles; ++c) {
-    CS_HIP(cs_mg_fine_setup(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, L0, skip));
-    if (nl > 1) CS_HIP(cs_mg_fine_build(st, u, nactive, m->brick_list, L0, m->mg_levels[1], skip));
+    CS_HIP(cs_mg_fine_setup(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, L0, skip));
+    if (nl > 1) CS_HIP(cs_mg_fine_build(st, P, nactive, m->brick_list, L0, m->mg_levels[1], skip));
     for (int l = 1; l + 1 < nl; ++l) CS_HIP(cs_mg_build(st, m->mg_levels[l], m->mg_levels[l + 1], skip));
     if (m->mg_rows)
       CS_HIP(cs_mg_rows(st, m->mg_levels[m->mg_wg_first], m->mg_levels[nl - 1],
                         m->mg_rows3 ? &m->mg_levels[m->mg_wg_first - 1] : nullptr, m->mg_row_buf, m->mg_row_counts,
                         m->mg_pmap, m->mg_pnb, m->mg_achild, skip));
-    rc = mg_visit(m, u, nactive, 0);
+    rc = mg_visit(m, P, nactive, 0);
     if (rc) return rc;
-    CS_HIP(cs_mg_fine_apply(st, u, nactive, m->brick_list, L0, m->pres, skip));
+    CS_HIP(cs_mg_fine_apply(st, P, nactive, m->brick_list, L0, m->pres, skip));
     // the cycle's residual test: later cycles return at once within the tolerance
-    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+    CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
                                      m->dstate, 0u, 0u, 0u, tol));
     CS_HIP(cs_mg_gate(st, m->dstate, tol, m->mg_flags, false));
   }
@@ -496,39 +510,22 @@ void mg_harvest(cs_sim* m) {
   m->mg_timed = false;
 }
 
-// The whole solve as one launch of co-resident workgroups (cs_rbgs_persist.hip):
-// same updates, same residual tests, same stopping iteration as the
-// launch-per-chunk solve; the pressure is solved in place in m->pres.
-int solve_persistent(cs_sim* m, const cs_params& u, uint32_t nactive, bool external) {
+// The persistent solve's launch, after the multigrid start when one is asked
+// for, sized for at most `cap` active bricks (the count itself is read on the
+// device).  `cooperative` has the runtime refuse a launch whose workgroups
+// cannot all be resident; a recorded substep uses a plain launch, whose width
+// the host has checked against the same limit.
+int persist_enqueue(cs_sim* m, const cs_pblock& P, uint32_t cap, bool cooperative) {
+  const cs_params& u = P.u;
   hipStream_t st = m->ctx->stream;
-  cs_solve_stats& S = m->stats;
   const bool fixed = m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS;
-  const double tol = (double)m->s.rbgs_tolerance;
-  if (!fixed && external) {
-    // a supplied start is judged by its residual before the solve (the fallback)
-    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
-                                     m->dstate, 0u, 0u, 0u, tol));
-    CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
-    CS_HIP(hipStreamSynchronize(st));
-    if (m->h_state->initial_residual > (double)m->s.warm_fallback_residual) {
-      CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
-      S.fell_back = 1;
-    }
-  }
   if (!fixed && m->s.rbgs_mg_cycles) {
-    int rc = mg_start(m, u, nactive);
+    int rc = mg_start(m, P, cap);
     if (rc) return rc;
-  }
-  const uint32_t span = cs_persist_epochs(fixed ? m->s.pressure_rbgs_pairs : m->s.rbgs_max_iters);
-  if ((uint64_t)m->persist_epoch + span >= 0x80000000ull) {
-    // epochs would leave the 31-bit range: forget every published word
-    CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->persist_fb),
-                              2 * (size_t)u.nbricks * CS_PERSIST_FACE_WORDS, 0));
-    m->persist_epoch = 1;
   }
   CS_HIP(cs_persist_prepare(st, u, m->brick_list, m->dstate, m->persist_slot, m->persist_pflags, m->persist_xcc));
   cs_persist_args a{};
-  a.u = u;
+  a.dp = P.d;
   a.list = m->brick_list;
   a.slot = m->persist_slot;
   a.xcc = m->persist_xcc;
@@ -538,28 +535,57 @@ int solve_persistent(cs_sim* m, const cs_params& u, uint32_t nactive, bool exter
   a.metrics = m->metrics;
   a.st = m->dstate;
   a.fb = m->persist_fb;
-  a.epoch_base = m->persist_epoch;
   a.pflags = m->persist_pflags;
   a.partials = m->persist_partials;
   a.max_runs = u.nbricks;
   a.fixed_pairs = fixed ? m->s.pressure_rbgs_pairs : 0u;
   a.every = m->s.rbgs_check_every ? m->s.rbgs_check_every : 8u;
   a.cap = m->s.rbgs_max_iters;
-  a.tol = tol;
-  a.spin_limit = 200000000ull;
-  a.poll_delay = m->persist_poll_delay;  // 2 s of the 100 MHz clock: far beyond any legitimate wait
+  a.tol = (double)m->s.rbgs_tolerance;
+  a.spin_limit = 200000000ull;  // 2 s of the 100 MHz clock: far beyond any legitimate wait
+  a.poll_delay = m->persist_poll_delay;
   // after a multigrid start the first residual test comes after one check interval
   a.min_gate = (!fixed && m->s.rbgs_mg_cycles) ? m->mg_flags + 1 : nullptr;
   a.min_iters = a.every;
   if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[0], st));
-  CS_HIP(cs_persist_launch(st, a, cs_persist_groups(nactive), true));
+  CS_HIP(cs_persist_launch(st, a, cs_persist_groups(cap), cooperative));
   if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[1], st));
+  return CS_OK;
+}
+
+// Epochs one persistent solve takes (the next solve's base moves by this).
+uint32_t persist_span(const cs_sim* m) {
+  const bool fixed = m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS;
+  return cs_persist_epochs(fixed ? m->s.pressure_rbgs_pairs : m->s.rbgs_max_iters);
+}
+
+// The whole solve as one launch of co-resident workgroups (cs_rbgs_persist.hip):
+// same updates, same residual tests, same stopping iteration as the
+// launch-per-chunk solve; the pressure is solved in place in m->pres.
+int solve_persistent(cs_sim* m, const cs_pblock& P, uint32_t nactive, bool external) {
+  hipStream_t st = m->ctx->stream;
+  cs_solve_stats& S = m->stats;
+  const bool fixed = m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS;
+  const double tol = (double)m->s.rbgs_tolerance;
+  if (!fixed && external) {
+    // a supplied start is judged by its residual before the solve (the fallback)
+    CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+                                     m->dstate, 0u, 0u, 0u, tol));
+    CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+    if (m->h_state->initial_residual > (double)m->s.warm_fallback_residual) {
+      CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
+      S.fell_back = 1;
+    }
+  }
+  int rc = persist_enqueue(m, P, nactive, true);
+  if (rc) return rc;
   CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
   CS_HIP(hipStreamSynchronize(st));
   const cs_solve_state& hs = *m->h_state;
   if (hs.faults) return fail(CS_ERR_DEVICE, "persistent pressure solve: a workgroup gave up waiting for its neighbours");
   ++m->persist_solves;
-  m->persist_epoch += span;
+  m->persist_epoch += persist_span(m);
   S.iterations = hs.iterations;
   S.initial_residual = fixed ? 0.0 : hs.initial_residual;
   S.final_residual = hs.residual;
@@ -574,18 +600,33 @@ int solve_persistent(cs_sim* m, const cs_params& u, uint32_t nactive, bool exter
   return CS_OK;
 }
 
-// Brick-sparse solve (cs_solver.hip) with the convergence test on the device.
-int solve_bricks(cs_sim* m, const cs_params& u) {
+// The brick solve of a recorded substep: the launches of solve_bricks' persistent
+// path, each sized for `cap` active bricks, with no host decision between them;
+// the solve's final state goes to the substep's record (m->solve_records).
+int solve_recorded(cs_sim* m, const cs_pblock& P, uint32_t cap) {
+  hipStream_t st = m->ctx->stream;
+  CS_HIP(cs_launch_brick_scan(st, P, m->brick_flag, m->brick_list, m->dstate));
+  CS_HIP(cs_launch_face_weights(st, P, m->kf, m->wc));
+  if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_warm_previous(st, P, m->kf, m->prev_p, m->pres));
+  int rc = persist_enqueue(m, P, cap, false);
+  if (rc) return rc;
+  CS_HIP(cs_launch_solve_record(st, P.d, m->dstate, m->solve_records, CS_SOLVE_RECORDS));
+  if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_copy_i32_gated(st, P.d, m->pres, m->prev_p, m->nc));
+  return CS_OK;
+}
+
+int solve_bricks(cs_sim* m, const cs_pblock& P) {
+  const cs_params& u = P.u;
   hipStream_t st = m->ctx->stream;
   cs_solve_stats& S = m->stats;
   memset(&S, 0, sizeof(S));
   S.final_residual = NAN;
   m->final_residual_pending = false;
-  CS_HIP(cs_launch_brick_scan(st, u, m->brick_flag, m->brick_list, m->dstate));
-  CS_HIP(cs_launch_face_weights(st, u, m->kf, m->wc));
+  CS_HIP(cs_launch_brick_scan(st, P, m->brick_flag, m->brick_list, m->dstate));
+  CS_HIP(cs_launch_face_weights(st, P, m->kf, m->wc));
   bool external = false;
   if (m->s.warm_start == CS_WARM_PREVIOUS) {
-    CS_HIP(cs_launch_warm_previous(st, u, m->kf, m->prev_p, m->pres));
+    CS_HIP(cs_launch_warm_previous(st, P, m->kf, m->prev_p, m->pres));
   } else if (m->s.warm_start == CS_WARM_EXTERNAL && m->has_warm) {
     CS_HIP(cs_launch_warm_external(st, u, m->kf, m->warm, m->pres, m->metrics));
     external = true;
@@ -605,7 +646,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
   if (m->persist && !m->s.rbgs_rescale_after && nactive > 0 && nactive <= CS_PERSIST_MAX_BRICKS &&
       cs_persist_groups(nactive) <= m->persist_max_groups &&
       (!fixed || m->s.pressure_rbgs_pairs > 0)) {
-    int rc = solve_persistent(m, u, nactive, external);
+    int rc = solve_persistent(m, P, nactive, external);
     if (rc) return rc;
     mg_harvest(m);
     if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_copy_i32(st, m->pres, m->prev_p, m->nc));
@@ -620,7 +661,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     if (rc) return rc;
     if (m->timing) CS_HIP(hipEventRecord(m->chunk_ev[ev0 + 1], st));
     // the final residual for the statistics (mode 0 also records launches = 0)
-    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur), m->brick_partials,
+    CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur), m->brick_partials,
                                      m->dstate, 0u, 0u, 0u, tol));
     S.iterations = m->s.pressure_rbgs_pairs;
     if (cur != 0) CS_HIP(cs_launch_select_final(st, u, nactive, m->brick_list, m->pres, m->pres2, m->dstate, false));
@@ -636,7 +677,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
       m->kt.rbgs_cells_swept += (uint64_t)launches * nactive * CS_BRICK * CS_BRICK * CS_BRICK;
     }
   } else {
-    CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+    CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
                                      m->dstate, 0u, 0u, 0u, tol));
     if (external) {
       CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
@@ -645,7 +686,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
         // the supplied start is worse than the zero start: fall back
         CS_HIP(cs_launch_fill_i32(st, m->pres, m->nc, 0));
         S.fell_back = 1;
-        CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+        CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
                                          m->dstate, 0u, 0u, 0u, tol));
       }
     }
@@ -680,7 +721,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
       // the multigrid start leaves the state's initial residual and `done` as
       // the oracle's loop sees them; its first chunk runs before any test (the
       // sweeps return at once when the start was within the tolerance)
-      int rc = mg_start(m, u, nactive);
+      int rc = mg_start(m, P, nactive);
       if (rc) return rc;
       const uint32_t chunk = std::min(every, cap);
       uint32_t launches = 0;
@@ -718,7 +759,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
           uint32_t chunk = std::min(every, cap - enq);
           const bool fuse = pending && m->fuse_residual && can_fuse(m, chunk);
           if (pending && !fuse) {
-            CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
+            CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
                                              m->brick_partials, m->dstate, 1u, prev.iters, prev.launches, tol));
           }
           uint32_t launches = 0;
@@ -743,7 +784,7 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
     }
     // the last chunk's test (a no-op once the state says done)
     if (pending)
-      CS_HIP(cs_launch_residual_bricks(st, u, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
+      CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, buf_of(m, cur),
                                        m->brick_partials, m->dstate, 1u, prev.iters, prev.launches, tol));
     CS_HIP(hipMemcpyAsync(m->h_state, m->dstate, sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
     CS_HIP(hipStreamSynchronize(st));
@@ -774,94 +815,300 @@ int solve_bricks(cs_sim* m, const cs_params& u) {
   return CS_OK;
 }
 
-int solve(cs_sim* m, const cs_params& u) { return m->dense_solver ? solve_dense(m, u) : solve_bricks(m, u); }
+int solve(cs_sim* m, const cs_pblock& P) { return m->dense_solver ? solve_dense(m, P) : solve_bricks(m, P); }
 
-int substep(cs_sim* m, float sub_dt) {
-  hipStream_t st = m->ctx->stream;
-  // 1. emission (inflow.rs:122-241): the flow model and the disk frame on the
-  // host, the particles on the device, appended after the live particles
-  cs_emit_frame ef;
+// ---- one substep (step_frame's loop body, mod.rs:305-416) ------------------------------------
+// What the host decides before a substep's first launch.
+struct SubstepPlan {
+  cs_dparams blk;   // the device block's host part
+  bool epoch_wrap;  // the persistent solve's face buffers are cleared before the substep
+};
+
+// Host side of a substep: the emission (the flow model and the disk frame,
+// inflow.rs:122-241; the particles are made on the device), the parameter block
+// (write_uniforms, mod.rs:569-670) and the persistent solve's epoch base.
+void plan_substep(cs_sim* m, float sub_dt, SubstepPlan* pl) {
+  memset(pl, 0, sizeof(*pl));
+  cs_dparams& b = pl->blk;
   // capacity counts the bed particles, which the reference's buffer holds first
-  const uint32_t emitted = cs::emit_plan(m->s, m->inflow, sub_dt, m->num_water + m->num_bed, &ef, &m->total_dropped);
-  ef.first = m->num_water;  // the water's own array
+  const uint32_t emitted = cs::emit_plan(m->s, m->inflow, sub_dt, m->num_water + m->num_bed, &b.ef, &m->total_dropped);
+  b.ef.first = m->num_water;  // the water's own array
+  b.ef.count = emitted;
   if (emitted) {
-    CS_HIP(cs_launch_emit(st, ef, m->part, m->aff));
     m->num_water += emitted;
     m->total_emitted_mass += (float)emitted * (80.0f / 320.0f);
   }
-  // 2. parameter block (write_uniforms, mod.rs:569-670)
-  cs_params u;
-  cs::make_params(m->s, m->num_water, m->num_bed_cells, sub_dt, &u);
-  // 3. clear (mod.rs:308-309) and metrics_clear (shader.rs:2282-2287)
+  cs::make_params(m->s, m->num_water, m->num_bed_cells, sub_dt, &b.u);
+  // epochs would leave the 31-bit range: forget every published word first
+  pl->epoch_wrap = (uint64_t)m->persist_epoch + persist_span(m) >= 0x80000000ull;
+  if (pl->epoch_wrap) m->persist_epoch = 1;
+  b.epoch_base = m->persist_epoch;
+  b.cap_bricks = 0xffffffffu;
+  m->total_time += sub_dt;
+}
+
+// The block's host part into the next pinned ring slot, then one copy to the
+// device block, in stream order.
+int upload_block(cs_sim* m, const cs_dparams& b) {
+  hipStream_t st = m->ctx->stream;
+  if (m->pring_next == CS_PARAM_RING) {
+    // every slot has been handed to a copy: let them all run before reuse
+    CS_HIP(hipStreamSynchronize(st));
+    m->pring_next = 0;
+  }
+  cs_dparams* h = m->h_pring + m->pring_next++;
+  memcpy(h, &b, CS_DPARAMS_HOST_BYTES);
+  CS_HIP(hipMemcpyAsync(m->dparams, h, CS_DPARAMS_HOST_BYTES, hipMemcpyHostToDevice, st));
+  return CS_OK;
+}
+
+int clear_face_buffers(cs_sim* m) {
+  CS_HIP(cs_launch_fill_i32(m->ctx->stream, reinterpret_cast<int32_t*>(m->persist_fb),
+                            2 * (size_t)m->nbricks * CS_PERSIST_FACE_WORDS, 0));
+  return CS_OK;
+}
+
+// Emission through classify (mod.rs:308-350); `emit_threads` covers the
+// substep's emitted particles.
+int substep_front(cs_sim* m, const cs_pblock& P, uint32_t emit_threads) {
+  hipStream_t st = m->ctx->stream;
+  if (emit_threads) CS_HIP(cs_launch_emit(st, P.d, emit_threads, m->part, m->aff));
+  // clear (mod.rs:308-309) and metrics_clear (shader.rs:2282-2287)
   mark(m, P_P2G);
-  CS_HIP(cs_launch_fill_i32(st, m->lanes, m->nc * 6, 0));
-  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->metrics), CS_METRICS_SLOTS, 0));
+  CS_HIP(cs_launch_fill_i32_gated(st, P.d, m->lanes, m->nc * 6, 0));
+  CS_HIP(cs_launch_fill_i32_gated(st, P.d, reinterpret_cast<int32_t*>(m->metrics), CS_METRICS_SLOTS, 0));
   // bed_lookup_clear / bed_lookup_scatter (mod.rs:322-331) and the bed cells' table
   if (m->num_bed)
-    CS_HIP(cs_launch_bed_lookup(st, u, m->num_bed, m->bed_part, m->bed_aff, m->bed_ext, m->num_bed_cells,
+    CS_HIP(cs_launch_bed_lookup(st, P, m->num_bed, m->bed_part, m->bed_aff, m->bed_ext, m->num_bed_cells,
                                 m->bed_lookup, m->bed_counter, m->bed_id, m->bed));
   kmark(m, 0);
   if (m->p2g_tiled)
-    CS_HIP(cs_launch_p2g_tiled(st, u, m->part, m->aff, m->lanes, m->metrics, m->tile_count, m->tile_start, m->tile_chunk,
+    CS_HIP(cs_launch_p2g_tiled(st, P, m->part, m->aff, m->lanes, m->metrics, m->tile_count, m->tile_start, m->tile_chunk,
                                m->p2g_key, m->p2g_sorted));
   else
-    CS_HIP(cs_launch_p2g(st, u, m->part, m->aff, m->lanes, m->metrics));
+    CS_HIP(cs_launch_p2g(st, P, m->part, m->aff, m->lanes, m->metrics));
   kmark(m, 1);
   mark(m, P_GRID);
-  CS_HIP(cs_launch_grid(st, u, m->lanes, m->gv, m->sdf));
+  CS_HIP(cs_launch_grid(st, P, m->lanes, m->gv, m->sdf));
   mark(m, P_CLASSIFY);
-  CS_HIP(cs_launch_classify(st, u, m->gv, m->cls, m->lanes, m->bed_id, m->bed, m->kind, m->kf, m->pres, m->rhs,
+  CS_HIP(cs_launch_classify(st, P, m->gv, m->cls, m->lanes, m->bed_id, m->bed, m->kind, m->kf, m->pres, m->rhs,
                             m->pack, m->tgt, m->metrics, m->sdf, m->brick_flag));
   mark(m, P_SOLVE);
-  int rc = solve(m, u);
-  if (rc) return rc;
+  return CS_OK;
+}
+
+// project_pressure through bed_dynamics (mod.rs:359-409).
+int substep_back(cs_sim* m, const cs_pblock& P, uint32_t bricks) {
+  hipStream_t st = m->ctx->stream;
   mark(m, P_POST);
   // the bed particles' grains take the reaction of the water's velocity change
   // (bed_impulse_add, shader.rs:1484-1491 and 1870-1878)
   const cs_grains gr = m->num_bed ? cs_grains{m->bed_lookup, m->bed_aff, m->bed_dw, m->num_bed} : cs_grains{};
-  CS_HIP(cs_launch_project(st, u, m->gv, m->kf, m->pres, m->bed_id, m->bed, m->sdf, gr));
+  CS_HIP(cs_launch_project(st, P, m->gv, m->kf, m->pres, m->bed_id, m->bed, m->sdf, gr));
   if (m->dense_solver)  // no brick list: the whole grid
-    CS_HIP(cs_launch_post_div(st, u, m->gv, m->kf, m->cls, m->sdf, m->tgt, m->metrics));
+    CS_HIP(cs_launch_post_div(st, P.u, m->gv, m->kf, m->cls, m->sdf, m->tgt, m->metrics));
   else
-    CS_HIP(cs_launch_post_div_bricks(st, u, m->nactive, m->brick_list, m->gv, m->kf, m->cls, m->sdf, m->tgt,
+    CS_HIP(cs_launch_post_div_bricks(st, P, bricks, m->brick_list, m->gv, m->kf, m->cls, m->sdf, m->tgt,
                                      m->metrics));
-  CS_HIP(cs_launch_pack(st, u, m->gv, m->kf, m->pack, m->sdf));
-  CS_HIP(cs_launch_visc(st, u, m->gv, m->gv2, m->cls, m->sdf, m->alias4));
+  CS_HIP(cs_launch_pack(st, P, m->gv, m->kf, m->pack, m->sdf));
+  CS_HIP(cs_launch_visc(st, P, m->gv, m->gv2, m->cls, m->sdf, m->alias4));
   mark(m, P_G2P);
   kmark(m, 2);
-  CS_HIP(cs_launch_g2p(st, u, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf, gr));
+  CS_HIP(cs_launch_g2p(st, P, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf, gr));
   kmark(m, 3);
   // bed_coupling and extraction_advect (mod.rs:391-402), outside the G2P timing
   if (m->num_bed)
-    CS_HIP(cs_launch_bed_coupling(st, u, m->part, m->aff, m->bed_lookup, m->num_bed, m->bed_ext, m->bed_dw));
+    CS_HIP(cs_launch_bed_coupling(st, P, m->part, m->aff, m->bed_lookup, m->num_bed, m->bed_ext, m->bed_dw));
   // bed_dynamics (mod.rs:404-409): the grains move under the water's drag,
   // the pressure gradient, packing contacts and the reaction impulses
   if (m->num_bed)
-    CS_HIP(cs_launch_bed_dynamics(st, u, m->num_bed, m->bed_part, m->bed_aff, m->bed_ext, m->bed_dw, m->bed_snap,
+    CS_HIP(cs_launch_bed_dynamics(st, P, m->num_bed, m->bed_part, m->bed_aff, m->bed_ext, m->bed_dw, m->bed_snap,
                                   m->bed_lookup, m->gv2, m->pres, m->alias4, m->sdf));
   mark(m, P_END);
-  if (m->timing) {
-    CS_HIP(hipEventSynchronize(m->ev[P_END]));
-    for (int p = 0; p < 6; ++p) {
-      float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, m->ev[p], m->ev[p + 1]);
-      m->pass_us[p] = ms * 1000.0f;
-    }
+  return CS_OK;
+}
+
+int harvest_timing(cs_sim* m, const cs_params& u) {
+  if (!m->timing) return CS_OK;
+  CS_HIP(hipEventSynchronize(m->ev[P_END]));
+  for (int p = 0; p < 6; ++p) {
     float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, m->ev[P_P2G], m->ev[P_END]);
-    m->pass_us[6] = ms * 1000.0f;
-    m->kt.p2g_us += kelapsed_us(m, 0, 1);
-    m->kt.p2g_launches += 1;
-    m->kt.p2g_particles += u.num_particles;
-    m->kt.g2p_us += kelapsed_us(m, 2, 3);
-    m->kt.g2p_launches += 1;
-    m->kt.g2p_particles += u.num_particles;
-    uint32_t slots[CS_METRICS_SLOTS];
-    CS_HIP(hipMemcpy(slots, m->metrics, sizeof(slots), hipMemcpyDeviceToHost));
-    m->kt.rbgs_iterations += m->stats.iterations;
-    m->kt.rbgs_cell_updates += (uint64_t)m->stats.iterations * slots[CS_M_FLUID_CELLS];
+    (void)hipEventElapsedTime(&ms, m->ev[p], m->ev[p + 1]);
+    m->pass_us[p] = ms * 1000.0f;
   }
-  m->total_time += sub_dt;
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, m->ev[P_P2G], m->ev[P_END]);
+  m->pass_us[6] = ms * 1000.0f;
+  m->kt.p2g_us += kelapsed_us(m, 0, 1);
+  m->kt.p2g_launches += 1;
+  m->kt.p2g_particles += u.num_particles;
+  m->kt.g2p_us += kelapsed_us(m, 2, 3);
+  m->kt.g2p_launches += 1;
+  m->kt.g2p_particles += u.num_particles;
+  uint32_t slots[CS_METRICS_SLOTS];
+  CS_HIP(hipMemcpy(slots, m->metrics, sizeof(slots), hipMemcpyDeviceToHost));
+  m->kt.rbgs_iterations += m->stats.iterations;
+  m->kt.rbgs_cell_updates += (uint64_t)m->stats.iterations * slots[CS_M_FLUID_CELLS];
+  return CS_OK;
+}
+
+// A planned substep, launch by launch.  With `from_solve` the launches up to
+// classify have already run (a recorded substep that halted there).
+int run_substep(cs_sim* m, const SubstepPlan& pl, bool from_solve) {
+  int rc = upload_block(m, pl.blk);
+  if (rc) return rc;
+  if (pl.epoch_wrap && !from_solve && (rc = clear_face_buffers(m))) return rc;
+  const cs_pblock P{pl.blk.u, m->dparams};
+  if (!from_solve && (rc = substep_front(m, P, pl.blk.ef.count))) return rc;
+  if ((rc = solve(m, P))) return rc;
+  if ((rc = substep_back(m, P, m->nactive))) return rc;
+  return harvest_timing(m, P.u);
+}
+
+int substep(cs_sim* m, float sub_dt) {
+  SubstepPlan pl;
+  plan_substep(m, sub_dt, &pl);
+  return run_substep(m, pl, false);
+}
+
+// ---- recorded substeps: one graph per simulation, replayed every substep -----------------------
+// Active bricks a recorded substep's launches are sized for, or 0 when this
+// simulation's substeps cannot be recorded: the persistent solve's limit and the
+// multigrid visit's (its three-level visit is chosen when it fits this many).
+uint32_t recorded_capacity(cs_sim* m) {
+  // kernel timing brackets every pass with events: launch by launch
+  if (!m->use_graphs || !m->substep_graphs || m->timing || m->dense_solver || !m->persist ||
+      m->s.rbgs_rescale_after)
+    return 0;
+  if (m->s.warm_start != CS_WARM_ZERO && m->s.warm_start != CS_WARM_PREVIOUS) return 0;
+  const bool fixed = m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS;
+  if (fixed && m->s.pressure_rbgs_pairs == 0) return 0;
+  uint32_t cap = std::min<uint32_t>(CS_PERSIST_MAX_BRICKS, m->nbricks);
+  if (!fixed && m->s.rbgs_mg_cycles && ensure_mg(m) == CS_OK && m->mg_rows3_shape) {
+    // the three-level visit needs its top level's operator within CS_MG_P3_MAX
+    // cells, which mg_start bounds by side^3 cells per active brick: give up the
+    // last few bricks of capacity for it, not more
+    const int lp = m->mg_wg_first - 1;
+    const uint64_t side = std::max<uint64_t>(1u, (uint64_t)CS_BRICK >> std::min(lp, 30));
+    const uint32_t fit = (uint32_t)(CS_MG_P3_MAX / (side * side * side));
+    if (fit < cap && fit >= cap - cap / 8u) cap = fit;
+  }
+  if (m->substep_cap) cap = std::min(cap, m->substep_cap);
+  if (cap == 0 || cs_persist_groups(cap) > m->persist_max_groups) return 0;
+  return cap;
+}
+
+// Records the substep's launch sequence (emission through bed_dynamics, the
+// solve as solve_recorded) unless the graph for this capacity and emission
+// width exists.
+int record_substep(cs_sim* m, const SubstepPlan& pl, uint32_t cap, uint32_t emit_threads) {
+  if (m->sg_exec && m->sg_cap == cap && m->sg_emit == emit_threads) return CS_OK;
+  const bool fixed = m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS;
+  if (!fixed && m->s.rbgs_mg_cycles) {
+    int rc = ensure_mg(m);  // allocations stay outside the capture
+    if (rc) return rc;
+  }
+  cs_params uc = pl.blk.u;
+  uc.num_particles = m->s.max_particles;  // launches cover every slot; the kernels read the count
+  const cs_pblock P{uc, m->dparams};
+  hipStream_t st = m->ctx->stream;
+  CS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = substep_front(m, P, emit_threads);
+  if (rc == CS_OK) rc = solve_recorded(m, P, cap);
+  if (rc == CS_OK) rc = substep_back(m, P, cap);
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(st, &g);
+  if (rc != CS_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ee != hipSuccess) return hip_fail(ee, "hipStreamEndCapture (substep)");
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(ie, "hipGraphInstantiate (substep)");
+  }
+  if (m->sg_exec) (void)hipGraphExecDestroy(m->sg_exec);
+  if (m->sg_graph) (void)hipGraphDestroy(m->sg_graph);
+  m->sg_graph = g;
+  m->sg_exec = ex;
+  m->sg_cap = cap;
+  m->sg_emit = emit_threads;
+  return CS_OK;
+}
+
+// `n` substeps of sub_dt as replays of the recorded substep, at most
+// CS_SOLVE_RECORDS between two waits.  A substep whose active bricks exceed the
+// recorded capacity halts at its brick scan, and so does every later replay;
+// the host then runs that substep from its solve on, and the rest of the
+// batch, launch by launch.
+int step_recorded(cs_sim* m, float sub_dt, uint32_t n, uint32_t cap) {
+  hipStream_t st = m->ctx->stream;
+  const bool fixed = m->s.rbgs_mode == CS_RBGS_FIXED_PAIRS;
+  std::vector<SubstepPlan> plans;
+  for (uint32_t i0 = 0; i0 < n; i0 += CS_SOLVE_RECORDS) {
+    const uint32_t cnt = std::min<uint32_t>(n - i0, CS_SOLVE_RECORDS);
+    plans.resize(cnt);
+    uint32_t emit_max = 0;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      plan_substep(m, sub_dt, &plans[k]);
+      plans[k].blk.slot = k;
+      plans[k].blk.cap_bricks = cap;
+      m->persist_epoch += persist_span(m);  // the replay's solve takes its epochs now
+      emit_max = std::max(emit_max, plans[k].blk.ef.count);
+    }
+    // emission widths in steps of 1024 threads, so that small changes of the
+    // flow keep the recorded graph
+    const uint32_t emit_threads = emit_max ? (emit_max + 1023u) / 1024u * 1024u : 0u;
+    const uint32_t width = std::max(emit_threads, m->sg_exec && m->sg_cap == cap ? m->sg_emit : 0u);
+    int rc = record_substep(m, plans[0], cap, width);
+    if (rc) {
+      // the runtime refused the recording: this simulation runs launch by launch
+      m->substep_graphs = false;
+      for (uint32_t k = 0; k < cnt; ++k) {
+        plans[k].blk.cap_bricks = 0xffffffffu;
+        if ((rc = run_substep(m, plans[k], false))) return rc;
+      }
+      continue;
+    }
+    for (uint32_t k = 0; k < cnt; ++k) {
+      if ((rc = upload_block(m, plans[k].blk))) return rc;
+      if (plans[k].epoch_wrap && (rc = clear_face_buffers(m))) return rc;
+      CS_HIP(hipGraphLaunch(m->sg_exec, st));
+    }
+    CS_HIP(hipMemcpyAsync(m->h_records, m->solve_records, cnt * sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipMemcpyAsync(m->h_halt, &m->dparams->halt, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+    const uint32_t halted = *m->h_halt ? *m->h_halt - 1u : cnt;  // the halting substep's slot + 1
+    for (uint32_t k = 0; k < halted; ++k)
+      if (m->h_records[k].faults)
+        return fail(CS_ERR_DEVICE, "persistent pressure solve: a workgroup gave up waiting for its neighbours");
+    m->persist_solves += halted;
+    ++m->recorded_batches;
+    m->recorded_substeps += halted;
+    if (halted > 0) {
+      const cs_solve_state& hs = m->h_records[halted - 1];
+      cs_solve_stats& S = m->stats;
+      memset(&S, 0, sizeof(S));
+      S.iterations = fixed ? m->s.pressure_rbgs_pairs : hs.iterations;
+      S.initial_residual = fixed ? 0.0 : hs.initial_residual;
+      S.final_residual = hs.residual;
+      S.converged = hs.residual <= (double)m->s.rbgs_tolerance ? 1u : 0u;
+      m->final_residual_pending = false;
+      m->nactive = hs.active_bricks;
+    }
+    if (halted < cnt) {
+      ++m->recorded_halts;
+      CS_HIP(hipMemsetAsync(&m->dparams->halt, 0, sizeof(uint32_t), st));
+      plans[halted].blk.cap_bricks = 0xffffffffu;
+      if ((rc = run_substep(m, plans[halted], true))) return rc;
+      for (uint32_t k = halted + 1; k < cnt; ++k) {
+        plans[k].blk.cap_bricks = 0xffffffffu;
+        if ((rc = run_substep(m, plans[k], false))) return rc;
+      }
+    }
+  }
   return CS_OK;
 }
 
@@ -1041,7 +1288,10 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   ALLOC(m->persist_xcc, geom.nbricks);
   ALLOC(m->persist_pflags, (size_t)geom.nbricks * CS_PERSIST_FLAG_STRIDE);
   ALLOC(m->persist_partials, 4 * (size_t)geom.nbricks);
+  ALLOC(m->dparams, 1);
+  ALLOC(m->solve_records, CS_SOLVE_RECORDS);
 #undef ALLOC
+  m->nbricks = geom.nbricks;
   // solver variant (benchmarking and cross-checks; both give identical results)
   // a named layout of the launch-per-chunk solve turns the persistent solve off
   for (const char* k : {"CS_SOLVER", "CS_RBGS_HALO", "CS_RBGS_MULTI", "CS_RBGS_FUSE", "CS_RBGS_GROUPS",
@@ -1059,6 +1309,8 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (const char* v = getenv("CS_RBGS_MULTI")) m->multi = atoi(v) != 0;
   if (const char* v = getenv("CS_RBGS_FUSE")) m->fuse_residual = atoi(v) != 0;
   if (const char* v = getenv("CS_GRAPHS")) m->use_graphs = atoi(v) != 0;
+  if (const char* v = getenv("CS_SUBSTEP_GRAPHS")) m->substep_graphs = atoi(v) != 0;
+  if (const char* v = getenv("CS_SUBSTEP_CAP")) m->substep_cap = (uint32_t)std::max(atoi(v), 0);
   if (const char* v = getenv("CS_P2G")) m->p2g_tiled = strcmp(v, "atomic") != 0;
   if (const char* v = getenv("CS_MG_ROWS")) m->mg_rows_allowed = atoi(v);
   {
@@ -1083,6 +1335,10 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_res), sizeof(double));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_state), sizeof(cs_solve_state));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_done), 2 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_pring), CS_PARAM_RING * sizeof(cs_dparams));
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&m->h_records), CS_SOLVE_RECORDS * sizeof(cs_solve_state));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->h_halt), sizeof(uint32_t));
   for (auto& ev : m->ev)
     if (e == hipSuccess) e = hipEventCreate(&ev);
   for (auto& ev : m->kev)
@@ -1098,6 +1354,8 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   hipStream_t st = ctx->stream;
   // no surface word is current before the first persistent solve
   e = hipMemsetAsync(m->persist_fb, 0, (size_t)geom.nbricks * CS_PERSIST_FACE_WORDS * sizeof(uint64_t), st);
+  // the device-written words of the parameter block start clear (no halt)
+  if (e == hipSuccess) e = hipMemsetAsync(m->dparams, 0, sizeof(cs_dparams), st);
   std::vector<uint32_t> sweep_table(CS_SWEEP_TABLE_WORDS);
   cs_build_sweep_table(sweep_table.data());
   if (e == hipSuccess)
@@ -1173,10 +1431,18 @@ int cs_sim_step_frame(cs_sim* m, float dt) {
   dt = cs_min(dt, 1.0f / 30.0f);  // mod.rs:265
   const uint32_t n = std::max(m->s.substeps, 1u);
   const float sub_dt = dt / (float)n;
+  if (const uint32_t cap = recorded_capacity(m)) return step_recorded(m, sub_dt, n, cap);
   for (uint32_t i = 0; i < n; ++i) {
     int rc = substep(m, sub_dt);
     if (rc) return rc;
   }
+  return CS_OK;
+}
+
+int cs_sim_get_recorded_counts(cs_sim* m, uint64_t* replayed, uint64_t* halted) {
+  if (!m || !replayed || !halted) return fail(CS_ERR_INVALID, "null argument");
+  *replayed = m->recorded_substeps;
+  *halted = m->recorded_halts;
   return CS_OK;
 }
 

@@ -15,11 +15,15 @@
 
 enum Pass { P_P2G = 0, P_GRID, P_CLASSIFY, P_SOLVE, P_POST, P_G2P, P_END, P_COUNT };
 
+#define CS_PARAM_RING 64     // pinned host copies of the parameter block in flight at most
+#define CS_SOLVE_RECORDS 64  // recorded substeps replayed between two waits of the host
+
 struct cs_sim {
   cs_ctx* ctx = nullptr;
   cs_settings s;
   uint32_t gx = 0, gy = 0, gz = 0;
   size_t nc = 0;
+  uint32_t nbricks = 0;
   // device state
   float* part = nullptr;     // 8 floats per particle
   float* aff = nullptr;      // 12 floats per particle
@@ -114,6 +118,24 @@ struct cs_sim {
   uint4* mg_achild = nullptr;          // the level below it: each list entry's children's compact ids
   hipEvent_t mg_ev[2] = {nullptr, nullptr};
   bool mg_timed = false;               // mg_ev holds an unharvested start
+  // the device-resident parameter block (cs_dparams) and the pinned ring its host
+  // part is uploaded from, one slot per substep
+  cs_dparams* dparams = nullptr;
+  cs_dparams* h_pring = nullptr;
+  uint32_t pring_next = 0;
+  // recorded substeps: the whole substep as one graph, replayed (step_recorded)
+  bool substep_graphs = true;        // CS_SUBSTEP_GRAPHS (and CS_GRAPHS)
+  uint32_t substep_cap = 0;          // CS_SUBSTEP_CAP: a smaller recorded brick capacity (0: the limit)
+  hipGraph_t sg_graph = nullptr;
+  hipGraphExec_t sg_exec = nullptr;
+  uint32_t sg_cap = 0;               // the recorded graph's active-brick capacity ...
+  uint32_t sg_emit = 0;              // ... and emission width
+  cs_solve_state* solve_records = nullptr;  // [CS_SOLVE_RECORDS] each replayed substep's final solve state
+  cs_solve_state* h_records = nullptr;      // pinned copy
+  uint32_t* h_halt = nullptr;               // pinned copy of the block's halt word
+  uint64_t recorded_batches = 0;     // host waits of step_recorded
+  uint64_t recorded_substeps = 0;    // substeps run as replays to the end
+  uint64_t recorded_halts = 0;       // replays halted at the brick scan (finished launch by launch)
   // tile-binned P2G scratch
   bool p2g_tiled = true;             // CS_P2G=atomic selects the one-thread-per-particle layout
   uint32_t* tile_count = nullptr;    // [nbricks]

@@ -21,10 +21,13 @@ struct cs_solve_state {
   uint64_t verdict;        // persistent solve: (residual test number << 32) | stop
 };
 
-hipError_t cs_launch_brick_scan(hipStream_t s, const cs_params& u, const uint32_t* flags, uint32_t* list,
+hipError_t cs_launch_brick_scan(hipStream_t s, const cs_pblock& P, const uint32_t* flags, uint32_t* list,
                                 cs_solve_state* st);
+// The state into rec[d->slot] (slots below n; nothing once the block is halted).
+hipError_t cs_launch_solve_record(hipStream_t s, const cs_dparams* d, const cs_solve_state* st, cs_solve_state* rec,
+                                  uint32_t n);
 // Face weights and diagonal of every cell, float4 per cell in `wc` [4 * nc].
-hipError_t cs_launch_face_weights(hipStream_t s, const cs_params& u, const float* kf, float* wc);
+hipError_t cs_launch_face_weights(hipStream_t s, const cs_pblock& P, const float* kf, float* wc);
 // One launch = `halo` half sweeps (colour color0, color0^1, ...) over every active brick;
 // (halo, threads) in {(1,256), (2,256), (2,512), (4,256), (4,512)}.
 hipError_t cs_launch_rbgs_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, int halo, int threads,
@@ -48,7 +51,7 @@ hipError_t cs_launch_rbgs_multi(hipStream_t s, const cs_params& u, uint32_t nact
                                 double tol, const uint32_t* sweep_table);
 // mode 0: initial residual (always runs); mode 1: after `iters` iterations in `launches` launches.
 // Two launches: per-brick partials into `partials` [2 * nactive], then a one-workgroup reduce.
-hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                                      const float* wc, const float* rhs, const int32_t* p, double* partials,
                                      cs_solve_state* st, uint32_t mode, uint32_t iters, uint32_t launches,
                                      double tol);

@@ -140,8 +140,9 @@ __device__ __forceinline__ void face_weights_cell(const cs_params& u, const floa
   }
   wc[idx] = make_float4(wplus[0], wplus[1], wplus[2], count);
 }
-__global__ void __launch_bounds__(256) k_face_weights(cs_params u, const float* __restrict__ kf,
+__global__ void __launch_bounds__(256) k_face_weights(const cs_dparams* __restrict__ dp, const float* __restrict__ kf,
                                                       float4* __restrict__ wc) {
+  CS_PARAMS_FROM(dp);
   face_weights_cell(u, kf, wc, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -170,10 +171,19 @@ __device__ __forceinline__ void brick_scan_block(const uint32_t* __restrict__ fl
     if (flags[b]) list[pos++] = b;
   if (t == 1023u) st->active_bricks = s_cnt[1023];
 }
-__global__ void __launch_bounds__(1024) k_brick_scan(const uint32_t* __restrict__ flags, uint32_t nbricks,
-                                                     uint32_t* __restrict__ list, cs_solve_state* __restrict__ st) {
+// The count also goes to the substep's parameter block, which halts when it
+// exceeds the bricks the substep's launches were sized for.
+__global__ void __launch_bounds__(1024) k_brick_scan(cs_dparams* __restrict__ dp, const uint32_t* __restrict__ flags,
+                                                     uint32_t nbricks, uint32_t* __restrict__ list,
+                                                     cs_solve_state* __restrict__ st) {
   __shared__ uint32_t s_cnt[1024];
+  if (dp->halt) return;
   brick_scan_block(flags, nbricks, list, st, s_cnt);
+  if (threadIdx.x == 1023u) {
+    const uint32_t n = s_cnt[1023];
+    dp->nactive = n;
+    if (n > dp->cap_bricks) dp->halt = dp->slot + 1u;  // which substep stopped
+  }
 }
 
 // ---- red-black sweeps over LDS-resident regions --------------------------------------------
@@ -730,14 +740,17 @@ __device__ __forceinline__ void brick_residual_sums(const cs_params& u, uint32_t
   }
 }
 
-__global__ void __launch_bounds__(256) k_residual_bricks(cs_params u, const uint32_t* __restrict__ list,
+__global__ void __launch_bounds__(256) k_residual_bricks(const cs_dparams* __restrict__ dp,
+                                                         const uint32_t* __restrict__ list,
                                                          const float4* __restrict__ wc, const float* __restrict__ rhs,
                                                          const int32_t* __restrict__ p, double* __restrict__ partials,
                                                          const cs_solve_state* __restrict__ st, uint32_t mode) {
+  CS_PARAMS_FROM(dp);
   constexpr int R3 = (BR + 2) * (BR + 2) * (BR + 2), NT = 256;
   __shared__ float4 s_wc[R3];
   __shared__ int32_t s_p[R3];
   __shared__ double s_rr[NT], s_bb[NT];
+  if (blockIdx.x >= dp->nactive) return;
   const uint32_t done = st->done;
   const uint32_t brick = list[blockIdx.x];
   if (mode != 0u && done) return;
@@ -770,12 +783,13 @@ __device__ __forceinline__ void reduce_partials_block(const double* __restrict__
     __syncthreads();
   }
 }
-__global__ void __launch_bounds__(1024) k_residual_reduce(const double* __restrict__ partials, uint32_t n,
+// One partial per active brick: the count is the brick scan's.
+__global__ void __launch_bounds__(1024) k_residual_reduce(const double* __restrict__ partials,
                                                           cs_solve_state* __restrict__ st, uint32_t mode,
                                                           uint32_t iters, uint32_t launches, double tol) {
   __shared__ double s_rr[1024], s_bb[1024];
   if (mode != 0u && st->done) return;
-  reduce_partials_block(partials, n, s_rr, s_bb);
+  reduce_partials_block(partials, st->active_bricks, s_rr, s_bb);
   if (threadIdx.x == 0) finalize_state(st, s_rr[0], s_bb[0], mode, iters, launches, tol);
 }
 
@@ -1120,16 +1134,30 @@ __global__ void k_batch_gather_states(const cs_scene_dev* __restrict__ sc, uint3
   }
 }
 
+// The solve state into the substep's record (the statistics of a replayed substep).
+__global__ void k_solve_record(const cs_dparams* __restrict__ dp, const cs_solve_state* __restrict__ st,
+                               cs_solve_state* __restrict__ rec, uint32_t n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || dp->halt || dp->slot >= n) return;
+  rec[dp->slot] = *st;
+}
+
 // ---- launchers ---------------------------------------------------------------------------
-hipError_t cs_launch_brick_scan(hipStream_t s, const cs_params& u, const uint32_t* flags, uint32_t* list,
+hipError_t cs_launch_brick_scan(hipStream_t s, const cs_pblock& P, const uint32_t* flags, uint32_t* list,
                                 cs_solve_state* st) {
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st);
-  hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, flags, u.nbricks, list, st);
+  hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, P.d, flags, P.u.nbricks, list, st);
   return hipGetLastError();
 }
 
-hipError_t cs_launch_face_weights(hipStream_t s, const cs_params& u, const float* kf, float* wc) {
-  hipLaunchKernelGGL(k_face_weights, dim3((u.nc + 255u) / 256u), dim3(256), 0, s, u, kf,
+hipError_t cs_launch_solve_record(hipStream_t s, const cs_dparams* d, const cs_solve_state* st, cs_solve_state* rec,
+                                  uint32_t n) {
+  hipLaunchKernelGGL(k_solve_record, dim3(1), dim3(64), 0, s, d, st, rec, n);
+  return hipGetLastError();
+}
+
+hipError_t cs_launch_face_weights(hipStream_t s, const cs_pblock& P, const float* kf, float* wc) {
+  const cs_params& u = P.u;
+  hipLaunchKernelGGL(k_face_weights, dim3((u.nc + 255u) / 256u), dim3(256), 0, s, P.d, kf,
                      reinterpret_cast<float4*>(wc));
   return hipGetLastError();
 }
@@ -1189,15 +1217,15 @@ void cs_build_sweep_table(uint32_t* out) {
   }
 }
 
-hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
+hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                                      const float* wc, const float* rhs, const int32_t* p, double* partials,
                                      cs_solve_state* st, uint32_t mode, uint32_t iters, uint32_t launches,
                                      double tol) {
+  const cs_params& u = P.u;
   if (nactive > 0) {
-    hipLaunchKernelGGL(k_residual_bricks, dim3(nactive), dim3(256), 0, s, u, list,
+    hipLaunchKernelGGL(k_residual_bricks, dim3(nactive), dim3(256), 0, s, P.d, list,
                        reinterpret_cast<const float4*>(wc), rhs, p, partials, st, mode);
-    hipLaunchKernelGGL(k_residual_reduce, dim3(1), dim3(1024), 0, s, partials, nactive, st, mode, iters, launches,
-                       tol);
+    hipLaunchKernelGGL(k_residual_reduce, dim3(1), dim3(1024), 0, s, partials, st, mode, iters, launches, tol);
   } else
     hipLaunchKernelGGL(k_residual_empty, dim3(1), dim3(64), 0, s, st, mode, iters, launches, tol);
   return hipGetLastError();

@@ -51,7 +51,9 @@ def main() -> int:
     for _ in range(args.substeps):
         sim.step_frame(w.frame_dt)
     sim.synchronize()
-    print(f"profiled substeps end: RBGS iterations of the last {sim.solve_stats().iterations}", flush=True)
+    replayed, halted = sim.recorded_counts()
+    print(f"profiled substeps end: RBGS iterations of the last {sim.solve_stats().iterations}; "
+          f"substeps run as graph replays since creation {replayed}, halted {halted}", flush=True)
     _KEEP.extend([sim, ctx, w])
     return 0
 

@@ -340,8 +340,22 @@ int cs_sim_reset(cs_sim* sim);
 
 /* MpmSim3D::step_frame (mod.rs:264-430): dt clamped to 1/30, split into
  * `substeps`, each substep = emit -> uniforms -> clear -> P2G -> grid update ->
- * boundary -> classify -> pressure -> project -> packing -> viscosity -> G2P. */
+ * boundary -> classify -> pressure -> project -> packing -> viscosity -> G2P.
+ * The reference records a substep's passes once and submits them as one
+ * command buffer (mod.rs:305-416) after writing the substep's uniforms
+ * (mod.rs:569-670).  Here, when the settings allow it (brick solver with the
+ * persistent solve, zero or previous-step start, no kernel timing), the whole
+ * substep is recorded once per simulation as a HIP graph whose kernels read a
+ * device-resident parameter block; each substep is one parameter copy and one
+ * graph launch, and the host waits once per frame (per 64 substeps).  A
+ * substep whose active bricks exceed the recorded launch widths stops at its
+ * brick scan and is finished launch by launch, with the same results. */
 int cs_sim_step_frame(cs_sim* sim, float dt);
+
+/* Substeps run as graph replays to the end, and replays that stopped at the
+ * brick scan and were finished launch by launch, since the handle was created
+ * (the recorded form of step_frame's single submit, mod.rs:305-416). */
+int cs_sim_get_recorded_counts(cs_sim* sim, uint64_t* replayed, uint64_t* halted);
 
 /* Blocks until every pass enqueued by this handle has finished (the reference's
  * device.poll(wait_indefinitely), physics_tests.rs:103). */
