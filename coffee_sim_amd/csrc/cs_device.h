@@ -13,6 +13,16 @@
 
 namespace {
 
+// ---- tile-binned particle passes (P2G in cs_p2g.hip, G2P in cs_kernels.hip) -------------
+// A tile is one CS_BRICK^3 brick; its footprint, the nodes its particles' 3x3x3
+// stencils reach, is (CS_BRICK + 2)^3 nodes from the brick's first cell; a work
+// item of the tile passes is at most kChunk particles of one tile.
+constexpr int TB = CS_BRICK;               // tile side in cells
+constexpr int FP = TB + 2;                 // footprint side in nodes
+constexpr int FP3 = FP * FP * FP;          // 1000 nodes
+constexpr uint32_t kNoTile = 0xffffffffu;  // the tile of a particle P2G skips
+constexpr uint32_t kChunk = 1024;          // particles per work item
+
 __device__ __forceinline__ uint32_t cell_index(const cs_params& u, uint32_t ix, uint32_t iy, uint32_t iz) {
   return iz * u.gx * u.gy + iy * u.gx + ix;  // shader.rs:148-150
 }

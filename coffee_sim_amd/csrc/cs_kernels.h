@@ -56,6 +56,13 @@ hipError_t cs_launch_visc(hipStream_t stream, const cs_pblock& P, const float* g
                           const float* sdf, uint8_t* alias4);
 hipError_t cs_launch_g2p(hipStream_t stream, const cs_pblock& P, float* part, float* aff, const float* gvel,
                          const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf, const cs_grains& grains);
+// The same over the tile bins of this substep's cs_launch_p2g_tiled (its
+// tile_count / tile_start / tile_chunk / key / sorted, the particles unmoved
+// since): each tile's footprint staged in LDS, same results as cs_launch_g2p.
+hipError_t cs_launch_g2p_tiled(hipStream_t stream, const cs_pblock& P, float* part, float* aff, const float* gvel,
+                               const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf, const cs_grains& grains,
+                               const uint32_t* tile_count, const uint32_t* tile_start, const uint32_t* tile_chunk,
+                               const uint2* key, const uint32_t* sorted);
 // prepare_render (shader.rs:2242-2276): 8 floats per particle into `out`; particles
 // below num_bed are bed particles, with their BedExtracts in bed_ext (may be null).
 hipError_t cs_launch_pack_render(hipStream_t stream, const cs_params& u, const float* part, const float* aff,

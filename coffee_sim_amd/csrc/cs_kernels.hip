@@ -667,10 +667,32 @@ __global__ void __launch_bounds__(256) k_visc(const cs_dparams* __restrict__ dp,
 }
 
 // ---- G2P (shader.rs:1683-1881) ------------------------------------------------------
+// Where G2P reads an on-grid node's grid velocity and bed-record index: the
+// global arrays, or a tile's footprint staged in LDS.  Every on-grid node of a
+// particle binned to a tile lies in that tile's footprint: the bin is the tile
+// of the base node clamped to the grid, so a base inside the grid is at most 7
+// cells past the tile's first cell (its nodes at most 9), a base below the grid
+// reaches nodes 0 and 1 only, and a base beyond it reaches no on-grid node.
+struct G2PGlobal {
+  const float4* __restrict__ gvel;
+  const int32_t* __restrict__ bed_id;
+  __device__ __forceinline__ float4 gv(uint32_t ci, int, int, int) const { return gvel[ci]; }
+  __device__ __forceinline__ int32_t bid(uint32_t ci, int, int, int) const { return bed_id[ci]; }
+};
+struct G2PTile {
+  const float4* s_gv;
+  const int32_t* s_bid;
+  int ox, oy, oz;  // the footprint's first node
+  __device__ __forceinline__ int local(int cx, int cy, int cz) const {
+    return ((cz - oz) * FP + (cy - oy)) * FP + (cx - ox);
+  }
+  __device__ __forceinline__ float4 gv(uint32_t, int cx, int cy, int cz) const { return s_gv[local(cx, cy, cz)]; }
+  __device__ __forceinline__ int32_t bid(uint32_t, int cx, int cy, int cz) const { return s_bid[local(cx, cy, cz)]; }
+};
+template <class Grid>
 __device__ __forceinline__ void g2p_particle(const cs_params& u, float4* __restrict__ part, float4* __restrict__ aff,
-                                             const float4* __restrict__ gvel, const int32_t* __restrict__ bed_id,
-                                             const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf,
-                                             const cs_grains& gr, uint32_t pid) {
+                                             const Grid& grid, const cs_bed_cell* __restrict__ bed,
+                                             const float* __restrict__ sdf, const cs_grains& gr, uint32_t pid) {
   if (pid >= u.num_particles) return;
   float4 P = part[2 * pid], Vv = part[2 * pid + 1];
   float phase = aff[3 * pid].w;
@@ -705,7 +727,7 @@ __device__ __forceinline__ void g2p_particle(const cs_params& u, float4* __restr
         if (!on_grid(u, cx, cy, cz)) continue;
         float w = wx[i] * wy[j] * wz[k];
         uint32_t ci = cell_index(u, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
-        float4 g = gvel[ci];
+        float4 g = grid.gv(ci, cx, cy, cz);
         cs_f3 gv = f3(g.x, g.y, g.z);
         cs_f3 dpos = (f3((float)i, (float)j, (float)k) - fx) * u.dx;
         if (g.w > 1e-6f) {
@@ -718,7 +740,7 @@ __device__ __forceinline__ void g2p_particle(const cs_params& u, float4* __restr
           local_mass += w * g.w;
         }
         if (u.num_bed_cells > 0u) {
-          int32_t id = bed_id[ci];
+          int32_t id = grid.bid(ci, cx, cy, cz);
           if (id >= 0 && (uint32_t)id < u.num_bed_cells) {
             const cs_bed_cell& b = bed[id];
             bed_overlap += w;
@@ -824,7 +846,63 @@ __global__ void __launch_bounds__(256) k_g2p(const cs_dparams* __restrict__ dp,
                                              const cs_bed_cell* __restrict__ bed, const float* __restrict__ sdf,
                                              cs_grains gr) {
   CS_PARAMS_FROM(dp);
-  g2p_particle(u, part, aff, gvel, bed_id, bed, sdf, gr, blockIdx.x * blockDim.x + threadIdx.x);
+  g2p_particle(u, part, aff, G2PGlobal{gvel, bed_id}, bed, sdf, gr, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// G2P over the tile bins P2G made this substep (the particles have not moved
+// since): one workgroup per chunk of at most kChunk particles of a tile, the
+// tile's footprint of grid velocities and bed-record indices staged in LDS.
+// Every particle gives the same result as in k_g2p.
+__global__ void __launch_bounds__(256) k_g2p_tile_chunks(const cs_dparams* __restrict__ dp,
+                                                         float4* __restrict__ part, float4* __restrict__ aff,
+                                                         const float4* __restrict__ gvel,
+                                                         const int32_t* __restrict__ bed_id,
+                                                         const cs_bed_cell* __restrict__ bed,
+                                                         const float* __restrict__ sdf, cs_grains gr,
+                                                         const uint32_t* __restrict__ tile_count,
+                                                         const uint32_t* __restrict__ start,
+                                                         const uint32_t* __restrict__ chunk,
+                                                         const uint32_t* __restrict__ sorted) {
+  CS_PARAMS_FROM(dp);
+  __shared__ float4 s_gv[FP3];
+  __shared__ int32_t s_bid[FP3];
+  const uint32_t w = blockIdx.x;
+  if (w >= chunk[u.nbricks]) return;
+  uint32_t lo = 0, hi = u.nbricks;  // chunk[lo] <= w < chunk[hi]
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (chunk[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t off = (w - chunk[lo]) * kChunk;
+  const uint32_t n = min(kChunk, tile_count[lo] - off);
+  const uint32_t first = start[lo] + off;
+  const int ox = (int)((lo % u.nbx) * TB), oy = (int)(((lo / u.nbx) % u.nby) * TB),
+            oz = (int)((lo / (u.nbx * u.nby)) * TB);
+  const bool beds = u.num_bed_cells > 0u;
+  for (uint32_t li = threadIdx.x; li < (uint32_t)FP3; li += 256u) {
+    const int gx = ox + (int)(li % FP), gy = oy + (int)((li / FP) % FP), gz = oz + (int)(li / (FP * FP));
+    const bool on = on_grid(u, gx, gy, gz);
+    const uint32_t ci = on ? cell_index(u, (uint32_t)gx, (uint32_t)gy, (uint32_t)gz) : 0u;
+    s_gv[li] = on ? gvel[ci] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s_bid[li] = on && beds ? bed_id[ci] : -1;
+  }
+  __syncthreads();
+  const G2PTile grid{s_gv, s_bid, ox, oy, oz};
+  for (uint32_t i = threadIdx.x; i < n; i += 256u) g2p_particle(u, part, aff, grid, bed, sdf, gr, sorted[first + i]);
+}
+// The particles P2G left unbinned (not water, or at most a tenth of the
+// nominal mass): G2P changes only their mass.
+__global__ void __launch_bounds__(256) k_g2p_unbinned(const cs_dparams* __restrict__ dp, float4* __restrict__ part,
+                                                      float4* __restrict__ aff, const float4* __restrict__ gvel,
+                                                      const int32_t* __restrict__ bed_id,
+                                                      const cs_bed_cell* __restrict__ bed,
+                                                      const float* __restrict__ sdf, cs_grains gr,
+                                                      const uint2* __restrict__ key) {
+  CS_PARAMS_FROM(dp);
+  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= u.num_particles || key[pid].x != kNoTile) return;
+  g2p_particle(u, part, aff, G2PGlobal{gvel, bed_id}, bed, sdf, gr, pid);
 }
 
 // ---- prepare_render (shader.rs:2242-2276) --------------------------------------------
@@ -1503,7 +1581,7 @@ __global__ void __launch_bounds__(256) k_batch_g2p(const cs_scene_dev* __restric
                                                    const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_PARTICLES);
   g2p_particle(u, reinterpret_cast<float4*>(d.part), reinterpret_cast<float4*>(d.aff),
-               reinterpret_cast<const float4*>(d.gv2), d.bed_id, d.bed, d.sdf, cs_grains{}, tid);
+               G2PGlobal{reinterpret_cast<const float4*>(d.gv2), d.bed_id}, d.bed, d.sdf, cs_grains{}, tid);
 }
 
 // ---- launchers -------------------------------------------------------------------------
@@ -1627,6 +1705,22 @@ hipError_t cs_launch_g2p(hipStream_t s, const cs_pblock& P, float* part, float* 
   if (u.num_particles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_g2p, blocks_for(u.num_particles), dim3(kBlock), 0, s, P.d, reinterpret_cast<float4*>(part),
                      reinterpret_cast<float4*>(aff), reinterpret_cast<const float4*>(gvel), bed_id, bed, sdf, gr);
+  return hipGetLastError();
+}
+hipError_t cs_launch_g2p_tiled(hipStream_t s, const cs_pblock& P, float* part, float* aff, const float* gvel,
+                               const int32_t* bed_id, const cs_bed_cell* bed, const float* sdf, const cs_grains& grains,
+                               const uint32_t* tile_count, const uint32_t* tile_start, const uint32_t* tile_chunk,
+                               const uint2* key, const uint32_t* sorted) {
+  const cs_params& u = P.u;
+  if (u.num_particles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_g2p_unbinned, blocks_for(u.num_particles), dim3(kBlock), 0, s, P.d,
+                     reinterpret_cast<float4*>(part), reinterpret_cast<float4*>(aff),
+                     reinterpret_cast<const float4*>(gvel), bed_id, bed, sdf, grains, key);
+  // at most one partial chunk per tile beyond the full ones
+  const uint32_t items = u.nbricks + (u.num_particles + kChunk - 1u) / kChunk;
+  hipLaunchKernelGGL(k_g2p_tile_chunks, dim3(items), dim3(256), 0, s, P.d, reinterpret_cast<float4*>(part),
+                     reinterpret_cast<float4*>(aff), reinterpret_cast<const float4*>(gvel), bed_id, bed, sdf, grains,
+                     tile_count, tile_start, tile_chunk, sorted);
   return hipGetLastError();
 }
 hipError_t cs_launch_pack_render(hipStream_t s, const cs_params& u, const float* part, const float* aff,

@@ -21,11 +21,6 @@
 
 namespace {
 
-constexpr int TB = CS_BRICK;      // tile side in cells (the solver's brick)
-constexpr int FP = TB + 2;        // footprint side in nodes
-constexpr int FP3 = FP * FP * FP;  // 1000 nodes
-constexpr uint32_t kNoTile = 0xffffffffu;
-
 // Tile of a particle's base node; bases one cell outside the grid (the
 // clamped domain reaches g = -0.5 .. n + 0.5) go to the edge tile.
 __device__ __forceinline__ uint32_t tile_of(const cs_params& u, int bx, int by, int bz) {
@@ -122,7 +117,6 @@ __global__ void __launch_bounds__(256) k_p2g_bin(const cs_dparams* __restrict__ 
 // Pass 2: exclusive scan of the tile counts (one workgroup).  With `chunk`,
 // also the exclusive scan of the tiles' chunk counts ceil(count / kChunk),
 // chunk[ntiles] holding the total: the work items of the chunked tile pass.
-constexpr uint32_t kChunk = 1024;  // particles per work item of k_p2g_tile_chunks
 __device__ __forceinline__ void p2g_scan_block(const uint32_t* __restrict__ count, uint32_t ntiles,
                                                uint32_t* __restrict__ start, uint32_t* __restrict__ chunk,
                                                uint32_t* s_sum, uint32_t* s_ch) {
@@ -156,9 +150,11 @@ __device__ __forceinline__ void p2g_scan_block(const uint32_t* __restrict__ coun
   }
   if (chunk && t == 1023u) chunk[ntiles] = s_ch[t];
 }
-__global__ void __launch_bounds__(1024) k_p2g_scan(const uint32_t* __restrict__ count, uint32_t ntiles,
+__global__ void __launch_bounds__(1024) k_p2g_scan(const cs_dparams* __restrict__ dp,
+                                                   const uint32_t* __restrict__ count, uint32_t ntiles,
                                                    uint32_t* __restrict__ start, uint32_t* __restrict__ chunk) {
   __shared__ uint32_t s_sum[1024], s_ch[1024];
+  if (dp->halt) return;  // the bins stay as a halted substep's G2P will read them
   p2g_scan_block(count, ntiles, start, chunk, s_sum, s_ch);
 }
 
@@ -345,12 +341,12 @@ hipError_t cs_launch_p2g_tiled(hipStream_t s, const cs_pblock& P, const float* p
   const cs_params& u = P.u;
   if (u.num_particles == 0) return hipSuccess;
   const dim3 pb((u.num_particles + 255u) / 256u);
-  hipError_t e = cs_launch_fill_i32(s, reinterpret_cast<int32_t*>(tile_count), u.nbricks, 0);
+  hipError_t e = cs_launch_fill_i32_gated(s, P.d, reinterpret_cast<int32_t*>(tile_count), u.nbricks, 0);
   if (e != hipSuccess) return e;
   const uint32_t per_block = 256u * kBinPerThread;
   hipLaunchKernelGGL(k_p2g_bin, dim3((u.num_particles + per_block - 1u) / per_block), dim3(256), 0, s, P.d,
                      reinterpret_cast<const float4*>(part), reinterpret_cast<const float4*>(aff), tile_count, key);
-  hipLaunchKernelGGL(k_p2g_scan, dim3(1), dim3(1024), 0, s, tile_count, u.nbricks, tile_start, tile_chunk);
+  hipLaunchKernelGGL(k_p2g_scan, dim3(1), dim3(1024), 0, s, P.d, tile_count, u.nbricks, tile_start, tile_chunk);
   hipLaunchKernelGGL(k_p2g_scatter, pb, dim3(256), 0, s, P.d, key, tile_start, sorted);
   // at most one partial chunk per tile beyond the full ones
   const uint32_t items = u.nbricks + (u.num_particles + kChunk - 1u) / kChunk;

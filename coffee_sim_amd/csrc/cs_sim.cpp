@@ -66,7 +66,7 @@ void free_sim(cs_sim* m) {
                  m->bed_snap,   m->alias4,
                  m->persist_fb, m->persist_pflags, m->persist_partials, m->persist_slot, m->persist_xcc,
                  m->mg_buf,     m->mg_flags,   m->mg_ops,     m->mg_row_buf, m->mg_row_counts, m->mg_pmap, m->mg_pnb,
-                 m->mg_achild,  m->dparams,    m->solve_records};
+                 m->mg_achild,  m->dparams};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->h_res) (void)hipHostFree(m->h_res);
@@ -602,7 +602,7 @@ int solve_persistent(cs_sim* m, const cs_pblock& P, uint32_t nactive, bool exter
 
 // The brick solve of a recorded substep: the launches of solve_bricks' persistent
 // path, each sized for `cap` active bricks, with no host decision between them;
-// the solve's final state goes to the substep's record (m->solve_records).
+// the solve's final state goes to the substep's record (m->h_records).
 int solve_recorded(cs_sim* m, const cs_pblock& P, uint32_t cap) {
   hipStream_t st = m->ctx->stream;
   CS_HIP(cs_launch_brick_scan(st, P, m->brick_flag, m->brick_list, m->dstate));
@@ -610,7 +610,7 @@ int solve_recorded(cs_sim* m, const cs_pblock& P, uint32_t cap) {
   if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_warm_previous(st, P, m->kf, m->prev_p, m->pres));
   int rc = persist_enqueue(m, P, cap, false);
   if (rc) return rc;
-  CS_HIP(cs_launch_solve_record(st, P.d, m->dstate, m->solve_records, CS_SOLVE_RECORDS));
+  CS_HIP(cs_launch_solve_record(st, P.d, m->dstate, m->h_records, m->h_halt, CS_SOLVE_RECORDS));
   if (m->s.warm_start == CS_WARM_PREVIOUS) CS_HIP(cs_launch_copy_i32_gated(st, P.d, m->pres, m->prev_p, m->nc));
   return CS_OK;
 }
@@ -914,7 +914,11 @@ int substep_back(cs_sim* m, const cs_pblock& P, uint32_t bricks) {
   CS_HIP(cs_launch_visc(st, P, m->gv, m->gv2, m->cls, m->sdf, m->alias4));
   mark(m, P_G2P);
   kmark(m, 2);
-  CS_HIP(cs_launch_g2p(st, P, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf, gr));
+  if (m->p2g_tiled && m->g2p_tiled)  // the bins P2G made this substep
+    CS_HIP(cs_launch_g2p_tiled(st, P, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf, gr, m->tile_count,
+                               m->tile_start, m->tile_chunk, m->p2g_key, m->p2g_sorted));
+  else
+    CS_HIP(cs_launch_g2p(st, P, m->part, m->aff, m->gv2, m->bed_id, m->bed, m->sdf, gr));
   kmark(m, 3);
   // bed_coupling and extraction_advect (mod.rs:391-402), outside the G2P timing
   if (m->num_bed)
@@ -1077,8 +1081,7 @@ int step_recorded(cs_sim* m, float sub_dt, uint32_t n, uint32_t cap) {
       if (plans[k].epoch_wrap && (rc = clear_face_buffers(m))) return rc;
       CS_HIP(hipGraphLaunch(m->sg_exec, st));
     }
-    CS_HIP(hipMemcpyAsync(m->h_records, m->solve_records, cnt * sizeof(cs_solve_state), hipMemcpyDeviceToHost, st));
-    CS_HIP(hipMemcpyAsync(m->h_halt, &m->dparams->halt, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    // the replays wrote their records and the halt word straight into pinned host memory
     CS_HIP(hipStreamSynchronize(st));
     const uint32_t halted = *m->h_halt ? *m->h_halt - 1u : cnt;  // the halting substep's slot + 1
     for (uint32_t k = 0; k < halted; ++k)
@@ -1289,7 +1292,6 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   ALLOC(m->persist_pflags, (size_t)geom.nbricks * CS_PERSIST_FLAG_STRIDE);
   ALLOC(m->persist_partials, 4 * (size_t)geom.nbricks);
   ALLOC(m->dparams, 1);
-  ALLOC(m->solve_records, CS_SOLVE_RECORDS);
 #undef ALLOC
   m->nbricks = geom.nbricks;
   // solver variant (benchmarking and cross-checks; both give identical results)
@@ -1312,6 +1314,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (const char* v = getenv("CS_SUBSTEP_GRAPHS")) m->substep_graphs = atoi(v) != 0;
   if (const char* v = getenv("CS_SUBSTEP_CAP")) m->substep_cap = (uint32_t)std::max(atoi(v), 0);
   if (const char* v = getenv("CS_P2G")) m->p2g_tiled = strcmp(v, "atomic") != 0;
+  if (const char* v = getenv("CS_G2P")) m->g2p_tiled = strcmp(v, "global") != 0;
   if (const char* v = getenv("CS_MG_ROWS")) m->mg_rows_allowed = atoi(v);
   {
     int cus = 0;

@@ -130,14 +130,15 @@ struct cs_sim {
   hipGraphExec_t sg_exec = nullptr;
   uint32_t sg_cap = 0;               // the recorded graph's active-brick capacity ...
   uint32_t sg_emit = 0;              // ... and emission width
-  cs_solve_state* solve_records = nullptr;  // [CS_SOLVE_RECORDS] each replayed substep's final solve state
-  cs_solve_state* h_records = nullptr;      // pinned copy
-  uint32_t* h_halt = nullptr;               // pinned copy of the block's halt word
+  cs_solve_state* h_records = nullptr;  // pinned [CS_SOLVE_RECORDS]: each replayed substep's final solve
+                                        // state, written by the device
+  uint32_t* h_halt = nullptr;           // pinned: the block's halt word, written by the device
   uint64_t recorded_batches = 0;     // host waits of step_recorded
   uint64_t recorded_substeps = 0;    // substeps run as replays to the end
   uint64_t recorded_halts = 0;       // replays halted at the brick scan (finished launch by launch)
   // tile-binned P2G scratch
   bool p2g_tiled = true;             // CS_P2G=atomic selects the one-thread-per-particle layout
+  bool g2p_tiled = true;             // G2P over P2G's tile bins, footprints in LDS (CS_G2P=global: per particle)
   uint32_t* tile_count = nullptr;    // [nbricks]
   uint32_t* tile_start = nullptr;    // [nbricks]
   uint32_t* tile_chunk = nullptr;    // [nbricks + 1] the chunked tile pass's work-item prefix

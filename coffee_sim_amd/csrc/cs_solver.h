@@ -23,9 +23,10 @@ struct cs_solve_state {
 
 hipError_t cs_launch_brick_scan(hipStream_t s, const cs_pblock& P, const uint32_t* flags, uint32_t* list,
                                 cs_solve_state* st);
-// The state into rec[d->slot] (slots below n; nothing once the block is halted).
+// The state into rec[d->slot] (slots below n; nothing once the block is halted)
+// and the halt word into *halt; rec and halt may be pinned host memory.
 hipError_t cs_launch_solve_record(hipStream_t s, const cs_dparams* d, const cs_solve_state* st, cs_solve_state* rec,
-                                  uint32_t n);
+                                  uint32_t* halt, uint32_t n);
 // Face weights and diagonal of every cell, float4 per cell in `wc` [4 * nc].
 hipError_t cs_launch_face_weights(hipStream_t s, const cs_pblock& P, const float* kf, float* wc);
 // One launch = `halo` half sweeps (colour color0, color0^1, ...) over every active brick;

@@ -1134,10 +1134,14 @@ __global__ void k_batch_gather_states(const cs_scene_dev* __restrict__ sc, uint3
   }
 }
 
-// The solve state into the substep's record (the statistics of a replayed substep).
+// The solve state into the substep's record (the statistics of a replayed
+// substep) and the block's halt word into `halt`; both may be host memory.
 __global__ void k_solve_record(const cs_dparams* __restrict__ dp, const cs_solve_state* __restrict__ st,
-                               cs_solve_state* __restrict__ rec, uint32_t n) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || dp->halt || dp->slot >= n) return;
+                               cs_solve_state* __restrict__ rec, uint32_t* __restrict__ halt, uint32_t n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t h = dp->halt;
+  *halt = h;
+  if (h || dp->slot >= n) return;
   rec[dp->slot] = *st;
 }
 
@@ -1150,8 +1154,8 @@ hipError_t cs_launch_brick_scan(hipStream_t s, const cs_pblock& P, const uint32_
 }
 
 hipError_t cs_launch_solve_record(hipStream_t s, const cs_dparams* d, const cs_solve_state* st, cs_solve_state* rec,
-                                  uint32_t n) {
-  hipLaunchKernelGGL(k_solve_record, dim3(1), dim3(64), 0, s, d, st, rec, n);
+                                  uint32_t* halt, uint32_t n) {
+  hipLaunchKernelGGL(k_solve_record, dim3(1), dim3(64), 0, s, d, st, rec, halt, n);
   return hipGetLastError();
 }
 

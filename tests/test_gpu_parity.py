@@ -417,6 +417,40 @@ def test_p2g_binning_with_scattered_particles_matches_oracle(ctx, orc):
     compare_state(sim, ref, "scattered P2G")
 
 
+@pytest.mark.parametrize("g2p", ["tiles", "global"])
+def test_g2p_over_tile_bins_with_unbinned_and_outside_particles_matches_oracle(ctx, orc, monkeypatch, g2p):
+    """G2P over P2G's tile bins (footprints staged in LDS) on a 90^3 box, whose
+    last tiles are partial, with particles P2G leaves unbinned (not water, or at
+    most a tenth of the nominal mass, including zero and negative masses) and
+    particles outside the domain on every side: two substeps equal the oracle
+    bit for bit, as does the per-particle G2P (CS_G2P=global)."""
+    from scenes import SceneArrays, box_settings
+
+    if g2p == "global":
+        monkeypatch.setenv("CS_G2P", "global")
+    n = 90
+    s = box_settings(n=n)
+    side = float(s.bounds_size[0])
+    rng = np.random.default_rng(31)
+    m = 30000
+    pos = rng.uniform(-0.45 * side, 0.45 * side, size=(m, 3))
+    out = rng.random(m) < 0.05
+    pos[out] = rng.uniform(-0.6 * side, 0.6 * side, size=(int(out.sum()), 3))
+    vel = rng.normal(0.0, 1.0, size=(m, 3))
+    mass = np.full(m, 0.25)
+    light = rng.random(m) < 0.03
+    mass[light] = rng.choice([0.02, 0.0, -0.01, 0.025], size=int(light.sum()))
+    part = np.concatenate([pos, np.ones((m, 1)), vel, mass[:, None]], axis=1).astype(np.float32)
+    aff = (rng.normal(0.0, 0.5, size=(m, 12)) * np.tile([1, 1, 1, 0], 3)).astype(np.float32)
+    aff[rng.random(m) < 0.03, 3] = 1.0  # not water
+    scene = SceneArrays(part, aff)
+    sim, ref = _pair(ctx, orc, s, scene.scene)
+    for k in range(2):
+        sim.step_frame(1.0 / 600.0)
+        ref.step_frame(1.0 / 600.0)
+        compare_state(sim, ref, f"G2P over tile bins ({g2p}), substep {k}")
+
+
 @pytest.mark.parametrize("variant", [{}, {"CS_RBGS_FUSE": "0"}, {"CS_SOLVER": "dense"}],
                          ids=["fused", "unfused", "dense"])
 def test_solve_with_cells_held_at_the_clamp_reaches_tolerance(ctx, orc, monkeypatch, variant):
