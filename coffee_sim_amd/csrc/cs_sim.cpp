@@ -1434,7 +1434,10 @@ int cs_sim_step_frame(cs_sim* m, float dt) {
   dt = cs_min(dt, 1.0f / 30.0f);  // mod.rs:265
   const uint32_t n = std::max(m->s.substeps, 1u);
   const float sub_dt = dt / (float)n;
-  if (const uint32_t cap = recorded_capacity(m)) return step_recorded(m, sub_dt, n, cap);
+  // replays while the last substep's active bricks fit the recorded launches
+  // (a scene beyond them would halt every replay at its brick scan)
+  if (const uint32_t cap = recorded_capacity(m))
+    if (m->nactive <= cap) return step_recorded(m, sub_dt, n, cap);
   for (uint32_t i = 0; i < n; ++i) {
     int rc = substep(m, sub_dt);
     if (rc) return rc;
