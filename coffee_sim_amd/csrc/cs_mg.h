@@ -12,7 +12,7 @@
 #define CS_MG_NU 2u              // red-black smoothing passes before and after each coarse correction
 #define CS_MG_GAMMA 2u           // coarse corrections per visit (W-cycle)
 #define CS_MG_SCALE 1.5f         // weight of a prolonged correction
-#define CS_MG_COARSE 10u         // passes on the coarsest level
+#define CS_MG_COARSE 10u         // passes per visit of the coarsest level, visited at most 4 times a cycle
 #define CS_MG_COARSEST_SIDE 8u   // levels are added until no side exceeds this
 #define CS_MG_MAX_LEVELS 12
 #define CS_MG_WG_CELLS 4096u     // levels this small run inside one workgroup
@@ -43,6 +43,16 @@ struct cs_mg_op {
   uint8_t level;
 };
 
+// Passes per visit of the coarsest of `nl` levels: CS_MG_COARSE while a cycle
+// visits it at most 4 times (CS_MG_GAMMA^(nl-2) visits), else the same 40
+// passes per cycle spread over the visits (at least 2 each).
+inline uint32_t cs_mg_coarse_passes(int nl) {
+  uint32_t visits = 1;
+  for (int l = 2; l < nl; ++l) visits *= CS_MG_GAMMA;
+  if (visits <= 4u) return CS_MG_COARSE;
+  const uint32_t p = 4u * CS_MG_COARSE / visits;
+  return p < 2u ? 2u : p;
+}
 // The level dimensions the start uses on a grid of the given size.
 std::vector<cs_mg_level> cs_mg_shape(uint32_t gx, uint32_t gy, uint32_t gz);
 // The operations of one W-cycle visit of `level` over levels [level, nlevels).
@@ -107,12 +117,13 @@ bool cs_mg_rows3_fit(const cs_mg_level* levels, int first, int nl);
 hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_level* P, cs_mg_row* rows,
                       uint32_t* counts, uint32_t* pmap, uint4* pnb, uint4* achild, const uint32_t* skip);
 hipError_t cs_mg_rows_visit(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
-                            const uint32_t* counts, const uint32_t* skip);
+                            const uint32_t* counts, uint32_t coarse_passes, const uint32_t* skip);
 // A whole visit of P (A's and B's visits inside) in one workgroup, all three
 // levels in LDS; P's corrections written back to P.E at the end.
 hipError_t cs_mg_rows3_prepare();  // once, before the first cs_mg_rows3_visit
 hipError_t cs_mg_rows3_visit(hipStream_t s, cs_mg_level P, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
-                             const uint32_t* counts, const uint4* pnb, const uint4* achild, const uint32_t* skip);
+                             const uint32_t* counts, const uint4* pnb, const uint4* achild, uint32_t coarse_passes,
+                             const uint32_t* skip);
 // After a residual test (mode 0, into st->initial_residual): flags[0] = skip
 // the next cycle, flags[1] = the start before any cycle was within the
 // tolerance (`first` sets it).  Every MG kernel reads flags[0].

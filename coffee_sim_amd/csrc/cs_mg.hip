@@ -19,7 +19,7 @@
 //  - a visit of a level: CS_MG_NU red-black passes, then CS_MG_GAMMA times
 //    {restrict the residual, zero and visit the coarser level, add
 //    CS_MG_SCALE times its correction, CS_MG_NU passes}; the coarsest level
-//    takes CS_MG_COARSE passes.  A cycle ends with p <- clamp(p + e), stored in
+//    takes cs_mg_coarse_passes(levels) passes.  A cycle ends with p <- clamp(p + e), stored in
 //    fixed point, and the residual test: the cycles stop once the start is
 //    within the tolerance (decided on the device).  The iteration that follows
 //    makes its first test after a chunk, so the clamp's own updates always run.
@@ -789,11 +789,12 @@ __device__ __forceinline__ void mg_vrestrict(vcell& v, const cs_mg_level& A, con
 
 // A visit of A (its right-hand side in a[q].r, its correction zero in EA):
 // CS_MG_NU passes, then CS_MG_GAMMA times {residuals into RA, restriction into
-// B, CS_MG_COARSE passes of B, prolongation, CS_MG_NU passes}.  a[2c + s] holds
+// B, `ncoarse` passes of B, prolongation, CS_MG_NU passes}.  a[2c + s] holds
 // A's colour-c list entry t + s * CS_MG_ROWS_THREADS, b[c] B's entry t.  RA
 // holds +0.0 outside A's operator.
 __device__ __forceinline__ void mg_visit_ab(vcell (&a)[4], vcell (&b)[2], const cs_mg_level& A,
-                                            const cs_mg_level& B, sfloat* EA, sfloat* RA, sfloat* EB) {
+                                            const cs_mg_level& B, sfloat* EA, sfloat* RA, sfloat* EB,
+                                            uint32_t ncoarse) {
   const uint32_t ay = A.X, az = A.X * A.Y, by = B.X, bz = B.X * B.Y;
   for (uint32_t k = 0; k < CS_MG_NU; ++k) {
     mg_vsmooth(a[0], EA, ay, az);
@@ -817,7 +818,7 @@ __device__ __forceinline__ void mg_visit_ab(vcell (&a)[4], vcell (&b)[2], const 
     mg_vrestrict(b[0], A, RA, EB);
     mg_vrestrict(b[1], A, RA, EB);
     __syncthreads();
-    for (uint32_t k = 0; k < CS_MG_COARSE; ++k) {
+    for (uint32_t k = 0; k < ncoarse; ++k) {
       mg_vsmooth(b[0], EB, by, bz);
       __syncthreads();
       mg_vsmooth(b[1], EB, by, bz);
@@ -857,6 +858,7 @@ __device__ __forceinline__ void mg_load_ab(vcell (&a)[4], vcell (&b)[2], const c
 __global__ void __launch_bounds__(CS_MG_ROWS_THREADS) k_mg_rows_visit(cs_mg_level A, cs_mg_level B,
                                                                       const cs_mg_row* __restrict__ rows,
                                                                       const uint32_t* __restrict__ counts,
+                                                                      uint32_t ncoarse,
                                                                       const uint32_t* __restrict__ skip) {
   if (*skip) return;
   __shared__ float s_ea[CS_MG_ROWS_A];
@@ -878,7 +880,7 @@ __global__ void __launch_bounds__(CS_MG_ROWS_THREADS) k_mg_rows_visit(cs_mg_leve
   }
   for (uint32_t k = t; k < nB; k += CS_MG_ROWS_THREADS) EB[k] = 0.0f;
   __syncthreads();
-  mg_visit_ab(a, b, A, B, EA, RA, EB);
+  mg_visit_ab(a, b, A, B, EA, RA, EB, ncoarse);
   for (uint32_t k = t; k < nA; k += CS_MG_ROWS_THREADS) A.E[k] = EA[k];
 }
 
@@ -1054,6 +1056,7 @@ __global__ void __launch_bounds__(CS_MG_ROWS3_THREADS) k_mg_rows3_visit(cs_mg_le
                                                                        const uint32_t* __restrict__ counts,
                                                                        const uint4* __restrict__ pnb,
                                                                        const uint4* __restrict__ achild,
+                                                                       uint32_t ncoarse,
                                                                        const uint32_t* __restrict__ skip) {
   if (*skip) return;
   extern __shared__ float lds_raw[];
@@ -1192,7 +1195,7 @@ __global__ void __launch_bounds__(CS_MG_ROWS3_THREADS) k_mg_rows3_visit(cs_mg_le
       restrictB(0u);
       restrictB(1u);
       __syncthreads();
-      for (uint32_t k = 0; k < CS_MG_COARSE; ++k) {
+      for (uint32_t k = 0; k < ncoarse; ++k) {
         smoothB(0u);
         __syncthreads();
         smoothB(1u);
@@ -1267,7 +1270,7 @@ void cs_mg_visit_ops(int level, int nlevels, std::vector<cs_mg_op>* out) {
     out->push_back(cs_mg_op{CS_MG_OP_SMOOTH1, (uint8_t)l});
   };
   if (level == nlevels - 1) {
-    for (uint32_t k = 0; k < CS_MG_COARSE; ++k) pass(level);
+    for (uint32_t k = 0; k < cs_mg_coarse_passes(nlevels); ++k) pass(level);
     return;
   }
   for (uint32_t k = 0; k < CS_MG_NU; ++k) pass(level);
@@ -1386,8 +1389,9 @@ hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_l
   return hipGetLastError();
 }
 hipError_t cs_mg_rows_visit(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
-                            const uint32_t* counts, const uint32_t* skip) {
-  hipLaunchKernelGGL(k_mg_rows_visit, dim3(1), dim3(CS_MG_ROWS_THREADS), 0, s, A, B, rows, counts, skip);
+                            const uint32_t* counts, uint32_t coarse_passes, const uint32_t* skip) {
+  hipLaunchKernelGGL(k_mg_rows_visit, dim3(1), dim3(CS_MG_ROWS_THREADS), 0, s, A, B, rows, counts, coarse_passes,
+                     skip);
   return hipGetLastError();
 }
 hipError_t cs_mg_rows3_prepare() {
@@ -1395,9 +1399,10 @@ hipError_t cs_mg_rows3_prepare() {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)mg_rows3_lds_bytes());
 }
 hipError_t cs_mg_rows3_visit(hipStream_t s, cs_mg_level P, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
-                             const uint32_t* counts, const uint4* pnb, const uint4* achild, const uint32_t* skip) {
+                             const uint32_t* counts, const uint4* pnb, const uint4* achild, uint32_t coarse_passes,
+                             const uint32_t* skip) {
   hipLaunchKernelGGL(k_mg_rows3_visit, dim3(1), dim3(CS_MG_ROWS3_THREADS), mg_rows3_lds_bytes(), s, P, A, B, rows,
-                     counts, pnb, achild, skip);
+                     counts, pnb, achild, coarse_passes, skip);
   return hipGetLastError();
 }
 hipError_t cs_mg_gate(hipStream_t s, const cs_solve_state* st, double tol, uint32_t* flags, bool first) {

@@ -400,11 +400,12 @@ int mg_visit(cs_sim* m, const cs_pblock& P, uint32_t nactive, int l) {
   const uint32_t* skip = m->mg_flags;
   if (l == m->mg_wg_first - 1 && m->mg_rows3) {
     CS_HIP(cs_mg_rows3_visit(st, m->mg_levels[l], m->mg_levels[l + 1], m->mg_levels[l + 2], m->mg_row_buf,
-                             m->mg_row_counts, m->mg_pnb, m->mg_achild, skip));
+                             m->mg_row_counts, m->mg_pnb, m->mg_achild, cs_mg_coarse_passes(nl), skip));
     return CS_OK;
   }
   if (l >= m->mg_wg_first && m->mg_rows) {
-    CS_HIP(cs_mg_rows_visit(st, m->mg_levels[l], m->mg_levels[l + 1], m->mg_row_buf, m->mg_row_counts, skip));
+    CS_HIP(cs_mg_rows_visit(st, m->mg_levels[l], m->mg_levels[l + 1], m->mg_row_buf, m->mg_row_counts,
+                            cs_mg_coarse_passes(nl), skip));
     return CS_OK;
   }
   if (l >= m->mg_wg_first) {
@@ -422,7 +423,7 @@ int mg_visit(cs_sim* m, const cs_pblock& P, uint32_t nactive, int l) {
       }
     return CS_OK;
   };
-  if (l == nl - 1) return passes(CS_MG_COARSE);
+  if (l == nl - 1) return passes(cs_mg_coarse_passes(nl));
   int rc = passes(CS_MG_NU);
   if (rc) return rc;
   const cs_mg_level& C = m->mg_levels[l + 1];

@@ -828,13 +828,13 @@ void orc_read_pressure(orc_sim* S, int32_t* p, int32_t* kind) {
  * MG_NU red-black smoothing passes, then twice { restrict the residual to
  * level l+1, zero its correction, visit it, add MG_SCALE times the prolonged
  * (piecewise-constant) correction, MG_NU passes }; the coarsest level gets
- * MG_COARSE passes.  Every per-cell sum runs in a fixed order (faces -x, +x,
+ * mg_coarse_passes(nl) passes.  Every per-cell sum runs in a fixed order (faces -x, +x,
  * -y, +y, -z, +z; children dz, dy, dx ascending), so the device reproduces it
  * bit for bit; a cycle ends with p <- clamp(p + e) stored in fixed point. */
 #define MG_NU 2u
 #define MG_GAMMA 2u
 #define MG_SCALE 1.5f
-#define MG_COARSE 10u
+#define MG_COARSE 10u /* per visit of the coarsest level while a cycle visits it at most 4 times */
 #define MG_COARSEST_SIDE 8u
 #define MG_MAX_LEVELS 12
 
@@ -931,9 +931,19 @@ static void mg_prolong(mg_level* f, const mg_level* c) {
         if (f->D[i] > 0.0f) f->E[i] = f->E[i] + MG_SCALE * c->E[mg_idx(c, x >> 1, y >> 1, z >> 1)];
       }
 }
+/* Passes per visit of the coarsest of nl levels: MG_COARSE while a cycle visits
+ * it at most 4 times (MG_GAMMA^(nl-2) visits), else the same 40 passes per
+ * cycle spread over the visits (at least 2 each). */
+static uint32_t mg_coarse_passes(int nl) {
+  uint32_t visits = 1;
+  for (int l = 2; l < nl; ++l) visits *= MG_GAMMA;
+  if (visits <= 4u) return MG_COARSE;
+  const uint32_t p = 4u * MG_COARSE / visits;
+  return p < 2u ? 2u : p;
+}
 static void mg_visit(mg_level* L, int l, int nl) {
   if (l == nl - 1) {
-    for (uint32_t k = 0; k < MG_COARSE; ++k) { mg_smooth(&L[l], 0u); mg_smooth(&L[l], 1u); }
+    for (uint32_t k = 0; k < mg_coarse_passes(nl); ++k) { mg_smooth(&L[l], 0u); mg_smooth(&L[l], 1u); }
     return;
   }
   for (uint32_t k = 0; k < MG_NU; ++k) { mg_smooth(&L[l], 0u); mg_smooth(&L[l], 1u); }
@@ -1022,7 +1032,7 @@ static double mg_start(orc_sim* S, const orc_params* u, uint32_t cycles, double 
     mg_fine_level(S, u, &L[0], fluid);
     for (int l = 0; l + 1 < nl; ++l) mg_build(&L[l], &L[l + 1]);
     if (nl > 1) mg_visit(L, 0, nl);
-    else for (uint32_t k = 0; k < MG_COARSE; ++k) { mg_smooth(&L[0], 0u); mg_smooth(&L[0], 1u); }
+    else for (uint32_t k = 0; k < mg_coarse_passes(nl); ++k) { mg_smooth(&L[0], 0u); mg_smooth(&L[0], 1u); }
     for (size_t i = 0; i < S->nc; ++i) {
       if (!(L[0].D[i] > 0.0f)) continue;
       const float pn = pressure_load(S, u, i) + L[0].E[i];
