@@ -25,6 +25,8 @@
 struct cs_mg_level {
   uint32_t X, Y, Z;
   uint32_t mX, mY;  // floor((2^32 - 1) / X) and / Y: division by X and Y as a multiply-high
+  uint32_t bnx, bny;  // nonzero: the arrays are brick-major (8^3 bricks of 512 cells, bnx x bny
+                      // bricks a layer), else x-fastest
   float* D;
   float* Wx;
   float* Wy;

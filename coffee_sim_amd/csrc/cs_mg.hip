@@ -59,6 +59,14 @@ template <class LV>
 __device__ __forceinline__ size_t lidx(const LV& L, uint32_t x, uint32_t y, uint32_t z) {
   return ((size_t)z * L.Y + y) * L.X + x;
 }
+// A level in HBM may be brick-major (the fine level: a brick's cells, which
+// its workgroup visits together, are one contiguous 2 KB run per array).
+__device__ __forceinline__ size_t lidx(const cs_mg_level& L, uint32_t x, uint32_t y, uint32_t z) {
+  if (L.bnx)
+    return ((((size_t)(z >> 3) * L.bny + (y >> 3)) * L.bnx + (x >> 3)) << 9) + ((z & 7u) << 6) + ((y & 7u) << 3) +
+           (x & 7u);
+  return ((size_t)z * L.Y + y) * L.X + x;
+}
 
 // The six neighbour terms of a cell against the corrections, summed in face order.
 // Neighbour indices clamped onto the grid (the cell itself past an edge): every
@@ -87,10 +95,33 @@ __device__ __forceinline__ nbr6 neighbours(uint32_t X, uint32_t Y, uint32_t Z, u
   return n;
 }
 
+// A level's neighbour indices (lidx of the clamped neighbours).
+template <class LV>
+__device__ __forceinline__ nbr6 lneighbours(const LV& L, uint32_t x, uint32_t y, uint32_t z, size_t i) {
+  return neighbours(L.X, L.Y, L.Z, x, y, z, i);
+}
+__device__ __forceinline__ nbr6 lneighbours(const cs_mg_level& L, uint32_t x, uint32_t y, uint32_t z, size_t i) {
+  if (!L.bnx) return neighbours(L.X, L.Y, L.Z, x, y, z, i);
+  nbr6 n;
+  n.bxm = x > 0u;
+  n.bxp = x + 1u < L.X;
+  n.bym = y > 0u;
+  n.byp = y + 1u < L.Y;
+  n.bzm = z > 0u;
+  n.bzp = z + 1u < L.Z;
+  n.xm = n.bxm ? lidx(L, x - 1u, y, z) : i;
+  n.xp = n.bxp ? lidx(L, x + 1u, y, z) : i;
+  n.ym = n.bym ? lidx(L, x, y - 1u, z) : i;
+  n.yp = n.byp ? lidx(L, x, y + 1u, z) : i;
+  n.zm = n.bzm ? lidx(L, x, y, z - 1u) : i;
+  n.zp = n.bzp ? lidx(L, x, y, z + 1u) : i;
+  return n;
+}
+
 template <class LV>
 __device__ __forceinline__ float offsum(const LV& L, uint32_t x, uint32_t y, uint32_t z, size_t i) {
   const auto v = L.E;
-  const nbr6 n = neighbours(L.X, L.Y, L.Z, x, y, z, i);
+  const nbr6 n = lneighbours(L, x, y, z, i);
   const float w0 = L.Wx[n.xm], w1 = L.Wx[i], w2 = L.Wy[n.ym], w3 = L.Wy[i], w4 = L.Wz[n.zm], w5 = L.Wz[i];
   const float e0 = v[n.xm], e1 = v[n.xp], e2 = v[n.ym], e3 = v[n.yp], e4 = v[n.zm], e5 = v[n.zp];
   const float t0 = n.bxm ? w0 * e0 : 0.0f;
@@ -187,7 +218,7 @@ __device__ __forceinline__ void smooth_prolonged_cell(const F& f, const C& c, ui
   const size_t i = lidx(f, x, y, z);
   const float d = f.D[i];
   if (!(d > 0.0f)) return;
-  const nbr6 n = neighbours(f.X, f.Y, f.Z, x, y, z, i);
+  const nbr6 n = lneighbours(f, x, y, z, i);
   const float w0 = f.Wx[n.xm], w1 = f.Wx[i], w2 = f.Wy[n.ym], w3 = f.Wy[i], w4 = f.Wz[n.zm], w5 = f.Wz[i];
   // neighbour coordinates clamped like the indices (a clamped term is discarded)
   const float e0 = prolonged(f, c, n.bxm ? x - 1u : x, y, z, n.xm);
@@ -287,7 +318,8 @@ __global__ void __launch_bounds__(256) k_mg_fine_setup(const cs_dparams* __restr
   for (uint32_t k = threadIdx.x; k < (uint32_t)(BR * BR * BR); k += 256u) {
     const uint32_t x = x0 + (k & 7u), y = y0 + ((k >> 3) & 7u), z = z0 + (k >> 6);
     if (x >= u.gx || y >= u.gy || z >= u.gz) continue;
-    const size_t i = lidx(L, x, y, z);
+    const size_t i = ((size_t)z * u.gy + y) * u.gx + x;  // the grid's arrays
+    const size_t li = lidx(L, x, y, z);                   // the level's
     const size_t sy = u.gx, sz = (size_t)u.gx * u.gy;
     const float4 w = wc[i];
     float d = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f, r = 0.0f;
@@ -298,12 +330,12 @@ __global__ void __launch_bounds__(256) k_mg_fine_setup(const cs_dparams* __restr
       if (z + 1u < u.gz && is_free(u, wc, rhs, p, x, y, z + 1u, i + sz)) wz = w.z;
       r = (fine_sum(u, wc, p, x, y, z, i, w) - rhs[i]) - w.w * ((float)p[i] * u.inv_fp_scale);
     }
-    L.D[i] = d;
-    L.Wx[i] = wx;
-    L.Wy[i] = wy;
-    L.Wz[i] = wz;
-    L.R[i] = r;
-    L.E[i] = 0.0f;
+    L.D[li] = d;
+    L.Wx[li] = wx;
+    L.Wy[li] = wy;
+    L.Wz[li] = wz;
+    L.R[li] = r;
+    L.E[li] = 0.0f;
   }
 }
 
@@ -376,9 +408,9 @@ __global__ void __launch_bounds__(256) k_mg_fine_apply(const cs_dparams* __restr
   for (uint32_t k = threadIdx.x; k < (uint32_t)(BR * BR * BR); k += 256u) {
     const uint32_t x = x0 + (k & 7u), y = y0 + ((k >> 3) & 7u), z = z0 + (k >> 6);
     if (x >= u.gx || y >= u.gy || z >= u.gz) continue;
-    const size_t i = lidx(L, x, y, z);
-    if (!(L.D[i] > 0.0f)) continue;
-    const float pn = (float)p[i] * u.inv_fp_scale + L.E[i];
+    const size_t i = ((size_t)z * u.gy + y) * u.gx + x, li = lidx(L, x, y, z);
+    if (!(L.D[li] > 0.0f)) continue;
+    const float pn = (float)p[i] * u.inv_fp_scale + L.E[li];
     const float cl = fminf(fmaxf(pn, -u.pressure_clamp), u.pressure_clamp);
     p[i] = cs_f2i(cl * u.fp_scale);
   }

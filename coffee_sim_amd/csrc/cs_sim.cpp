@@ -339,13 +339,21 @@ int capture_batch(cs_sim* m, const cs_params& u, uint32_t nactive, uint32_t ever
 int ensure_mg(cs_sim* m) {
   if (m->mg_buf) return CS_OK;
   std::vector<cs_mg_level> lv = cs_mg_shape(m->s.grid_dims[0], m->s.grid_dims[1], m->s.grid_dims[2]);
+  // the fine level brick-major: a workgroup's brick is one contiguous run per array
+  auto cells = [](const cs_mg_level& l) {
+    return l.bnx ? (size_t)l.bnx * l.bny * ((l.Z + 7u) / 8u) * 512u : (size_t)l.X * l.Y * l.Z;
+  };
+  if (m->mg_brick_major) {
+    lv[0].bnx = (lv[0].X + 7u) / 8u;
+    lv[0].bny = (lv[0].Y + 7u) / 8u;
+  }
   size_t total = 0;
-  for (const auto& l : lv) total += 6 * (size_t)l.X * l.Y * l.Z;
+  for (const auto& l : lv) total += 6 * cells(l);
   CS_HIP(dalloc(&m->mg_buf, total));
   CS_HIP(dalloc(&m->mg_flags, 3));
   size_t off = 0;
   for (auto& l : lv) {
-    const size_t n = (size_t)l.X * l.Y * l.Z;
+    const size_t n = cells(l);
     float* b = m->mg_buf + off;
     l.D = b;
     l.Wx = b + n;
@@ -356,6 +364,7 @@ int ensure_mg(cs_sim* m) {
     off += 6 * n;
   }
   m->mg_floats = total;
+  m->mg_n0 = cells(lv[0]);
   const int nl = (int)lv.size();
   // the first coarse level small enough for one workgroup; its visits run whole
   m->mg_wg_first = nl;
@@ -461,8 +470,8 @@ int mg_start(cs_sim* m, const cs_pblock& P, uint32_t nactive) {
   // kernels touch only the active bricks' fine cells: the fine correction and
   // every coarse level start from zero everywhere else
   const cs_mg_level& L0 = m->mg_levels[0];
-  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(L0.E), m->nc, 0));
-  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->mg_buf + 6 * m->nc), m->mg_floats - 6 * m->nc, 0));
+  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(L0.E), m->mg_n0, 0));
+  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->mg_buf + 6 * m->mg_n0), m->mg_floats - 6 * m->mg_n0, 0));
   const int nl = (int)m->mg_levels.size();
   const uint32_t* skip = m->mg_flags;
   // the level above the small ones joins their visit when its operator fits
@@ -1317,6 +1326,7 @@ int cs_sim_create(cs_ctx* ctx, const cs_settings* s, const cs_scene* scene, cs_s
   if (const char* v = getenv("CS_P2G")) m->p2g_tiled = strcmp(v, "atomic") != 0;
   if (const char* v = getenv("CS_G2P")) m->g2p_tiled = strcmp(v, "global") != 0;
   if (const char* v = getenv("CS_MG_ROWS")) m->mg_rows_allowed = atoi(v);
+  if (const char* v = getenv("CS_MG_BRICKS")) m->mg_brick_major = atoi(v) != 0;
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->ctx->device) == hipSuccess && cus > 0)

@@ -100,6 +100,8 @@ struct cs_sim {
   std::vector<cs_mg_level> mg_levels;  // level 0 = the fine grid
   float* mg_buf = nullptr;             // every level's D, Wx, Wy, Wz, R, E
   size_t mg_floats = 0;
+  size_t mg_n0 = 0;                    // cells of the fine level's arrays (brick-major: whole bricks)
+  bool mg_brick_major = true;          // CS_MG_BRICKS: the fine level's arrays brick-major (0: x-fastest)
   uint32_t* mg_flags = nullptr;        // [0] skip the next cycle, [1] no cycle ran (start within the tolerance),
                                        // [2] cycles run in the current start
   cs_mg_op* mg_ops = nullptr;          // one visit of level mg_wg_first, run by one workgroup

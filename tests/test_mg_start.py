@@ -209,17 +209,21 @@ def _pair(ctx, orc, s, scene):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("visit", ["rows3", "rows", "oplist"])
+@pytest.mark.parametrize("visit", ["rows3", "rows", "oplist", "xfastest"])
 @pytest.mark.parametrize("layout", ["persistent", "launches"])
 @pytest.mark.parametrize("scale", [4, 2])
 def test_mg_start_matches_oracle(ctx, orc, monkeypatch, layout, scale, visit):
     """visit: the small levels' visit with its rows in registers, joined by the
     level above them where that is not the fine level (the default; on the
     32^3 grid the same as "rows"), the two-level register visit
-    (CS_MG_ROWS=1), or the operation-list visit (CS_MG_ROWS=0)."""
+    (CS_MG_ROWS=1), or the operation-list visit (CS_MG_ROWS=0).  The fine
+    level's arrays are brick-major, or (xfastest, CS_MG_BRICKS=0) laid out
+    like the grid."""
     if layout == "launches":
         monkeypatch.setenv("CS_RBGS_PERSIST", "0")
-    monkeypatch.setenv("CS_MG_ROWS", {"rows3": "2", "rows": "1", "oplist": "0"}[visit])
+    if visit == "xfastest":
+        monkeypatch.setenv("CS_MG_BRICKS", "0")
+    monkeypatch.setenv("CS_MG_ROWS", {"rows3": "2", "rows": "1", "oplist": "0", "xfastest": "2"}[visit])
     w = cs.SeededWorkload("v60_working", seed=6, scale=scale)
     nc = int(np.prod(list(w.settings.grid_dims)))
     starts = (cs.abi.CS_WARM_ZERO, cs.abi.CS_WARM_PREVIOUS, cs.abi.CS_WARM_EXTERNAL) if scale == 4 else \
