@@ -533,6 +533,8 @@ def main() -> int:
         args.warmup = 1 if args.warmup is None else args.warmup
         return bench_batch(args)
     if args.workload == "unet128":
+        args.steps = 20 if args.steps is None else args.steps
+        args.warmup = 3 if args.warmup is None else args.warmup
         return bench_unet(args)
     if args.workload == "learned128":
         args.steps = 8 if args.steps is None else args.steps
