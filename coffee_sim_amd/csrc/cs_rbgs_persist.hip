@@ -477,35 +477,31 @@ __global__ void __launch_bounds__(NT, 4) k_rbgs_persist(cs_persist_args a) {
 
 // Clears the slot map, the residual-check flags, the XCD table and the
 // failure state of a solve.
-__global__ void __launch_bounds__(256) k_persist_prepare(cs_solve_state* __restrict__ st, uint32_t nbricks,
-                                                         int32_t* __restrict__ slot, uint32_t* __restrict__ pflags,
-                                                         uint32_t* __restrict__ xcc) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nbricks) {
+// One workgroup: every brick's slot, check flag and XCD word cleared, then
+// each listed brick's slot set (the clears land before the barrier).
+__global__ void __launch_bounds__(1024) k_persist_prepare(const uint32_t* __restrict__ list,
+                                                          cs_solve_state* __restrict__ st, uint32_t nbricks,
+                                                          int32_t* __restrict__ slot, uint32_t* __restrict__ pflags,
+                                                          uint32_t* __restrict__ xcc) {
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < nbricks; i += 1024u) {
     slot[i] = -1;
     pflags[(size_t)i * CS_PERSIST_FLAG_STRIDE] = 0u;
     xcc[i] = 0u;
   }
-  if (i == 0) {
+  if (t == 0) {
     st->abort = 0u;
     st->faults = 0u;
     st->verdict = 0ull;
   }
-}
-
-// Each listed brick's position in the list (its workgroup's run).
-__global__ void __launch_bounds__(256) k_persist_slots(const uint32_t* __restrict__ list,
-                                                       const cs_solve_state* __restrict__ st,
-                                                       int32_t* __restrict__ slot) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < st->active_bricks) slot[list[i]] = (int32_t)i;
+  __syncthreads();
+  const uint32_t n = st->active_bricks;
+  for (uint32_t i = t; i < n; i += 1024u) slot[list[i]] = (int32_t)i;
 }
 
 hipError_t cs_persist_prepare(hipStream_t s, const cs_params& u, const uint32_t* list, cs_solve_state* st,
                               int32_t* slot, uint32_t* pflags, uint32_t* xcc) {
-  const dim3 g((u.nbricks + 255u) / 256u);
-  hipLaunchKernelGGL(k_persist_prepare, g, dim3(256), 0, s, st, u.nbricks, slot, pflags, xcc);
-  hipLaunchKernelGGL(k_persist_slots, g, dim3(256), 0, s, list, st, slot);
+  hipLaunchKernelGGL(k_persist_prepare, dim3(1), dim3(1024), 0, s, list, st, u.nbricks, slot, pflags, xcc);
   return hipGetLastError();
 }
 

@@ -470,8 +470,8 @@ int mg_start(cs_sim* m, const cs_pblock& P, uint32_t nactive) {
   // kernels touch only the active bricks' fine cells: the fine correction and
   // every coarse level start from zero everywhere else
   const cs_mg_level& L0 = m->mg_levels[0];
-  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(L0.E), m->mg_n0, 0));
-  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(m->mg_buf + 6 * m->mg_n0), m->mg_floats - 6 * m->mg_n0, 0));
+  // (the fine correction is the last of the fine arrays, the coarse levels follow it: one clear)
+  CS_HIP(cs_launch_fill_i32(st, reinterpret_cast<int32_t*>(L0.E), m->mg_floats - 5 * m->mg_n0, 0));
   const int nl = (int)m->mg_levels.size();
   const uint32_t* skip = m->mg_flags;
   // the level above the small ones joins their visit when its operator fits

@@ -146,6 +146,16 @@ __global__ void __launch_bounds__(256) k_face_weights(const cs_dparams* __restri
   face_weights_cell(u, kf, wc, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// The solve state's reset before a solve's first residual test.
+__device__ __forceinline__ void state_init(cs_solve_state* st) {
+  st->done = 0u;
+  st->iterations = 0u;
+  st->launches = 0u;
+  st->pending_fires = 0u;
+  st->residual = 0.0;
+  st->initial_residual = 0.0;
+}
+
 // ---- active brick list ----------------------------------------------------------------
 // Single block: ascending list of flagged bricks and their count.
 __device__ __forceinline__ void brick_scan_block(const uint32_t* __restrict__ flags, uint32_t nbricks,
@@ -178,6 +188,7 @@ __global__ void __launch_bounds__(1024) k_brick_scan(cs_dparams* __restrict__ dp
                                                      cs_solve_state* __restrict__ st) {
   __shared__ uint32_t s_cnt[1024];
   if (dp->halt) return;
+  if (threadIdx.x == 0) state_init(st);
   brick_scan_block(flags, nbricks, list, st, s_cnt);
   if (threadIdx.x == 1023u) {
     const uint32_t n = s_cnt[1023];
@@ -868,16 +879,7 @@ __global__ void __launch_bounds__(256) k_select_final(cs_params u, const uint32_
   }
 }
 
-__global__ void k_state_init(cs_solve_state* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->done = 0u;
-    st->iterations = 0u;
-    st->launches = 0u;
-    st->pending_fires = 0u;
-    st->residual = 0.0;
-    st->initial_residual = 0.0;
-  }
-}
+
 
 // ---- batched solve: every scene of a batch in one launch per step ---------------------------
 // The active bricks of the scenes still solving are compacted into one list of
@@ -893,21 +895,14 @@ __global__ void k_state_init(cs_solve_state* st) {
   const uint32_t blk = blockIdx.x - st_[scene_];                                                      \
   const cs_params u = d.u
 
-// Solve-state reset and active-brick scan (k_state_init + k_brick_scan), one
+// Solve-state reset and active-brick scan (as k_brick_scan), one
 // workgroup per scene.
 __global__ void __launch_bounds__(1024) k_batch_scan(const cs_scene_dev* __restrict__ sc,
                                                      const uint32_t* __restrict__ starts, uint32_t n) {
   __shared__ uint32_t s_cnt[1024];
   CS_BATCH_SCENE(CS_SPACE_SCENES);
   (void)blk;
-  if (threadIdx.x == 0) {
-    d.st->done = 0u;
-    d.st->iterations = 0u;
-    d.st->launches = 0u;
-    d.st->pending_fires = 0u;
-    d.st->residual = 0.0;
-    d.st->initial_residual = 0.0;
-  }
+  if (threadIdx.x == 0) state_init(d.st);
   brick_scan_block(d.brick_flag, u.nbricks, d.brick_list, d.st, s_cnt);
 }
 
@@ -1148,7 +1143,6 @@ __global__ void k_solve_record(const cs_dparams* __restrict__ dp, const cs_solve
 // ---- launchers ---------------------------------------------------------------------------
 hipError_t cs_launch_brick_scan(hipStream_t s, const cs_pblock& P, const uint32_t* flags, uint32_t* list,
                                 cs_solve_state* st) {
-  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st);
   hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, P.d, flags, P.u.nbricks, list, st);
   return hipGetLastError();
 }
