@@ -126,10 +126,6 @@ hipError_t cs_mg_rows3_prepare();  // once, before the first cs_mg_rows3_visit
 hipError_t cs_mg_rows3_visit(hipStream_t s, cs_mg_level P, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,
                              const uint32_t* counts, const uint4* pnb, const uint4* achild, uint32_t coarse_passes,
                              const uint32_t* skip);
-// After a residual test (mode 0, into st->initial_residual): flags[0] = skip
-// the next cycle, flags[1] = the start before any cycle was within the
-// tolerance (`first` sets it).  Every MG kernel reads flags[0].
-hipError_t cs_mg_gate(hipStream_t s, const cs_solve_state* st, double tol, uint32_t* flags, bool first);
-// Clears st->done unless flags[1]: after a multigrid start the iteration runs
-// at least one chunk before its first test.
-hipError_t cs_mg_finish(hipStream_t s, cs_solve_state* st, const uint32_t* flags);
+// flags, set by each residual test of the start (cs_launch_residual_gate):
+// flags[0] = skip the next cycle, flags[1] = the start before any cycle was
+// within the tolerance.  Every MG kernel reads flags[0].

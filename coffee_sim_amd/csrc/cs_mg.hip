@@ -1250,25 +1250,6 @@ __global__ void __launch_bounds__(CS_MG_ROWS3_THREADS) k_mg_rows3_visit(cs_mg_le
   mg_p3_list<3>(q1, lP1, nP1, cP1, P, A, EP, RP, EA, t);
 }
 
-// flags[0]: skip the next cycle; flags[1]: the start was within the tolerance
-// (no cycle ran); flags[2]: cycles run so far.  The first gate sets all three,
-// later ones count the cycle just run and set flags[0].
-__global__ void k_mg_gate(const cs_solve_state* __restrict__ st, double tol, uint32_t* __restrict__ flags,
-                          uint32_t first) {
-  const uint32_t within = st->initial_residual <= tol ? 1u : 0u;
-  if (first) {
-    flags[1] = within;
-    flags[2] = 0u;
-  } else if (!flags[0]) {
-    flags[2] += 1u;
-  }
-  flags[0] = flags[1] | within;
-}
-// After the cycles: the iteration runs (its first test comes after a chunk)
-// unless the start was already within the tolerance.
-__global__ void k_mg_finish(cs_solve_state* __restrict__ st, const uint32_t* __restrict__ flags) {
-  if (!flags[1]) st->done = 0u;
-}
 
 inline dim3 cells_grid(uint32_t n) { return dim3((n + 255u) / 256u); }
 inline uint32_t cells(const cs_mg_level& L) { return L.X * L.Y * L.Z; }
@@ -1435,13 +1416,5 @@ hipError_t cs_mg_rows3_visit(hipStream_t s, cs_mg_level P, cs_mg_level A, cs_mg_
                              const uint32_t* skip) {
   hipLaunchKernelGGL(k_mg_rows3_visit, dim3(1), dim3(CS_MG_ROWS3_THREADS), mg_rows3_lds_bytes(), s, P, A, B, rows,
                      counts, pnb, achild, coarse_passes, skip);
-  return hipGetLastError();
-}
-hipError_t cs_mg_gate(hipStream_t s, const cs_solve_state* st, double tol, uint32_t* flags, bool first) {
-  hipLaunchKernelGGL(k_mg_gate, dim3(1), dim3(1), 0, s, st, tol, flags, first ? 1u : 0u);
-  return hipGetLastError();
-}
-hipError_t cs_mg_finish(hipStream_t s, cs_solve_state* st, const uint32_t* flags) {
-  hipLaunchKernelGGL(k_mg_finish, dim3(1), dim3(1), 0, s, st, flags);
   return hipGetLastError();
 }

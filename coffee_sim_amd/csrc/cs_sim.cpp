@@ -464,9 +464,8 @@ int mg_start(cs_sim* m, const cs_pblock& P, uint32_t nactive) {
   hipStream_t st = m->ctx->stream;
   const double tol = (double)m->s.rbgs_tolerance;
   if (m->timing) CS_HIP(hipEventRecord(m->mg_ev[0], st));
-  CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
-                                   m->dstate, 0u, 0u, 0u, tol));
-  CS_HIP(cs_mg_gate(st, m->dstate, tol, m->mg_flags, true));
+  CS_HIP(cs_launch_residual_gate(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+                                 m->dstate, tol, m->mg_flags, true, m->s.rbgs_mg_cycles == 0));
   // kernels touch only the active bricks' fine cells: the fine correction and
   // every coarse level start from zero everywhere else
   const cs_mg_level& L0 = m->mg_levels[0];
@@ -496,11 +495,9 @@ int mg_start(cs_sim* m, const cs_pblock& P, uint32_t nactive) {
     if (rc) return rc;
     CS_HIP(cs_mg_fine_apply(st, P, nactive, m->brick_list, L0, m->pres, skip));
     // the cycle's residual test: later cycles return at once within the tolerance
-    CS_HIP(cs_launch_residual_bricks(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
-                                     m->dstate, 0u, 0u, 0u, tol));
-    CS_HIP(cs_mg_gate(st, m->dstate, tol, m->mg_flags, false));
+    CS_HIP(cs_launch_residual_gate(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, m->brick_partials,
+                                   m->dstate, tol, m->mg_flags, false, c + 1 == m->s.rbgs_mg_cycles));
   }
-  CS_HIP(cs_mg_finish(st, m->dstate, m->mg_flags));
   if (m->timing) {
     CS_HIP(hipEventRecord(m->mg_ev[1], st));
     m->mg_timed = true;

@@ -56,6 +56,14 @@ hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_pblock& P, uint32_t
                                      const float* wc, const float* rhs, const int32_t* p, double* partials,
                                      cs_solve_state* st, uint32_t mode, uint32_t iters, uint32_t launches,
                                      double tol);
+// The initial residual (mode 0) followed, in the reduce's launch, by the
+// multigrid start's gate on it: flags[0] skip the next cycle, flags[1] the
+// start was within `tol` (set by the first test), flags[2] cycles run.  After
+// the last test st->done is cleared unless flags[1]: after a multigrid start
+// the iteration runs at least one chunk before its first test.
+hipError_t cs_launch_residual_gate(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
+                                   const float* wc, const float* rhs, const int32_t* p, double* partials,
+                                   cs_solve_state* st, double tol, uint32_t* flags, bool first, bool last);
 // The one-time rescale p <- alpha p (alpha = <b,p>/<p,Ap> over the residual's rows) of the
 // fluid cells of `p`, skipped when the state says done; alpha is left in st->rescale.
 hipError_t cs_launch_rescale(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,

@@ -794,22 +794,46 @@ __device__ __forceinline__ void reduce_partials_block(const double* __restrict__
     __syncthreads();
   }
 }
-// One partial per active brick: the count is the brick scan's.
+// The multigrid start's gate on the initial residual just finalised (mode 0).
+// flags[0]: skip the next cycle; flags[1]: the start was within the tolerance
+// (no cycle runs); flags[2]: cycles run so far.  The first test sets all
+// three, later ones count the cycle just run and set flags[0].  After the
+// start's last test (mode bit 1) the iteration runs (its first test comes
+// after a chunk) unless the start was already within the tolerance.
+__device__ __forceinline__ void mg_gate(cs_solve_state* st, double tol, uint32_t* flags, uint32_t mode) {
+  const uint32_t within = st->initial_residual <= tol ? 1u : 0u;
+  if (mode & 1u) {
+    flags[1] = within;
+    flags[2] = 0u;
+  } else if (!flags[0]) {
+    flags[2] += 1u;
+  }
+  flags[0] = flags[1] | within;
+  if ((mode & 2u) && !flags[1]) st->done = 0u;
+}
+// One partial per active brick: the count is the brick scan's.  With `gate`,
+// the multigrid start's gate follows in the same thread (gate_mode: bit 0 its
+// first test, bit 1 its last).
 __global__ void __launch_bounds__(1024) k_residual_reduce(const double* __restrict__ partials,
                                                           cs_solve_state* __restrict__ st, uint32_t mode,
-                                                          uint32_t iters, uint32_t launches, double tol) {
+                                                          uint32_t iters, uint32_t launches, double tol,
+                                                          uint32_t* __restrict__ gate, uint32_t gate_mode) {
   __shared__ double s_rr[1024], s_bb[1024];
   if (mode != 0u && st->done) return;
   reduce_partials_block(partials, st->active_bricks, s_rr, s_bb);
-  if (threadIdx.x == 0) finalize_state(st, s_rr[0], s_bb[0], mode, iters, launches, tol);
+  if (threadIdx.x == 0) {
+    finalize_state(st, s_rr[0], s_bb[0], mode, iters, launches, tol);
+    if (gate) mg_gate(st, tol, gate, gate_mode);
+  }
 }
 
 // The residual of a system with no fluid cell (r = 0: nothing to solve).
 __global__ void k_residual_empty(cs_solve_state* __restrict__ st, uint32_t mode, uint32_t iters, uint32_t launches,
-                                 double tol) {
+                                 double tol, uint32_t* __restrict__ gate, uint32_t gate_mode) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (mode != 0u && st->done) return;
   finalize_state(st, 0.0, 0.0, mode, iters, launches, tol);
+  if (gate) mg_gate(st, tol, gate, gate_mode);
 }
 
 // ---- one-time Galerkin rescale ----------------------------------------------------------------
@@ -1215,18 +1239,30 @@ void cs_build_sweep_table(uint32_t* out) {
   }
 }
 
+static hipError_t residual_bricks(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
+                                  const float* wc, const float* rhs, const int32_t* p, double* partials,
+                                  cs_solve_state* st, uint32_t mode, uint32_t iters, uint32_t launches, double tol,
+                                  uint32_t* gate, uint32_t gate_mode) {
+  if (nactive > 0) {
+    hipLaunchKernelGGL(k_residual_bricks, dim3(nactive), dim3(256), 0, s, P.d, list,
+                       reinterpret_cast<const float4*>(wc), rhs, p, partials, st, mode);
+    hipLaunchKernelGGL(k_residual_reduce, dim3(1), dim3(1024), 0, s, partials, st, mode, iters, launches, tol, gate,
+                       gate_mode);
+  } else
+    hipLaunchKernelGGL(k_residual_empty, dim3(1), dim3(64), 0, s, st, mode, iters, launches, tol, gate, gate_mode);
+  return hipGetLastError();
+}
 hipError_t cs_launch_residual_bricks(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                                      const float* wc, const float* rhs, const int32_t* p, double* partials,
                                      cs_solve_state* st, uint32_t mode, uint32_t iters, uint32_t launches,
                                      double tol) {
-  const cs_params& u = P.u;
-  if (nactive > 0) {
-    hipLaunchKernelGGL(k_residual_bricks, dim3(nactive), dim3(256), 0, s, P.d, list,
-                       reinterpret_cast<const float4*>(wc), rhs, p, partials, st, mode);
-    hipLaunchKernelGGL(k_residual_reduce, dim3(1), dim3(1024), 0, s, partials, st, mode, iters, launches, tol);
-  } else
-    hipLaunchKernelGGL(k_residual_empty, dim3(1), dim3(64), 0, s, st, mode, iters, launches, tol);
-  return hipGetLastError();
+  return residual_bricks(s, P, nactive, list, wc, rhs, p, partials, st, mode, iters, launches, tol, nullptr, 0u);
+}
+hipError_t cs_launch_residual_gate(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
+                                   const float* wc, const float* rhs, const int32_t* p, double* partials,
+                                   cs_solve_state* st, double tol, uint32_t* flags, bool first, bool last) {
+  return residual_bricks(s, P, nactive, list, wc, rhs, p, partials, st, 0u, 0u, 0u, tol, flags,
+                         (first ? 1u : 0u) | (last ? 2u : 0u));
 }
 
 hipError_t cs_launch_rescale(hipStream_t s, const cs_params& u, uint32_t nactive, const uint32_t* list,
