@@ -208,11 +208,13 @@ __device__ __forceinline__ void classify_cell(const cs_params& u, const float4* 
   cm.p_clamp = fires_p;
   brick_flag[((iz / CS_BRICK) * u.nby + iy / CS_BRICK) * u.nbx + ix / CS_BRICK] = 1u;
 }
-// classify_cell for one cell per lane, then its metrics (max |div|, max
-// |div - target|, fluid cells, clamp fires) reduced over the wavefront and
-// added with one atomic per slot and workgroup (of at most 256 threads): the
-// slots are a max and counts, so the result does not depend on how the cells
-// were grouped.  Every thread of the workgroup must call it.
+// classify_cell for NC cells per lane (idx0, idx0 + stride, ...), then their
+// metrics (max |div|, max |div - target|, fluid cells, clamp fires) reduced
+// over the wavefront and added with one atomic per slot and workgroup (of at
+// most 256 threads): the slots are a max and counts, so the result does not
+// depend on how the cells were grouped, and NC cells a lane cut the atomics on
+// the five addresses NC-fold.  Every thread of the workgroup must call it.
+template <int NC>
 __device__ __forceinline__ void classify_wave(const cs_params& u, const float4* __restrict__ gvel,
                                               const uint8_t* __restrict__ cls, const int32_t* __restrict__ lanes,
                                               const int32_t* __restrict__ bed_id,
@@ -221,10 +223,19 @@ __device__ __forceinline__ void classify_wave(const cs_params& u, const float4* 
                                               float* __restrict__ rhs, int32_t* __restrict__ pack,
                                               float* __restrict__ tgt, uint32_t* __restrict__ metrics,
                                               const float* __restrict__ sdf, uint32_t* __restrict__ brick_flag,
-                                              uint32_t idx) {
+                                              uint32_t idx0, uint32_t stride) {
   cs_cls_metrics cm{};
-  classify_cell(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, tgt, metrics, sdf, brick_flag,
-                idx, cm);
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    cs_cls_metrics c{};
+    classify_cell(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, tgt, metrics, sdf, brick_flag,
+                  idx0 + (uint32_t)j * stride, c);
+    cm.max_div = max(cm.max_div, c.max_div);
+    cm.max_res = max(cm.max_res, c.max_res);
+    cm.fluid += c.fluid;
+    cm.div_clamp += c.div_clamp;
+    cm.p_clamp += c.p_clamp;
+  }
   for (int off = 32; off > 0; off >>= 1) {
     cm.max_div = max(cm.max_div, (uint32_t)__shfl_xor((int)cm.max_div, off));
     cm.max_res = max(cm.max_res, (uint32_t)__shfl_xor((int)cm.max_res, off));
@@ -255,6 +266,7 @@ __device__ __forceinline__ void classify_wave(const cs_params& u, const float4* 
     }
   }
 }
+constexpr uint32_t kClassifyCells = 4;  // cells per thread of k_classify, a workgroup's stride apart
 __global__ void __launch_bounds__(256) k_classify(const cs_dparams* __restrict__ dp, const float4* __restrict__ gvel,
                                                   const uint8_t* __restrict__ cls, const int32_t* __restrict__ lanes,
                                                   const int32_t* __restrict__ bed_id,
@@ -265,8 +277,8 @@ __global__ void __launch_bounds__(256) k_classify(const cs_dparams* __restrict__
                                                   uint32_t* __restrict__ metrics, const float* __restrict__ sdf,
                                                   uint32_t* __restrict__ brick_flag) {
   CS_PARAMS_FROM(dp);
-  classify_wave(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, tgt, metrics, sdf, brick_flag,
-                blockIdx.x * blockDim.x + threadIdx.x);
+  classify_wave<kClassifyCells>(u, gvel, cls, lanes, bed_id, bed, kind_out, kf_out, pres, rhs, pack, tgt, metrics,
+                                sdf, brick_flag, blockIdx.x * blockDim.x * kClassifyCells + threadIdx.x, blockDim.x);
 }
 
 // ---- warm starts (injected after classify zeroed the pressure) -------------------
@@ -1545,8 +1557,8 @@ __global__ void __launch_bounds__(256) k_batch_grid(const cs_scene_dev* __restri
 __global__ void __launch_bounds__(256) k_batch_classify(const cs_scene_dev* __restrict__ sc,
                                                         const uint32_t* __restrict__ starts, uint32_t n) {
   CS_BATCH_SCENE(CS_SPACE_CELLS);
-  classify_wave(u, reinterpret_cast<const float4*>(d.gv), d.cls, d.lanes, d.bed_id, d.bed, d.kind, d.kf, d.pres,
-                d.rhs, d.pack, d.tgt, d.metrics, d.sdf, d.brick_flag, tid);
+  classify_wave<1>(u, reinterpret_cast<const float4*>(d.gv), d.cls, d.lanes, d.bed_id, d.bed, d.kind, d.kf, d.pres,
+                   d.rhs, d.pack, d.tgt, d.metrics, d.sdf, d.brick_flag, tid, 0u);
 }
 
 __global__ void __launch_bounds__(256) k_batch_project(const cs_scene_dev* __restrict__ sc,
@@ -1634,7 +1646,8 @@ hipError_t cs_launch_classify(hipStream_t s, const cs_pblock& P, const float* gv
   const cs_params& u = P.u;
   hipLaunchKernelGGL(k_fill_i32_gated, dim3((u.nbricks + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P.d,
                      reinterpret_cast<int32_t*>(brick_flag), (size_t)u.nbricks, 0);
-  hipLaunchKernelGGL(k_classify, blocks_for(u.nc), dim3(kBlock), 0, s, P.d, reinterpret_cast<const float4*>(gvel), cls,
+  hipLaunchKernelGGL(k_classify, dim3((unsigned)((u.nc + kBlock * kClassifyCells - 1) / (kBlock * kClassifyCells))),
+                     dim3(kBlock), 0, s, P.d, reinterpret_cast<const float4*>(gvel), cls,
                      lanes, bed_id, bed, kind, kf, pres, rhs, pack, tgt, metrics, sdf, brick_flag);
   return hipGetLastError();
 }
