@@ -64,8 +64,11 @@ void cs_mg_visit_ops(int level, int nlevels, std::vector<cs_mg_op>* out);
 // correction of the free cells; smoothing, restriction to / prolongation from
 // level 1; and the final p <- clamp(p + e) in fixed point.  Every kernel returns
 // at once when *skip is set.
+// cs_mg_fine_setup also zeroes row_counts (cs_mg_rows's six counts) when given,
+// skipped cycle or not.
 hipError_t cs_mg_fine_setup(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
-                            const float* wc, const float* rhs, const int32_t* p, cs_mg_level L0, const uint32_t* skip);
+                            const float* wc, const float* rhs, const int32_t* p, cs_mg_level L0, const uint32_t* skip,
+                            uint32_t* row_counts);
 hipError_t cs_mg_fine_smooth(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
                              cs_mg_level L0, uint32_t colour, const uint32_t* skip);
 hipError_t cs_mg_fine_restrict(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
@@ -115,7 +118,8 @@ bool cs_mg_rows_fit(const cs_mg_level* levels, int first, int nl);
 bool cs_mg_rows3_fit(const cs_mg_level* levels, int first, int nl);
 // pmap (|P| words), pnb (2|P| entries) and achild (2|A| entries): P's compact
 // ids, P's entries' neighbour ids and A's entries' children ids, written when
-// P is given.
+// P is given.  counts must be zero before the call (the cycle's
+// cs_mg_fine_setup zeroes them).
 hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_level* P, cs_mg_row* rows,
                       uint32_t* counts, uint32_t* pmap, uint4* pnb, uint4* achild, const uint32_t* skip);
 hipError_t cs_mg_rows_visit(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_row* rows,

@@ -309,8 +309,11 @@ __global__ void __launch_bounds__(256) k_mg_fine_setup(const cs_dparams* __restr
                                                        const uint32_t* __restrict__ list,
                                                        const float4* __restrict__ wc, const float* __restrict__ rhs,
                                                        const int32_t* __restrict__ p, cs_mg_level L,
-                                                       const uint32_t* __restrict__ skip) {
+                                                       const uint32_t* __restrict__ skip,
+                                                       uint32_t* __restrict__ row_counts) {
   CS_PARAMS_FROM(dp);
+  // the cycle's row lists start empty (cs_mg_rows appends to them)
+  if (row_counts && blockIdx.x == 0u && threadIdx.x < 6u) row_counts[threadIdx.x] = 0u;
   if (blockIdx.x >= dp->nactive) return;
   if (*skip) return;
   uint32_t x0, y0, z0;
@@ -1297,12 +1300,13 @@ void cs_mg_visit_ops(int level, int nlevels, std::vector<cs_mg_op>* out) {
 }
 
 hipError_t cs_mg_fine_setup(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
-                            const float* wc, const float* rhs, const int32_t* p, cs_mg_level L0, const uint32_t* skip) {
+                            const float* wc, const float* rhs, const int32_t* p, cs_mg_level L0, const uint32_t* skip,
+                            uint32_t* row_counts) {
   const cs_params& u = P.u;
-  if (!nactive) return hipSuccess;
+  if (!nactive) return row_counts ? cs_launch_fill_i32(s, reinterpret_cast<int32_t*>(row_counts), 6, 0) : hipSuccess;
   hipLaunchKernelGGL(k_mg_fine_setup, dim3(nactive), dim3(256), 0, s, P.d,
                      list, reinterpret_cast<const float4*>(wc), rhs,
-                     p, L0, skip);
+                     p, L0, skip, row_counts);
   return hipGetLastError();
 }
 hipError_t cs_mg_fine_smooth(hipStream_t s, const cs_pblock& P, uint32_t nactive, const uint32_t* list,
@@ -1391,8 +1395,6 @@ hipError_t cs_mg_rows(hipStream_t s, cs_mg_level A, cs_mg_level B, const cs_mg_l
   cs_mg_level none{};
   const cs_mg_level PP = P ? *P : none;
   const uint32_t nA = cells(A), nB = cells(B), nP = P ? cells(*P) : 0u;
-  hipError_t e = cs_launch_fill_i32(s, reinterpret_cast<int32_t*>(counts), 6, 0);
-  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_mg_rows_list, cells_grid(nA + nB + nP), dim3(256), 0, s, A, B, PP, rows, counts, skip);
   if (P) {
     hipLaunchKernelGGL(k_mg_rows_map, cells_grid(nP), dim3(256), 0, s, A, B, PP, rows, counts, pmap, skip);

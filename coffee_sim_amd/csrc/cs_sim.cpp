@@ -484,7 +484,8 @@ int mg_start(cs_sim* m, const cs_pblock& P, uint32_t nactive) {
     m->mg_rows3 = bound <= CS_MG_P3_MAX;
   }
   for (uint32_t c = 0; c < m->s.rbgs_mg_cycles; ++c) {
-    CS_HIP(cs_mg_fine_setup(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, L0, skip));
+    CS_HIP(cs_mg_fine_setup(st, P, nactive, m->brick_list, m->wc, m->rhs, m->pres, L0, skip,
+                            m->mg_rows ? m->mg_row_counts : nullptr));
     if (nl > 1) CS_HIP(cs_mg_fine_build(st, P, nactive, m->brick_list, L0, m->mg_levels[1], skip));
     for (int l = 1; l + 1 < nl; ++l) CS_HIP(cs_mg_build(st, m->mg_levels[l], m->mg_levels[l + 1], skip));
     if (m->mg_rows)
